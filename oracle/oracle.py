@@ -1,0 +1,263 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes access to the oracle and the compiled reference.
+
+``Oracle`` wraps ``liboracle.so`` (the plain-C FP64 restatement in
+``ifdda_oracle.c``).  ``Ref`` wraps ``_ref/libcdmref.so``, the reference's own
+cdmlib Fortran compiled in place by ``oracle/Makefile``; it exists only where
+the reference sources were available at build time.  Only tests/, smoke() and
+bench.py's cpu_baseline leg import this module, as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libcdmref.so")
+
+METHODS = {"GPBICG1": 1, "QMRBICGSTAB1": 2, "QMRBICGSTAB2": 3, "TFQMR": 4}
+
+_p = np.ctypeslib.ndpointer
+_zc = _p(np.complex128, flags="C_CONTIGUOUS")
+_i32 = _p(np.int32, flags="C_CONTIGUOUS")
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class _Trace(C.Structure):
+    _fields_ = [("cap", C.c_int), ("n", C.c_int), ("s", C.c_void_p), ("res", C.c_void_p)]
+
+
+class _Info(C.Structure):
+    _fields_ = [("tol1", C.c_double), ("nloop", C.c_int), ("ncompte", C.c_int),
+                ("nstop", C.c_int), ("status", C.c_int), ("steperr", C.c_int)]
+
+
+class Oracle:
+    """The C restatement (FP64, single thread)."""
+
+    def __init__(self, path: str = ORACLE_SO):
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (run make -C oracle)")
+        self.lib = C.CDLL(path)
+        L = self.lib
+        L.orc_propa_espace_libre.argtypes = [C.c_double] * 7 + [_zc]
+        L.orc_propa_int_sim.argtypes = [C.c_double] * 9 + [_zc]
+        L.orc_fft3d.argtypes = [_zc, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_fft1d.argtypes = [_zc, C.c_int, C.c_int]
+        L.orc_green_fft.argtypes = [C.c_int] * 6 + [C.c_double, C.c_double, C.c_int, _zc]
+        L.orc_matvec.argtypes = [_zc, C.c_void_p] + [C.c_int] * 6 + [_zc, _zc]
+        L.orc_matvec_direct.argtypes = [C.c_void_p] + [C.c_int] * 3 + [C.c_double, C.c_double, _zc, _zc]
+        L.orc_inverserig.argtypes = ([_zc, C.c_void_p] + [C.c_int] * 7 + [_zc, C.c_int, C.c_int,
+                                     C.c_double, C.c_int, _zc, _zc, _zc, C.POINTER(_Info), C.c_void_p])
+
+    def propa(self, r_obs, r_src, k0):
+        T = np.zeros(9, np.complex128)
+        self.lib.orc_propa_espace_libre(*map(float, r_obs), *map(float, r_src), float(k0), T)
+        return T.reshape(3, 3)
+
+    def propa_int(self, r_obs, r_src, k0, d, lim=1.0):
+        T = np.zeros(9, np.complex128)
+        self.lib.orc_propa_int_sim(*map(float, r_obs), *map(float, r_src), float(k0), float(d),
+                                   float(lim), T)
+        return T.reshape(3, 3)
+
+    def fft3d(self, v, sign):
+        """v: array indexed [k][j][i] (x fastest)."""
+        a = np.ascontiguousarray(v, np.complex128).copy()
+        n3, n2, n1 = a.shape
+        self.lib.orc_fft3d(a.reshape(-1), n1, n2, n3, int(sign))
+        return a
+
+    def green_fft(self, nx, ny, nz, L, d, k0, nquad=0):
+        """Returns G[6, L3, L2, L1] (xx,xy,xz,yy,yz,zz), FFT sign +1."""
+        L1, L2, L3 = L
+        G = np.zeros(6 * L1 * L2 * L3, np.complex128)
+        self.lib.orc_green_fft(nx, ny, nz, L1, L2, L3, float(d), float(k0), int(nquad), G)
+        return G.reshape(6, L3, L2, L1)
+
+    def matvec(self, G, tabdip, nx, ny, nz, FFX, FFB):
+        """FFB - conv(FFX) at dipole cells; tabdip None = dense box."""
+        L3, L2, L1 = G.shape[1:]
+        out = np.ascontiguousarray(FFB, np.complex128).copy()
+        tab = None if tabdip is None else np.ascontiguousarray(tabdip, np.int32)
+        self.lib.orc_matvec(np.ascontiguousarray(G).reshape(-1), _ptr(tab), nx, ny, nz, L1, L2, L3,
+                            np.ascontiguousarray(FFX, np.complex128), out)
+        return out
+
+    def matvec_direct(self, tabdip, nx, ny, nz, d, k0, FFX, FFB):
+        out = np.ascontiguousarray(FFB, np.complex128).copy()
+        tab = None if tabdip is None else np.ascontiguousarray(tabdip, np.int32)
+        self.lib.orc_matvec_direct(_ptr(tab), nx, ny, nz, float(d), float(k0),
+                                   np.ascontiguousarray(FFX, np.complex128), out)
+        return out
+
+    def inverserig(self, G, tabdip, nx, ny, nz, polarisa, FF0, method="GPBICG1", tol=1e-4,
+                   maxit=10000, trace_cap=0):
+        """polarisa: complex (nmax,3,3) array (Fortran meaning, [i,a,b])."""
+        L3, L2, L1 = G.shape[1:]
+        nb = FF0.size // 3
+        nmax = polarisa.shape[0]
+        pol_f = np.asfortranarray(polarisa, np.complex128).reshape(-1, order="F")
+        pol_f = np.ascontiguousarray(pol_f)
+        FF = np.zeros(3 * nb, np.complex128)
+        FFloc = np.zeros(3 * nb, np.complex128)
+        info = _Info()
+        tr = None
+        if trace_cap:
+            s = np.zeros(5 * trace_cap, np.complex128)
+            res = np.zeros(trace_cap, np.float64)
+            tr = _Trace(trace_cap, 0, s.ctypes.data, res.ctypes.data)
+        tab = None if tabdip is None else np.ascontiguousarray(tabdip, np.int32)
+        self.lib.orc_inverserig(np.ascontiguousarray(G).reshape(-1), _ptr(tab), nx, ny, nz, L1, L2,
+                                L3, nb, pol_f, nmax, METHODS[method], float(tol), int(maxit),
+                                np.ascontiguousarray(FF0, np.complex128), FF, FFloc, C.byref(info),
+                                None if tr is None else C.byref(tr))
+        out = dict(FF=FF, FFloc=FFloc, tol1=info.tol1, nloop=info.nloop, ncompte=info.ncompte,
+                   nstop=info.nstop, status=info.status, steperr=info.steperr)
+        if tr is not None:
+            out["trace"] = s.reshape(-1, 5)[: tr.n].copy()
+            out["trace_res"] = res[: tr.n].copy()
+        return out
+
+
+# --------------------------------------------------------------------------
+# The compiled reference (Fortran calling convention: everything by reference,
+# hidden character lengths appended as size_t).
+# --------------------------------------------------------------------------
+
+def _ri(v):
+    return C.byref(C.c_int(int(v)))
+
+
+def _rd(v):
+    return C.byref(C.c_double(float(v)))
+
+
+def _ri8(v=0):
+    return C.byref(C.c_int64(int(v)))
+
+
+class Ref:
+    def __init__(self, path: str = REF_SO):
+        if not os.path.exists(path):
+            raise RuntimeError(f"reference library missing: {path}")
+        self.lib = C.CDLL(path)
+
+    @staticmethod
+    def available(path: str = REF_SO) -> bool:
+        return os.path.exists(path)
+
+    def propa(self, r_obs, r_src, k0):
+        T = np.zeros((3, 3), np.complex128, order="F")
+        a = [C.c_double(float(v)) for v in (*r_obs, *r_src, k0)]
+        self.lib.propa_espace_libre_(*[C.byref(x) for x in a], T.ctypes.data_as(C.c_void_p))
+        return np.array(T)
+
+    def propa_int(self, r_obs, r_src, k0, d, lim=1.0, relreq=1e-4):
+        T = np.zeros((3, 3), np.complex128, order="F")
+        a = [C.c_double(float(v)) for v in (*r_obs, *r_src, k0, d, lim, relreq)]
+        self.lib.propaespacelibreintsim_(*[C.byref(x) for x in a], T.ctypes.data_as(C.c_void_p))
+        return np.array(T)
+
+    def green_fft(self, nx, ny, nz, d, k0, nquad=0, tolinit=1e-4):
+        """greencalculfft_ / greencalculquadfft_ on the 2n grid; returns G[6,L3,L2,L1]."""
+        nx2, ny2, nz2 = 2 * nx, 2 * ny, 2 * nz
+        M = nx2 * ny2 * nz2
+        G = [np.zeros(M, np.complex128) for _ in range(6)]
+        gp = [g.ctypes.data_as(C.c_void_p) for g in G]
+        common = [_ri(nx), _ri(ny), _ri(nz), _ri(nx2), _ri(ny2), _ri(nz2), _ri(nx2 * ny2),
+                  _ri(nx), _ri(ny), _ri(nz)]
+        if nquad:
+            self.lib.greencalculquadfft_(*common, _ri(nquad), _rd(tolinit), _rd(d), _rd(k0), *gp,
+                                         _ri8())
+        else:
+            self.lib.greencalculfft_(*common, _rd(d), _rd(k0), *gp, _ri8())
+        return np.stack(G).reshape(6, nz2, ny2, nx2)
+
+    def _vect(self, M):
+        return [np.zeros(M, np.complex128) for _ in range(3)]
+
+    def matvec3(self, G, tabdip, nx, ny, nz, FFX, FFB):
+        """produitfftmatvect3_: returns FFB - conv(FFX)."""
+        nz2, ny2, nx2 = G.shape[1:]
+        M = nx2 * ny2 * nz2
+        Gs = [np.ascontiguousarray(G[c]).reshape(-1) for c in range(6)]
+        V = self._vect(M)
+        nd = FFX.size // 3
+        out = np.ascontiguousarray(FFB, np.complex128).copy()
+        tab = np.ascontiguousarray(tabdip, np.int32)
+        self.lib.produitfftmatvect3_(*[g.ctypes.data_as(C.c_void_p) for g in Gs],
+                                     *[v.ctypes.data_as(C.c_void_p) for v in V],
+                                     tab.ctypes.data_as(C.c_void_p), _ri(M), _ri(M), _ri(nd),
+                                     _ri(nx * ny * nz), _ri(nx), _ri(ny), _ri(nz), _ri(nx), _ri(ny),
+                                     _ri(nz), _ri(nx2), _ri(ny2), _ri(nx2 * ny2), _ri(nz2),
+                                     np.ascontiguousarray(FFX, np.complex128).ctypes.data_as(C.c_void_p),
+                                     out.ctypes.data_as(C.c_void_p), _ri8(), _ri8())
+        return out
+
+    def matvec_opt(self, G, nx, ny, nz, FFX, FFB):
+        """produitfftmatvectopt_ (dense box): returns FFB - conv(FFX)."""
+        nz2, ny2, nx2 = G.shape[1:]
+        M = nx2 * ny2 * nz2
+        Gs = [np.ascontiguousarray(G[c]).reshape(-1) for c in range(6)]
+        V = self._vect(M)
+        nmax = nx * ny * nz
+        out = np.ascontiguousarray(FFB, np.complex128).copy()
+        self.lib.produitfftmatvectopt_(*[g.ctypes.data_as(C.c_void_p) for g in Gs],
+                                       *[v.ctypes.data_as(C.c_void_p) for v in V],
+                                       _ri(M), _ri(M), _ri(nmax), _ri(nx), _ri(ny), _ri(nz), _ri(nx),
+                                       _ri(ny), _ri(nz), _ri(nx2), _ri(ny2), _ri(nx2 * ny2), _ri(nz2),
+                                       np.ascontiguousarray(FFX, np.complex128).ctypes.data_as(C.c_void_p),
+                                       out.ctypes.data_as(C.c_void_p), _ri8(), _ri8())
+        return out
+
+    def inverserig(self, G, tabdip, nx, ny, nz, polarisa, FF0, method="GPBICG1", tol=1e-4):
+        """inverserig_ (x0 = E_inc).  polarisa: (nmax,3,3) complex."""
+        nz2, ny2, nx2 = G.shape[1:]
+        M = nx2 * ny2 * nz2
+        nmax = nx * ny * nz
+        nb = FF0.size // 3
+        Gs = [np.ascontiguousarray(G[c]).reshape(-1) for c in range(6)]
+        V = self._vect(M)
+        tab = np.ascontiguousarray(tabdip, np.int32)
+        XI = np.zeros(3 * nmax, np.complex128)
+        XR = np.zeros(3 * nmax, np.complex128)
+        wrk = np.zeros(3 * nmax * 12, np.complex128)
+        FF = np.zeros(3 * nmax, np.complex128)
+        FF0p = np.zeros(3 * nmax, np.complex128)
+        FF0p[: 3 * nb] = FF0
+        FFloc = np.zeros(3 * nmax, np.complex128)
+        pol = np.zeros((nmax, 3, 3), np.complex128)
+        pol[: polarisa.shape[0]] = polarisa
+        pol_f = np.asfortranarray(pol)
+        meth = method.ljust(12).encode()
+        info = C.create_string_buffer(b" " * 64, 64)
+        tol1 = C.c_double(0)
+        nloop = C.c_int(0)
+        ncompte = C.c_int(0)
+        nstop = C.c_int(0)
+        P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+        self.lib.inverserig_(*[P(g) for g in Gs], *[P(v) for v in V], P(tab), _ri(M), _ri(M),
+                             _ri(3 * nmax), _ri(12), _ri(nmax), _ri(nmax), _ri(nx), _ri(ny), _ri(nz),
+                             _ri(nx), _ri(ny), _ri(nz), _ri(nx2), _ri(ny2), _ri(nx2 * ny2), _ri(nz2),
+                             _ri(nb), _ri(3 * nb), P(XI), P(XR), P(wrk), P(FF), P(FF0p), P(FFloc),
+                             P(pol_f), C.c_char_p(meth), _rd(tol), C.byref(tol1), C.byref(nloop),
+                             C.byref(ncompte), _ri8(), _ri8(), C.byref(nstop), info,
+                             C.c_size_t(12), C.c_size_t(64))
+        return dict(FF=FF[: 3 * nb].copy(), FFloc=FFloc[: 3 * nb].copy(), tol1=tol1.value,
+                    nloop=nloop.value, ncompte=ncompte.value, nstop=nstop.value,
+                    infostr=info.raw.decode(errors="replace").rstrip())
+
+    def poladiff(self, d, eps, k0, method="RR", eps0=1.0):
+        out = C.c_double * 2
+        res = out()
+        e = (C.c_double * 2)(eps.real, eps.imag)
+        m = method.ljust(2).encode()
+        self.lib.poladiff_(_rd(d), C.byref(e), _rd(eps0), _rd(k0), _ri(1), C.c_char_p(m), C.byref(res),
+                           C.c_size_t(2))
+        return complex(res[0], res[1])
