@@ -1,0 +1,845 @@
+// ifdda_fft.hip -- LDS-staged mixed-radix (2,3,4,5,8) Stockham FFT passes for
+// gfx950 and the five fused passes of the DDA matvec
+//
+//   P1 scatter(alpha.x) + FFT-x   (k_xpass, MODE 1)
+//   P2 FFT-y                      (k_strided)
+//   P3 FFT-z . G . IFFT-z         (k_strided, CONV)
+//   P4 IFFT-y                     (k_strided)
+//   P5 IFFT-x + gather, b - v/M   (k_xpass, MODE 2)
+//
+// which together compute produitfftmatvect3.f:2-129 / produitfftmatvectopt.f
+// on the zero-padded (L1,L2,L3) grid without touching the all-zero lines of
+// the padding (only nz z-planes and ny y-rows are ever stored).
+//
+// One workgroup transforms a tile of NL = B*NC lines held in LDS as
+// [point][line] (line fastest, so a wavefront reads contiguous LDS words);
+// stages ping-pong between two LDS buffers.  For the strided axes the B
+// lines of a tile are B consecutive kx values (contiguous in HBM), so every
+// global row access is a contiguous B*NC-element segment.
+#include "ifdda_internal.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace ifdda {
+
+// Radix-r DFT constants (exact trigonometric values).
+#define IFDDA_C51 0.309016994374947424102293417183    // cos(2 pi/5)
+#define IFDDA_C52 (-0.809016994374947424102293417183) // cos(4 pi/5)
+#define IFDDA_S51 0.951056516295153572116439333379    // sin(2 pi/5)
+#define IFDDA_S52 0.587785252292473129168705954639    // sin(4 pi/5)
+#define IFDDA_S3 0.866025403784438646763723170753     // sin(2 pi/3)
+#define IFDDA_R2 0.707106781186547524400844362105     // 1/sqrt(2)
+
+// In-place DFT of R values, y_q = sum_p a_p exp(SGN 2 pi i p q / R).
+template <int R, int SGN, typename V> struct Dft;
+
+template <int SGN, typename V> struct Dft<2, SGN, V> {
+  static __device__ __forceinline__ void run(V* a) {
+    V t = a[0];
+    a[0] = cadd(t, a[1]);
+    a[1] = csub(t, a[1]);
+  }
+};
+
+template <int SGN, typename V> struct Dft<3, SGN, V> {
+  static __device__ __forceinline__ void run(V* a) {
+    using S = decltype(V::x);
+    V t = cadd(a[1], a[2]), u = csub(a[1], a[2]);
+    V m = csub(a[0], cscal(t, S(0.5)));
+    V v = mul_si<SGN>(cscal(u, S(IFDDA_S3)));
+    a[0] = cadd(a[0], t);
+    a[1] = cadd(m, v);
+    a[2] = csub(m, v);
+  }
+};
+
+template <int SGN, typename V> struct Dft<4, SGN, V> {
+  static __device__ __forceinline__ void run(V* a) {
+    V t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
+    V t2 = cadd(a[1], a[3]), t3 = mul_si<SGN>(csub(a[1], a[3]));
+    a[0] = cadd(t0, t2);
+    a[2] = csub(t0, t2);
+    a[1] = cadd(t1, t3);
+    a[3] = csub(t1, t3);
+  }
+};
+
+template <int SGN, typename V> struct Dft<5, SGN, V> {
+  static __device__ __forceinline__ void run(V* a) {
+    using S = decltype(V::x);
+    V t1 = cadd(a[1], a[4]), t2 = cadd(a[2], a[3]);
+    V u1 = csub(a[1], a[4]), u2 = csub(a[2], a[3]);
+    V m1 = cadd(a[0], cadd(cscal(t1, S(IFDDA_C51)), cscal(t2, S(IFDDA_C52))));
+    V m2 = cadd(a[0], cadd(cscal(t1, S(IFDDA_C52)), cscal(t2, S(IFDDA_C51))));
+    V v1 = mul_si<SGN>(cadd(cscal(u1, S(IFDDA_S51)), cscal(u2, S(IFDDA_S52))));
+    V v2 = mul_si<SGN>(csub(cscal(u1, S(IFDDA_S52)), cscal(u2, S(IFDDA_S51))));
+    a[0] = cadd(a[0], cadd(t1, t2));
+    a[1] = cadd(m1, v1);
+    a[4] = csub(m1, v1);
+    a[2] = cadd(m2, v2);
+    a[3] = csub(m2, v2);
+  }
+};
+
+template <int SGN, typename V> struct Dft<8, SGN, V> {
+  static __device__ __forceinline__ void run(V* a) {
+    using S = decltype(V::x);
+    V e[4] = {a[0], a[2], a[4], a[6]};
+    V o[4] = {a[1], a[3], a[5], a[7]};
+    Dft<4, SGN, V>::run(e);
+    Dft<4, SGN, V>::run(o);
+    const S h = S(IFDDA_R2);
+    // o1 *= (1 + SGN i)/sqrt2, o2 *= SGN i, o3 *= (-1 + SGN i)/sqrt2
+    V w1 = SGN > 0 ? mk((o[1].x - o[1].y) * h, (o[1].y + o[1].x) * h)
+                   : mk((o[1].x + o[1].y) * h, (o[1].y - o[1].x) * h);
+    V w2 = mul_si<SGN>(o[2]);
+    V w3 = SGN > 0 ? mk((-o[3].x - o[3].y) * h, (-o[3].y + o[3].x) * h)
+                   : mk((-o[3].x + o[3].y) * h, (-o[3].y - o[3].x) * h);
+    a[0] = cadd(e[0], o[0]);
+    a[4] = csub(e[0], o[0]);
+    a[1] = cadd(e[1], w1);
+    a[5] = csub(e[1], w1);
+    a[2] = cadd(e[2], w2);
+    a[6] = csub(e[2], w2);
+    a[3] = cadd(e[3], w3);
+    a[7] = csub(e[3], w3);
+  }
+};
+
+// One Stockham stage of radix R over the tile.  Sub-transforms of length Ns
+// are already done; butterfly j reads points j + q*L/R, twiddles input q by
+// exp(SGN 2 pi i k q / (Ns R)) with k = j mod Ns, and writes points
+// (j-k)*R + k + q*Ns.  tw[e] = exp(+2 pi i e / L).
+template <int R, int SGN, typename V>
+__device__ __forceinline__ void stage_r(const V* __restrict__ in, V* __restrict__ out,
+                                        const V* __restrict__ tw, int L, int Ns, int NL,
+                                        int line, int j0, int jstep) {
+  const int nb = L / R;
+  const int tws = L / (Ns * R);
+  for (int j = j0; j < nb; j += jstep) {
+    const int k = j % Ns;
+    V a[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) a[q] = in[(j + q * nb) * NL + line];
+    if (Ns > 1) {
+#pragma unroll
+      for (int q = 1; q < R; ++q) {
+        V w = tw[k * q * tws];
+        if (SGN < 0) w.y = -w.y;
+        a[q] = cmul(a[q], w);
+      }
+    }
+    Dft<R, SGN, V>::run(a);
+    const int ob = (j - k) * R + k;
+#pragma unroll
+    for (int q = 0; q < R; ++q) out[(ob + q * Ns) * NL + line] = a[q];
+  }
+}
+
+// Full transform of the tile; returns the buffer holding the result.
+// Thread layout: line = tid % NL, jt = tid / NL, JT = blockDim / NL.
+template <int SGN, typename V>
+__device__ V* tile_fft(V* b0, V* b1, const V* tw, const AxisPlan& ax, int NL, int line, int jt,
+                       int JT) {
+  int Ns = 1;
+  for (int s = 0; s < ax.nst; ++s) {
+    const int r = ax.rad[s];
+    if (jt < JT) {
+      switch (r) {
+        case 2: stage_r<2, SGN>(b0, b1, tw, ax.L, Ns, NL, line, jt, JT); break;
+        case 3: stage_r<3, SGN>(b0, b1, tw, ax.L, Ns, NL, line, jt, JT); break;
+        case 4: stage_r<4, SGN>(b0, b1, tw, ax.L, Ns, NL, line, jt, JT); break;
+        case 5: stage_r<5, SGN>(b0, b1, tw, ax.L, Ns, NL, line, jt, JT); break;
+        default: stage_r<8, SGN>(b0, b1, tw, ax.L, Ns, NL, line, jt, JT); break;
+      }
+    }
+    __syncthreads();
+    V* t = b0;
+    b0 = b1;
+    b1 = t;
+    Ns *= r;
+  }
+  return b0;
+}
+
+// ---------------------------------------------------------------------------
+// Strided-axis pass (y or z): tile = B consecutive kx x NC components, lines
+// along the axis with element stride sp; blockIdx.y selects the outer index.
+// ---------------------------------------------------------------------------
+template <typename V> struct SArgs {
+  const V* src;
+  V* dst;
+  const V* tw;
+  AxisPlan ax;
+  int n_in, n_out;  // points loaded (rest zero) / stored
+  int nbx;          // extent of the contiguous kx dimension
+  int B;            // kx per tile
+  long long so_src, sp_src, so_dst, sp_dst;
+  // CONV only: folded G[((ky'*F3 + kz')*F1 + kx')*6 + c], outer index = ky
+  const V* G;
+  int F1, F3, L1, L2;
+  const int* skip;  // when non-null and *skip != 0 the pass does nothing
+};
+
+__device__ __forceinline__ int fold_idx(int k, int L, int& sgn) {
+  if (2 * k > L) {
+    sgn = -1;
+    return L - k;
+  }
+  sgn = 1;
+  return k;
+}
+
+template <typename V, int NC, int SGN, bool CONV>
+__global__ void __launch_bounds__(256) k_strided(SArgs<V> a) {
+  if (a.skip && *a.skip) return;
+  extern __shared__ double4 smem_d4[];
+  V* tw = reinterpret_cast<V*>(smem_d4);
+  const int L = a.ax.L;
+  const int NL = a.B * NC;
+  V* b0 = tw + L;
+  V* b1 = b0 + L * NL;
+  const int tid = threadIdx.x;
+  const int nthr = blockDim.x;
+  const int line = tid % NL, jt = tid / NL, JT = nthr / NL;
+  const int o = blockIdx.y;
+  const int kx0 = blockIdx.x * a.B;
+  const int nvalid = min(a.B, a.nbx - kx0) * NC;
+
+  for (int e = tid; e < L; e += nthr) tw[e] = a.tw[e];
+  const V* src = a.src + (long long)o * a.so_src + (long long)kx0 * NC;
+  if (jt < JT) {
+    for (int p = jt; p < L; p += JT) {
+      V v = czero<V>();
+      if (p < a.n_in && line < nvalid) v = src[(long long)p * a.sp_src + line];
+      b0[p * NL + line] = v;
+    }
+  }
+  __syncthreads();
+
+  V* res;
+  if (CONV) {
+    res = tile_fft<+1>(b0, b1, tw, a.ax, NL, line, jt, JT);
+    // Symmetric 3x3 product with G(kx, ky, kz) (produitfftmatvect3.f:76-90).
+    int sy;
+    const int kyf = fold_idx(o, a.L2, sy);
+    for (int idx = tid; idx < L * a.B; idx += nthr) {
+      const int kz = idx / a.B, kxl = idx - kz * a.B;
+      const int kx = kx0 + kxl;
+      if (kx >= a.nbx) continue;
+      int sx, sz;
+      const int kxf = fold_idx(kx, a.L1, sx);
+      const int kzf = fold_idx(kz, L, sz);
+      const V* g = a.G + ((long long)(kyf * a.F3 + kzf) * a.F1 + kxf) * 6;
+      const V gxx = g[0], gyy = g[3], gzz = g[5];
+      V gxy = g[1], gxz = g[2], gyz = g[4];
+      if (sx * sy < 0) gxy = mk(-gxy.x, -gxy.y);
+      if (sx * sz < 0) gxz = mk(-gxz.x, -gxz.y);
+      if (sy * sz < 0) gyz = mk(-gyz.x, -gyz.y);
+      V* v = res + kz * NL + kxl * 3;
+      const V ex = v[0], ey = v[1], ez = v[2];
+      v[0] = cadd(cadd(cmul(gxx, ex), cmul(gxy, ey)), cmul(gxz, ez));
+      v[1] = cadd(cadd(cmul(gxy, ex), cmul(gyy, ey)), cmul(gyz, ez));
+      v[2] = cadd(cadd(cmul(gxz, ex), cmul(gyz, ey)), cmul(gzz, ez));
+    }
+    __syncthreads();
+    V* other = (res == b0) ? b1 : b0;
+    res = tile_fft<-1>(res, other, tw, a.ax, NL, line, jt, JT);
+  } else {
+    res = tile_fft<SGN>(b0, b1, tw, a.ax, NL, line, jt, JT);
+  }
+
+  V* dst = a.dst + (long long)o * a.so_dst + (long long)kx0 * NC;
+  if (jt < JT && line < nvalid) {
+    for (int p = jt; p < a.n_out; p += JT) dst[(long long)p * a.sp_dst + line] = res[p * NL + line];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// x-axis pass: lines are contiguous in HBM (ld elements apart).  MODE 0 plain,
+// MODE 1 scatter from the dipole list (alpha applied), MODE 2 gather to the
+// dipole list: out = b - v/M.  Box line g covers cells i + nx*g.
+// ---------------------------------------------------------------------------
+template <typename V> struct XArgs {
+  const V* src;
+  V* dst;
+  const V* tw;
+  AxisPlan ax;
+  int n_in, n_out;
+  int nlines;  // total lines
+  int B;       // lines per tile
+  long long ld_src, ld_dst;
+  // MODE 1 / 2
+  const int32_t* tab;  // box -> 1-based dipole (0 = empty); null = dense box
+  int nx;
+  const V* xin;        // MODE 1: x (3 per dipole); MODE 2: b
+  const V* alpha;      // MODE 1
+  int alpha_mode;
+  V* xout;             // MODE 2
+  decltype(V::x) inv_m;
+  const int* skip;     // when non-null and *skip != 0 the pass does nothing
+};
+
+template <typename V>
+__device__ __forceinline__ void alpha_apply(const V* __restrict__ al, int mode, long long dip, V& e0,
+                                            V& e1, V& e2) {
+  if (mode == kAlphaScalar) {
+    const V s = al[dip];
+    e0 = cmul(s, e0);
+    e1 = cmul(s, e1);
+    e2 = cmul(s, e2);
+  } else if (mode == kAlphaTensor) {
+    const V* m = al + dip * 9;
+    const V f0 = cadd(cadd(cmul(m[0], e0), cmul(m[1], e1)), cmul(m[2], e2));
+    const V f1 = cadd(cadd(cmul(m[3], e0), cmul(m[4], e1)), cmul(m[5], e2));
+    const V f2 = cadd(cadd(cmul(m[6], e0), cmul(m[7], e1)), cmul(m[8], e2));
+    e0 = f0;
+    e1 = f1;
+    e2 = f2;
+  }
+}
+
+template <typename V, int NC, int SGN, int MODE>
+__global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
+  if (a.skip && *a.skip) return;
+  extern __shared__ double4 smem_d4[];
+  V* tw = reinterpret_cast<V*>(smem_d4);
+  const int L = a.ax.L;
+  const int NL = a.B * NC;
+  V* b0 = tw + L;
+  V* b1 = b0 + L * NL;
+  const int tid = threadIdx.x;
+  const int nthr = blockDim.x;
+  const int line = tid % NL, jt = tid / NL, JT = nthr / NL;
+  const int g0 = blockIdx.x * a.B;
+  const int nl = min(a.B, a.nlines - g0);
+
+  for (int e = tid; e < L; e += nthr) tw[e] = a.tw[e];
+  if (MODE == 1) {
+    for (int e = tid; e < a.B * L; e += nthr) {
+      const int l = e / L, i = e - l * L;
+      V e0 = czero<V>(), e1 = e0, e2 = e0;
+      if (l < nl && i < a.n_in) {
+        const long long cell = (long long)i + (long long)a.nx * (g0 + l);
+        long long dip = a.tab ? (long long)a.tab[cell] - 1 : cell;
+        if (dip >= 0) {
+          const V* x = a.xin + dip * 3;
+          e0 = x[0];
+          e1 = x[1];
+          e2 = x[2];
+          alpha_apply(a.alpha, a.alpha_mode, dip, e0, e1, e2);
+        }
+      }
+      V* d = b0 + i * NL + l * 3;
+      d[0] = e0;
+      d[1] = e1;
+      d[2] = e2;
+    }
+  } else {
+    const int per = L * NC;
+    for (int e = tid; e < a.B * per; e += nthr) {
+      const int l = e / per, r = e - l * per;
+      const int p = r / NC, c = r - p * NC;
+      V v = czero<V>();
+      if (l < nl && p < a.n_in) v = a.src[(long long)(g0 + l) * a.ld_src + (long long)p * NC + c];
+      b0[p * NL + l * NC + c] = v;
+    }
+  }
+  __syncthreads();
+
+  V* res = tile_fft<SGN>(b0, b1, tw, a.ax, NL, line, jt, JT);
+
+  if (MODE == 2) {
+    for (int e = tid; e < a.B * a.n_out; e += nthr) {
+      const int l = e / a.n_out, i = e - l * a.n_out;
+      if (l >= nl) continue;
+      const long long cell = (long long)i + (long long)a.nx * (g0 + l);
+      long long dip = a.tab ? (long long)a.tab[cell] - 1 : cell;
+      if (dip < 0) continue;
+      const V* v = res + i * NL + l * 3;
+      const V* b = a.xin + dip * 3;
+      V* y = a.xout + dip * 3;
+      y[0] = csub(b[0], cscal(v[0], a.inv_m));
+      y[1] = csub(b[1], cscal(v[1], a.inv_m));
+      y[2] = csub(b[2], cscal(v[2], a.inv_m));
+    }
+  } else {
+    const int per = a.n_out * NC;
+    for (int e = tid; e < a.B * per; e += nthr) {
+      const int l = e / per, r = e - l * per;
+      if (l >= nl) continue;
+      const int p = r / NC, c = r - p * NC;
+      a.dst[(long long)(g0 + l) * a.ld_dst + (long long)p * NC + c] = res[p * NL + l * NC + c];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Green tensor (FP64): fill one component on the circulant grid
+// (calculgreenfft.f:22-71), near-cell Simpson integration
+// (propaesplibreintsim.f:40-101), fold to one octant.
+// ---------------------------------------------------------------------------
+
+// T(R) for R = -(ox,oy,oz)*d (espace_libre.f:2-43 with the observation at the
+// origin); comps xx,xy,xz,yy,yz,zz.
+__device__ __forceinline__ void green_T(double rx, double ry, double rz, double k0, double2 T[6]) {
+  const double r2 = rx * rx + ry * ry + rz * rz;
+  if (r2 == 0.0) {
+    for (int c = 0; c < 6; ++c) T[c] = make_double2(0.0, 0.0);
+    return;
+  }
+  const double R = sqrt(r2);
+  double sn, cs;
+  sincos(k0 * R, &sn, &cs);
+  // c1 = (1/R - i k0)/r2, c2 = k0^2/R (real)
+  const double c1r = 1.0 / (R * r2), c1i = -k0 / r2;
+  const double c2 = k0 * k0 / R;
+  const double rv[3] = {rx, ry, rz};
+  const int ia[6] = {0, 0, 0, 1, 1, 2}, ib[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const double rr = rv[ia[c]] * rv[ib[c]] / r2;
+    const double id = (ia[c] == ib[c]) ? 1.0 : 0.0;
+    const double f = 3.0 * rr - id, g = id - rr;
+    const double ar = f * c1r + g * c2, ai = f * c1i;
+    T[c] = make_double2(ar * cs - ai * sn, ar * sn + ai * cs);
+  }
+}
+
+__device__ __forceinline__ int circ_off(int idx, int n, int L, bool& ok) {
+  if (idx < n) { ok = true; return idx; }
+  if (idx > L - n) { ok = true; return idx - L; }
+  ok = false;
+  return 0;
+}
+
+__global__ void k_green_fill(double2* __restrict__ out, int comp, int nx, int ny, int nz, int L1,
+                             int L2, int L3, double d, double k0,
+                             const double2* __restrict__ near, int nquad) {
+  const long long M = (long long)L1 * L2 * L3;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < M;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int ii = idx % L1;
+    const int jj = (idx / L1) % L2;
+    const int kk = idx / ((long long)L1 * L2);
+    bool okx, oky, okz;
+    const int i = circ_off(ii, nx, L1, okx), j = circ_off(jj, ny, L2, oky),
+              k = circ_off(kk, nz, L3, okz);
+    double2 v = make_double2(0.0, 0.0);
+    if (okx && oky && okz) {
+      if (nquad != 0 && i * i + j * j + k * k <= nquad * nquad) {
+        const int w = 2 * nquad + 1;
+        v = near[(((k + nquad) * w + (j + nquad)) * w + (i + nquad)) * 6 + comp];
+      } else {
+        double2 T[6];
+        green_T(-(double)i * d, -(double)j * d, -(double)k * d, k0, T);
+        v = T[comp];
+      }
+    }
+    out[idx] = v;
+  }
+}
+
+// Simpson partial sums: block (i-plane, cell).  Nodes x_a = -d/2 + h*a of the
+// cube centred on the observation point (origin), integrand T(x - o*d).
+__global__ void __launch_bounds__(256) k_green_near_partial(double2* __restrict__ part,
+                                                            const int* __restrict__ offs,
+                                                            double d, double k0) {
+  constexpr int NS = 128;
+  const int ip = blockIdx.x;  // 0..NS
+  const int cell = blockIdx.y;
+  const int ox = offs[cell * 3 + 0], oy = offs[cell * 3 + 1], oz = offs[cell * 3 + 2];
+  const double h = d / NS;
+  const double A = -0.5 * d;
+  auto wgt = [](int a) { return (a == 0 || a == NS) ? 1.0 : ((a & 1) ? 4.0 : 2.0); };
+  double2 acc[6];
+  for (int c = 0; c < 6; ++c) acc[c] = make_double2(0.0, 0.0);
+  const double wi = wgt(ip);
+  const double xa = A + h * ip;
+  for (int t = threadIdx.x; t < (NS + 1) * (NS + 1); t += blockDim.x) {
+    const int jn = t / (NS + 1), kn = t - jn * (NS + 1);
+    const double ya = A + h * jn, za = A + h * kn;
+    const double ww = wi * wgt(jn) * wgt(kn);
+    double2 T[6];
+    green_T(xa - ox * d, ya - oy * d, za - oz * d, k0, T);
+    for (int c = 0; c < 6; ++c) {
+      acc[c].x += T[c].x * ww;
+      acc[c].y += T[c].y * ww;
+    }
+  }
+  __shared__ double red[256][12];
+  for (int c = 0; c < 6; ++c) {
+    red[threadIdx.x][2 * c] = acc[c].x;
+    red[threadIdx.x][2 * c + 1] = acc[c].y;
+  }
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int q = 0; q < 12; ++q) red[threadIdx.x][q] += red[threadIdx.x + s][q];
+    __syncthreads();
+  }
+  if (threadIdx.x < 6)
+    part[((long long)cell * (NS + 1) + ip) * 6 + threadIdx.x] =
+        make_double2(red[0][2 * threadIdx.x], red[0][2 * threadIdx.x + 1]);
+}
+
+// Finish the near cells: sum plane partials, scale, zero off-diagonals along
+// axes with |offset| <= d/10, zero the self cell, point value beyond lambda.
+__global__ void k_green_near_final(double2* __restrict__ near, const double2* __restrict__ part,
+                                   const int* __restrict__ offs, int ncell, int nquad, double d,
+                                   double k0, double lim) {
+  constexpr int NS = 128;
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= ncell) return;
+  const int ox = offs[cell * 3 + 0], oy = offs[cell * 3 + 1], oz = offs[cell * 3 + 2];
+  const int w = 2 * nquad + 1;
+  double2* out = near + ((long long)((oz + nquad) * w + (oy + nquad)) * w + (ox + nquad)) * 6;
+  const double rx = ox * d, ry = oy * d, rz = oz * d;
+  const double R = sqrt(rx * rx + ry * ry + rz * rz);
+  const double dm = 0.1 * d;
+  const double lambda = 6.283185307179586476925286766559 / k0;
+  if (R <= dm) {
+    for (int c = 0; c < 6; ++c) out[c] = make_double2(0.0, 0.0);
+    return;
+  }
+  if (R >= lim * lambda) {
+    double2 T[6];
+    green_T(-rx, -ry, -rz, k0, T);
+    for (int c = 0; c < 6; ++c) out[c] = T[c];
+    return;
+  }
+  double2 s[6];
+  for (int c = 0; c < 6; ++c) s[c] = make_double2(0.0, 0.0);
+  for (int ip = 0; ip <= NS; ++ip)
+    for (int c = 0; c < 6; ++c) {
+      const double2 p = part[((long long)cell * (NS + 1) + ip) * 6 + c];
+      s[c].x += p.x;
+      s[c].y += p.y;
+    }
+  const double h = d / NS;
+  const double sc = (h / 3.0) * (h / 3.0) * (h / 3.0) / (d * d * d);
+  const double fx = fabs(rx) <= dm ? 0.0 : 1.0, fy = fabs(ry) <= dm ? 0.0 : 1.0,
+               fz = fabs(rz) <= dm ? 0.0 : 1.0;
+  const double f[6] = {1.0, fx * fy, fx * fz, 1.0, fy * fz, 1.0};
+  for (int c = 0; c < 6; ++c) out[c] = make_double2(s[c].x * sc * f[c], s[c].y * sc * f[c]);
+}
+
+// Fold one FFT'd component (full L grid) into the octant array (FP64 + FP32).
+__global__ void k_green_fold(double2* __restrict__ g64, float2* __restrict__ g32,
+                             const double2* __restrict__ full, int comp, int L1, int L2, int F1,
+                             int F2, int F3) {
+  const long long n = (long long)F1 * F2 * F3;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int kx = t % F1;
+    const int kz = (t / F1) % F3;
+    const int ky = t / ((long long)F1 * F3);
+    const double2 v = full[((long long)kz * L2 + ky) * L1 + kx];
+    g64[t * 6 + comp] = v;
+    g32[t * 6 + comp] = make_float2((float)v.x, (float)v.y);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side: plans, tile shapes and launchers.
+// ---------------------------------------------------------------------------
+
+int smooth235_at_least(int n) {
+  for (int m = std::max(n, 1);; ++m) {
+    int r = m;
+    for (int p : {2, 3, 5})
+      while (r % p == 0) r /= p;
+    if (r == 1) return m;
+  }
+}
+
+// Radices in execution order: 8s first, then a 4 or 2 for the rest of the
+// power of two, then 5s and 3s.
+bool make_axis_plan(int L, AxisPlan* out) {
+  AxisPlan a{};
+  a.L = L;
+  int r = L, n = 0;
+  while (r % 8 == 0 && n < kMaxStages) { a.rad[n++] = 8; r /= 8; }
+  if (r % 4 == 0 && n < kMaxStages) { a.rad[n++] = 4; r /= 4; }
+  if (r % 2 == 0 && n < kMaxStages) { a.rad[n++] = 2; r /= 2; }
+  while (r % 5 == 0 && n < kMaxStages) { a.rad[n++] = 5; r /= 5; }
+  while (r % 3 == 0 && n < kMaxStages) { a.rad[n++] = 3; r /= 3; }
+  if (r != 1) return false;
+  a.nst = n;
+  *out = a;
+  return true;
+}
+
+// Largest batch width whose ping-pong tile fits the LDS budget; threads are a
+// multiple of the line count NL = B*NC, close to 256.
+static TileShape tile_for(int L, int NC, int esz, int nbatch, size_t lds_limit) {
+  const size_t per_line = 2 * (size_t)L * esz;  // two buffers
+  long Bmax = (long)((lds_limit - (size_t)L * esz) / (per_line * NC));
+  Bmax = std::max(1L, std::min(Bmax, 64L));
+  Bmax = std::min<long>(Bmax, std::max(1, nbatch));
+  while (Bmax * NC > 256) --Bmax;
+  TileShape t;
+  t.B = (int)Bmax;
+  const int NL = t.B * NC;
+  t.threads = NL * std::max(1, 256 / NL);
+  t.lds = ((size_t)L + 2 * (size_t)L * NL) * esz;
+  return t;
+}
+
+void make_tiles(GridPlan& g, size_t lds_limit) {
+  const int L1 = g.L[0], L2 = g.L[1], L3 = g.L[2];
+  for (int p = 0; p < 2; ++p) {
+    const int esz = p == 0 ? 16 : 8;
+    g.p1[p] = tile_for(L1, 3, esz, g.ny * g.nz, lds_limit);
+    g.p2[p] = tile_for(L2, 3, esz, L1, lds_limit);
+    g.p3[p] = tile_for(L3, 3, esz, L1, lds_limit);
+    g.p4[p] = tile_for(L2, 3, esz, L1, lds_limit);
+    g.p5[p] = tile_for(L1, 3, esz, g.ny * g.nz, lds_limit);
+  }
+  g.gx = tile_for(L1, 1, 16, L2 * L3, lds_limit);
+  g.gy = tile_for(L2, 1, 16, L1, lds_limit);
+  g.gz = tile_for(L3, 1, 16, L1, lds_limit);
+}
+
+template <typename K> static bool allow_lds(K kernel, size_t bytes) {
+  static_assert(sizeof(K) > 0, "");
+  if (bytes <= 65536) return true;
+  return hip_ok(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes),
+                "hipFuncSetAttribute(MaxDynamicSharedMemorySize)");
+}
+
+template <typename V>
+static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, const V* b, V* y,
+                        hipStream_t s, int pi, const int* skip) {
+  using S = decltype(V::x);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int L1 = g.L[0], L2 = g.L[1], L3 = g.L[2];
+  const long long l1c = (long long)L1 * 3;
+  {  // P1: scatter(alpha x) + FFT-x
+    const TileShape& t = g.p1[pi];
+    XArgs<V> a{};
+    a.dst = m.A;
+    a.tw = m.tw[0];
+    a.ax = g.ax[0];
+    a.n_in = nx;
+    a.n_out = L1;
+    a.nlines = ny * nz;
+    a.B = t.B;
+    a.skip = skip;
+    a.ld_dst = l1c;
+    a.tab = m.tab;
+    a.nx = nx;
+    a.xin = x;
+    a.alpha = m.alpha;
+    a.alpha_mode = m.alpha_mode;
+    auto k = k_xpass<V, 3, +1, 1>;
+    if (!allow_lds(k, t.lds)) return false;
+    hipLaunchKernelGGL(k, dim3((a.nlines + t.B - 1) / t.B), dim3(t.threads), t.lds, s, a);
+  }
+  {  // P2: FFT-y over the nz non-zero planes
+    const TileShape& t = g.p2[pi];
+    SArgs<V> a{};
+    a.src = m.A;
+    a.dst = m.B;
+    a.tw = m.tw[1];
+    a.ax = g.ax[1];
+    a.n_in = ny;
+    a.n_out = L2;
+    a.nbx = L1;
+    a.B = t.B;
+    a.skip = skip;
+    a.so_src = (long long)ny * l1c;
+    a.sp_src = l1c;
+    a.so_dst = (long long)L2 * l1c;
+    a.sp_dst = l1c;
+    auto k = k_strided<V, 3, +1, false>;
+    if (!allow_lds(k, t.lds)) return false;
+    hipLaunchKernelGGL(k, dim3((L1 + t.B - 1) / t.B, nz), dim3(t.threads), t.lds, s, a);
+  }
+  {  // P3: FFT-z . G . IFFT-z, in place, keeps kz < nz
+    const TileShape& t = g.p3[pi];
+    SArgs<V> a{};
+    a.src = m.B;
+    a.dst = m.B;
+    a.tw = m.tw[2];
+    a.ax = g.ax[2];
+    a.n_in = nz;
+    a.n_out = nz;
+    a.nbx = L1;
+    a.B = t.B;
+    a.skip = skip;
+    a.so_src = a.so_dst = l1c;
+    a.sp_src = a.sp_dst = (long long)L2 * l1c;
+    a.G = m.G;
+    a.F1 = g.F[0];
+    a.F3 = g.F[2];
+    a.L1 = L1;
+    a.L2 = L2;
+    auto k = k_strided<V, 3, +1, true>;
+    if (!allow_lds(k, t.lds)) return false;
+    hipLaunchKernelGGL(k, dim3((L1 + t.B - 1) / t.B, L2), dim3(t.threads), t.lds, s, a);
+  }
+  {  // P4: IFFT-y, keeps j < ny
+    const TileShape& t = g.p4[pi];
+    SArgs<V> a{};
+    a.src = m.B;
+    a.dst = m.A;
+    a.tw = m.tw[1];
+    a.ax = g.ax[1];
+    a.n_in = L2;
+    a.n_out = ny;
+    a.nbx = L1;
+    a.B = t.B;
+    a.skip = skip;
+    a.so_src = (long long)L2 * l1c;
+    a.sp_src = l1c;
+    a.so_dst = (long long)ny * l1c;
+    a.sp_dst = l1c;
+    auto k = k_strided<V, 3, -1, false>;
+    if (!allow_lds(k, t.lds)) return false;
+    hipLaunchKernelGGL(k, dim3((L1 + t.B - 1) / t.B, nz), dim3(t.threads), t.lds, s, a);
+  }
+  {  // P5: IFFT-x + gather, y = b - v/M
+    const TileShape& t = g.p5[pi];
+    XArgs<V> a{};
+    a.src = m.A;
+    a.tw = m.tw[0];
+    a.ax = g.ax[0];
+    a.n_in = L1;
+    a.n_out = nx;
+    a.nlines = ny * nz;
+    a.B = t.B;
+    a.skip = skip;
+    a.ld_src = l1c;
+    a.tab = m.tab;
+    a.nx = nx;
+    a.xin = b;
+    a.xout = y;
+    a.inv_m = (S)(1.0 / ((double)L1 * (double)L2 * (double)L3));
+    auto k = k_xpass<V, 3, -1, 2>;
+    if (!allow_lds(k, t.lds)) return false;
+    hipLaunchKernelGGL(k, dim3((a.nlines + t.B - 1) / t.B), dim3(t.threads), t.lds, s, a);
+  }
+  return hip_ok(hipGetLastError(), "matvec launch");
+}
+
+bool launch_matvec_f64(const GridPlan& g, const MatvecDev<double2>& m, const double2* x,
+                       const double2* b, double2* y, hipStream_t s, const int* skip) {
+  return matvec_impl(g, m, x, b, y, s, 0, skip);
+}
+bool launch_matvec_f32(const GridPlan& g, const MatvecDev<float2>& m, const float2* x,
+                       const float2* b, float2* y, hipStream_t s, const int* skip) {
+  return matvec_impl(g, m, x, b, y, s, 1, skip);
+}
+
+// In-place forward 3D FFT (sign +1) of one FP64 component on the full grid.
+static bool fft3d_full_f64(const GridPlan& g, const double2* tw[3], double2* v, hipStream_t s) {
+  const int L1 = g.L[0], L2 = g.L[1], L3 = g.L[2];
+  {
+    XArgs<double2> a{};
+    a.src = v;
+    a.dst = v;
+    a.tw = tw[0];
+    a.ax = g.ax[0];
+    a.n_in = a.n_out = L1;
+    a.nlines = L2 * L3;
+    a.B = g.gx.B;
+    a.ld_src = a.ld_dst = L1;
+    auto k = k_xpass<double2, 1, +1, 0>;
+    if (!allow_lds(k, g.gx.lds)) return false;
+    hipLaunchKernelGGL(k, dim3((a.nlines + a.B - 1) / a.B), dim3(g.gx.threads), g.gx.lds, s, a);
+  }
+  {
+    SArgs<double2> a{};
+    a.src = v;
+    a.dst = v;
+    a.tw = tw[1];
+    a.ax = g.ax[1];
+    a.n_in = a.n_out = L2;
+    a.nbx = L1;
+    a.B = g.gy.B;
+    a.so_src = a.so_dst = (long long)L1 * L2;
+    a.sp_src = a.sp_dst = L1;
+    auto k = k_strided<double2, 1, +1, false>;
+    if (!allow_lds(k, g.gy.lds)) return false;
+    hipLaunchKernelGGL(k, dim3((L1 + a.B - 1) / a.B, L3), dim3(g.gy.threads), g.gy.lds, s, a);
+  }
+  {
+    SArgs<double2> a{};
+    a.src = v;
+    a.dst = v;
+    a.tw = tw[2];
+    a.ax = g.ax[2];
+    a.n_in = a.n_out = L3;
+    a.nbx = L1;
+    a.B = g.gz.B;
+    a.so_src = a.so_dst = L1;
+    a.sp_src = a.sp_dst = (long long)L1 * L2;
+    auto k = k_strided<double2, 1, +1, false>;
+    if (!allow_lds(k, g.gz.lds)) return false;
+    hipLaunchKernelGGL(k, dim3((L1 + a.B - 1) / a.B, L2), dim3(g.gz.threads), g.gz.lds, s, a);
+  }
+  return hip_ok(hipGetLastError(), "fft3d launch");
+}
+
+bool green_fold_component(const GridPlan& g, const double2* full, int comp, double2* g64,
+                          float2* g32, hipStream_t s) {
+  const long long nf = (long long)g.F[0] * g.F[1] * g.F[2];
+  const int blocks = (int)std::min<long long>((nf + 255) / 256, 65535);
+  hipLaunchKernelGGL(k_green_fold, dim3(blocks), dim3(256), 0, s, g64, g32, full, comp, g.L[0],
+                     g.L[1], g.F[0], g.F[1], g.F[2]);
+  return hip_ok(hipGetLastError(), "green fold");
+}
+
+bool green_build_device(const GridPlan& g, const double2* tw64[3], double d, double k0, int nquad,
+                        double2* full, double2* g64, float2* g32, hipStream_t s) {
+  const long long M = (long long)g.L[0] * g.L[1] * g.L[2];
+  double2* near = nullptr;
+  double2* part = nullptr;
+  int* offs = nullptr;
+  bool ok = true;
+  if (nquad > 0) {
+    // Near cells: |offset|^2 <= nquad^2 inside the box reach (calculgreenquadfft.f:50).
+    std::vector<int> h;
+    for (int k = -nquad; k <= nquad; ++k)
+      for (int j = -nquad; j <= nquad; ++j)
+        for (int i = -nquad; i <= nquad; ++i)
+          if (i * i + j * j + k * k <= nquad * nquad) {
+            h.push_back(i);
+            h.push_back(j);
+            h.push_back(k);
+          }
+    const int ncell = (int)h.size() / 3;
+    const int w = 2 * nquad + 1;
+    ok = hip_ok(hipMalloc(&near, sizeof(double2) * 6 * w * w * w), "hipMalloc near") &&
+         hip_ok(hipMalloc(&part, sizeof(double2) * 6 * 129 * (size_t)ncell), "hipMalloc part") &&
+         hip_ok(hipMalloc(&offs, sizeof(int) * h.size()), "hipMalloc offs") &&
+         hip_ok(hipMemcpyAsync(offs, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, s),
+                "copy offs") &&
+         hip_ok(hipMemsetAsync(near, 0, sizeof(double2) * 6 * w * w * w, s), "memset near");
+    if (ok) {
+      hipLaunchKernelGGL(k_green_near_partial, dim3(129, ncell), dim3(256), 0, s, part, offs, d, k0);
+      hipLaunchKernelGGL(k_green_near_final, dim3((ncell + 63) / 64), dim3(64), 0, s, near, part,
+                         offs, ncell, nquad, d, k0, 1.0);
+      ok = hip_ok(hipGetLastError(), "green near");
+    }
+  }
+  for (int c = 0; ok && c < 6; ++c) {
+    const int blocks = (int)std::min<long long>((M + 255) / 256, 65535);
+    hipLaunchKernelGGL(k_green_fill, dim3(blocks), dim3(256), 0, s, full, c, g.nx, g.ny, g.nz,
+                       g.L[0], g.L[1], g.L[2], d, k0, near, nquad);
+    ok = hip_ok(hipGetLastError(), "green fill") && fft3d_full_f64(g, tw64, full, s) &&
+         green_fold_component(g, full, c, g64, g32, s);
+  }
+  if (near || part || offs) {
+    hip_ok(hipStreamSynchronize(s), "green sync");
+    (void)hipFree(near);
+    (void)hipFree(part);
+    (void)hipFree(offs);
+  }
+  return ok;
+}
+
+}  // namespace ifdda

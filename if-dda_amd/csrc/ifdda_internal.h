@@ -1,0 +1,120 @@
+// ifdda_internal.h -- host-side plan and launcher declarations shared by the
+// translation units of libifdda_hip.so (not part of the C ABI).
+#pragma once
+
+#include <string>
+
+#include "ifdda_common.h"
+
+namespace ifdda {
+
+// Tile shape of one FFT pass: B batch lines per workgroup, `threads` threads.
+struct TileShape {
+  int B;
+  int threads;
+  size_t lds;  // dynamic LDS bytes
+};
+
+// Geometry of one box and its padded FFT grid.
+struct GridPlan {
+  int nx, ny, nz;   // box cells
+  int L[3];         // padded FFT lengths (2,3,5-smooth, >= 2n)
+  int F[3];         // folded lengths L/2 + 1
+  AxisPlan ax[3];
+  // tile shapes per pass and precision (index 0 = FP64, 1 = FP32)
+  TileShape p1[2], p2[2], p3[2], p4[2], p5[2];
+  TileShape gx, gy, gz;  // Green full-grid passes (FP64, 1 component)
+};
+
+// Device-side matvec operands of one precision.
+template <typename V> struct MatvecDev {
+  V* A;               // nz*ny*L1*3 scratch (after x pass)
+  V* B;               // nz*L2*L1*3 scratch (after y pass)
+  const V* G;         // folded Green tensor, F2*F3*F1*6
+  const V* tw[3];     // twiddles exp(+2 pi i e/L) per axis
+  const int32_t* tab; // box -> 1-based dipole, or null (dense box)
+  const V* alpha;     // per dipole, see alpha_mode
+  int alpha_mode;
+};
+
+// Report the first error (HIP or argument) and keep its text.
+void set_error(const std::string& msg);
+bool hip_ok(hipError_t e, const char* what);
+
+int smooth235_at_least(int n);
+bool make_axis_plan(int L, AxisPlan* out);
+void make_tiles(GridPlan& g, size_t lds_limit);
+
+// y = b - (1/M) IFFT[G . FFT[scatter(alpha . x)]]  (all device pointers).
+// skip: optional device flag; when it reads non-zero the passes do nothing.
+bool launch_matvec_f64(const GridPlan& g, const MatvecDev<double2>& m, const double2* x,
+                       const double2* b, double2* y, hipStream_t s, const int* skip = nullptr);
+bool launch_matvec_f32(const GridPlan& g, const MatvecDev<float2>& m, const float2* x,
+                       const float2* b, float2* y, hipStream_t s, const int* skip = nullptr);
+
+// Green tensor on the device: fills g64/g32 (folded, F2*F3*F1*6) using
+// `full` (L1*L2*L3 FP64 scratch).  nquad > 0 integrates near cells.
+bool green_build_device(const GridPlan& g, const double2* tw64[3], double d, double k0, int nquad,
+                        double2* full, double2* g64, float2* g32, hipStream_t s);
+// Fold one caller-supplied FFT'd component already in `full` into g64/g32.
+bool green_fold_component(const GridPlan& g, const double2* full, int comp, double2* g64,
+                          float2* g32, hipStream_t s);
+
+}  // namespace ifdda
+
+// The context behind the opaque ifdda_ctx handle of ifdda.h.
+struct ifdda_ctx {
+  ifdda::GridPlan g{};
+  hipStream_t stream = nullptr;
+  int device = 0;
+  double2* tw64[3] = {nullptr, nullptr, nullptr};
+  float2* tw32[3] = {nullptr, nullptr, nullptr};
+  double2* G64 = nullptr;  // folded Green tensor, F2*F3*F1*6
+  float2* G32 = nullptr;
+  bool green_ready = false;
+  int32_t* tab = nullptr;  // device dipole map (null: dense box)
+  long long nd = 0;        // dipoles
+  bool have_dipoles = false;
+  double2* al64 = nullptr;  // polarisability (scalar or 3x3 row-major per dipole)
+  float2* al32 = nullptr;
+  int alpha_mode = ifdda::kAlphaNone;
+  void* scrA = nullptr;  // FFT scratch sized for FP64
+  void* scrB = nullptr;
+  double2* hx = nullptr;  // device staging for host-array entry points
+  double2* hb = nullptr;
+  size_t hcap = 0;        // complex elements in hx / hb
+  void* solver = nullptr; // Krylov workspace (ifdda_krylov.hip)
+  // provenance of the device Green tensor for the reference-shaped entries
+  const void* green_host = nullptr;
+  double green_sample[6][2 * 16] = {};
+  uint64_t tab_hash = 0;
+  bool tab_dense = false;
+
+  ifdda::MatvecDev<double2> mv64() const {
+    ifdda::MatvecDev<double2> m;
+    m.A = (double2*)scrA;
+    m.B = (double2*)scrB;
+    m.G = G64;
+    for (int i = 0; i < 3; ++i) m.tw[i] = tw64[i];
+    m.tab = tab;
+    m.alpha = al64;
+    m.alpha_mode = alpha_mode;
+    return m;
+  }
+  ifdda::MatvecDev<float2> mv32() const {
+    ifdda::MatvecDev<float2> m;
+    m.A = (float2*)scrA;
+    m.B = (float2*)scrB;
+    m.G = G32;
+    for (int i = 0; i < 3; ++i) m.tw[i] = tw32[i];
+    m.tab = tab;
+    m.alpha = al32;
+    m.alpha_mode = alpha_mode;
+    return m;
+  }
+};
+
+namespace ifdda {
+// Release the Krylov workspace (ifdda_krylov.hip).
+void solver_free(ifdda_ctx* c);
+}  // namespace ifdda

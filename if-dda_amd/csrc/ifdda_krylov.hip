@@ -1,0 +1,1312 @@
+// ifdda_krylov.hip -- device-resident Krylov solvers behind ifdda_solve():
+//   GPBICG       gpbicg.f:81-338
+//   QMRBICGSTAB  qmrbicgstab.f:1-253 (methode 1 and 2)
+//   TFQMR        tfqmr.f:1-218
+// and the inverserig.f driver around them (x0, true residual, FF = alpha.E).
+//
+// Every scalar of the recurrences lives on the device (KState) and is
+// computed in FP64 by one-block "finish" kernels from per-block partial sums
+// (fixed block count, fixed summation order: results do not depend on what
+// else runs).  Vector kernels and matvec passes read KState.done and return at
+// once when the solve has ended, so the host may queue whole iterations and
+// poll the state only now and then.  Dots are sum conj(a) b in FP64 for both
+// vector precisions.
+//
+// precision 32: iterative refinement.  The residual r = E_inc - A x is formed
+// with the FP64 matvec; A e = r is solved with FP32 vectors and the FP32
+// matvec to a relative tolerance; x += e; repeat until |r|/|E_inc| <= tol.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "../../include/ifdda.h"
+#include "ifdda_internal.h"
+
+namespace ifdda {
+
+constexpr int kThreads = 256;
+constexpr int kMaxBlocks = 1024;
+constexpr int kMaxDots = 5;
+
+struct KState {
+  double norm;    // |b|
+  double residu;  // recurrence residual estimate
+  double tol;
+  double2 alpha, beta, eta, dzeta, r0rn;    // GPBICG (eta/alpha shared by all)
+  double2 rho, omega, tau, taut, theta, thetat, etat;  // QMRBICGSTAB / TFQMR
+  double2 dots[kMaxDots];                   // last finished reduction
+  int it;          // ITNO of the iteration in progress
+  int done;        // solve ended (converged, breakdown or iteration cap)
+  int status;      // 1 converged / cap, -1 breakdown
+  int steperr;
+  int maxit;
+  int methode;     // QMRBICGSTAB omega formula
+  int ntrace, tracecap;
+};
+
+// ---------------------------------------------------------------- complex FP64
+__device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 zsub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 zmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 zconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ bool zzero(double2 a) { return a.x == 0.0 && a.y == 0.0; }
+__device__ __forceinline__ double zabs(double2 a) { return hypot(a.x, a.y); }
+// Smith's division
+__device__ __forceinline__ double2 zdiv(double2 a, double2 b) {
+  if (fabs(b.x) >= fabs(b.y)) {
+    const double r = b.y / b.x, d = b.x + b.y * r;
+    return make_double2((a.x + a.y * r) / d, (a.y - a.x * r) / d);
+  }
+  const double r = b.x / b.y, d = b.y + b.x * r;
+  return make_double2((a.x * r + a.y) / d, (a.y * r - a.x) / d);
+}
+// principal square root
+__device__ __forceinline__ double2 zsqrt(double2 z) {
+  if (z.x == 0.0 && z.y == 0.0) return make_double2(0.0, 0.0);
+  const double r = hypot(z.x, z.y);
+  if (z.x >= 0.0) {
+    const double t = sqrt(0.5 * (r + z.x));
+    return make_double2(t, z.y / (2.0 * t));
+  }
+  const double t = sqrt(0.5 * (r - z.x));
+  return make_double2(fabs(z.y) / (2.0 * t), copysign(t, z.y));
+}
+__device__ __forceinline__ double2 zone() { return make_double2(1.0, 0.0); }
+
+// ------------------------------------------------------- vector element access
+template <typename V> struct Vt;
+template <> struct Vt<double2> {
+  static __device__ __forceinline__ double2 ld(const double2* p, long long i) { return p[i]; }
+  static __device__ __forceinline__ void st(double2* p, long long i, double2 v) { p[i] = v; }
+};
+template <> struct Vt<float2> {
+  static __device__ __forceinline__ double2 ld(const float2* p, long long i) {
+    const float2 v = p[i];
+    return make_double2((double)v.x, (double)v.y);
+  }
+  static __device__ __forceinline__ void st(float2* p, long long i, double2 v) {
+    p[i] = make_float2((float)v.x, (float)v.y);
+  }
+};
+// Vector updates are evaluated in FP64 registers and stored in V.
+#define LD(p, i) Vt<V>::ld(p, i)
+#define ST(p, i, v) Vt<V>::st(p, i, v)
+
+// -------------------------------------------------------------- reductions
+template <int NR>
+__device__ __forceinline__ void block_partials(double2 (&v)[NR], double2* part, int nblk) {
+  for (int off = 32; off > 0; off >>= 1)
+    for (int r = 0; r < NR; ++r) {
+      v[r].x += __shfl_down(v[r].x, off, 64);
+      v[r].y += __shfl_down(v[r].y, off, 64);
+    }
+  __shared__ double2 sh[NR][kThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int r = 0; r < NR; ++r) sh[r][wid] = v[r];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int r = 0; r < NR; ++r) {
+      double2 s = make_double2(0.0, 0.0);
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s = zadd(s, sh[r][w]);
+      part[r * nblk + blockIdx.x] = s;
+    }
+}
+
+// Sum the partials of NR reductions (one block of kThreads threads) into
+// st->dots; returns in every thread after a barrier.
+template <int NR>
+__device__ __forceinline__ void finish_sum(KState* st, const double2* part, int nblk) {
+  __shared__ double2 sh[NR][kThreads];
+  for (int r = 0; r < NR; ++r) {
+    double2 s = make_double2(0.0, 0.0);
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s = zadd(s, part[r * nblk + b]);
+    sh[r][threadIdx.x] = s;
+  }
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int r = 0; r < NR; ++r) sh[r][threadIdx.x] = zadd(sh[r][threadIdx.x], sh[r][threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int r = 0; r < NR; ++r) st->dots[r] = sh[r][0];
+  __syncthreads();
+}
+
+#define GRID_LOOP(i, n) \
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < (n); i += (long long)gridDim.x * blockDim.x)
+
+__device__ __forceinline__ void push_trace(KState* st, double2* trace, double2 a, double2 b,
+                                           double2 c, double2 d, double2 e, double res) {
+  if (trace && st->ntrace < st->tracecap) {
+    double2* t = trace + 6 * st->ntrace;
+    t[0] = a; t[1] = b; t[2] = c; t[3] = d; t[4] = e; t[5] = make_double2(res, 0.0);
+    st->ntrace++;
+  }
+}
+
+// ======================================================= shared set-up kernels
+
+// x = x0 (copy of src, or 0); partial |b|^2
+template <typename V>
+__global__ void __launch_bounds__(kThreads) k_init(const V* b, V* x, const V* src, long long n,
+                                                  double2* part, int nblk) {
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  GRID_LOOP(i, n) {
+    const double2 bi = LD(b, i);
+    acc[0].x += bi.x * bi.x + bi.y * bi.y;
+    ST(x, i, src ? LD(src, i) : make_double2(0.0, 0.0));
+  }
+  block_partials<1>(acc, part, nblk);
+}
+
+__global__ void __launch_bounds__(kThreads) f_init(KState* st, const double2* part, int nblk) {
+  finish_sum<1>(st, part, nblk);
+  if (threadIdx.x == 0) st->norm = sqrt(st->dots[0].x);
+}
+
+// ============================================================ GPBICG kernels
+template <typename V> struct GV {  // gpbicg.f:103-114 work columns
+  V *x, *b, *r, *r0s, *p, *Ap, *t, *At, *y, *u, *w, *z, *told, *Ax;
+};
+
+// r = b - Ax (Ax = 0 when x0 = 0); r0* = conj(r); p = t = w = u = z = 0;
+// partial sum r0* r  (gpbicg.f:133-148)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) g_start(GV<V> v, int x0zero, long long n,
+                                                   double2* part, int nblk) {
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  const double2 z0 = make_double2(0.0, 0.0);
+  GRID_LOOP(i, n) {
+    double2 r = LD(v.b, i);
+    if (!x0zero) r = zsub(r, LD(v.Ax, i));
+    ST(v.r, i, r);
+    ST(v.r0s, i, zconj(r));
+    ST(v.p, i, z0); ST(v.t, i, z0); ST(v.w, i, z0); ST(v.u, i, z0); ST(v.z, i, z0);
+    ST(v.Ap, i, z0); ST(v.At, i, z0);
+    const double2 rs = zconj(r);
+    acc[0] = zadd(acc[0], zmul(rs, r));
+  }
+  block_partials<1>(acc, part, nblk);
+}
+
+__global__ void __launch_bounds__(kThreads) g_fstart(KState* st, const double2* part, int nblk) {
+  finish_sum<1>(st, part, nblk);
+  if (threadIdx.x == 0) {
+    st->r0rn = st->dots[0];
+    st->beta = make_double2(0.0, 0.0);
+    st->it = 0;
+  }
+}
+
+// w = At + beta Ap (label 40 of the previous iteration, gpbicg.f:319-325);
+// p = r + beta (p - u) (gpbicg.f:161-169)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) g_k1(const KState* st, GV<V> v, long long n) {
+  if (st->done) return;
+  const double2 beta = st->beta;
+  const bool first = st->it == 0;
+  GRID_LOOP(i, n) {
+    if (!first) ST(v.w, i, zadd(LD(v.At, i), zmul(beta, LD(v.Ap, i))));
+    ST(v.p, i, zadd(LD(v.r, i), zmul(beta, zsub(LD(v.p, i), LD(v.u, i)))));
+  }
+}
+
+// partial sum r0* Ap (gpbicg.f:174-181)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) g_k2(const KState* st, GV<V> v, long long n,
+                                                double2* part, int nblk) {
+  if (st->done) return;
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  GRID_LOOP(i, n) acc[0] = zadd(acc[0], zmul(LD(v.r0s, i), LD(v.Ap, i)));
+  block_partials<1>(acc, part, nblk);
+}
+
+__global__ void __launch_bounds__(kThreads) g_f2(KState* st, const double2* part, int nblk) {
+  if (st->done) return;
+  finish_sum<1>(st, part, nblk);
+  if (threadIdx.x == 0) {
+    const double2 c = st->dots[0];
+    if (zzero(c)) {  // gpbicg.f:183-187
+      st->status = -1;
+      st->steperr = 1;
+      st->done = 1;
+    } else {
+      st->alpha = zdiv(st->r0rn, c);
+    }
+  }
+}
+
+// y = t - r + alpha (Ap - w); t_old = t; t = r - alpha Ap (gpbicg.f:192-200)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) g_k3(const KState* st, GV<V> v, long long n) {
+  if (st->done) return;
+  const double2 al = st->alpha;
+  GRID_LOOP(i, n) {
+    const double2 t = LD(v.t, i), r = LD(v.r, i), Ap = LD(v.Ap, i);
+    ST(v.y, i, zadd(zsub(t, r), zmul(al, zsub(Ap, LD(v.w, i)))));
+    ST(v.told, i, t);
+    ST(v.t, i, zsub(r, zmul(al, Ap)));
+  }
+}
+
+// <At,At>, <y,y>, <At,y>, <At,t>, <y,t> (gpbicg.f:222-253)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) g_k4(const KState* st, GV<V> v, long long n,
+                                                double2* part, int nblk) {
+  if (st->done) return;
+  double2 acc[5];
+  for (int r = 0; r < 5; ++r) acc[r] = make_double2(0.0, 0.0);
+  GRID_LOOP(i, n) {
+    const double2 At = LD(v.At, i), y = LD(v.y, i), t = LD(v.t, i);
+    const double2 cAt = zconj(At), cy = zconj(y);
+    acc[0] = zadd(acc[0], zmul(cAt, At));
+    acc[1] = zadd(acc[1], zmul(cy, y));
+    acc[2] = zadd(acc[2], zmul(cAt, y));
+    acc[3] = zadd(acc[3], zmul(cAt, t));
+    acc[4] = zadd(acc[4], zmul(cy, t));
+  }
+  block_partials<5>(acc, part, nblk);
+}
+
+__global__ void __launch_bounds__(kThreads) g_f4(KState* st, const double2* part, int nblk) {
+  if (st->done) return;
+  finish_sum<5>(st, part, nblk);
+  if (threadIdx.x == 0) {
+    const double2 c1 = st->dots[0], c2 = st->dots[1], c3 = st->dots[2], c4 = st->dots[3],
+                  c5 = st->dots[4];
+    if (st->it == 0) {  // gpbicg.f:217-235
+      st->eta = make_double2(0.0, 0.0);
+      if (zzero(c1)) {
+        st->status = -1;
+        st->steperr = 2;
+        st->done = 1;
+        return;
+      }
+      st->dzeta = zdiv(c4, c1);
+    } else {  // gpbicg.f:255-264
+      const double2 den = zsub(zmul(c1, c2), zmul(c3, zconj(c3)));
+      if (zzero(den)) {
+        st->status = -1;
+        st->steperr = 3;
+        st->done = 1;
+        return;
+      }
+      st->dzeta = zdiv(zsub(zmul(c2, c4), zmul(c5, c3)), den);
+      st->eta = zdiv(zsub(zmul(c1, c5), zmul(zconj(c3), c4)), den);
+    }
+  }
+}
+
+// u, z, x, r updates; partial |r|^2 and r0* r (gpbicg.f:270-281, :300-307)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) g_k5(const KState* st, GV<V> v, long long n,
+                                                double2* part, int nblk) {
+  if (st->done) return;
+  const double2 al = st->alpha, be = st->beta, et = st->eta, dz = st->dzeta;
+  double2 acc[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+  GRID_LOOP(i, n) {
+    const double2 Ap = LD(v.Ap, i), r = LD(v.r, i), t = LD(v.t, i), y = LD(v.y, i),
+                  At = LD(v.At, i);
+    const double2 u = zadd(zmul(dz, Ap), zmul(et, zadd(zsub(LD(v.told, i), r), zmul(be, LD(v.u, i)))));
+    const double2 z = zsub(zadd(zmul(dz, r), zmul(et, LD(v.z, i))), zmul(al, u));
+    ST(v.u, i, u);
+    ST(v.z, i, z);
+    ST(v.x, i, zadd(zadd(LD(v.x, i), zmul(al, LD(v.p, i))), z));
+    const double2 rn = zsub(zsub(t, zmul(et, y)), zmul(dz, At));
+    ST(v.r, i, rn);
+    const double2 rs = LD(v.r, i);  // the stored value feeds the dots
+    acc[0].x += rs.x * rs.x + rs.y * rs.y;
+    acc[1] = zadd(acc[1], zmul(LD(v.r0s, i), rs));
+  }
+  block_partials<2>(acc, part, nblk);
+}
+
+__global__ void __launch_bounds__(kThreads) g_f5(KState* st, const double2* part, int nblk,
+                                                double2* trace) {
+  if (st->done) return;
+  finish_sum<2>(st, part, nblk);
+  if (threadIdx.x != 0) return;
+  const double res = sqrt(st->dots[0].x) / st->norm;
+  st->residu = res;
+  push_trace(st, trace, st->alpha, st->beta, st->eta, st->dzeta, st->r0rn, res);
+  if (res <= st->tol) {  // gpbicg.f:285-296
+    st->status = 1;
+    st->done = 1;
+    return;
+  }
+  const double2 ct = st->dots[1];
+  if (zzero(st->r0rn)) {  // gpbicg.f:308-312
+    st->status = -1;
+    st->steperr = 4;
+    st->done = 1;
+    return;
+  }
+  st->beta = zdiv(zdiv(zmul(st->alpha, ct), st->dzeta), st->r0rn);
+  st->r0rn = ct;
+  if (st->it > st->maxit) {  // gpbicg.f:327-330
+    st->status = 1;
+    st->steperr = 0;
+    st->done = 1;
+    return;
+  }
+  st->it++;
+}
+
+// ======================================================= QMRBICGSTAB kernels
+template <typename V> struct QV {  // qmrbicgstab.f:23-31 work columns
+  V *x, *b, *r, *p, *v, *d, *rt, *s, *xt, *dt, *t, *Ax;
+};
+
+// r = b - Ax; p = v = d = 0; rt = r; partial r.conj(r) (qmrbicgstab.f:50-62)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) q_start(QV<V> v, int x0zero, long long n,
+                                                   double2* part, int nblk) {
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  const double2 z0 = make_double2(0.0, 0.0);
+  GRID_LOOP(i, n) {
+    double2 r = LD(v.b, i);
+    if (!x0zero) r = zsub(r, LD(v.Ax, i));
+    ST(v.r, i, r);
+    ST(v.rt, i, r);
+    ST(v.p, i, z0); ST(v.v, i, z0); ST(v.d, i, z0);
+    acc[0] = zadd(acc[0], zmul(r, zconj(r)));
+  }
+  block_partials<1>(acc, part, nblk);
+}
+
+__global__ void __launch_bounds__(kThreads) q_fstart(KState* st, const double2* part, int nblk) {
+  finish_sum<1>(st, part, nblk);
+  if (threadIdx.x == 0) {  // qmrbicgstab.f:66-75
+    st->rho = zone();
+    st->alpha = zone();
+    st->omega = zone();
+    st->tau = zsqrt(st->dots[0]);
+    st->theta = make_double2(0.0, 0.0);
+    st->eta = make_double2(0.0, 0.0);
+    st->it = 1;
+  }
+}
+
+// partial rt* r (qmrbicgstab.f:79-87)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) q_k1(const KState* st, QV<V> v, long long n,
+                                                double2* part, int nblk) {
+  if (st->done) return;
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  GRID_LOOP(i, n) acc[0] = zadd(acc[0], zmul(zconj(LD(v.rt, i)), LD(v.r, i)));
+  block_partials<1>(acc, part, nblk);
+}
+
+__global__ void __launch_bounds__(kThreads) q_f1(KState* st, const double2* part, int nblk) {
+  if (st->done) return;
+  finish_sum<1>(st, part, nblk);
+  if (threadIdx.x == 0) {
+    const double2 rho0 = st->rho;
+    st->rho = st->dots[0];
+    st->beta = zdiv(zdiv(zmul(st->rho, st->alpha), rho0), st->omega);  // qmrbicgstab.f:88
+  }
+}
+
+// p = r + beta (p - omega v) (qmrbicgstab.f:92-100)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) q_k2(const KState* st, QV<V> v, long long n) {
+  if (st->done) return;
+  const double2 be = st->beta, om = st->omega;
+  GRID_LOOP(i, n)
+  ST(v.p, i, zadd(LD(v.r, i), zmul(be, zsub(LD(v.p, i), zmul(om, LD(v.v, i))))));
+}
+
+// partial rt* v (qmrbicgstab.f:105-113)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) q_k3(const KState* st, QV<V> v, long long n,
+                                                double2* part, int nblk) {
+  if (st->done) return;
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  GRID_LOOP(i, n) acc[0] = zadd(acc[0], zmul(zconj(LD(v.rt, i)), LD(v.v, i)));
+  block_partials<1>(acc, part, nblk);
+}
+
+__global__ void __launch_bounds__(kThreads) q_f3(KState* st, const double2* part, int nblk) {
+  if (st->done) return;
+  finish_sum<1>(st, part, nblk);
+  if (threadIdx.x == 0) {
+    const double2 a = st->dots[0];
+    if (zabs(a) == 0.0) {  // qmrbicgstab.f:114-118
+      st->status = -1;
+      st->steperr = 1;
+      st->done = 1;
+      return;
+    }
+    st->alpha = zdiv(st->rho, a);
+  }
+}
+
+// s = r - alpha v; partial s* s (qmrbicgstab.f:123-131)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) q_k4(const KState* st, QV<V> v, long long n,
+                                                double2* part, int nblk) {
+  if (st->done) return;
+  const double2 al = st->alpha;
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  GRID_LOOP(i, n) {
+    ST(v.s, i, zsub(LD(v.r, i), zmul(al, LD(v.v, i))));
+    const double2 s = LD(v.s, i);
+    acc[0] = zadd(acc[0], zmul(zconj(s), s));
+  }
+  block_partials<1>(acc, part, nblk);
+}
+
+// thetat, c, taut, etat (qmrbicgstab.f:132-146)
+__global__ void __launch_bounds__(kThreads) q_f4(KState* st, const double2* part, int nblk) {
+  if (st->done) return;
+  finish_sum<1>(st, part, nblk);
+  if (threadIdx.x == 0) {
+    if (zabs(st->tau) == 0.0) {
+      st->status = -1;
+      st->steperr = 2;
+      st->done = 1;
+      return;
+    }
+    st->thetat = zdiv(zsqrt(st->dots[0]), st->tau);
+    const double2 c = zdiv(zone(), zsqrt(zadd(zone(), zmul(st->thetat, st->thetat))));
+    st->taut = zmul(zmul(st->tau, st->thetat), c);
+    st->etat = zmul(zmul(c, c), st->alpha);
+  }
+}
+
+// dt = p + theta^2 eta/alpha d; xt = x + etat dt (qmrbicgstab.f:151-159)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) q_k5(const KState* st, QV<V> v, long long n) {
+  if (st->done) return;
+  const double2 f = zdiv(zmul(zmul(st->theta, st->theta), st->eta), st->alpha);
+  const double2 et = st->etat;
+  GRID_LOOP(i, n) {
+    const double2 dt = zadd(LD(v.p, i), zmul(f, LD(v.d, i)));
+    ST(v.dt, i, dt);
+    ST(v.xt, i, zadd(LD(v.x, i), zmul(et, LD(v.dt, i))));
+  }
+}
+
+// omega dots (qmrbicgstab.f:163-185)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) q_k6(const KState* st, QV<V> v, long long n,
+                                                double2* part, int nblk) {
+  if (st->done) return;
+  double2 acc[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+  const bool m1 = st->methode == 1;
+  GRID_LOOP(i, n) {
+    const double2 t = LD(v.t, i), s = LD(v.s, i);
+    if (m1) {
+      acc[0] = zadd(acc[0], zmul(zconj(t), t));
+      acc[1] = zadd(acc[1], zmul(zconj(s), t));
+    } else {
+      acc[0] = zadd(acc[0], zmul(zconj(s), t));
+      acc[1] = zadd(acc[1], zmul(zconj(s), s));
+    }
+  }
+  block_partials<2>(acc, part, nblk);
+}
+
+__global__ void __launch_bounds__(kThreads) q_f6(KState* st, const double2* part, int nblk) {
+  if (st->done) return;
+  finish_sum<2>(st, part, nblk);
+  if (threadIdx.x == 0) {
+    const double2 om = st->dots[0];
+    if (zabs(om) == 0.0) {  // qmrbicgstab.f:187-191
+      st->status = -1;
+      st->steperr = 3;
+      st->done = 1;
+      return;
+    }
+    st->omega = zdiv(st->dots[1], om);
+  }
+}
+
+// r = s - omega t; partial r* r (qmrbicgstab.f:197-205)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) q_k7(const KState* st, QV<V> v, long long n,
+                                                double2* part, int nblk) {
+  if (st->done) return;
+  const double2 om = st->omega;
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  GRID_LOOP(i, n) {
+    ST(v.r, i, zsub(LD(v.s, i), zmul(om, LD(v.t, i))));
+    const double2 r = LD(v.r, i);
+    acc[0] = zadd(acc[0], zmul(zconj(r), r));
+  }
+  block_partials<1>(acc, part, nblk);
+}
+
+// theta, c, tau, eta (qmrbicgstab.f:206-229); the x update of this iteration
+// still runs when the residual test below is met, so `done` is set by q_f8.
+__global__ void __launch_bounds__(kThreads) q_f7(KState* st, const double2* part, int nblk) {
+  if (st->done) return;
+  finish_sum<1>(st, part, nblk);
+  if (threadIdx.x == 0) {
+    if (zabs(st->taut) == 0.0) {
+      st->status = -1;
+      st->steperr = 4;
+      st->done = 1;
+      return;
+    }
+    st->theta = zdiv(zsqrt(st->dots[0]), st->taut);
+    const double2 c = zdiv(zone(), zsqrt(zadd(zone(), zmul(st->theta, st->theta))));
+    st->tau = zmul(zmul(st->taut, st->theta), c);
+    st->eta = zmul(zmul(c, c), st->omega);
+    if (zabs(st->omega) == 0.0) {
+      st->status = -1;
+      st->steperr = 5;
+      st->done = 1;
+    }
+  }
+}
+
+// d = s + thetat^2 etat/omega dt; x = xt + eta d (qmrbicgstab.f:231-238)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) q_k8(const KState* st, QV<V> v, long long n) {
+  if (st->done) return;
+  const double2 f = zdiv(zmul(zmul(st->thetat, st->thetat), st->etat), st->omega);
+  const double2 et = st->eta;
+  GRID_LOOP(i, n) {
+    ST(v.d, i, zadd(LD(v.s, i), zmul(f, LD(v.dt, i))));
+    ST(v.x, i, zadd(LD(v.xt, i), zmul(et, LD(v.d, i))));
+  }
+}
+
+// QMR residual bound (qmrbicgstab.f:241-251)
+__global__ void q_f8(KState* st, double2* trace) {
+  if (st->done) return;
+  const double res = sqrt((double)(st->it + 1)) * zabs(st->tau) / st->norm;
+  st->residu = res;
+  push_trace(st, trace, st->alpha, st->beta, st->omega, st->tau, st->eta, res);
+  if (res <= st->tol) {
+    st->status = 1;
+    st->done = 1;
+    return;
+  }
+  if (st->it > st->maxit) {
+    st->status = 1;
+    st->steperr = 0;
+    st->done = 1;
+    return;
+  }
+  st->it++;
+}
+
+// ============================================================== TFQMR kernels
+template <typename V> struct TV {  // tfqmr.f:34-41 work columns
+  V *x, *b, *r, *u, *w, *v, *d, *rt, *u1, *Au, *Auo, *Ax;
+};
+
+// r = b - Ax; w = u = r (tfqmr.f:59-69)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) t_start(TV<V> v, int x0zero, long long n) {
+  GRID_LOOP(i, n) {
+    double2 r = LD(v.b, i);
+    if (!x0zero) r = zsub(r, LD(v.Ax, i));
+    ST(v.r, i, r);
+    ST(v.w, i, r);
+    ST(v.u, i, r);
+  }
+}
+
+// v = Au (given in v.v by the matvec); d = 0; Au = v; rt = r; partial rt* r
+// (tfqmr.f:73-84)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) t_start2(TV<V> v, long long n, double2* part,
+                                                    int nblk) {
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  GRID_LOOP(i, n) {
+    ST(v.d, i, make_double2(0.0, 0.0));
+    ST(v.Au, i, LD(v.v, i));
+    const double2 r = LD(v.r, i);
+    ST(v.rt, i, r);
+    acc[0] = zadd(acc[0], zmul(zconj(r), r));
+  }
+  block_partials<1>(acc, part, nblk);
+}
+
+__global__ void __launch_bounds__(kThreads) t_fstart(KState* st, const double2* part, int nblk) {
+  finish_sum<1>(st, part, nblk);
+  if (threadIdx.x == 0) {  // tfqmr.f:87-96
+    st->rho = st->dots[0];
+    st->tau = make_double2(st->norm, 0.0);
+    st->theta = make_double2(0.0, 0.0);
+    st->eta = make_double2(0.0, 0.0);
+    st->alpha = make_double2(0.0, 0.0);
+    st->beta = make_double2(0.0, 0.0);
+    st->it = 0;
+  }
+}
+
+// even step: partial rt* v (tfqmr.f:104-112)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) t_k1(const KState* st, TV<V> v, long long n,
+                                                double2* part, int nblk) {
+  if (st->done) return;
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  GRID_LOOP(i, n) acc[0] = zadd(acc[0], zmul(zconj(LD(v.rt, i)), LD(v.v, i)));
+  block_partials<1>(acc, part, nblk);
+}
+
+__global__ void __launch_bounds__(kThreads) t_f1(KState* st, const double2* part, int nblk) {
+  if (st->done) return;
+  finish_sum<1>(st, part, nblk);
+  if (threadIdx.x == 0) st->alpha = zdiv(st->rho, st->dots[0]);
+}
+
+// (even) u1 = u - alpha v; w = w - alpha Au; d = u + theta^2/alpha eta d;
+// partial w* w (tfqmr.f:115-136)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) t_k2(const KState* st, TV<V> v, int even, long long n,
+                                                double2* part, int nblk) {
+  if (st->done) return;
+  const double2 al = st->alpha;
+  const double2 f = zmul(zdiv(zmul(st->theta, st->theta), al), st->eta);
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  GRID_LOOP(i, n) {
+    const double2 u = LD(v.u, i);
+    if (even) ST(v.u1, i, zsub(u, zmul(al, LD(v.v, i))));
+    ST(v.w, i, zsub(LD(v.w, i), zmul(al, LD(v.Au, i))));
+    ST(v.d, i, zadd(u, zmul(f, LD(v.d, i))));
+    const double2 w = LD(v.w, i);
+    acc[0] = zadd(acc[0], zmul(zconj(w), w));
+  }
+  block_partials<1>(acc, part, nblk);
+}
+
+// theta, c, tau, eta (tfqmr.f:139-144)
+__global__ void __launch_bounds__(kThreads) t_f2(KState* st, const double2* part, int nblk) {
+  if (st->done) return;
+  finish_sum<1>(st, part, nblk);
+  if (threadIdx.x == 0) {
+    st->theta = zdiv(zsqrt(st->dots[0]), st->tau);
+    const double2 c = zdiv(zone(), zsqrt(zadd(zone(), zmul(st->theta, st->theta))));
+    st->tau = zmul(zmul(st->tau, st->theta), c);
+    st->eta = zmul(zmul(c, c), st->alpha);
+  }
+}
+
+// x = x + eta d (tfqmr.f:147-153)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) t_k3(const KState* st, TV<V> v, long long n) {
+  if (st->done) return;
+  const double2 et = st->eta;
+  GRID_LOOP(i, n) ST(v.x, i, zadd(LD(v.x, i), zmul(et, LD(v.d, i))));
+}
+
+// residual bound (tfqmr.f:156-160)
+__global__ void t_f3(KState* st, double2* trace) {
+  if (st->done) return;
+  const double res = sqrt((double)(st->it + 1)) * zabs(st->tau) / st->norm;
+  st->residu = res;
+  push_trace(st, trace, st->alpha, st->beta, st->rho, st->tau, st->eta, res);
+  if (res <= st->tol) {
+    st->status = 1;
+    st->done = 1;
+  }
+}
+
+// (odd) partial rt* w (tfqmr.f:163-173)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) t_k4(const KState* st, TV<V> v, long long n,
+                                                double2* part, int nblk) {
+  if (st->done) return;
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  GRID_LOOP(i, n) acc[0] = zadd(acc[0], zmul(zconj(LD(v.rt, i)), LD(v.w, i)));
+  block_partials<1>(acc, part, nblk);
+}
+
+__global__ void __launch_bounds__(kThreads) t_f4(KState* st, const double2* part, int nblk) {
+  if (st->done) return;
+  finish_sum<1>(st, part, nblk);
+  if (threadIdx.x == 0) {
+    const double2 rho0 = st->rho;
+    st->rho = st->dots[0];
+    st->beta = zdiv(st->rho, rho0);
+  }
+}
+
+// (odd) u1 = w + beta u; Auo = Au (tfqmr.f:176-183); then u = u1 (:187-194)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) t_k5(const KState* st, TV<V> v, int odd, long long n) {
+  if (st->done) return;
+  const double2 be = st->beta;
+  GRID_LOOP(i, n) {
+    if (odd) {
+      ST(v.u, i, zadd(LD(v.w, i), zmul(be, LD(v.u, i))));
+      ST(v.Auo, i, LD(v.Au, i));
+    } else {
+      ST(v.u, i, LD(v.u1, i));
+    }
+  }
+}
+
+// (odd) v = Au + beta (Auo + beta v) (tfqmr.f:206-214)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) t_k6(const KState* st, TV<V> v, long long n) {
+  if (st->done) return;
+  const double2 be = st->beta;
+  GRID_LOOP(i, n)
+  ST(v.v, i, zadd(LD(v.Au, i), zmul(be, zadd(LD(v.Auo, i), zmul(be, LD(v.v, i))))));
+}
+
+// end of a half step: cap check, ITNO + 1 (tfqmr.f:216)
+__global__ void t_f5(KState* st) {
+  if (st->done) return;
+  if (st->it > st->maxit) {
+    st->status = 1;
+    st->steperr = 0;
+    st->done = 1;
+    return;
+  }
+  st->it++;
+}
+
+// ===================================================== driver / refinement
+// res = Ax - b, partial |res|^2 ; FF = alpha x (inverserig.f:95-105)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) k_resid(const V* Ax, const V* b, long long n,
+                                                   double2* part, int nblk) {
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  GRID_LOOP(i, n) {
+    const double2 e = zsub(LD(Ax, i), LD(b, i));
+    acc[0].x += e.x * e.x + e.y * e.y;
+  }
+  block_partials<1>(acc, part, nblk);
+}
+
+__global__ void __launch_bounds__(kThreads) f_resid(KState* st, const double2* part, int nblk) {
+  finish_sum<1>(st, part, nblk);
+}
+
+// FF = alpha . E per dipole (inverserig.f:70-82 applied to the solution)
+__global__ void __launch_bounds__(kThreads) k_alpha_apply(const double2* x, double2* out,
+                                                         const double2* al, int mode,
+                                                         long long nd) {
+  GRID_LOOP(i, nd) {
+    double2 e0 = x[3 * i], e1 = x[3 * i + 1], e2 = x[3 * i + 2];
+    if (mode == kAlphaScalar) {
+      const double2 a = al[i];
+      e0 = zmul(a, e0);
+      e1 = zmul(a, e1);
+      e2 = zmul(a, e2);
+    } else if (mode == kAlphaTensor) {
+      const double2* m = al + 9 * i;
+      const double2 f0 = zadd(zadd(zmul(m[0], e0), zmul(m[1], e1)), zmul(m[2], e2));
+      const double2 f1 = zadd(zadd(zmul(m[3], e0), zmul(m[4], e1)), zmul(m[5], e2));
+      const double2 f2 = zadd(zadd(zmul(m[6], e0), zmul(m[7], e1)), zmul(m[8], e2));
+      e0 = f0;
+      e1 = f1;
+      e2 = f2;
+    }
+    out[3 * i] = e0;
+    out[3 * i + 1] = e1;
+    out[3 * i + 2] = e2;
+  }
+}
+
+// r = b - Ax (FP64), cast to FP32 for the inner solve; partial |r|^2
+__global__ void __launch_bounds__(kThreads) k_refine_resid(const double2* b, const double2* Ax,
+                                                          int have_ax, double2* r64, float2* r32,
+                                                          long long n, double2* part, int nblk) {
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  GRID_LOOP(i, n) {
+    double2 r = b[i];
+    if (have_ax) r = zsub(r, Ax[i]);
+    r64[i] = r;
+    r32[i] = make_float2((float)r.x, (float)r.y);
+    acc[0].x += r.x * r.x + r.y * r.y;
+  }
+  block_partials<1>(acc, part, nblk);
+}
+
+// x += e (FP32 correction into the FP64 iterate)
+__global__ void __launch_bounds__(kThreads) k_refine_add(double2* x, const float2* e, long long n) {
+  GRID_LOOP(i, n) {
+    const float2 v = e[i];
+    x[i] = make_double2(x[i].x + (double)v.x, x[i].y + (double)v.y);
+  }
+}
+
+#undef LD
+#undef ST
+
+// ------------------------------------------------------------- host driver
+struct Solver {
+  long long n = 0;  // 3 * dipoles
+  int nblk = 1;
+  double2* v64[14] = {};
+  float2* v32[14] = {};
+  double2* part = nullptr;
+  KState* st = nullptr;
+  KState* st_host = nullptr;  // pinned
+  double2* trace = nullptr;
+  int tracecap = 0;
+  double2* rr64 = nullptr;  // refinement residual (FP64)
+};
+
+void solver_free(ifdda_ctx* c) {
+  Solver* S = (Solver*)c->solver;
+  if (!S) return;
+  for (int i = 0; i < 14; ++i) {
+    (void)hipFree(S->v64[i]);
+    (void)hipFree(S->v32[i]);
+  }
+  (void)hipFree(S->part);
+  (void)hipFree(S->st);
+  (void)hipHostFree(S->st_host);
+  (void)hipFree(S->trace);
+  (void)hipFree(S->rr64);
+  delete S;
+  c->solver = nullptr;
+}
+
+static Solver* solver_get(ifdda_ctx* c, bool need32) {
+  Solver* S = (Solver*)c->solver;
+  const long long n = 3 * c->nd;
+  if (S && S->n != n) {
+    solver_free(c);
+    S = nullptr;
+  }
+  if (!S) {
+    S = new Solver();
+    c->solver = S;
+    S->n = n;
+    S->nblk = (int)std::max<long long>(1, std::min<long long>(kMaxBlocks, (n + kThreads - 1) / kThreads));
+    bool ok = hip_ok(hipMalloc(&S->part, sizeof(double2) * kMaxDots * S->nblk), "hipMalloc partials") &&
+              hip_ok(hipMalloc(&S->st, sizeof(KState)), "hipMalloc state") &&
+              hip_ok(hipHostMalloc(&S->st_host, sizeof(KState)), "hipHostMalloc state");
+    for (int i = 0; ok && i < 14; ++i)
+      ok = hip_ok(hipMalloc(&S->v64[i], sizeof(double2) * std::max<long long>(n, 1)), "hipMalloc krylov");
+    if (!ok) {
+      solver_free(c);
+      return nullptr;
+    }
+  }
+  if (need32 && !S->v32[0]) {
+    bool ok = hip_ok(hipMalloc(&S->rr64, sizeof(double2) * std::max<long long>(n, 1)), "hipMalloc resid");
+    for (int i = 0; ok && i < 14; ++i)
+      ok = hip_ok(hipMalloc(&S->v32[i], sizeof(float2) * std::max<long long>(n, 1)), "hipMalloc krylov32");
+    if (!ok) {
+      solver_free(c);
+      return nullptr;
+    }
+  }
+  return S;
+}
+
+template <typename V> static bool mv(ifdda_ctx* c, const V* x, const V* b, V* y, hipStream_t s,
+                                     const int* skip);
+template <> bool mv<double2>(ifdda_ctx* c, const double2* x, const double2* b, double2* y,
+                             hipStream_t s, const int* skip) {
+  return launch_matvec_f64(c->g, c->mv64(), x, b, y, s, skip);
+}
+template <> bool mv<float2>(ifdda_ctx* c, const float2* x, const float2* b, float2* y,
+                            hipStream_t s, const int* skip) {
+  return launch_matvec_f32(c->g, c->mv32(), x, b, y, s, skip);
+}
+
+// Outcome of one Krylov run.
+struct KRun {
+  int status = 0, steperr = -1, nloop = 0, nmv = 0;
+  bool cancelled = false;
+};
+
+static bool poll(Solver* S, hipStream_t s) {
+  return hip_ok(hipMemcpyAsync(S->st_host, S->st, sizeof(KState), hipMemcpyDeviceToHost, s),
+                "copy state") &&
+         hip_ok(hipStreamSynchronize(s), "solver step");
+}
+
+// Runs one solve of A x = b on vectors `vv` (14 slots of type V, slot 0 = x
+// holding x0 unless x0zero, slot 1 = b).  Returns false on a HIP error.
+template <typename V>
+static bool krylov(ifdda_ctx* c, Solver* S, V** vv, int method, double tol, int maxit,
+                   int x0zero, int check_every, volatile int* nstop, hipStream_t s, KRun* out) {
+  const long long n = S->n;
+  const int nb = S->nblk;
+  const dim3 G(nb), T(kThreads);
+  KState init{};
+  init.tol = tol;
+  init.maxit = maxit;
+  init.methode = method == IFDDA_QMRBICGSTAB2 ? 2 : 1;
+  init.tracecap = S->tracecap;
+  init.steperr = -1;
+  if (!hip_ok(hipMemcpyAsync(S->st, &init, sizeof(KState), hipMemcpyHostToDevice, s), "state init"))
+    return false;
+  int* done = &S->st->done;
+  int nmv = 0;
+  V* x = vv[0];
+  V* b = vv[1];
+  V* Ax = vv[13];
+  hipLaunchKernelGGL(k_init<V>, G, T, 0, s, (const V*)b, x, x0zero ? (const V*)nullptr : (const V*)x,
+                     n, S->part, nb);
+  hipLaunchKernelGGL(f_init, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+  if (!x0zero) {
+    if (!mv<V>(c, x, x, Ax, s, nullptr)) return false;
+    ++nmv;
+  }
+  if (check_every <= 0) check_every = 1;
+
+  if (method == IFDDA_GPBICG1) {
+    GV<V> v{x, b, vv[2], vv[3], vv[4], vv[5], vv[6], vv[7], vv[8], vv[9], vv[10], vv[11], vv[12], Ax};
+    hipLaunchKernelGGL(g_start<V>, G, T, 0, s, v, x0zero, n, S->part, nb);
+    hipLaunchKernelGGL(g_fstart, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+    for (int it = 0;; ++it) {
+      hipLaunchKernelGGL(g_k1<V>, G, T, 0, s, (const KState*)S->st, v, n);
+      if (!mv<V>(c, v.p, v.p, v.Ap, s, done)) return false;
+      hipLaunchKernelGGL(g_k2<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
+      hipLaunchKernelGGL(g_f2, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(g_k3<V>, G, T, 0, s, (const KState*)S->st, v, n);
+      if (!mv<V>(c, v.t, v.t, v.At, s, done)) return false;
+      hipLaunchKernelGGL(g_k4<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
+      hipLaunchKernelGGL(g_f4, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(g_k5<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
+      hipLaunchKernelGGL(g_f5, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb, S->trace);
+      if (!hip_ok(hipGetLastError(), "gpbicg launch")) return false;
+      if ((it + 1) % check_every == 0 || it >= maxit + 1) {
+        if (!poll(S, s)) return false;
+        if (S->st_host->done) break;
+        if (nstop && *nstop == -1) {
+          out->cancelled = true;
+          break;
+        }
+      }
+    }
+    const KState& h = *S->st_host;
+    out->status = h.status;
+    out->steperr = h.steperr;
+    out->nloop = h.it + 1;  // ITNO starts at -1 (gpbicg.f:156), the driver adds one
+    // two matvecs per finished iteration; a breakdown at STEPERR 1 stops
+    // after the first
+    nmv += 2 * h.it + (h.status == -1 && h.steperr == 1 ? 1 : 2);
+  } else if (method == IFDDA_QMRBICGSTAB1 || method == IFDDA_QMRBICGSTAB2) {
+    QV<V> v{x, b, vv[2], vv[3], vv[4], vv[5], vv[6], vv[7], vv[8], vv[9], vv[10], Ax};
+    hipLaunchKernelGGL(q_start<V>, G, T, 0, s, v, x0zero, n, S->part, nb);
+    hipLaunchKernelGGL(q_fstart, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+    for (int it = 0;; ++it) {
+      hipLaunchKernelGGL(q_k1<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
+      hipLaunchKernelGGL(q_f1, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(q_k2<V>, G, T, 0, s, (const KState*)S->st, v, n);
+      if (!mv<V>(c, v.p, v.p, v.v, s, done)) return false;
+      hipLaunchKernelGGL(q_k3<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
+      hipLaunchKernelGGL(q_f3, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(q_k4<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
+      hipLaunchKernelGGL(q_f4, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(q_k5<V>, G, T, 0, s, (const KState*)S->st, v, n);
+      if (!mv<V>(c, v.s, v.s, v.t, s, done)) return false;
+      hipLaunchKernelGGL(q_k6<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
+      hipLaunchKernelGGL(q_f6, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(q_k7<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
+      hipLaunchKernelGGL(q_f7, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(q_k8<V>, G, T, 0, s, (const KState*)S->st, v, n);
+      hipLaunchKernelGGL(q_f8, dim3(1), dim3(1), 0, s, S->st, S->trace);
+      if (!hip_ok(hipGetLastError(), "qmrbicgstab launch")) return false;
+      if ((it + 1) % check_every == 0 || it >= maxit + 1) {
+        if (!poll(S, s)) return false;
+        if (S->st_host->done) break;
+        if (nstop && *nstop == -1) {
+          out->cancelled = true;
+          break;
+        }
+      }
+    }
+    const KState& h = *S->st_host;
+    out->status = h.status;
+    out->steperr = h.steperr;
+    out->nloop = h.it;  // ITNO counts from 1 (qmrbicgstab.f:75); no driver increment
+    const int full = h.it - 1;
+    int last = 2;
+    if (h.status == -1 && (h.steperr == 1 || h.steperr == 2)) last = 1;
+    nmv += 2 * full + last;
+  } else if (method == IFDDA_TFQMR) {
+    TV<V> v{x, b, vv[2], vv[3], vv[4], vv[5], vv[6], vv[7], vv[8], vv[9], vv[10], Ax};
+    hipLaunchKernelGGL(t_start<V>, G, T, 0, s, v, x0zero, n);
+    if (!mv<V>(c, v.u, v.u, v.v, s, nullptr)) return false;
+    ++nmv;
+    hipLaunchKernelGGL(t_start2<V>, G, T, 0, s, v, n, S->part, nb);
+    hipLaunchKernelGGL(t_fstart, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+    for (int it = 0;; ++it) {
+      const int even = (it % 2 == 0);
+      if (even) {
+        hipLaunchKernelGGL(t_k1<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
+        hipLaunchKernelGGL(t_f1, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+      }
+      hipLaunchKernelGGL(t_k2<V>, G, T, 0, s, (const KState*)S->st, v, even, n, S->part, nb);
+      hipLaunchKernelGGL(t_f2, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(t_k3<V>, G, T, 0, s, (const KState*)S->st, v, n);
+      hipLaunchKernelGGL(t_f3, dim3(1), dim3(1), 0, s, S->st, S->trace);
+      if (!even) {
+        hipLaunchKernelGGL(t_k4<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
+        hipLaunchKernelGGL(t_f4, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+      }
+      hipLaunchKernelGGL(t_k5<V>, G, T, 0, s, (const KState*)S->st, v, !even, n);
+      if (!mv<V>(c, v.u, v.u, v.Au, s, done)) return false;
+      if (!even) hipLaunchKernelGGL(t_k6<V>, G, T, 0, s, (const KState*)S->st, v, n);
+      hipLaunchKernelGGL(t_f5, dim3(1), dim3(1), 0, s, S->st);
+      if (!hip_ok(hipGetLastError(), "tfqmr launch")) return false;
+      if ((it + 1) % check_every == 0 || it >= maxit + 1) {
+        if (!poll(S, s)) return false;
+        if (S->st_host->done) break;
+        if (nstop && *nstop == -1) {
+          out->cancelled = true;
+          break;
+        }
+      }
+    }
+    const KState& h = *S->st_host;
+    out->status = h.status;
+    out->steperr = h.steperr;
+    out->nloop = h.it + 1;  // ITNO starts at -1 (tfqmr.f:96), the driver adds one
+    nmv += h.it;            // one matvec per half step that did not converge
+  } else {
+    set_error("unknown method");
+    return false;
+  }
+  out->nmv = nmv;
+  return true;
+}
+
+static bool resid_norm(Solver* S, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(f_resid, dim3(1), dim3(kThreads), 0, s, S->st, (const double2*)S->part, S->nblk);
+  if (!poll(S, s)) return false;
+  *out = std::sqrt(S->st_host->dots[0].x);
+  return true;
+}
+
+static void set_info_str(ifdda_solve_info* info, const char* m) {
+  std::memset(info->infostr, ' ', sizeof(info->infostr));
+  std::memcpy(info->infostr, m, std::min(std::strlen(m), sizeof(info->infostr)));
+}
+
+// Device solve: E_inc, FF, FFloc are device complex128 arrays (any may alias
+// the workspace-independent caller buffers).
+static int solve_dev(ifdda_ctx* c, const ifdda_solve_opts* o, const double2* Einc, double2* FF,
+                     double2* FFloc, ifdda_solve_info* info, volatile int* nstop, hipStream_t s) {
+  const auto t0 = std::chrono::steady_clock::now();
+  std::memset(info, 0, sizeof(*info));
+  set_info_str(info, "");
+  if (o->method < IFDDA_GPBICG1 || o->method > IFDDA_TFQMR) {
+    set_error("ifdda_solve: unknown method");
+    return IFDDA_ERR_ARG;
+  }
+  if (o->precision != 64 && o->precision != 32) {
+    set_error("ifdda_solve: precision must be 64 or 32");
+    return IFDDA_ERR_ARG;
+  }
+  if (!c->green_ready || !c->have_dipoles || c->alpha_mode == kAlphaNone) {
+    set_error("ifdda_solve: Green tensor, dipoles and polarisability must be set");
+    return IFDDA_ERR_STATE;
+  }
+  const int maxit = o->maxit > 0 ? o->maxit : 10000;
+  Solver* S = solver_get(c, o->precision == 32);
+  if (!S) return IFDDA_ERR_HIP;
+  const long long n = S->n;
+  const int nb = S->nblk;
+  const dim3 G(nb), T(kThreads);
+  double2* x = S->v64[0];
+  double2* b = S->v64[1];
+  double2* Ax = S->v64[13];
+  if (!hip_ok(hipMemcpyAsync(b, Einc, sizeof(double2) * n, hipMemcpyDeviceToDevice, s), "copy E_inc"))
+    return IFDDA_ERR_HIP;
+  if (!o->x0_zero &&
+      !hip_ok(hipMemcpyAsync(x, Einc, sizeof(double2) * n, hipMemcpyDeviceToDevice, s), "copy x0"))
+    return IFDDA_ERR_HIP;
+  if (nstop && *nstop == -1) {
+    info->nstop = -1;
+    set_info_str(info, "Calculation cancelled during iterative method");
+    return IFDDA_OK;
+  }
+  KRun run;
+  int ncompte = 0;
+  if (o->precision == 64) {
+    if (!krylov<double2>(c, S, S->v64, o->method, o->tol, maxit, o->x0_zero, o->check_every,
+                         nstop, s, &run))
+      return IFDDA_ERR_HIP;
+    ncompte = run.nmv;
+    info->status = run.status;
+    info->steperr = run.steperr;
+    info->nloop = run.nloop;
+  } else {
+    // Iterative refinement with FP32 inner solves.
+    double2* r64 = S->rr64;
+    float2** v32 = S->v32;
+    double bnorm = 0.0;
+    hipLaunchKernelGGL(k_init<double2>, G, T, 0, s, (const double2*)b, x,
+                       o->x0_zero ? (const double2*)nullptr : (const double2*)x, n, S->part, nb);
+    if (!resid_norm(S, &bnorm, s)) return IFDDA_ERR_HIP;
+    bool have_ax = !o->x0_zero;
+    int total_it = 0;
+    for (int outer = 0;; ++outer) {
+      if (have_ax) {
+        if (!mv<double2>(c, x, x, Ax, s, nullptr)) return IFDDA_ERR_HIP;
+        ++ncompte;
+      }
+      hipLaunchKernelGGL(k_refine_resid, G, T, 0, s, (const double2*)b, (const double2*)Ax,
+                         (int)have_ax, r64, v32[1], n, S->part, nb);
+      double rn = 0.0;
+      if (!resid_norm(S, &rn, s)) return IFDDA_ERR_HIP;
+      have_ax = true;
+      const double rho = bnorm > 0.0 ? rn / bnorm : 0.0;
+      info->outer = outer;
+      if (rho <= o->tol || outer >= 30) {
+        info->status = 1;
+        info->steperr = rho <= o->tol ? -1 : 0;
+        break;
+      }
+      if (nstop && *nstop == -1) {
+        run.cancelled = true;
+        break;
+      }
+      // Inner relative tolerance: half the remaining gap, not below what an
+      // FP32 recurrence resolves.
+      const double tin = std::max(0.5 * o->tol / rho, 1e-5);
+      KRun in;
+      if (!krylov<float2>(c, S, v32, o->method, tin, maxit, 1, o->check_every, nstop, s, &in))
+        return IFDDA_ERR_HIP;
+      ncompte += in.nmv;
+      total_it += in.nloop;
+      if (in.cancelled) {
+        run.cancelled = true;
+        break;
+      }
+      if (in.status < 0) {
+        info->status = in.status;
+        info->steperr = in.steperr;
+        break;
+      }
+      hipLaunchKernelGGL(k_refine_add, G, T, 0, s, x, (const float2*)v32[0], n);
+    }
+    info->nloop = total_it;
+  }
+  if (run.cancelled) {
+    info->nstop = -1;
+    info->ncompte = ncompte;
+    set_info_str(info, "Calculation cancelled during iterative method");
+    return IFDDA_OK;
+  }
+  if (info->status < 0) {  // inverserig.f:47-52
+    info->nstop = 1;
+    info->ncompte = ncompte;
+    set_info_str(info, "Problem to solve Ax=b!");
+    return IFDDA_OK;
+  }
+  // Final true residual with the FP64 matvec (inverserig.f:95-105).
+  if (!mv<double2>(c, x, x, Ax, s, nullptr)) return IFDDA_ERR_HIP;
+  ++ncompte;
+  hipLaunchKernelGGL(k_resid<double2>, G, T, 0, s, (const double2*)Ax, (const double2*)b, n,
+                     S->part, nb);
+  double e = 0.0, bn = 0.0;
+  if (!resid_norm(S, &e, s)) return IFDDA_ERR_HIP;
+  hipLaunchKernelGGL(k_init<double2>, G, T, 0, s, (const double2*)b, Ax, (const double2*)Ax, n,
+                     S->part, nb);
+  if (!resid_norm(S, &bn, s)) return IFDDA_ERR_HIP;
+  info->tol1 = bn > 0.0 ? e / bn : 0.0;
+  info->ncompte = ncompte;
+  if (FFloc &&
+      !hip_ok(hipMemcpyAsync(FFloc, x, sizeof(double2) * n, hipMemcpyDeviceToDevice, s), "copy E"))
+    return IFDDA_ERR_HIP;
+  if (FF) {
+    hipLaunchKernelGGL(k_alpha_apply, G, T, 0, s, (const double2*)x, FF, (const double2*)c->al64,
+                       c->alpha_mode, c->nd);
+  }
+  if (!hip_ok(hipStreamSynchronize(s), "solve")) return IFDDA_ERR_HIP;
+  info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return IFDDA_OK;
+}
+
+}  // namespace ifdda
+
+using namespace ifdda;
+
+extern "C" int ifdda_solve_device(ifdda_ctx* c, const ifdda_solve_opts* o, const void* E_inc,
+                                  void* FF, void* FFloc, ifdda_solve_info* info,
+                                  volatile int* nstop, void* stream) {
+  if (!c || !o || !E_inc || !info) {
+    set_error("ifdda_solve_device: NULL argument");
+    return IFDDA_ERR_ARG;
+  }
+  return solve_dev(c, o, (const double2*)E_inc, (double2*)FF, (double2*)FFloc, info, nstop,
+                   stream ? (hipStream_t)stream : c->stream);
+}
+
+extern "C" int ifdda_solve(ifdda_ctx* c, const ifdda_solve_opts* o, const double* E_inc,
+                           double* FF, double* FFloc, ifdda_solve_info* info,
+                           volatile int* nstop) {
+  if (!c || !o || !E_inc || !info) {
+    set_error("ifdda_solve: NULL argument");
+    return IFDDA_ERR_ARG;
+  }
+  if (!c->have_dipoles) {
+    set_error("ifdda_solve: no dipoles");
+    return IFDDA_ERR_STATE;
+  }
+  const size_t n = 3 * (size_t)c->nd;
+  if (n == 0) {
+    std::memset(info, 0, sizeof(*info));
+    return IFDDA_OK;
+  }
+  double2 *dE = nullptr, *dFF = nullptr, *dL = nullptr;
+  int rc = IFDDA_OK;
+  if (!hip_ok(hipMalloc(&dE, sizeof(double2) * n), "hipMalloc E_inc") ||
+      !hip_ok(hipMalloc(&dFF, sizeof(double2) * n), "hipMalloc FF") ||
+      !hip_ok(hipMalloc(&dL, sizeof(double2) * n), "hipMalloc FFloc") ||
+      !hip_ok(hipMemcpy(dE, E_inc, sizeof(double2) * n, hipMemcpyHostToDevice), "copy E_inc"))
+    rc = IFDDA_ERR_HIP;
+  if (!rc) rc = solve_dev(c, o, dE, dFF, dL, info, nstop, c->stream);
+  if (!rc && info->nstop == 0) {
+    if (FF && !hip_ok(hipMemcpy(FF, dFF, sizeof(double2) * n, hipMemcpyDeviceToHost), "copy FF"))
+      rc = IFDDA_ERR_HIP;
+    if (FFloc &&
+        !hip_ok(hipMemcpy(FFloc, dL, sizeof(double2) * n, hipMemcpyDeviceToHost), "copy FFloc"))
+      rc = IFDDA_ERR_HIP;
+  }
+  (void)hipFree(dE);
+  (void)hipFree(dFF);
+  (void)hipFree(dL);
+  return rc;
+}
+
+// Trace access for parity tests: copies up to cap rows of
+// (alpha, beta, s2, s3, s4, residual) recorded by the last solve.
+extern "C" int ifdda_solve_trace(ifdda_ctx* c, int cap, double* out, int* nrows) {
+  Solver* S = c ? (Solver*)c->solver : nullptr;
+  if (!S || !out || !nrows) {
+    set_error("ifdda_solve_trace: no solver state");
+    return IFDDA_ERR_STATE;
+  }
+  if (!hip_ok(hipMemcpy(S->st_host, S->st, sizeof(KState), hipMemcpyDeviceToHost), "copy state"))
+    return IFDDA_ERR_HIP;
+  const int n = std::min(cap, S->st_host->ntrace);
+  *nrows = n;
+  if (n > 0 && !hip_ok(hipMemcpy(out, S->trace, sizeof(double2) * 6 * n, hipMemcpyDeviceToHost),
+                       "copy trace"))
+    return IFDDA_ERR_HIP;
+  return IFDDA_OK;
+}
+
+// Enable per-iteration scalar recording for the next solves (cap rows).
+extern "C" int ifdda_solve_trace_enable(ifdda_ctx* c, int cap) {
+  if (!c || cap < 0) {
+    set_error("ifdda_solve_trace_enable: bad argument");
+    return IFDDA_ERR_ARG;
+  }
+  Solver* S = solver_get(c, false);
+  if (!S) return IFDDA_ERR_HIP;
+  (void)hipFree(S->trace);
+  S->trace = nullptr;
+  S->tracecap = 0;
+  if (cap > 0) {
+    if (!hip_ok(hipMalloc(&S->trace, sizeof(double2) * 6 * cap), "hipMalloc trace"))
+      return IFDDA_ERR_HIP;
+    S->tracecap = cap;
+  }
+  return IFDDA_OK;
+}

@@ -1,0 +1,224 @@
+"""ctypes binding of libifdda_hip.so (the C ABI declared in include/ifdda.h).
+
+``Solver`` wraps one ``ifdda_ctx``: a box of nx*ny*nz cells with its
+device-resident Green tensor, dipole map, polarisability and Krylov state.
+Host arrays are numpy complex128 in the reference's layouts (dipole vectors
+AoS (Ex,Ey,Ez); grids x-fastest; polarisa (nmax,3,3) with Fortran meaning).
+Device arrays may be passed as integer device pointers (e.g. from torch
+tensors, used here only as memory).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+lib_path = os.path.join(PKG, "libifdda_hip.so")
+
+GPBICG1, QMRBICGSTAB1, QMRBICGSTAB2, TFQMR = 1, 2, 3, 4
+METHODS = {"GPBICG1": GPBICG1, "QMRBICGSTAB1": QMRBICGSTAB1, "QMRBICGSTAB2": QMRBICGSTAB2,
+           "TFQMR": TFQMR}
+
+
+class IfddaError(RuntimeError):
+    pass
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("method", C.c_int), ("precision", C.c_int), ("x0_zero", C.c_int),
+                ("maxit", C.c_int), ("tol", C.c_double), ("check_every", C.c_int)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("tol1", C.c_double), ("nloop", C.c_int), ("ncompte", C.c_int), ("nstop", C.c_int),
+                ("status", C.c_int), ("steperr", C.c_int), ("outer", C.c_int),
+                ("seconds", C.c_double), ("infostr", C.c_char * 64)]
+
+    def as_dict(self):
+        return dict(tol1=self.tol1, nloop=self.nloop, ncompte=self.ncompte, nstop=self.nstop,
+                    status=self.status, steperr=self.steperr, outer=self.outer,
+                    seconds=self.seconds, infostr=self.infostr.decode(errors="replace").rstrip())
+
+
+_lib = None
+
+
+def lib():
+    """Load libifdda_hip.so once; raise if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(lib_path):
+        raise IfddaError(f"{lib_path} is missing: run __graft_entry__.build() first")
+    L = C.CDLL(lib_path)
+    vp, ip, dp = C.c_void_p, C.c_int, C.c_double
+    L.ifdda_last_error.restype = C.c_char_p
+    L.ifdda_create.argtypes = [ip, ip, ip, C.POINTER(vp)]
+    L.ifdda_destroy.argtypes = [vp]
+    L.ifdda_destroy.restype = None
+    L.ifdda_fft_lengths.argtypes = [vp, C.POINTER(C.c_int)]
+    L.ifdda_green_build.argtypes = [vp, dp, dp, ip, dp]
+    L.ifdda_green_set.argtypes = [vp, C.POINTER(vp)]
+    L.ifdda_green_get.argtypes = [vp, C.POINTER(vp)]
+    L.ifdda_set_dipoles.argtypes = [vp, vp, ip]
+    L.ifdda_set_polarizability.argtypes = [vp, vp, ip, ip]
+    L.ifdda_matvec.argtypes = [vp, vp, vp]
+    L.ifdda_matvec_device.argtypes = [vp, ip, ip, vp, vp, vp, vp]
+    L.ifdda_solve.argtypes = [vp, C.POINTER(SolveOpts), vp, vp, vp, C.POINTER(SolveInfo),
+                              C.POINTER(C.c_int)]
+    L.ifdda_solve_device.argtypes = [vp, C.POINTER(SolveOpts), vp, vp, vp, C.POINTER(SolveInfo),
+                                     C.POINTER(C.c_int), vp]
+    L.ifdda_solve_trace.argtypes = [vp, ip, vp, C.POINTER(C.c_int)]
+    L.ifdda_solve_trace_enable.argtypes = [vp, ip]
+    _lib = L
+    return L
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise IfddaError(f"{what} failed ({rc}): {lib().ifdda_last_error().decode()}")
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+class Solver:
+    """One box (nx, ny, nz) on the GPU; mirrors the cdmlib hot-path routines."""
+
+    def __init__(self, nx: int, ny: int, nz: int):
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        h = C.c_void_p()
+        _check(lib().ifdda_create(self.nx, self.ny, self.nz, C.byref(h)), "ifdda_create")
+        self.h = h
+        L = (C.c_int * 3)()
+        _check(lib().ifdda_fft_lengths(self.h, L), "ifdda_fft_lengths")
+        self.L = tuple(L)
+        self.nd = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ifdda_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- Green tensor (calculgreenfft.f / calculgreenquadfft.f) -------------
+    def greencalculfft(self, aretecube: float, k0: float, nquad: int = 0, tolinit: float = 1e-4):
+        _check(lib().ifdda_green_build(self.h, float(aretecube), float(k0), int(nquad),
+                                       float(tolinit)), "ifdda_green_build")
+
+    def green_set(self, G):
+        """G: complex128 [6, 2nz, 2ny, 2nx] in the reference layout."""
+        comps = [_c128(G[c]).reshape(-1) for c in range(6)]
+        arr = (C.c_void_p * 6)(*[_p(x) for x in comps])
+        _check(lib().ifdda_green_set(self.h, arr), "ifdda_green_set")
+
+    def green_get(self):
+        L1, L2, L3 = self.L
+        G = np.zeros((6, L3, L2, L1), np.complex128)
+        arr = (C.c_void_p * 6)(*[_p(G[c]) for c in range(6)])
+        _check(lib().ifdda_green_get(self.h, arr), "ifdda_green_get")
+        return G
+
+    # -- object ---------------------------------------------------------------
+    def set_dipoles(self, tabdip=None, ndipole: int | None = None):
+        """tabdip: int32 box map (1-based dipole or 0), None = dense box."""
+        if tabdip is None:
+            _check(lib().ifdda_set_dipoles(self.h, None, 0), "ifdda_set_dipoles")
+            self.nd = self.nx * self.ny * self.nz
+            return
+        t = np.ascontiguousarray(tabdip, np.int32).reshape(-1)
+        if t.size != self.nx * self.ny * self.nz:
+            raise IfddaError("tabdip must hold nx*ny*nz entries")
+        nd = int(t.max()) if ndipole is None else int(ndipole)
+        self._tab = t
+        _check(lib().ifdda_set_dipoles(self.h, _p(t), nd), "ifdda_set_dipoles")
+        self.nd = nd
+
+    def set_polarizability(self, pola):
+        """pola: (nd,) isotropic complex, or (nmax,3,3) tensor (Fortran meaning [i,a,b])."""
+        a = np.asarray(pola)
+        if a.ndim == 1:
+            a = _c128(a)
+            _check(lib().ifdda_set_polarizability(self.h, _p(a), a.size, 1),
+                   "ifdda_set_polarizability")
+        else:
+            f = np.ascontiguousarray(np.asfortranarray(a, np.complex128).reshape(-1, order="F"))
+            _check(lib().ifdda_set_polarizability(self.h, _p(f), a.shape[0], 0),
+                   "ifdda_set_polarizability")
+
+    # -- matvec (produitfftmatvect3.f) ---------------------------------------
+    def produitfftmatvect3(self, FFX, FFB):
+        """Returns FFB - conv(FFX) (FFX already alpha-multiplied), FP64."""
+        x = _c128(FFX).reshape(-1)
+        y = _c128(FFB).reshape(-1).copy()
+        _check(lib().ifdda_matvec(self.h, _p(x), _p(y)), "ifdda_matvec")
+        return y
+
+    def matvec_device(self, x_ptr, b_ptr, y_ptr, prec=64, apply_alpha=True, stream=None):
+        _check(lib().ifdda_matvec_device(self.h, int(prec), int(apply_alpha), C.c_void_p(x_ptr),
+                                         C.c_void_p(b_ptr), C.c_void_p(y_ptr),
+                                         C.c_void_p(stream) if stream else None),
+               "ifdda_matvec_device")
+
+    # -- solve (inverserig.f) -------------------------------------------------
+    @staticmethod
+    def _opts(method, tol, precision, x0_zero, maxit, check_every):
+        return SolveOpts(METHODS[method] if isinstance(method, str) else int(method),
+                         int(precision), int(bool(x0_zero)), int(maxit), float(tol),
+                         int(check_every))
+
+    def inverserig(self, E_inc, method="GPBICG1", tol=1e-4, precision=64, x0_zero=False,
+                   maxit=10000, check_every=1, nstop=None):
+        """Solve (I - T alpha) E = E_inc.  Returns dict(FF, FFloc, tol1, nloop, ...)."""
+        b = _c128(E_inc).reshape(-1)
+        FF = np.zeros_like(b)
+        FFloc = np.zeros_like(b)
+        o = self._opts(method, tol, precision, x0_zero, maxit, check_every)
+        info = SolveInfo()
+        stop = C.c_int(0) if nstop is None else nstop
+        _check(lib().ifdda_solve(self.h, C.byref(o), _p(b), _p(FF), _p(FFloc), C.byref(info),
+                                 C.byref(stop)), "ifdda_solve")
+        out = info.as_dict()
+        out.update(FF=FF, FFloc=FFloc)
+        return out
+
+    def solve_device(self, Einc_ptr, FF_ptr, FFloc_ptr, method="GPBICG1", tol=1e-4, precision=64,
+                     x0_zero=False, maxit=10000, check_every=1, stream=None):
+        o = self._opts(method, tol, precision, x0_zero, maxit, check_every)
+        info = SolveInfo()
+        stop = C.c_int(0)
+        _check(lib().ifdda_solve_device(self.h, C.byref(o), C.c_void_p(Einc_ptr),
+                                        C.c_void_p(FF_ptr) if FF_ptr else None,
+                                        C.c_void_p(FFloc_ptr) if FFloc_ptr else None,
+                                        C.byref(info), C.byref(stop),
+                                        C.c_void_p(stream) if stream else None),
+               "ifdda_solve_device")
+        return info.as_dict()
+
+    def trace_enable(self, cap: int):
+        _check(lib().ifdda_solve_trace_enable(self.h, int(cap)), "ifdda_solve_trace_enable")
+
+    def trace(self, cap: int = 100000):
+        out = np.zeros(6 * cap, np.complex128)
+        n = C.c_int(0)
+        _check(lib().ifdda_solve_trace(self.h, int(cap), _p(out), C.byref(n)), "ifdda_solve_trace")
+        return out.reshape(-1, 6)[: n.value].copy()

@@ -1,0 +1,155 @@
+"""Host-side producers of solve inputs: lattices, polarisability, incident waves.
+
+These restate the reference's callers of the hot path (object, beam and
+polarisability routines, which stay on the host and call the solve
+unchanged) so that benchmarks and tests can build the workloads:
+
+  sphere lattice        objectsphere.f:98-113  (centres -r + d(k - 1/2), |r| < R)
+  cube lattice          objectcube.f:95-106    (centres k d - (side + d)/2)
+  polarisability        polarisabilite.f:20-44 (CM, RR)
+  linear plane wave     ondeplane.f:32-49
+  circular plane wave   ondecirce.f:23-52
+  random permittivity   a seeded Gaussian random field with correlation
+                        sigma^2 exp(-r^2/l^2) (the statistics of
+                        objectparainhomo.f:150-205; not its realisation)
+
+Units are SI (metres); the reference converts its nm inputs the same way.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+
+@dataclass
+class Scene:
+    nx: int
+    ny: int
+    nz: int
+    d: float                 # lattice spacing [m]
+    tabdip: np.ndarray       # int32 [nx*ny*nz], 1-based dipole or 0
+    pos: np.ndarray          # float64 [nd, 3] dipole positions [m]
+    eps: np.ndarray          # complex128 [nd] permittivity
+
+    @property
+    def nd(self) -> int:
+        return int(self.pos.shape[0])
+
+
+def _lattice(centres_x, centres_y, centres_z, inside):
+    """Box order x fastest; inside: bool [nz, ny, nx]."""
+    nz, ny, nx = inside.shape
+    flat = inside.reshape(-1)
+    nd = int(flat.sum())
+    tab = np.zeros(flat.size, np.int32)
+    tab[flat] = np.arange(1, nd + 1, dtype=np.int32)
+    Z, Y, X = np.meshgrid(centres_z, centres_y, centres_x, indexing="ij")
+    pos = np.stack([X.reshape(-1)[flat], Y.reshape(-1)[flat], Z.reshape(-1)[flat]], axis=1)
+    return tab, pos
+
+
+def sphere(nnnr: int, radius: float, eps: complex) -> Scene:
+    d = 2.0 * radius / nnnr
+    c = -radius + d * (np.arange(1, nnnr + 1) - 0.5)
+    Z, Y, X = np.meshgrid(c, c, c, indexing="ij")
+    inside = np.sqrt(X * X + Y * Y + Z * Z) < radius
+    tab, pos = _lattice(c, c, c, inside)
+    return Scene(nnnr, nnnr, nnnr, d, tab, pos, np.full(pos.shape[0], eps, np.complex128))
+
+
+def cube(nnnr: int, side: float, eps) -> Scene:
+    d = side / nnnr
+    c = np.arange(1, nnnr + 1) * d - (side + d) / 2.0
+    inside = np.ones((nnnr, nnnr, nnnr), bool)
+    tab, pos = _lattice(c, c, c, inside)
+    e = np.broadcast_to(np.asarray(eps, np.complex128), (pos.shape[0],)).copy()
+    return Scene(nnnr, nnnr, nnnr, d, tab, pos, e)
+
+
+def gaussian_random_field(shape, d: float, sigma: float, corr: float, seed: int) -> np.ndarray:
+    """Real field on a (nz,ny,nx) lattice, covariance sigma^2 exp(-r^2/corr^2).
+
+    White noise filtered in k-space by the square root of the spectrum
+    exp(-k^2 corr^2 / 4), then rescaled to the exact sample std sigma."""
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal(shape)
+    ks = [2 * np.pi * np.fft.fftfreq(n, d) for n in shape]
+    K2 = ks[0][:, None, None] ** 2 + ks[1][None, :, None] ** 2 + ks[2][None, None, :] ** 2
+    f = np.real(np.fft.ifftn(np.fft.fftn(w) * np.exp(-K2 * corr * corr / 8.0)))
+    f -= f.mean()
+    return f * (sigma / f.std())
+
+
+def inhomogeneous_cube(nnnr: int, side: float, eps_mean: complex, sigma: float, corr: float,
+                       seed: int) -> Scene:
+    s = cube(nnnr, side, eps_mean)
+    field = gaussian_random_field((nnnr, nnnr, nnnr), s.d, sigma, corr, seed)
+    s.eps = (eps_mean + field.reshape(-1)).astype(np.complex128)
+    return s
+
+
+def polarizability(eps, d: float, k0: float, method: str = "RR", eps0: float = 1.0):
+    """poladiff with dddis = 1 (polarisabilite.f:20-44): cell of side d."""
+    eps = np.asarray(eps, np.complex128)
+    a3 = (d * (0.75 / np.pi) ** (1.0 / 3.0)) ** 3
+    cm = (eps - eps0) / (eps + 2.0 * eps0) * a3
+    if method == "CM":
+        return cm * eps0
+    if method == "RR":
+        k03 = k0 ** 3 * np.sqrt(eps0) ** 3
+        return eps0 * cm / (1.0 - 2.0 / 3.0 * 1j * k03 * cm)
+    raise ValueError(f"polarizability method {method!r} not provided")
+
+
+def plane_wave(pos, k0: float, theta_deg: float = 0.0, phi_deg: float = 0.0, pp: float = 1.0,
+               E0: complex = 1.0) -> np.ndarray:
+    """ondeplane.f:32-49; returns AoS (Ex,Ey,Ez) per dipole, complex128 [3*nd]."""
+    th, ph = np.radians(theta_deg), np.radians(phi_deg)
+    p, s = np.sqrt(pp), np.sqrt(1.0 - pp)
+    sg = np.sign(th) if th != 0 else 1.0
+    E0s, E0p = E0 * s * sg, E0 * p * sg
+    k = k0 * np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+    e = np.array([-E0s * np.sin(ph) + E0p * np.cos(th) * np.cos(ph),
+                  E0s * np.cos(ph) + E0p * np.cos(th) * np.sin(ph),
+                  -E0p * np.sin(th)], np.complex128)
+    ph_ = np.exp(1j * (pos @ k))
+    return (ph_[:, None] * e[None, :]).reshape(-1)
+
+
+def circular_wave(pos, k0: float, theta_deg: float = 0.0, phi_deg: float = 0.0,
+                  helicity: float = 1.0, E0: complex = 1.0) -> np.ndarray:
+    """ondecirce.f:23-52: s part E0/sqrt2 plus p part i sign(helicity) E0/sqrt2."""
+    th, ph = np.radians(theta_deg), np.radians(phi_deg)
+    E0s = E0 / np.sqrt(2.0)
+    E0p = E0 * 1j * np.sign(helicity) / np.sqrt(2.0)
+    k = k0 * np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+    e = np.array([-E0s * np.sin(ph) + E0p * np.cos(th) * np.cos(ph),
+                  E0s * np.cos(ph) + E0p * np.cos(th) * np.sin(ph),
+                  -E0p * np.sin(th)], np.complex128)
+    ph_ = np.exp(1j * (pos @ k))
+    return (ph_[:, None] * e[None, :]).reshape(-1)
+
+
+def config1():
+    """Sphere of diameter 400 nm, eps 2.25, 20^3 lattice, lambda 632.8 nm, RR,
+    x-polarised plane wave along +z."""
+    lam = 632.8e-9
+    k0 = 2 * np.pi / lam
+    s = sphere(20, 200e-9, 2.25 + 0j)
+    alpha = polarizability(s.eps, s.d, k0, "RR")
+    E = plane_wave(s.pos, k0, 0.0, 0.0, pp=1.0)
+    return s, k0, alpha, E
+
+
+def config2(n: int = 100, seed: int = 2):
+    """n^3 cube of side n*20 nm, eps 2.25+0.01i with a Gaussian random real
+    perturbation (std 0.1, correlation 100 nm), lambda 632.8 nm, circular
+    plane wave at theta = 30 deg, phi = 0."""
+    lam = 632.8e-9
+    k0 = 2 * np.pi / lam
+    d = 20e-9
+    s = inhomogeneous_cube(n, n * d, 2.25 + 0.01j, 0.1, 100e-9, seed)
+    alpha = polarizability(s.eps, s.d, k0, "RR")
+    E = circular_wave(s.pos, k0, 30.0, 0.0, 1.0)
+    return s, k0, alpha, E

@@ -1,0 +1,206 @@
+/*
+ * ifdda.h -- C ABI of libifdda_hip.so, the gfx950 (MI355X) implementation of
+ * IF-DDA's FFT-accelerated discrete-dipole solve.
+ *
+ * Two layers:
+ *   1. a handle API (ifdda_ctx) holding the device-resident Green tensor,
+ *      dipole map, polarisability and Krylov state of one box;
+ *   2. entry points with the exact argument lists of the cdmlib Fortran
+ *      routines they replace (every argument by reference, Fortran default
+ *      INTEGER = int32, DOUBLE COMPLEX = two doubles, plans = int64).  The
+ *      iso_c_binding module if-dda_amd/fortran/ifdda_dropin.f90 exposes them
+ *      to Fortran callers under the reference's own subroutine names.
+ *
+ * Complex arrays are interleaved (re, im) doubles in the reference's layout:
+ * dipole vectors AoS (Ex,Ey,Ez) per dipole; grids x-fastest; polarisa
+ * (nmax,3,3) column-major.  All functions return IFDDA_OK (0) or an error
+ * code; ifdda_last_error() gives the text.  One GPU (the current HIP device).
+ */
+#ifndef IFDDA_H
+#define IFDDA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IFDDA_OK 0
+#define IFDDA_ERR_ARG 1     /* invalid argument or unsupported size          */
+#define IFDDA_ERR_HIP 2     /* a HIP runtime call failed                       */
+#define IFDDA_ERR_STATE 3   /* Green tensor / dipoles / alpha not set up       */
+
+/* Solver selection: the methodeit strings of inverserig.f:44,607,816,880. */
+#define IFDDA_GPBICG1 1
+#define IFDDA_QMRBICGSTAB1 2
+#define IFDDA_QMRBICGSTAB2 3
+#define IFDDA_TFQMR 4
+
+typedef struct ifdda_ctx ifdda_ctx;
+
+/* Text of the last error on this thread. */
+const char *ifdda_last_error(void);
+
+/* ---------------------------------------------------------------------- */
+/* Handle API                                                             */
+/* ---------------------------------------------------------------------- */
+
+/* Context for a box of nx*ny*nz cells.  Replaces the grid sizing and FFTW
+ * plan creation of cdmlib.f:1141-1165: the padded FFT lengths are the
+ * smallest 2,3,5-smooth L >= 2n per axis (L = 2n whenever 2n is smooth). */
+int ifdda_create(int nx, int ny, int nz, ifdda_ctx **out);
+void ifdda_destroy(ifdda_ctx *ctx);
+int ifdda_fft_lengths(const ifdda_ctx *ctx, int L[3]);
+
+/* FFT'd Green tensor built on the device in FP64 (kept also as FP32):
+ * greencalculfft (calculgreenfft.f:1-104) for nquad = 0, and
+ * greencalculquadfft (calculgreenquadfft.f:1-105) for nquad > 0.
+ * aretecube = lattice spacing d [m], k0 = 2 pi / lambda [1/m]. */
+int ifdda_green_build(ifdda_ctx *ctx, double aretecube, double k0, int nquad, double tolinit);
+
+/* Take a caller-computed FFT'd Green tensor in the reference layout: six
+ * complex128 arrays (xx,xy,xz,yy,yz,zz) of (2nx)(2ny)(2nz) points, as filled
+ * by the reference's greencalculfft.  Requires L = 2n on every axis.  The
+ * even/odd symmetry of the tensor is checked; the device keeps one octant. */
+int ifdda_green_set(ifdda_ctx *ctx, const double *const G[6]);
+
+/* Unfold the device tensor into six full (L1 L2 L3) complex128 arrays. */
+int ifdda_green_get(ifdda_ctx *ctx, double *const G[6]);
+
+/* Dipole map: tabdip[i + nx*(j + ny*k)] = 1-based dipole number or 0
+ * (the Tabdip array of produitfftmatvect3.f:37-56).  tabdip == NULL selects
+ * the dense box, dipole index = box index (produitfftmatvectopt.f:37-50). */
+int ifdda_set_dipoles(ifdda_ctx *ctx, const int32_t *tabdip, int ndipole);
+
+/* Polarisability per dipole.  layout 0: the reference's polarisa(nmax,3,3)
+ * column-major complex128 (inverserig.f:70-82); layout 1: one isotropic
+ * complex128 per dipole. */
+int ifdda_set_polarizability(ifdda_ctx *ctx, const double *pola, int nmax, int layout);
+
+/* FFB <- FFB - (1/M) IFFT[G . FFT[scatter(FFX)]] on host arrays of
+ * 3*ndipole complex128, FP64 (produitfftmatvect3.f:2-129 semantics: FFX is
+ * already alpha-multiplied). */
+int ifdda_matvec(ifdda_ctx *ctx, const double *FFX, double *FFB);
+
+/* Device-resident matvec y = b - T(alpha.x) (alpha applied if apply_alpha,
+ * else y = b - T x).  prec = 64: complex128 buffers; prec = 32: complex64.
+ * x, b, y are device pointers to 3*ndipole complex values; y may alias b.
+ * stream: a hipStream_t or NULL for the context's stream. */
+int ifdda_matvec_device(ifdda_ctx *ctx, int prec, int apply_alpha, const void *x, const void *b,
+                        void *y, void *stream);
+
+/* Solve options (the reference's methodeit / tol / nlim, plus precision). */
+typedef struct {
+  int method;      /* IFDDA_GPBICG1 ... IFDDA_TFQMR                        */
+  int precision;   /* 64: plain FP64; 32: FP32 matvec + FP64 refinement    */
+  int x0_zero;     /* 0: x0 = E_inc as inverserig.f:36-42; 1: x0 = 0        */
+  int maxit;       /* iteration cap; <= 0 means 10000 (inverserig.f:31)     */
+  double tol;      /* relative residual tolerance                          */
+  int check_every; /* iterations between host convergence polls (0: auto) */
+} ifdda_solve_opts;
+
+typedef struct {
+  double tol1;     /* true residual |A E - E_inc| / |E_inc| (FP64 matvec)  */
+  int nloop;       /* iterations, counted as the reference reports them    */
+  int ncompte;     /* matvecs performed (FP64 + FP32)                      */
+  int nstop;       /* 0 ok, 1 failure (infostr), -1 cancelled              */
+  int status;      /* solver STATUS: 1 done, -1 breakdown                  */
+  int steperr;     /* solver STEPERR                                       */
+  int outer;       /* refinement sweeps (precision 32), else 0             */
+  double seconds;  /* wall time of the solve                               */
+  char infostr[64];
+} ifdda_solve_info;
+
+/* inverserig.f:1-1090 on host arrays: solves (I - T alpha) E = E_inc for the
+ * 3*ndipole complex128 vector E.  Outputs FFloc = E (local field) and
+ * FF = alpha.E (dipole moments); either may be NULL.  nstop, if not NULL,
+ * is polled between iterations: another thread setting it to -1 cancels. */
+int ifdda_solve(ifdda_ctx *ctx, const ifdda_solve_opts *opts, const double *E_inc, double *FF,
+                double *FFloc, ifdda_solve_info *info, volatile int *nstop);
+
+/* Same with device pointers (complex128, 3*ndipole each). */
+int ifdda_solve_device(ifdda_ctx *ctx, const ifdda_solve_opts *opts, const void *E_inc, void *FF,
+                       void *FFloc, ifdda_solve_info *info, volatile int *nstop, void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* Reference-shaped entry points (all arguments by reference).            */
+/* ---------------------------------------------------------------------- */
+
+/* produitfftmatvect3.f:2-5 */
+void ifdda_produitfftmatvect3(const double *Gxx, const double *Gxy, const double *Gxz,
+                              const double *Gyy, const double *Gyz, const double *Gzz, double *vectx,
+                              double *vecty, double *vectz, const int32_t *Tabdip,
+                              const int32_t *ntotalm, const int32_t *ntotal, const int32_t *nmax,
+                              const int32_t *ndipole, const int32_t *nxm, const int32_t *nym,
+                              const int32_t *nzm, const int32_t *nx, const int32_t *ny,
+                              const int32_t *nz, const int32_t *nx2, const int32_t *ny2,
+                              const int32_t *nxy2, const int32_t *nz2, const double *FFX,
+                              double *FFB, const int64_t *planb, const int64_t *planf);
+
+/* produitfftmatvectopt.f:3-6 (note the plan order planf, planb) */
+void ifdda_produitfftmatvectopt(const double *Gxx, const double *Gxy, const double *Gxz,
+                                const double *Gyy, const double *Gyz, const double *Gzz,
+                                double *vectx, double *vecty, double *vectz, const int32_t *ntotalm,
+                                const int32_t *ntotal, const int32_t *nmax, const int32_t *nxm,
+                                const int32_t *nym, const int32_t *nzm, const int32_t *nx,
+                                const int32_t *ny, const int32_t *nz, const int32_t *nx2,
+                                const int32_t *ny2, const int32_t *nxy2, const int32_t *nz2,
+                                const double *FFX, double *FFB, const int64_t *planf,
+                                const int64_t *planb);
+
+/* calculgreenfft.f:1-3 */
+void ifdda_greencalculfft(const int32_t *nx, const int32_t *ny, const int32_t *nz,
+                          const int32_t *nx2, const int32_t *ny2, const int32_t *nz2,
+                          const int32_t *nxy2, const int32_t *nxm, const int32_t *nym,
+                          const int32_t *nzm, const double *aretecube, const double *k0,
+                          double *Gxx, double *Gxy, double *Gxz, double *Gyy, double *Gyz,
+                          double *Gzz, const int64_t *planb);
+
+/* calculgreenquadfft.f:1-3 */
+void ifdda_greencalculquadfft(const int32_t *nx, const int32_t *ny, const int32_t *nz,
+                              const int32_t *nx2, const int32_t *ny2, const int32_t *nz2,
+                              const int32_t *nxy2, const int32_t *nxm, const int32_t *nym,
+                              const int32_t *nzm, const int32_t *nquad, const double *tolinit,
+                              const double *aretecube, const double *k0, double *Gxx, double *Gxy,
+                              double *Gxz, double *Gyy, double *Gyz, double *Gzz,
+                              const int64_t *planb);
+
+/* inverserig.f:1-6.  methodeit / infostr are passed with explicit lengths
+ * (the hidden Fortran character lengths, made explicit by the shim). */
+void ifdda_inverserig(const double *Gxx, const double *Gxy, const double *Gxz, const double *Gyy,
+                      const double *Gyz, const double *Gzz, double *vectx, double *vecty,
+                      double *vectz, const int32_t *Tabdip, const int32_t *ntotalm,
+                      const int32_t *ntotal, const int32_t *ldabi, const int32_t *nlar,
+                      const int32_t *nmax, const int32_t *ndipole, const int32_t *nxm,
+                      const int32_t *nym, const int32_t *nzm, const int32_t *nx, const int32_t *ny,
+                      const int32_t *nz, const int32_t *nx2, const int32_t *ny2,
+                      const int32_t *nxy2, const int32_t *nz2, const int32_t *nbsphere,
+                      const int32_t *nbsphere3, double *XI, double *XR, double *wrk, double *FF,
+                      const double *FF0, double *FFloc, const double *polarisa,
+                      const char *methodeit, const double *tol, double *tol1, int32_t *nloop,
+                      int32_t *ncompte, const int64_t *planf, const int64_t *planb,
+                      volatile int32_t *nstop, char *infostr, size_t len_methodeit,
+                      size_t len_infostr);
+
+/* inverserigopt.f:1-6: the dense-box driver (no Tabdip). */
+void ifdda_inverserigopt(const double *Gxx, const double *Gxy, const double *Gxz,
+                         const double *Gyy, const double *Gyz, const double *Gzz, double *vectx,
+                         double *vecty, double *vectz, const int32_t *ntotalm,
+                         const int32_t *ntotal, const int32_t *ldabi, const int32_t *nlar,
+                         const int32_t *nmax, const int32_t *nxm, const int32_t *nym,
+                         const int32_t *nzm, const int32_t *nx, const int32_t *ny,
+                         const int32_t *nz, const int32_t *nx2, const int32_t *ny2,
+                         const int32_t *nxy2, const int32_t *nz2, const int32_t *nbsphere,
+                         const int32_t *nbsphere3, double *XI, double *XR, double *wrk,
+                         double *FF, const double *FF0, double *FFloc, const double *polarisa,
+                         const char *methodeit, const double *tol, double *tol1, int32_t *nloop,
+                         int32_t *ncompte, const int64_t *planf, const int64_t *planb,
+                         volatile int32_t *nstop, char *infostr, size_t len_methodeit,
+                         size_t len_infostr);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IFDDA_H */

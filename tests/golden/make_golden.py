@@ -1,0 +1,95 @@
+"""Write golden vectors from the compiled reference (oracle/_ref/libcdmref.so).
+
+Run in a container where the reference sources were available when
+``make -C oracle ref`` ran:
+
+    python tests/golden/make_golden.py
+
+Fixtures (inputs and the reference's outputs, nothing else):
+  green_654.npz    greencalculfft_ / greencalculquadfft_ (nquad 0,1,2) on a
+                   6x5x4 box, d = 20 nm, lambda = 632.8 nm
+  matvec_654.npz   produitfftmatvect3_ (seeded 60% mask) and
+                   produitfftmatvectopt_ on that box, seeded FFX / FFB
+  green_91012.npz  greencalculfft_ and produitfftmatvect3_ on a 9x10x12 box
+                   (FFT lengths 18, 20, 24: radices 2, 3, 5 mixed)
+  solve_cfg1.npz   inverserig_ on the 400 nm sphere (20^3 lattice, 4224
+                   dipoles, eps 2.25, RR, x-polarised plane wave), tol 1e-4,
+                   for GPBICG1, QMRBICGSTAB1, QMRBICGSTAB2, TFQMR: E (FFloc),
+                   tol1, nloop, ncompte
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "if-dda_amd"))
+
+from oracle import Ref  # noqa: E402
+from ifdda import scene  # noqa: E402
+
+LAM = 632.8e-9
+K0 = 2 * np.pi / LAM
+D = 20e-9
+
+
+def seeded_case(nx, ny, nz, seed, frac=0.6):
+    rng = np.random.default_rng(seed)
+    N = nx * ny * nz
+    mask = rng.random(N) < frac
+    nd = int(mask.sum())
+    tab = np.zeros(N, np.int32)
+    tab[mask] = np.arange(1, nd + 1, dtype=np.int32)
+    ffx = rng.standard_normal(3 * nd) + 1j * rng.standard_normal(3 * nd)
+    ffb = rng.standard_normal(3 * nd) + 1j * rng.standard_normal(3 * nd)
+    ffx_d = rng.standard_normal(3 * N) + 1j * rng.standard_normal(3 * N)
+    ffb_d = rng.standard_normal(3 * N) + 1j * rng.standard_normal(3 * N)
+    return tab, ffx, ffb, ffx_d, ffb_d
+
+
+def main():
+    r = Ref()
+    nx, ny, nz = 6, 5, 4
+    G0 = r.green_fft(nx, ny, nz, D, K0)
+    G1 = r.green_fft(nx, ny, nz, D, K0, nquad=1)
+    G2 = r.green_fft(nx, ny, nz, D, K0, nquad=2)
+    np.savez_compressed(os.path.join(HERE, "green_654.npz"), nx=nx, ny=ny, nz=nz, d=D, k0=K0,
+                        G_nquad0=G0, G_nquad1=G1, G_nquad2=G2)
+    tab, ffx, ffb, ffx_d, ffb_d = seeded_case(nx, ny, nz, 654)
+    out3 = r.matvec3(G0, tab, nx, ny, nz, ffx, ffb)
+    outo = r.matvec_opt(G0, nx, ny, nz, ffx_d, ffb_d)
+    np.savez_compressed(os.path.join(HERE, "matvec_654.npz"), nx=nx, ny=ny, nz=nz, d=D, k0=K0,
+                        tabdip=tab, FFX=ffx, FFB=ffb, out3=out3, FFX_dense=ffx_d, FFB_dense=ffb_d,
+                        out_opt=outo)
+
+    nx, ny, nz = 9, 10, 12
+    G = r.green_fft(nx, ny, nz, D, K0)
+    tab, ffx, ffb, _, _ = seeded_case(nx, ny, nz, 91012, frac=0.5)
+    out3 = r.matvec3(G, tab, nx, ny, nz, ffx, ffb)
+    np.savez_compressed(os.path.join(HERE, "green_91012.npz"), nx=nx, ny=ny, nz=nz, d=D, k0=K0,
+                        G=G, tabdip=tab, FFX=ffx, FFB=ffb, out3=out3)
+
+    s, k0, alpha, E = scene.config1()
+    Gs = r.green_fft(s.nx, s.ny, s.nz, s.d, k0)
+    pol = np.zeros((s.nx * s.ny * s.nz, 3, 3), np.complex128)
+    for a in range(3):
+        pol[: s.nd, a, a] = alpha
+    res = {}
+    for m in ["GPBICG1", "QMRBICGSTAB1", "QMRBICGSTAB2", "TFQMR"]:
+        o = r.inverserig(Gs, s.tabdip, s.nx, s.ny, s.nz, pol, E, m, 1e-4)
+        assert o["nstop"] == 0, o
+        res[f"{m}_E"] = o["FFloc"]
+        res[f"{m}_tol1"] = o["tol1"]
+        res[f"{m}_nloop"] = o["nloop"]
+        res[f"{m}_ncompte"] = o["ncompte"]
+        print(m, o["nloop"], o["ncompte"], o["tol1"])
+    np.savez_compressed(os.path.join(HERE, "solve_cfg1.npz"), tabdip=s.tabdip, d=s.d, k0=k0,
+                        alpha=alpha, E_inc=E, **res)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,123 @@
+"""GPU solves against the reference's golden solves and the oracle: iteration
+counts equal, local field within 1e-10 relative of the reference (FP64
+path; the solver tolerance is 1e-4, the FP64 paths agree far below it),
+mixed precision within the refinement tolerance, failure paths."""
+import numpy as np
+import pytest
+
+from conftest import golden, relerr
+
+pytestmark = pytest.mark.gpu
+
+METHODS = ["GPBICG1", "QMRBICGSTAB1", "QMRBICGSTAB2", "TFQMR"]
+
+
+@pytest.fixture(scope="module")
+def cfg1(gpu):
+    from ifdda import scene
+    s, k0, alpha, E = scene.config1()
+    sv = gpu.Solver(s.nx, s.ny, s.nz)
+    sv.greencalculfft(s.d, k0)
+    sv.set_dipoles(s.tabdip)
+    sv.set_polarizability(alpha)
+    yield s, k0, alpha, E, sv
+    sv.close()
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_config1_matches_reference(cfg1, method):
+    s, k0, alpha, E, sv = cfg1
+    g = golden("solve_cfg1.npz")
+    assert np.array_equal(g["tabdip"], s.tabdip)
+    assert relerr(g["E_inc"], E) < 1e-15
+    print(f"solve config1 {method}", flush=True)
+    r = sv.inverserig(E, method, 1e-4)
+    assert r["nstop"] == 0
+    assert r["nloop"] == int(g[f"{method}_nloop"])
+    assert r["ncompte"] == int(g[f"{method}_ncompte"])
+    assert abs(r["tol1"] - float(g[f"{method}_tol1"])) < 1e-9
+    assert relerr(r["FFloc"], g[f"{method}_E"]) < 1e-10
+    ff = (alpha[:, None] * r["FFloc"].reshape(-1, 3)).reshape(-1)
+    assert relerr(r["FF"], ff) < 1e-14
+
+
+def test_gpbicg_trajectory_matches_oracle(cfg1, oracle):
+    s, k0, alpha, E, sv = cfg1
+    sv.trace_enable(64)
+    r = sv.inverserig(E, "GPBICG1", 1e-6)
+    tr = sv.trace()
+    pol = np.zeros((s.nx * s.ny * s.nz, 3, 3), np.complex128)
+    for a in range(3):
+        pol[: s.nd, a, a] = alpha
+    G = oracle.green_fft(s.nx, s.ny, s.nz, (40, 40, 40), s.d, k0)
+    o = oracle.inverserig(G, s.tabdip, s.nx, s.ny, s.nz, pol, E, "GPBICG1", 1e-6, trace_cap=64)
+    assert r["nloop"] == o["nloop"] == tr.shape[0]
+    # alpha, beta, eta, dzeta, R0RN and the recurrence residual per iteration
+    assert relerr(tr[:, :5], o["trace"], np.abs(o["trace"]).max(axis=0)) < 1e-9
+    assert np.allclose(tr[:, 5].real, o["trace_res"], rtol=1e-9, atol=0)
+    sv.trace_enable(0)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_mixed_precision_refinement(cfg1, method):
+    """FP32 matvec + FP64 refinement: true residual <= tol (FP64 matvec) and
+    the field within the tolerance of the FP64 solve."""
+    s, k0, alpha, E, sv = cfg1
+    ref = sv.inverserig(E, method, 1e-8)
+    for tol in (1e-4, 1e-8):
+        r = sv.inverserig(E, method, tol, precision=32, x0_zero=True)
+        assert r["nstop"] == 0, r
+        assert r["tol1"] <= tol
+        assert relerr(r["FFloc"], ref["FFloc"]) < 10 * tol
+
+
+def test_x0_zero_and_dense_solution(gpu, oracle):
+    """Small system: GPU solution vs numpy.linalg.solve of the dense operator."""
+    from ifdda import scene
+    lam = 632.8e-9
+    k0 = 2 * np.pi / lam
+    s = scene.sphere(6, 60e-9, 3.0 + 0.5j)
+    alpha = scene.polarizability(s.eps, s.d, k0, "RR")
+    E = scene.plane_wave(s.pos, k0, 20.0, 30.0, pp=0.3)
+    n = 3 * s.nd
+    G = oracle.green_fft(s.nx, s.ny, s.nz, (12, 12, 12), s.d, k0)
+    A = np.zeros((n, n), np.complex128)
+    I = np.eye(n)
+    for j in range(n):  # columns of A = I - T alpha via the oracle matvec
+        ax = (np.repeat(alpha, 3) * I[:, j])
+        A[:, j] = oracle.matvec(G, s.tabdip, s.nx, s.ny, s.nz, ax, I[:, j])
+    xs = np.linalg.solve(A, E)
+    with gpu.Solver(s.nx, s.ny, s.nz) as sv:
+        sv.greencalculfft(s.d, k0)
+        sv.set_dipoles(s.tabdip)
+        sv.set_polarizability(alpha)
+        for m in METHODS:
+            r = sv.inverserig(E, m, 1e-10, x0_zero=True)
+            assert r["nstop"] == 0 and r["tol1"] < 1e-10
+            assert relerr(r["FFloc"], xs) < 1e-8
+
+
+def test_zero_rhs_breakdown(cfg1):
+    """E_inc = 0 gives r = 0, <r0*,Ap> = 0: STEPERR 1 (gpbicg.f:183-187)."""
+    s, k0, alpha, E, sv = cfg1
+    r = sv.inverserig(np.zeros_like(E), "GPBICG1", 1e-4)
+    assert r["nstop"] == 1 and r["status"] == -1 and r["steperr"] == 1
+    assert r["infostr"].startswith("Problem to solve Ax=b!")
+
+
+def test_cancel_flag(cfg1):
+    """nstop = -1 set by another thread before the solve cancels it."""
+    import ctypes as C
+    s, k0, alpha, E, sv = cfg1
+    flag = C.c_int(-1)
+    r = sv.inverserig(E, "GPBICG1", 1e-4, nstop=flag)
+    assert r["nstop"] == -1
+    assert r["infostr"].startswith("Calculation cancelled")
+
+
+def test_iteration_cap(cfg1):
+    s, k0, alpha, E, sv = cfg1
+    r = sv.inverserig(E, "GPBICG1", 1e-12, maxit=1)
+    # ITNO runs 0..maxit+1 (gpbicg.f:327), then the true residual is reported
+    assert r["nstop"] == 0 and r["nloop"] == 3 and r["steperr"] == 0
+    assert r["tol1"] > 1e-12

@@ -1,0 +1,258 @@
+"""Pins of the oracle (CPU): each check is chosen so that a dropped term, a
+wrong sign or index, or a transposed operand in the restatement fails one.
+
+  T(r)          far-field limit, static near-field limit, symmetry, parity,
+                the compiled reference (when built here)
+  FFT           numpy.fft, brute-force DFT, Parseval
+  Green G       golden fixtures from the reference, brute-force DFT of the
+                spatial fill, even/odd parity
+  matvec        golden fixtures, the O(Nd^2) direct sum, linearity,
+                reciprocity A^T = A for scalar alpha
+  quadrature    golden fixtures; Simpson limit vs the point value far away
+  solvers       golden reference solves (iteration counts, fields), dense
+                numpy.linalg.solve, zero-RHS breakdown
+  Mie           BHMIE cross sections vs DDA on config 1 (a few %)
+"""
+import numpy as np
+import pytest
+
+from conftest import golden, relerr
+
+LAM = 632.8e-9
+K0 = 2 * np.pi / LAM
+D = 20e-9
+
+
+# ------------------------------------------------------------------ T(r)
+def test_propa_far_field_limit(oracle):
+    """|R| >> lambda: T -> k^2 e^{ikR}/R (I - RR), transverse."""
+    R = np.array([3.0, -4.0, 12.0]) * 1e-3  # 13 mm
+    T = oracle.propa((0, 0, 0), -R, K0)  # R = obs - src
+    r = np.linalg.norm(R)
+    u = R / r
+    far = K0 ** 2 * np.exp(1j * K0 * r) / r * (np.eye(3) - np.outer(u, u))
+    assert relerr(T, far) < 1e-4
+    assert np.abs(T @ u).max() < 1e-4 * np.abs(T).max()
+
+
+def test_propa_static_limit(oracle):
+    """k R -> 0: T -> (3 RR - I)/R^3 (dipole field)."""
+    R = np.array([1.0, 2.0, -2.0]) * 1e-12
+    T = oracle.propa(R, (0, 0, 0), K0)
+    r = np.linalg.norm(R)
+    u = R / r
+    stat = (3 * np.outer(u, u) - np.eye(3)) / r ** 3
+    assert relerr(T, stat) < 1e-9
+
+
+def test_propa_symmetry_and_parity(oracle):
+    R = np.array([0.3, -0.7, 1.1]) * 1e-7
+    T = oracle.propa(R, (0, 0, 0), K0)
+    assert np.allclose(T, T.T, rtol=0, atol=1e-15 * np.abs(T).max())
+    assert relerr(oracle.propa(-R, (0, 0, 0), K0), T) < 1e-15
+    assert np.all(oracle.propa((0, 0, 0), (0, 0, 0), K0) == 0)
+
+
+def test_propa_matches_reference(oracle, ref):
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        a, b = rng.standard_normal(3) * 1e-7, rng.standard_normal(3) * 1e-7
+        assert relerr(oracle.propa(a, b, K0), ref.propa(a, b, K0)) < 1e-14
+
+
+def test_propa_int_matches_reference(oracle, ref):
+    for off in [(1, 0, 0), (1, 1, 0), (2, 1, 1), (0, 0, 1)]:
+        o = np.array(off) * D
+        assert relerr(oracle.propa_int((0, 0, 0), o, K0, D), ref.propa_int((0, 0, 0), o, K0, D)) < 1e-12
+
+
+def test_propa_int_far_cell_is_point_value(oracle):
+    """Beyond lim*lambda the integrated tensor is the point value; just
+    inside, the Simpson average differs from it by O((d/R)^2)."""
+    o = np.array([40.0, 0, 0]) * D  # 800 nm > lambda
+    assert relerr(oracle.propa_int((0, 0, 0), o, K0, D), oracle.propa((0, 0, 0), o, K0)) == 0.0
+    o = np.array([20.0, 0, 0]) * D  # 400 nm < lambda
+    Ti = oracle.propa_int((0, 0, 0), o, K0, D)
+    Tp = oracle.propa((0, 0, 0), o, K0)
+    assert 1e-6 < relerr(Ti, Tp) < 1e-2
+
+
+# ------------------------------------------------------------------- FFT
+@pytest.mark.parametrize("shape", [(4, 6, 10), (5, 9, 8), (7, 3, 2)])
+def test_fft3d_vs_numpy(oracle, shape):
+    rng = np.random.default_rng(1)
+    v = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    # sign -1 is numpy's forward transform; sign +1 is n * ifft
+    assert relerr(oracle.fft3d(v, -1), np.fft.fftn(v)) < 1e-13
+    assert relerr(oracle.fft3d(v, +1), np.fft.ifftn(v) * v.size) < 1e-13
+
+
+def test_fft_brute_force_and_parseval(oracle):
+    n = 15
+    rng = np.random.default_rng(2)
+    v = rng.standard_normal((1, 1, n)) + 1j * rng.standard_normal((1, 1, n))
+    k = np.arange(n)
+    W = np.exp(2j * np.pi * np.outer(k, k) / n)
+    assert relerr(oracle.fft3d(v, +1)[0, 0], W @ v[0, 0]) < 1e-13
+    F = oracle.fft3d(v, -1)
+    assert abs(np.sum(np.abs(F) ** 2) / n - np.sum(np.abs(v) ** 2)) < 1e-12 * np.sum(np.abs(v) ** 2)
+
+
+# ----------------------------------------------------------------- Green
+def test_green_matches_golden(oracle):
+    g = golden("green_654.npz")
+    for nq in (0, 1, 2):
+        G = oracle.green_fft(6, 5, 4, (12, 10, 8), D, K0, nquad=nq)
+        assert relerr(G, g[f"G_nquad{nq}"]) < 1e-12, nq
+
+
+def test_green_is_dft_of_the_fill(oracle):
+    """Brute-force: G_hat(k) = sum_r T(offset(r) d) e^{+2 pi i k.r/L} on a tiny grid."""
+    nx, ny, nz = 3, 2, 2
+    L = (6, 4, 4)
+    G = oracle.green_fft(nx, ny, nz, L, D, K0)
+    comps = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    fill = np.zeros((6, L[2], L[1], L[0]), complex)
+
+    def off(i, n, Ln):
+        return i if i < n else (i - Ln if i > Ln - n else None)
+    for k in range(L[2]):
+        for j in range(L[1]):
+            for i in range(L[0]):
+                o = (off(i, nx, L[0]), off(j, ny, L[1]), off(k, nz, L[2]))
+                if None in o:
+                    continue
+                T = oracle.propa((0, 0, 0), np.array(o) * D, K0)
+                for c, (a, b) in enumerate(comps):
+                    fill[c, k, j, i] = T[a, b]
+    ref = np.fft.ifftn(fill, axes=(1, 2, 3)) * np.prod(L)
+    assert relerr(G, ref) < 1e-13
+
+
+def test_green_parity(oracle):
+    G = oracle.green_fft(6, 5, 4, (12, 10, 8), D, K0)
+    mirror = lambda a, ax: np.roll(np.flip(a, axis=ax), 1, axis=ax)  # noqa: E731  k -> L - k
+    # axes of G[c]: 0 = z, 1 = y, 2 = x
+    par = {0: (1, 1, 1), 1: (1, -1, -1), 2: (-1, 1, -1), 3: (1, 1, 1), 4: (-1, -1, 1), 5: (1, 1, 1)}
+    for c, (pz, py, px) in par.items():
+        for ax, p in ((0, pz), (1, py), (2, px)):
+            assert relerr(mirror(G[c], ax), p * G[c], np.abs(G[c]).max()) < 1e-13
+
+
+# ---------------------------------------------------------------- matvec
+def test_matvec_matches_golden(oracle):
+    f = golden("matvec_654.npz")
+    G = oracle.green_fft(6, 5, 4, (12, 10, 8), D, K0)
+    y = oracle.matvec(G, f["tabdip"], 6, 5, 4, f["FFX"], f["FFB"])
+    assert relerr(y, f["out3"], np.abs(f["out3"] - f["FFB"]).max()) < 1e-13
+    y = oracle.matvec(G, None, 6, 5, 4, f["FFX_dense"], f["FFB_dense"])
+    assert relerr(y, f["out_opt"], np.abs(f["out_opt"] - f["FFB_dense"]).max()) < 1e-13
+    g = golden("green_91012.npz")
+    G = oracle.green_fft(9, 10, 12, (18, 20, 24), D, K0)
+    assert relerr(G, g["G"]) < 1e-12
+    y = oracle.matvec(G, g["tabdip"], 9, 10, 12, g["FFX"], g["FFB"])
+    assert relerr(y, g["out3"], np.abs(g["out3"] - g["FFB"]).max()) < 1e-13
+
+
+def test_matvec_direct_sum_and_padding(oracle):
+    """FFT convolution = direct dipole sum, on 2n and on larger padded grids."""
+    nx, ny, nz = 5, 4, 3
+    f = golden("matvec_654.npz")
+    rng = np.random.default_rng(3)
+    N = nx * ny * nz
+    mask = rng.random(N) < 0.7
+    nd = int(mask.sum())
+    tab = np.zeros(N, np.int32)
+    tab[mask] = np.arange(1, nd + 1)
+    x = rng.standard_normal(3 * nd) + 1j * rng.standard_normal(3 * nd)
+    b = np.zeros(3 * nd, complex)
+    ref = oracle.matvec_direct(tab, nx, ny, nz, D, K0, x, b)
+    for L in [(10, 8, 6), (12, 9, 8), (15, 10, 6)]:
+        G = oracle.green_fft(nx, ny, nz, L, D, K0)
+        assert relerr(oracle.matvec(G, tab, nx, ny, nz, x, b), ref) < 1e-13
+    assert f["out3"].size  # fixture present
+
+
+def test_matvec_linear_and_reciprocal(oracle):
+    """A = I - T alpha with scalar alpha: T is symmetric (x^T T y = y^T T x)."""
+    nx, ny, nz = 4, 4, 3
+    G = oracle.green_fft(nx, ny, nz, (8, 8, 6), D, K0)
+    rng = np.random.default_rng(4)
+    N = nx * ny * nz
+    x, y = (rng.standard_normal(3 * N) + 1j * rng.standard_normal(3 * N) for _ in range(2))
+    z = np.zeros(3 * N, complex)
+    Tx = -oracle.matvec(G, None, nx, ny, nz, x, z)
+    Ty = -oracle.matvec(G, None, nx, ny, nz, y, z)
+    assert abs(y @ Tx - x @ Ty) < 1e-12 * abs(y @ Tx)
+    T2 = -oracle.matvec(G, None, nx, ny, nz, 2 * x + 3j * y, z)
+    assert relerr(T2, 2 * Tx + 3j * Ty) < 1e-13
+
+
+# ---------------------------------------------------------------- solvers
+@pytest.mark.parametrize("method", ["GPBICG1", "QMRBICGSTAB1", "QMRBICGSTAB2", "TFQMR"])
+def test_solvers_match_reference_golden(oracle, method):
+    import ifdda.scene as scene
+    g = golden("solve_cfg1.npz")
+    s, k0, alpha, E = scene.config1()
+    assert np.array_equal(s.tabdip, g["tabdip"]) and relerr(alpha, g["alpha"]) < 1e-15
+    G = oracle.green_fft(20, 20, 20, (40, 40, 40), s.d, k0)
+    pol = np.zeros((8000, 3, 3), complex)
+    for a in range(3):
+        pol[: s.nd, a, a] = alpha
+    o = oracle.inverserig(G, s.tabdip, 20, 20, 20, pol, E, method, 1e-4)
+    assert o["nloop"] == int(g[f"{method}_nloop"])
+    assert o["ncompte"] == int(g[f"{method}_ncompte"])
+    assert abs(o["tol1"] - float(g[f"{method}_tol1"])) < 1e-9
+    assert relerr(o["FFloc"], g[f"{method}_E"]) < 1e-11
+
+
+def test_solver_vs_dense_solve(oracle):
+    import ifdda.scene as scene
+    s = scene.sphere(4, 40e-9, 4.0 + 1.0j)
+    alpha = scene.polarizability(s.eps, s.d, K0, "RR")
+    E = scene.plane_wave(s.pos, K0, 30.0, 10.0, pp=0.5)
+    G = oracle.green_fft(4, 4, 4, (8, 8, 8), s.d, K0)
+    n = 3 * s.nd
+    A = np.zeros((n, n), complex)
+    for j in range(n):
+        e = np.zeros(n, complex)
+        e[j] = 1
+        A[:, j] = oracle.matvec(G, s.tabdip, 4, 4, 4, np.repeat(alpha, 3) * e, e)
+    xs = np.linalg.solve(A, E)
+    pol = np.zeros((64, 3, 3), complex)
+    for a in range(3):
+        pol[: s.nd, a, a] = alpha
+    for m in ["GPBICG1", "QMRBICGSTAB1", "QMRBICGSTAB2", "TFQMR"]:
+        o = oracle.inverserig(G, s.tabdip, 4, 4, 4, pol, E, m, 1e-11)
+        assert o["tol1"] < 1e-10 and relerr(o["FFloc"], xs) < 1e-9, m
+
+
+def test_zero_rhs_breakdown(oracle):
+    import ifdda.scene as scene
+    s = scene.sphere(4, 40e-9, 2.0)
+    G = oracle.green_fft(4, 4, 4, (8, 8, 8), s.d, K0)
+    pol = np.zeros((64, 3, 3), complex)
+    o = oracle.inverserig(G, s.tabdip, 4, 4, 4, pol, np.zeros(3 * s.nd, complex), "GPBICG1", 1e-4)
+    assert o["nstop"] == 1 and o["status"] == -1 and o["steperr"] == 1
+
+
+# ------------------------------------------------------------------- Mie
+def test_mie_cross_sections_vs_dda(oracle):
+    """BHMIE (Bohren-Huffman) vs DDA Cext on the 400 nm sphere: the DDA at
+    20 cells per diameter is within a few percent; Cabs = 0 for real eps."""
+    import ifdda.scene as scene
+    from mie import bhmie
+    g = golden("solve_cfg1.npz")
+    s, k0, alpha, E = scene.config1()
+    Eloc = g["GPBICG1_E"].reshape(-1, 3)
+    p = alpha[:, None] * Eloc
+    E0 = E.reshape(-1, 3)
+    # cdmlib.f:2730-2746
+    cext = 4 * np.pi * k0 * np.sum(np.imag(np.sum(np.conj(E0) * p, axis=1)))
+    cabs = 4 * np.pi * k0 * np.sum(np.imag(np.sum(p * np.conj(Eloc), axis=1))
+                                   - 2.0 / 3.0 * k0 ** 3 * np.sum(np.abs(p) ** 2, axis=1))
+    qext, qsca, gsca = bhmie(k0 * 200e-9, np.sqrt(2.25 + 0j), 200)
+    cext_mie = qext * np.pi * (200e-9) ** 2
+    assert abs(cext / cext_mie - 1) < 0.05
+    assert abs(cabs) < 1e-3 * cext
+    assert abs(qext - qsca) < 1e-10  # lossless sphere: extinction = scattering
