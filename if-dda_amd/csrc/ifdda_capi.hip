@@ -64,6 +64,7 @@ extern "C" void ifdda_destroy(ifdda_ctx* c) {
   (void)hipFree(c->scrB);
   (void)hipFree(c->hx);
   (void)hipFree(c->hb);
+  ifdda_timing_enable(c, 0);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -245,6 +246,43 @@ extern "C" int ifdda_green_get(ifdda_ctx* c, double* const G[6]) {
           G[comp][2 * idx + 1] = sgn[comp] * v[comp].y;
         }
       }
+  return IFDDA_OK;
+}
+
+extern "C" int ifdda_timing_enable(ifdda_ctx* c, int cap) {
+  if (!c || cap < 0) return arg_error("ifdda_timing_enable: bad argument");
+  PassTimer& t = c->timer;
+  if (t.ev) {
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int i = 0; i < 2 * t.cap; ++i) (void)hipEventDestroy(t.ev[i]);
+    delete[] t.ev;
+    delete[] t.tag;
+  }
+  t = PassTimer();
+  if (cap == 0) return IFDDA_OK;
+  t.ev = new hipEvent_t[2 * (size_t)cap];
+  t.tag = new int[cap];
+  for (int i = 0; i < 2 * cap; ++i) CHECK_HIP(hipEventCreate(&t.ev[i]), "hipEventCreate");
+  t.cap = cap;
+  return IFDDA_OK;
+}
+
+extern "C" int ifdda_timing_read(ifdda_ctx* c, double ms_sum[10], int count[10]) {
+  if (!c || !ms_sum || !count) return arg_error("ifdda_timing_read: NULL argument");
+  for (int i = 0; i < 10; ++i) {
+    ms_sum[i] = 0.0;
+    count[i] = 0;
+  }
+  PassTimer& t = c->timer;
+  if (!t.cap) return IFDDA_OK;
+  CHECK_HIP(hipStreamSynchronize(c->stream), "timing sync");
+  for (int i = 0; i < t.used; ++i) {
+    float ms = 0.f;
+    CHECK_HIP(hipEventElapsedTime(&ms, t.ev[2 * i], t.ev[2 * i + 1]), "hipEventElapsedTime");
+    ms_sum[t.tag[i]] += ms;
+    count[t.tag[i]] += 1;
+  }
+  t.used = 0;
   return IFDDA_OK;
 }
 

@@ -612,7 +612,7 @@ template <typename K> static bool allow_lds(K kernel, size_t bytes) {
 
 template <typename V>
 static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, const V* b, V* y,
-                        hipStream_t s, int pi, const int* skip) {
+                        hipStream_t s, int pi, const int* skip, PassTimer* tm) {
   using S = decltype(V::x);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int L1 = g.L[0], L2 = g.L[1], L3 = g.L[2];
@@ -636,7 +636,9 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.alpha_mode = m.alpha_mode;
     auto k = k_xpass<V, 3, +1, 1>;
     if (!allow_lds(k, t.lds)) return false;
+    if (tm) tm->begin(pi * 5 + 0, s);
     hipLaunchKernelGGL(k, dim3((a.nlines + t.B - 1) / t.B), dim3(t.threads), t.lds, s, a);
+    if (tm) tm->end(s);
   }
   {  // P2: FFT-y over the nz non-zero planes
     const TileShape& t = g.p2[pi];
@@ -656,7 +658,9 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.sp_dst = l1c;
     auto k = k_strided<V, 3, +1, false>;
     if (!allow_lds(k, t.lds)) return false;
+    if (tm) tm->begin(pi * 5 + 1, s);
     hipLaunchKernelGGL(k, dim3((L1 + t.B - 1) / t.B, nz), dim3(t.threads), t.lds, s, a);
+    if (tm) tm->end(s);
   }
   {  // P3: FFT-z . G . IFFT-z, in place, keeps kz < nz
     const TileShape& t = g.p3[pi];
@@ -679,7 +683,9 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.L2 = L2;
     auto k = k_strided<V, 3, +1, true>;
     if (!allow_lds(k, t.lds)) return false;
+    if (tm) tm->begin(pi * 5 + 2, s);
     hipLaunchKernelGGL(k, dim3((L1 + t.B - 1) / t.B, L2), dim3(t.threads), t.lds, s, a);
+    if (tm) tm->end(s);
   }
   {  // P4: IFFT-y, keeps j < ny
     const TileShape& t = g.p4[pi];
@@ -699,7 +705,9 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.sp_dst = l1c;
     auto k = k_strided<V, 3, -1, false>;
     if (!allow_lds(k, t.lds)) return false;
+    if (tm) tm->begin(pi * 5 + 3, s);
     hipLaunchKernelGGL(k, dim3((L1 + t.B - 1) / t.B, nz), dim3(t.threads), t.lds, s, a);
+    if (tm) tm->end(s);
   }
   {  // P5: IFFT-x + gather, y = b - v/M
     const TileShape& t = g.p5[pi];
@@ -720,18 +728,22 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.inv_m = (S)(1.0 / ((double)L1 * (double)L2 * (double)L3));
     auto k = k_xpass<V, 3, -1, 2>;
     if (!allow_lds(k, t.lds)) return false;
+    if (tm) tm->begin(pi * 5 + 4, s);
     hipLaunchKernelGGL(k, dim3((a.nlines + t.B - 1) / t.B), dim3(t.threads), t.lds, s, a);
+    if (tm) tm->end(s);
   }
   return hip_ok(hipGetLastError(), "matvec launch");
 }
 
 bool launch_matvec_f64(const GridPlan& g, const MatvecDev<double2>& m, const double2* x,
-                       const double2* b, double2* y, hipStream_t s, const int* skip) {
-  return matvec_impl(g, m, x, b, y, s, 0, skip);
+                       const double2* b, double2* y, hipStream_t s, const int* skip,
+                       PassTimer* timer) {
+  return matvec_impl(g, m, x, b, y, s, 0, skip, timer ? timer : m.timer);
 }
 bool launch_matvec_f32(const GridPlan& g, const MatvecDev<float2>& m, const float2* x,
-                       const float2* b, float2* y, hipStream_t s, const int* skip) {
-  return matvec_impl(g, m, x, b, y, s, 1, skip);
+                       const float2* b, float2* y, hipStream_t s, const int* skip,
+                       PassTimer* timer) {
+  return matvec_impl(g, m, x, b, y, s, 1, skip, timer ? timer : m.timer);
 }
 
 // In-place forward 3D FFT (sign +1) of one FP64 component on the full grid.

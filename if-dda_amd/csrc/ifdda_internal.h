@@ -26,6 +26,8 @@ struct GridPlan {
   TileShape gx, gy, gz;  // Green full-grid passes (FP64, 1 component)
 };
 
+struct PassTimer;
+
 // Device-side matvec operands of one precision.
 template <typename V> struct MatvecDev {
   V* A;               // nz*ny*L1*3 scratch (after x pass)
@@ -35,6 +37,26 @@ template <typename V> struct MatvecDev {
   const int32_t* tab; // box -> 1-based dipole, or null (dense box)
   const V* alpha;     // per dipole, see alpha_mode
   int alpha_mode;
+  PassTimer* timer = nullptr;
+};
+
+// HIP-event timing of individual matvec passes on their own stream.  Slot
+// tag = prec_index * 5 + pass (pass 0..4 = P1..P5).  Events come from a pool
+// created when timing is enabled; recording stops when the pool is used up.
+struct PassTimer {
+  int cap = 0;
+  int used = 0;
+  hipEvent_t* ev = nullptr;  // 2 * cap events
+  int* tag = nullptr;
+  void begin(int t, hipStream_t s) {
+    if (used < cap) {
+      tag[used] = t;
+      (void)hipEventRecord(ev[2 * used], s);
+    }
+  }
+  void end(hipStream_t s) {
+    if (used < cap) (void)hipEventRecord(ev[2 * used + 1], s), ++used;
+  }
 };
 
 // Report the first error (HIP or argument) and keep its text.
@@ -47,10 +69,13 @@ void make_tiles(GridPlan& g, size_t lds_limit);
 
 // y = b - (1/M) IFFT[G . FFT[scatter(alpha . x)]]  (all device pointers).
 // skip: optional device flag; when it reads non-zero the passes do nothing.
+// timer: optional per-pass event timing.
 bool launch_matvec_f64(const GridPlan& g, const MatvecDev<double2>& m, const double2* x,
-                       const double2* b, double2* y, hipStream_t s, const int* skip = nullptr);
+                       const double2* b, double2* y, hipStream_t s, const int* skip = nullptr,
+                       PassTimer* timer = nullptr);
 bool launch_matvec_f32(const GridPlan& g, const MatvecDev<float2>& m, const float2* x,
-                       const float2* b, float2* y, hipStream_t s, const int* skip = nullptr);
+                       const float2* b, float2* y, hipStream_t s, const int* skip = nullptr,
+                       PassTimer* timer = nullptr);
 
 // Green tensor on the device: fills g64/g32 (folded, F2*F3*F1*6) using
 // `full` (L1*L2*L3 FP64 scratch).  nquad > 0 integrates near cells.
@@ -89,6 +114,7 @@ struct ifdda_ctx {
   double green_sample[6][2 * 16] = {};
   uint64_t tab_hash = 0;
   bool tab_dense = false;
+  ifdda::PassTimer timer;  // enabled by ifdda_timing_enable
 
   ifdda::MatvecDev<double2> mv64() const {
     ifdda::MatvecDev<double2> m;
@@ -99,6 +125,7 @@ struct ifdda_ctx {
     m.tab = tab;
     m.alpha = al64;
     m.alpha_mode = alpha_mode;
+    m.timer = timer.cap ? const_cast<ifdda::PassTimer*>(&timer) : nullptr;
     return m;
   }
   ifdda::MatvecDev<float2> mv32() const {
@@ -110,6 +137,7 @@ struct ifdda_ctx {
     m.tab = tab;
     m.alpha = al32;
     m.alpha_mode = alpha_mode;
+    m.timer = timer.cap ? const_cast<ifdda::PassTimer*>(&timer) : nullptr;
     return m;
   }
 };

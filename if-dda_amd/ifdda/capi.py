@@ -73,6 +73,8 @@ def lib():
                                      C.POINTER(C.c_int), vp]
     L.ifdda_solve_trace.argtypes = [vp, ip, vp, C.POINTER(C.c_int)]
     L.ifdda_solve_trace_enable.argtypes = [vp, ip]
+    L.ifdda_timing_enable.argtypes = [vp, ip]
+    L.ifdda_timing_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     _lib = L
     return L
 
@@ -213,6 +215,17 @@ class Solver:
                                         C.c_void_p(stream) if stream else None),
                "ifdda_solve_device")
         return info.as_dict()
+
+    def timing_enable(self, cap: int):
+        """Record HIP events around each matvec pass (up to cap passes)."""
+        _check(lib().ifdda_timing_enable(self.h, int(cap)), "ifdda_timing_enable")
+
+    def timing_read(self):
+        """Returns (ms_sum[2,5], count[2,5]): [FP64/FP32][P1..P5]; resets."""
+        ms = (C.c_double * 10)()
+        cnt = (C.c_int * 10)()
+        _check(lib().ifdda_timing_read(self.h, ms, cnt), "ifdda_timing_read")
+        return np.array(ms[:]).reshape(2, 5), np.array(cnt[:]).reshape(2, 5)
 
     def trace_enable(self, cap: int):
         _check(lib().ifdda_solve_trace_enable(self.h, int(cap)), "ifdda_solve_trace_enable")

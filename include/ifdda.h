@@ -123,6 +123,21 @@ int ifdda_solve(ifdda_ctx *ctx, const ifdda_solve_opts *opts, const double *E_in
 int ifdda_solve_device(ifdda_ctx *ctx, const ifdda_solve_opts *opts, const void *E_inc, void *FF,
                        void *FFloc, ifdda_solve_info *info, volatile int *nstop, void *stream);
 
+/* Instrumentation (no counterpart in the reference, which only prints wall
+ * and CPU time, cdmlib.f:1956-1959): record HIP events around each matvec
+ * pass on the stream it runs on, for up to cap passes; cap = 0 disables.
+ * ifdda_timing_read returns the summed milliseconds and launch counts per
+ * slot [precision (0: FP64, 1: FP32) * 5 + pass (0..4 = P1..P5)] and resets. */
+int ifdda_timing_enable(ifdda_ctx *ctx, int cap);
+int ifdda_timing_read(ifdda_ctx *ctx, double ms_sum[10], int count[10]);
+
+/* Per-iteration scalars of the last solve: rows of six complex values
+ * (GPBICG: alpha, beta, eta, dzeta, R0RN; QMRBICGSTAB: alpha, beta, omega,
+ * tau, eta; TFQMR: alpha, beta, rho, tau, eta; then the residual), recorded
+ * when enabled with a row capacity cap > 0. */
+int ifdda_solve_trace_enable(ifdda_ctx *ctx, int cap);
+int ifdda_solve_trace(ifdda_ctx *ctx, int cap, double *out, int *nrows);
+
 /* ---------------------------------------------------------------------- */
 /* Reference-shaped entry points (all arguments by reference).            */
 /* ---------------------------------------------------------------------- */
