@@ -13,7 +13,8 @@
 //
 // One workgroup transforms a tile of NL = B*NC lines held in LDS as
 // [point][line] (line fastest, so a wavefront reads contiguous LDS words);
-// stages ping-pong between two LDS buffers.  For the strided axes the B
+// stages run in place (registers hold a stage's butterflies across a
+// barrier), so a tile needs one LDS buffer.  For the strided axes the B
 // lines of a tile are B consecutive kx values (contiguous in HBM), so every
 // global row access is a contiguous B*NC-element segment.
 #include "ifdda_internal.h"
@@ -107,60 +108,81 @@ template <int SGN, typename V> struct Dft<8, SGN, V> {
   }
 };
 
-// One Stockham stage of radix R over the tile.  Sub-transforms of length Ns
-// are already done; butterfly j reads points j + q*L/R, twiddles input q by
-// exp(SGN 2 pi i k q / (Ns R)) with k = j mod Ns, and writes points
-// (j-k)*R + k + q*Ns.  tw[e] = exp(+2 pi i e / L).
-template <int R, int SGN, typename V>
-__device__ __forceinline__ void stage_r(const V* __restrict__ in, V* __restrict__ out,
-                                        const V* __restrict__ tw, int L, int Ns, int NL,
-                                        int line, int j0, int jstep) {
-  const int nb = L / R;
-  const int tws = L / (Ns * R);
-  for (int j = j0; j < nb; j += jstep) {
-    const int k = j % Ns;
-    V a[R];
-#pragma unroll
-    for (int q = 0; q < R; ++q) a[q] = in[(j + q * nb) * NL + line];
-    if (Ns > 1) {
-#pragma unroll
-      for (int q = 1; q < R; ++q) {
-        V w = tw[k * q * tws];
-        if (SGN < 0) w.y = -w.y;
-        a[q] = cmul(a[q], w);
-      }
-    }
-    Dft<R, SGN, V>::run(a);
-    const int ob = (j - k) * R + k;
-#pragma unroll
-    for (int q = 0; q < R; ++q) out[(ob + q * Ns) * NL + line] = a[q];
-  }
+// Wait for this thread's LDS operations, then a workgroup barrier.  Global
+// loads already issued stay in flight across it (the compiler inserts the
+// vmcnt wait at their first use), which lets prefetches overlap FFT stages.
+__device__ __forceinline__ void lds_sync() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-// Full transform of the tile; returns the buffer holding the result.
-// Thread layout: line = tid % NL, jt = tid / NL, JT = blockDim / NL.
+// Butterflies per thread per stage held in registers (in-place stages).
+constexpr int kMaxJ = 4;
+// Global loads issued back to back before their LDS stores.
+constexpr int kBatch = 16;
+// Green-tensor items per thread prefetched by the convolution pass.
+constexpr int kMaxG = 6;
+
+// One in-place Stockham stage of radix R over the tile in LDS.  Sub-transforms
+// of length Ns are done; butterfly j reads points j + q*L/R, twiddles input q
+// by exp(SGN 2 pi i k q / (Ns R)) with k = j mod Ns, and writes points
+// (j-k)*R + k + q*Ns.  Every thread first reads all its butterflies (at most
+// kMaxJ, j = jt + i*JT) into registers, then all threads write after a
+// barrier.  tw[e] = exp(+2 pi i e / L).
+template <int R, int SGN, typename V>
+__device__ __forceinline__ void stage_ip(V* buf, const V* __restrict__ tw, int L, int Ns, int NL,
+                                         int line, int jt, int JT) {
+  const int nb = L / R;
+  const int tws = L / (Ns * R);
+  V a[kMaxJ][R];
+#pragma unroll
+  for (int i = 0; i < kMaxJ; ++i) {
+    const int j = jt + i * JT;
+    if (j < nb) {
+      const int k = j % Ns;
+#pragma unroll
+      for (int q = 0; q < R; ++q) a[i][q] = buf[(j + q * nb) * NL + line];
+      if (Ns > 1) {
+#pragma unroll
+        for (int q = 1; q < R; ++q) {
+          V w = tw[k * q * tws];
+          if (SGN < 0) w.y = -w.y;
+          a[i][q] = cmul(a[i][q], w);
+        }
+      }
+      Dft<R, SGN, V>::run(a[i]);
+    }
+  }
+  lds_sync();
+#pragma unroll
+  for (int i = 0; i < kMaxJ; ++i) {
+    const int j = jt + i * JT;
+    if (j < nb) {
+      const int k = j % Ns;
+      const int ob = (j - k) * R + k;
+#pragma unroll
+      for (int q = 0; q < R; ++q) buf[(ob + q * Ns) * NL + line] = a[i][q];
+    }
+  }
+  lds_sync();
+}
+
+// Full in-place transform of the tile.  Thread layout: line = tid % NL,
+// jt = tid / NL, JT = blockDim / NL, with JT * kMaxJ >= L / min radix.
 template <int SGN, typename V>
-__device__ V* tile_fft(V* b0, V* b1, const V* tw, const AxisPlan& ax, int NL, int line, int jt,
-                       int JT) {
+__device__ void tile_fft(V* buf, const V* tw, const AxisPlan& ax, int NL, int line, int jt,
+                         int JT) {
   int Ns = 1;
   for (int s = 0; s < ax.nst; ++s) {
     const int r = ax.rad[s];
-    if (jt < JT) {
-      switch (r) {
-        case 2: stage_r<2, SGN>(b0, b1, tw, ax.L, Ns, NL, line, jt, JT); break;
-        case 3: stage_r<3, SGN>(b0, b1, tw, ax.L, Ns, NL, line, jt, JT); break;
-        case 4: stage_r<4, SGN>(b0, b1, tw, ax.L, Ns, NL, line, jt, JT); break;
-        case 5: stage_r<5, SGN>(b0, b1, tw, ax.L, Ns, NL, line, jt, JT); break;
-        default: stage_r<8, SGN>(b0, b1, tw, ax.L, Ns, NL, line, jt, JT); break;
-      }
+    switch (r) {
+      case 2: stage_ip<2, SGN>(buf, tw, ax.L, Ns, NL, line, jt, JT); break;
+      case 3: stage_ip<3, SGN>(buf, tw, ax.L, Ns, NL, line, jt, JT); break;
+      case 4: stage_ip<4, SGN>(buf, tw, ax.L, Ns, NL, line, jt, JT); break;
+      case 5: stage_ip<5, SGN>(buf, tw, ax.L, Ns, NL, line, jt, JT); break;
+      default: stage_ip<8, SGN>(buf, tw, ax.L, Ns, NL, line, jt, JT); break;
     }
-    __syncthreads();
-    V* t = b0;
-    b0 = b1;
-    b1 = t;
     Ns *= r;
   }
-  return b0;
 }
 
 // ---------------------------------------------------------------------------
@@ -198,8 +220,7 @@ __global__ void __launch_bounds__(256) k_strided(SArgs<V> a) {
   V* tw = reinterpret_cast<V*>(smem_d4);
   const int L = a.ax.L;
   const int NL = a.B * NC;
-  V* b0 = tw + L;
-  V* b1 = b0 + L * NL;
+  V* buf = tw + L;
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
   const int line = tid % NL, jt = tid / NL, JT = nthr / NL;
@@ -207,52 +228,79 @@ __global__ void __launch_bounds__(256) k_strided(SArgs<V> a) {
   const int kx0 = blockIdx.x * a.B;
   const int nvalid = min(a.B, a.nbx - kx0) * NC;
 
-  for (int e = tid; e < L; e += nthr) tw[e] = a.tw[e];
-  const V* src = a.src + (long long)o * a.so_src + (long long)kx0 * NC;
-  if (jt < JT) {
-    for (int p = jt; p < L; p += JT) {
-      V v = czero<V>();
-      if (p < a.n_in && line < nvalid) v = src[(long long)p * a.sp_src + line];
-      b0[p * NL + line] = v;
-    }
-  }
-  __syncthreads();
-
-  V* res;
+  // CONV: prefetch this thread's Green-tensor entries (items idx = tid + it*nthr
+  // over (kz, kxl)); their latency overlaps the tile load and forward FFT.
+  V gr[CONV ? kMaxG : 1][6];
+  int gsx[CONV ? kMaxG : 1];
   if (CONV) {
-    res = tile_fft<+1>(b0, b1, tw, a.ax, NL, line, jt, JT);
-    // Symmetric 3x3 product with G(kx, ky, kz) (produitfftmatvect3.f:76-90).
     int sy;
     const int kyf = fold_idx(o, a.L2, sy);
-    for (int idx = tid; idx < L * a.B; idx += nthr) {
+#pragma unroll
+    for (int it = 0; it < kMaxG; ++it) {
+      const int idx = tid + it * nthr;
+      gsx[it] = 0;
+      if (idx < L * a.B) {
+        const int kz = idx / a.B, kxl = idx - kz * a.B;
+        const int kx = kx0 + kxl;
+        if (kx < a.nbx) {
+          int sx, sz;
+          const int kxf = fold_idx(kx, a.L1, sx);
+          const int kzf = fold_idx(kz, L, sz);
+          const V* g = a.G + ((long long)(kyf * a.F3 + kzf) * a.F1 + kxf) * 6;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) gr[it][c] = g[c];
+          gsx[it] = 1 | ((sx * sy < 0) << 1) | ((sx * sz < 0) << 2) | ((sy * sz < 0) << 3);
+        }
+      }
+    }
+  }
+
+  for (int e = tid; e < L; e += nthr) tw[e] = a.tw[e];
+  const V* src = a.src + (long long)o * a.so_src + (long long)kx0 * NC;
+  for (int p0 = jt; p0 < L; p0 += kBatch * JT) {
+    V v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int p = p0 + u * JT;
+      v[u] = czero<V>();
+      if (p < a.n_in && line < nvalid) v[u] = src[(long long)p * a.sp_src + line];
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int p = p0 + u * JT;
+      if (p < L) buf[p * NL + line] = v[u];
+    }
+  }
+  lds_sync();
+
+  if (CONV) {
+    tile_fft<+1>(buf, tw, a.ax, NL, line, jt, JT);
+    // Symmetric 3x3 product with G(kx, ky, kz) (produitfftmatvect3.f:76-90).
+#pragma unroll
+    for (int it = 0; it < kMaxG; ++it) {
+      if (!(gsx[it] & 1)) continue;
+      const int idx = tid + it * nthr;
       const int kz = idx / a.B, kxl = idx - kz * a.B;
-      const int kx = kx0 + kxl;
-      if (kx >= a.nbx) continue;
-      int sx, sz;
-      const int kxf = fold_idx(kx, a.L1, sx);
-      const int kzf = fold_idx(kz, L, sz);
-      const V* g = a.G + ((long long)(kyf * a.F3 + kzf) * a.F1 + kxf) * 6;
-      const V gxx = g[0], gyy = g[3], gzz = g[5];
-      V gxy = g[1], gxz = g[2], gyz = g[4];
-      if (sx * sy < 0) gxy = mk(-gxy.x, -gxy.y);
-      if (sx * sz < 0) gxz = mk(-gxz.x, -gxz.y);
-      if (sy * sz < 0) gyz = mk(-gyz.x, -gyz.y);
-      V* v = res + kz * NL + kxl * 3;
+      const V gxx = gr[it][0], gyy = gr[it][3], gzz = gr[it][5];
+      V gxy = gr[it][1], gxz = gr[it][2], gyz = gr[it][4];
+      if (gsx[it] & 2) gxy = mk(-gxy.x, -gxy.y);
+      if (gsx[it] & 4) gxz = mk(-gxz.x, -gxz.y);
+      if (gsx[it] & 8) gyz = mk(-gyz.x, -gyz.y);
+      V* v = buf + kz * NL + kxl * 3;
       const V ex = v[0], ey = v[1], ez = v[2];
       v[0] = cadd(cadd(cmul(gxx, ex), cmul(gxy, ey)), cmul(gxz, ez));
       v[1] = cadd(cadd(cmul(gxy, ex), cmul(gyy, ey)), cmul(gyz, ez));
       v[2] = cadd(cadd(cmul(gxz, ex), cmul(gyz, ey)), cmul(gzz, ez));
     }
-    __syncthreads();
-    V* other = (res == b0) ? b1 : b0;
-    res = tile_fft<-1>(res, other, tw, a.ax, NL, line, jt, JT);
+    lds_sync();
+    tile_fft<-1>(buf, tw, a.ax, NL, line, jt, JT);
   } else {
-    res = tile_fft<SGN>(b0, b1, tw, a.ax, NL, line, jt, JT);
+    tile_fft<SGN>(buf, tw, a.ax, NL, line, jt, JT);
   }
 
   V* dst = a.dst + (long long)o * a.so_dst + (long long)kx0 * NC;
-  if (jt < JT && line < nvalid) {
-    for (int p = jt; p < a.n_out; p += JT) dst[(long long)p * a.sp_dst + line] = res[p * NL + line];
+  if (line < nvalid) {
+    for (int p = jt; p < a.n_out; p += JT) dst[(long long)p * a.sp_dst + line] = buf[p * NL + line];
   }
 }
 
@@ -300,6 +348,12 @@ __device__ __forceinline__ void alpha_apply(const V* __restrict__ al, int mode, 
   }
 }
 
+// Dipole index of cell i of box line g, or -1.
+__device__ __forceinline__ long long dip_of(const int32_t* __restrict__ tab, int nx, int g, int i) {
+  const long long cell = (long long)i + (long long)nx * g;
+  return tab ? (long long)tab[cell] - 1 : cell;
+}
+
 template <typename V, int NC, int SGN, int MODE>
 __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
   if (a.skip && *a.skip) return;
@@ -307,8 +361,7 @@ __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
   V* tw = reinterpret_cast<V*>(smem_d4);
   const int L = a.ax.L;
   const int NL = a.B * NC;
-  V* b0 = tw + L;
-  V* b1 = b0 + L * NL;
+  V* buf = tw + L;
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
   const int line = tid % NL, jt = tid / NL, JT = nthr / NL;
@@ -317,52 +370,99 @@ __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
 
   for (int e = tid; e < L; e += nthr) tw[e] = a.tw[e];
   if (MODE == 1) {
-    for (int e = tid; e < a.B * L; e += nthr) {
-      const int l = e / L, i = e - l * L;
-      V e0 = czero<V>(), e1 = e0, e2 = e0;
-      if (l < nl && i < a.n_in) {
-        const long long cell = (long long)i + (long long)a.nx * (g0 + l);
-        long long dip = a.tab ? (long long)a.tab[cell] - 1 : cell;
-        if (dip >= 0) {
-          const V* x = a.xin + dip * 3;
-          e0 = x[0];
-          e1 = x[1];
-          e2 = x[2];
-          alpha_apply(a.alpha, a.alpha_mode, dip, e0, e1, e2);
+    constexpr int U = kBatch / 2;
+    for (int e0 = tid; e0 < a.B * L; e0 += U * nthr) {
+      long long dip[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = e0 + u * nthr;
+        const int l = e / L, i = e - l * L;
+        dip[u] = -1;
+        if (e < a.B * L && l < nl && i < a.n_in) dip[u] = dip_of(a.tab, a.nx, g0 + l, i);
+      }
+      V x[U][3];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        x[u][0] = x[u][1] = x[u][2] = czero<V>();
+        if (dip[u] >= 0) {
+          const V* xp = a.xin + dip[u] * 3;
+          x[u][0] = xp[0];
+          x[u][1] = xp[1];
+          x[u][2] = xp[2];
+          alpha_apply(a.alpha, a.alpha_mode, dip[u], x[u][0], x[u][1], x[u][2]);
         }
       }
-      V* d = b0 + i * NL + l * 3;
-      d[0] = e0;
-      d[1] = e1;
-      d[2] = e2;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = e0 + u * nthr;
+        if (e < a.B * L) {
+          const int l = e / L, i = e - l * L;
+          V* d = buf + i * NL + l * 3;
+          d[0] = x[u][0];
+          d[1] = x[u][1];
+          d[2] = x[u][2];
+        }
+      }
     }
   } else {
     const int per = L * NC;
-    for (int e = tid; e < a.B * per; e += nthr) {
-      const int l = e / per, r = e - l * per;
-      const int p = r / NC, c = r - p * NC;
-      V v = czero<V>();
-      if (l < nl && p < a.n_in) v = a.src[(long long)(g0 + l) * a.ld_src + (long long)p * NC + c];
-      b0[p * NL + l * NC + c] = v;
+    for (int e0 = tid; e0 < a.B * per; e0 += kBatch * nthr) {
+      V v[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int e = e0 + u * nthr;
+        const int l = e / per, r = e - l * per;
+        const int p = r / NC, c = r - p * NC;
+        v[u] = czero<V>();
+        if (e < a.B * per && l < nl && p < a.n_in)
+          v[u] = a.src[(long long)(g0 + l) * a.ld_src + (long long)p * NC + c];
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int e = e0 + u * nthr;
+        if (e < a.B * per) {
+          const int l = e / per, r = e - l * per;
+          const int p = r / NC, c = r - p * NC;
+          buf[p * NL + l * NC + c] = v[u];
+        }
+      }
     }
   }
-  __syncthreads();
+  lds_sync();
 
-  V* res = tile_fft<SGN>(b0, b1, tw, a.ax, NL, line, jt, JT);
+  tile_fft<SGN>(buf, tw, a.ax, NL, line, jt, JT);
 
   if (MODE == 2) {
-    for (int e = tid; e < a.B * a.n_out; e += nthr) {
-      const int l = e / a.n_out, i = e - l * a.n_out;
-      if (l >= nl) continue;
-      const long long cell = (long long)i + (long long)a.nx * (g0 + l);
-      long long dip = a.tab ? (long long)a.tab[cell] - 1 : cell;
-      if (dip < 0) continue;
-      const V* v = res + i * NL + l * 3;
-      const V* b = a.xin + dip * 3;
-      V* y = a.xout + dip * 3;
-      y[0] = csub(b[0], cscal(v[0], a.inv_m));
-      y[1] = csub(b[1], cscal(v[1], a.inv_m));
-      y[2] = csub(b[2], cscal(v[2], a.inv_m));
+    constexpr int U = kBatch / 2;
+    for (int e0 = tid; e0 < a.B * a.n_out; e0 += U * nthr) {
+      long long dip[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = e0 + u * nthr;
+        const int l = e / a.n_out, i = e - l * a.n_out;
+        dip[u] = -1;
+        if (e < a.B * a.n_out && l < nl) dip[u] = dip_of(a.tab, a.nx, g0 + l, i);
+      }
+      V b[U][3];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (dip[u] >= 0) {
+          const V* bp = a.xin + dip[u] * 3;
+          b[u][0] = bp[0];
+          b[u][1] = bp[1];
+          b[u][2] = bp[2];
+        }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (dip[u] >= 0) {
+          const int e = e0 + u * nthr;
+          const int l = e / a.n_out, i = e - l * a.n_out;
+          const V* v = buf + i * NL + l * 3;
+          V* y = a.xout + dip[u] * 3;
+          y[0] = csub(b[u][0], cscal(v[0], a.inv_m));
+          y[1] = csub(b[u][1], cscal(v[1], a.inv_m));
+          y[2] = csub(b[u][2], cscal(v[2], a.inv_m));
+        }
     }
   } else {
     const int per = a.n_out * NC;
@@ -370,7 +470,7 @@ __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
       const int l = e / per, r = e - l * per;
       if (l >= nl) continue;
       const int p = r / NC, c = r - p * NC;
-      a.dst[(long long)(g0 + l) * a.ld_dst + (long long)p * NC + c] = res[p * NL + l * NC + c];
+      a.dst[(long long)(g0 + l) * a.ld_dst + (long long)p * NC + c] = buf[p * NL + l * NC + c];
     }
   }
 }
@@ -555,14 +655,22 @@ int smooth235_at_least(int n) {
 }
 
 // Radices in execution order: 8s first, then a 4 or 2 for the rest of the
-// power of two, then 5s and 3s.
+// power of two (8*2 is taken as 4*4), then 5s and 3s.
 bool make_axis_plan(int L, AxisPlan* out) {
   AxisPlan a{};
   a.L = L;
   int r = L, n = 0;
   while (r % 8 == 0 && n < kMaxStages) { a.rad[n++] = 8; r /= 8; }
   if (r % 4 == 0 && n < kMaxStages) { a.rad[n++] = 4; r /= 4; }
-  if (r % 2 == 0 && n < kMaxStages) { a.rad[n++] = 2; r /= 2; }
+  if (r % 2 == 0 && n < kMaxStages) {
+    if (n > 0 && a.rad[n - 1] == 8 && n + 1 < kMaxStages) {
+      a.rad[n - 1] = 4;  // 8*2 -> 4*4: no radix-2 stage (half the butterflies)
+      a.rad[n++] = 4;
+    } else {
+      a.rad[n++] = 2;
+    }
+    r /= 2;
+  }
   while (r % 5 == 0 && n < kMaxStages) { a.rad[n++] = 5; r /= 5; }
   while (r % 3 == 0 && n < kMaxStages) { a.rad[n++] = 3; r /= 3; }
   if (r != 1) return false;
@@ -571,35 +679,50 @@ bool make_axis_plan(int L, AxisPlan* out) {
   return true;
 }
 
-// Largest batch width whose ping-pong tile fits the LDS budget; threads are a
-// multiple of the line count NL = B*NC, close to 256.
-static TileShape tile_for(int L, int NC, int esz, int nbatch, size_t lds_limit) {
-  const size_t per_line = 2 * (size_t)L * esz;  // two buffers
-  long Bmax = (long)((lds_limit - (size_t)L * esz) / (per_line * NC));
-  Bmax = std::max(1L, std::min(Bmax, 64L));
-  Bmax = std::min<long>(Bmax, std::max(1, nbatch));
-  while (Bmax * NC > 256) --Bmax;
-  TileShape t;
-  t.B = (int)Bmax;
-  const int NL = t.B * NC;
-  t.threads = NL * std::max(1, 256 / NL);
-  t.lds = ((size_t)L + 2 * (size_t)L * NL) * esz;
+// Largest batch width B (lines per tile = B*NC) such that the one-buffer tile
+// fits the LDS budget, every thread holds at most kMaxJ butterflies per stage
+// (JT * kMaxJ >= L / smallest radix), and, for the convolution pass, every
+// thread prefetches at most kMaxG Green-tensor items (L*B <= kMaxG*threads).
+static TileShape tile_for(const AxisPlan& ax, int NC, int esz, int nbatch, size_t lds_limit,
+                          bool conv) {
+  const int L = ax.L;
+  int rmin = 8;
+  for (int s = 0; s < ax.nst; ++s) rmin = std::min(rmin, ax.rad[s]);
+  const int max_nb = L / rmin;
+  TileShape t{0, 0, 0};  // B = 0: no feasible tile
+  for (int B = 1; B <= 64 && B <= std::max(1, nbatch); ++B) {
+    const int NL = B * NC;
+    if (NL > 256) break;
+    const int JT = 256 / NL;
+    if (JT * kMaxJ < max_nb) break;
+    const int threads = NL * JT;
+    const size_t lds = ((size_t)L + (size_t)L * NL) * esz;
+    if (lds > lds_limit) break;
+    if (conv && L * B > kMaxG * threads) break;
+    t = TileShape{B, threads, lds};
+  }
   return t;
 }
 
-void make_tiles(GridPlan& g, size_t lds_limit) {
+bool make_tiles(GridPlan& g, size_t lds_limit) {
   const int L1 = g.L[0], L2 = g.L[1], L3 = g.L[2];
+  (void)L2;
+  (void)L3;
   for (int p = 0; p < 2; ++p) {
     const int esz = p == 0 ? 16 : 8;
-    g.p1[p] = tile_for(L1, 3, esz, g.ny * g.nz, lds_limit);
-    g.p2[p] = tile_for(L2, 3, esz, L1, lds_limit);
-    g.p3[p] = tile_for(L3, 3, esz, L1, lds_limit);
-    g.p4[p] = tile_for(L2, 3, esz, L1, lds_limit);
-    g.p5[p] = tile_for(L1, 3, esz, g.ny * g.nz, lds_limit);
+    g.p1[p] = tile_for(g.ax[0], 3, esz, g.ny * g.nz, lds_limit, false);
+    g.p2[p] = tile_for(g.ax[1], 3, esz, L1, lds_limit, false);
+    g.p3[p] = tile_for(g.ax[2], 3, esz, L1, lds_limit, true);
+    g.p4[p] = tile_for(g.ax[1], 3, esz, L1, lds_limit, false);
+    g.p5[p] = tile_for(g.ax[0], 3, esz, g.ny * g.nz, lds_limit, false);
   }
-  g.gx = tile_for(L1, 1, 16, L2 * L3, lds_limit);
-  g.gy = tile_for(L2, 1, 16, L1, lds_limit);
-  g.gz = tile_for(L3, 1, 16, L1, lds_limit);
+  g.gx = tile_for(g.ax[0], 1, 16, g.L[1] * g.L[2], lds_limit, false);
+  g.gy = tile_for(g.ax[1], 1, 16, L1, lds_limit, false);
+  g.gz = tile_for(g.ax[2], 1, 16, L1, lds_limit, false);
+  for (const TileShape* t : {&g.p1[0], &g.p2[0], &g.p3[0], &g.p4[0], &g.p5[0], &g.p1[1], &g.p2[1],
+                             &g.p3[1], &g.p4[1], &g.p5[1], &g.gx, &g.gy, &g.gz})
+    if (t->B == 0) return false;
+  return true;
 }
 
 template <typename K> static bool allow_lds(K kernel, size_t bytes) {

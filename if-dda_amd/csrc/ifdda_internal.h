@@ -65,7 +65,8 @@ bool hip_ok(hipError_t e, const char* what);
 
 int smooth235_at_least(int n);
 bool make_axis_plan(int L, AxisPlan* out);
-void make_tiles(GridPlan& g, size_t lds_limit);
+// false when an axis is too long for the register-staged FFT tiles
+bool make_tiles(GridPlan& g, size_t lds_limit);
 
 // y = b - (1/M) IFFT[G . FFT[scatter(alpha . x)]]  (all device pointers).
 // skip: optional device flag; when it reads non-zero the passes do nothing.
