@@ -171,8 +171,9 @@ def main():
 
     # dominant kernel of the timed region (FP32 passes dominate mixed solves)
     names = ["P1 scatter+FFT-x", "P2 FFT-y", "P3 FFT-z.G.IFFT-z", "P4 IFFT-y", "P5 IFFT-x+gather"]
-    kern = ["k_xpass<float2,3,1,1>", "k_strided<float2,3,1,false>", "k_strided<float2,3,1,true>",
-            "k_strided<float2,3,-1,false>", "k_xpass<float2,3,-1,2>"]
+    kern = ["ifdda::k_xpass<float2, 3, 1, 1>", "ifdda::k_strided<float2, 3, 1>",
+            "ifdda::k_zconv<float2", "ifdda::k_strided<float2, 3, -1>",
+            "ifdda::k_xpass<float2, 3, -1, 2>"]
     b32 = pass_bytes(s.nx, s.ny, s.nz, sv.L, nd, 8)
     b64 = pass_bytes(s.nx, s.ny, s.nz, sv.L, nd, 16)
     passes = {}
@@ -187,6 +188,17 @@ def main():
     p_dom = int(np.argmax(ms_sum[1]))
     avg_dom = ms_sum[1, p_dom] / max(1, cnt[1, p_dom])
     achieved = b32[p_dom] / (avg_dom * 1e-3) / 1e9
+
+    # HBM bytes per launch of that kernel from the committed rocprofv3 counter
+    # summary (FETCH_SIZE x2.0 and WRITE_SIZE x1.0, calibrated on a plain copy)
+    traffic, traffic_src = None, None
+    tj = os.path.join(ROOT, "profiles", "matvec_config2_hbm_traffic.json")
+    if n == 100 and os.path.exists(tj):
+        kj = json.load(open(tj))["kernels"]
+        for name, v in kj.items():
+            if name.startswith(kern[p_dom]):
+                traffic = v["fetch_bytes"] + v["write_bytes"]
+                traffic_src = f"profiles/matvec_config2_hbm_traffic.json ({name})"
 
     out = {
         "metric": METRIC,
@@ -220,7 +232,8 @@ def main():
         "passes": passes,
         "roofline": {"bound": "hbm", "kernel": f"{names[p_dom]} ({kern[p_dom]})",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "alg_bytes_per_launch": b32[p_dom], "avg_us": avg_dom * 1e3},
     }
     if not args.no_cpu:

@@ -54,6 +54,7 @@ extern "C" void ifdda_destroy(ifdda_ctx* c) {
   for (int i = 0; i < 3; ++i) {
     (void)hipFree(c->tw64[i]);
     (void)hipFree(c->tw32[i]);
+    (void)hipFree(c->pos[i]);
   }
   (void)hipFree(c->G64);
   (void)hipFree(c->G32);
@@ -99,7 +100,7 @@ extern "C" int ifdda_create(int nx, int ny, int nz, ifdda_ctx** out) {
   // Tiles stay within 40 KB of LDS so that four workgroups share a CU.
   if (!make_tiles(g, (size_t)std::min(lds, 40960))) {
     delete c;
-    return arg_error("ifdda_create: padded FFT length above the supported maximum (640)");
+    return arg_error("ifdda_create: padded FFT length above the supported maximum");
   }
 
   auto fail = [&](int code) {
@@ -117,7 +118,12 @@ extern "C" int ifdda_create(int nx, int ny, int nz, ifdda_ctx** out) {
       t64[e] = make_double2(std::cos(ang), std::sin(ang));
       t32[e] = make_float2((float)t64[e].x, (float)t64[e].y);
     }
-    if (!hip_ok(hipMalloc(&c->tw64[a], sizeof(double2) * L), "hipMalloc tw64") ||
+    std::vector<int> pos(L);
+    axis_positions(g.ax[a], pos.data());
+    if (!hip_ok(hipMalloc(&c->pos[a], sizeof(int) * L), "hipMalloc positions") ||
+        !hip_ok(hipMemcpy(c->pos[a], pos.data(), sizeof(int) * L, hipMemcpyHostToDevice),
+                "copy positions") ||
+        !hip_ok(hipMalloc(&c->tw64[a], sizeof(double2) * L), "hipMalloc tw64") ||
         !hip_ok(hipMalloc(&c->tw32[a], sizeof(float2) * L), "hipMalloc tw32") ||
         !hip_ok(hipMemcpy(c->tw64[a], t64.data(), sizeof(double2) * L, hipMemcpyHostToDevice),
                 "copy tw64") ||
@@ -153,8 +159,10 @@ extern "C" int ifdda_green_build(ifdda_ctx* c, double aretecube, double k0, int 
   double2* full = nullptr;
   CHECK_HIP(hipMalloc(&full, sizeof(double2) * M), "hipMalloc green scratch");
   const double2* tw[3] = {c->tw64[0], c->tw64[1], c->tw64[2]};
+  const int* pos[3] = {c->pos[0], c->pos[1], c->pos[2]};
   c->green_ready = false;
-  bool ok = green_build_device(c->g, tw, aretecube, k0, nquad, full, c->G64, c->G32, c->stream);
+  bool ok = green_build_device(c->g, tw, pos, aretecube, k0, nquad, full, c->G64, c->G32,
+                               c->stream);
   ok = ok && hip_ok(hipStreamSynchronize(c->stream), "green build");
   (void)hipFree(full);
   if (!ok) return IFDDA_ERR_HIP;

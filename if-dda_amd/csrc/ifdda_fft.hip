@@ -3,7 +3,7 @@
 //
 //   P1 scatter(alpha.x) + FFT-x   (k_xpass, MODE 1)
 //   P2 FFT-y                      (k_strided)
-//   P3 FFT-z . G . IFFT-z         (k_strided, CONV)
+//   P3 FFT-z . G . IFFT-z         (k_zconv)
 //   P4 IFFT-y                     (k_strided)
 //   P5 IFFT-x + gather, b - v/M   (k_xpass, MODE 2)
 //
@@ -13,8 +13,9 @@
 //
 // One workgroup transforms a tile of NL = B*NC lines held in LDS as
 // [point][line] (line fastest, so a wavefront reads contiguous LDS words);
-// stages run in place (registers hold a stage's butterflies across a
-// barrier), so a tile needs one LDS buffer.  For the strided axes the B
+// the FFT runs in place in that one buffer (decimation in time from a
+// digit-reversed load; its transpose, decimation in frequency, for the
+// inverse inside P3), one butterfly per thread at a time.  For the strided axes the B
 // lines of a tile are B consecutive kx values (contiguous in HBM), so every
 // global row access is a contiguous B*NC-element segment.
 #include "ifdda_internal.h"
@@ -110,78 +111,111 @@ template <int SGN, typename V> struct Dft<8, SGN, V> {
 
 // Wait for this thread's LDS operations, then a workgroup barrier.  Global
 // loads already issued stay in flight across it (the compiler inserts the
-// vmcnt wait at their first use), which lets prefetches overlap FFT stages.
+// vmcnt wait at their first use).
 __device__ __forceinline__ void lds_sync() {
   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-// Butterflies per thread per stage held in registers (in-place stages).
-constexpr int kMaxJ = 4;
 // Global loads issued back to back before their LDS stores.
-constexpr int kBatch = 16;
-// Green-tensor items per thread prefetched by the convolution pass.
-constexpr int kMaxG = 6;
+constexpr int kBatch = 8;
+// Green-tensor items per thread in the convolution pass (k_zconv<V, NG> is
+// instantiated for NG = 1..kMaxG and launched with the smallest that fits).
+constexpr int kMaxG = 4;
 
-// One in-place Stockham stage of radix R over the tile in LDS.  Sub-transforms
-// of length Ns are done; butterfly j reads points j + q*L/R, twiddles input q
-// by exp(SGN 2 pi i k q / (Ns R)) with k = j mod Ns, and writes points
-// (j-k)*R + k + q*Ns.  Every thread first reads all its butterflies (at most
-// kMaxJ, j = jt + i*JT) into registers, then all threads write after a
-// barrier.  tw[e] = exp(+2 pi i e / L).
-template <int R, int SGN, typename V>
-__device__ __forceinline__ void stage_ip(V* buf, const V* __restrict__ tw, int L, int Ns, int NL,
+// In-place mixed-radix FFT of a tile held in LDS as [pos][line].
+//
+// Radices r_1..r_k (AxisPlan order), m_s = r_1...r_{s-1}.  Position
+// pos = sum_s q_s m_s (q_s < r_s) corresponds to signal index
+// n = sum_s q_s r_{s+1}...r_k; perm[n] gives pos.  Decimation in time (DIT)
+// takes the signal in permuted positions and leaves the transform in natural
+// order; its transpose, decimation in frequency (DIF), takes natural order
+// and leaves the transform permuted.  Every butterfly of stage s reads and
+// writes the same positions base + j + q*m_s (base = block*m_s*R, j < m_s), so
+// a thread holds one butterfly (R values) at a time.  tw[e] = exp(+2 pi i e/L).
+template <int R, int SGN, bool DIT, typename V>
+__device__ __forceinline__ void stage_ip(V* buf, const V* __restrict__ tw, int L, int m, int NL,
                                          int line, int jt, int JT) {
   const int nb = L / R;
-  const int tws = L / (Ns * R);
-  V a[kMaxJ][R];
+  const int tws = L / (m * R);
+  for (int t = jt; t < nb; t += JT) {
+    const int j = t % m;
+    const int base = (t - j) * R + j;
+    V a[R];
 #pragma unroll
-  for (int i = 0; i < kMaxJ; ++i) {
-    const int j = jt + i * JT;
-    if (j < nb) {
-      const int k = j % Ns;
+    for (int q = 0; q < R; ++q) a[q] = buf[(base + q * m) * NL + line];
+    if (DIT && m > 1) {
 #pragma unroll
-      for (int q = 0; q < R; ++q) a[i][q] = buf[(j + q * nb) * NL + line];
-      if (Ns > 1) {
-#pragma unroll
-        for (int q = 1; q < R; ++q) {
-          V w = tw[k * q * tws];
-          if (SGN < 0) w.y = -w.y;
-          a[i][q] = cmul(a[i][q], w);
-        }
+      for (int q = 1; q < R; ++q) {
+        V w = tw[j * q * tws];
+        if (SGN < 0) w.y = -w.y;
+        a[q] = cmul(a[q], w);
       }
-      Dft<R, SGN, V>::run(a[i]);
     }
-  }
-  lds_sync();
+    Dft<R, SGN, V>::run(a);
+    if (!DIT && m > 1) {
 #pragma unroll
-  for (int i = 0; i < kMaxJ; ++i) {
-    const int j = jt + i * JT;
-    if (j < nb) {
-      const int k = j % Ns;
-      const int ob = (j - k) * R + k;
-#pragma unroll
-      for (int q = 0; q < R; ++q) buf[(ob + q * Ns) * NL + line] = a[i][q];
+      for (int q = 1; q < R; ++q) {
+        V w = tw[j * q * tws];
+        if (SGN < 0) w.y = -w.y;
+        a[q] = cmul(a[q], w);
+      }
     }
+#pragma unroll
+    for (int q = 0; q < R; ++q) buf[(base + q * m) * NL + line] = a[q];
   }
   lds_sync();
 }
 
-// Full in-place transform of the tile.  Thread layout: line = tid % NL,
-// jt = tid / NL, JT = blockDim / NL, with JT * kMaxJ >= L / min radix.
+template <int SGN, bool DIT, typename V>
+__device__ __forceinline__ void stage_any(int r, V* buf, const V* tw, int L, int m, int NL,
+                                          int line, int jt, int JT) {
+  switch (r) {
+    case 2: stage_ip<2, SGN, DIT>(buf, tw, L, m, NL, line, jt, JT); break;
+    case 3: stage_ip<3, SGN, DIT>(buf, tw, L, m, NL, line, jt, JT); break;
+    case 4: stage_ip<4, SGN, DIT>(buf, tw, L, m, NL, line, jt, JT); break;
+    case 5: stage_ip<5, SGN, DIT>(buf, tw, L, m, NL, line, jt, JT); break;
+    default: stage_ip<8, SGN, DIT>(buf, tw, L, m, NL, line, jt, JT); break;
+  }
+}
+
+// Thread layout: line = tid % NL, jt = tid / NL, JT = blockDim / NL.
 template <int SGN, typename V>
-__device__ void tile_fft(V* buf, const V* tw, const AxisPlan& ax, int NL, int line, int jt,
-                         int JT) {
-  int Ns = 1;
+__device__ void tile_fft_dit(V* buf, const V* tw, const AxisPlan& ax, int NL, int line, int jt,
+                             int JT) {
+  int m = 1;
   for (int s = 0; s < ax.nst; ++s) {
-    const int r = ax.rad[s];
-    switch (r) {
-      case 2: stage_ip<2, SGN>(buf, tw, ax.L, Ns, NL, line, jt, JT); break;
-      case 3: stage_ip<3, SGN>(buf, tw, ax.L, Ns, NL, line, jt, JT); break;
-      case 4: stage_ip<4, SGN>(buf, tw, ax.L, Ns, NL, line, jt, JT); break;
-      case 5: stage_ip<5, SGN>(buf, tw, ax.L, Ns, NL, line, jt, JT); break;
-      default: stage_ip<8, SGN>(buf, tw, ax.L, Ns, NL, line, jt, JT); break;
-    }
-    Ns *= r;
+    stage_any<SGN, true>(ax.rad[s], buf, tw, ax.L, m, NL, line, jt, JT);
+    m *= ax.rad[s];
+  }
+}
+
+template <int SGN, typename V>
+__device__ void tile_fft_dif(V* buf, const V* tw, const AxisPlan& ax, int NL, int line, int jt,
+                             int JT) {
+  int m = ax.L;
+  for (int s = ax.nst - 1; s >= 0; --s) {
+    m /= ax.rad[s];
+    stage_any<SGN, false>(ax.rad[s], buf, tw, ax.L, m, NL, line, jt, JT);
+  }
+}
+
+// LDS layout of every FFT pass: twiddles (L values), positions (L ints),
+// then the tile (L * NL values).
+template <typename V> __device__ __forceinline__ V* tile_base(V* tw, int L) {
+  const size_t off = ((size_t)L * sizeof(V) + (size_t)L * sizeof(int) + 15) & ~(size_t)15;
+  return reinterpret_cast<V*>(reinterpret_cast<char*>(tw) + off);
+}
+template <typename V> size_t tile_lds_bytes(int L, int NL) {
+  return (((size_t)L * sizeof(V) + (size_t)L * sizeof(int) + 15) & ~(size_t)15) +
+         (size_t)L * NL * sizeof(V);
+}
+
+template <typename V>
+__device__ __forceinline__ void load_tables(V* tw, int* pos, const V* gtw, const int* gpos, int L,
+                                            int tid, int nthr) {
+  for (int e = tid; e < L; e += nthr) {
+    tw[e] = gtw[e];
+    pos[e] = gpos[e];
   }
 }
 
@@ -193,12 +227,13 @@ template <typename V> struct SArgs {
   const V* src;
   V* dst;
   const V* tw;
+  const int* pos;   // perm[n] of the axis
   AxisPlan ax;
   int n_in, n_out;  // points loaded (rest zero) / stored
   int nbx;          // extent of the contiguous kx dimension
   int B;            // kx per tile
   long long so_src, sp_src, so_dst, sp_dst;
-  // CONV only: folded G[((ky'*F3 + kz')*F1 + kx')*6 + c], outer index = ky
+  // k_zconv only: folded G[((ky'*F3 + kz')*F1 + kx')*6 + c]
   const V* G;
   int F1, F3, L1, L2;
   const int* skip;  // when non-null and *skip != 0 the pass does nothing
@@ -213,94 +248,153 @@ __device__ __forceinline__ int fold_idx(int k, int L, int& sgn) {
   return k;
 }
 
-template <typename V, int NC, int SGN, bool CONV>
-__global__ void __launch_bounds__(256) k_strided(SArgs<V> a) {
-  if (a.skip && *a.skip) return;
-  extern __shared__ double4 smem_d4[];
-  V* tw = reinterpret_cast<V*>(smem_d4);
-  const int L = a.ax.L;
-  const int NL = a.B * NC;
-  V* buf = tw + L;
-  const int tid = threadIdx.x;
-  const int nthr = blockDim.x;
-  const int line = tid % NL, jt = tid / NL, JT = nthr / NL;
-  const int o = blockIdx.y;
-  const int kx0 = blockIdx.x * a.B;
-  const int nvalid = min(a.B, a.nbx - kx0) * NC;
+// Block ids are dispatched round-robin over the 8 XCDs (id % 8), each with its
+// own L2.  Remap the linear id so that the blocks of one XCD take a
+// contiguous range of logical tiles: neighbouring tiles, which share cache
+// lines at their edges, then meet in the same L2.  A bijection of [0, total).
+__device__ __forceinline__ int xcd_remap(int lin, int total) {
+  const int q = total >> 3, r = total & 7;
+  const int xcd = lin & 7, j = lin >> 3;
+  return xcd * q + min(xcd, r) + j;
+}
 
-  // CONV: prefetch this thread's Green-tensor entries (items idx = tid + it*nthr
-  // over (kz, kxl)); their latency overlaps the tile load and forward FFT.
-  V gr[CONV ? kMaxG : 1][6];
-  int gsx[CONV ? kMaxG : 1];
-  if (CONV) {
-    int sy;
-    const int kyf = fold_idx(o, a.L2, sy);
-#pragma unroll
-    for (int it = 0; it < kMaxG; ++it) {
-      const int idx = tid + it * nthr;
-      gsx[it] = 0;
-      if (idx < L * a.B) {
-        const int kz = idx / a.B, kxl = idx - kz * a.B;
-        const int kx = kx0 + kxl;
-        if (kx < a.nbx) {
-          int sx, sz;
-          const int kxf = fold_idx(kx, a.L1, sx);
-          const int kzf = fold_idx(kz, L, sz);
-          const V* g = a.G + ((long long)(kyf * a.F3 + kzf) * a.F1 + kxf) * 6;
-#pragma unroll
-          for (int c = 0; c < 6; ++c) gr[it][c] = g[c];
-          gsx[it] = 1 | ((sx * sy < 0) << 1) | ((sx * sz < 0) << 2) | ((sy * sz < 0) << 3);
-        }
-      }
-    }
-  }
-
-  for (int e = tid; e < L; e += nthr) tw[e] = a.tw[e];
-  const V* src = a.src + (long long)o * a.so_src + (long long)kx0 * NC;
+// Load rows p < n_in of a strided tile into permuted LDS positions (zeros
+// elsewhere).
+template <typename V>
+__device__ __forceinline__ void load_strided(V* buf, const int* pos, const V* src, long long sp,
+                                             int L, int n_in, int NL, int line, int nvalid,
+                                             int jt, int JT) {
   for (int p0 = jt; p0 < L; p0 += kBatch * JT) {
     V v[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       const int p = p0 + u * JT;
       v[u] = czero<V>();
-      if (p < a.n_in && line < nvalid) v[u] = src[(long long)p * a.sp_src + line];
+      if (p < n_in && line < nvalid) v[u] = src[(long long)p * sp + line];
     }
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       const int p = p0 + u * JT;
-      if (p < L) buf[p * NL + line] = v[u];
+      if (p < L) buf[pos[p] * NL + line] = v[u];
     }
   }
   lds_sync();
+}
 
-  if (CONV) {
-    tile_fft<+1>(buf, tw, a.ax, NL, line, jt, JT);
-    // Symmetric 3x3 product with G(kx, ky, kz) (produitfftmatvect3.f:76-90).
-#pragma unroll
-    for (int it = 0; it < kMaxG; ++it) {
-      if (!(gsx[it] & 1)) continue;
-      const int idx = tid + it * nthr;
-      const int kz = idx / a.B, kxl = idx - kz * a.B;
-      const V gxx = gr[it][0], gyy = gr[it][3], gzz = gr[it][5];
-      V gxy = gr[it][1], gxz = gr[it][2], gyz = gr[it][4];
-      if (gsx[it] & 2) gxy = mk(-gxy.x, -gxy.y);
-      if (gsx[it] & 4) gxz = mk(-gxz.x, -gxz.y);
-      if (gsx[it] & 8) gyz = mk(-gyz.x, -gyz.y);
-      V* v = buf + kz * NL + kxl * 3;
-      const V ex = v[0], ey = v[1], ez = v[2];
-      v[0] = cadd(cadd(cmul(gxx, ex), cmul(gxy, ey)), cmul(gxz, ez));
-      v[1] = cadd(cadd(cmul(gxy, ex), cmul(gyy, ey)), cmul(gyz, ez));
-      v[2] = cadd(cadd(cmul(gxz, ex), cmul(gyz, ey)), cmul(gzz, ez));
-    }
-    lds_sync();
-    tile_fft<-1>(buf, tw, a.ax, NL, line, jt, JT);
-  } else {
-    tile_fft<SGN>(buf, tw, a.ax, NL, line, jt, JT);
-  }
+template <typename V, int NC, int SGN>
+__global__ void __launch_bounds__(256) k_strided(SArgs<V> a) {
+  if (a.skip && *a.skip) return;
+  extern __shared__ double4 smem_d4[];
+  V* tw = reinterpret_cast<V*>(smem_d4);
+  const int L = a.ax.L;
+  int* pos = reinterpret_cast<int*>(tw + L);
+  V* buf = tile_base(tw, L);
+  const int NL = a.B * NC;
+  const int tid = threadIdx.x;
+  const int nthr = blockDim.x;
+  const int line = tid % NL, jt = tid / NL, JT = nthr / NL;
+  const int lg = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int o = lg / gridDim.x;
+  const int kx0 = (lg - o * gridDim.x) * a.B;
+  const int nvalid = min(a.B, a.nbx - kx0) * NC;
 
+  load_tables(tw, pos, a.tw, a.pos, L, tid, nthr);
+  lds_sync();
+  load_strided(buf, pos, a.src + (long long)o * a.so_src + (long long)kx0 * NC, a.sp_src, L,
+               a.n_in, NL, line, nvalid, jt, JT);
+  tile_fft_dit<SGN>(buf, tw, a.ax, NL, line, jt, JT);
   V* dst = a.dst + (long long)o * a.so_dst + (long long)kx0 * NC;
   if (line < nvalid) {
     for (int p = jt; p < a.n_out; p += JT) dst[(long long)p * a.sp_dst + line] = buf[p * NL + line];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// P3: FFT-z . G . IFFT-z on the y-transformed planes, in place.  One block
+// takes a tile of B consecutive kx and the two mirror rows ky' and L2-ky'
+// (one after the other); each thread loads the folded Green entries of its
+// items (kz', kxl), kz' <= L3/2, and applies them to kz' and L3-kz' with the
+// parity signs, so the octant array is read once per (kx tile, ky').  The
+// forward transform is DIT (natural output for the product), the inverse DIF
+// (natural input, permuted output read back through pos).
+// ---------------------------------------------------------------------------
+template <typename V, int NG>
+__global__ void __launch_bounds__(256) k_zconv(SArgs<V> a) {
+  if (a.skip && *a.skip) return;
+  extern __shared__ double4 smem_d4[];
+  V* tw = reinterpret_cast<V*>(smem_d4);
+  const int L = a.ax.L;  // L3
+  int* pos = reinterpret_cast<int*>(tw + L);
+  V* buf = tile_base(tw, L);
+  const int NL = a.B * 3;
+  const int tid = threadIdx.x;
+  const int nthr = blockDim.x;
+  const int line = tid % NL, jt = tid / NL, JT = nthr / NL;
+  const int lg = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int kyf = lg / gridDim.x;  // 0 .. L2/2
+  const int kx0 = (lg - kyf * gridDim.x) * a.B;
+  const int nvalid = min(a.B, a.nbx - kx0) * 3;
+  const int F3 = L / 2 + 1;
+
+  load_tables(tw, pos, a.tw, a.pos, L, tid, nthr);
+  lds_sync();
+  const int nrow = (kyf == 0 || 2 * kyf == a.L2) ? 1 : 2;
+  for (int rr = 0; rr < nrow; ++rr) {
+    const int ky = rr == 0 ? kyf : a.L2 - kyf;
+    const bool sy = rr == 1;  // y odd-parity sign for the mirror row
+    load_strided(buf, pos, a.src + (long long)ky * a.so_src + (long long)kx0 * 3, a.sp_src, L,
+                 a.n_in, NL, line, nvalid, jt, JT);
+    tile_fft_dit<+1>(buf, tw, a.ax, NL, line, jt, JT);
+    // Green entries of this thread's items, all loads issued first.
+    V gr[NG][6];
+    int gx[NG];  // bit0 valid, bit1 x odd-parity sign
+#pragma unroll
+    for (int it = 0; it < NG; ++it) {
+      const int idx = tid + it * nthr;
+      gx[it] = 0;
+      if (idx < F3 * a.B) {
+        const int kzf = idx / a.B, kxl = idx - kzf * a.B;
+        const int kx = kx0 + kxl;
+        if (kx < a.nbx) {
+          int sx;
+          const int kxf = fold_idx(kx, a.L1, sx);
+          const V* g = a.G + ((long long)(kyf * a.F3 + kzf) * a.F1 + kxf) * 6;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) gr[it][c] = g[c];
+          gx[it] = 1 | ((sx < 0) << 1);
+        }
+      }
+    }
+    // Symmetric 3x3 product (produitfftmatvect3.f:76-90) at kz' and L3-kz'.
+#pragma unroll
+    for (int it = 0; it < NG; ++it) {
+      if (!(gx[it] & 1)) continue;
+      const int idx = tid + it * nthr;
+      const int kzf = idx / a.B, kxl = idx - kzf * a.B;
+      const bool sxn = gx[it] & 2;
+      const int nz2 = (kzf == 0 || 2 * kzf == L) ? 1 : 2;
+      for (int zz = 0; zz < nz2; ++zz) {
+        const int kz = zz == 0 ? kzf : L - kzf;
+        const bool szn = zz == 1;
+        V gxy = gr[it][1], gxz = gr[it][2], gyz = gr[it][4];
+        if (sxn != sy) gxy = mk(-gxy.x, -gxy.y);
+        if (sxn != szn) gxz = mk(-gxz.x, -gxz.y);
+        if (sy != szn) gyz = mk(-gyz.x, -gyz.y);
+        V* v = buf + kz * NL + kxl * 3;
+        const V ex = v[0], ey = v[1], ez = v[2];
+        v[0] = cadd(cadd(cmul(gr[it][0], ex), cmul(gxy, ey)), cmul(gxz, ez));
+        v[1] = cadd(cadd(cmul(gxy, ex), cmul(gr[it][3], ey)), cmul(gyz, ez));
+        v[2] = cadd(cadd(cmul(gxz, ex), cmul(gyz, ey)), cmul(gr[it][5], ez));
+      }
+    }
+    lds_sync();
+    tile_fft_dif<-1>(buf, tw, a.ax, NL, line, jt, JT);
+    V* dst = a.dst + (long long)ky * a.so_dst + (long long)kx0 * 3;
+    if (line < nvalid) {
+      for (int p = jt; p < a.n_out; p += JT)
+        dst[(long long)p * a.sp_dst + line] = buf[pos[p] * NL + line];
+    }
+    lds_sync();  // the next row reuses the LDS tile
   }
 }
 
@@ -313,6 +407,7 @@ template <typename V> struct XArgs {
   const V* src;
   V* dst;
   const V* tw;
+  const int* pos;
   AxisPlan ax;
   int n_in, n_out;
   int nlines;  // total lines
@@ -360,15 +455,17 @@ __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
   extern __shared__ double4 smem_d4[];
   V* tw = reinterpret_cast<V*>(smem_d4);
   const int L = a.ax.L;
+  int* pos = reinterpret_cast<int*>(tw + L);
+  V* buf = tile_base(tw, L);
   const int NL = a.B * NC;
-  V* buf = tw + L;
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
   const int line = tid % NL, jt = tid / NL, JT = nthr / NL;
-  const int g0 = blockIdx.x * a.B;
+  const int g0 = xcd_remap(blockIdx.x, gridDim.x) * a.B;
   const int nl = min(a.B, a.nlines - g0);
 
-  for (int e = tid; e < L; e += nthr) tw[e] = a.tw[e];
+  load_tables(tw, pos, a.tw, a.pos, L, tid, nthr);
+  lds_sync();
   if (MODE == 1) {
     constexpr int U = kBatch / 2;
     for (int e0 = tid; e0 < a.B * L; e0 += U * nthr) {
@@ -397,7 +494,7 @@ __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
         const int e = e0 + u * nthr;
         if (e < a.B * L) {
           const int l = e / L, i = e - l * L;
-          V* d = buf + i * NL + l * 3;
+          V* d = buf + pos[i] * NL + l * 3;
           d[0] = x[u][0];
           d[1] = x[u][1];
           d[2] = x[u][2];
@@ -423,14 +520,14 @@ __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
         if (e < a.B * per) {
           const int l = e / per, r = e - l * per;
           const int p = r / NC, c = r - p * NC;
-          buf[p * NL + l * NC + c] = v[u];
+          buf[pos[p] * NL + l * NC + c] = v[u];
         }
       }
     }
   }
   lds_sync();
 
-  tile_fft<SGN>(buf, tw, a.ax, NL, line, jt, JT);
+  tile_fft_dit<SGN>(buf, tw, a.ax, NL, line, jt, JT);
 
   if (MODE == 2) {
     constexpr int U = kBatch / 2;
@@ -679,29 +776,40 @@ bool make_axis_plan(int L, AxisPlan* out) {
   return true;
 }
 
-// Largest batch width B (lines per tile = B*NC) such that the one-buffer tile
-// fits the LDS budget, every thread holds at most kMaxJ butterflies per stage
-// (JT * kMaxJ >= L / smallest radix), and, for the convolution pass, every
-// thread prefetches at most kMaxG Green-tensor items (L*B <= kMaxG*threads).
+// Largest batch width B (lines per tile = B*NC) such that the tile fits the
+// LDS budget and, for the convolution pass, every thread holds at most kMaxG
+// Green-tensor items ((L/2+1)*B <= kMaxG*threads).
 static TileShape tile_for(const AxisPlan& ax, int NC, int esz, int nbatch, size_t lds_limit,
                           bool conv) {
   const int L = ax.L;
-  int rmin = 8;
-  for (int s = 0; s < ax.nst; ++s) rmin = std::min(rmin, ax.rad[s]);
-  const int max_nb = L / rmin;
   TileShape t{0, 0, 0};  // B = 0: no feasible tile
   for (int B = 1; B <= 64 && B <= std::max(1, nbatch); ++B) {
     const int NL = B * NC;
     if (NL > 256) break;
     const int JT = 256 / NL;
-    if (JT * kMaxJ < max_nb) break;
     const int threads = NL * JT;
-    const size_t lds = ((size_t)L + (size_t)L * NL) * esz;
+    const size_t lds = esz == 16 ? tile_lds_bytes<double2>(L, NL) : tile_lds_bytes<float2>(L, NL);
     if (lds > lds_limit) break;
-    if (conv && L * B > kMaxG * threads) break;
+    if (conv && (L / 2 + 1) * B > kMaxG * threads) break;
     t = TileShape{B, threads, lds};
   }
   return t;
+}
+
+void axis_positions(const AxisPlan& ax, int* pos) {
+  // pos[n] for n = sum_s q_s r_{s+1}...r_k, position sum_s q_s m_s
+  const int k = ax.nst;
+  for (int p = 0; p < ax.L; ++p) {
+    int rem = p, n = 0;
+    for (int s = 0; s < k; ++s) {
+      const int q = rem % ax.rad[s];
+      rem /= ax.rad[s];
+      int P = 1;
+      for (int t = s + 1; t < k; ++t) P *= ax.rad[t];
+      n += q * P;
+    }
+    pos[n] = p;
+  }
 }
 
 bool make_tiles(GridPlan& g, size_t lds_limit) {
@@ -745,6 +853,7 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     XArgs<V> a{};
     a.dst = m.A;
     a.tw = m.tw[0];
+    a.pos = m.pos[0];
     a.ax = g.ax[0];
     a.n_in = nx;
     a.n_out = L1;
@@ -769,6 +878,7 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.src = m.A;
     a.dst = m.B;
     a.tw = m.tw[1];
+    a.pos = m.pos[1];
     a.ax = g.ax[1];
     a.n_in = ny;
     a.n_out = L2;
@@ -779,7 +889,7 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.sp_src = l1c;
     a.so_dst = (long long)L2 * l1c;
     a.sp_dst = l1c;
-    auto k = k_strided<V, 3, +1, false>;
+    auto k = k_strided<V, 3, +1>;
     if (!allow_lds(k, t.lds)) return false;
     if (tm) tm->begin(pi * 5 + 1, s);
     hipLaunchKernelGGL(k, dim3((L1 + t.B - 1) / t.B, nz), dim3(t.threads), t.lds, s, a);
@@ -791,6 +901,7 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.src = m.B;
     a.dst = m.B;
     a.tw = m.tw[2];
+    a.pos = m.pos[2];
     a.ax = g.ax[2];
     a.n_in = nz;
     a.n_out = nz;
@@ -804,10 +915,13 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.F3 = g.F[2];
     a.L1 = L1;
     a.L2 = L2;
-    auto k = k_strided<V, 3, +1, true>;
+    const int items = g.F[2] * t.B;
+    const int ng = (items + t.threads - 1) / t.threads;
+    auto k = ng <= 1 ? k_zconv<V, 1> : ng == 2 ? k_zconv<V, 2> : ng == 3 ? k_zconv<V, 3>
+                                                                        : k_zconv<V, kMaxG>;
     if (!allow_lds(k, t.lds)) return false;
     if (tm) tm->begin(pi * 5 + 2, s);
-    hipLaunchKernelGGL(k, dim3((L1 + t.B - 1) / t.B, L2), dim3(t.threads), t.lds, s, a);
+    hipLaunchKernelGGL(k, dim3((L1 + t.B - 1) / t.B, L2 / 2 + 1), dim3(t.threads), t.lds, s, a);
     if (tm) tm->end(s);
   }
   {  // P4: IFFT-y, keeps j < ny
@@ -816,6 +930,7 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.src = m.B;
     a.dst = m.A;
     a.tw = m.tw[1];
+    a.pos = m.pos[1];
     a.ax = g.ax[1];
     a.n_in = L2;
     a.n_out = ny;
@@ -826,7 +941,7 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.sp_src = l1c;
     a.so_dst = (long long)ny * l1c;
     a.sp_dst = l1c;
-    auto k = k_strided<V, 3, -1, false>;
+    auto k = k_strided<V, 3, -1>;
     if (!allow_lds(k, t.lds)) return false;
     if (tm) tm->begin(pi * 5 + 3, s);
     hipLaunchKernelGGL(k, dim3((L1 + t.B - 1) / t.B, nz), dim3(t.threads), t.lds, s, a);
@@ -837,6 +952,7 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     XArgs<V> a{};
     a.src = m.A;
     a.tw = m.tw[0];
+    a.pos = m.pos[0];
     a.ax = g.ax[0];
     a.n_in = L1;
     a.n_out = nx;
@@ -870,13 +986,15 @@ bool launch_matvec_f32(const GridPlan& g, const MatvecDev<float2>& m, const floa
 }
 
 // In-place forward 3D FFT (sign +1) of one FP64 component on the full grid.
-static bool fft3d_full_f64(const GridPlan& g, const double2* tw[3], double2* v, hipStream_t s) {
+static bool fft3d_full_f64(const GridPlan& g, const double2* tw[3], const int* pos[3], double2* v,
+                           hipStream_t s) {
   const int L1 = g.L[0], L2 = g.L[1], L3 = g.L[2];
   {
     XArgs<double2> a{};
     a.src = v;
     a.dst = v;
     a.tw = tw[0];
+    a.pos = pos[0];
     a.ax = g.ax[0];
     a.n_in = a.n_out = L1;
     a.nlines = L2 * L3;
@@ -891,13 +1009,14 @@ static bool fft3d_full_f64(const GridPlan& g, const double2* tw[3], double2* v, 
     a.src = v;
     a.dst = v;
     a.tw = tw[1];
+    a.pos = pos[1];
     a.ax = g.ax[1];
     a.n_in = a.n_out = L2;
     a.nbx = L1;
     a.B = g.gy.B;
     a.so_src = a.so_dst = (long long)L1 * L2;
     a.sp_src = a.sp_dst = L1;
-    auto k = k_strided<double2, 1, +1, false>;
+    auto k = k_strided<double2, 1, +1>;
     if (!allow_lds(k, g.gy.lds)) return false;
     hipLaunchKernelGGL(k, dim3((L1 + a.B - 1) / a.B, L3), dim3(g.gy.threads), g.gy.lds, s, a);
   }
@@ -906,13 +1025,14 @@ static bool fft3d_full_f64(const GridPlan& g, const double2* tw[3], double2* v, 
     a.src = v;
     a.dst = v;
     a.tw = tw[2];
+    a.pos = pos[2];
     a.ax = g.ax[2];
     a.n_in = a.n_out = L3;
     a.nbx = L1;
     a.B = g.gz.B;
     a.so_src = a.so_dst = L1;
     a.sp_src = a.sp_dst = (long long)L1 * L2;
-    auto k = k_strided<double2, 1, +1, false>;
+    auto k = k_strided<double2, 1, +1>;
     if (!allow_lds(k, g.gz.lds)) return false;
     hipLaunchKernelGGL(k, dim3((L1 + a.B - 1) / a.B, L2), dim3(g.gz.threads), g.gz.lds, s, a);
   }
@@ -928,8 +1048,9 @@ bool green_fold_component(const GridPlan& g, const double2* full, int comp, doub
   return hip_ok(hipGetLastError(), "green fold");
 }
 
-bool green_build_device(const GridPlan& g, const double2* tw64[3], double d, double k0, int nquad,
-                        double2* full, double2* g64, float2* g32, hipStream_t s) {
+bool green_build_device(const GridPlan& g, const double2* tw64[3], const int* pos[3], double d,
+                        double k0, int nquad, double2* full, double2* g64, float2* g32,
+                        hipStream_t s) {
   const long long M = (long long)g.L[0] * g.L[1] * g.L[2];
   double2* near = nullptr;
   double2* part = nullptr;
@@ -965,7 +1086,7 @@ bool green_build_device(const GridPlan& g, const double2* tw64[3], double d, dou
     const int blocks = (int)std::min<long long>((M + 255) / 256, 65535);
     hipLaunchKernelGGL(k_green_fill, dim3(blocks), dim3(256), 0, s, full, c, g.nx, g.ny, g.nz,
                        g.L[0], g.L[1], g.L[2], d, k0, near, nquad);
-    ok = hip_ok(hipGetLastError(), "green fill") && fft3d_full_f64(g, tw64, full, s) &&
+    ok = hip_ok(hipGetLastError(), "green fill") && fft3d_full_f64(g, tw64, pos, full, s) &&
          green_fold_component(g, full, c, g64, g32, s);
   }
   if (near || part || offs) {

@@ -34,6 +34,7 @@ template <typename V> struct MatvecDev {
   V* B;               // nz*L2*L1*3 scratch (after y pass)
   const V* G;         // folded Green tensor, F2*F3*F1*6
   const V* tw[3];     // twiddles exp(+2 pi i e/L) per axis
+  const int* pos[3];  // in-place FFT positions per axis (axis_positions)
   const int32_t* tab; // box -> 1-based dipole, or null (dense box)
   const V* alpha;     // per dipole, see alpha_mode
   int alpha_mode;
@@ -65,6 +66,8 @@ bool hip_ok(hipError_t e, const char* what);
 
 int smooth235_at_least(int n);
 bool make_axis_plan(int L, AxisPlan* out);
+// pos[n]: LDS position of signal index n for the in-place mixed-radix FFT.
+void axis_positions(const AxisPlan& ax, int* pos);
 // false when an axis is too long for the register-staged FFT tiles
 bool make_tiles(GridPlan& g, size_t lds_limit);
 
@@ -80,8 +83,9 @@ bool launch_matvec_f32(const GridPlan& g, const MatvecDev<float2>& m, const floa
 
 // Green tensor on the device: fills g64/g32 (folded, F2*F3*F1*6) using
 // `full` (L1*L2*L3 FP64 scratch).  nquad > 0 integrates near cells.
-bool green_build_device(const GridPlan& g, const double2* tw64[3], double d, double k0, int nquad,
-                        double2* full, double2* g64, float2* g32, hipStream_t s);
+bool green_build_device(const GridPlan& g, const double2* tw64[3], const int* pos[3], double d,
+                        double k0, int nquad, double2* full, double2* g64, float2* g32,
+                        hipStream_t s);
 // Fold one caller-supplied FFT'd component already in `full` into g64/g32.
 bool green_fold_component(const GridPlan& g, const double2* full, int comp, double2* g64,
                           float2* g32, hipStream_t s);
@@ -95,6 +99,7 @@ struct ifdda_ctx {
   int device = 0;
   double2* tw64[3] = {nullptr, nullptr, nullptr};
   float2* tw32[3] = {nullptr, nullptr, nullptr};
+  int* pos[3] = {nullptr, nullptr, nullptr};  // in-place FFT positions per axis
   double2* G64 = nullptr;  // folded Green tensor, F2*F3*F1*6
   float2* G32 = nullptr;
   bool green_ready = false;
@@ -122,7 +127,7 @@ struct ifdda_ctx {
     m.A = (double2*)scrA;
     m.B = (double2*)scrB;
     m.G = G64;
-    for (int i = 0; i < 3; ++i) m.tw[i] = tw64[i];
+    for (int i = 0; i < 3; ++i) m.tw[i] = tw64[i], m.pos[i] = pos[i];
     m.tab = tab;
     m.alpha = al64;
     m.alpha_mode = alpha_mode;
@@ -134,7 +139,7 @@ struct ifdda_ctx {
     m.A = (float2*)scrA;
     m.B = (float2*)scrB;
     m.G = G32;
-    for (int i = 0; i < 3; ++i) m.tw[i] = tw32[i];
+    for (int i = 0; i < 3; ++i) m.tw[i] = tw32[i], m.pos[i] = pos[i];
     m.tab = tab;
     m.alpha = al32;
     m.alpha_mode = alpha_mode;
