@@ -97,8 +97,7 @@ extern "C" int ifdda_create(int nx, int ny, int nz, ifdda_ctx** out) {
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) !=
       hipSuccess)
     lds = 65536;
-  // Tiles stay within 40 KB of LDS so that four workgroups share a CU.
-  if (!make_tiles(g, (size_t)std::min(lds, 40960))) {
+  if (!make_tiles(g, (size_t)std::min(lds, 65536))) {
     delete c;
     return arg_error("ifdda_create: padded FFT length above the supported maximum");
   }

@@ -813,16 +813,21 @@ void axis_positions(const AxisPlan& ax, int* pos) {
 }
 
 bool make_tiles(GridPlan& g, size_t lds_limit) {
-  const int L1 = g.L[0], L2 = g.L[1], L3 = g.L[2];
-  (void)L2;
-  (void)L3;
+  const int L1 = g.L[0];
+  // LDS budgets per block from a tile-width sweep on MI355X (200^3 grid):
+  // FP32 passes are fastest at ~26 KB (B = 5 for L = 200), FP64 y/x passes
+  // at ~33 KB (B = 3) and the FP64 z convolution at ~52 KB (B = 5); smaller
+  // tiles leave more workgroups per CU to overlap load, FFT and store phases.
+  const size_t b32 = std::min<size_t>(lds_limit, 26624), b64 = std::min<size_t>(lds_limit, 33792),
+               b64c = std::min<size_t>(lds_limit, 53248);
   for (int p = 0; p < 2; ++p) {
     const int esz = p == 0 ? 16 : 8;
-    g.p1[p] = tile_for(g.ax[0], 3, esz, g.ny * g.nz, lds_limit, false);
-    g.p2[p] = tile_for(g.ax[1], 3, esz, L1, lds_limit, false);
-    g.p3[p] = tile_for(g.ax[2], 3, esz, L1, lds_limit, true);
-    g.p4[p] = tile_for(g.ax[1], 3, esz, L1, lds_limit, false);
-    g.p5[p] = tile_for(g.ax[0], 3, esz, g.ny * g.nz, lds_limit, false);
+    const size_t bs = p == 0 ? b64 : b32, bc = p == 0 ? b64c : b32;
+    g.p1[p] = tile_for(g.ax[0], 3, esz, g.ny * g.nz, bs, false);
+    g.p2[p] = tile_for(g.ax[1], 3, esz, L1, bs, false);
+    g.p3[p] = tile_for(g.ax[2], 3, esz, L1, bc, true);
+    g.p4[p] = tile_for(g.ax[1], 3, esz, L1, bs, false);
+    g.p5[p] = tile_for(g.ax[0], 3, esz, g.ny * g.nz, bs, false);
   }
   g.gx = tile_for(g.ax[0], 1, 16, g.L[1] * g.L[2], lds_limit, false);
   g.gy = tile_for(g.ax[1], 1, 16, L1, lds_limit, false);

@@ -117,24 +117,22 @@ __device__ __forceinline__ void block_partials(double2 (&v)[NR], double2* part, 
     }
 }
 
-// Sum the partials of NR reductions (one block of kThreads threads) into
-// st->dots; returns in every thread after a barrier.
+// Sum the partials of NR reductions into st->dots.  Launched as one block;
+// the first wavefront does the work with shuffles (fixed order), the other
+// threads of the block wait at the final barrier.
 template <int NR>
 __device__ __forceinline__ void finish_sum(KState* st, const double2* part, int nblk) {
-  __shared__ double2 sh[NR][kThreads];
-  for (int r = 0; r < NR; ++r) {
-    double2 s = make_double2(0.0, 0.0);
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s = zadd(s, part[r * nblk + b]);
-    sh[r][threadIdx.x] = s;
+  if (threadIdx.x < 64) {
+    for (int r = 0; r < NR; ++r) {
+      double2 s = make_double2(0.0, 0.0);
+      for (int b = threadIdx.x; b < nblk; b += 64) s = zadd(s, part[r * nblk + b]);
+      for (int off = 32; off > 0; off >>= 1) {
+        s.x += __shfl_down(s.x, off, 64);
+        s.y += __shfl_down(s.y, off, 64);
+      }
+      if (threadIdx.x == 0) st->dots[r] = s;
+    }
   }
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
-      for (int r = 0; r < NR; ++r) sh[r][threadIdx.x] = zadd(sh[r][threadIdx.x], sh[r][threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0)
-    for (int r = 0; r < NR; ++r) st->dots[r] = sh[r][0];
   __syncthreads();
 }
 
@@ -948,28 +946,30 @@ static bool krylov(ifdda_ctx* c, Solver* S, V** vv, int method, double tol, int 
   V* Ax = vv[13];
   hipLaunchKernelGGL(k_init<V>, G, T, 0, s, (const V*)b, x, x0zero ? (const V*)nullptr : (const V*)x,
                      n, S->part, nb);
-  hipLaunchKernelGGL(f_init, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+  hipLaunchKernelGGL(f_init, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
   if (!x0zero) {
     if (!mv<V>(c, x, x, Ax, s, nullptr)) return false;
     ++nmv;
   }
-  if (check_every <= 0) check_every = 1;
+  // Poll the device state every 8 iterations by default: work queued after
+  // the solve ended is dropped on the device (KState.done, matvec skip flag).
+  if (check_every <= 0) check_every = 8;
 
   if (method == IFDDA_GPBICG1) {
     GV<V> v{x, b, vv[2], vv[3], vv[4], vv[5], vv[6], vv[7], vv[8], vv[9], vv[10], vv[11], vv[12], Ax};
     hipLaunchKernelGGL(g_start<V>, G, T, 0, s, v, x0zero, n, S->part, nb);
-    hipLaunchKernelGGL(g_fstart, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+    hipLaunchKernelGGL(g_fstart, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
     for (int it = 0;; ++it) {
       hipLaunchKernelGGL(g_k1<V>, G, T, 0, s, (const KState*)S->st, v, n);
       if (!mv<V>(c, v.p, v.p, v.Ap, s, done)) return false;
       hipLaunchKernelGGL(g_k2<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(g_f2, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(g_f2, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(g_k3<V>, G, T, 0, s, (const KState*)S->st, v, n);
       if (!mv<V>(c, v.t, v.t, v.At, s, done)) return false;
       hipLaunchKernelGGL(g_k4<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(g_f4, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(g_f4, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(g_k5<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(g_f5, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb, S->trace);
+      hipLaunchKernelGGL(g_f5, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb, S->trace);
       if (!hip_ok(hipGetLastError(), "gpbicg launch")) return false;
       if ((it + 1) % check_every == 0 || it >= maxit + 1) {
         if (!poll(S, s)) return false;
@@ -990,22 +990,22 @@ static bool krylov(ifdda_ctx* c, Solver* S, V** vv, int method, double tol, int 
   } else if (method == IFDDA_QMRBICGSTAB1 || method == IFDDA_QMRBICGSTAB2) {
     QV<V> v{x, b, vv[2], vv[3], vv[4], vv[5], vv[6], vv[7], vv[8], vv[9], vv[10], Ax};
     hipLaunchKernelGGL(q_start<V>, G, T, 0, s, v, x0zero, n, S->part, nb);
-    hipLaunchKernelGGL(q_fstart, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+    hipLaunchKernelGGL(q_fstart, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
     for (int it = 0;; ++it) {
       hipLaunchKernelGGL(q_k1<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(q_f1, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(q_f1, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(q_k2<V>, G, T, 0, s, (const KState*)S->st, v, n);
       if (!mv<V>(c, v.p, v.p, v.v, s, done)) return false;
       hipLaunchKernelGGL(q_k3<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(q_f3, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(q_f3, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(q_k4<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(q_f4, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(q_f4, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(q_k5<V>, G, T, 0, s, (const KState*)S->st, v, n);
       if (!mv<V>(c, v.s, v.s, v.t, s, done)) return false;
       hipLaunchKernelGGL(q_k6<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(q_f6, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(q_f6, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(q_k7<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(q_f7, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(q_f7, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(q_k8<V>, G, T, 0, s, (const KState*)S->st, v, n);
       hipLaunchKernelGGL(q_f8, dim3(1), dim3(1), 0, s, S->st, S->trace);
       if (!hip_ok(hipGetLastError(), "qmrbicgstab launch")) return false;
@@ -1032,20 +1032,20 @@ static bool krylov(ifdda_ctx* c, Solver* S, V** vv, int method, double tol, int 
     if (!mv<V>(c, v.u, v.u, v.v, s, nullptr)) return false;
     ++nmv;
     hipLaunchKernelGGL(t_start2<V>, G, T, 0, s, v, n, S->part, nb);
-    hipLaunchKernelGGL(t_fstart, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+    hipLaunchKernelGGL(t_fstart, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
     for (int it = 0;; ++it) {
       const int even = (it % 2 == 0);
       if (even) {
         hipLaunchKernelGGL(t_k1<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-        hipLaunchKernelGGL(t_f1, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+        hipLaunchKernelGGL(t_f1, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
       }
       hipLaunchKernelGGL(t_k2<V>, G, T, 0, s, (const KState*)S->st, v, even, n, S->part, nb);
-      hipLaunchKernelGGL(t_f2, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(t_f2, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(t_k3<V>, G, T, 0, s, (const KState*)S->st, v, n);
       hipLaunchKernelGGL(t_f3, dim3(1), dim3(1), 0, s, S->st, S->trace);
       if (!even) {
         hipLaunchKernelGGL(t_k4<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-        hipLaunchKernelGGL(t_f4, dim3(1), T, 0, s, S->st, (const double2*)S->part, nb);
+        hipLaunchKernelGGL(t_f4, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
       }
       hipLaunchKernelGGL(t_k5<V>, G, T, 0, s, (const KState*)S->st, v, !even, n);
       if (!mv<V>(c, v.u, v.u, v.Au, s, done)) return false;
@@ -1075,7 +1075,7 @@ static bool krylov(ifdda_ctx* c, Solver* S, V** vv, int method, double tol, int 
 }
 
 static bool resid_norm(Solver* S, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(f_resid, dim3(1), dim3(kThreads), 0, s, S->st, (const double2*)S->part, S->nblk);
+  hipLaunchKernelGGL(f_resid, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, S->nblk);
   if (!poll(S, s)) return false;
   *out = std::sqrt(S->st_host->dots[0].x);
   return true;

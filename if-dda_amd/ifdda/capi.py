@@ -189,7 +189,7 @@ class Solver:
                          int(check_every))
 
     def inverserig(self, E_inc, method="GPBICG1", tol=1e-4, precision=64, x0_zero=False,
-                   maxit=10000, check_every=1, nstop=None):
+                   maxit=10000, check_every=0, nstop=None):
         """Solve (I - T alpha) E = E_inc.  Returns dict(FF, FFloc, tol1, nloop, ...)."""
         b = _c128(E_inc).reshape(-1)
         FF = np.zeros_like(b)
@@ -204,7 +204,7 @@ class Solver:
         return out
 
     def solve_device(self, Einc_ptr, FF_ptr, FFloc_ptr, method="GPBICG1", tol=1e-4, precision=64,
-                     x0_zero=False, maxit=10000, check_every=1, stream=None):
+                     x0_zero=False, maxit=10000, check_every=0, stream=None):
         o = self._opts(method, tol, precision, x0_zero, maxit, check_every)
         info = SolveInfo()
         stop = C.c_int(0)
