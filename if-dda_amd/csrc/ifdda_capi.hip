@@ -142,6 +142,22 @@ extern "C" int ifdda_create(int nx, int ny, int nz, ifdda_ctx** out) {
   return IFDDA_OK;
 }
 
+int ifdda::ensure_scratch(ifdda_ctx* c, int R) {
+  if (R <= c->scr_rhs) return IFDDA_OK;
+  const GridPlan& g = c->g;
+  const size_t nA = (size_t)g.nz * g.ny * g.L[0] * 3 * R;
+  const size_t nB = (size_t)g.nz * g.L[1] * g.L[0] * 3 * R;
+  CHECK_HIP(hipStreamSynchronize(c->stream), "scratch sync");
+  (void)hipFree(c->scrA);
+  (void)hipFree(c->scrB);
+  c->scrA = c->scrB = nullptr;
+  c->scr_rhs = 0;
+  CHECK_HIP(hipMalloc(&c->scrA, sizeof(double2) * nA), "hipMalloc scratch A");
+  CHECK_HIP(hipMalloc(&c->scrB, sizeof(double2) * nB), "hipMalloc scratch B");
+  c->scr_rhs = R;
+  return IFDDA_OK;
+}
+
 extern "C" int ifdda_fft_lengths(const ifdda_ctx* c, int L[3]) {
   if (!c || !L) return arg_error("ifdda_fft_lengths: NULL argument");
   for (int a = 0; a < 3; ++a) L[a] = c->g.L[a];

@@ -236,7 +236,11 @@ template <typename V> struct SArgs {
   // k_zconv only: folded G[((ky'*F3 + kz')*F1 + kx')*6 + c]
   const V* G;
   int F1, F3, L1, L2;
-  const int* skip;  // when non-null and *skip != 0 the pass does nothing
+  // right-hand side blockIdx.z: element offsets per RHS, and a per-RHS skip
+  // flag (when non-null and skip[z*skip_stride] != 0 the pass does nothing)
+  long long zs_src, zs_dst;
+  const int* skip;
+  int skip_stride;
 };
 
 __device__ __forceinline__ int fold_idx(int k, int L, int& sgn) {
@@ -283,7 +287,10 @@ __device__ __forceinline__ void load_strided(V* buf, const int* pos, const V* sr
 
 template <typename V, int NC, int SGN>
 __global__ void __launch_bounds__(256) k_strided(SArgs<V> a) {
-  if (a.skip && *a.skip) return;
+  // a stays unmodified (kernel-argument memory); per-RHS offsets go to locals
+  if (a.skip && a.skip[blockIdx.z * a.skip_stride]) return;
+  const V* const zsrc = a.src + blockIdx.z * a.zs_src;
+  V* const zdst = a.dst + blockIdx.z * a.zs_dst;
   extern __shared__ double4 smem_d4[];
   V* tw = reinterpret_cast<V*>(smem_d4);
   const int L = a.ax.L;
@@ -300,10 +307,10 @@ __global__ void __launch_bounds__(256) k_strided(SArgs<V> a) {
 
   load_tables(tw, pos, a.tw, a.pos, L, tid, nthr);
   lds_sync();
-  load_strided(buf, pos, a.src + (long long)o * a.so_src + (long long)kx0 * NC, a.sp_src, L,
+  load_strided(buf, pos, zsrc + (long long)o * a.so_src + (long long)kx0 * NC, a.sp_src, L,
                a.n_in, NL, line, nvalid, jt, JT);
   tile_fft_dit<SGN>(buf, tw, a.ax, NL, line, jt, JT);
-  V* dst = a.dst + (long long)o * a.so_dst + (long long)kx0 * NC;
+  V* dst = zdst + (long long)o * a.so_dst + (long long)kx0 * NC;
   if (line < nvalid) {
     for (int p = jt; p < a.n_out; p += JT) dst[(long long)p * a.sp_dst + line] = buf[p * NL + line];
   }
@@ -320,7 +327,9 @@ __global__ void __launch_bounds__(256) k_strided(SArgs<V> a) {
 // ---------------------------------------------------------------------------
 template <typename V, int NG>
 __global__ void __launch_bounds__(256) k_zconv(SArgs<V> a) {
-  if (a.skip && *a.skip) return;
+  if (a.skip && a.skip[blockIdx.z * a.skip_stride]) return;
+  const V* const zsrc = a.src + blockIdx.z * a.zs_src;
+  V* const zdst = a.dst + blockIdx.z * a.zs_dst;
   extern __shared__ double4 smem_d4[];
   V* tw = reinterpret_cast<V*>(smem_d4);
   const int L = a.ax.L;  // L3
@@ -342,7 +351,7 @@ __global__ void __launch_bounds__(256) k_zconv(SArgs<V> a) {
   for (int rr = 0; rr < nrow; ++rr) {
     const int ky = rr == 0 ? kyf : a.L2 - kyf;
     const bool sy = rr == 1;  // y odd-parity sign for the mirror row
-    load_strided(buf, pos, a.src + (long long)ky * a.so_src + (long long)kx0 * 3, a.sp_src, L,
+    load_strided(buf, pos, zsrc + (long long)ky * a.so_src + (long long)kx0 * 3, a.sp_src, L,
                  a.n_in, NL, line, nvalid, jt, JT);
     tile_fft_dit<+1>(buf, tw, a.ax, NL, line, jt, JT);
     // Green entries of this thread's items, all loads issued first.
@@ -389,7 +398,7 @@ __global__ void __launch_bounds__(256) k_zconv(SArgs<V> a) {
     }
     lds_sync();
     tile_fft_dif<-1>(buf, tw, a.ax, NL, line, jt, JT);
-    V* dst = a.dst + (long long)ky * a.so_dst + (long long)kx0 * 3;
+    V* dst = zdst + (long long)ky * a.so_dst + (long long)kx0 * 3;
     if (line < nvalid) {
       for (int p = jt; p < a.n_out; p += JT)
         dst[(long long)p * a.sp_dst + line] = buf[pos[p] * NL + line];
@@ -421,7 +430,11 @@ template <typename V> struct XArgs {
   int alpha_mode;
   V* xout;             // MODE 2
   decltype(V::x) inv_m;
-  const int* skip;     // when non-null and *skip != 0 the pass does nothing
+  // right-hand side blockIdx.z: element offsets per RHS (grids, dipole
+  // vectors) and a per-RHS skip flag as in SArgs
+  long long zs_src, zs_dst, zs_x;
+  const int* skip;
+  int skip_stride;
 };
 
 template <typename V>
@@ -451,7 +464,12 @@ __device__ __forceinline__ long long dip_of(const int32_t* __restrict__ tab, int
 
 template <typename V, int NC, int SGN, int MODE>
 __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
-  if (a.skip && *a.skip) return;
+  if (a.skip && a.skip[blockIdx.z * a.skip_stride]) return;
+  // a stays unmodified (kernel-argument memory); per-RHS offsets go to locals
+  const V* const zsrc = a.src ? a.src + blockIdx.z * a.zs_src : nullptr;
+  V* const zdst = a.dst ? a.dst + blockIdx.z * a.zs_dst : nullptr;
+  const V* const zxin = a.xin ? a.xin + blockIdx.z * a.zs_x : nullptr;
+  V* const zxout = a.xout ? a.xout + blockIdx.z * a.zs_x : nullptr;
   extern __shared__ double4 smem_d4[];
   V* tw = reinterpret_cast<V*>(smem_d4);
   const int L = a.ax.L;
@@ -482,7 +500,7 @@ __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
       for (int u = 0; u < U; ++u) {
         x[u][0] = x[u][1] = x[u][2] = czero<V>();
         if (dip[u] >= 0) {
-          const V* xp = a.xin + dip[u] * 3;
+          const V* xp = zxin + dip[u] * 3;
           x[u][0] = xp[0];
           x[u][1] = xp[1];
           x[u][2] = xp[2];
@@ -512,7 +530,7 @@ __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
         const int p = r / NC, c = r - p * NC;
         v[u] = czero<V>();
         if (e < a.B * per && l < nl && p < a.n_in)
-          v[u] = a.src[(long long)(g0 + l) * a.ld_src + (long long)p * NC + c];
+          v[u] = zsrc[(long long)(g0 + l) * a.ld_src + (long long)p * NC + c];
       }
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
@@ -544,7 +562,7 @@ __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
 #pragma unroll
       for (int u = 0; u < U; ++u)
         if (dip[u] >= 0) {
-          const V* bp = a.xin + dip[u] * 3;
+          const V* bp = zxin + dip[u] * 3;
           b[u][0] = bp[0];
           b[u][1] = bp[1];
           b[u][2] = bp[2];
@@ -555,7 +573,7 @@ __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
           const int e = e0 + u * nthr;
           const int l = e / a.n_out, i = e - l * a.n_out;
           const V* v = buf + i * NL + l * 3;
-          V* y = a.xout + dip[u] * 3;
+          V* y = zxout + dip[u] * 3;
           y[0] = csub(b[u][0], cscal(v[0], a.inv_m));
           y[1] = csub(b[u][1], cscal(v[1], a.inv_m));
           y[2] = csub(b[u][2], cscal(v[2], a.inv_m));
@@ -567,7 +585,7 @@ __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
       const int l = e / per, r = e - l * per;
       if (l >= nl) continue;
       const int p = r / NC, c = r - p * NC;
-      a.dst[(long long)(g0 + l) * a.ld_dst + (long long)p * NC + c] = buf[p * NL + l * NC + c];
+      zdst[(long long)(g0 + l) * a.ld_dst + (long long)p * NC + c] = buf[p * NL + l * NC + c];
     }
   }
 }
@@ -849,6 +867,9 @@ template <typename K> static bool allow_lds(K kernel, size_t bytes) {
 template <typename V>
 static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, const V* b, V* y,
                         hipStream_t s, int pi, const int* skip, PassTimer* tm) {
+  const int R = std::max(1, m.nrhs);
+  const long long zA = (long long)g.nz * g.ny * g.L[0] * 3;  // A per RHS
+  const long long zB = (long long)g.nz * g.L[1] * g.L[0] * 3;  // B per RHS
   using S = decltype(V::x);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int L1 = g.L[0], L2 = g.L[1], L3 = g.L[2];
@@ -865,16 +886,19 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.nlines = ny * nz;
     a.B = t.B;
     a.skip = skip;
+    a.skip_stride = m.skip_stride;
     a.ld_dst = l1c;
     a.tab = m.tab;
     a.nx = nx;
     a.xin = x;
+    a.zs_x = m.vstride;
+    a.zs_dst = zA;
     a.alpha = m.alpha;
     a.alpha_mode = m.alpha_mode;
     auto k = k_xpass<V, 3, +1, 1>;
     if (!allow_lds(k, t.lds)) return false;
     if (tm) tm->begin(pi * 5 + 0, s);
-    hipLaunchKernelGGL(k, dim3((a.nlines + t.B - 1) / t.B), dim3(t.threads), t.lds, s, a);
+    hipLaunchKernelGGL(k, dim3((a.nlines + t.B - 1) / t.B, 1, R), dim3(t.threads), t.lds, s, a);
     if (tm) tm->end(s);
   }
   {  // P2: FFT-y over the nz non-zero planes
@@ -890,14 +914,17 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.nbx = L1;
     a.B = t.B;
     a.skip = skip;
+    a.skip_stride = m.skip_stride;
     a.so_src = (long long)ny * l1c;
     a.sp_src = l1c;
     a.so_dst = (long long)L2 * l1c;
     a.sp_dst = l1c;
+    a.zs_src = zA;
+    a.zs_dst = zB;
     auto k = k_strided<V, 3, +1>;
     if (!allow_lds(k, t.lds)) return false;
     if (tm) tm->begin(pi * 5 + 1, s);
-    hipLaunchKernelGGL(k, dim3((L1 + t.B - 1) / t.B, nz), dim3(t.threads), t.lds, s, a);
+    hipLaunchKernelGGL(k, dim3((L1 + t.B - 1) / t.B, nz, R), dim3(t.threads), t.lds, s, a);
     if (tm) tm->end(s);
   }
   {  // P3: FFT-z . G . IFFT-z, in place, keeps kz < nz
@@ -913,8 +940,10 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.nbx = L1;
     a.B = t.B;
     a.skip = skip;
+    a.skip_stride = m.skip_stride;
     a.so_src = a.so_dst = l1c;
     a.sp_src = a.sp_dst = (long long)L2 * l1c;
+    a.zs_src = a.zs_dst = zB;
     a.G = m.G;
     a.F1 = g.F[0];
     a.F3 = g.F[2];
@@ -926,7 +955,7 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
                                                                         : k_zconv<V, kMaxG>;
     if (!allow_lds(k, t.lds)) return false;
     if (tm) tm->begin(pi * 5 + 2, s);
-    hipLaunchKernelGGL(k, dim3((L1 + t.B - 1) / t.B, L2 / 2 + 1), dim3(t.threads), t.lds, s, a);
+    hipLaunchKernelGGL(k, dim3((L1 + t.B - 1) / t.B, L2 / 2 + 1, R), dim3(t.threads), t.lds, s, a);
     if (tm) tm->end(s);
   }
   {  // P4: IFFT-y, keeps j < ny
@@ -942,14 +971,17 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.nbx = L1;
     a.B = t.B;
     a.skip = skip;
+    a.skip_stride = m.skip_stride;
     a.so_src = (long long)L2 * l1c;
     a.sp_src = l1c;
     a.so_dst = (long long)ny * l1c;
     a.sp_dst = l1c;
+    a.zs_src = zB;
+    a.zs_dst = zA;
     auto k = k_strided<V, 3, -1>;
     if (!allow_lds(k, t.lds)) return false;
     if (tm) tm->begin(pi * 5 + 3, s);
-    hipLaunchKernelGGL(k, dim3((L1 + t.B - 1) / t.B, nz), dim3(t.threads), t.lds, s, a);
+    hipLaunchKernelGGL(k, dim3((L1 + t.B - 1) / t.B, nz, R), dim3(t.threads), t.lds, s, a);
     if (tm) tm->end(s);
   }
   {  // P5: IFFT-x + gather, y = b - v/M
@@ -964,16 +996,19 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.nlines = ny * nz;
     a.B = t.B;
     a.skip = skip;
+    a.skip_stride = m.skip_stride;
     a.ld_src = l1c;
     a.tab = m.tab;
     a.nx = nx;
     a.xin = b;
     a.xout = y;
+    a.zs_x = m.vstride;
+    a.zs_src = zA;
     a.inv_m = (S)(1.0 / ((double)L1 * (double)L2 * (double)L3));
     auto k = k_xpass<V, 3, -1, 2>;
     if (!allow_lds(k, t.lds)) return false;
     if (tm) tm->begin(pi * 5 + 4, s);
-    hipLaunchKernelGGL(k, dim3((a.nlines + t.B - 1) / t.B), dim3(t.threads), t.lds, s, a);
+    hipLaunchKernelGGL(k, dim3((a.nlines + t.B - 1) / t.B, 1, R), dim3(t.threads), t.lds, s, a);
     if (tm) tm->end(s);
   }
   return hip_ok(hipGetLastError(), "matvec launch");

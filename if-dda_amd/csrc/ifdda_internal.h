@@ -39,6 +39,11 @@ template <typename V> struct MatvecDev {
   const V* alpha;     // per dipole, see alpha_mode
   int alpha_mode;
   PassTimer* timer = nullptr;
+  // batch of right-hand sides: nrhs vectors vstride elements apart, A and B
+  // sized for nrhs, skip flags skip_stride ints apart
+  int nrhs = 1;
+  long long vstride = 0;
+  int skip_stride = 0;
 };
 
 // HIP-event timing of individual matvec passes on their own stream.  Slot
@@ -109,8 +114,9 @@ struct ifdda_ctx {
   double2* al64 = nullptr;  // polarisability (scalar or 3x3 row-major per dipole)
   float2* al32 = nullptr;
   int alpha_mode = ifdda::kAlphaNone;
-  void* scrA = nullptr;  // FFT scratch sized for FP64
+  void* scrA = nullptr;  // FFT scratch sized for FP64 and scr_rhs right-hand sides
   void* scrB = nullptr;
+  int scr_rhs = 1;
   double2* hx = nullptr;  // device staging for host-array entry points
   double2* hb = nullptr;
   size_t hcap = 0;        // complex elements in hx / hb
@@ -151,4 +157,6 @@ struct ifdda_ctx {
 namespace ifdda {
 // Release the Krylov workspace (ifdda_krylov.hip).
 void solver_free(ifdda_ctx* c);
+// Grow the FFT scratch to hold R right-hand sides (ifdda_capi.hip).
+int ensure_scratch(ifdda_ctx* c, int R);
 }  // namespace ifdda

@@ -154,6 +154,11 @@ __device__ __forceinline__ void push_trace(KState* st, double2* trace, double2 a
 template <typename V>
 __global__ void __launch_bounds__(kThreads) k_init(const V* b, V* x, const V* src, long long n,
                                                   double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  part += (long long)rh_ * kMaxDots * nblk;
+  b += rh_ * n;
+  x += rh_ * n;
+  if (src) src += rh_ * n;
   double2 acc[1] = {make_double2(0.0, 0.0)};
   GRID_LOOP(i, n) {
     const double2 bi = LD(b, i);
@@ -164,6 +169,9 @@ __global__ void __launch_bounds__(kThreads) k_init(const V* b, V* x, const V* sr
 }
 
 __global__ void __launch_bounds__(kThreads) f_init(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
   finish_sum<1>(st, part, nblk);
   if (threadIdx.x == 0) st->norm = sqrt(st->dots[0].x);
 }
@@ -171,6 +179,10 @@ __global__ void __launch_bounds__(kThreads) f_init(KState* st, const double2* pa
 // ============================================================ GPBICG kernels
 template <typename V> struct GV {  // gpbicg.f:103-114 work columns
   V *x, *b, *r, *r0s, *p, *Ap, *t, *At, *y, *u, *w, *z, *told, *Ax;
+  __device__ GV shifted(long long o) const {
+    return GV{x + o, b + o, r + o, r0s + o, p + o, Ap + o, t + o, At + o, y + o, u + o, w + o,
+              z + o, told + o, Ax + o};
+  }
 };
 
 // r = b - Ax (Ax = 0 when x0 = 0); r0* = conj(r); p = t = w = u = z = 0;
@@ -178,6 +190,9 @@ template <typename V> struct GV {  // gpbicg.f:103-114 work columns
 template <typename V>
 __global__ void __launch_bounds__(kThreads) g_start(GV<V> v, int x0zero, long long n,
                                                    double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  v = v.shifted(rh_ * n);
+  part += (long long)rh_ * kMaxDots * nblk;
   double2 acc[1] = {make_double2(0.0, 0.0)};
   const double2 z0 = make_double2(0.0, 0.0);
   GRID_LOOP(i, n) {
@@ -194,6 +209,9 @@ __global__ void __launch_bounds__(kThreads) g_start(GV<V> v, int x0zero, long lo
 }
 
 __global__ void __launch_bounds__(kThreads) g_fstart(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
   finish_sum<1>(st, part, nblk);
   if (threadIdx.x == 0) {
     st->r0rn = st->dots[0];
@@ -206,6 +224,9 @@ __global__ void __launch_bounds__(kThreads) g_fstart(KState* st, const double2* 
 // p = r + beta (p - u) (gpbicg.f:161-169)
 template <typename V>
 __global__ void __launch_bounds__(kThreads) g_k1(const KState* st, GV<V> v, long long n) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(rh_ * n);
   if (st->done) return;
   const double2 beta = st->beta;
   const bool first = st->it == 0;
@@ -219,6 +240,10 @@ __global__ void __launch_bounds__(kThreads) g_k1(const KState* st, GV<V> v, long
 template <typename V>
 __global__ void __launch_bounds__(kThreads) g_k2(const KState* st, GV<V> v, long long n,
                                                 double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(rh_ * n);
+  part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   double2 acc[1] = {make_double2(0.0, 0.0)};
   GRID_LOOP(i, n) acc[0] = zadd(acc[0], zmul(LD(v.r0s, i), LD(v.Ap, i)));
@@ -226,6 +251,9 @@ __global__ void __launch_bounds__(kThreads) g_k2(const KState* st, GV<V> v, long
 }
 
 __global__ void __launch_bounds__(kThreads) g_f2(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   finish_sum<1>(st, part, nblk);
   if (threadIdx.x == 0) {
@@ -243,6 +271,9 @@ __global__ void __launch_bounds__(kThreads) g_f2(KState* st, const double2* part
 // y = t - r + alpha (Ap - w); t_old = t; t = r - alpha Ap (gpbicg.f:192-200)
 template <typename V>
 __global__ void __launch_bounds__(kThreads) g_k3(const KState* st, GV<V> v, long long n) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(rh_ * n);
   if (st->done) return;
   const double2 al = st->alpha;
   GRID_LOOP(i, n) {
@@ -257,6 +288,10 @@ __global__ void __launch_bounds__(kThreads) g_k3(const KState* st, GV<V> v, long
 template <typename V>
 __global__ void __launch_bounds__(kThreads) g_k4(const KState* st, GV<V> v, long long n,
                                                 double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(rh_ * n);
+  part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   double2 acc[5];
   for (int r = 0; r < 5; ++r) acc[r] = make_double2(0.0, 0.0);
@@ -273,6 +308,9 @@ __global__ void __launch_bounds__(kThreads) g_k4(const KState* st, GV<V> v, long
 }
 
 __global__ void __launch_bounds__(kThreads) g_f4(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   finish_sum<5>(st, part, nblk);
   if (threadIdx.x == 0) {
@@ -305,6 +343,10 @@ __global__ void __launch_bounds__(kThreads) g_f4(KState* st, const double2* part
 template <typename V>
 __global__ void __launch_bounds__(kThreads) g_k5(const KState* st, GV<V> v, long long n,
                                                 double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(rh_ * n);
+  part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   const double2 al = st->alpha, be = st->beta, et = st->eta, dz = st->dzeta;
   double2 acc[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
@@ -327,6 +369,9 @@ __global__ void __launch_bounds__(kThreads) g_k5(const KState* st, GV<V> v, long
 
 __global__ void __launch_bounds__(kThreads) g_f5(KState* st, const double2* part, int nblk,
                                                 double2* trace) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   finish_sum<2>(st, part, nblk);
   if (threadIdx.x != 0) return;
@@ -359,12 +404,18 @@ __global__ void __launch_bounds__(kThreads) g_f5(KState* st, const double2* part
 // ======================================================= QMRBICGSTAB kernels
 template <typename V> struct QV {  // qmrbicgstab.f:23-31 work columns
   V *x, *b, *r, *p, *v, *d, *rt, *s, *xt, *dt, *t, *Ax;
+  __device__ QV shifted(long long o) const {
+    return QV{x + o, b + o, r + o, p + o, v + o, d + o, rt + o, s + o, xt + o, dt + o, t + o, Ax + o};
+  }
 };
 
 // r = b - Ax; p = v = d = 0; rt = r; partial r.conj(r) (qmrbicgstab.f:50-62)
 template <typename V>
 __global__ void __launch_bounds__(kThreads) q_start(QV<V> v, int x0zero, long long n,
                                                    double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  v = v.shifted(rh_ * n);
+  part += (long long)rh_ * kMaxDots * nblk;
   double2 acc[1] = {make_double2(0.0, 0.0)};
   const double2 z0 = make_double2(0.0, 0.0);
   GRID_LOOP(i, n) {
@@ -379,6 +430,9 @@ __global__ void __launch_bounds__(kThreads) q_start(QV<V> v, int x0zero, long lo
 }
 
 __global__ void __launch_bounds__(kThreads) q_fstart(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
   finish_sum<1>(st, part, nblk);
   if (threadIdx.x == 0) {  // qmrbicgstab.f:66-75
     st->rho = zone();
@@ -395,6 +449,10 @@ __global__ void __launch_bounds__(kThreads) q_fstart(KState* st, const double2* 
 template <typename V>
 __global__ void __launch_bounds__(kThreads) q_k1(const KState* st, QV<V> v, long long n,
                                                 double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(rh_ * n);
+  part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   double2 acc[1] = {make_double2(0.0, 0.0)};
   GRID_LOOP(i, n) acc[0] = zadd(acc[0], zmul(zconj(LD(v.rt, i)), LD(v.r, i)));
@@ -402,6 +460,9 @@ __global__ void __launch_bounds__(kThreads) q_k1(const KState* st, QV<V> v, long
 }
 
 __global__ void __launch_bounds__(kThreads) q_f1(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   finish_sum<1>(st, part, nblk);
   if (threadIdx.x == 0) {
@@ -414,6 +475,9 @@ __global__ void __launch_bounds__(kThreads) q_f1(KState* st, const double2* part
 // p = r + beta (p - omega v) (qmrbicgstab.f:92-100)
 template <typename V>
 __global__ void __launch_bounds__(kThreads) q_k2(const KState* st, QV<V> v, long long n) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(rh_ * n);
   if (st->done) return;
   const double2 be = st->beta, om = st->omega;
   GRID_LOOP(i, n)
@@ -424,6 +488,10 @@ __global__ void __launch_bounds__(kThreads) q_k2(const KState* st, QV<V> v, long
 template <typename V>
 __global__ void __launch_bounds__(kThreads) q_k3(const KState* st, QV<V> v, long long n,
                                                 double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(rh_ * n);
+  part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   double2 acc[1] = {make_double2(0.0, 0.0)};
   GRID_LOOP(i, n) acc[0] = zadd(acc[0], zmul(zconj(LD(v.rt, i)), LD(v.v, i)));
@@ -431,6 +499,9 @@ __global__ void __launch_bounds__(kThreads) q_k3(const KState* st, QV<V> v, long
 }
 
 __global__ void __launch_bounds__(kThreads) q_f3(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   finish_sum<1>(st, part, nblk);
   if (threadIdx.x == 0) {
@@ -449,6 +520,10 @@ __global__ void __launch_bounds__(kThreads) q_f3(KState* st, const double2* part
 template <typename V>
 __global__ void __launch_bounds__(kThreads) q_k4(const KState* st, QV<V> v, long long n,
                                                 double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(rh_ * n);
+  part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   const double2 al = st->alpha;
   double2 acc[1] = {make_double2(0.0, 0.0)};
@@ -462,6 +537,9 @@ __global__ void __launch_bounds__(kThreads) q_k4(const KState* st, QV<V> v, long
 
 // thetat, c, taut, etat (qmrbicgstab.f:132-146)
 __global__ void __launch_bounds__(kThreads) q_f4(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   finish_sum<1>(st, part, nblk);
   if (threadIdx.x == 0) {
@@ -481,6 +559,9 @@ __global__ void __launch_bounds__(kThreads) q_f4(KState* st, const double2* part
 // dt = p + theta^2 eta/alpha d; xt = x + etat dt (qmrbicgstab.f:151-159)
 template <typename V>
 __global__ void __launch_bounds__(kThreads) q_k5(const KState* st, QV<V> v, long long n) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(rh_ * n);
   if (st->done) return;
   const double2 f = zdiv(zmul(zmul(st->theta, st->theta), st->eta), st->alpha);
   const double2 et = st->etat;
@@ -495,6 +576,10 @@ __global__ void __launch_bounds__(kThreads) q_k5(const KState* st, QV<V> v, long
 template <typename V>
 __global__ void __launch_bounds__(kThreads) q_k6(const KState* st, QV<V> v, long long n,
                                                 double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(rh_ * n);
+  part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   double2 acc[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
   const bool m1 = st->methode == 1;
@@ -512,6 +597,9 @@ __global__ void __launch_bounds__(kThreads) q_k6(const KState* st, QV<V> v, long
 }
 
 __global__ void __launch_bounds__(kThreads) q_f6(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   finish_sum<2>(st, part, nblk);
   if (threadIdx.x == 0) {
@@ -530,6 +618,10 @@ __global__ void __launch_bounds__(kThreads) q_f6(KState* st, const double2* part
 template <typename V>
 __global__ void __launch_bounds__(kThreads) q_k7(const KState* st, QV<V> v, long long n,
                                                 double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(rh_ * n);
+  part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   const double2 om = st->omega;
   double2 acc[1] = {make_double2(0.0, 0.0)};
@@ -544,6 +636,9 @@ __global__ void __launch_bounds__(kThreads) q_k7(const KState* st, QV<V> v, long
 // theta, c, tau, eta (qmrbicgstab.f:206-229); the x update of this iteration
 // still runs when the residual test below is met, so `done` is set by q_f8.
 __global__ void __launch_bounds__(kThreads) q_f7(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   finish_sum<1>(st, part, nblk);
   if (threadIdx.x == 0) {
@@ -568,6 +663,9 @@ __global__ void __launch_bounds__(kThreads) q_f7(KState* st, const double2* part
 // d = s + thetat^2 etat/omega dt; x = xt + eta d (qmrbicgstab.f:231-238)
 template <typename V>
 __global__ void __launch_bounds__(kThreads) q_k8(const KState* st, QV<V> v, long long n) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(rh_ * n);
   if (st->done) return;
   const double2 f = zdiv(zmul(zmul(st->thetat, st->thetat), st->etat), st->omega);
   const double2 et = st->eta;
@@ -579,6 +677,8 @@ __global__ void __launch_bounds__(kThreads) q_k8(const KState* st, QV<V> v, long
 
 // QMR residual bound (qmrbicgstab.f:241-251)
 __global__ void q_f8(KState* st, double2* trace) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
   if (st->done) return;
   const double res = sqrt((double)(st->it + 1)) * zabs(st->tau) / st->norm;
   st->residu = res;
@@ -600,11 +700,16 @@ __global__ void q_f8(KState* st, double2* trace) {
 // ============================================================== TFQMR kernels
 template <typename V> struct TV {  // tfqmr.f:34-41 work columns
   V *x, *b, *r, *u, *w, *v, *d, *rt, *u1, *Au, *Auo, *Ax;
+  __device__ TV shifted(long long o) const {
+    return TV{x + o, b + o, r + o, u + o, w + o, v + o, d + o, rt + o, u1 + o, Au + o, Auo + o, Ax + o};
+  }
 };
 
 // r = b - Ax; w = u = r (tfqmr.f:59-69)
 template <typename V>
 __global__ void __launch_bounds__(kThreads) t_start(TV<V> v, int x0zero, long long n) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  v = v.shifted(rh_ * n);
   GRID_LOOP(i, n) {
     double2 r = LD(v.b, i);
     if (!x0zero) r = zsub(r, LD(v.Ax, i));
@@ -619,6 +724,9 @@ __global__ void __launch_bounds__(kThreads) t_start(TV<V> v, int x0zero, long lo
 template <typename V>
 __global__ void __launch_bounds__(kThreads) t_start2(TV<V> v, long long n, double2* part,
                                                     int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  v = v.shifted(rh_ * n);
+  part += (long long)rh_ * kMaxDots * nblk;
   double2 acc[1] = {make_double2(0.0, 0.0)};
   GRID_LOOP(i, n) {
     ST(v.d, i, make_double2(0.0, 0.0));
@@ -631,6 +739,9 @@ __global__ void __launch_bounds__(kThreads) t_start2(TV<V> v, long long n, doubl
 }
 
 __global__ void __launch_bounds__(kThreads) t_fstart(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
   finish_sum<1>(st, part, nblk);
   if (threadIdx.x == 0) {  // tfqmr.f:87-96
     st->rho = st->dots[0];
@@ -640,6 +751,13 @@ __global__ void __launch_bounds__(kThreads) t_fstart(KState* st, const double2* 
     st->alpha = make_double2(0.0, 0.0);
     st->beta = make_double2(0.0, 0.0);
     st->it = 0;
+    // |b| = 0: tfqmr.f divides by NORM and never meets its test; report a
+    // breakdown at once instead (the reference does not return).
+    if (st->norm == 0.0 && !st->done) {
+      st->status = -1;
+      st->steperr = 1;
+      st->done = 1;
+    }
   }
 }
 
@@ -647,6 +765,10 @@ __global__ void __launch_bounds__(kThreads) t_fstart(KState* st, const double2* 
 template <typename V>
 __global__ void __launch_bounds__(kThreads) t_k1(const KState* st, TV<V> v, long long n,
                                                 double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(rh_ * n);
+  part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   double2 acc[1] = {make_double2(0.0, 0.0)};
   GRID_LOOP(i, n) acc[0] = zadd(acc[0], zmul(zconj(LD(v.rt, i)), LD(v.v, i)));
@@ -654,6 +776,9 @@ __global__ void __launch_bounds__(kThreads) t_k1(const KState* st, TV<V> v, long
 }
 
 __global__ void __launch_bounds__(kThreads) t_f1(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   finish_sum<1>(st, part, nblk);
   if (threadIdx.x == 0) st->alpha = zdiv(st->rho, st->dots[0]);
@@ -664,6 +789,10 @@ __global__ void __launch_bounds__(kThreads) t_f1(KState* st, const double2* part
 template <typename V>
 __global__ void __launch_bounds__(kThreads) t_k2(const KState* st, TV<V> v, int even, long long n,
                                                 double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(rh_ * n);
+  part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   const double2 al = st->alpha;
   const double2 f = zmul(zdiv(zmul(st->theta, st->theta), al), st->eta);
@@ -681,6 +810,9 @@ __global__ void __launch_bounds__(kThreads) t_k2(const KState* st, TV<V> v, int 
 
 // theta, c, tau, eta (tfqmr.f:139-144)
 __global__ void __launch_bounds__(kThreads) t_f2(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   finish_sum<1>(st, part, nblk);
   if (threadIdx.x == 0) {
@@ -694,6 +826,9 @@ __global__ void __launch_bounds__(kThreads) t_f2(KState* st, const double2* part
 // x = x + eta d (tfqmr.f:147-153)
 template <typename V>
 __global__ void __launch_bounds__(kThreads) t_k3(const KState* st, TV<V> v, long long n) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(rh_ * n);
   if (st->done) return;
   const double2 et = st->eta;
   GRID_LOOP(i, n) ST(v.x, i, zadd(LD(v.x, i), zmul(et, LD(v.d, i))));
@@ -701,6 +836,8 @@ __global__ void __launch_bounds__(kThreads) t_k3(const KState* st, TV<V> v, long
 
 // residual bound (tfqmr.f:156-160)
 __global__ void t_f3(KState* st, double2* trace) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
   if (st->done) return;
   const double res = sqrt((double)(st->it + 1)) * zabs(st->tau) / st->norm;
   st->residu = res;
@@ -715,6 +852,10 @@ __global__ void t_f3(KState* st, double2* trace) {
 template <typename V>
 __global__ void __launch_bounds__(kThreads) t_k4(const KState* st, TV<V> v, long long n,
                                                 double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(rh_ * n);
+  part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   double2 acc[1] = {make_double2(0.0, 0.0)};
   GRID_LOOP(i, n) acc[0] = zadd(acc[0], zmul(zconj(LD(v.rt, i)), LD(v.w, i)));
@@ -722,6 +863,9 @@ __global__ void __launch_bounds__(kThreads) t_k4(const KState* st, TV<V> v, long
 }
 
 __global__ void __launch_bounds__(kThreads) t_f4(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   finish_sum<1>(st, part, nblk);
   if (threadIdx.x == 0) {
@@ -734,6 +878,9 @@ __global__ void __launch_bounds__(kThreads) t_f4(KState* st, const double2* part
 // (odd) u1 = w + beta u; Auo = Au (tfqmr.f:176-183); then u = u1 (:187-194)
 template <typename V>
 __global__ void __launch_bounds__(kThreads) t_k5(const KState* st, TV<V> v, int odd, long long n) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(rh_ * n);
   if (st->done) return;
   const double2 be = st->beta;
   GRID_LOOP(i, n) {
@@ -749,6 +896,9 @@ __global__ void __launch_bounds__(kThreads) t_k5(const KState* st, TV<V> v, int 
 // (odd) v = Au + beta (Auo + beta v) (tfqmr.f:206-214)
 template <typename V>
 __global__ void __launch_bounds__(kThreads) t_k6(const KState* st, TV<V> v, long long n) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(rh_ * n);
   if (st->done) return;
   const double2 be = st->beta;
   GRID_LOOP(i, n)
@@ -757,6 +907,8 @@ __global__ void __launch_bounds__(kThreads) t_k6(const KState* st, TV<V> v, long
 
 // end of a half step: cap check, ITNO + 1 (tfqmr.f:216)
 __global__ void t_f5(KState* st) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
   if (st->done) return;
   if (st->it > st->maxit) {
     st->status = 1;
@@ -772,6 +924,10 @@ __global__ void t_f5(KState* st) {
 template <typename V>
 __global__ void __launch_bounds__(kThreads) k_resid(const V* Ax, const V* b, long long n,
                                                    double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  part += (long long)rh_ * kMaxDots * nblk;
+  Ax += rh_ * n;
+  b += rh_ * n;
   double2 acc[1] = {make_double2(0.0, 0.0)};
   GRID_LOOP(i, n) {
     const double2 e = zsub(LD(Ax, i), LD(b, i));
@@ -781,6 +937,9 @@ __global__ void __launch_bounds__(kThreads) k_resid(const V* Ax, const V* b, lon
 }
 
 __global__ void __launch_bounds__(kThreads) f_resid(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
   finish_sum<1>(st, part, nblk);
 }
 
@@ -788,6 +947,9 @@ __global__ void __launch_bounds__(kThreads) f_resid(KState* st, const double2* p
 __global__ void __launch_bounds__(kThreads) k_alpha_apply(const double2* x, double2* out,
                                                          const double2* al, int mode,
                                                          long long nd) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  x += rh_ * 3 * nd;
+  out += rh_ * 3 * nd;
   GRID_LOOP(i, nd) {
     double2 e0 = x[3 * i], e1 = x[3 * i + 1], e2 = x[3 * i + 2];
     if (mode == kAlphaScalar) {
@@ -814,6 +976,12 @@ __global__ void __launch_bounds__(kThreads) k_alpha_apply(const double2* x, doub
 __global__ void __launch_bounds__(kThreads) k_refine_resid(const double2* b, const double2* Ax,
                                                           int have_ax, double2* r64, float2* r32,
                                                           long long n, double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  part += (long long)rh_ * kMaxDots * nblk;
+  b += rh_ * n;
+  Ax += rh_ * n;
+  r64 += rh_ * n;
+  r32 += rh_ * n;
   double2 acc[1] = {make_double2(0.0, 0.0)};
   GRID_LOOP(i, n) {
     double2 r = b[i];
@@ -826,7 +994,12 @@ __global__ void __launch_bounds__(kThreads) k_refine_resid(const double2* b, con
 }
 
 // x += e (FP32 correction into the FP64 iterate)
-__global__ void __launch_bounds__(kThreads) k_refine_add(double2* x, const float2* e, long long n) {
+__global__ void __launch_bounds__(kThreads) k_refine_add(double2* x, const float2* e, long long n,
+                                                        const int* skip) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  if (skip[rh_]) return;
+  x += rh_ * n;
+  e += rh_ * n;
   GRID_LOOP(i, n) {
     const float2 v = e[i];
     x[i] = make_double2(x[i].x + (double)v.x, x[i].y + (double)v.y);
@@ -837,17 +1010,20 @@ __global__ void __launch_bounds__(kThreads) k_refine_add(double2* x, const float
 #undef ST
 
 // ------------------------------------------------------------- host driver
+// Workspace for up to R right-hand sides of n = 3*nd complex values each.
 struct Solver {
-  long long n = 0;  // 3 * dipoles
-  int nblk = 1;
+  long long n = 0;  // 3 * dipoles per RHS
+  int R = 0;        // right-hand sides the buffers hold
+  int nblk = 1;     // blocks per RHS in vector kernels
   double2* v64[14] = {};
   float2* v32[14] = {};
-  double2* part = nullptr;
-  KState* st = nullptr;
-  KState* st_host = nullptr;  // pinned
-  double2* trace = nullptr;
+  double2* part = nullptr;   // R * kMaxDots * nblk partial sums
+  KState* st = nullptr;      // R states
+  KState* st_host = nullptr; // pinned copy
+  int* act = nullptr;        // R refinement flags: 1 = converged (skip)
+  double2* trace = nullptr;  // per-iteration scalars of RHS 0 (single-RHS solves)
   int tracecap = 0;
-  double2* rr64 = nullptr;  // refinement residual (FP64)
+  double2* rr64 = nullptr;   // refinement residual (FP64), R * n
 };
 
 void solver_free(ifdda_ctx* c) {
@@ -860,38 +1036,55 @@ void solver_free(ifdda_ctx* c) {
   (void)hipFree(S->part);
   (void)hipFree(S->st);
   (void)hipHostFree(S->st_host);
+  (void)hipFree(S->act);
   (void)hipFree(S->trace);
   (void)hipFree(S->rr64);
   delete S;
   c->solver = nullptr;
 }
 
-static Solver* solver_get(ifdda_ctx* c, bool need32) {
+static Solver* solver_get(ifdda_ctx* c, int R, bool need32) {
   Solver* S = (Solver*)c->solver;
   const long long n = 3 * c->nd;
-  if (S && S->n != n) {
+  if (S && (S->n != n || S->R < R)) {
+    const int cap = S->tracecap;
     solver_free(c);
     S = nullptr;
+    if (cap) {  // keep trace recording enabled across re-allocation
+      S = new Solver();
+      S->tracecap = cap;
+      if (!hip_ok(hipMalloc(&S->trace, sizeof(double2) * 6 * cap), "hipMalloc trace")) {
+        delete S;
+        return nullptr;
+      }
+      c->solver = S;
+    }
   }
-  if (!S) {
-    S = new Solver();
-    c->solver = S;
+  if (!S || !S->st) {
+    if (!S) {
+      S = new Solver();
+      c->solver = S;
+    }
     S->n = n;
+    S->R = R;
     S->nblk = (int)std::max<long long>(1, std::min<long long>(kMaxBlocks, (n + kThreads - 1) / kThreads));
-    bool ok = hip_ok(hipMalloc(&S->part, sizeof(double2) * kMaxDots * S->nblk), "hipMalloc partials") &&
-              hip_ok(hipMalloc(&S->st, sizeof(KState)), "hipMalloc state") &&
-              hip_ok(hipHostMalloc(&S->st_host, sizeof(KState)), "hipHostMalloc state");
+    const size_t nv = (size_t)std::max<long long>(n, 1) * R;
+    bool ok = hip_ok(hipMalloc(&S->part, sizeof(double2) * kMaxDots * S->nblk * R), "hipMalloc partials") &&
+              hip_ok(hipMalloc(&S->st, sizeof(KState) * R), "hipMalloc state") &&
+              hip_ok(hipHostMalloc(&S->st_host, sizeof(KState) * R), "hipHostMalloc state") &&
+              hip_ok(hipMalloc(&S->act, sizeof(int) * R), "hipMalloc flags");
     for (int i = 0; ok && i < 14; ++i)
-      ok = hip_ok(hipMalloc(&S->v64[i], sizeof(double2) * std::max<long long>(n, 1)), "hipMalloc krylov");
+      ok = hip_ok(hipMalloc(&S->v64[i], sizeof(double2) * nv), "hipMalloc krylov");
     if (!ok) {
       solver_free(c);
       return nullptr;
     }
   }
   if (need32 && !S->v32[0]) {
-    bool ok = hip_ok(hipMalloc(&S->rr64, sizeof(double2) * std::max<long long>(n, 1)), "hipMalloc resid");
+    const size_t nv = (size_t)std::max<long long>(n, 1) * S->R;
+    bool ok = hip_ok(hipMalloc(&S->rr64, sizeof(double2) * nv), "hipMalloc resid");
     for (int i = 0; ok && i < 14; ++i)
-      ok = hip_ok(hipMalloc(&S->v32[i], sizeof(float2) * std::max<long long>(n, 1)), "hipMalloc krylov32");
+      ok = hip_ok(hipMalloc(&S->v32[i], sizeof(float2) * nv), "hipMalloc krylov32");
     if (!ok) {
       solver_free(c);
       return nullptr;
@@ -900,184 +1093,220 @@ static Solver* solver_get(ifdda_ctx* c, bool need32) {
   return S;
 }
 
+// Batched matvec over R right-hand sides (vectors n apart); skip flags
+// skip[r * skip_stride] drop converged RHS on the device.
 template <typename V> static bool mv(ifdda_ctx* c, const V* x, const V* b, V* y, hipStream_t s,
-                                     const int* skip);
+                                     const int* skip, int R, long long n, int skip_stride);
 template <> bool mv<double2>(ifdda_ctx* c, const double2* x, const double2* b, double2* y,
-                             hipStream_t s, const int* skip) {
-  return launch_matvec_f64(c->g, c->mv64(), x, b, y, s, skip);
+                             hipStream_t s, const int* skip, int R, long long n, int skip_stride) {
+  MatvecDev<double2> m = c->mv64();
+  m.nrhs = R;
+  m.vstride = n;
+  m.skip_stride = skip_stride;
+  return launch_matvec_f64(c->g, m, x, b, y, s, skip);
 }
 template <> bool mv<float2>(ifdda_ctx* c, const float2* x, const float2* b, float2* y,
-                            hipStream_t s, const int* skip) {
-  return launch_matvec_f32(c->g, c->mv32(), x, b, y, s, skip);
+                            hipStream_t s, const int* skip, int R, long long n, int skip_stride) {
+  MatvecDev<float2> m = c->mv32();
+  m.nrhs = R;
+  m.vstride = n;
+  m.skip_stride = skip_stride;
+  return launch_matvec_f32(c->g, m, x, b, y, s, skip);
 }
 
-// Outcome of one Krylov run.
+// Outcome of one Krylov run for one right-hand side.
 struct KRun {
   int status = 0, steperr = -1, nloop = 0, nmv = 0;
-  bool cancelled = false;
 };
 
-static bool poll(Solver* S, hipStream_t s) {
-  return hip_ok(hipMemcpyAsync(S->st_host, S->st, sizeof(KState), hipMemcpyDeviceToHost, s),
+static bool poll(Solver* S, int R, hipStream_t s) {
+  return hip_ok(hipMemcpyAsync(S->st_host, S->st, sizeof(KState) * R, hipMemcpyDeviceToHost, s),
                 "copy state") &&
          hip_ok(hipStreamSynchronize(s), "solver step");
 }
 
-// Runs one solve of A x = b on vectors `vv` (14 slots of type V, slot 0 = x
-// holding x0 unless x0zero, slot 1 = b).  Returns false on a HIP error.
+static bool all_done(const Solver* S, int R) {
+  for (int r = 0; r < R; ++r)
+    if (!S->st_host[r].done) return false;
+  return true;
+}
+
+// Solves A x = b for R right-hand sides at once on vectors `vv` (14 slots of
+// R*n values of type V; slot 0 = x, holding x0 unless x0zero; slot 1 = b).
+// tols[r]: tolerance per RHS; active[r] == 0 starts RHS r as finished.
+// Returns false on a HIP error; *cancelled when nstop became -1.
 template <typename V>
-static bool krylov(ifdda_ctx* c, Solver* S, V** vv, int method, double tol, int maxit,
-                   int x0zero, int check_every, volatile int* nstop, hipStream_t s, KRun* out) {
+static bool krylov(ifdda_ctx* c, Solver* S, V** vv, int R, int method, const double* tols,
+                   const char* active, int maxit, int x0zero, int check_every,
+                   volatile int* nstop, hipStream_t s, KRun* out, bool* cancelled) {
   const long long n = S->n;
   const int nb = S->nblk;
-  const dim3 G(nb), T(kThreads);
-  KState init{};
-  init.tol = tol;
-  init.maxit = maxit;
-  init.methode = method == IFDDA_QMRBICGSTAB2 ? 2 : 1;
-  init.tracecap = S->tracecap;
-  init.steperr = -1;
-  if (!hip_ok(hipMemcpyAsync(S->st, &init, sizeof(KState), hipMemcpyHostToDevice, s), "state init"))
+  const dim3 G(nb, R), T(kThreads), F(1, R), F64(64), One(1);
+  double2* trace = R == 1 ? S->trace : nullptr;
+  for (int r = 0; r < R; ++r) {
+    KState& k = S->st_host[r];
+    std::memset(&k, 0, sizeof(KState));
+    k.tol = tols[r];
+    k.maxit = maxit;
+    k.methode = method == IFDDA_QMRBICGSTAB2 ? 2 : 1;
+    k.tracecap = S->tracecap;
+    k.steperr = -1;
+    if (active && !active[r]) {
+      k.done = 1;
+      k.status = 1;
+    }
+  }
+  if (!hip_ok(hipMemcpyAsync(S->st, S->st_host, sizeof(KState) * R, hipMemcpyHostToDevice, s),
+              "state init"))
     return false;
-  int* done = &S->st->done;
-  int nmv = 0;
+  const int* done = &S->st->done;
+  const int dstride = (int)(sizeof(KState) / sizeof(int));
+  int nmv0 = 0;
   V* x = vv[0];
   V* b = vv[1];
   V* Ax = vv[13];
   hipLaunchKernelGGL(k_init<V>, G, T, 0, s, (const V*)b, x, x0zero ? (const V*)nullptr : (const V*)x,
                      n, S->part, nb);
-  hipLaunchKernelGGL(f_init, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
+  hipLaunchKernelGGL(f_init, F, F64, 0, s, S->st, (const double2*)S->part, nb);
   if (!x0zero) {
-    if (!mv<V>(c, x, x, Ax, s, nullptr)) return false;
-    ++nmv;
+    if (!mv<V>(c, x, x, Ax, s, done, R, n, dstride)) return false;
+    ++nmv0;
   }
   // Poll the device state every 8 iterations by default: work queued after
-  // the solve ended is dropped on the device (KState.done, matvec skip flag).
+  // a RHS ended is dropped on the device (KState.done, matvec skip flags).
   if (check_every <= 0) check_every = 8;
+  *cancelled = false;
+  auto step_poll = [&](int it) -> int {  // 1: stop, 0: continue, -1: error
+    if ((it + 1) % check_every == 0 || it >= maxit + 1) {
+      if (!poll(S, R, s)) return -1;
+      if (all_done(S, R)) return 1;
+      if (nstop && *nstop == -1) {
+        *cancelled = true;
+        return 1;
+      }
+    }
+    return 0;
+  };
 
   if (method == IFDDA_GPBICG1) {
     GV<V> v{x, b, vv[2], vv[3], vv[4], vv[5], vv[6], vv[7], vv[8], vv[9], vv[10], vv[11], vv[12], Ax};
     hipLaunchKernelGGL(g_start<V>, G, T, 0, s, v, x0zero, n, S->part, nb);
-    hipLaunchKernelGGL(g_fstart, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
+    hipLaunchKernelGGL(g_fstart, F, F64, 0, s, S->st, (const double2*)S->part, nb);
     for (int it = 0;; ++it) {
       hipLaunchKernelGGL(g_k1<V>, G, T, 0, s, (const KState*)S->st, v, n);
-      if (!mv<V>(c, v.p, v.p, v.Ap, s, done)) return false;
+      if (!mv<V>(c, v.p, v.p, v.Ap, s, done, R, n, dstride)) return false;
       hipLaunchKernelGGL(g_k2<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(g_f2, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(g_f2, F, F64, 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(g_k3<V>, G, T, 0, s, (const KState*)S->st, v, n);
-      if (!mv<V>(c, v.t, v.t, v.At, s, done)) return false;
+      if (!mv<V>(c, v.t, v.t, v.At, s, done, R, n, dstride)) return false;
       hipLaunchKernelGGL(g_k4<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(g_f4, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(g_f4, F, F64, 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(g_k5<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(g_f5, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb, S->trace);
+      hipLaunchKernelGGL(g_f5, F, F64, 0, s, S->st, (const double2*)S->part, nb, trace);
       if (!hip_ok(hipGetLastError(), "gpbicg launch")) return false;
-      if ((it + 1) % check_every == 0 || it >= maxit + 1) {
-        if (!poll(S, s)) return false;
-        if (S->st_host->done) break;
-        if (nstop && *nstop == -1) {
-          out->cancelled = true;
-          break;
-        }
-      }
+      const int st = step_poll(it);
+      if (st < 0) return false;
+      if (st > 0) break;
     }
-    const KState& h = *S->st_host;
-    out->status = h.status;
-    out->steperr = h.steperr;
-    out->nloop = h.it + 1;  // ITNO starts at -1 (gpbicg.f:156), the driver adds one
-    // two matvecs per finished iteration; a breakdown at STEPERR 1 stops
-    // after the first
-    nmv += 2 * h.it + (h.status == -1 && h.steperr == 1 ? 1 : 2);
+    for (int r = 0; r < R; ++r) {
+      const KState& h = S->st_host[r];
+      KRun& o = out[r];
+      o.status = h.status;
+      o.steperr = h.steperr;
+      if (active && !active[r]) continue;
+      o.nloop = h.it + 1;  // ITNO starts at -1 (gpbicg.f:156), the driver adds one
+      // two matvecs per finished iteration; a breakdown at STEPERR 1 stops
+      // after the first
+      o.nmv = nmv0 + 2 * h.it + (h.status == -1 && h.steperr == 1 ? 1 : 2);
+    }
   } else if (method == IFDDA_QMRBICGSTAB1 || method == IFDDA_QMRBICGSTAB2) {
     QV<V> v{x, b, vv[2], vv[3], vv[4], vv[5], vv[6], vv[7], vv[8], vv[9], vv[10], Ax};
     hipLaunchKernelGGL(q_start<V>, G, T, 0, s, v, x0zero, n, S->part, nb);
-    hipLaunchKernelGGL(q_fstart, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
+    hipLaunchKernelGGL(q_fstart, F, F64, 0, s, S->st, (const double2*)S->part, nb);
     for (int it = 0;; ++it) {
       hipLaunchKernelGGL(q_k1<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(q_f1, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(q_f1, F, F64, 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(q_k2<V>, G, T, 0, s, (const KState*)S->st, v, n);
-      if (!mv<V>(c, v.p, v.p, v.v, s, done)) return false;
+      if (!mv<V>(c, v.p, v.p, v.v, s, done, R, n, dstride)) return false;
       hipLaunchKernelGGL(q_k3<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(q_f3, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(q_f3, F, F64, 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(q_k4<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(q_f4, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(q_f4, F, F64, 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(q_k5<V>, G, T, 0, s, (const KState*)S->st, v, n);
-      if (!mv<V>(c, v.s, v.s, v.t, s, done)) return false;
+      if (!mv<V>(c, v.s, v.s, v.t, s, done, R, n, dstride)) return false;
       hipLaunchKernelGGL(q_k6<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(q_f6, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(q_f6, F, F64, 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(q_k7<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(q_f7, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(q_f7, F, F64, 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(q_k8<V>, G, T, 0, s, (const KState*)S->st, v, n);
-      hipLaunchKernelGGL(q_f8, dim3(1), dim3(1), 0, s, S->st, S->trace);
+      hipLaunchKernelGGL(q_f8, F, One, 0, s, S->st, trace);
       if (!hip_ok(hipGetLastError(), "qmrbicgstab launch")) return false;
-      if ((it + 1) % check_every == 0 || it >= maxit + 1) {
-        if (!poll(S, s)) return false;
-        if (S->st_host->done) break;
-        if (nstop && *nstop == -1) {
-          out->cancelled = true;
-          break;
-        }
-      }
+      const int st = step_poll(it);
+      if (st < 0) return false;
+      if (st > 0) break;
     }
-    const KState& h = *S->st_host;
-    out->status = h.status;
-    out->steperr = h.steperr;
-    out->nloop = h.it;  // ITNO counts from 1 (qmrbicgstab.f:75); no driver increment
-    const int full = h.it - 1;
-    int last = 2;
-    if (h.status == -1 && (h.steperr == 1 || h.steperr == 2)) last = 1;
-    nmv += 2 * full + last;
+    for (int r = 0; r < R; ++r) {
+      const KState& h = S->st_host[r];
+      KRun& o = out[r];
+      o.status = h.status;
+      o.steperr = h.steperr;
+      if (active && !active[r]) continue;
+      o.nloop = h.it;  // ITNO counts from 1 (qmrbicgstab.f:75); no driver increment
+      int last = 2;
+      if (h.status == -1 && (h.steperr == 1 || h.steperr == 2)) last = 1;
+      o.nmv = nmv0 + 2 * (h.it - 1) + last;
+    }
   } else if (method == IFDDA_TFQMR) {
     TV<V> v{x, b, vv[2], vv[3], vv[4], vv[5], vv[6], vv[7], vv[8], vv[9], vv[10], Ax};
     hipLaunchKernelGGL(t_start<V>, G, T, 0, s, v, x0zero, n);
-    if (!mv<V>(c, v.u, v.u, v.v, s, nullptr)) return false;
-    ++nmv;
+    if (!mv<V>(c, v.u, v.u, v.v, s, done, R, n, dstride)) return false;
     hipLaunchKernelGGL(t_start2<V>, G, T, 0, s, v, n, S->part, nb);
-    hipLaunchKernelGGL(t_fstart, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
+    hipLaunchKernelGGL(t_fstart, F, F64, 0, s, S->st, (const double2*)S->part, nb);
     for (int it = 0;; ++it) {
       const int even = (it % 2 == 0);
       if (even) {
         hipLaunchKernelGGL(t_k1<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-        hipLaunchKernelGGL(t_f1, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
+        hipLaunchKernelGGL(t_f1, F, F64, 0, s, S->st, (const double2*)S->part, nb);
       }
       hipLaunchKernelGGL(t_k2<V>, G, T, 0, s, (const KState*)S->st, v, even, n, S->part, nb);
-      hipLaunchKernelGGL(t_f2, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(t_f2, F, F64, 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(t_k3<V>, G, T, 0, s, (const KState*)S->st, v, n);
-      hipLaunchKernelGGL(t_f3, dim3(1), dim3(1), 0, s, S->st, S->trace);
+      hipLaunchKernelGGL(t_f3, F, One, 0, s, S->st, trace);
       if (!even) {
         hipLaunchKernelGGL(t_k4<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-        hipLaunchKernelGGL(t_f4, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, nb);
+        hipLaunchKernelGGL(t_f4, F, F64, 0, s, S->st, (const double2*)S->part, nb);
       }
       hipLaunchKernelGGL(t_k5<V>, G, T, 0, s, (const KState*)S->st, v, !even, n);
-      if (!mv<V>(c, v.u, v.u, v.Au, s, done)) return false;
+      if (!mv<V>(c, v.u, v.u, v.Au, s, done, R, n, dstride)) return false;
       if (!even) hipLaunchKernelGGL(t_k6<V>, G, T, 0, s, (const KState*)S->st, v, n);
-      hipLaunchKernelGGL(t_f5, dim3(1), dim3(1), 0, s, S->st);
+      hipLaunchKernelGGL(t_f5, F, One, 0, s, S->st);
       if (!hip_ok(hipGetLastError(), "tfqmr launch")) return false;
-      if ((it + 1) % check_every == 0 || it >= maxit + 1) {
-        if (!poll(S, s)) return false;
-        if (S->st_host->done) break;
-        if (nstop && *nstop == -1) {
-          out->cancelled = true;
-          break;
-        }
-      }
+      const int st = step_poll(it);
+      if (st < 0) return false;
+      if (st > 0) break;
     }
-    const KState& h = *S->st_host;
-    out->status = h.status;
-    out->steperr = h.steperr;
-    out->nloop = h.it + 1;  // ITNO starts at -1 (tfqmr.f:96), the driver adds one
-    nmv += h.it;            // one matvec per half step that did not converge
+    for (int r = 0; r < R; ++r) {
+      const KState& h = S->st_host[r];
+      KRun& o = out[r];
+      o.status = h.status;
+      o.steperr = h.steperr;
+      if (active && !active[r]) continue;
+      o.nloop = h.it + 1;        // ITNO starts at -1 (tfqmr.f:96), the driver adds one
+      o.nmv = nmv0 + 1 + h.it;   // A u0, then one matvec per half step that did not converge
+    }
   } else {
     set_error("unknown method");
     return false;
   }
-  out->nmv = nmv;
   return true;
 }
 
-static bool resid_norm(Solver* S, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(f_resid, dim3(1), dim3(64), 0, s, S->st, (const double2*)S->part, S->nblk);
-  if (!poll(S, s)) return false;
-  *out = std::sqrt(S->st_host->dots[0].x);
+// Norms of R residual vectors whose partial sums k_resid / k_init /
+// k_refine_resid left in S->part.
+static bool resid_norms(Solver* S, int R, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(f_resid, dim3(1, R), dim3(64), 0, s, S->st, (const double2*)S->part, S->nblk);
+  if (!poll(S, R, s)) return false;
+  for (int r = 0; r < R; ++r) out[r] = std::sqrt(S->st_host[r].dots[0].x);
   return true;
 }
 
@@ -1086,13 +1315,16 @@ static void set_info_str(ifdda_solve_info* info, const char* m) {
   std::memcpy(info->infostr, m, std::min(std::strlen(m), sizeof(info->infostr)));
 }
 
-// Device solve: E_inc, FF, FFloc are device complex128 arrays (any may alias
-// the workspace-independent caller buffers).
-static int solve_dev(ifdda_ctx* c, const ifdda_solve_opts* o, const double2* Einc, double2* FF,
-                     double2* FFloc, ifdda_solve_info* info, volatile int* nstop, hipStream_t s) {
+// Device solve of R right-hand sides: E_inc, FF, FFloc are device complex128
+// arrays of R * 3*nd values (FF / FFloc may be null).
+static int solve_dev(ifdda_ctx* c, const ifdda_solve_opts* o, int R, const double2* Einc,
+                     double2* FF, double2* FFloc, ifdda_solve_info* infos, volatile int* nstop,
+                     hipStream_t s) {
   const auto t0 = std::chrono::steady_clock::now();
-  std::memset(info, 0, sizeof(*info));
-  set_info_str(info, "");
+  for (int r = 0; r < R; ++r) {
+    std::memset(&infos[r], 0, sizeof(ifdda_solve_info));
+    set_info_str(&infos[r], "");
+  }
   if (o->method < IFDDA_GPBICG1 || o->method > IFDDA_TFQMR) {
     set_error("ifdda_solve: unknown method");
     return IFDDA_ERR_ARG;
@@ -1101,125 +1333,165 @@ static int solve_dev(ifdda_ctx* c, const ifdda_solve_opts* o, const double2* Ein
     set_error("ifdda_solve: precision must be 64 or 32");
     return IFDDA_ERR_ARG;
   }
+  if (R < 1) {
+    set_error("ifdda_solve: nrhs must be >= 1");
+    return IFDDA_ERR_ARG;
+  }
   if (!c->green_ready || !c->have_dipoles || c->alpha_mode == kAlphaNone) {
     set_error("ifdda_solve: Green tensor, dipoles and polarisability must be set");
     return IFDDA_ERR_STATE;
   }
+  if (ensure_scratch(c, R) != IFDDA_OK) return IFDDA_ERR_HIP;
   const int maxit = o->maxit > 0 ? o->maxit : 10000;
-  Solver* S = solver_get(c, o->precision == 32);
+  Solver* S = solver_get(c, R, o->precision == 32);
   if (!S) return IFDDA_ERR_HIP;
   const long long n = S->n;
   const int nb = S->nblk;
-  const dim3 G(nb), T(kThreads);
+  const dim3 G(nb, R), T(kThreads);
   double2* x = S->v64[0];
   double2* b = S->v64[1];
   double2* Ax = S->v64[13];
-  if (!hip_ok(hipMemcpyAsync(b, Einc, sizeof(double2) * n, hipMemcpyDeviceToDevice, s), "copy E_inc"))
+  if (!hip_ok(hipMemcpyAsync(b, Einc, sizeof(double2) * n * R, hipMemcpyDeviceToDevice, s),
+              "copy E_inc"))
     return IFDDA_ERR_HIP;
   if (!o->x0_zero &&
-      !hip_ok(hipMemcpyAsync(x, Einc, sizeof(double2) * n, hipMemcpyDeviceToDevice, s), "copy x0"))
+      !hip_ok(hipMemcpyAsync(x, Einc, sizeof(double2) * n * R, hipMemcpyDeviceToDevice, s),
+              "copy x0"))
     return IFDDA_ERR_HIP;
   if (nstop && *nstop == -1) {
-    info->nstop = -1;
-    set_info_str(info, "Calculation cancelled during iterative method");
+    for (int r = 0; r < R; ++r) {
+      infos[r].nstop = -1;
+      set_info_str(&infos[r], "Calculation cancelled during iterative method");
+    }
     return IFDDA_OK;
   }
-  KRun run;
-  int ncompte = 0;
+  std::vector<KRun> run(R);
+  std::vector<int> ncompte(R, 0), nloop(R, 0), outer(R, 0);
+  bool cancelled = false;
   if (o->precision == 64) {
-    if (!krylov<double2>(c, S, S->v64, o->method, o->tol, maxit, o->x0_zero, o->check_every,
-                         nstop, s, &run))
+    std::vector<double> tols(R, o->tol);
+    if (!krylov<double2>(c, S, S->v64, R, o->method, tols.data(), nullptr, maxit, o->x0_zero,
+                         o->check_every, nstop, s, run.data(), &cancelled))
       return IFDDA_ERR_HIP;
-    ncompte = run.nmv;
-    info->status = run.status;
-    info->steperr = run.steperr;
-    info->nloop = run.nloop;
+    for (int r = 0; r < R; ++r) {
+      ncompte[r] = run[r].nmv;
+      nloop[r] = run[r].nloop;
+    }
   } else {
-    // Iterative refinement with FP32 inner solves.
+    // Iterative refinement with FP32 inner solves, per RHS.
     double2* r64 = S->rr64;
     float2** v32 = S->v32;
-    double bnorm = 0.0;
+    std::vector<double> bnorm(R), rn(R), rho(R), tin(R);
+    std::vector<char> active(R, 1);
+    std::vector<int> act(R, 0);
     hipLaunchKernelGGL(k_init<double2>, G, T, 0, s, (const double2*)b, x,
                        o->x0_zero ? (const double2*)nullptr : (const double2*)x, n, S->part, nb);
-    if (!resid_norm(S, &bnorm, s)) return IFDDA_ERR_HIP;
+    if (!resid_norms(S, R, bnorm.data(), s)) return IFDDA_ERR_HIP;
     bool have_ax = !o->x0_zero;
-    int total_it = 0;
-    for (int outer = 0;; ++outer) {
+    if (!hip_ok(hipMemcpyAsync(S->act, act.data(), sizeof(int) * R, hipMemcpyHostToDevice, s),
+                "copy flags"))
+      return IFDDA_ERR_HIP;
+    for (int sweep = 0;; ++sweep) {
       if (have_ax) {
-        if (!mv<double2>(c, x, x, Ax, s, nullptr)) return IFDDA_ERR_HIP;
-        ++ncompte;
+        if (!mv<double2>(c, x, x, Ax, s, S->act, R, n, 1)) return IFDDA_ERR_HIP;
+        for (int r = 0; r < R; ++r) ncompte[r] += active[r] ? 1 : 0;
       }
       hipLaunchKernelGGL(k_refine_resid, G, T, 0, s, (const double2*)b, (const double2*)Ax,
                          (int)have_ax, r64, v32[1], n, S->part, nb);
-      double rn = 0.0;
-      if (!resid_norm(S, &rn, s)) return IFDDA_ERR_HIP;
+      if (!resid_norms(S, R, rn.data(), s)) return IFDDA_ERR_HIP;
       have_ax = true;
-      const double rho = bnorm > 0.0 ? rn / bnorm : 0.0;
-      info->outer = outer;
-      if (rho <= o->tol || outer >= 30) {
-        info->status = 1;
-        info->steperr = rho <= o->tol ? -1 : 0;
-        break;
+      int nact = 0;
+      for (int r = 0; r < R; ++r) {
+        if (!active[r]) continue;
+        rho[r] = bnorm[r] > 0.0 ? rn[r] / bnorm[r] : 0.0;
+        outer[r] = sweep;
+        if (rho[r] <= o->tol || sweep >= 30) {
+          active[r] = 0;
+          act[r] = 1;
+          run[r].status = 1;
+          run[r].steperr = rho[r] <= o->tol ? -1 : 0;
+        } else {
+          ++nact;
+          // Inner relative tolerance: 0.9 of the remaining gap (the FP32
+          // recurrence residual tracks the FP64 true residual to ~1e-6), not
+          // below what an FP32 recurrence resolves.
+          tin[r] = std::max(0.9 * o->tol / rho[r], 1e-5);
+        }
       }
-      if (nstop && *nstop == -1) {
-        run.cancelled = true;
-        break;
-      }
-      // Inner relative tolerance: 0.9 of the remaining gap (the FP32 recurrence
-      // residual tracks the FP64 true residual to ~1e-6), not below what an
-      // FP32 recurrence resolves.
-      const double tin = std::max(0.9 * o->tol / rho, 1e-5);
-      KRun in;
-      if (!krylov<float2>(c, S, v32, o->method, tin, maxit, 1, o->check_every, nstop, s, &in))
+      if (!hip_ok(hipMemcpyAsync(S->act, act.data(), sizeof(int) * R, hipMemcpyHostToDevice, s),
+                  "copy flags"))
         return IFDDA_ERR_HIP;
-      ncompte += in.nmv;
-      total_it += in.nloop;
-      if (in.cancelled) {
-        run.cancelled = true;
+      if (nact == 0) break;
+      if (nstop && *nstop == -1) {
+        cancelled = true;
         break;
       }
-      if (in.status < 0) {
-        info->status = in.status;
-        info->steperr = in.steperr;
-        break;
+      std::vector<KRun> in(R);
+      if (!krylov<float2>(c, S, v32, R, o->method, tin.data(), active.data(), maxit, 1,
+                          o->check_every, nstop, s, in.data(), &cancelled))
+        return IFDDA_ERR_HIP;
+      bool broke = false;
+      for (int r = 0; r < R; ++r) {
+        if (!active[r]) continue;
+        ncompte[r] += in[r].nmv;
+        nloop[r] += in[r].nloop;
+        if (in[r].status < 0) {  // breakdown in the inner solve ends this RHS
+          run[r] = in[r];
+          active[r] = 0;
+          act[r] = 1;
+          broke = true;
+        }
       }
-      hipLaunchKernelGGL(k_refine_add, G, T, 0, s, x, (const float2*)v32[0], n);
+      if (cancelled) break;
+      if (broke &&
+          !hip_ok(hipMemcpyAsync(S->act, act.data(), sizeof(int) * R, hipMemcpyHostToDevice, s),
+                  "copy flags"))
+        return IFDDA_ERR_HIP;
+      hipLaunchKernelGGL(k_refine_add, G, T, 0, s, x, (const float2*)v32[0], n, (const int*)S->act);
     }
-    info->nloop = total_it;
   }
-  if (run.cancelled) {
-    info->nstop = -1;
-    info->ncompte = ncompte;
-    set_info_str(info, "Calculation cancelled during iterative method");
-    return IFDDA_OK;
-  }
-  if (info->status < 0) {  // inverserig.f:47-52
-    info->nstop = 1;
-    info->ncompte = ncompte;
-    set_info_str(info, "Problem to solve Ax=b!");
+  if (cancelled) {
+    for (int r = 0; r < R; ++r) {
+      infos[r].nstop = -1;
+      infos[r].ncompte = ncompte[r];
+      set_info_str(&infos[r], "Calculation cancelled during iterative method");
+    }
     return IFDDA_OK;
   }
   // Final true residual with the FP64 matvec (inverserig.f:95-105).
-  if (!mv<double2>(c, x, x, Ax, s, nullptr)) return IFDDA_ERR_HIP;
-  ++ncompte;
+  if (!mv<double2>(c, x, x, Ax, s, nullptr, R, n, 0)) return IFDDA_ERR_HIP;
   hipLaunchKernelGGL(k_resid<double2>, G, T, 0, s, (const double2*)Ax, (const double2*)b, n,
                      S->part, nb);
-  double e = 0.0, bn = 0.0;
-  if (!resid_norm(S, &e, s)) return IFDDA_ERR_HIP;
+  std::vector<double> e(R), bn(R);
+  if (!resid_norms(S, R, e.data(), s)) return IFDDA_ERR_HIP;
   hipLaunchKernelGGL(k_init<double2>, G, T, 0, s, (const double2*)b, Ax, (const double2*)Ax, n,
                      S->part, nb);
-  if (!resid_norm(S, &bn, s)) return IFDDA_ERR_HIP;
-  info->tol1 = bn > 0.0 ? e / bn : 0.0;
-  info->ncompte = ncompte;
+  if (!resid_norms(S, R, bn.data(), s)) return IFDDA_ERR_HIP;
   if (FFloc &&
-      !hip_ok(hipMemcpyAsync(FFloc, x, sizeof(double2) * n, hipMemcpyDeviceToDevice, s), "copy E"))
+      !hip_ok(hipMemcpyAsync(FFloc, x, sizeof(double2) * n * R, hipMemcpyDeviceToDevice, s),
+              "copy E"))
     return IFDDA_ERR_HIP;
   if (FF) {
     hipLaunchKernelGGL(k_alpha_apply, G, T, 0, s, (const double2*)x, FF, (const double2*)c->al64,
                        c->alpha_mode, c->nd);
   }
   if (!hip_ok(hipStreamSynchronize(s), "solve")) return IFDDA_ERR_HIP;
-  info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int r = 0; r < R; ++r) {
+    ifdda_solve_info& inf = infos[r];
+    inf.status = run[r].status;
+    inf.steperr = run[r].steperr;
+    inf.nloop = nloop[r];
+    inf.ncompte = ncompte[r] + 1;
+    inf.outer = o->precision == 32 ? outer[r] : 0;
+    inf.seconds = secs;
+    inf.tol1 = bn[r] > 0.0 ? e[r] / bn[r] : 0.0;
+    if (run[r].status < 0) {  // inverserig.f:47-52
+      inf.nstop = 1;
+      inf.ncompte = ncompte[r];
+      set_info_str(&inf, "Problem to solve Ax=b!");
+    }
+  }
   return IFDDA_OK;
 }
 
@@ -1227,31 +1499,42 @@ static int solve_dev(ifdda_ctx* c, const ifdda_solve_opts* o, const double2* Ein
 
 using namespace ifdda;
 
-extern "C" int ifdda_solve_device(ifdda_ctx* c, const ifdda_solve_opts* o, const void* E_inc,
-                                  void* FF, void* FFloc, ifdda_solve_info* info,
-                                  volatile int* nstop, void* stream) {
-  if (!c || !o || !E_inc || !info) {
-    set_error("ifdda_solve_device: NULL argument");
+extern "C" int ifdda_solve_multi_device(ifdda_ctx* c, const ifdda_solve_opts* o, int nrhs,
+                                        const void* E_inc, void* FF, void* FFloc,
+                                        ifdda_solve_info* infos, volatile int* nstop,
+                                        void* stream) {
+  if (!c || !o || !E_inc || !infos) {
+    set_error("ifdda_solve_multi_device: NULL argument");
     return IFDDA_ERR_ARG;
   }
-  return solve_dev(c, o, (const double2*)E_inc, (double2*)FF, (double2*)FFloc, info, nstop,
+  return solve_dev(c, o, nrhs, (const double2*)E_inc, (double2*)FF, (double2*)FFloc, infos, nstop,
                    stream ? (hipStream_t)stream : c->stream);
 }
 
-extern "C" int ifdda_solve(ifdda_ctx* c, const ifdda_solve_opts* o, const double* E_inc,
-                           double* FF, double* FFloc, ifdda_solve_info* info,
-                           volatile int* nstop) {
-  if (!c || !o || !E_inc || !info) {
-    set_error("ifdda_solve: NULL argument");
+extern "C" int ifdda_solve_device(ifdda_ctx* c, const ifdda_solve_opts* o, const void* E_inc,
+                                  void* FF, void* FFloc, ifdda_solve_info* info,
+                                  volatile int* nstop, void* stream) {
+  return ifdda_solve_multi_device(c, o, 1, E_inc, FF, FFloc, info, nstop, stream);
+}
+
+extern "C" int ifdda_solve_multi(ifdda_ctx* c, const ifdda_solve_opts* o, int nrhs,
+                                 const double* E_inc, double* FF, double* FFloc,
+                                 ifdda_solve_info* infos, volatile int* nstop) {
+  if (!c || !o || !E_inc || !infos) {
+    set_error("ifdda_solve_multi: NULL argument");
     return IFDDA_ERR_ARG;
   }
   if (!c->have_dipoles) {
     set_error("ifdda_solve: no dipoles");
     return IFDDA_ERR_STATE;
   }
-  const size_t n = 3 * (size_t)c->nd;
+  if (nrhs < 1) {
+    set_error("ifdda_solve_multi: nrhs must be >= 1");
+    return IFDDA_ERR_ARG;
+  }
+  const size_t n = 3 * (size_t)c->nd * nrhs;
   if (n == 0) {
-    std::memset(info, 0, sizeof(*info));
+    for (int r = 0; r < nrhs; ++r) std::memset(&infos[r], 0, sizeof(ifdda_solve_info));
     return IFDDA_OK;
   }
   double2 *dE = nullptr, *dFF = nullptr, *dL = nullptr;
@@ -1261,8 +1544,8 @@ extern "C" int ifdda_solve(ifdda_ctx* c, const ifdda_solve_opts* o, const double
       !hip_ok(hipMalloc(&dL, sizeof(double2) * n), "hipMalloc FFloc") ||
       !hip_ok(hipMemcpy(dE, E_inc, sizeof(double2) * n, hipMemcpyHostToDevice), "copy E_inc"))
     rc = IFDDA_ERR_HIP;
-  if (!rc) rc = solve_dev(c, o, dE, dFF, dL, info, nstop, c->stream);
-  if (!rc && info->nstop == 0) {
+  if (!rc) rc = solve_dev(c, o, nrhs, dE, dFF, dL, infos, nstop, c->stream);
+  if (!rc) {
     if (FF && !hip_ok(hipMemcpy(FF, dFF, sizeof(double2) * n, hipMemcpyDeviceToHost), "copy FF"))
       rc = IFDDA_ERR_HIP;
     if (FFloc &&
@@ -1275,11 +1558,17 @@ extern "C" int ifdda_solve(ifdda_ctx* c, const ifdda_solve_opts* o, const double
   return rc;
 }
 
+extern "C" int ifdda_solve(ifdda_ctx* c, const ifdda_solve_opts* o, const double* E_inc,
+                           double* FF, double* FFloc, ifdda_solve_info* info,
+                           volatile int* nstop) {
+  return ifdda_solve_multi(c, o, 1, E_inc, FF, FFloc, info, nstop);
+}
+
 // Trace access for parity tests: copies up to cap rows of
-// (alpha, beta, s2, s3, s4, residual) recorded by the last solve.
+// (alpha, beta, s2, s3, s4, residual) recorded by the last single-RHS solve.
 extern "C" int ifdda_solve_trace(ifdda_ctx* c, int cap, double* out, int* nrows) {
   Solver* S = c ? (Solver*)c->solver : nullptr;
-  if (!S || !out || !nrows) {
+  if (!S || !S->st || !out || !nrows) {
     set_error("ifdda_solve_trace: no solver state");
     return IFDDA_ERR_STATE;
   }
@@ -1293,14 +1582,17 @@ extern "C" int ifdda_solve_trace(ifdda_ctx* c, int cap, double* out, int* nrows)
   return IFDDA_OK;
 }
 
-// Enable per-iteration scalar recording for the next solves (cap rows).
+// Enable per-iteration scalar recording for the next single-RHS solves.
 extern "C" int ifdda_solve_trace_enable(ifdda_ctx* c, int cap) {
   if (!c || cap < 0) {
     set_error("ifdda_solve_trace_enable: bad argument");
     return IFDDA_ERR_ARG;
   }
-  Solver* S = solver_get(c, false);
-  if (!S) return IFDDA_ERR_HIP;
+  Solver* S = (Solver*)c->solver;
+  if (!S) {
+    S = new Solver();
+    c->solver = S;
+  }
   (void)hipFree(S->trace);
   S->trace = nullptr;
   S->tracecap = 0;

@@ -71,6 +71,10 @@ def lib():
                               C.POINTER(C.c_int)]
     L.ifdda_solve_device.argtypes = [vp, C.POINTER(SolveOpts), vp, vp, vp, C.POINTER(SolveInfo),
                                      C.POINTER(C.c_int), vp]
+    L.ifdda_solve_multi.argtypes = [vp, C.POINTER(SolveOpts), ip, vp, vp, vp, C.POINTER(SolveInfo),
+                                    C.POINTER(C.c_int)]
+    L.ifdda_solve_multi_device.argtypes = [vp, C.POINTER(SolveOpts), ip, vp, vp, vp,
+                                           C.POINTER(SolveInfo), C.POINTER(C.c_int), vp]
     L.ifdda_solve_trace.argtypes = [vp, ip, vp, C.POINTER(C.c_int)]
     L.ifdda_solve_trace_enable.argtypes = [vp, ip]
     L.ifdda_timing_enable.argtypes = [vp, ip]
@@ -202,6 +206,33 @@ class Solver:
         out = info.as_dict()
         out.update(FF=FF, FFloc=FFloc)
         return out
+
+    def inverserig_multi(self, E_inc, method="GPBICG1", tol=1e-4, precision=64, x0_zero=False,
+                         maxit=10000, check_every=0):
+        """R incidences at once: E_inc complex128 [R, 3*nd].  Returns (FF, FFloc, infos)."""
+        b = _c128(E_inc)
+        R = b.shape[0]
+        FF = np.zeros_like(b)
+        FFloc = np.zeros_like(b)
+        o = self._opts(method, tol, precision, x0_zero, maxit, check_every)
+        infos = (SolveInfo * R)()
+        stop = C.c_int(0)
+        _check(lib().ifdda_solve_multi(self.h, C.byref(o), R, _p(b), _p(FF), _p(FFloc), infos,
+                                       C.byref(stop)), "ifdda_solve_multi")
+        return FF, FFloc, [i.as_dict() for i in infos]
+
+    def solve_multi_device(self, R, Einc_ptr, FF_ptr, FFloc_ptr, method="GPBICG1", tol=1e-4,
+                           precision=64, x0_zero=False, maxit=10000, check_every=0, stream=None):
+        o = self._opts(method, tol, precision, x0_zero, maxit, check_every)
+        infos = (SolveInfo * R)()
+        stop = C.c_int(0)
+        _check(lib().ifdda_solve_multi_device(self.h, C.byref(o), int(R), C.c_void_p(Einc_ptr),
+                                              C.c_void_p(FF_ptr) if FF_ptr else None,
+                                              C.c_void_p(FFloc_ptr) if FFloc_ptr else None,
+                                              infos, C.byref(stop),
+                                              C.c_void_p(stream) if stream else None),
+               "ifdda_solve_multi_device")
+        return [i.as_dict() for i in infos]
 
     def solve_device(self, Einc_ptr, FF_ptr, FFloc_ptr, method="GPBICG1", tol=1e-4, precision=64,
                      x0_zero=False, maxit=10000, check_every=0, stream=None):

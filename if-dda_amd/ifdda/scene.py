@@ -153,3 +153,49 @@ def config2(n: int = 100, seed: int = 2):
     alpha = polarizability(s.eps, s.d, k0, "RR")
     E = circular_wave(s.pos, k0, 30.0, 0.0, 1.0)
     return s, k0, alpha, E
+
+
+def random_spheres(nnnr: int, box: float, nsph: int, radius: float, eps: complex, seed: int) -> Scene:
+    """nsph non-overlapping spheres (seeded uniform centres, rejection) on an
+    nnnr^3 lattice filling a cube of side `box`; cell centres as
+    objectcube.f:95-106, a cell is inside when its centre is in a sphere."""
+    rng = np.random.default_rng(seed)
+    centres = []
+    tries = 0
+    while len(centres) < nsph:
+        tries += 1
+        if tries > 100000:
+            raise RuntimeError("could not place the spheres")
+        c = rng.uniform(-box / 2 + radius, box / 2 - radius, 3)
+        if all(np.linalg.norm(c - o) >= 2 * radius for o in centres):
+            centres.append(c)
+    d = box / nnnr
+    g = np.arange(1, nnnr + 1) * d - (box + d) / 2.0
+    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
+    inside = np.zeros(X.shape, bool)
+    for c in centres:
+        inside |= (X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2 < radius ** 2
+    tab, pos = _lattice(g, g, g, inside)
+    return Scene(nnnr, nnnr, nnnr, d, tab, pos, np.full(pos.shape[0], eps, np.complex128))
+
+
+def pupil_incidences(n_inc: int, na: float, seed: int):
+    """(theta, phi) in degrees of n_inc directions uniform over the pupil disk
+    |sin theta| <= na (the illumination cone of a brightfield microscope)."""
+    rng = np.random.default_rng(seed)
+    rr = na * np.sqrt(rng.uniform(0.0, 1.0, n_inc))
+    ph = rng.uniform(0.0, 2 * np.pi, n_inc)
+    return np.degrees(np.arcsin(rr)), np.degrees(ph)
+
+
+def config3(n_inc: int = 128, seed: int = 3):
+    """20 spheres r = 150 nm, eps 2.0+0.001i, in a 1.28 um box on a 64^3
+    lattice, lambda 632.8 nm; n_inc incidences in an NA 0.95 cone x 2 linear
+    polarisations (p then s): E_inc is [2*n_inc, 3*nd]."""
+    lam = 632.8e-9
+    k0 = 2 * np.pi / lam
+    s = random_spheres(64, 1.28e-6, 20, 150e-9, 2.0 + 0.001j, seed)
+    alpha = polarizability(s.eps, s.d, k0, "RR")
+    th, ph = pupil_incidences(n_inc, 0.95, seed + 1)
+    E = np.stack([plane_wave(s.pos, k0, t, p, pp=pp) for pp in (1.0, 0.0) for t, p in zip(th, ph)])
+    return s, k0, alpha, E

@@ -123,6 +123,18 @@ int ifdda_solve(ifdda_ctx *ctx, const ifdda_solve_opts *opts, const double *E_in
 int ifdda_solve_device(ifdda_ctx *ctx, const ifdda_solve_opts *opts, const void *E_inc, void *FF,
                        void *FFloc, ifdda_solve_info *info, volatile int *nstop, void *stream);
 
+/* Several right-hand sides (incidences) of one object in one batched solve:
+ * E_inc, FF, FFloc hold nrhs vectors of 3*ndipole complex128, one after the
+ * other; infos has nrhs entries.  Each RHS runs the same recurrence as a
+ * single solve with its own scalars and convergence test; RHS that converge
+ * stop costing work while the others continue.  Replaces the sequential
+ * incidence loop of microsbf.f:214-281 (one inverserig call per incidence). */
+int ifdda_solve_multi(ifdda_ctx *ctx, const ifdda_solve_opts *opts, int nrhs, const double *E_inc,
+                      double *FF, double *FFloc, ifdda_solve_info *infos, volatile int *nstop);
+int ifdda_solve_multi_device(ifdda_ctx *ctx, const ifdda_solve_opts *opts, int nrhs,
+                             const void *E_inc, void *FF, void *FFloc, ifdda_solve_info *infos,
+                             volatile int *nstop, void *stream);
+
 /* Instrumentation (no counterpart in the reference, which only prints wall
  * and CPU time, cdmlib.f:1956-1959): record HIP events around each matvec
  * pass on the stream it runs on, for up to cap passes; cap = 0 disables.
