@@ -122,6 +122,86 @@ constexpr int kBatch = 8;
 // instantiated for NG = 1..kMaxG and launched with the smallest that fits).
 constexpr int kMaxG = 4;
 
+// R-point DFT of a half-zero input: a[q] = 0 for q >= R/2 (only the first R/2
+// are read).  Even outputs are the R/2-point DFT of a, odd outputs that of
+// a[p] W^p, W = exp(SGN 2 pi i / R).
+template <int R, int SGN, typename V> struct DftHalfIn;
+template <int SGN, typename V> struct DftHalfIn<2, SGN, V> {
+  static __device__ __forceinline__ void run(V* a) { a[1] = a[0]; }
+};
+template <int SGN, typename V> struct DftHalfIn<4, SGN, V> {
+  static __device__ __forceinline__ void run(V* a) {
+    const V w = mul_si<SGN>(a[1]);
+    const V x = a[0];
+    a[0] = cadd(x, a[1]);
+    a[2] = csub(x, a[1]);
+    a[1] = cadd(x, w);
+    a[3] = csub(x, w);
+  }
+};
+template <int SGN, typename V> struct DftHalfIn<8, SGN, V> {
+  static __device__ __forceinline__ void run(V* a) {
+    using S = decltype(V::x);
+    const S h = S(IFDDA_R2);
+    V e[4] = {a[0], a[1], a[2], a[3]};
+    V o[4];
+    o[0] = a[0];
+    o[1] = SGN > 0 ? mk((a[1].x - a[1].y) * h, (a[1].y + a[1].x) * h)
+                   : mk((a[1].x + a[1].y) * h, (a[1].y - a[1].x) * h);
+    o[2] = mul_si<SGN>(a[2]);
+    o[3] = SGN > 0 ? mk((-a[3].x - a[3].y) * h, (-a[3].y + a[3].x) * h)
+                   : mk((-a[3].x + a[3].y) * h, (-a[3].y - a[3].x) * h);
+    Dft<4, SGN, V>::run(e);
+    Dft<4, SGN, V>::run(o);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      a[2 * q] = e[q];
+      a[2 * q + 1] = o[q];
+    }
+  }
+};
+
+// R-point DFT of which only outputs q < R/2 are wanted (left in a[0..R/2)):
+// y_q = sum_{p<R/2} (a_p + (-1)^q a_{p+R/2}) W^{pq}.
+template <int R, int SGN, typename V> struct DftHalfOut;
+template <int SGN, typename V> struct DftHalfOut<2, SGN, V> {
+  static __device__ __forceinline__ void run(V* a) { a[0] = cadd(a[0], a[1]); }
+};
+template <int SGN, typename V> struct DftHalfOut<4, SGN, V> {
+  static __device__ __forceinline__ void run(V* a) {
+    const V b0 = cadd(a[0], a[2]), b1 = cadd(a[1], a[3]);
+    const V c0 = csub(a[0], a[2]), c1 = mul_si<SGN>(csub(a[1], a[3]));
+    a[0] = cadd(b0, b1);
+    a[1] = cadd(c0, c1);
+  }
+};
+template <int SGN, typename V> struct DftHalfOut<8, SGN, V> {
+  static __device__ __forceinline__ void run(V* a) {
+    using S = decltype(V::x);
+    const S h = S(IFDDA_R2);
+    V b[4], c[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      b[p] = cadd(a[p], a[p + 4]);
+      c[p] = csub(a[p], a[p + 4]);
+    }
+    c[1] = SGN > 0 ? mk((c[1].x - c[1].y) * h, (c[1].y + c[1].x) * h)
+                   : mk((c[1].x + c[1].y) * h, (c[1].y - c[1].x) * h);
+    c[2] = mul_si<SGN>(c[2]);
+    c[3] = SGN > 0 ? mk((-c[3].x - c[3].y) * h, (-c[3].y + c[3].x) * h)
+                   : mk((-c[3].x + c[3].y) * h, (-c[3].y - c[3].x) * h);
+    // outputs 0 and 1 of the 4-point DFTs of b (-> y0, y2) and c (-> y1, y3)
+    const V bs = cadd(cadd(b[0], b[2]), cadd(b[1], b[3]));
+    const V bd = cadd(csub(b[0], b[2]), mul_si<SGN>(csub(b[1], b[3])));
+    const V cs = cadd(cadd(c[0], c[2]), cadd(c[1], c[3]));
+    const V cd = cadd(csub(c[0], c[2]), mul_si<SGN>(csub(c[1], c[3])));
+    a[0] = bs;
+    a[1] = cs;
+    a[2] = bd;
+    a[3] = cd;
+  }
+};
+
 // In-place mixed-radix FFT of a tile held in LDS as [pos][line].
 //
 // Radices r_1..r_k (AxisPlan order), m_s = r_1...r_{s-1}.  Position
@@ -132,36 +212,73 @@ constexpr int kMaxG = 4;
 // and leaves the transform permuted.  Every butterfly of stage s reads and
 // writes the same positions base + j + q*m_s (base = block*m_s*R, j < m_s), so
 // a thread holds one butterfly (R values) at a time.  tw[e] = exp(+2 pi i e/L).
-template <int R, int SGN, bool DIT, typename V>
+//
+// Thread jt of a line takes butterflies t = jt, jt + JT, ...; (block, j) =
+// divmod(t, m), the LDS element of point 0 and the twiddle exponent are
+// advanced by constant steps, so the loop has no integer division and no
+// per-point multiply.
+//
+// PRUNE 1 (first DIT stage, m = 1): points q >= R/2 of every butterfly are
+// signal indices n >= L/2, zero padding when the line holds n_in <= L/2
+// values; they are neither read nor need to be initialised.  PRUNE 2 (last
+// DIF stage, m = 1): outputs q >= R/2 are transform indices >= L/2, which no
+// caller keeps (n_out <= L/2); they are not computed or written.
+template <int R, int SGN, bool DIT, int PRUNE, typename V>
 __device__ __forceinline__ void stage_ip(V* buf, const V* __restrict__ tw, int L, int m, int NL,
                                          int line, int jt, int JT) {
-  const int nb = L / R;
-  const int tws = L / (m * R);
-  for (int t = jt; t < nb; t += JT) {
-    const int j = t % m;
-    const int base = (t - j) * R + j;
+  const unsigned um = (unsigned)m, uJT = (unsigned)JT, uNL = (unsigned)NL;
+  const unsigned nb = (unsigned)L / R;
+  const unsigned stride = um * uNL;  // LDS elements between the points of a butterfly
+  const unsigned tws = (unsigned)L / (um * R);
+  const unsigned dq = uJT / um, dr = uJT - dq * um;  // divmod(JT, m)
+  unsigned j = (unsigned)jt % um;
+  unsigned e0 = (((unsigned)jt - j) * R + j) * uNL + (unsigned)line;
+  unsigned ex = j * tws;
+  const unsigned de = (dq * um * R + dr) * uNL, dwrap = (um * (R - 1)) * uNL;
+  const unsigned dx = dr * tws, xwrap = um * tws;
+  for (unsigned t = (unsigned)jt; t < nb; t += uJT) {
+    V* p = buf + e0;
     V a[R];
+    constexpr int RIN = PRUNE == 1 ? R / 2 : R;
+    constexpr int ROUT = PRUNE == 2 ? R / 2 : R;
 #pragma unroll
-    for (int q = 0; q < R; ++q) a[q] = buf[(base + q * m) * NL + line];
-    if (DIT && m > 1) {
+    for (int q = 0; q < RIN; ++q) a[q] = p[q * stride];
+    if constexpr (DIT && PRUNE == 0) if (um > 1) {
+      unsigned xq = ex;
 #pragma unroll
       for (int q = 1; q < R; ++q) {
-        V w = tw[j * q * tws];
+        V w = tw[xq];
+        xq += ex;
         if (SGN < 0) w.y = -w.y;
         a[q] = cmul(a[q], w);
       }
     }
-    Dft<R, SGN, V>::run(a);
-    if (!DIT && m > 1) {
+    if constexpr (PRUNE == 1)
+      DftHalfIn<R, SGN, V>::run(a);
+    else if constexpr (PRUNE == 2)
+      DftHalfOut<R, SGN, V>::run(a);
+    else
+      Dft<R, SGN, V>::run(a);
+    if constexpr (!DIT && PRUNE == 0) if (um > 1) {
+      unsigned xq = ex;
 #pragma unroll
       for (int q = 1; q < R; ++q) {
-        V w = tw[j * q * tws];
+        V w = tw[xq];
+        xq += ex;
         if (SGN < 0) w.y = -w.y;
         a[q] = cmul(a[q], w);
       }
     }
 #pragma unroll
-    for (int q = 0; q < R; ++q) buf[(base + q * m) * NL + line] = a[q];
+    for (int q = 0; q < ROUT; ++q) p[q * stride] = a[q];
+    j += dr;
+    e0 += de;
+    ex += dx;
+    if (j >= um) {
+      j -= um;
+      e0 += dwrap;
+      ex -= xwrap;
+    }
   }
   lds_sync();
 }
@@ -170,33 +287,65 @@ template <int SGN, bool DIT, typename V>
 __device__ __forceinline__ void stage_any(int r, V* buf, const V* tw, int L, int m, int NL,
                                           int line, int jt, int JT) {
   switch (r) {
-    case 2: stage_ip<2, SGN, DIT>(buf, tw, L, m, NL, line, jt, JT); break;
-    case 3: stage_ip<3, SGN, DIT>(buf, tw, L, m, NL, line, jt, JT); break;
-    case 4: stage_ip<4, SGN, DIT>(buf, tw, L, m, NL, line, jt, JT); break;
-    case 5: stage_ip<5, SGN, DIT>(buf, tw, L, m, NL, line, jt, JT); break;
-    default: stage_ip<8, SGN, DIT>(buf, tw, L, m, NL, line, jt, JT); break;
+    case 2: stage_ip<2, SGN, DIT, 0>(buf, tw, L, m, NL, line, jt, JT); break;
+    case 3: stage_ip<3, SGN, DIT, 0>(buf, tw, L, m, NL, line, jt, JT); break;
+    case 4: stage_ip<4, SGN, DIT, 0>(buf, tw, L, m, NL, line, jt, JT); break;
+    case 5: stage_ip<5, SGN, DIT, 0>(buf, tw, L, m, NL, line, jt, JT); break;
+    default: stage_ip<8, SGN, DIT, 0>(buf, tw, L, m, NL, line, jt, JT); break;
   }
 }
 
+// Pruned first DIT / last DIF stage (m = 1, even radix).
+template <int SGN, bool DIT, typename V>
+__device__ __forceinline__ void stage_pruned(int r, V* buf, const V* tw, int L, int NL, int line,
+                                             int jt, int JT) {
+  constexpr int P = DIT ? 1 : 2;
+  switch (r) {
+    case 2: stage_ip<2, SGN, DIT, P>(buf, tw, L, 1, NL, line, jt, JT); break;
+    case 4: stage_ip<4, SGN, DIT, P>(buf, tw, L, 1, NL, line, jt, JT); break;
+    default: stage_ip<8, SGN, DIT, P>(buf, tw, L, 1, NL, line, jt, JT); break;
+  }
+}
+
+// Whether the first radix of the plan allows the half-line pruning.
+__device__ __forceinline__ bool prunable(const AxisPlan& ax) {
+  return (ax.rad[0] & 1) == 0;
+}
+
 // Thread layout: line = tid % NL, jt = tid / NL, JT = blockDim / NL.
+// half_in: the signal is zero at n >= L/2 (caller checked 2 n_in <= L).
 template <int SGN, typename V>
 __device__ void tile_fft_dit(V* buf, const V* tw, const AxisPlan& ax, int NL, int line, int jt,
-                             int JT) {
-  int m = 1;
-  for (int s = 0; s < ax.nst; ++s) {
+                             int JT, bool half_in) {
+  int m = 1, s = 0;
+  if (half_in && prunable(ax)) {
+    stage_pruned<SGN, true>(ax.rad[0], buf, tw, ax.L, NL, line, jt, JT);
+    m = ax.rad[0];
+    s = 1;
+  }
+  for (; s < ax.nst; ++s) {
     stage_any<SGN, true>(ax.rad[s], buf, tw, ax.L, m, NL, line, jt, JT);
     m *= ax.rad[s];
   }
 }
 
+// half_out: only transform indices < L/2 are read afterwards.
 template <int SGN, typename V>
 __device__ void tile_fft_dif(V* buf, const V* tw, const AxisPlan& ax, int NL, int line, int jt,
-                             int JT) {
+                             int JT, bool half_out) {
   int m = ax.L;
-  for (int s = ax.nst - 1; s >= 0; --s) {
+  const int last = half_out && prunable(ax) ? 1 : 0;
+  for (int s = ax.nst - 1; s >= last; --s) {
     m /= ax.rad[s];
     stage_any<SGN, false>(ax.rad[s], buf, tw, ax.L, m, NL, line, jt, JT);
   }
+  if (last) stage_pruned<SGN, false>(ax.rad[0], buf, tw, ax.L, NL, line, jt, JT);
+}
+
+// Rows the load must initialise: with the first-stage pruning, positions of
+// signal indices >= L/2 are never read.
+__device__ __forceinline__ int rows_to_fill(const AxisPlan& ax, bool half_in) {
+  return half_in && prunable(ax) ? ax.L / 2 : ax.L;
 }
 
 // LDS layout of every FFT pass: twiddles (L values), positions (L ints),
@@ -262,13 +411,13 @@ __device__ __forceinline__ int xcd_remap(int lin, int total) {
   return xcd * q + min(xcd, r) + j;
 }
 
-// Load rows p < n_in of a strided tile into permuted LDS positions (zeros
-// elsewhere).
-template <typename V>
+// Load rows p < n_in of a strided tile into LDS (zeros for n_in <= p < fill):
+// at permuted positions pos[p] (PERM, input of a DIT transform) or at p.
+template <bool PERM, typename V>
 __device__ __forceinline__ void load_strided(V* buf, const int* pos, const V* src, long long sp,
-                                             int L, int n_in, int NL, int line, int nvalid,
+                                             int fill, int n_in, int NL, int line, int nvalid,
                                              int jt, int JT) {
-  for (int p0 = jt; p0 < L; p0 += kBatch * JT) {
+  for (int p0 = jt; p0 < fill; p0 += kBatch * JT) {
     V v[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
@@ -279,12 +428,15 @@ __device__ __forceinline__ void load_strided(V* buf, const int* pos, const V* sr
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       const int p = p0 + u * JT;
-      if (p < L) buf[pos[p] * NL + line] = v[u];
+      if (p < fill) buf[(PERM ? pos[p] : p) * NL + line] = v[u];
     }
   }
   lds_sync();
 }
 
+// Forward transforms (SGN > 0) run DIT from a permuted load with the
+// first-stage pruning; inverse transforms (SGN < 0) run DIF from a natural
+// load with the last-stage pruning and store through pos.
 template <typename V, int NC, int SGN>
 __global__ void __launch_bounds__(256) k_strided(SArgs<V> a) {
   // a stays unmodified (kernel-argument memory); per-RHS offsets go to locals
@@ -307,12 +459,21 @@ __global__ void __launch_bounds__(256) k_strided(SArgs<V> a) {
 
   load_tables(tw, pos, a.tw, a.pos, L, tid, nthr);
   lds_sync();
-  load_strided(buf, pos, zsrc + (long long)o * a.so_src + (long long)kx0 * NC, a.sp_src, L,
-               a.n_in, NL, line, nvalid, jt, JT);
-  tile_fft_dit<SGN>(buf, tw, a.ax, NL, line, jt, JT);
+  const V* src = zsrc + (long long)o * a.so_src + (long long)kx0 * NC;
   V* dst = zdst + (long long)o * a.so_dst + (long long)kx0 * NC;
-  if (line < nvalid) {
-    for (int p = jt; p < a.n_out; p += JT) dst[(long long)p * a.sp_dst + line] = buf[p * NL + line];
+  if (SGN > 0) {
+    const bool half = 2 * a.n_in <= L;
+    load_strided<true>(buf, pos, src, a.sp_src, rows_to_fill(a.ax, half), a.n_in, NL, line,
+                       nvalid, jt, JT);
+    tile_fft_dit<SGN>(buf, tw, a.ax, NL, line, jt, JT, half);
+    if (line < nvalid)
+      for (int p = jt; p < a.n_out; p += JT) dst[(long long)p * a.sp_dst + line] = buf[p * NL + line];
+  } else {
+    load_strided<false>(buf, pos, src, a.sp_src, L, a.n_in, NL, line, nvalid, jt, JT);
+    tile_fft_dif<SGN>(buf, tw, a.ax, NL, line, jt, JT, 2 * a.n_out <= L);
+    if (line < nvalid)
+      for (int p = jt; p < a.n_out; p += JT)
+        dst[(long long)p * a.sp_dst + line] = buf[pos[p] * NL + line];
   }
 }
 
@@ -347,13 +508,14 @@ __global__ void __launch_bounds__(256) k_zconv(SArgs<V> a) {
 
   load_tables(tw, pos, a.tw, a.pos, L, tid, nthr);
   lds_sync();
+  const bool half_in = 2 * a.n_in <= L, half_out = 2 * a.n_out <= L;
   const int nrow = (kyf == 0 || 2 * kyf == a.L2) ? 1 : 2;
   for (int rr = 0; rr < nrow; ++rr) {
     const int ky = rr == 0 ? kyf : a.L2 - kyf;
     const bool sy = rr == 1;  // y odd-parity sign for the mirror row
-    load_strided(buf, pos, zsrc + (long long)ky * a.so_src + (long long)kx0 * 3, a.sp_src, L,
-                 a.n_in, NL, line, nvalid, jt, JT);
-    tile_fft_dit<+1>(buf, tw, a.ax, NL, line, jt, JT);
+    load_strided<true>(buf, pos, zsrc + (long long)ky * a.so_src + (long long)kx0 * 3, a.sp_src,
+                       rows_to_fill(a.ax, half_in), a.n_in, NL, line, nvalid, jt, JT);
+    tile_fft_dit<+1>(buf, tw, a.ax, NL, line, jt, JT, half_in);
     // Green entries of this thread's items, all loads issued first.
     V gr[NG][6];
     int gx[NG];  // bit0 valid, bit1 x odd-parity sign
@@ -397,7 +559,7 @@ __global__ void __launch_bounds__(256) k_zconv(SArgs<V> a) {
       }
     }
     lds_sync();
-    tile_fft_dif<-1>(buf, tw, a.ax, NL, line, jt, JT);
+    tile_fft_dif<-1>(buf, tw, a.ax, NL, line, jt, JT, half_out);
     V* dst = zdst + (long long)ky * a.so_dst + (long long)kx0 * 3;
     if (line < nvalid) {
       for (int p = jt; p < a.n_out; p += JT)
@@ -462,6 +624,30 @@ __device__ __forceinline__ long long dip_of(const int32_t* __restrict__ tab, int
   return tab ? (long long)tab[cell] - 1 : cell;
 }
 
+// (line, point) = divmod(e, n) for e = e0 + k * step, advanced without
+// division: step = dl * n + di.
+struct DivStep {
+  int l, i;
+  int dl, di, n;
+  __device__ __forceinline__ DivStep(int e0, int step, int n_) : n(n_) {
+    l = e0 / n;
+    i = e0 - l * n;
+    dl = step / n;
+    di = step - dl * n;
+  }
+  __device__ __forceinline__ void next() {
+    l += dl;
+    i += di;
+    if (i >= n) {
+      i -= n;
+      ++l;
+    }
+  }
+};
+
+// Forward (SGN > 0): DIT from a permuted load, first stage pruned when
+// 2 n_in <= L.  Inverse (SGN < 0): DIF from a natural load, last stage pruned
+// when 2 n_out <= L, output read through pos.
 template <typename V, int NC, int SGN, int MODE>
 __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
   if (a.skip && a.skip[blockIdx.z * a.skip_stride]) return;
@@ -470,6 +656,7 @@ __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
   V* const zdst = a.dst ? a.dst + blockIdx.z * a.zs_dst : nullptr;
   const V* const zxin = a.xin ? a.xin + blockIdx.z * a.zs_x : nullptr;
   V* const zxout = a.xout ? a.xout + blockIdx.z * a.zs_x : nullptr;
+  constexpr bool DIT = SGN > 0;
   extern __shared__ double4 smem_d4[];
   V* tw = reinterpret_cast<V*>(smem_d4);
   const int L = a.ax.L;
@@ -481,82 +668,85 @@ __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
   const int line = tid % NL, jt = tid / NL, JT = nthr / NL;
   const int g0 = xcd_remap(blockIdx.x, gridDim.x) * a.B;
   const int nl = min(a.B, a.nlines - g0);
+  const bool half_in = DIT && 2 * a.n_in <= L;
+  const bool half_out = !DIT && 2 * a.n_out <= L;
+  const int fill = DIT ? rows_to_fill(a.ax, half_in) : L;
 
   load_tables(tw, pos, a.tw, a.pos, L, tid, nthr);
   lds_sync();
-  if (MODE == 1) {
-    constexpr int U = kBatch / 2;
-    for (int e0 = tid; e0 < a.B * L; e0 += U * nthr) {
-      long long dip[U];
+  {
+    constexpr int U = MODE == 1 ? kBatch / 2 : kBatch / NC;
+    DivStep ix(tid, nthr, fill);  // (tile line l, point i)
+    for (int e0 = tid; e0 < a.B * fill; e0 += U * nthr) {
+      int ll[U], ii[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int e = e0 + u * nthr;
-        const int l = e / L, i = e - l * L;
-        dip[u] = -1;
-        if (e < a.B * L && l < nl && i < a.n_in) dip[u] = dip_of(a.tab, a.nx, g0 + l, i);
+        ll[u] = ix.l;
+        ii[u] = ix.i;
+        ix.next();
       }
-      V x[U][3];
+      V x[U][NC];
+      if (MODE == 1) {
+        long long dip[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        x[u][0] = x[u][1] = x[u][2] = czero<V>();
-        if (dip[u] >= 0) {
-          const V* xp = zxin + dip[u] * 3;
-          x[u][0] = xp[0];
-          x[u][1] = xp[1];
-          x[u][2] = xp[2];
-          alpha_apply(a.alpha, a.alpha_mode, dip[u], x[u][0], x[u][1], x[u][2]);
+        for (int u = 0; u < U; ++u) {
+          dip[u] = -1;
+          if (ll[u] < nl && ii[u] < a.n_in) dip[u] = dip_of(a.tab, a.nx, g0 + ll[u], ii[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+          for (int c = 0; c < NC; ++c) x[u][c] = czero<V>();
+          if (dip[u] >= 0) {
+            const V* xp = zxin + dip[u] * 3;
+            x[u][0] = xp[0];
+            x[u][1] = xp[1];
+            x[u][2] = xp[2];
+            alpha_apply(a.alpha, a.alpha_mode, dip[u], x[u][0], x[u][1], x[u][2]);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+          for (int c = 0; c < NC; ++c) x[u][c] = czero<V>();
+          if (ll[u] < nl && ii[u] < a.n_in) {
+            const V* sp = zsrc + (long long)(g0 + ll[u]) * a.ld_src + (long long)ii[u] * NC;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) x[u][c] = sp[c];
+          }
         }
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int e = e0 + u * nthr;
-        if (e < a.B * L) {
-          const int l = e / L, i = e - l * L;
-          V* d = buf + pos[i] * NL + l * 3;
-          d[0] = x[u][0];
-          d[1] = x[u][1];
-          d[2] = x[u][2];
-        }
-      }
-    }
-  } else {
-    const int per = L * NC;
-    for (int e0 = tid; e0 < a.B * per; e0 += kBatch * nthr) {
-      V v[kBatch];
+        if (ll[u] < a.B) {
+          V* d = buf + (DIT ? pos[ii[u]] : ii[u]) * NL + ll[u] * NC;
 #pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
-        const int e = e0 + u * nthr;
-        const int l = e / per, r = e - l * per;
-        const int p = r / NC, c = r - p * NC;
-        v[u] = czero<V>();
-        if (e < a.B * per && l < nl && p < a.n_in)
-          v[u] = zsrc[(long long)(g0 + l) * a.ld_src + (long long)p * NC + c];
-      }
-#pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
-        const int e = e0 + u * nthr;
-        if (e < a.B * per) {
-          const int l = e / per, r = e - l * per;
-          const int p = r / NC, c = r - p * NC;
-          buf[pos[p] * NL + l * NC + c] = v[u];
+          for (int c = 0; c < NC; ++c) d[c] = x[u][c];
         }
       }
     }
   }
   lds_sync();
 
-  tile_fft_dit<SGN>(buf, tw, a.ax, NL, line, jt, JT);
+  if (DIT)
+    tile_fft_dit<SGN>(buf, tw, a.ax, NL, line, jt, JT, half_in);
+  else
+    tile_fft_dif<SGN>(buf, tw, a.ax, NL, line, jt, JT, half_out);
 
   if (MODE == 2) {
     constexpr int U = kBatch / 2;
+    DivStep ix(tid, nthr, a.n_out);
     for (int e0 = tid; e0 < a.B * a.n_out; e0 += U * nthr) {
+      int ll[U], ii[U];
       long long dip[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int e = e0 + u * nthr;
-        const int l = e / a.n_out, i = e - l * a.n_out;
+        ll[u] = ix.l;
+        ii[u] = ix.i;
+        ix.next();
         dip[u] = -1;
-        if (e < a.B * a.n_out && l < nl) dip[u] = dip_of(a.tab, a.nx, g0 + l, i);
+        if (ll[u] < nl) dip[u] = dip_of(a.tab, a.nx, g0 + ll[u], ii[u]);
       }
       V b[U][3];
 #pragma unroll
@@ -570,9 +760,7 @@ __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
 #pragma unroll
       for (int u = 0; u < U; ++u)
         if (dip[u] >= 0) {
-          const int e = e0 + u * nthr;
-          const int l = e / a.n_out, i = e - l * a.n_out;
-          const V* v = buf + i * NL + l * 3;
+          const V* v = buf + (DIT ? ii[u] : pos[ii[u]]) * NL + ll[u] * 3;
           V* y = zxout + dip[u] * 3;
           y[0] = csub(b[u][0], cscal(v[0], a.inv_m));
           y[1] = csub(b[u][1], cscal(v[1], a.inv_m));
@@ -580,12 +768,13 @@ __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
         }
     }
   } else {
-    const int per = a.n_out * NC;
-    for (int e = tid; e < a.B * per; e += nthr) {
-      const int l = e / per, r = e - l * per;
-      if (l >= nl) continue;
-      const int p = r / NC, c = r - p * NC;
-      zdst[(long long)(g0 + l) * a.ld_dst + (long long)p * NC + c] = buf[p * NL + l * NC + c];
+    DivStep ix(tid, nthr, a.n_out);
+    for (int e = tid; e < a.B * a.n_out; e += nthr, ix.next()) {
+      if (ix.l >= nl) continue;
+      const V* v = buf + (DIT ? ix.i : pos[ix.i]) * NL + ix.l * NC;
+      V* d = zdst + (long long)(g0 + ix.l) * a.ld_dst + (long long)ix.i * NC;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) d[c] = v[c];
     }
   }
 }
