@@ -29,6 +29,7 @@ namespace ifdda {
 constexpr int kThreads = 256;
 constexpr int kMaxBlocks = 1024;
 constexpr int kMaxDots = 5;
+static_assert(kMaxBlocks % kThreads == 0, "finish_sum splits the partials evenly");
 
 struct KState {
   double norm;    // |b|
@@ -82,6 +83,7 @@ template <typename V> struct Vt;
 template <> struct Vt<double2> {
   static __device__ __forceinline__ double2 ld(const double2* p, long long i) { return p[i]; }
   static __device__ __forceinline__ void st(double2* p, long long i, double2 v) { p[i] = v; }
+  static __device__ __forceinline__ double2 rnd(double2 v) { return v; }
 };
 template <> struct Vt<float2> {
   static __device__ __forceinline__ double2 ld(const float2* p, long long i) {
@@ -90,6 +92,10 @@ template <> struct Vt<float2> {
   }
   static __device__ __forceinline__ void st(float2* p, long long i, double2 v) {
     p[i] = make_float2((float)v.x, (float)v.y);
+  }
+  // the value a store followed by a load returns
+  static __device__ __forceinline__ double2 rnd(double2 v) {
+    return make_double2((double)(float)v.x, (double)(float)v.y);
   }
 };
 // Vector updates are evaluated in FP64 registers and stored in V.
@@ -117,20 +123,42 @@ __device__ __forceinline__ void block_partials(double2 (&v)[NR], double2* part, 
     }
 }
 
-// Sum the partials of NR reductions into st->dots.  Launched as one block;
-// the first wavefront does the work with shuffles (fixed order), the other
-// threads of the block wait at the final barrier.
+// Sum the partials of NR reductions into st->dots.  Launched as one block of
+// kThreads: every thread issues its (at most kMaxBlocks / kThreads) partial
+// loads back to back, then a fixed-order reduction (thread, LDS, first
+// wavefront shuffles) so the result does not depend on timing.
 template <int NR>
 __device__ __forceinline__ void finish_sum(KState* st, const double2* part, int nblk) {
-  if (threadIdx.x < 64) {
+  constexpr int kPer = kMaxBlocks / kThreads;
+  __shared__ double2 red[NR][kThreads];
+  const int tid = threadIdx.x;
+  double2 v[NR][kPer];
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int b = tid + k * kThreads;
+      v[r][k] = b < nblk ? part[r * nblk + b] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    double2 s = v[r][0];
+#pragma unroll
+    for (int k = 1; k < kPer; ++k) s = zadd(s, v[r][k]);
+    red[r][tid] = s;
+  }
+  __syncthreads();
+  if (tid < 64) {
+#pragma unroll
     for (int r = 0; r < NR; ++r) {
-      double2 s = make_double2(0.0, 0.0);
-      for (int b = threadIdx.x; b < nblk; b += 64) s = zadd(s, part[r * nblk + b]);
+      double2 s = red[r][tid];
+#pragma unroll
+      for (int k = 1; k < kThreads / 64; ++k) s = zadd(s, red[r][tid + 64 * k]);
       for (int off = 32; off > 0; off >>= 1) {
         s.x += __shfl_down(s.x, off, 64);
         s.y += __shfl_down(s.y, off, 64);
       }
-      if (threadIdx.x == 0) st->dots[r] = s;
+      if (tid == 0) st->dots[r] = s;
     }
   }
   __syncthreads();
@@ -177,15 +205,20 @@ __global__ void __launch_bounds__(kThreads) f_init(KState* st, const double2* pa
 }
 
 // ============================================================ GPBICG kernels
-template <typename V> struct GV {  // gpbicg.f:103-114 work columns
-  V *x, *b, *r, *r0s, *p, *Ap, *t, *At, *y, *u, *w, *z, *told, *Ax;
+// gpbicg.f:103-114 work columns.  t/told and Ap/Apold are two buffer pairs
+// whose roles the host exchanges every iteration: this replaces the copy
+// told = t (gpbicg.f:192-194) and the stored w = At + beta Ap
+// (gpbicg.f:319-325), which is formed where it is used from At and the
+// previous iteration's Ap.
+template <typename V> struct GV {
+  V *x, *b, *r, *r0s, *p, *Ap, *t, *At, *y, *u, *Apold, *z, *told, *Ax;
   __device__ GV shifted(long long o) const {
-    return GV{x + o, b + o, r + o, r0s + o, p + o, Ap + o, t + o, At + o, y + o, u + o, w + o,
+    return GV{x + o, b + o, r + o, r0s + o, p + o, Ap + o, t + o, At + o, y + o, u + o, Apold + o,
               z + o, told + o, Ax + o};
   }
 };
 
-// r = b - Ax (Ax = 0 when x0 = 0); r0* = conj(r); p = t = w = u = z = 0;
+// r = b - Ax (Ax = 0 when x0 = 0); r0* = conj(r); p = t = told = u = z = 0;
 // partial sum r0* r  (gpbicg.f:133-148)
 template <typename V>
 __global__ void __launch_bounds__(kThreads) g_start(GV<V> v, int x0zero, long long n,
@@ -200,8 +233,8 @@ __global__ void __launch_bounds__(kThreads) g_start(GV<V> v, int x0zero, long lo
     if (!x0zero) r = zsub(r, LD(v.Ax, i));
     ST(v.r, i, r);
     ST(v.r0s, i, zconj(r));
-    ST(v.p, i, z0); ST(v.t, i, z0); ST(v.w, i, z0); ST(v.u, i, z0); ST(v.z, i, z0);
-    ST(v.Ap, i, z0); ST(v.At, i, z0);
+    ST(v.p, i, z0); ST(v.t, i, z0); ST(v.told, i, z0); ST(v.u, i, z0); ST(v.z, i, z0);
+    ST(v.Ap, i, z0); ST(v.Apold, i, z0); ST(v.At, i, z0);
     const double2 rs = zconj(r);
     acc[0] = zadd(acc[0], zmul(rs, r));
   }
@@ -220,7 +253,6 @@ __global__ void __launch_bounds__(kThreads) g_fstart(KState* st, const double2* 
   }
 }
 
-// w = At + beta Ap (label 40 of the previous iteration, gpbicg.f:319-325);
 // p = r + beta (p - u) (gpbicg.f:161-169)
 template <typename V>
 __global__ void __launch_bounds__(kThreads) g_k1(const KState* st, GV<V> v, long long n) {
@@ -229,11 +261,7 @@ __global__ void __launch_bounds__(kThreads) g_k1(const KState* st, GV<V> v, long
   v = v.shifted(rh_ * n);
   if (st->done) return;
   const double2 beta = st->beta;
-  const bool first = st->it == 0;
-  GRID_LOOP(i, n) {
-    if (!first) ST(v.w, i, zadd(LD(v.At, i), zmul(beta, LD(v.Ap, i))));
-    ST(v.p, i, zadd(LD(v.r, i), zmul(beta, zsub(LD(v.p, i), LD(v.u, i)))));
-  }
+  GRID_LOOP(i, n) ST(v.p, i, zadd(LD(v.r, i), zmul(beta, zsub(LD(v.p, i), LD(v.u, i)))));
 }
 
 // partial sum r0* Ap (gpbicg.f:174-181)
@@ -268,18 +296,21 @@ __global__ void __launch_bounds__(kThreads) g_f2(KState* st, const double2* part
   }
 }
 
-// y = t - r + alpha (Ap - w); t_old = t; t = r - alpha Ap (gpbicg.f:192-200)
+// w = At + beta Apold (0 in the first iteration; gpbicg.f:319-325 of the
+// previous iteration), y = told - r + alpha (Ap - w), t = r - alpha Ap
+// (gpbicg.f:192-200); told holds the previous t.
 template <typename V>
 __global__ void __launch_bounds__(kThreads) g_k3(const KState* st, GV<V> v, long long n) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
   v = v.shifted(rh_ * n);
   if (st->done) return;
-  const double2 al = st->alpha;
+  const double2 al = st->alpha, be = st->beta;
+  const bool first = st->it == 0;
   GRID_LOOP(i, n) {
-    const double2 t = LD(v.t, i), r = LD(v.r, i), Ap = LD(v.Ap, i);
-    ST(v.y, i, zadd(zsub(t, r), zmul(al, zsub(Ap, LD(v.w, i)))));
-    ST(v.told, i, t);
+    const double2 t = LD(v.told, i), r = LD(v.r, i), Ap = LD(v.Ap, i);
+    const double2 w = first ? make_double2(0.0, 0.0) : zadd(LD(v.At, i), zmul(be, LD(v.Apold, i)));
+    ST(v.y, i, zadd(zsub(t, r), zmul(al, zsub(Ap, w))));
     ST(v.t, i, zsub(r, zmul(al, Ap)));
   }
 }
@@ -360,7 +391,7 @@ __global__ void __launch_bounds__(kThreads) g_k5(const KState* st, GV<V> v, long
     ST(v.x, i, zadd(zadd(LD(v.x, i), zmul(al, LD(v.p, i))), z));
     const double2 rn = zsub(zsub(t, zmul(et, y)), zmul(dz, At));
     ST(v.r, i, rn);
-    const double2 rs = LD(v.r, i);  // the stored value feeds the dots
+    const double2 rs = Vt<V>::rnd(rn);  // the stored value feeds the dots
     acc[0].x += rs.x * rs.x + rs.y * rs.y;
     acc[1] = zadd(acc[1], zmul(LD(v.r0s, i), rs));
   }
@@ -1141,7 +1172,7 @@ static bool krylov(ifdda_ctx* c, Solver* S, V** vv, int R, int method, const dou
                    volatile int* nstop, hipStream_t s, KRun* out, bool* cancelled) {
   const long long n = S->n;
   const int nb = S->nblk;
-  const dim3 G(nb, R), T(kThreads), F(1, R), F64(64), One(1);
+  const dim3 G(nb, R), T(kThreads), F(1, R), One(1);
   double2* trace = R == 1 ? S->trace : nullptr;
   for (int r = 0; r < R; ++r) {
     KState& k = S->st_host[r];
@@ -1167,7 +1198,7 @@ static bool krylov(ifdda_ctx* c, Solver* S, V** vv, int R, int method, const dou
   V* Ax = vv[13];
   hipLaunchKernelGGL(k_init<V>, G, T, 0, s, (const V*)b, x, x0zero ? (const V*)nullptr : (const V*)x,
                      n, S->part, nb);
-  hipLaunchKernelGGL(f_init, F, F64, 0, s, S->st, (const double2*)S->part, nb);
+  hipLaunchKernelGGL(f_init, F, T, 0, s, S->st, (const double2*)S->part, nb);
   if (!x0zero) {
     if (!mv<V>(c, x, x, Ax, s, done, R, n, dstride)) return false;
     ++nmv0;
@@ -1189,20 +1220,26 @@ static bool krylov(ifdda_ctx* c, Solver* S, V** vv, int R, int method, const dou
   };
 
   if (method == IFDDA_GPBICG1) {
-    GV<V> v{x, b, vv[2], vv[3], vv[4], vv[5], vv[6], vv[7], vv[8], vv[9], vv[10], vv[11], vv[12], Ax};
+    V* const tb[2] = {vv[6], vv[12]};  // t / told
+    V* const ab[2] = {vv[5], vv[10]};  // Ap / Apold
+    GV<V> v{x, b, vv[2], vv[3], vv[4], ab[0], tb[0], vv[7], vv[8], vv[9], ab[1], vv[11], tb[1], Ax};
     hipLaunchKernelGGL(g_start<V>, G, T, 0, s, v, x0zero, n, S->part, nb);
-    hipLaunchKernelGGL(g_fstart, F, F64, 0, s, S->st, (const double2*)S->part, nb);
+    hipLaunchKernelGGL(g_fstart, F, T, 0, s, S->st, (const double2*)S->part, nb);
     for (int it = 0;; ++it) {
+      v.told = tb[it & 1];
+      v.t = tb[(it + 1) & 1];
+      v.Ap = ab[it & 1];
+      v.Apold = ab[(it + 1) & 1];
       hipLaunchKernelGGL(g_k1<V>, G, T, 0, s, (const KState*)S->st, v, n);
       if (!mv<V>(c, v.p, v.p, v.Ap, s, done, R, n, dstride)) return false;
       hipLaunchKernelGGL(g_k2<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(g_f2, F, F64, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(g_f2, F, T, 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(g_k3<V>, G, T, 0, s, (const KState*)S->st, v, n);
       if (!mv<V>(c, v.t, v.t, v.At, s, done, R, n, dstride)) return false;
       hipLaunchKernelGGL(g_k4<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(g_f4, F, F64, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(g_f4, F, T, 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(g_k5<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(g_f5, F, F64, 0, s, S->st, (const double2*)S->part, nb, trace);
+      hipLaunchKernelGGL(g_f5, F, T, 0, s, S->st, (const double2*)S->part, nb, trace);
       if (!hip_ok(hipGetLastError(), "gpbicg launch")) return false;
       const int st = step_poll(it);
       if (st < 0) return false;
@@ -1222,22 +1259,22 @@ static bool krylov(ifdda_ctx* c, Solver* S, V** vv, int R, int method, const dou
   } else if (method == IFDDA_QMRBICGSTAB1 || method == IFDDA_QMRBICGSTAB2) {
     QV<V> v{x, b, vv[2], vv[3], vv[4], vv[5], vv[6], vv[7], vv[8], vv[9], vv[10], Ax};
     hipLaunchKernelGGL(q_start<V>, G, T, 0, s, v, x0zero, n, S->part, nb);
-    hipLaunchKernelGGL(q_fstart, F, F64, 0, s, S->st, (const double2*)S->part, nb);
+    hipLaunchKernelGGL(q_fstart, F, T, 0, s, S->st, (const double2*)S->part, nb);
     for (int it = 0;; ++it) {
       hipLaunchKernelGGL(q_k1<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(q_f1, F, F64, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(q_f1, F, T, 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(q_k2<V>, G, T, 0, s, (const KState*)S->st, v, n);
       if (!mv<V>(c, v.p, v.p, v.v, s, done, R, n, dstride)) return false;
       hipLaunchKernelGGL(q_k3<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(q_f3, F, F64, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(q_f3, F, T, 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(q_k4<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(q_f4, F, F64, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(q_f4, F, T, 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(q_k5<V>, G, T, 0, s, (const KState*)S->st, v, n);
       if (!mv<V>(c, v.s, v.s, v.t, s, done, R, n, dstride)) return false;
       hipLaunchKernelGGL(q_k6<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(q_f6, F, F64, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(q_f6, F, T, 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(q_k7<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-      hipLaunchKernelGGL(q_f7, F, F64, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(q_f7, F, T, 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(q_k8<V>, G, T, 0, s, (const KState*)S->st, v, n);
       hipLaunchKernelGGL(q_f8, F, One, 0, s, S->st, trace);
       if (!hip_ok(hipGetLastError(), "qmrbicgstab launch")) return false;
@@ -1261,20 +1298,20 @@ static bool krylov(ifdda_ctx* c, Solver* S, V** vv, int R, int method, const dou
     hipLaunchKernelGGL(t_start<V>, G, T, 0, s, v, x0zero, n);
     if (!mv<V>(c, v.u, v.u, v.v, s, done, R, n, dstride)) return false;
     hipLaunchKernelGGL(t_start2<V>, G, T, 0, s, v, n, S->part, nb);
-    hipLaunchKernelGGL(t_fstart, F, F64, 0, s, S->st, (const double2*)S->part, nb);
+    hipLaunchKernelGGL(t_fstart, F, T, 0, s, S->st, (const double2*)S->part, nb);
     for (int it = 0;; ++it) {
       const int even = (it % 2 == 0);
       if (even) {
         hipLaunchKernelGGL(t_k1<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-        hipLaunchKernelGGL(t_f1, F, F64, 0, s, S->st, (const double2*)S->part, nb);
+        hipLaunchKernelGGL(t_f1, F, T, 0, s, S->st, (const double2*)S->part, nb);
       }
       hipLaunchKernelGGL(t_k2<V>, G, T, 0, s, (const KState*)S->st, v, even, n, S->part, nb);
-      hipLaunchKernelGGL(t_f2, F, F64, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(t_f2, F, T, 0, s, S->st, (const double2*)S->part, nb);
       hipLaunchKernelGGL(t_k3<V>, G, T, 0, s, (const KState*)S->st, v, n);
       hipLaunchKernelGGL(t_f3, F, One, 0, s, S->st, trace);
       if (!even) {
         hipLaunchKernelGGL(t_k4<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
-        hipLaunchKernelGGL(t_f4, F, F64, 0, s, S->st, (const double2*)S->part, nb);
+        hipLaunchKernelGGL(t_f4, F, T, 0, s, S->st, (const double2*)S->part, nb);
       }
       hipLaunchKernelGGL(t_k5<V>, G, T, 0, s, (const KState*)S->st, v, !even, n);
       if (!mv<V>(c, v.u, v.u, v.Au, s, done, R, n, dstride)) return false;
@@ -1304,7 +1341,7 @@ static bool krylov(ifdda_ctx* c, Solver* S, V** vv, int R, int method, const dou
 // Norms of R residual vectors whose partial sums k_resid / k_init /
 // k_refine_resid left in S->part.
 static bool resid_norms(Solver* S, int R, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(f_resid, dim3(1, R), dim3(64), 0, s, S->st, (const double2*)S->part, S->nblk);
+  hipLaunchKernelGGL(f_resid, dim3(1, R), dim3(kThreads), 0, s, S->st, (const double2*)S->part, S->nblk);
   if (!poll(S, R, s)) return false;
   for (int r = 0; r < R; ++r) out[r] = std::sqrt(S->st_host[r].dots[0].x);
   return true;
