@@ -284,12 +284,20 @@ extern "C" int ifdda_timing_enable(ifdda_ctx* c, int cap) {
     delete[] t.ev;
     delete[] t.tag;
   }
+  const int mask = t.mask;
   t = PassTimer();
+  t.mask = mask;
   if (cap == 0) return IFDDA_OK;
   t.ev = new hipEvent_t[2 * (size_t)cap];
   t.tag = new int[cap];
   for (int i = 0; i < 2 * cap; ++i) CHECK_HIP(hipEventCreate(&t.ev[i]), "hipEventCreate");
   t.cap = cap;
+  return IFDDA_OK;
+}
+
+extern "C" int ifdda_timing_passes(ifdda_ctx* c, int mask) {
+  if (!c || mask < 0 || mask > 31) return arg_error("ifdda_timing_passes: bad argument");
+  c->timer.mask = mask;
   return IFDDA_OK;
 }
 

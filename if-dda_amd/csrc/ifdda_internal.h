@@ -49,19 +49,25 @@ template <typename V> struct MatvecDev {
 // HIP-event timing of individual matvec passes on their own stream.  Slot
 // tag = prec_index * 5 + pass (pass 0..4 = P1..P5).  Events come from a pool
 // created when timing is enabled; recording stops when the pool is used up.
+// Only the passes whose bit is set in mask are bracketed (each event pair
+// adds a few microseconds between launches).
 struct PassTimer {
   int cap = 0;
   int used = 0;
+  int mask = 31;
+  bool open = false;
   hipEvent_t* ev = nullptr;  // 2 * cap events
   int* tag = nullptr;
   void begin(int t, hipStream_t s) {
-    if (used < cap) {
+    open = used < cap && ((mask >> (t % 5)) & 1);
+    if (open) {
       tag[used] = t;
       (void)hipEventRecord(ev[2 * used], s);
     }
   }
   void end(hipStream_t s) {
-    if (used < cap) (void)hipEventRecord(ev[2 * used + 1], s), ++used;
+    if (open) (void)hipEventRecord(ev[2 * used + 1], s), ++used;
+    open = false;
   }
 };
 

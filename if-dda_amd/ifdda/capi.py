@@ -78,6 +78,7 @@ def lib():
     L.ifdda_solve_trace.argtypes = [vp, ip, vp, C.POINTER(C.c_int)]
     L.ifdda_solve_trace_enable.argtypes = [vp, ip]
     L.ifdda_timing_enable.argtypes = [vp, ip]
+    L.ifdda_timing_passes.argtypes = [vp, ip]
     L.ifdda_timing_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     _lib = L
     return L
@@ -250,6 +251,10 @@ class Solver:
     def timing_enable(self, cap: int):
         """Record HIP events around each matvec pass (up to cap passes)."""
         _check(lib().ifdda_timing_enable(self.h, int(cap)), "ifdda_timing_enable")
+
+    def timing_passes(self, mask: int):
+        """Bracket only the passes whose bit is set (bit p = P(p+1))."""
+        _check(lib().ifdda_timing_passes(self.h, int(mask)), "ifdda_timing_passes")
 
     def timing_read(self):
         """Returns (ms_sum[2,5], count[2,5]): [FP64/FP32][P1..P5]; resets."""
