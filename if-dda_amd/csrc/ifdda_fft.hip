@@ -417,13 +417,15 @@ template <bool PERM, typename V>
 __device__ __forceinline__ void load_strided(V* buf, const int* pos, const V* src, long long sp,
                                              int fill, int n_in, int NL, int line, int nvalid,
                                              int jt, int JT) {
+  const long long ustep = (long long)JT * sp;
   for (int p0 = jt; p0 < fill; p0 += kBatch * JT) {
+    const V* s0 = src + (long long)p0 * sp + line;
     V v[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       const int p = p0 + u * JT;
       v[u] = czero<V>();
-      if (p < n_in && line < nvalid) v[u] = src[(long long)p * sp + line];
+      if (p < n_in && line < nvalid) v[u] = s0[u * ustep];
     }
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
@@ -466,14 +468,19 @@ __global__ void __launch_bounds__(256) k_strided(SArgs<V> a) {
     load_strided<true>(buf, pos, src, a.sp_src, rows_to_fill(a.ax, half), a.n_in, NL, line,
                        nvalid, jt, JT);
     tile_fft_dit<SGN>(buf, tw, a.ax, NL, line, jt, JT, half);
-    if (line < nvalid)
-      for (int p = jt; p < a.n_out; p += JT) dst[(long long)p * a.sp_dst + line] = buf[p * NL + line];
+    if (line < nvalid) {
+      V* d = dst + line + (long long)jt * a.sp_dst;
+      const long long step = (long long)JT * a.sp_dst;
+      for (int p = jt; p < a.n_out; p += JT, d += step) *d = buf[p * NL + line];
+    }
   } else {
     load_strided<false>(buf, pos, src, a.sp_src, L, a.n_in, NL, line, nvalid, jt, JT);
     tile_fft_dif<SGN>(buf, tw, a.ax, NL, line, jt, JT, 2 * a.n_out <= L);
-    if (line < nvalid)
-      for (int p = jt; p < a.n_out; p += JT)
-        dst[(long long)p * a.sp_dst + line] = buf[pos[p] * NL + line];
+    if (line < nvalid) {
+      V* d = dst + line + (long long)jt * a.sp_dst;
+      const long long step = (long long)JT * a.sp_dst;
+      for (int p = jt; p < a.n_out; p += JT, d += step) *d = buf[pos[p] * NL + line];
+    }
   }
 }
 
@@ -486,6 +493,19 @@ __global__ void __launch_bounds__(256) k_strided(SArgs<V> a) {
 // forward transform is DIT (natural output for the product), the inverse DIF
 // (natural input, permuted output read back through pos).
 // ---------------------------------------------------------------------------
+// v[0..2] = G v with G symmetric: xx, xy, xz, yy, yz, zz = g[0..5]; nxy, nxz,
+// nyz negate the off-diagonal entries.
+template <typename V>
+__device__ __forceinline__ void sym_product(V* v, const V (&g)[6], bool nxy, bool nxz, bool nyz) {
+  const V gxy = nxy ? mk(-g[1].x, -g[1].y) : g[1];
+  const V gxz = nxz ? mk(-g[2].x, -g[2].y) : g[2];
+  const V gyz = nyz ? mk(-g[4].x, -g[4].y) : g[4];
+  const V ex = v[0], ey = v[1], ez = v[2];
+  v[0] = cadd(cadd(cmul(g[0], ex), cmul(gxy, ey)), cmul(gxz, ez));
+  v[1] = cadd(cadd(cmul(gxy, ex), cmul(g[3], ey)), cmul(gyz, ez));
+  v[2] = cadd(cadd(cmul(gxz, ex), cmul(gyz, ey)), cmul(g[5], ez));
+}
+
 template <typename V, int NG>
 __global__ void __launch_bounds__(256) k_zconv(SArgs<V> a) {
   if (a.skip && a.skip[blockIdx.z * a.skip_stride]) return;
@@ -516,7 +536,9 @@ __global__ void __launch_bounds__(256) k_zconv(SArgs<V> a) {
     load_strided<true>(buf, pos, zsrc + (long long)ky * a.so_src + (long long)kx0 * 3, a.sp_src,
                        rows_to_fill(a.ax, half_in), a.n_in, NL, line, nvalid, jt, JT);
     tile_fft_dit<+1>(buf, tw, a.ax, NL, line, jt, JT, half_in);
-    // Green entries of this thread's items, all loads issued first.
+    // Green entries of this thread's items (kz', kxl), all loads issued
+    // first.  (Keeping the item indices live across the FFTs instead of
+    // recomputing them here measured slower.)
     V gr[NG][6];
     int gx[NG];  // bit0 valid, bit1 x odd-parity sign
 #pragma unroll
@@ -536,7 +558,8 @@ __global__ void __launch_bounds__(256) k_zconv(SArgs<V> a) {
         }
       }
     }
-    // Symmetric 3x3 product (produitfftmatvect3.f:76-90) at kz' and L3-kz'.
+    // Symmetric 3x3 product (produitfftmatvect3.f:76-90) at kz' and L3-kz';
+    // the off-diagonal entries carry the parity signs of their two axes.
 #pragma unroll
     for (int it = 0; it < NG; ++it) {
       if (!(gx[it] & 1)) continue;
@@ -547,23 +570,15 @@ __global__ void __launch_bounds__(256) k_zconv(SArgs<V> a) {
       for (int zz = 0; zz < nz2; ++zz) {
         const int kz = zz == 0 ? kzf : L - kzf;
         const bool szn = zz == 1;
-        V gxy = gr[it][1], gxz = gr[it][2], gyz = gr[it][4];
-        if (sxn != sy) gxy = mk(-gxy.x, -gxy.y);
-        if (sxn != szn) gxz = mk(-gxz.x, -gxz.y);
-        if (sy != szn) gyz = mk(-gyz.x, -gyz.y);
-        V* v = buf + kz * NL + kxl * 3;
-        const V ex = v[0], ey = v[1], ez = v[2];
-        v[0] = cadd(cadd(cmul(gr[it][0], ex), cmul(gxy, ey)), cmul(gxz, ez));
-        v[1] = cadd(cadd(cmul(gxy, ex), cmul(gr[it][3], ey)), cmul(gyz, ez));
-        v[2] = cadd(cadd(cmul(gxz, ex), cmul(gyz, ey)), cmul(gr[it][5], ez));
+        sym_product(buf + kz * NL + kxl * 3, gr[it], sxn != sy, sxn != szn, sy != szn);
       }
     }
     lds_sync();
     tile_fft_dif<-1>(buf, tw, a.ax, NL, line, jt, JT, half_out);
-    V* dst = zdst + (long long)ky * a.so_dst + (long long)kx0 * 3;
     if (line < nvalid) {
-      for (int p = jt; p < a.n_out; p += JT)
-        dst[(long long)p * a.sp_dst + line] = buf[pos[p] * NL + line];
+      V* d = zdst + (long long)ky * a.so_dst + (long long)kx0 * 3 + line + (long long)jt * a.sp_dst;
+      const long long step = (long long)JT * a.sp_dst;
+      for (int p = jt; p < a.n_out; p += JT, d += step) *d = buf[pos[p] * NL + line];
     }
     lds_sync();  // the next row reuses the LDS tile
   }
@@ -1026,10 +1041,10 @@ bool make_tiles(GridPlan& g, size_t lds_limit) {
   // at ~33 KB (B = 3) and the FP64 z convolution at ~52 KB (B = 5); smaller
   // tiles leave more workgroups per CU to overlap load, FFT and store phases.
   const size_t b32 = std::min<size_t>(lds_limit, 26624), b64 = std::min<size_t>(lds_limit, 33792),
-               b64c = std::min<size_t>(lds_limit, 53248);
+               b32c = std::min<size_t>(lds_limit, 26624), b64c = std::min<size_t>(lds_limit, 53248);
   for (int p = 0; p < 2; ++p) {
     const int esz = p == 0 ? 16 : 8;
-    const size_t bs = p == 0 ? b64 : b32, bc = p == 0 ? b64c : b32;
+    const size_t bs = p == 0 ? b64 : b32, bc = p == 0 ? b64c : b32c;
     g.p1[p] = tile_for(g.ax[0], 3, esz, g.ny * g.nz, bs, false);
     g.p2[p] = tile_for(g.ax[1], 3, esz, L1, bs, false);
     g.p3[p] = tile_for(g.ax[2], 3, esz, L1, bc, true);
