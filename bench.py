@@ -223,9 +223,9 @@ def main():
 
     # dominant kernel of the timed region (FP32 passes dominate mixed solves)
     names = ["P1 scatter+FFT-x", "P2 FFT-y", "P3 FFT-z.G.IFFT-z", "P4 IFFT-y", "P5 IFFT-x+gather"]
-    kern = ["ifdda::k_xpass<float2, 3, 1, 1>", "ifdda::k_strided<float2, 3, 1>",
-            "ifdda::k_zconv<float2", "ifdda::k_strided<float2, 3, -1>",
-            "ifdda::k_xpass<float2, 3, -1, 2>"]
+    zk = (f"ifdda::k_zconv_ct<float2, {sv.L[2]}," if sv.specialized_z(32) else "ifdda::k_zconv<float2,")
+    kern = ["ifdda::k_xpass<float2, 3, 1, 1>", "ifdda::k_strided<float2, 3, 1>", zk,
+            "ifdda::k_strided<float2, 3, -1>", "ifdda::k_xpass<float2, 3, -1, 2>"]
     b32 = alg_pass_bytes(s.nx, s.ny, s.nz, sv.L, nd, 8)
     b64 = alg_pass_bytes(s.nx, s.ny, s.nz, sv.L, nd, 16)
     m32 = pass_bytes(s.nx, s.ny, s.nz, sv.L, nd, 8)

@@ -54,6 +54,8 @@ extern "C" void ifdda_destroy(ifdda_ctx* c) {
   for (int i = 0; i < 3; ++i) {
     (void)hipFree(c->tw64[i]);
     (void)hipFree(c->tw32[i]);
+    (void)hipFree(c->tws64[i]);
+    (void)hipFree(c->tws32[i]);
     (void)hipFree(c->pos[i]);
   }
   (void)hipFree(c->G64);
@@ -117,6 +119,14 @@ extern "C" int ifdda_create(int nx, int ny, int nz, ifdda_ctx** out) {
       t64[e] = make_double2(std::cos(ang), std::sin(ang));
       t32[e] = make_float2((float)t64[e].x, (float)t64[e].y);
     }
+    std::vector<double> sre(L, 0.0), sim(L, 0.0);
+    stage_twiddles(g.ax[a], sre.data(), sim.data());
+    std::vector<double2> s64(L);
+    std::vector<float2> s32(L);
+    for (int e = 0; e < L; ++e) {
+      s64[e] = make_double2(sre[e], sim[e]);
+      s32[e] = make_float2((float)sre[e], (float)sim[e]);
+    }
     std::vector<int> pos(L);
     axis_positions(g.ax[a], pos.data());
     if (!hip_ok(hipMalloc(&c->pos[a], sizeof(int) * L), "hipMalloc positions") ||
@@ -127,7 +137,13 @@ extern "C" int ifdda_create(int nx, int ny, int nz, ifdda_ctx** out) {
         !hip_ok(hipMemcpy(c->tw64[a], t64.data(), sizeof(double2) * L, hipMemcpyHostToDevice),
                 "copy tw64") ||
         !hip_ok(hipMemcpy(c->tw32[a], t32.data(), sizeof(float2) * L, hipMemcpyHostToDevice),
-                "copy tw32"))
+                "copy tw32") ||
+        !hip_ok(hipMalloc(&c->tws64[a], sizeof(double2) * L), "hipMalloc tws64") ||
+        !hip_ok(hipMalloc(&c->tws32[a], sizeof(float2) * L), "hipMalloc tws32") ||
+        !hip_ok(hipMemcpy(c->tws64[a], s64.data(), sizeof(double2) * L, hipMemcpyHostToDevice),
+                "copy tws64") ||
+        !hip_ok(hipMemcpy(c->tws32[a], s32.data(), sizeof(float2) * L, hipMemcpyHostToDevice),
+                "copy tws32"))
       return fail(IFDDA_ERR_HIP);
   }
   const size_t nf = (size_t)g.F[0] * g.F[1] * g.F[2] * 6;
@@ -292,6 +308,18 @@ extern "C" int ifdda_timing_enable(ifdda_ctx* c, int cap) {
   t.tag = new int[cap];
   for (int i = 0; i < 2 * cap; ++i) CHECK_HIP(hipEventCreate(&t.ev[i]), "hipEventCreate");
   t.cap = cap;
+  return IFDDA_OK;
+}
+
+extern "C" int ifdda_use_specialized(ifdda_ctx* c, int on) {
+  if (!c) return arg_error("ifdda_use_specialized: NULL context");
+  c->specialized = on != 0;
+  return IFDDA_OK;
+}
+
+extern "C" int ifdda_specialized_z(ifdda_ctx* c, int prec, int* has) {
+  if (!c || !has || (prec != 32 && prec != 64)) return arg_error("ifdda_specialized_z: bad argument");
+  *has = has_specialized_z(c->g, prec) ? 1 : 0;
   return IFDDA_OK;
 }
 

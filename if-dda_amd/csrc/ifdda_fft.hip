@@ -376,6 +376,7 @@ template <typename V> struct SArgs {
   const V* src;
   V* dst;
   const V* tw;
+  const V* tws;     // stage-major twiddles (k_zconv_ct)
   const int* pos;   // perm[n] of the axis
   AxisPlan ax;
   int n_in, n_out;  // points loaded (rest zero) / stored
@@ -582,6 +583,209 @@ __global__ void __launch_bounds__(256) k_zconv(SArgs<V> a) {
     }
     lds_sync();  // the next row reuses the LDS tile
   }
+}
+
+// ---------------------------------------------------------------------------
+// P3 with the FFT length L and tile width B fixed at compile time, for the
+// grid sizes of the benchmark workloads (other sizes take k_zconv): the same
+// algorithm and LDS layout, with every butterfly loop of known trip count,
+// every LDS offset inside a butterfly an instruction immediate, and a
+// butterfly's twiddles read from the stage-major table at one base address.
+// ---------------------------------------------------------------------------
+struct PlanC {
+  int nst;
+  int rad[kMaxStages];
+};
+// The radix plan of make_axis_plan, evaluated at compile time.
+constexpr PlanC plan_c(int L) {
+  PlanC a{0, {}};
+  int r = L, n = 0;
+  while (r % 8 == 0 && n < kMaxStages) a.rad[n++] = 8, r /= 8;
+  if (r % 4 == 0 && n < kMaxStages) a.rad[n++] = 4, r /= 4;
+  if (r % 2 == 0 && n < kMaxStages) {
+    if (n > 0 && a.rad[n - 1] == 8 && n + 1 < kMaxStages) {
+      a.rad[n - 1] = 4;
+      a.rad[n++] = 4;
+    } else {
+      a.rad[n++] = 2;
+    }
+    r /= 2;
+  }
+  while (r % 5 == 0 && n < kMaxStages) a.rad[n++] = 5, r /= 5;
+  while (r % 3 == 0 && n < kMaxStages) a.rad[n++] = 3, r /= 3;
+  a.nst = r == 1 ? n : -1;
+  return a;
+}
+constexpr int plan_m(const PlanC& p, int s) {
+  int m = 1;
+  for (int i = 0; i < s; ++i) m *= p.rad[i];
+  return m;
+}
+
+// One stage (radix R, span M) of the in-place FFT of stage_ip, for a tile of
+// NL lines and JT threads per line.  PRUNE as in stage_ip.
+template <int L, int R, int M, int NL, int JT, int SGN, bool DIT, int PRUNE, typename V>
+__device__ __forceinline__ void ct_stage(V* buf, const V* __restrict__ tws, int line, int jt) {
+  constexpr int NB = L / R;
+  constexpr int NIT = (NB + JT - 1) / JT;
+  constexpr int STRIDE = M * NL;
+  constexpr int DQ = JT / M, DR = JT % M;
+  constexpr int DE = (DQ * M * R + DR) * NL, DWRAP = M * (R - 1) * NL;
+  constexpr int RIN = PRUNE == 1 ? R / 2 : R, ROUT = PRUNE == 2 ? R / 2 : R;
+  int j = jt % M;
+  int e0 = ((jt - j) * R + j) * NL + line;
+  int tb = (M - 1) + j * (R - 1);
+#pragma unroll
+  for (int i = 0; i < NIT; ++i) {
+    if (NB % JT == 0 || i + 1 < NIT || jt + i * JT < NB) {
+      V* p = buf + e0;
+      V a[R];
+#pragma unroll
+      for (int q = 0; q < RIN; ++q) a[q] = p[q * STRIDE];
+      if constexpr (DIT && PRUNE == 0 && M > 1) {
+#pragma unroll
+        for (int q = 1; q < R; ++q) {
+          const V w = tws[tb + q - 1];
+          a[q] = SGN > 0 ? cmul(a[q], w) : cmulc(w, a[q]);
+        }
+      }
+      if constexpr (PRUNE == 1)
+        DftHalfIn<R, SGN, V>::run(a);
+      else if constexpr (PRUNE == 2)
+        DftHalfOut<R, SGN, V>::run(a);
+      else
+        Dft<R, SGN, V>::run(a);
+      if constexpr (!DIT && PRUNE == 0 && M > 1) {
+#pragma unroll
+        for (int q = 1; q < R; ++q) {
+          const V w = tws[tb + q - 1];
+          a[q] = SGN > 0 ? cmul(a[q], w) : cmulc(w, a[q]);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < ROUT; ++q) p[q * STRIDE] = a[q];
+    }
+    j += DR;
+    e0 += DE;
+    tb += DR * (R - 1);
+    if constexpr (M > 1) {
+      if (j >= M) {
+        j -= M;
+        e0 += DWRAP;
+        tb -= M * (R - 1);
+      }
+    }
+  }
+  lds_sync();
+}
+
+// DIT over stages S..nst-1 (first stage half-input pruned when its radix is even).
+template <int L, int NL, int JT, int SGN, int S, typename V>
+__device__ __forceinline__ void ct_dit(V* buf, const V* tws, int line, int jt) {
+  constexpr PlanC P = plan_c(L);
+  if constexpr (S < P.nst) {
+    constexpr int R = P.rad[S], M = plan_m(P, S);
+    ct_stage<L, R, M, NL, JT, SGN, true, (S == 0 && R % 2 == 0) ? 1 : 0>(buf, tws, line, jt);
+    ct_dit<L, NL, JT, SGN, S + 1>(buf, tws, line, jt);
+  }
+}
+// DIF over stages S down to 0 (last stage half-output pruned when even).
+template <int L, int NL, int JT, int SGN, int S, typename V>
+__device__ __forceinline__ void ct_dif(V* buf, const V* tws, int line, int jt) {
+  constexpr PlanC P = plan_c(L);
+  if constexpr (S >= 0) {
+    constexpr int R = P.rad[S], M = plan_m(P, S);
+    ct_stage<L, R, M, NL, JT, SGN, false, (S == 0 && R % 2 == 0) ? 2 : 0>(buf, tws, line, jt);
+    ct_dif<L, NL, JT, SGN, S - 1>(buf, tws, line, jt);
+  }
+}
+
+// Green items per thread of k_zconv_ct<V, L, B>.
+constexpr int zct_ng(int L, int B) {
+  return ((L / 2 + 1) * B + (256 / (3 * B)) * 3 * B - 1) / ((256 / (3 * B)) * 3 * B);
+}
+
+// k_zconv for L3 = L, tile width B; launched with (256 / 3B) * 3B threads.
+// Requires 2 n_in <= L and 2 n_out <= L (true for every box: L >= 2n).
+template <typename V, int L, int B>
+__global__ void __launch_bounds__(256) k_zconv_ct(SArgs<V> a) {
+  constexpr int NL = 3 * B, JT = 256 / NL, NTHR = NL * JT, F3 = L / 2 + 1;
+  constexpr int NG = zct_ng(L, B);
+  constexpr PlanC P = plan_c(L);
+  static_assert(P.nst > 0 && NG >= 1 && NG <= kMaxG, "unsupported compile-time z plan");
+  constexpr int FILL = P.rad[0] % 2 == 0 ? L / 2 : L;
+  if (a.skip && a.skip[blockIdx.z * a.skip_stride]) return;
+  const V* const zsrc = a.src + blockIdx.z * a.zs_src;
+  V* const zdst = a.dst + blockIdx.z * a.zs_dst;
+  extern __shared__ double4 smem_d4[];
+  V* tws = reinterpret_cast<V*>(smem_d4);
+  int* pos = reinterpret_cast<int*>(tws + L);
+  V* buf = tile_base(tws, L);
+  const int tid = threadIdx.x;
+  const int line = tid % NL, jt = tid / NL;
+  const int lg = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int kyf = lg / gridDim.x;  // 0 .. L2/2
+  const int kx0 = (lg - kyf * gridDim.x) * B;
+  const int nvalid = min(B, a.nbx - kx0) * 3;
+
+  load_tables(tws, pos, a.tws, a.pos, L, tid, NTHR);
+  lds_sync();
+  const int nrow = (kyf == 0 || 2 * kyf == a.L2) ? 1 : 2;
+  for (int rr = 0; rr < nrow; ++rr) {
+    const int ky = rr == 0 ? kyf : a.L2 - kyf;
+    const bool sy = rr == 1;  // y odd-parity sign for the mirror row
+    load_strided<true>(buf, pos, zsrc + (long long)ky * a.so_src + (long long)kx0 * 3, a.sp_src,
+                       FILL, a.n_in, NL, line, nvalid, jt, JT);
+    ct_dit<L, NL, JT, +1, 0>(buf, tws, line, jt);
+    V gr[NG][6];
+    int gx[NG];  // bit0 valid, bit1 x odd-parity sign
+#pragma unroll
+    for (int it = 0; it < NG; ++it) {
+      const int idx = tid + it * NTHR;
+      gx[it] = 0;
+      if (idx < F3 * B) {
+        const int kzf = idx / B, kxl = idx - kzf * B;
+        const int kx = kx0 + kxl;
+        if (kx < a.nbx) {
+          int sx;
+          const int kxf = fold_idx(kx, a.L1, sx);
+          const V* g = a.G + ((long long)(kyf * F3 + kzf) * a.F1 + kxf) * 6;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) gr[it][c] = g[c];
+          gx[it] = 1 | ((sx < 0) << 1);
+        }
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < NG; ++it) {
+      if (!(gx[it] & 1)) continue;
+      const int idx = tid + it * NTHR;
+      const int kzf = idx / B, kxl = idx - kzf * B;
+      const bool sxn = gx[it] & 2;
+      sym_product(buf + kzf * NL + kxl * 3, gr[it], sxn != sy, sxn, sy);
+      if (kzf != 0 && 2 * kzf != L)
+        sym_product(buf + (L - kzf) * NL + kxl * 3, gr[it], sxn != sy, !sxn, !sy);
+    }
+    lds_sync();
+    ct_dif<L, NL, JT, -1, P.nst - 1>(buf, tws, line, jt);
+    if (line < nvalid) {
+      V* d = zdst + (long long)ky * a.so_dst + (long long)kx0 * 3 + line + (long long)jt * a.sp_dst;
+      const long long step = (long long)JT * a.sp_dst;
+      for (int p = jt; p < a.n_out; p += JT, d += step) *d = buf[pos[p] * NL + line];
+    }
+    lds_sync();  // the next row reuses the LDS tile
+  }
+}
+
+// The compile-time z convolutions built into the library: (L, B) of the
+// 40^3, 128^3, 192^3 and 200^3 grids at the default LDS budgets.
+template <typename V> using ZconvKernel = void (*)(SArgs<V>);
+template <typename V> ZconvKernel<V> zconv_ct_for(int L, int B) {
+  if (L == 200 && B == 5) return k_zconv_ct<V, 200, 5>;
+  if (L == 128 && B == 8) return k_zconv_ct<V, 128, 8>;
+  if (L == 192 && B == 5) return k_zconv_ct<V, 192, 5>;
+  if (L == 40 && B == 27) return k_zconv_ct<V, 40, 27>;
+  return nullptr;
 }
 
 // ---------------------------------------------------------------------------
@@ -998,6 +1202,13 @@ bool make_axis_plan(int L, AxisPlan* out) {
   return true;
 }
 
+bool has_specialized_z(const GridPlan& g, int prec) {
+  const int pi = prec == 32 ? 1 : 0;
+  if (2 * g.nz > g.L[2]) return false;
+  return prec == 32 ? zconv_ct_for<float2>(g.L[2], g.p3[pi].B) != nullptr
+                    : zconv_ct_for<double2>(g.L[2], g.p3[pi].B) != nullptr;
+}
+
 // Largest batch width B (lines per tile = B*NC) such that the tile fits the
 // LDS budget and, for the convolution pass, every thread holds at most kMaxG
 // Green-tensor items ((L/2+1)*B <= kMaxG*threads).
@@ -1031,6 +1242,22 @@ void axis_positions(const AxisPlan& ax, int* pos) {
       n += q * P;
     }
     pos[n] = p;
+  }
+}
+
+void stage_twiddles(const AxisPlan& ax, double* re, double* im) {
+  int m = 1;
+  for (int s = 0; s < ax.nst; ++s) {
+    const int R = ax.rad[s], tws = ax.L / (m * R);
+    for (int j = 0; j < m; ++j)
+      for (int q = 1; q < R; ++q) {
+        // the same exponent e = j q tws (< L) as the natural table tw[e]
+        const double ang = 6.283185307179586476925286766559 * (double)(j * q * tws) / (double)ax.L;
+        const int i = m - 1 + j * (R - 1) + q - 1;
+        re[i] = std::cos(ang);
+        im[i] = std::sin(ang);
+      }
+    m *= R;
   }
 }
 
@@ -1153,10 +1380,13 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.F3 = g.F[2];
     a.L1 = L1;
     a.L2 = L2;
+    a.tws = m.tws[2];
     const int items = g.F[2] * t.B;
     const int ng = (items + t.threads - 1) / t.threads;
-    auto k = ng <= 1 ? k_zconv<V, 1> : ng == 2 ? k_zconv<V, 2> : ng == 3 ? k_zconv<V, 3>
-                                                                        : k_zconv<V, kMaxG>;
+    ZconvKernel<V> k = m.specialized && 2 * nz <= g.L[2] ? zconv_ct_for<V>(g.L[2], t.B) : nullptr;
+    if (!k)
+      k = ng <= 1 ? k_zconv<V, 1> : ng == 2 ? k_zconv<V, 2> : ng == 3 ? k_zconv<V, 3>
+                                                                     : k_zconv<V, kMaxG>;
     if (!allow_lds(k, t.lds)) return false;
     if (tm) tm->begin(pi * 5 + 2, s);
     hipLaunchKernelGGL(k, dim3((L1 + t.B - 1) / t.B, L2 / 2 + 1, R), dim3(t.threads), t.lds, s, a);

@@ -34,11 +34,13 @@ template <typename V> struct MatvecDev {
   V* B;               // nz*L2*L1*3 scratch (after y pass)
   const V* G;         // folded Green tensor, F2*F3*F1*6
   const V* tw[3];     // twiddles exp(+2 pi i e/L) per axis
+  const V* tws[3];    // stage-major twiddles per axis (stage_twiddles)
   const int* pos[3];  // in-place FFT positions per axis (axis_positions)
   const int32_t* tab; // box -> 1-based dipole, or null (dense box)
   const V* alpha;     // per dipole, see alpha_mode
   int alpha_mode;
   PassTimer* timer = nullptr;
+  bool specialized = true;  // use compile-time kernels where built
   // batch of right-hand sides: nrhs vectors vstride elements apart, A and B
   // sized for nrhs, skip flags skip_stride ints apart
   int nrhs = 1;
@@ -79,8 +81,14 @@ int smooth235_at_least(int n);
 bool make_axis_plan(int L, AxisPlan* out);
 // pos[n]: LDS position of signal index n for the in-place mixed-radix FFT.
 void axis_positions(const AxisPlan& ax, int* pos);
+// Stage-major twiddles: stage s (m_s = r_1...r_{s-1}) holds exp(+2 pi i j q /
+// (m_s r_s)) for j < m_s, q = 1..r_s-1 at index m_s - 1 + j (r_s - 1) + q - 1;
+// L - 1 values in all (the stage sizes telescope).
+void stage_twiddles(const AxisPlan& ax, double* re, double* im);
 // false when an axis is too long for the register-staged FFT tiles
 bool make_tiles(GridPlan& g, size_t lds_limit);
+// whether the z pass of this grid has a compile-time kernel (prec 64 or 32)
+bool has_specialized_z(const GridPlan& g, int prec);
 
 // y = b - (1/M) IFFT[G . FFT[scatter(alpha . x)]]  (all device pointers).
 // skip: optional device flag; when it reads non-zero the passes do nothing.
@@ -110,6 +118,8 @@ struct ifdda_ctx {
   int device = 0;
   double2* tw64[3] = {nullptr, nullptr, nullptr};
   float2* tw32[3] = {nullptr, nullptr, nullptr};
+  double2* tws64[3] = {nullptr, nullptr, nullptr};  // stage-major twiddles per axis
+  float2* tws32[3] = {nullptr, nullptr, nullptr};
   int* pos[3] = {nullptr, nullptr, nullptr};  // in-place FFT positions per axis
   double2* G64 = nullptr;  // folded Green tensor, F2*F3*F1*6
   float2* G32 = nullptr;
@@ -133,17 +143,19 @@ struct ifdda_ctx {
   uint64_t tab_hash = 0;
   bool tab_dense = false;
   ifdda::PassTimer timer;  // enabled by ifdda_timing_enable
+  bool specialized = true;  // ifdda_use_specialized
 
   ifdda::MatvecDev<double2> mv64() const {
     ifdda::MatvecDev<double2> m;
     m.A = (double2*)scrA;
     m.B = (double2*)scrB;
     m.G = G64;
-    for (int i = 0; i < 3; ++i) m.tw[i] = tw64[i], m.pos[i] = pos[i];
+    for (int i = 0; i < 3; ++i) m.tw[i] = tw64[i], m.tws[i] = tws64[i], m.pos[i] = pos[i];
     m.tab = tab;
     m.alpha = al64;
     m.alpha_mode = alpha_mode;
     m.timer = timer.cap ? const_cast<ifdda::PassTimer*>(&timer) : nullptr;
+    m.specialized = specialized;
     return m;
   }
   ifdda::MatvecDev<float2> mv32() const {
@@ -151,11 +163,12 @@ struct ifdda_ctx {
     m.A = (float2*)scrA;
     m.B = (float2*)scrB;
     m.G = G32;
-    for (int i = 0; i < 3; ++i) m.tw[i] = tw32[i], m.pos[i] = pos[i];
+    for (int i = 0; i < 3; ++i) m.tw[i] = tw32[i], m.tws[i] = tws32[i], m.pos[i] = pos[i];
     m.tab = tab;
     m.alpha = al32;
     m.alpha_mode = alpha_mode;
     m.timer = timer.cap ? const_cast<ifdda::PassTimer*>(&timer) : nullptr;
+    m.specialized = specialized;
     return m;
   }
 };

@@ -79,6 +79,8 @@ def lib():
     L.ifdda_solve_trace_enable.argtypes = [vp, ip]
     L.ifdda_timing_enable.argtypes = [vp, ip]
     L.ifdda_timing_passes.argtypes = [vp, ip]
+    L.ifdda_use_specialized.argtypes = [vp, ip]
+    L.ifdda_specialized_z.argtypes = [vp, ip, C.POINTER(C.c_int)]
     L.ifdda_timing_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     _lib = L
     return L
@@ -251,6 +253,16 @@ class Solver:
     def timing_enable(self, cap: int):
         """Record HIP events around each matvec pass (up to cap passes)."""
         _check(lib().ifdda_timing_enable(self.h, int(cap)), "ifdda_timing_enable")
+
+    def use_specialized(self, on: bool):
+        """Select compile-time (default) or generic z-pass kernels."""
+        _check(lib().ifdda_use_specialized(self.h, int(bool(on))), "ifdda_use_specialized")
+
+    def specialized_z(self, prec: int = 64) -> bool:
+        """Whether this grid's z pass has a compile-time kernel."""
+        has = C.c_int(0)
+        _check(lib().ifdda_specialized_z(self.h, int(prec), C.byref(has)), "ifdda_specialized_z")
+        return bool(has.value)
 
     def timing_passes(self, mask: int):
         """Bracket only the passes whose bit is set (bit p = P(p+1))."""

@@ -147,6 +147,14 @@ int ifdda_timing_read(ifdda_ctx *ctx, double ms_sum[10], int count[10]);
  * brackets only the pass it reports. */
 int ifdda_timing_passes(ifdda_ctx *ctx, int mask);
 
+/* Kernel selection (no counterpart in the reference): on = 1 (default)
+ * runs the z pass with the kernels compiled for a fixed FFT length where the
+ * library has one for this grid, 0 always runs the generic kernels.  Both
+ * compute the same products; tests compare them. */
+int ifdda_use_specialized(ifdda_ctx *ctx, int on);
+/* *has = 1 when this grid's z pass has a compile-time kernel for prec 64/32. */
+int ifdda_specialized_z(ifdda_ctx *ctx, int prec, int *has);
+
 /* Per-iteration scalars of the last solve: rows of six complex values
  * (GPBICG: alpha, beta, eta, dzeta, R0RN; QMRBICGSTAB: alpha, beta, omega,
  * tau, eta; TFQMR: alpha, beta, rho, tau, eta; then the residual), recorded

@@ -179,3 +179,34 @@ def test_fortran_dropin_matvec(gpu):
                           ri(nx2), ri(ny2), ri(nx2 * ny2), ri(nz2), P(f["FFX"].copy()), P(out),
                           C.byref(C.c_int64(0)), C.byref(C.c_int64(0)))
     assert relerr(out, f["out3"], np.abs(f["out3"] - f["FFB"]).max()) < 1e-12
+
+
+@pytest.mark.parametrize("n", [20, 64, 96, 100])
+def test_specialized_z_pass_equals_generic(gpu, n):
+    """The z-pass kernels compiled for a fixed FFT length (the 40^3, 128^3,
+    192^3 and 200^3 grids) give the generic kernel's product: FP64 to 1e-14,
+    FP32 to 1e-6 relative to |T alpha x|."""
+    import torch
+    tab, x, _ = _case(n, n, n, 10 + n, frac=0.5)
+    nd = int(tab.max())
+    rng = np.random.default_rng(n)
+    alpha = (1 + rng.random(nd)) * 1e-24 + 1j * rng.random(nd) * 1e-26
+    with gpu.Solver(n, n, n) as sv:
+        print(f"specialized z pass {n}^3 L={sv.L}", flush=True)
+        sv.greencalculfft(D, K0)
+        sv.set_dipoles(tab)
+        sv.set_polarizability(alpha)
+        for prec, tol in ((64, 1e-14), (32, 1e-6)):
+            assert sv.specialized_z(prec)
+            cdt, rdt = (np.complex128, np.float64) if prec == 64 else (np.complex64, np.float32)
+            xd = torch.from_numpy(x.astype(cdt).view(rdt).copy()).cuda()
+            outs = []
+            for on in (True, False):
+                sv.use_specialized(on)
+                yd = torch.zeros_like(xd)
+                sv.matvec_device(_dev(xd), _dev(xd), _dev(yd), prec=prec)
+                torch.cuda.synchronize()
+                y = yd.cpu().numpy().view(cdt).astype(np.complex128)
+                outs.append(y - x.astype(cdt).astype(np.complex128))  # -T alpha x
+            sv.use_specialized(True)
+            assert relerr(outs[0], outs[1]) < tol, (prec, relerr(outs[0], outs[1]))
