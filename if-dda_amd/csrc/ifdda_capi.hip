@@ -300,9 +300,10 @@ extern "C" int ifdda_timing_enable(ifdda_ctx* c, int cap) {
     delete[] t.ev;
     delete[] t.tag;
   }
-  const int mask = t.mask;
+  const int mask = t.mask, every = t.every;
   t = PassTimer();
   t.mask = mask;
+  t.every = every;
   if (cap == 0) return IFDDA_OK;
   t.ev = new hipEvent_t[2 * (size_t)cap];
   t.tag = new int[cap];
@@ -323,9 +324,11 @@ extern "C" int ifdda_specialized_z(ifdda_ctx* c, int prec, int* has) {
   return IFDDA_OK;
 }
 
-extern "C" int ifdda_timing_passes(ifdda_ctx* c, int mask) {
-  if (!c || mask < 0 || mask > 31) return arg_error("ifdda_timing_passes: bad argument");
+extern "C" int ifdda_timing_passes(ifdda_ctx* c, int mask, int every) {
+  if (!c || mask < 0 || mask > 31 || every < 1) return arg_error("ifdda_timing_passes: bad argument");
   c->timer.mask = mask;
+  c->timer.every = every;
+  c->timer.seen = 0;
   return IFDDA_OK;
 }
 
@@ -383,11 +386,21 @@ extern "C" int ifdda_set_dipoles(ifdda_ctx* c, const int32_t* tabdip, int ndipol
       seen[t] = 1;
     }
   }
-  if (!c->tab) CHECK_HIP(hipMalloc(&c->tab, sizeof(int32_t) * N), "hipMalloc tabdip");
-  CHECK_HIP(hipMemcpy(c->tab, tabdip, sizeof(int32_t) * N, hipMemcpyHostToDevice), "copy tabdip");
+  // Every cell holding dipole cell+1 is the dense box of
+  // produitfftmatvectopt.f:43-44: the passes then index dipoles by cell and
+  // read no map.
+  bool identity = (size_t)ndipole == N;
+  for (size_t i = 0; identity && i < N; ++i) identity = tabdip[i] == (int32_t)(i + 1);
+  if (identity) {
+    (void)hipFree(c->tab);
+    c->tab = nullptr;
+  } else {
+    if (!c->tab) CHECK_HIP(hipMalloc(&c->tab, sizeof(int32_t) * N), "hipMalloc tabdip");
+    CHECK_HIP(hipMemcpy(c->tab, tabdip, sizeof(int32_t) * N, hipMemcpyHostToDevice), "copy tabdip");
+  }
   c->nd = ndipole;
   c->have_dipoles = true;
-  c->tab_dense = false;
+  c->tab_dense = false;  // a caller's map (sync_tab compares its hash)
   return IFDDA_OK;
 }
 

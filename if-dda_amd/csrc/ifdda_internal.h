@@ -51,17 +51,20 @@ template <typename V> struct MatvecDev {
 // HIP-event timing of individual matvec passes on their own stream.  Slot
 // tag = prec_index * 5 + pass (pass 0..4 = P1..P5).  Events come from a pool
 // created when timing is enabled; recording stops when the pool is used up.
-// Only the passes whose bit is set in mask are bracketed (each event pair
-// adds a few microseconds between launches).
+// Only the passes whose bit is set in mask are bracketed, and of those only
+// every every-th launch (each event pair adds a few microseconds between
+// launches).
 struct PassTimer {
   int cap = 0;
   int used = 0;
   int mask = 31;
+  int every = 1;
+  long long seen = 0;  // launches of masked passes so far
   bool open = false;
   hipEvent_t* ev = nullptr;  // 2 * cap events
   int* tag = nullptr;
   void begin(int t, hipStream_t s) {
-    open = used < cap && ((mask >> (t % 5)) & 1);
+    open = used < cap && ((mask >> (t % 5)) & 1) && (seen++ % every == 0);
     if (open) {
       tag[used] = t;
       (void)hipEventRecord(ev[2 * used], s);

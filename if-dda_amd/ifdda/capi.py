@@ -78,7 +78,7 @@ def lib():
     L.ifdda_solve_trace.argtypes = [vp, ip, vp, C.POINTER(C.c_int)]
     L.ifdda_solve_trace_enable.argtypes = [vp, ip]
     L.ifdda_timing_enable.argtypes = [vp, ip]
-    L.ifdda_timing_passes.argtypes = [vp, ip]
+    L.ifdda_timing_passes.argtypes = [vp, ip, ip]
     L.ifdda_use_specialized.argtypes = [vp, ip]
     L.ifdda_specialized_z.argtypes = [vp, ip, C.POINTER(C.c_int)]
     L.ifdda_timing_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
@@ -264,9 +264,10 @@ class Solver:
         _check(lib().ifdda_specialized_z(self.h, int(prec), C.byref(has)), "ifdda_specialized_z")
         return bool(has.value)
 
-    def timing_passes(self, mask: int):
-        """Bracket only the passes whose bit is set (bit p = P(p+1))."""
-        _check(lib().ifdda_timing_passes(self.h, int(mask)), "ifdda_timing_passes")
+    def timing_passes(self, mask: int, every: int = 1):
+        """Bracket only the passes whose bit is set (bit p = P(p+1)), and of
+        those every every-th launch."""
+        _check(lib().ifdda_timing_passes(self.h, int(mask), int(every)), "ifdda_timing_passes")
 
     def timing_read(self):
         """Returns (ms_sum[2,5], count[2,5]): [FP64/FP32][P1..P5]; resets."""

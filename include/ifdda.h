@@ -142,10 +142,11 @@ int ifdda_solve_multi_device(ifdda_ctx *ctx, const ifdda_solve_opts *opts, int n
  * slot [precision (0: FP64, 1: FP32) * 5 + pass (0..4 = P1..P5)] and resets. */
 int ifdda_timing_enable(ifdda_ctx *ctx, int cap);
 int ifdda_timing_read(ifdda_ctx *ctx, double ms_sum[10], int count[10]);
-/* Which passes are bracketed (bit p = pass p+1; default 31 = all).  Each
- * event pair adds a few microseconds between launches, so a timed solve
- * brackets only the pass it reports. */
-int ifdda_timing_passes(ifdda_ctx *ctx, int mask);
+/* Which passes are bracketed (bit p = pass p+1; default 31 = all), and only
+ * every every-th launch of them (default 1).  Each event pair adds a few
+ * microseconds between launches, so a timed solve samples only the pass it
+ * reports. */
+int ifdda_timing_passes(ifdda_ctx *ctx, int mask, int every);
 
 /* Kernel selection (no counterpart in the reference): on = 1 (default)
  * runs the z pass with the kernels compiled for a fixed FFT length where the

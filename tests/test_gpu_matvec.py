@@ -79,6 +79,27 @@ def test_matvec_vs_oracle_shapes(gpu, oracle, shape):
     assert relerr(y, ref, max(np.abs(ref - b).max(), 1e-300)) < 1e-12
 
 
+@pytest.mark.parametrize("shape", [(9, 7, 6), (20, 20, 20)])
+def test_identity_map_runs_as_dense_box(gpu, oracle, shape):
+    """A Tabdip mapping every cell to dipole cell+1 is the dense box of
+    produitfftmatvectopt.f: the device drops the map; the product equals the
+    oracle's with that map, and the dense (map-less) call."""
+    nx, ny, nz = shape
+    tab, x, b = _case(nx, ny, nz, 21, frac=1.0)
+    assert (tab == np.arange(1, tab.size + 1)).all()
+    G = oracle.green_fft(nx, ny, nz, (2 * nx, 2 * ny, 2 * nz), D, K0)
+    ref = oracle.matvec(G, tab, nx, ny, nz, x, b)
+    with gpu.Solver(nx, ny, nz) as sv:
+        sv.greencalculfft(D, K0)
+        sv.set_dipoles(tab)
+        y = sv.produitfftmatvect3(x, b)
+        sv.set_dipoles(None)
+        yd = sv.produitfftmatvect3(x, b)
+    scale = np.abs(ref - b).max()
+    assert relerr(y, ref, scale) < 1e-12
+    assert relerr(yd, y, scale) < 1e-15
+
+
 def test_matvec_direct_sum(gpu, oracle):
     """The FFT convolution equals the O(Nd^2) dipole sum."""
     nx, ny, nz = 7, 6, 5
