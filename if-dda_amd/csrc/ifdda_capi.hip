@@ -67,6 +67,7 @@ extern "C" void ifdda_destroy(ifdda_ctx* c) {
   (void)hipFree(c->scrB);
   (void)hipFree(c->hx);
   (void)hipFree(c->hb);
+  (void)hipFree(c->ffbuf);
   ifdda_timing_enable(c, 0);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -783,4 +784,121 @@ extern "C" void ifdda_inverserigopt(
   driver_entry(G, nullptr, *nx, *ny, *nz, *nx2, *ny2, *nz2, nm, *nbsphere, FF, FF0, FFloc,
                polarisa, methodeit, *tol, tol1, nloop, ncompte, nstop, infostr, len_methodeit,
                len_infostr);
+}
+
+// ------------------------------------------------- far field, reference-shaped
+
+namespace ifdda {
+std::string farfield_geometry(int nfft2d, double k0, double d, int* imax, double* dk);
+int farfield_dev(ifdda_ctx* c, int R, const double2* FF, const int32_t* tab, long long nd,
+                 int nfft2d, double k0, double d, double x0, double y0, double z0, double2* Epos,
+                 double2* Eneg, int* imax_out, double* dk_out, hipStream_t s);
+int gc_dev(ifdda_ctx* c, int R, const double2* Epos, const double2* Eneg, int imax, double dk,
+           double k0, const double* I0, const double* theta, const double* phi, double* out,
+           hipStream_t s);
+}  // namespace ifdda
+
+// diffractefft2d.f:1-3.  FF is box-ordered (the reference forces the dense
+// box for this method, cdmlib.f:630); xs, ys, zs per cell; Ediffkzpos/neg are
+// the caller's (nfft2d, nfft2d, 3) arrays (only the (2 imax + 1)^2 corner is
+// filled, the rest zeroed, as the reference).  The Eloin scratch arrays and
+// plan handles are accepted and not used.
+extern "C" void ifdda_diffractefft2d(
+    const int32_t* nx, const int32_t* ny, const int32_t* nz, const int32_t* nxm,
+    const int32_t* nym, const int32_t* nzm, const int32_t* nfft2d, int32_t* tabfft2,
+    const double* k0, const double* xs, const double* ys, const double* zs,
+    const double* aretecube, double* Eloinx, double* Eloiny, double* Eloinz, const double* FF,
+    int32_t* imax, double* deltakx, double* deltaky, double* Ediffkzpos, double* Ediffkzneg,
+    const int64_t* plan2f, const int64_t* plan2b, int32_t* nstop, char* infostr,
+    size_t len_infostr) {
+  (void)nxm; (void)nym; (void)nzm; (void)Eloinx; (void)Eloiny; (void)Eloinz; (void)plan2f;
+  (void)plan2b;
+  const int N = *nfft2d;
+  const int h = N / 2;
+  for (int i = 1; i <= N; ++i) tabfft2[i - 1] = i - h - 1 >= 0 ? i - h : h + i;
+  const size_t full = (size_t)N * N * 3;
+  std::memset(Ediffkzpos, 0, sizeof(double) * 2 * full);
+  std::memset(Ediffkzneg, 0, sizeof(double) * 2 * full);
+  int im = 0;
+  double dk = 0.0;
+  const std::string msg = farfield_geometry(N, *k0, *aretecube, &im, &dk);
+  *deltakx = *deltaky = dk;
+  *imax = im;
+  if (!msg.empty()) {
+    *nstop = 1;
+    fill_info(infostr, len_infostr, msg.c_str());
+    return;
+  }
+  ifdda_ctx* c = ctx_for(*nx, *ny, *nz);
+  const long long ncell = (long long)(*nx) * (*ny) * (*nz);
+  const int n = 2 * im + 1;
+  double2 *dF = nullptr, *dE = nullptr;
+  bool ok = c && hip_ok(hipMalloc(&dF, sizeof(double2) * 3 * ncell), "hipMalloc FF") &&
+            hip_ok(hipMalloc(&dE, sizeof(double2) * 6 * (size_t)n * n), "hipMalloc Ediff") &&
+            hip_ok(hipMemcpy(dF, FF, sizeof(double2) * 3 * ncell, hipMemcpyHostToDevice), "copy FF");
+  if (ok)
+    ok = farfield_dev(c, 1, dF, nullptr, ncell, N, *k0, *aretecube, xs[0], ys[0], zs[0], dE,
+                      dE + 3 * (size_t)n * n, &im, &dk, c->stream) == IFDDA_OK;
+  std::vector<double2> hE(ok ? 6 * (size_t)n * n : 0);
+  if (ok)
+    ok = hip_ok(hipMemcpy(hE.data(), dE, sizeof(double2) * hE.size(), hipMemcpyDeviceToHost),
+                "copy Ediff");
+  (void)hipFree(dF);
+  (void)hipFree(dE);
+  if (!ok) {
+    report("diffractefft2d");
+    *nstop = 1;
+    fill_info(infostr, len_infostr, "GPU far field failed");
+    return;
+  }
+  // device [c][jj][ii] compact -> Fortran E(ii, jj, c) with leading dimension N
+  for (int h2 = 0; h2 < 2; ++h2) {
+    double* E = h2 == 0 ? Ediffkzpos : Ediffkzneg;
+    const double2* src = hE.data() + (size_t)h2 * 3 * n * n;
+    for (int cc = 0; cc < 3; ++cc)
+      for (int jj = 0; jj < n; ++jj)
+        for (int ii = 0; ii < n; ++ii) {
+          const double2 v = src[((size_t)cc * n + jj) * n + ii];
+          const size_t o = 2 * ((size_t)ii + (size_t)N * jj + (size_t)N * N * cc);
+          E[o] = v.x;
+          E[o + 1] = v.y;
+        }
+  }
+}
+
+// computegcfft2d.f:1-3 on the caller's (nfft2d, nfft2d, 3) far-field arrays.
+extern "C" void ifdda_computegcfft2d(const int32_t* imax, const double* deltakx,
+                                     const double* deltaky, const double* k0, const double* I0,
+                                     const double* theta, const double* phi,
+                                     const int32_t* nfft2d, const double* Ediffkzpos,
+                                     const double* Ediffkzneg, double* gasym, double* Cscai) {
+  (void)deltaky;  // the reference's grid is square (deltaky = deltakx)
+  const int N = *nfft2d, n = 2 * (*imax) + 1;
+  std::vector<double2> h(6 * (size_t)n * n);
+  for (int h2 = 0; h2 < 2; ++h2) {
+    const double* E = h2 == 0 ? Ediffkzpos : Ediffkzneg;
+    for (int cc = 0; cc < 3; ++cc)
+      for (int jj = 0; jj < n; ++jj)
+        for (int ii = 0; ii < n; ++ii) {
+          const size_t o = 2 * ((size_t)ii + (size_t)N * jj + (size_t)N * N * cc);
+          h[(size_t)h2 * 3 * n * n + ((size_t)cc * n + jj) * n + ii] = make_double2(E[o], E[o + 1]);
+        }
+  }
+  *gasym = 0.0;
+  *Cscai = 0.0;
+  ifdda_ctx* c = ctx_for(1, 1, 1);
+  double2* dE = nullptr;
+  double out[2] = {0.0, 0.0};
+  bool ok = c && hip_ok(hipMalloc(&dE, sizeof(double2) * h.size()), "hipMalloc Ediff") &&
+            hip_ok(hipMemcpy(dE, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice),
+                   "copy Ediff") &&
+            gc_dev(c, 1, dE, dE + 3 * (size_t)n * n, *imax, *deltakx, *k0, I0, theta, phi, out,
+                   c->stream) == IFDDA_OK;
+  (void)hipFree(dE);
+  if (!ok) {
+    report("computegcfft2d");
+    return;
+  }
+  *Cscai = out[0];
+  *gasym = out[1];
 }

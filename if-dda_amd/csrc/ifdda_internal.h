@@ -140,6 +140,8 @@ struct ifdda_ctx {
   double2* hb = nullptr;
   size_t hcap = 0;        // complex elements in hx / hb
   void* solver = nullptr; // Krylov workspace (ifdda_krylov.hip)
+  void* ffbuf = nullptr;  // far-field / cross-section workspace (ifdda_farfield.hip)
+  size_t ffcap = 0;
   // provenance of the device Green tensor for the reference-shaped entries
   const void* green_host = nullptr;
   double green_sample[6][2 * 16] = {};

@@ -90,6 +90,25 @@ module ifdda_capi
       integer(c_int64_t) :: planf, planb
       integer(c_size_t), value :: lmeth, linfo
     end subroutine
+    subroutine c_diffractefft2d(nx, ny, nz, nxm, nym, nzm, nfft2d, tabfft2, k0, xs, ys, zs, &
+         aretecube, eloinx, eloiny, eloinz, ff, imax, deltakx, deltaky, ediffkzpos, ediffkzneg, &
+         plan2f, plan2b, nstop, infostr, linfo) bind(C, name="ifdda_diffractefft2d")
+      import :: c_double, c_double_complex, c_int32_t, c_int64_t, c_char, c_size_t
+      integer(c_int32_t) :: nx, ny, nz, nxm, nym, nzm, nfft2d, tabfft2(*), imax, nstop
+      real(c_double) :: k0, xs(*), ys(*), zs(*), aretecube, deltakx, deltaky
+      complex(c_double_complex) :: eloinx(*), eloiny(*), eloinz(*), ff(*), ediffkzpos(*), &
+           ediffkzneg(*)
+      integer(c_int64_t) :: plan2f, plan2b
+      character(kind=c_char) :: infostr(*)
+      integer(c_size_t), value :: linfo
+    end subroutine
+    subroutine c_computegcfft2d(imax, deltakx, deltaky, k0, i0, theta, phi, nfft2d, ediffkzpos, &
+         ediffkzneg, gasym, cscai) bind(C, name="ifdda_computegcfft2d")
+      import :: c_double, c_double_complex, c_int32_t
+      integer(c_int32_t) :: imax, nfft2d
+      real(c_double) :: deltakx, deltaky, k0, i0, theta, phi, gasym, cscai
+      complex(c_double_complex) :: ediffkzpos(*), ediffkzneg(*)
+    end subroutine
   end interface
 end module ifdda_capi
 
@@ -199,3 +218,32 @@ subroutine inverserigopt(FFTTENSORxx, FFTTENSORxy, FFTTENSORxz, FFTTENSORyy, FFT
        methodeit, tol, tol1, nloop, ncompte, planf, planb, nstop, infostr, &
        int(len(methodeit), c_size_t), int(len(infostr), c_size_t))
 end subroutine inverserigopt
+
+subroutine diffractefft2d(nx, ny, nz, nxm, nym, nzm, nfft2d, tabfft2, k0, xs, ys, zs, aretecube, &
+     Eloinx, Eloiny, Eloinz, FF, imax, deltakx, deltaky, Ediffkzpos, Ediffkzneg, plan2f, plan2b, &
+     nstop, infostr)
+  use ifdda_capi
+  implicit none
+  integer :: nx, ny, nz, nxm, nym, nzm, nfft2d, nstop, imax
+  integer :: tabfft2(nfft2d)
+  double precision :: k0, aretecube, deltakx, deltaky
+  double precision, dimension(nxm*nym*nzm) :: xs, ys, zs
+  double complex, dimension(nfft2d*nfft2d) :: Eloinx, Eloiny, Eloinz
+  double complex :: FF(3*nxm*nym*nzm), Ediffkzpos(nfft2d, nfft2d, 3), Ediffkzneg(nfft2d, nfft2d, 3)
+  integer*8 :: plan2f, plan2b
+  character(64) :: infostr
+  call c_diffractefft2d(nx, ny, nz, nxm, nym, nzm, nfft2d, tabfft2, k0, xs, ys, zs, aretecube, &
+       Eloinx, Eloiny, Eloinz, FF, imax, deltakx, deltaky, Ediffkzpos, Ediffkzneg, plan2f, &
+       plan2b, nstop, infostr, int(len(infostr), c_size_t))
+end subroutine diffractefft2d
+
+subroutine computegcfft2d(imax, deltakx, deltaky, k0, I0, theta, phi, nfft2d, Ediffkzpos, &
+     Ediffkzneg, gasym, Cscai)
+  use ifdda_capi
+  implicit none
+  integer :: imax, nfft2d
+  double precision :: deltakx, deltaky, k0, I0, theta, phi, gasym, Cscai
+  double complex :: Ediffkzpos(nfft2d, nfft2d, 3), Ediffkzneg(nfft2d, nfft2d, 3)
+  call c_computegcfft2d(imax, deltakx, deltaky, k0, I0, theta, phi, nfft2d, Ediffkzpos, &
+       Ediffkzneg, gasym, Cscai)
+end subroutine computegcfft2d

@@ -80,6 +80,12 @@ def lib():
     L.ifdda_timing_enable.argtypes = [vp, ip]
     L.ifdda_timing_passes.argtypes = [vp, ip, ip]
     L.ifdda_use_specialized.argtypes = [vp, ip]
+    pd = C.POINTER(C.c_double)
+    L.ifdda_cross_sections_device.argtypes = [vp, ip, vp, vp, vp, dp, pd, pd, vp]
+    L.ifdda_farfield_device.argtypes = [vp, ip, vp, ip, dp, dp, dp, dp, dp, vp, vp,
+                                        C.POINTER(C.c_int), pd, vp]
+    L.ifdda_farfield_size.argtypes = [ip, dp, dp, C.POINTER(C.c_int), pd]
+    L.ifdda_computegcfft2d_device.argtypes = [vp, ip, vp, vp, ip, dp, dp, pd, pd, pd, pd, vp]
     L.ifdda_specialized_z.argtypes = [vp, ip, C.POINTER(C.c_int)]
     L.ifdda_timing_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     _lib = L
@@ -253,6 +259,54 @@ class Solver:
     def timing_enable(self, cap: int):
         """Record HIP events around each matvec pass (up to cap passes)."""
         _check(lib().ifdda_timing_enable(self.h, int(cap)), "ifdda_timing_enable")
+
+    # -- outputs from solved fields (cdmlib.f cross sections, diffractefft2d) --
+    def cross_sections_device(self, R, E0_ptr, FF_ptr, FFloc_ptr, k0, I0, stream=None):
+        """Returns [R, 3] (Cext, Cabs, Csca); I0 per RHS (|E0|^2)."""
+        i0 = np.ascontiguousarray(np.broadcast_to(np.asarray(I0, np.float64), (R,)))
+        out = np.zeros(3 * R)
+        D = C.POINTER(C.c_double)
+        _check(lib().ifdda_cross_sections_device(self.h, int(R), C.c_void_p(E0_ptr), C.c_void_p(FF_ptr),
+                                                 C.c_void_p(FFloc_ptr), float(k0),
+                                                 i0.ctypes.data_as(D), out.ctypes.data_as(D),
+                                                 C.c_void_p(stream) if stream else None),
+               "ifdda_cross_sections_device")
+        return out.reshape(R, 3)
+
+    @staticmethod
+    def farfield_size(nfft2d, k0, d):
+        im = C.c_int(0)
+        dk = C.c_double(0.0)
+        _check(lib().ifdda_farfield_size(int(nfft2d), float(k0), float(d), C.byref(im), C.byref(dk)),
+               "ifdda_farfield_size")
+        return im.value, dk.value
+
+    def farfield_device(self, R, FF_ptr, nfft2d, k0, d, origin, Epos_ptr, Eneg_ptr, stream=None):
+        """Far field into device buffers of R * 3 * (2 imax + 1)^2 complex128
+        each; returns (imax, deltak)."""
+        im = C.c_int(0)
+        dk = C.c_double(0.0)
+        x0, y0, z0 = (float(v) for v in origin)
+        _check(lib().ifdda_farfield_device(self.h, int(R), C.c_void_p(FF_ptr), int(nfft2d), float(k0),
+                                           float(d), x0, y0, z0, C.c_void_p(Epos_ptr),
+                                           C.c_void_p(Eneg_ptr), C.byref(im), C.byref(dk),
+                                           C.c_void_p(stream) if stream else None),
+               "ifdda_farfield_device")
+        return im.value, dk.value
+
+    def computegcfft2d_device(self, R, Epos_ptr, Eneg_ptr, imax, dk, k0, I0, theta, phi, stream=None):
+        """Returns [R, 2] (Csca, g)."""
+        D = C.POINTER(C.c_double)
+        arr = lambda v: np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), (R,)))
+        i0, th, ph = arr(I0), arr(theta), arr(phi)
+        out = np.zeros(2 * R)
+        _check(lib().ifdda_computegcfft2d_device(self.h, int(R), C.c_void_p(Epos_ptr), C.c_void_p(Eneg_ptr),
+                                                 int(imax), float(dk), float(k0), i0.ctypes.data_as(D),
+                                                 th.ctypes.data_as(D), ph.ctypes.data_as(D),
+                                                 out.ctypes.data_as(D),
+                                                 C.c_void_p(stream) if stream else None),
+               "ifdda_computegcfft2d_device")
+        return out.reshape(R, 2)
 
     def use_specialized(self, on: bool):
         """Select compile-time (default) or generic z-pass kernels."""

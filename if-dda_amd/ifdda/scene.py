@@ -31,6 +31,7 @@ class Scene:
     tabdip: np.ndarray       # int32 [nx*ny*nz], 1-based dipole or 0
     pos: np.ndarray          # float64 [nd, 3] dipole positions [m]
     eps: np.ndarray          # complex128 [nd] permittivity
+    origin: tuple = (0.0, 0.0, 0.0)  # centre of box cell (1,1,1) [m]
 
     @property
     def nd(self) -> int:
@@ -55,7 +56,8 @@ def sphere(nnnr: int, radius: float, eps: complex) -> Scene:
     Z, Y, X = np.meshgrid(c, c, c, indexing="ij")
     inside = np.sqrt(X * X + Y * Y + Z * Z) < radius
     tab, pos = _lattice(c, c, c, inside)
-    return Scene(nnnr, nnnr, nnnr, d, tab, pos, np.full(pos.shape[0], eps, np.complex128))
+    return Scene(nnnr, nnnr, nnnr, d, tab, pos, np.full(pos.shape[0], eps, np.complex128),
+                 (c[0], c[0], c[0]))
 
 
 def cube(nnnr: int, side: float, eps) -> Scene:
@@ -64,7 +66,7 @@ def cube(nnnr: int, side: float, eps) -> Scene:
     inside = np.ones((nnnr, nnnr, nnnr), bool)
     tab, pos = _lattice(c, c, c, inside)
     e = np.broadcast_to(np.asarray(eps, np.complex128), (pos.shape[0],)).copy()
-    return Scene(nnnr, nnnr, nnnr, d, tab, pos, e)
+    return Scene(nnnr, nnnr, nnnr, d, tab, pos, e, (c[0], c[0], c[0]))
 
 
 def gaussian_random_field(shape, d: float, sigma: float, corr: float, seed: int) -> np.ndarray:
@@ -176,7 +178,8 @@ def random_spheres(nnnr: int, box: float, nsph: int, radius: float, eps: complex
     for c in centres:
         inside |= (X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2 < radius ** 2
     tab, pos = _lattice(g, g, g, inside)
-    return Scene(nnnr, nnnr, nnnr, d, tab, pos, np.full(pos.shape[0], eps, np.complex128))
+    return Scene(nnnr, nnnr, nnnr, d, tab, pos, np.full(pos.shape[0], eps, np.complex128),
+                 (g[0], g[0], g[0]))
 
 
 def pupil_incidences(n_inc: int, na: float, seed: int):

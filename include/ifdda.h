@@ -239,6 +239,63 @@ void ifdda_inverserigopt(const double *Gxx, const double *Gxy, const double *Gxz
                          volatile int32_t *nstop, char *infostr, size_t len_methodeit,
                          size_t len_infostr);
 
+/* ---------------------------------------------------------------------------
+ * Outputs from solved fields (device vectors, complex128; R right-hand sides
+ * at offsets r * 3 * nd as in ifdda_solve_multi_device).
+ * ------------------------------------------------------------------------- */
+
+/* Cross sections of cdmlib.f:2725-2753: Cext = 4 pi k0 sum Im(E0* . p) / I0,
+ * Cabs = 4 pi k0 sum [Im(p . Eloc*) - 2/3 k0^3 |p|^2] / I0, Csca = Cext - Cabs
+ * (Cabs then set to 0 when <= 1e-12 Cext).  FF0 = incident field at the
+ * dipoles, FF = dipole moments, FFloc = local field; I0[r] = |E0|^2 of RHS r.
+ * out[3 r + {0,1,2}] = Cext, Cabs, Csca (host). */
+int ifdda_cross_sections_device(ifdda_ctx *ctx, int nrhs, const void *FF0, const void *FF,
+                                const void *FFloc, double k0, const double *I0, double *out,
+                                void *stream);
+
+/* Far field by the quick 2D-FFT method of diffractefft2d.f:1-183 for the
+ * dipole moments FF (device, dipole order; the context's dipole map places
+ * them in the box): lattice spacing d, (x0, y0, z0) = centre of box cell
+ * (1,1,1), FFT size nfft2d.  Epos / Eneg (device): per RHS 3 x (2 imax + 1)^2
+ * complex128, [c][jj][ii] with kx = (ii - imax) deltak, ky = (jj - imax)
+ * deltak, the field radiated into z > 0 / z < 0 (zero outside the light
+ * cone).  *imax, *deltak as the reference; its errors ('nfft2d too large',
+ * 'In FFT Poynting nfft2d too small', 'In FFT diffract nfft2d too small')
+ * return IFDDA_ERR_ARG with that text. */
+int ifdda_farfield_device(ifdda_ctx *ctx, int nrhs, const void *FF, int nfft2d, double k0, double d,
+                          double x0, double y0, double z0, void *Epos, void *Eneg, int *imax,
+                          double *deltak, void *stream);
+/* imax and deltak of that far field (to size Epos / Eneg before the call). */
+int ifdda_farfield_size(int nfft2d, double k0, double d, int *imax, double *deltak);
+
+/* Csca and asymmetry g (computegcfft2d.f:1-54) from that far field; theta,
+ * phi = incidence direction of RHS r in degrees; out[2 r + {0,1}] = Csca, g. */
+int ifdda_computegcfft2d_device(ifdda_ctx *ctx, int nrhs, const void *Epos, const void *Eneg,
+                                int imax, double deltak, double k0, const double *I0,
+                                const double *theta, const double *phi, double *out,
+                                void *stream);
+
+/* Reference-shaped (the drop-in binds these to diffractefft2d_ /
+ * computegcfft2d_).  diffractefft2d.f:1-3: FF box-ordered (3 nx ny nz, the
+ * dense box the reference uses for this method, cdmlib.f:630), xs/ys/zs per
+ * cell; fills tabfft2 and the caller's (nfft2d, nfft2d, 3) Ediffkzpos /
+ * Ediffkzneg; nstop = 1 and infostr on the reference's errors.  Eloinx/y/z
+ * and the plans are accepted and unused. */
+void ifdda_diffractefft2d(const int32_t *nx, const int32_t *ny, const int32_t *nz,
+                          const int32_t *nxm, const int32_t *nym, const int32_t *nzm,
+                          const int32_t *nfft2d, int32_t *tabfft2, const double *k0,
+                          const double *xs, const double *ys, const double *zs,
+                          const double *aretecube, double *Eloinx, double *Eloiny,
+                          double *Eloinz, const double *FF, int32_t *imax, double *deltakx,
+                          double *deltaky, double *Ediffkzpos, double *Ediffkzneg,
+                          const int64_t *plan2f, const int64_t *plan2b, int32_t *nstop,
+                          char *infostr, size_t len_infostr);
+/* computegcfft2d.f:1-3 (argument order of the reference). */
+void ifdda_computegcfft2d(const int32_t *imax, const double *deltakx, const double *deltaky,
+                          const double *k0, const double *I0, const double *theta,
+                          const double *phi, const int32_t *nfft2d, const double *Ediffkzpos,
+                          const double *Ediffkzneg, double *gasym, double *Cscai);
+
 #ifdef __cplusplus
 }
 #endif

@@ -261,3 +261,50 @@ class Ref:
         self.lib.poladiff_(_rd(d), C.byref(e), _rd(eps0), _rd(k0), _ri(1), C.c_char_p(m), C.byref(res),
                            C.c_size_t(2))
         return complex(res[0], res[1])
+
+    def diffractefft2d(self, FFbox, nx, ny, nz, nfft2d, k0, xs1, ys1, zs, d):
+        """diffractefft2d_ (Singleton FFT build).  FFbox box-ordered moments;
+        zs[k] the z of plane k.  Returns (Epos, Eneg, imax, deltak, nstop,
+        infostr) with E[ii, jj, c] cut to ii, jj < 2 imax + 1."""
+        N = nx * ny * nz
+        cell = np.arange(N)
+        xs = np.full(N, float(xs1))
+        ys = np.full(N, float(ys1))
+        zz = np.asarray(zs, np.float64)[cell // (nx * ny)].copy()
+        xs = np.ascontiguousarray(xs)
+        ys = np.ascontiguousarray(ys)
+        tab = np.zeros(nfft2d, np.int32)
+        scratch = [np.zeros(nfft2d * nfft2d, np.complex128) for _ in range(3)]
+        F = np.ascontiguousarray(FFbox, np.complex128).reshape(-1)
+        Ep = np.zeros((3, nfft2d, nfft2d), np.complex128)  # Fortran (nfft2d, nfft2d, 3)
+        En = np.zeros((3, nfft2d, nfft2d), np.complex128)
+        imax = C.c_int(0)
+        dkx = C.c_double(0.0)
+        dky = C.c_double(0.0)
+        nstop = C.c_int(0)
+        info = C.create_string_buffer(b" " * 64, 64)
+        P = lambda a: a.ctypes.data_as(C.c_void_p)
+        self.lib.diffractefft2d_(_ri(nx), _ri(ny), _ri(nz), _ri(nx), _ri(ny), _ri(nz), _ri(nfft2d),
+                                 P(tab), _rd(k0), P(xs), P(ys), P(zz), _rd(d), *[P(s) for s in scratch],
+                                 P(F), C.byref(imax), C.byref(dkx), C.byref(dky), P(Ep), P(En), _ri8(),
+                                 _ri8(), C.byref(nstop), info, C.c_size_t(64))
+        n = 2 * imax.value + 1
+        # Fortran E(ii, jj, c) is C-order [c][jj][ii]
+        cut = lambda E: np.transpose(E[:, :n, :n], (2, 1, 0)).copy()
+        return (cut(Ep), cut(En), imax.value, dkx.value, nstop.value,
+                info.raw.decode(errors="replace").rstrip())
+
+    def computegcfft2d(self, Epos, Eneg, imax, dk, k0, I0, theta, phi, nfft2d):
+        """computegcfft2d_: (Csca, g) from E[ii, jj, c] (compact)."""
+        n = 2 * imax + 1
+        full = []
+        for E in (Epos, Eneg):
+            A = np.zeros((3, nfft2d, nfft2d), np.complex128)
+            A[:, :n, :n] = np.transpose(E, (2, 1, 0))
+            full.append(A)
+        g = C.c_double(0.0)
+        cs = C.c_double(0.0)
+        self.lib.computegcfft2d_(_ri(imax), _rd(dk), _rd(dk), _rd(k0), _rd(I0), _rd(theta), _rd(phi),
+                                 _ri(nfft2d), full[0].ctypes.data_as(C.c_void_p),
+                                 full[1].ctypes.data_as(C.c_void_p), C.byref(g), C.byref(cs))
+        return cs.value, g.value

@@ -16,6 +16,10 @@ Fixtures (inputs and the reference's outputs, nothing else):
                    dipoles, eps 2.25, RR, x-polarised plane wave), tol 1e-4,
                    for GPBICG1, QMRBICGSTAB1, QMRBICGSTAB2, TFQMR: E (FFloc),
                    tol1, nloop, ncompte
+  farfield.npz     diffractefft2d_ and computegcfft2d_: a seeded box-ordered
+                   FF on 6x5x4 at nfft2d 64 and 65 (odd), and the config-1
+                   GPBICG1 moments (alpha E) at nfft2d 128; Epos / Eneg cut to
+                   the (2 imax + 1)^2 corner, imax, deltak, Csca, g
 """
 from __future__ import annotations
 
@@ -89,6 +93,36 @@ def main():
         print(m, o["nloop"], o["ncompte"], o["tol1"])
     np.savez_compressed(os.path.join(HERE, "solve_cfg1.npz"), tabdip=s.tabdip, d=s.d, k0=k0,
                         alpha=alpha, E_inc=E, **res)
+    farfield(r, s, k0, alpha, res["GPBICG1_E"])
+
+
+def farfield(r, s, k0, alpha, Eloc):
+    out = {}
+    rng = np.random.default_rng(11)
+    nx, ny, nz = 6, 5, 4
+    F = rng.standard_normal(3 * nx * ny * nz) + 1j * rng.standard_normal(3 * nx * ny * nz)
+    zs = -30e-9 + D * np.arange(nz)
+    out.update(box_FF=F, box_zs=zs, box_x0=-50e-9, box_y0=-40e-9)
+    for N in (64, 65):
+        Ep, En, imax, dk, nstop, info = r.diffractefft2d(F, nx, ny, nz, N, K0, -50e-9, -40e-9, zs, D)
+        assert nstop == 0, info
+        cs, g = r.computegcfft2d(Ep, En, imax, dk, K0, 1.0, 20.0, 30.0, N)
+        out.update({f"box{N}_Epos": Ep, f"box{N}_Eneg": En, f"box{N}_imax": imax, f"box{N}_dk": dk,
+                    f"box{N}_csca": cs, f"box{N}_g": g})
+    # config 1: moments alpha E in box order (zeros outside the sphere)
+    FFbox = np.zeros((s.nx * s.ny * s.nz, 3), np.complex128)
+    cells = np.nonzero(s.tabdip)[0]
+    FFbox[cells] = (alpha[:, None] * Eloc.reshape(-1, 3))[s.tabdip[cells] - 1]
+    zs1 = s.origin[2] + s.d * np.arange(s.nz)
+    N = 128
+    Ep, En, imax, dk, nstop, info = r.diffractefft2d(FFbox.reshape(-1), s.nx, s.ny, s.nz, N, k0,
+                                                     s.origin[0], s.origin[1], zs1, s.d)
+    assert nstop == 0, info
+    cs, g = r.computegcfft2d(Ep, En, imax, dk, k0, 1.0, 0.0, 0.0, N)
+    print("config 1 far field: Csca", cs, "g", g, "imax", imax)
+    out.update(cfg1_Epos=Ep, cfg1_Eneg=En, cfg1_imax=imax, cfg1_dk=dk, cfg1_csca=cs, cfg1_g=g,
+               cfg1_nfft2d=N)
+    np.savez_compressed(os.path.join(HERE, "farfield.npz"), **out)
 
 
 if __name__ == "__main__":

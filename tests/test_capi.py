@@ -47,7 +47,8 @@ def test_fortran_dropin_exports_reference_names():
     C.CDLL(FLIB)
     syms = exported(FLIB)
     for n in ("produitfftmatvect3_", "produitfftmatvectopt_", "greencalculfft_",
-              "greencalculquadfft_", "inverserig_", "inverserigopt_"):
+              "greencalculquadfft_", "inverserig_", "inverserigopt_", "diffractefft2d_",
+              "computegcfft2d_"):
         assert n in syms
 
 

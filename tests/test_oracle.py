@@ -12,6 +12,11 @@ wrong sign or index, or a transposed operand in the restatement fails one.
   solvers       golden reference solves (iteration counts, fields), dense
                 numpy.linalg.solve, zero-RHS breakdown
   Mie           BHMIE cross sections vs DDA on config 1 (a few %)
+  far field     the numpy restatement of diffractefft2d / computegcfft2d
+                vs the reference's golden fixtures (even and odd nfft2d,
+                config 1) and the compiled reference; physics: the far-field
+                Csca of config 1 converges to Cext (optical theorem for a
+                lossless sphere) and g to Mie's
 """
 import numpy as np
 import pytest
@@ -256,3 +261,75 @@ def test_mie_cross_sections_vs_dda(oracle):
     assert abs(cext / cext_mie - 1) < 0.05
     assert abs(cabs) < 1e-3 * cext
     assert abs(qext - qsca) < 1e-10  # lossless sphere: extinction = scattering
+
+
+# ------------------------------------------------------------- far field
+def _farfield_box_case():
+    f = golden("farfield.npz")
+    return f, 6, 5, 4
+
+
+@pytest.mark.parametrize("N", [64, 65])
+def test_farfield_restatement_matches_reference_fixture(N):
+    import farfield as ff
+    f, nx, ny, nz = _farfield_box_case()
+    Ep, En, imax, dk = ff.diffractefft2d(f["box_FF"], nx, ny, nz, N, K0, float(f["box_x0"]),
+                                         float(f["box_y0"]), f["box_zs"], D)
+    assert imax == int(f[f"box{N}_imax"]) and abs(dk / float(f[f"box{N}_dk"]) - 1) < 1e-15
+    scale = np.abs(f[f"box{N}_Epos"]).max()
+    assert relerr(Ep, f[f"box{N}_Epos"], scale) < 1e-12
+    assert relerr(En, f[f"box{N}_Eneg"], scale) < 1e-12
+    cs, g = ff.computegcfft2d(Ep, En, imax, dk, K0, 1.0, 20.0, 30.0)
+    assert abs(cs / float(f[f"box{N}_csca"]) - 1) < 1e-12
+    assert abs(g - float(f[f"box{N}_g"])) < 1e-12
+
+
+def test_farfield_restatement_matches_compiled_reference(ref):
+    import farfield as ff
+    rng = np.random.default_rng(5)
+    nx, ny, nz = 7, 4, 3
+    F = rng.standard_normal(3 * nx * ny * nz) + 1j * rng.standard_normal(3 * nx * ny * nz)
+    zs = 10e-9 + D * np.arange(nz)
+    for N in (50, 96):
+        a = ff.diffractefft2d(F, nx, ny, nz, N, K0, -70e-9, 5e-9, zs, D)
+        b = ref.diffractefft2d(F, nx, ny, nz, N, K0, -70e-9, 5e-9, zs, D)
+        assert b[4] == 0 and a[2] == b[2]
+        scale = np.abs(b[0]).max()
+        assert relerr(a[0], b[0], scale) < 1e-12 and relerr(a[1], b[1], scale) < 1e-12
+        ca = ff.computegcfft2d(a[0], a[1], a[2], a[3], K0, 2.0, 35.0, 120.0)
+        cb = ref.computegcfft2d(b[0], b[1], b[2], b[3], K0, 2.0, 35.0, 120.0, N)
+        assert abs(ca[0] / cb[0] - 1) < 1e-12 and abs(ca[1] - cb[1]) < 1e-12
+    # the reference's error paths
+    with pytest.raises(ValueError, match="too small"):
+        ff.diffractefft2d(F, nx, ny, nz, 8, K0, 0.0, 0.0, zs, D)
+    assert ref.diffractefft2d(F, nx, ny, nz, 8, K0, 0.0, 0.0, zs, D)[4] == 1
+
+
+def test_farfield_config1_physics():
+    """Config 1 (lossless sphere): the far-field Csca integrates to Cext within
+    the k-grid quadrature (2% at nfft2d = 512), g within 2% of Mie's, and the
+    restatement reproduces the reference's nfft2d = 128 fixture."""
+    import farfield as ff
+    import ifdda.scene as scene
+    from mie import bhmie
+    s, k0, alpha, E = scene.config1()
+    g = golden("solve_cfg1.npz")
+    f = golden("farfield.npz")
+    El = g["GPBICG1_E"]
+    p = (alpha[:, None] * El.reshape(-1, 3)).reshape(-1)
+    Cext, Cabs, Csca = ff.cross_sections(E, p, El, k0, 1.0)
+    assert Cabs == 0.0 and Csca == Cext
+    FFbox = np.zeros((s.nx * s.ny * s.nz, 3), np.complex128)
+    cells = np.nonzero(s.tabdip)[0]
+    FFbox[cells] = p.reshape(-1, 3)[s.tabdip[cells] - 1]
+    zs = s.origin[2] + s.d * np.arange(s.nz)
+    Ep, En, im, dk = ff.diffractefft2d(FFbox.reshape(-1), s.nx, s.ny, s.nz, 128, k0, s.origin[0],
+                                       s.origin[1], zs, s.d)
+    scale = np.abs(f["cfg1_Epos"]).max()
+    assert relerr(Ep, f["cfg1_Epos"], scale) < 1e-12 and relerr(En, f["cfg1_Eneg"], scale) < 1e-12
+    Ep, En, im, dk = ff.diffractefft2d(FFbox.reshape(-1), s.nx, s.ny, s.nz, 512, k0, s.origin[0],
+                                       s.origin[1], zs, s.d)
+    cs, gg = ff.computegcfft2d(Ep, En, im, dk, k0, 1.0, 0.0, 0.0)
+    assert abs(cs / Cext - 1) < 0.03
+    _, _, gmie = bhmie(k0 * 200e-9, np.sqrt(2.25 + 0j), 200)
+    assert abs(gg / gmie - 1) < 0.02
