@@ -68,6 +68,8 @@ extern "C" void ifdda_destroy(ifdda_ctx* c) {
   (void)hipFree(c->hx);
   (void)hipFree(c->hb);
   (void)hipFree(c->ffbuf);
+  (void)hipFree(c->dipcell);
+  (void)hipFree(c->mbuf);
   ifdda_timing_enable(c, 0);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -371,9 +373,13 @@ extern "C" int ifdda_set_dipoles(ifdda_ctx* c, const int32_t* tabdip, int ndipol
   if (!tabdip) {
     (void)hipFree(c->tab);
     c->tab = nullptr;
+    (void)hipFree(c->dipcell);
+    c->dipcell = nullptr;
     c->nd = (long long)N;
     c->have_dipoles = true;
     c->tab_dense = true;
+    const int box[6] = {0, g.nx - 1, 0, g.ny - 1, 0, g.nz - 1};
+    std::memcpy(c->cellbox, box, sizeof(box));
     return IFDDA_OK;
   }
   if (ndipole < 0) return arg_error("ifdda_set_dipoles: ndipole < 0");
@@ -392,12 +398,31 @@ extern "C" int ifdda_set_dipoles(ifdda_ctx* c, const int32_t* tabdip, int ndipol
   // read no map.
   bool identity = (size_t)ndipole == N;
   for (size_t i = 0; identity && i < N; ++i) identity = tabdip[i] == (int32_t)(i + 1);
+  // cell of each dipole (-1: not in the map) and the occupied-cell box
+  std::vector<int32_t> cell((size_t)ndipole, -1);
+  int box[6] = {g.nx, -1, g.ny, -1, g.nz, -1};
+  for (size_t i = 0; i < N; ++i)
+    if (tabdip[i] > 0) {
+      cell[tabdip[i] - 1] = (int32_t)i;
+      const int ii = (int)(i % g.nx), jj = (int)((i / g.nx) % g.ny), kk = (int)(i / ((size_t)g.nx * g.ny));
+      box[0] = std::min(box[0], ii), box[1] = std::max(box[1], ii);
+      box[2] = std::min(box[2], jj), box[3] = std::max(box[3], jj);
+      box[4] = std::min(box[4], kk), box[5] = std::max(box[5], kk);
+    }
+  std::memcpy(c->cellbox, box, sizeof(box));
+  (void)hipFree(c->dipcell);
+  c->dipcell = nullptr;
   if (identity) {
     (void)hipFree(c->tab);
     c->tab = nullptr;
   } else {
     if (!c->tab) CHECK_HIP(hipMalloc(&c->tab, sizeof(int32_t) * N), "hipMalloc tabdip");
     CHECK_HIP(hipMemcpy(c->tab, tabdip, sizeof(int32_t) * N, hipMemcpyHostToDevice), "copy tabdip");
+    if (ndipole > 0) {
+      CHECK_HIP(hipMalloc(&c->dipcell, sizeof(int32_t) * ndipole), "hipMalloc dipole cells");
+      CHECK_HIP(hipMemcpy(c->dipcell, cell.data(), sizeof(int32_t) * ndipole, hipMemcpyHostToDevice),
+                "copy dipole cells");
+    }
   }
   c->nd = ndipole;
   c->have_dipoles = true;
@@ -792,7 +817,8 @@ namespace ifdda {
 std::string farfield_geometry(int nfft2d, double k0, double d, int* imax, double* dk);
 int farfield_dev(ifdda_ctx* c, int R, const double2* FF, const int32_t* tab, long long nd,
                  int nfft2d, double k0, double d, double x0, double y0, double z0, double2* Epos,
-                 double2* Eneg, int* imax_out, double* dk_out, hipStream_t s);
+                 double2* Eneg, int* imax_out, double* dk_out, hipStream_t s, int mode,
+                 double numaper, int nside);
 int gc_dev(ifdda_ctx* c, int R, const double2* Epos, const double2* Eneg, int imax, double dk,
            double k0, const double* I0, const double* theta, const double* phi, double* out,
            hipStream_t s);
@@ -838,7 +864,7 @@ extern "C" void ifdda_diffractefft2d(
             hip_ok(hipMemcpy(dF, FF, sizeof(double2) * 3 * ncell, hipMemcpyHostToDevice), "copy FF");
   if (ok)
     ok = farfield_dev(c, 1, dF, nullptr, ncell, N, *k0, *aretecube, xs[0], ys[0], zs[0], dE,
-                      dE + 3 * (size_t)n * n, &im, &dk, c->stream) == IFDDA_OK;
+                      dE + 3 * (size_t)n * n, &im, &dk, c->stream, 0, 0.0, 1) == IFDDA_OK;
   std::vector<double2> hE(ok ? 6 * (size_t)n * n : 0);
   if (ok)
     ok = hip_ok(hipMemcpy(hE.data(), dE, sizeof(double2) * hE.size(), hipMemcpyDeviceToHost),

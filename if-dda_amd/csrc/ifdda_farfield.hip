@@ -112,22 +112,28 @@ __global__ void __launch_bounds__(kFfThreads) k_ff_rows(const double2* FF, long 
 }
 
 // y and z sums for one (kx, ky) bin per wavefront (blockIdx.x = ii + n jj,
-// blockIdx.y = r), then the transverse projection and the origin phase
-// (diffractefft2d.f:139-176).  Bins outside the light cone stay zero.
+// blockIdx.y = r), then the transverse projection and the origin phase.
+// MODE 0, far field (diffractefft2d.f:139-176): both half spaces, bins
+// outside the light cone zero.  MODE 1, lens pupil (diffractefft2dlens.f:
+// 137-178): the side dside only, bins with kx^2 + ky^2 < na2, weight
+// k0^2 / (-2 pi i kz); pz then holds exp(-i dside kz z).
+template <int MODE>
 __global__ void __launch_bounds__(64) k_ff_cols(const double2* X, int ny, int nz, int n,
                                                 const double2* py, const double2* pz,
                                                 const double2* pv, double dk, double k0,
-                                                double2* Epos, double2* Eneg) {
+                                                double na2, double dside, double2* Epos,
+                                                double2* Eneg) {
   const int ij = blockIdx.x, r = blockIdx.y;
   const int ii = ij % n, jj = ij / n;
   const int imax = (n - 1) / 2;
   const double kx = (ii - imax) * dk, ky = (jj - imax) * dk;
   const long long no = (long long)r * 3 * n * n;
   const double kz2 = k0 * k0 - kx * kx - ky * ky;
-  if (!(kz2 > 0.0)) {
+  const bool inside = MODE == 0 ? kz2 > 0.0 : kx * kx + ky * ky < na2;
+  if (!inside) {
     if (threadIdx.x < 3) {
       Epos[no + (long long)threadIdx.x * n * n + ij] = make_double2(0.0, 0.0);
-      Eneg[no + (long long)threadIdx.x * n * n + ij] = make_double2(0.0, 0.0);
+      if (MODE == 0) Eneg[no + (long long)threadIdx.x * n * n + ij] = make_double2(0.0, 0.0);
     }
     return;
   }
@@ -140,12 +146,16 @@ __global__ void __launch_bounds__(64) k_ff_cols(const double2* X, int ny, int nz
   for (long long p = threadIdx.x; p < plane; p += 64) {
     const int k = (int)(p / ny), b = (int)(p - (long long)k * ny);
     const double2 w = cmul(pyj[b], pzij[k]);     // e^{-i ky y} e^{-i kz z}
-    const double2 wn = cmul(pyj[b], make_double2(pzij[k].x, -pzij[k].y));  // e^{+i kz z}
     const double2* x = Xi + p * 3;
-    for (int c = 0; c < 3; ++c) {
-      const double2 v = x[c];
-      sp[c] = cadd(sp[c], cmul(v, w));
-      sn[c] = cadd(sn[c], cmul(v, wn));
+    if (MODE == 0) {
+      const double2 wn = cmul(pyj[b], make_double2(pzij[k].x, -pzij[k].y));  // e^{+i kz z}
+      for (int c = 0; c < 3; ++c) {
+        const double2 v = x[c];
+        sp[c] = cadd(sp[c], cmul(v, w));
+        sn[c] = cadd(sn[c], cmul(v, wn));
+      }
+    } else {
+      for (int c = 0; c < 3; ++c) sp[c] = cadd(sp[c], cmul(x[c], w));
     }
   }
   for (int off = 32; off > 0; off >>= 1)
@@ -158,6 +168,16 @@ __global__ void __launch_bounds__(64) k_ff_cols(const double2* X, int ny, int nz
   if (threadIdx.x != 0) return;
   const double kz = sqrt(kz2);
   const double2 v = pv[ij];
+  if (MODE == 1) {
+    // weight k0^2 / (-2 pi i kz) = i k0^2 / (2 pi kz)
+    const double2 f = cmul(v, make_double2(0.0, k0 * k0 / (6.283185307179586476925286766559 * kz)));
+    const double kk[3] = {kx, ky, dside * kz};
+    double2 ct = make_double2(0.0, 0.0);
+    for (int c = 0; c < 3; ++c) ct = cadd(ct, cscal(sp[c], kk[c] / k0));
+    for (int c = 0; c < 3; ++c)
+      Epos[no + (long long)c * n * n + ij] = cmul(csub(sp[c], cscal(ct, kk[c] / k0)), f);
+    return;
+  }
   const double kv[3] = {kx, ky, kz};
   // k = (kx, ky, +kz) for the upper half space, (kx, ky, -kz) for the lower
   for (int h = 0; h < 2; ++h) {
@@ -251,9 +271,22 @@ std::string farfield_geometry(int nfft2d, double k0, double d, int* imax, double
   return "";
 }
 
+// The same for the lens pupil, numaper in k units (diffractefft2dlens.f:24-72).
+std::string lens_geometry(int nfft2d, double k0, double d, double numaper, int* imax, double* dk) {
+  *dk = 2.0 * 3.141592653589793238462643383279502884 / ((double)nfft2d * d);
+  *imax = 0;
+  if (nfft2d > 16384) return "nfft2d too large";
+  if (*dk >= numaper) return "In FFT lens nfft2d too small";
+  *imax = (int)std::lround(k0 / *dk) + 1;
+  if (2 * *imax + 1 > nfft2d) return "In FFT diffraction for lens nfft2d too small!";
+  if (2 * *imax + 1 < 7) return "In FFT diffract nfft2d too small";
+  return "";
+}
+
 int farfield_dev(ifdda_ctx* c, int R, const double2* FF, const int32_t* tab, long long nd,
                  int nfft2d, double k0, double d, double x0, double y0, double z0, double2* Epos,
-                 double2* Eneg, int* imax_out, double* dk_out, hipStream_t s) {
+                 double2* Eneg, int* imax_out, double* dk_out, hipStream_t s, int mode,
+                 double numaper, int nside) {
   const GridPlan& g = c->g;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (nx > nfft2d || ny > nfft2d) {
@@ -262,7 +295,9 @@ int farfield_dev(ifdda_ctx* c, int R, const double2* FF, const int32_t* tab, lon
   }
   int imax;
   double dk;
-  const std::string msg = farfield_geometry(nfft2d, k0, d, &imax, &dk);
+  const std::string msg = mode == 0 ? farfield_geometry(nfft2d, k0, d, &imax, &dk)
+                                    : lens_geometry(nfft2d, k0, d, numaper, &imax, &dk);
+  const double dside = mode == 0 ? 1.0 : (double)nside;
   *imax_out = imax;
   *dk_out = dk;
   if (!msg.empty()) {
@@ -295,7 +330,7 @@ int farfield_dev(ifdda_ctx* c, int R, const double2* FF, const int32_t* tab, lon
       const double kz = kz2 > 0.0 ? std::sqrt(kz2) : 0.0;
       const size_t ij = (size_t)ii + (size_t)n * jj;
       for (int k = 0; k < nz; ++k) {
-        const double ang = -kz * (z0 + k * d);
+        const double ang = -dside * kz * (z0 + k * d);
         pz[ij * nz + k] = make_double2(std::cos(ang), std::sin(ang));
       }
       const double av = -(var1 * (ii - imax) + var2 * (jj - imax));
@@ -322,8 +357,14 @@ int farfield_dev(ifdda_ctx* c, int R, const double2* FF, const int32_t* tab, lon
   }
   hipLaunchKernelGGL(k_ff_rows, dim3(nz * ny, R), dim3(kFfThreads), lds, s, FF, nd, tab, nx, ny, nz,
                      (const double2*)dpx, n, X);
-  hipLaunchKernelGGL(k_ff_cols, dim3(n * n, R), dim3(64), 0, s, (const double2*)X, ny, nz, n,
-                     (const double2*)dpy, (const double2*)dpz, (const double2*)dpv, dk, k0, Epos, Eneg);
+  if (mode == 0)
+    hipLaunchKernelGGL(k_ff_cols<0>, dim3(n * n, R), dim3(64), 0, s, (const double2*)X, ny, nz, n,
+                       (const double2*)dpy, (const double2*)dpz, (const double2*)dpv, dk, k0, 0.0,
+                       1.0, Epos, Eneg);
+  else
+    hipLaunchKernelGGL(k_ff_cols<1>, dim3(n * n, R), dim3(64), 0, s, (const double2*)X, ny, nz, n,
+                       (const double2*)dpy, (const double2*)dpz, (const double2*)dpv, dk, k0,
+                       numaper * numaper, dside, Epos, Eneg);
   if (!hip_ok(hipGetLastError(), "far-field launch") || !hip_ok(hipStreamSynchronize(s), "far field"))
     return IFDDA_ERR_HIP;
   return IFDDA_OK;
@@ -390,7 +431,7 @@ extern "C" int ifdda_farfield_device(ifdda_ctx* c, int nrhs, const void* FF, int
     return IFDDA_ERR_STATE;
   }
   return farfield_dev(c, nrhs, (const double2*)FF, c->tab, c->nd, nfft2d, k0, d, x0, y0, z0,
-                      (double2*)Epos, (double2*)Eneg, imax, deltak, pick(c, stream));
+                      (double2*)Epos, (double2*)Eneg, imax, deltak, pick(c, stream), 0, 0.0, 1);
 }
 
 extern "C" int ifdda_farfield_size(int nfft2d, double k0, double d, int* imax, double* deltak) {

@@ -142,6 +142,10 @@ struct ifdda_ctx {
   void* solver = nullptr; // Krylov workspace (ifdda_krylov.hip)
   void* ffbuf = nullptr;  // far-field / cross-section workspace (ifdda_farfield.hip)
   size_t ffcap = 0;
+  int32_t* dipcell = nullptr;  // device: box cell of each dipole (null: dense box)
+  int cellbox[6] = {0, 0, 0, 0, 0, 0};  // occupied cells: i, j, k min / max (0-based)
+  void* mbuf = nullptr;   // microscope workspace (ifdda_microscope.hip)
+  size_t mcap = 0;
   // provenance of the device Green tensor for the reference-shaped entries
   const void* green_host = nullptr;
   double green_sample[6][2 * 16] = {};

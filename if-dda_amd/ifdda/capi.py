@@ -46,6 +46,14 @@ class SolveInfo(C.Structure):
 _lib = None
 
 
+class MicroscopeOpts(C.Structure):
+    """ifdda_microscope_opts (include/ifdda.h)."""
+    _fields_ = [("nfft2d", C.c_int), ("numaper", C.c_double), ("numaperinc", C.c_double),
+                ("gross", C.c_double), ("zlens", C.c_double), ("nside", C.c_int),
+                ("npolainc", C.c_int), ("E0", C.c_double * 2), ("k0", C.c_double),
+                ("d", C.c_double), ("origin", C.c_double * 3), ("max_batch", C.c_int)]
+
+
 def lib():
     """Load libifdda_hip.so once; raise if it was not built."""
     global _lib
@@ -86,6 +94,10 @@ def lib():
                                         C.POINTER(C.c_int), pd, vp]
     L.ifdda_farfield_size.argtypes = [ip, dp, dp, C.POINTER(C.c_int), pd]
     L.ifdda_computegcfft2d_device.argtypes = [vp, ip, vp, vp, ip, dp, dp, pd, pd, pd, pd, vp]
+    L.ifdda_microscope_incidences.argtypes = [vp, C.POINTER(MicroscopeOpts), C.POINTER(C.c_int), pd,
+                                              ip, pd]
+    L.ifdda_microsbf_device.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(MicroscopeOpts), pd,
+                                        C.POINTER(C.c_int), C.POINTER(SolveInfo), ip, C.POINTER(C.c_int)]
     L.ifdda_specialized_z.argtypes = [vp, ip, C.POINTER(C.c_int)]
     L.ifdda_timing_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     _lib = L
@@ -307,6 +319,45 @@ class Solver:
                                                  C.c_void_p(stream) if stream else None),
                "ifdda_computegcfft2d_device")
         return out.reshape(R, 2)
+
+    # -- brightfield microscope (microsbf.f, quick lens) ---------------------
+    @staticmethod
+    def microscope_opts(nfft2d, numaper, numaperinc, k0, d, origin, gross=100.0, zlens=0.0,
+                        nside=1, npolainc=0, E0=1.0, max_batch=0):
+        m = MicroscopeOpts()
+        m.nfft2d, m.numaper, m.numaperinc, m.gross, m.zlens = int(nfft2d), numaper, numaperinc, gross, zlens
+        m.nside, m.npolainc, m.k0, m.d, m.max_batch = int(nside), int(npolainc), k0, d, int(max_batch)
+        m.E0[0], m.E0[1] = complex(E0).real, complex(E0).imag
+        for i in range(3):
+            m.origin[i] = float(origin[i])
+        return m
+
+    def microscope_incidences(self, m):
+        """[(kx, ky, pp)] of the condenser lattice, and its step deltak."""
+        n = C.c_int(0)
+        dk = C.c_double(0.0)
+        _check(lib().ifdda_microscope_incidences(self.h, C.byref(m), C.byref(n), None, 0, C.byref(dk)),
+               "ifdda_microscope_incidences")
+        kxy = np.zeros(3 * n.value)
+        D = C.POINTER(C.c_double)
+        _check(lib().ifdda_microscope_incidences(self.h, C.byref(m), C.byref(n), kxy.ctypes.data_as(D),
+                                                 n.value, C.byref(dk)), "ifdda_microscope_incidences")
+        return kxy.reshape(-1, 3), dk.value
+
+    def microsbf(self, m, method="GPBICG1", tol=1e-4, precision=64, x0_zero=False, maxit=10000,
+                 check_every=0, nstop=None):
+        """Returns (images [6, N, N]: scattered x, y, z, total x, y, z; infos)."""
+        o = self._opts(method, tol, precision, x0_zero, maxit, check_every)
+        ninc, _ = self.microscope_incidences(m)
+        infos = (SolveInfo * max(1, ninc.shape[0]))()
+        N = m.nfft2d
+        img = np.zeros(6 * N * N)
+        n = C.c_int(0)
+        stop = nstop if nstop is not None else C.c_int(0)
+        _check(lib().ifdda_microsbf_device(self.h, C.byref(o), C.byref(m),
+                                           img.ctypes.data_as(C.POINTER(C.c_double)), C.byref(n),
+                                           infos, len(infos), C.byref(stop)), "ifdda_microsbf_device")
+        return img.reshape(6, N, N), [infos[i].as_dict() for i in range(n.value)]
 
     def use_specialized(self, on: bool):
         """Select compile-time (default) or generic z-pass kernels."""

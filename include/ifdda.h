@@ -296,6 +296,45 @@ void ifdda_computegcfft2d(const int32_t *imax, const double *deltakx, const doub
                           const double *phi, const int32_t *nfft2d, const double *Ediffkzpos,
                           const double *Ediffkzneg, double *gasym, double *Cscai);
 
+/* ---------------------------------------------------------------------------
+ * Brightfield microscope (microsbf.f:1-965, quick-lens path nquicklens = 1):
+ * the condenser's incidences solved together, lens, image.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  int nfft2d;         /* FFT size of the pupil / image grid (even)             */
+  double numaper;     /* objective NA, 0 < NA <= 1                             */
+  double numaperinc;  /* condenser NA, 0 < NA < 1                              */
+  double gross;       /* magnification                                         */
+  double zlens;       /* position of the focal plane [m]                       */
+  int nside;          /* 1: objective on the z > 0 side, -1: z < 0             */
+  int npolainc;       /* 0: both polarisations (pp = 0 then 1), 1: pp = 0, 2: pp = 1 */
+  double E0[2];       /* amplitude of each illumination (re, im), as microsbf.f:205-209 */
+  double k0;          /* wave number [1/m]                                     */
+  double d;           /* lattice spacing [m]                                   */
+  double origin[3];   /* centre of box cell (1,1,1) [m]                         */
+  int max_batch;      /* incidences per multi-RHS solve (0: all at once)        */
+} ifdda_microscope_opts;
+
+/* The incidences microsbf.f:90-172 takes (polarisation outer, idelta,
+ * jdelta inner): *ninc, and for the first cap of them (kx, ky, pp) in kxy;
+ * *deltak = step of the condenser lattice. */
+int ifdda_microscope_incidences(ifdda_ctx *ctx, const ifdda_microscope_opts *m, int *ninc,
+                                double *kxy, int cap, double *deltak);
+
+/* microsbf.f: for every incidence of the condenser (plane waves of
+ * ondeplanekxky.f), solve (opts: method, tolerance, precision; all
+ * incidences of a batch in one multi-RHS solve), take the pupil field
+ * (diffractefft2dlens.f), add the incident wave at its pupil bin, apply the
+ * lens rotation and focus, form the image (fouriertoimage.f) and sum |E|^2
+ * over incidences.  images (host, 6 * nfft2d^2 doubles): square roots of
+ * the sums for the scattered field x, y, z and the total field x, y, z,
+ * [component][y][x] (x fastest) as the reference's Eimagex ... Eimageincz.
+ * infos[i] (optional, first infos_cap incidences) report each solve.  The
+ * context's Green tensor, dipoles and polarisability must be set. */
+int ifdda_microsbf_device(ifdda_ctx *ctx, const ifdda_solve_opts *opts,
+                          const ifdda_microscope_opts *m, double *images, int *ninc,
+                          ifdda_solve_info *infos, int infos_cap, volatile int *nstop);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,6 +13,11 @@ from __future__ import annotations
 import numpy as np
 
 
+def nint(x):
+    """Fortran NINT / IDNINT: nearest integer, halves away from zero."""
+    return int(np.copysign(np.floor(abs(x) + 0.5), x))
+
+
 def cross_sections(FF0, FF, FFloc, k0, I0):
     """(Cext, Cabs, Csca): cdmlib.f:2730-2753.  FF0 incident field, FF dipole
     moments, FFloc local field, AoS per dipole; I0 = |E0|^2."""
@@ -52,7 +57,7 @@ def diffractefft2d(FFbox, nx, ny, nz, nfft2d, k0, xs1, ys1, zs, d):
     if dk >= k0:
         raise ValueError("In FFT Poynting nfft2d too small")
     tab = tabfft2(N)
-    imax = int(np.rint(k0 / dk)) + 1
+    imax = nint((k0 / dk)) + 1
     if 2 * imax + 1 > N or 2 * imax + 1 < 7:
         raise ValueError("In FFT diffract nfft2d too small")
     F = np.asarray(FFbox).reshape(nz, ny, nx, 3)

@@ -1,0 +1,201 @@
+"""TEST INFRASTRUCTURE ONLY: numpy restatement of the brightfield microscope
+of cdmlib (microsbf.f, quick-lens path nquicklens = 1) and its pieces:
+
+  ondeplanekxky       plane wave of wave vector (kx, ky, kz)   ondeplanekxky.f
+  deltakroutine       bin of the incident wave                 deltakroutine.f
+  irradiance          E0 from power and waist                  irradiance.f
+  diffractefft2dlens  field in the lens pupil                  diffractefft2dlens.f
+  fouriertoimage      pupil -> image plane                     fouriertoimage.f
+  incidences          the condenser's (kx, ky) lattice         microsbf.f:90-172
+  image_assembly      pupil fill, lens rotation, |E|^2 sums    microsbf.f:676-845
+
+Only tests/ imports this module.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from farfield import nint, tabfft2
+
+
+def ondeplanekxky(pos, k0, E0, pp, kx, ky):
+    """ondeplanekxky.f: AoS (Ex, Ey, Ez) at positions pos [n, 3]; the
+    reference's ss argument is unused (s = sqrt(1 - pp))."""
+    if pp < 0.0 or pp > 1.0:
+        raise ValueError("problem in plance incident wave")
+    p, s = np.sqrt(pp), np.sqrt(1.0 - pp)
+    E0s, E0p = E0 * s, E0 * p
+    kz = np.sqrt(k0 * k0 - kx * kx - ky * ky)
+    if kz >= 0.99999 * k0:
+        e = np.array([E0p, E0s, 0.0], np.complex128)
+    else:
+        r = np.sqrt(kx * kx + ky * ky)
+        nsx, nsy = -ky / r, kx / r
+        npx, npy, npz = kz * nsy, -kz * nsx, -kx * nsy + ky * nsx
+        nn = np.sqrt(npx * npx + npy * npy + npz * npz)
+        npx, npy, npz = npx / nn, npy / nn, npz / nn
+        e = np.array([E0s * nsx + E0p * npx, E0s * nsy + E0p * npy, E0p * npz], np.complex128)
+    ph = np.exp(1j * (np.asarray(pos) @ np.array([kx, ky, kz])))
+    return (ph[:, None] * e[None, :]).reshape(-1)
+
+
+def deltakroutine(kxinc, kyinc, dkx, dky, numaper):
+    i = nint((kxinc / dkx))
+    j = nint((kyinc / dky))
+    if (dkx * i) ** 2 + (dky * j) ** 2 <= numaper * numaper:
+        return i, j
+    return int(kxinc / dkx), int(kyinc / dky)
+
+
+def irradiance(P0, w0):
+    c = 299792458.0
+    eps0 = 1.0 / (c * c * 4.0 * np.pi * 1e-7)
+    irra = P0 / (w0 * w0 * np.pi)
+    return np.sqrt(2.0 * irra / c / eps0) + 0j, irra
+
+
+def diffractefft2dlens(FFbox, nx, ny, nz, nfft2d, k0, xs1, ys1, zs, d, numaper, nside):
+    """diffractefft2dlens.f: E[ii, jj, c], imax, deltak (numaper in k units)."""
+    N = nfft2d
+    h = N // 2
+    dk = 2.0 * np.pi / (N * d)
+    if N > 16384:
+        raise ValueError("nfft2d too large")
+    if dk >= numaper:
+        raise ValueError("In FFT lens nfft2d too small")
+    var1 = (xs1 + h * d) * dk
+    var2 = (ys1 + h * d) * dk
+    tab = tabfft2(N)
+    imax = nint((k0 / dk)) + 1
+    if 2 * imax + 1 > N:
+        raise ValueError("In FFT diffraction for lens nfft2d too small!")
+    if 2 * imax + 1 < 7:
+        raise ValueError("In FFT diffract nfft2d too small")
+    ds = float(nside)
+    F = np.asarray(FFbox).reshape(nz, ny, nx, 3)
+    n = 2 * imax + 1
+    E = np.zeros((n, n, 3), np.complex128)
+    iv = np.arange(-imax, imax + 1)
+    KX, KY = np.meshgrid(iv * dk, iv * dk, indexing="ij")
+    ap = KX * KX + KY * KY < numaper * numaper
+    KZ = np.sqrt(np.where(ap, k0 * k0 - KX * KX - KY * KY, 1.0))
+    bins = tab[iv + h] - 1
+    vph = np.exp(-1j * (var1 * iv[:, None] + var2 * iv[None, :]))
+    xpos, ypos = tab[:nx] - 1, tab[:ny] - 1
+    for k in range(nz):
+        P = np.zeros((3, N, N), np.complex128)
+        P[:, ypos[:, None], xpos[None, :]] = np.moveaxis(F[k], 2, 0)
+        P = np.fft.fft2(P, axes=(1, 2))
+        Ex, Ey, Ez = P[:, bins[None, :], bins[:, None]]
+        ct = (KX * Ex + KY * Ey + KZ * ds * Ez) / k0
+        ph = np.exp(-1j * ds * KZ * zs[k]) * vph / (-2.0 * np.pi * 1j * KZ)
+        E[..., 0] += np.where(ap, (Ex - KX * ct / k0) * ph, 0.0)
+        E[..., 1] += np.where(ap, (Ey - KY * ct / k0) * ph, 0.0)
+        E[..., 2] += np.where(ap, (Ez - KZ * ds * ct / k0) * ph, 0.0)
+    return E * k0 * k0, imax, dk
+
+
+def fouriertoimage(dk, gross, planes):
+    """fouriertoimage.f on N x N planes ([y][x], x fastest, in place): forward
+    FFT, then the quadrant swap scaled by deltak^2 / gross."""
+    N = planes[0].shape[0]
+    h = N // 2
+    tmp = dk * dk / gross
+    out = []
+    for P in planes:
+        A = np.fft.fft2(P).reshape(-1).copy()
+        for i in range(-h, h):
+            for j in range(-h, 0):
+                ix = i + 1 if i >= 0 else N + i + 1
+                iy = j + 1 if j >= 0 else N + j + 1
+                ind = ix + N * (iy - 1) - 1
+                kk = i + h + 1 + N * (j + h) - 1
+                A[kk], A[ind] = A[ind] * tmp, A[kk] * tmp
+        out.append(A.reshape(N, N))
+    return out
+
+
+def fouriertoimage_even(dk, gross, planes):
+    """fouriertoimage for an even N: the quadrant swap is fftshift, so the
+    result is fftshift(fft2(P)) * deltak^2 / gross (tests check it against
+    the literal swap and the compiled reference)."""
+    tmp = dk * dk / gross
+    return [np.fft.fftshift(np.fft.fft2(P)) * tmp for P in planes]
+
+
+def incidences(k0, numaperinc, d, nfft2d, xs, ys):
+    """microsbf.f:90-172 with nquicklens = 1: the condenser's (kx, ky) list
+    (idelta outer, jdelta inner), deltak, deltakx."""
+    deltakm = np.pi / max(np.max(xs) - np.min(xs), np.max(ys) - np.min(ys))
+    dkx = 2.0 * np.pi / (d * nfft2d)
+    numaperk = k0 * numaperinc
+    if dkx >= numaperk:
+        raise ValueError("In FFT lens nfft2d too small:Increase size of FFT 1")
+    imul = int(deltakm / dkx)
+    if imul == 0:
+        raise ValueError("In FFT lens nfft2d too small:Increase size of FFT 2")
+    while True:
+        dkinc = dkx * imul
+        imaxinc = nint((k0 / dkinc)) + 1
+        if imaxinc > 3:
+            break
+        imul -= 1
+        if imul == 0:
+            raise ValueError("In FFT lens nfft2d too small:Increase size of FFT 3")
+    inc = []
+    for idel in range(-imaxinc, imaxinc + 1):
+        for jdel in range(-imaxinc, imaxinc + 1):
+            kx, ky = idel * dkinc, jdel * dkinc
+            if kx * kx + ky * ky <= numaperk * numaperk:
+                inc.append((idel, jdel, kx, ky))
+    return inc, dkinc, dkx
+
+
+def image_assembly(Elens_list, inc_list, k0, E0, numaper, zlens, gross, dkinc, imax, dk, nfft2d):
+    """microsbf.f:676-845: for each (pupil field, (idelta, jdelta, kxinc, kyinc,
+    pp)) fill the pupil (focus phase; the incident wave at its bin), rotate by
+    the lens, fouriertoimage, add |E|^2; returns the six sqrt-images
+    (scattered x, y, z, with incident x, y, z) as [y][x] arrays."""
+    N = nfft2d
+    acc = [np.zeros((N, N)) for _ in range(6)]
+    iv = np.arange(-imax, imax + 1)
+    for El, (idel, jdel, kxinc, kyinc, pp) in zip(Elens_list, inc_list):
+        ikx, jky = deltakroutine(kxinc, kyinc, dkinc, dkinc, k0)
+        planes = [np.zeros((N, N), np.complex128) for _ in range(6)]
+        for a, i in enumerate(iv):
+            for b, j in enumerate(iv):
+                kx, ky = dk * i, dk * j
+                t = kx * kx + ky * ky
+                if t > numaper * numaper:
+                    continue
+                kz = np.sqrt(k0 * k0 - t)
+                zf = np.exp(1j * kz * zlens)
+                ix, iy = i % N, j % N
+                s = El[a, b] * zf
+                w = s.copy()
+                if i == ikx and j == jky:
+                    e = ondeplanekxky(np.zeros((1, 3)), k0, E0, pp, kx, ky)
+                    w = w + np.array([e[0] / (dk * dk), e[1] / (dk * dk), e[2] / dk]) * zf
+                if t != 0.0:
+                    u = np.array([kx / k0, ky / k0, 0.0])
+                    u[2] = np.sqrt(1.0 - u[0] ** 2 - u[1] ** 2)
+                    v = np.array([-kx / k0 / gross, -ky / k0 / gross, 0.0])
+                    v[2] = np.sqrt(1.0 - v[0] ** 2 - v[1] ** 2)
+                    u1 = u[1] * v[2] - u[2] * v[1]
+                    u2 = -u[0] * v[2] + u[2] * v[0]
+                    co = u @ v
+                    si = np.sqrt(u1 * u1 + u2 * u2)
+                    u1, u2 = u1 / si, u2 / si
+                    sc = np.sqrt(u[2] / v[2])
+                    Rm = np.array([[u1 * u1 + (1 - u1 * u1) * co, u1 * u2 * (1 - co), u2 * si],
+                                   [u1 * u2 * (1 - co), u2 * u2 + (1 - u2 * u2) * co, -u1 * si],
+                                   [-u2 * si, u1 * si, co]])
+                    s = (Rm @ s) * sc
+                    w = (Rm @ w) * sc
+                for c in range(3):
+                    planes[c][iy, ix] = s[c]
+                    planes[3 + c][iy, ix] = w[c]
+        img = fouriertoimage_even(dk, gross, planes) if N % 2 == 0 else fouriertoimage(dk, gross, planes)
+        for c in range(6):
+            acc[c] += np.abs(img[c]) ** 2
+    return [np.sqrt(a) for a in acc]

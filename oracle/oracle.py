@@ -308,3 +308,61 @@ class Ref:
                                  _ri(nfft2d), full[0].ctypes.data_as(C.c_void_p),
                                  full[1].ctypes.data_as(C.c_void_p), C.byref(g), C.byref(cs))
         return cs.value, g.value
+
+    def ondeplanekxky(self, pos, k0, E0, pp, kx, ky):
+        out = np.zeros((len(pos), 3), np.complex128)
+        e0 = (C.c_double * 2)(E0.real, E0.imag)
+        nstop = C.c_int(0)
+        info = C.create_string_buffer(b" " * 64, 64)
+        for n, (x, y, z) in enumerate(pos):
+            ex, ey, ez = ((C.c_double * 2)() for _ in range(3))
+            self.lib.ondeplanekxky_(_rd(x), _rd(y), _rd(z), _rd(k0), C.byref(e0), _rd(1.0 - pp), _rd(pp),
+                                    _rd(kx), _rd(ky), C.byref(ex), C.byref(ey), C.byref(ez),
+                                    C.byref(nstop), info, C.c_size_t(64))
+            out[n] = [complex(ex[0], ex[1]), complex(ey[0], ey[1]), complex(ez[0], ez[1])]
+        return out.reshape(-1)
+
+    def deltakroutine(self, kxinc, kyinc, dkx, dky, numaper):
+        i, j = C.c_int(0), C.c_int(0)
+        self.lib.deltakroutine_(_rd(kxinc), _rd(kyinc), _rd(dkx), _rd(dky), _rd(numaper), C.byref(i),
+                                C.byref(j))
+        return i.value, j.value
+
+    def irradiance(self, P0, w0):
+        e0 = (C.c_double * 2)()
+        irra = C.c_double(0.0)
+        self.lib.irradiance_(_rd(P0), _rd(w0), C.byref(e0), C.byref(irra))
+        return complex(e0[0], e0[1]), irra.value
+
+    def diffractefft2dlens(self, FFbox, nx, ny, nz, nfft2d, k0, xs1, ys1, zs, d, numaper, nside):
+        """diffractefft2dlens_: (E[ii, jj, c] compact, imax, deltak, nstop, infostr)."""
+        N = nx * ny * nz
+        xs = np.full(N, float(xs1))
+        ys = np.full(N, float(ys1))
+        zz = np.asarray(zs, np.float64)[np.arange(N) // (nx * ny)].copy()
+        tab = np.zeros(nfft2d, np.int32)
+        scratch = [np.zeros(nfft2d * nfft2d, np.complex128) for _ in range(3)]
+        F = np.ascontiguousarray(FFbox, np.complex128).reshape(-1)
+        Ep = np.zeros((3, nfft2d, nfft2d), np.complex128)
+        imax, nstop = C.c_int(0), C.c_int(0)
+        dkx, dky = C.c_double(0.0), C.c_double(0.0)
+        info = C.create_string_buffer(b" " * 64, 64)
+        P = lambda a: a.ctypes.data_as(C.c_void_p)
+        self.lib.diffractefft2dlens_(_ri(nx), _ri(ny), _ri(nz), _ri(nx), _ri(ny), _ri(nz), _ri(nfft2d),
+                                     P(tab), _rd(k0), P(xs), P(ys), P(zz), _rd(d),
+                                     *[P(s) for s in scratch], P(F), C.byref(imax), C.byref(dkx),
+                                     C.byref(dky), P(Ep), _rd(numaper), _ri(nside), _ri8(), _ri8(),
+                                     C.byref(nstop), info, C.c_size_t(64))
+        n = 2 * imax.value + 1
+        E = np.transpose(Ep[:, :n, :n], (2, 1, 0)).copy()
+        return E, imax.value, dkx.value, nstop.value, info.raw.decode(errors="replace").rstrip()
+
+    def fouriertoimage(self, dk, gross, planes):
+        """fouriertoimage_ on six N x N planes ([y][x]): scattered x, y, z then
+        with-incident x, y, z."""
+        N = planes[0].shape[0]
+        A = [np.ascontiguousarray(p, np.complex128).copy() for p in planes]
+        P = lambda a: a.ctypes.data_as(C.c_void_p)
+        self.lib.fouriertoimage_(_rd(dk), _rd(dk), _rd(gross), *[P(a) for a in A], _ri(N), _ri(N // 2),
+                                 _ri8(), _ri8())
+        return A

@@ -16,6 +16,8 @@ Fixtures (inputs and the reference's outputs, nothing else):
                    dipoles, eps 2.25, RR, x-polarised plane wave), tol 1e-4,
                    for GPBICG1, QMRBICGSTAB1, QMRBICGSTAB2, TFQMR: E (FFloc),
                    tol1, nloop, ncompte
+  microscope.npz   diffractefft2dlens_ (nside +1 and -1, two NA) on the seeded
+                   6x5x4 box and fouriertoimage_ on seeded 32 x 32 planes
   farfield.npz     diffractefft2d_ and computegcfft2d_: a seeded box-ordered
                    FF on 6x5x4 at nfft2d 64 and 65 (odd), and the config-1
                    GPBICG1 moments (alpha E) at nfft2d 128; Epos / Eneg cut to
@@ -94,6 +96,25 @@ def main():
     np.savez_compressed(os.path.join(HERE, "solve_cfg1.npz"), tabdip=s.tabdip, d=s.d, k0=k0,
                         alpha=alpha, E_inc=E, **res)
     farfield(r, s, k0, alpha, res["GPBICG1_E"])
+    microscope(r)
+
+
+def microscope(r):
+    out = {}
+    rng = np.random.default_rng(12)
+    nx, ny, nz = 6, 5, 4
+    F = rng.standard_normal(3 * nx * ny * nz) + 1j * rng.standard_normal(3 * nx * ny * nz)
+    zs = -30e-9 + D * np.arange(nz)
+    out.update(box_FF=F, box_zs=zs)
+    for tag, N, na, side in (("a", 64, 0.9, 1), ("b", 96, 0.5, -1)):
+        E, imax, dk, nstop, info = r.diffractefft2dlens(F, nx, ny, nz, N, K0, -50e-9, -40e-9, zs, D,
+                                                        na * K0, side)
+        assert nstop == 0, info
+        out.update({f"lens_{tag}": E, f"lens_{tag}_imax": imax, f"lens_{tag}_dk": dk})
+    planes = [rng.standard_normal((32, 32)) + 1j * rng.standard_normal((32, 32)) for _ in range(6)]
+    img = r.fouriertoimage(3e5, 100.0, planes)
+    out.update(f2i_in=np.stack(planes), f2i_out=np.stack(img))
+    np.savez_compressed(os.path.join(HERE, "microscope.npz"), **out)
 
 
 def farfield(r, s, k0, alpha, Eloc):

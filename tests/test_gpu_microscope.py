@@ -1,0 +1,102 @@
+"""Brightfield microscope (microsbf.f, quick lens) on the GPU against an
+independent CPU pipeline: the oracle's inverserig per incidence, the numpy
+restatements of diffractefft2dlens / ondeplanekxky / the image assembly of
+microsbf.f (whose pieces are pinned to the compiled reference in
+test_microscopy_oracle.py).  Images agree to the solver tolerance."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+LAM = 632.8e-9
+K0 = 2 * np.pi / LAM
+
+
+@pytest.fixture(scope="module")
+def small(gpu):
+    from ifdda import scene
+    s = scene.sphere(6, 60e-9, 2.25 + 0.1j)
+    alpha = scene.polarizability(s.eps, s.d, K0, "RR")
+    sv = gpu.Solver(s.nx, s.ny, s.nz)
+    sv.greencalculfft(s.d, K0)
+    sv.set_dipoles(s.tabdip)
+    sv.set_polarizability(alpha)
+    m = sv.microscope_opts(128, 0.8, 0.5, K0, s.d, s.origin, gross=100.0, zlens=50e-9, nside=1,
+                           npolainc=0, E0=1.0 + 0.5j)
+    yield s, alpha, sv, m
+    sv.close()
+
+
+def test_incidence_lattice(small):
+    import microscopy as mi
+    s, alpha, sv, m = small
+    kxy, dk = sv.microscope_incidences(m)
+    inc, dkinc, _ = mi.incidences(K0, 0.5, s.d, 128, s.pos[:, 0], s.pos[:, 1])
+    assert abs(dk / dkinc - 1) < 1e-15
+    ref = [(kx, ky, pp) for pp in (0.0, 1.0) for (_, _, kx, ky) in inc]
+    assert kxy.shape == (len(ref), 3)
+    assert np.allclose(kxy, np.array(ref), rtol=0, atol=1e-9 * K0)
+
+
+def _cpu_images(s, alpha, oracle, m, tol):
+    import microscopy as mi
+    inc, dkinc, _ = mi.incidences(K0, m.numaperinc, s.d, m.nfft2d, s.pos[:, 0], s.pos[:, 1])
+    G = oracle.green_fft(s.nx, s.ny, s.nz, (2 * s.nx, 2 * s.ny, 2 * s.nz), s.d, K0)
+    pol = np.zeros((s.nx * s.ny * s.nz, 3, 3), np.complex128)
+    for a in range(3):
+        pol[: s.nd, a, a] = alpha
+    E0 = complex(m.E0[0], m.E0[1])
+    zs = s.origin[2] + s.d * np.arange(s.nz)
+    cells = np.nonzero(s.tabdip)[0]
+    lens, lst = [], []
+    for pp in (0.0, 1.0):
+        for (idl, jdl, kx, ky) in inc:
+            Einc = mi.ondeplanekxky(s.pos, K0, E0, pp, kx, ky)
+            o = oracle.inverserig(G, s.tabdip, s.nx, s.ny, s.nz, pol, Einc, "GPBICG1", tol)
+            p = (alpha[:, None] * o["FFloc"].reshape(-1, 3))
+            FFbox = np.zeros((s.nx * s.ny * s.nz, 3), np.complex128)
+            FFbox[cells] = p[s.tabdip[cells] - 1]
+            El, imax, dk = mi.diffractefft2dlens(FFbox.reshape(-1), s.nx, s.ny, s.nz, m.nfft2d, K0,
+                                                 s.origin[0], s.origin[1], zs, s.d, m.numaper * K0, m.nside)
+            lens.append(El)
+            lst.append((idl, jdl, kx, ky, pp))
+    return mi.image_assembly(lens, lst, K0, E0, m.numaper * K0, m.zlens, m.gross, dkinc, imax, dk,
+                             m.nfft2d)
+
+
+def test_brightfield_images_match_cpu_pipeline(small, oracle):
+    s, alpha, sv, m = small
+    print(f"brightfield {s.nd} dipoles nfft2d {m.nfft2d}", flush=True)
+    img, infos = sv.microsbf(m, "GPBICG1", 1e-10, precision=64)
+    assert all(i["nstop"] == 0 and i["tol1"] <= 1e-10 for i in infos)
+    ref = _cpu_images(s, alpha, oracle, m, 1e-10)
+    for c in range(6):
+        scale = np.abs(ref[c]).max()
+        assert scale > 0
+        assert np.abs(img[c] - ref[c]).max() < 1e-7 * scale, (c, np.abs(img[c] - ref[c]).max() / scale)
+    # the incident light dominates the total image; the scattered image is weaker
+    assert np.abs(ref[3:]).max() > np.abs(ref[:3]).max()
+
+
+def test_brightfield_batches_and_precision(small):
+    """Incidences solved in batches of 5 give the all-at-once images; the mixed
+    precision images agree to the solve tolerance."""
+    s, alpha, sv, m = small
+    full, _ = sv.microsbf(m, "GPBICG1", 1e-8, precision=64)
+    m5 = sv.microscope_opts(128, 0.8, 0.5, K0, s.d, s.origin, gross=100.0, zlens=50e-9, nside=1,
+                            npolainc=0, E0=1.0 + 0.5j, max_batch=5)
+    part, infos = sv.microsbf(m5, "GPBICG1", 1e-8, precision=64)
+    assert np.abs(part - full).max() < 1e-12 * np.abs(full).max()
+    mixed, infos = sv.microsbf(m, "GPBICG1", 1e-6, precision=32, x0_zero=True)
+    assert all(i["tol1"] <= 1e-6 for i in infos)
+    assert np.abs(mixed - full).max() < 1e-4 * np.abs(full).max()
+
+
+def test_brightfield_argument_errors(small, gpu):
+    s, alpha, sv, m = small
+    bad = sv.microscope_opts(128, 0.8, 1.0, K0, s.d, s.origin)
+    with pytest.raises(gpu.IfddaError, match="NA inc strictly between 0 and 1"):
+        sv.microsbf(bad)
+    tiny = sv.microscope_opts(16, 0.8, 0.5, K0, s.d, s.origin)
+    with pytest.raises(gpu.IfddaError, match="nfft2d too small"):
+        sv.microsbf(tiny)

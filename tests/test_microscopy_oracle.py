@@ -1,0 +1,52 @@
+"""Pins (CPU) of the numpy restatement of the brightfield microscope pieces
+(oracle/microscopy.py) against the reference's fixtures and, where it is
+built here, the compiled reference: ondeplanekxky, deltakroutine,
+irradiance, diffractefft2dlens, fouriertoimage (literal quadrant swap and
+its even-N form)."""
+import numpy as np
+import pytest
+
+from conftest import golden, relerr
+
+LAM = 632.8e-9
+K0 = 2 * np.pi / LAM
+D = 20e-9
+
+
+@pytest.mark.parametrize("tag,N,na,side", [("a", 64, 0.9, 1), ("b", 96, 0.5, -1)])
+def test_lens_matches_reference_fixture(tag, N, na, side):
+    import microscopy as mi
+    f = golden("microscope.npz")
+    E, imax, dk = mi.diffractefft2dlens(f["box_FF"], 6, 5, 4, N, K0, -50e-9, -40e-9, f["box_zs"], D,
+                                        na * K0, side)
+    assert imax == int(f[f"lens_{tag}_imax"]) and abs(dk / float(f[f"lens_{tag}_dk"]) - 1) < 1e-15
+    assert relerr(E, f[f"lens_{tag}"]) < 1e-12
+
+
+def test_fouriertoimage_matches_reference_fixture():
+    import microscopy as mi
+    f = golden("microscope.npz")
+    planes = list(f["f2i_in"])
+    for out in (mi.fouriertoimage(3e5, 100.0, planes), mi.fouriertoimage_even(3e5, 100.0, planes)):
+        assert relerr(np.stack(out), f["f2i_out"]) < 1e-12
+
+
+def test_pieces_match_compiled_reference(ref):
+    import microscopy as mi
+    rng = np.random.default_rng(4)
+    pos = rng.standard_normal((7, 3)) * 1e-7
+    for pp, kx, ky in ((0.0, 0.0, 0.0), (1.0, 0.0, 0.0), (0.4, 3e6, -2e6), (1.0, -6e6, 2e6)):
+        assert relerr(mi.ondeplanekxky(pos, K0, 0.7 - 0.3j, pp, kx, ky),
+                      ref.ondeplanekxky(pos, K0, 0.7 - 0.3j, pp, kx, ky)) < 1e-14
+    for kx, ky, dk in ((3.1e6, -2.2e6, 1e6), (9.8e6, 0.5e6, 1e6), (-4.5e6, 4.5e6, 1e6)):
+        assert mi.deltakroutine(kx, ky, dk, dk, K0) == ref.deltakroutine(kx, ky, dk, dk, K0)
+    a, b = mi.irradiance(2e-3, 5e-7), ref.irradiance(2e-3, 5e-7)
+    assert abs(a[0] / b[0] - 1) < 1e-14 and abs(a[1] / b[1] - 1) < 1e-14
+    F = rng.standard_normal(3 * 60) + 1j * rng.standard_normal(3 * 60)
+    zs = 5e-9 + D * np.arange(3)
+    E1, i1, _ = mi.diffractefft2dlens(F, 5, 4, 3, 72, K0, -30e-9, 10e-9, zs, D, 0.7 * K0, -1)
+    E2, i2, _, nstop, _ = ref.diffractefft2dlens(F, 5, 4, 3, 72, K0, -30e-9, 10e-9, zs, D, 0.7 * K0, -1)
+    assert nstop == 0 and i1 == i2 and relerr(E1, E2) < 1e-12
+    planes = [rng.standard_normal((16, 16)) + 1j * rng.standard_normal((16, 16)) for _ in range(6)]
+    got, want = mi.fouriertoimage(2e5, 40.0, planes), ref.fouriertoimage(2e5, 40.0, planes)
+    assert relerr(np.stack(got), np.stack(want)) < 1e-12
