@@ -369,6 +369,31 @@ def bench_batch(args):
         Es = FLd.reshape(-1)[: 2 * n3].cpu().numpy().view(np.complex128)
         errs.append(float(np.linalg.norm(Eb[r] - Es) / np.linalg.norm(Es)))
     t_seq = float(np.mean(lat))
+
+    # outputs of the batch: far field (nfft2d 256) and cross sections of all RHS
+    infos = batch()  # fields of every RHS back in FFd / FLd
+    imax, dk = sv.farfield_size(256, k0, s.d)
+    nff = 2 * imax + 1
+    Ep = torch.empty(R * 3 * nff * nff * 2, dtype=torch.float64, device="cuda")
+    En = torch.empty_like(Ep)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    sv.farfield_device(R, FFd.data_ptr(), 256, k0, s.d, s.origin, Ep.data_ptr(), En.data_ptr())
+    xs = sv.cross_sections_device(R, Ed.data_ptr(), FFd.data_ptr(), FLd.data_ptr(), k0, 1.0)
+    gc = sv.computegcfft2d_device(R, Ep.data_ptr(), En.data_ptr(), imax, dk, k0, 1.0, 0.0, 0.0)
+    torch.cuda.synchronize()
+    t_out = time.perf_counter() - t1
+    del Ep, En
+
+    # brightfield microscope on the same object: the reference's condenser
+    # lattice (NA 0.95), objective NA 0.95, 256 x 256 image
+    m = sv.microscope_opts(256, 0.95, 0.95, k0, s.d, s.origin, gross=100.0)
+    kxy, dkinc = sv.microscope_incidences(m)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    img, minfo = sv.microsbf(m, "GPBICG1", 1e-4, precision=32, x0_zero=True)
+    torch.cuda.synchronize()
+    t_mic = time.perf_counter() - t1
     sv.close()
     per_rhs_batch = R / t_batch
     per_rhs_seq = 1.0 / t_seq
@@ -385,7 +410,17 @@ def bench_batch(args):
                                 "latency_ms": {"p50": float(np.percentile(lat, 50)) * 1e3,
                                                "p99": float(np.percentile(lat, 99)) * 1e3}},
             "speedup_per_rhs": per_rhs_batch / per_rhs_seq,
-            "field_rel_err_batch_vs_fp64_max": max(errs)}
+            "field_rel_err_batch_vs_fp64_max": max(errs),
+            "outputs": {"seconds": t_out, "farfield_nfft2d": 256, "farfield_bins": nff * nff,
+                        "cext_mean": float(np.mean(xs[:, 0])), "csca_farfield_mean": float(np.mean(gc[:, 0])),
+                        "what": "far field (diffractefft2d), Csca/g (computegcfft2d) and Cext/Cabs/Csca "
+                                "of all RHS"},
+            "brightfield": {"incidences": int(kxy.shape[0]), "deltak_inc": dkinc, "seconds": t_mic,
+                            "image": [256, 256], "na_condenser": 0.95, "na_objective": 0.95,
+                            "iterations_mean": float(np.mean([i["nloop"] for i in minfo])),
+                            "tol1_max": max(i["tol1"] for i in minfo),
+                            "what": "microsbf quick-lens path: all incidences in one multi-RHS mixed "
+                                    "solve, lens, image, incoherent sum"}}
 
 
 if __name__ == "__main__":
