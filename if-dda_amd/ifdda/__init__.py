@@ -7,5 +7,5 @@ routines they replace: ``greencalculfft``, ``produitfftmatvect3``,
 library is missing.
 """
 from .capi import (  # noqa: F401
-    GPBICG1, QMRBICGSTAB1, QMRBICGSTAB2, TFQMR, METHODS, IfddaError, Solver, lib, lib_path,
+    GPBICG1, QMRBICGSTAB1, QMRBICGSTAB2, TFQMR, METHODS, IfddaError, Runner, Solver, lib, lib_path,
 )

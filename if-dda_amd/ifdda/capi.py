@@ -54,6 +54,14 @@ class MicroscopeOpts(C.Structure):
                 ("d", C.c_double), ("origin", C.c_double * 3), ("max_batch", C.c_int)]
 
 
+class Job(C.Structure):
+    """ifdda_job (include/ifdda.h)."""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("d", C.c_double),
+                ("k0", C.c_double), ("nquad", C.c_int), ("tolinit", C.c_double),
+                ("tabdip", C.c_void_p), ("nd", C.c_int), ("alpha", C.c_void_p),
+                ("nrhs", C.c_int), ("E_inc", C.c_void_p), ("opts", SolveOpts)]
+
+
 def lib():
     """Load libifdda_hip.so once; raise if it was not built."""
     global _lib
@@ -94,6 +102,10 @@ def lib():
                                         C.POINTER(C.c_int), pd, vp]
     L.ifdda_farfield_size.argtypes = [ip, dp, dp, C.POINTER(C.c_int), pd]
     L.ifdda_computegcfft2d_device.argtypes = [vp, ip, vp, vp, ip, dp, dp, pd, pd, pd, pd, vp]
+    L.ifdda_runner_create.argtypes = [ip, dp, C.POINTER(vp)]
+    L.ifdda_runner_submit.argtypes = [vp, C.POINTER(Job), C.POINTER(C.c_longlong)]
+    L.ifdda_runner_wait.argtypes = [vp, C.c_longlong, pd, pd, C.POINTER(SolveInfo), pd]
+    L.ifdda_runner_destroy.argtypes = [vp]
     L.ifdda_microscope_incidences.argtypes = [vp, C.POINTER(MicroscopeOpts), C.POINTER(C.c_int), pd,
                                               ip, pd]
     L.ifdda_microsbf_device.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(MicroscopeOpts), pd,
@@ -389,3 +401,66 @@ class Solver:
         n = C.c_int(0)
         _check(lib().ifdda_solve_trace(self.h, int(cap), _p(out), C.byref(n)), "ifdda_solve_trace")
         return out.reshape(-1, 6)[: n.value].copy()
+
+
+class Runner:
+    """Many independent solves at once (ifdda_runner_*): a pool of workers
+    with their own contexts and streams, admission by device footprint."""
+
+    def __init__(self, nworkers: int = 8, mem_budget: float = 240e9):
+        self.h = C.c_void_p()
+        _check(lib().ifdda_runner_create(int(nworkers), float(mem_budget), C.byref(self.h)),
+               "ifdda_runner_create")
+        self._shape = {}
+
+    def close(self):
+        if self.h:
+            lib().ifdda_runner_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, nx, ny, nz, d, k0, alpha, E_inc, tabdip=None, method="GPBICG1", tol=1e-4,
+               precision=32, x0_zero=True, maxit=10000, nquad=0, tolinit=1e-4):
+        """E_inc: [R, 3 nd] or [3 nd] complex; returns a ticket (the arrays
+        are copied before this returns)."""
+        al = _c128(alpha).reshape(-1)
+        E = _c128(E_inc)
+        E = E.reshape(1, -1) if E.ndim == 1 else E.reshape(E.shape[0], -1)
+        j = Job()
+        j.nx, j.ny, j.nz, j.d, j.k0 = int(nx), int(ny), int(nz), float(d), float(k0)
+        j.nquad, j.tolinit, j.nd, j.nrhs = int(nquad), float(tolinit), al.size, E.shape[0]
+        tab = None
+        if tabdip is not None:
+            tab = np.ascontiguousarray(tabdip, np.int32).reshape(-1)
+            j.tabdip = tab.ctypes.data
+        j.alpha = al.ctypes.data
+        j.E_inc = E.ctypes.data
+        j.opts = Solver._opts(method, tol, precision, x0_zero, maxit, 0)
+        t = C.c_longlong(0)
+        _check(lib().ifdda_runner_submit(self.h, C.byref(j), C.byref(t)), "ifdda_runner_submit")
+        self._shape[t.value] = (E.shape[0], al.size)
+        return t.value
+
+    def wait(self, ticket):
+        R, nd = self._shape.pop(ticket)
+        FL = np.zeros((R, 3 * nd), np.complex128)
+        FF = np.zeros_like(FL)
+        infos = (SolveInfo * R)()
+        times = np.zeros(3)
+        D = C.POINTER(C.c_double)
+        _check(lib().ifdda_runner_wait(self.h, C.c_longlong(ticket), FL.ctypes.data_as(D),
+                                       FF.ctypes.data_as(D), infos, times.ctypes.data_as(D)),
+               "ifdda_runner_wait")
+        return {"FFloc": FL, "FF": FF, "infos": [i.as_dict() for i in infos],
+                "queued_s": times[0], "run_s": times[1], "latency_s": times[2]}

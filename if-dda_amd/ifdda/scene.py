@@ -9,6 +9,9 @@ unchanged) so that benchmarks and tests can build the workloads:
   polarisability        polarisabilite.f:20-44 (CM, RR)
   linear plane wave     ondeplane.f:32-49
   circular plane wave   ondecirce.f:23-52
+  box lattice           nx x ny x nz cells of side d, centres per axis as
+                        objectcube.f:95-106; sphere / ellipsoid / cylinder
+                        are the cells whose centre lies inside
   random permittivity   a seeded Gaussian random field with correlation
                         sigma^2 exp(-r^2/l^2) (the statistics of
                         objectparainhomo.f:150-205; not its realisation)
@@ -202,3 +205,58 @@ def config3(n_inc: int = 128, seed: int = 3):
     th, ph = pupil_incidences(n_inc, 0.95, seed + 1)
     E = np.stack([plane_wave(s.pos, k0, t, p, pp=pp) for pp in (1.0, 0.0) for t, p in zip(th, ph)])
     return s, k0, alpha, E
+
+
+def _box_centres(n: int, d: float) -> np.ndarray:
+    return np.arange(1, n + 1) * d - (n * d + d) / 2.0
+
+
+def shape_box(kind: str, nx: int, ny: int, nz: int, d: float, eps: complex) -> Scene:
+    """kind: 'sphere' (diameter = smallest side), 'cube' (every cell),
+    'ellipsoid' (semi-axes = half sides), 'cylinder' (axis z, diameter =
+    smaller of the x, y sides, full height)."""
+    gx, gy, gz = _box_centres(nx, d), _box_centres(ny, d), _box_centres(nz, d)
+    Z, Y, X = np.meshgrid(gz, gy, gx, indexing="ij")
+    a, b, c = nx * d / 2, ny * d / 2, nz * d / 2
+    if kind == "sphere":
+        r = min(a, b, c)
+        inside = X * X + Y * Y + Z * Z < r * r
+    elif kind == "cube":
+        inside = np.ones(X.shape, bool)
+    elif kind == "ellipsoid":
+        inside = (X / a) ** 2 + (Y / b) ** 2 + (Z / c) ** 2 < 1.0
+    elif kind == "cylinder":
+        r = min(a, b)
+        inside = X * X + Y * Y < r * r
+    else:
+        raise ValueError(f"unknown shape {kind!r}")
+    tab, pos = _lattice(gx, gy, gz, inside)
+    return Scene(nx, ny, nz, d, tab, pos, np.full(pos.shape[0], eps, np.complex128),
+                 (gx[0], gy[0], gz[0]))
+
+
+def config4(count: int = 512, seed: int = 4):
+    """Uneven batch: per solve a shape from {sphere, cube, ellipsoid,
+    cylinder}, nx, ny, nz uniform in [8, 48], Re eps in [1.5, 4], Im eps in
+    [0, 0.5], lambda in [400, 800] nm, d = lambda / (10 |n|), a plane wave of
+    random direction (uniform on the sphere) and polarisation (pp in [0, 1]).
+    Returns a list of (Scene, k0, alpha, E_inc)."""
+    rng = np.random.default_rng(seed)
+    kinds = ["sphere", "cube", "ellipsoid", "cylinder"]
+    out = []
+    while len(out) < count:
+        kind = kinds[rng.integers(4)]
+        nx, ny, nz = (int(v) for v in rng.integers(8, 49, 3))
+        eps = complex(rng.uniform(1.5, 4.0), rng.uniform(0.0, 0.5))
+        lam = rng.uniform(400e-9, 800e-9)
+        d = lam / (10.0 * abs(np.sqrt(eps)))
+        k0 = 2 * np.pi / lam
+        th = np.degrees(np.arccos(rng.uniform(-1.0, 1.0)))
+        ph = rng.uniform(0.0, 360.0)
+        pp = rng.uniform(0.0, 1.0)
+        s = shape_box(kind, nx, ny, nz, d, eps)
+        if s.nd == 0:
+            continue
+        alpha = polarizability(s.eps, d, k0, "RR")
+        out.append((s, k0, alpha, plane_wave(s.pos, k0, th, ph, pp=pp)))
+    return out

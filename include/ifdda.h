@@ -335,6 +335,39 @@ int ifdda_microsbf_device(ifdda_ctx *ctx, const ifdda_solve_opts *opts,
                           const ifdda_microscope_opts *m, double *images, int *ninc,
                           ifdda_solve_info *infos, int infos_cap, volatile int *nstop);
 
+/* ---------------------------------------------------------------------------
+ * Runner: many independent solves at once (no counterpart in the reference,
+ * whose callers loop over objects, wavelengths and requests one cdmlib call
+ * at a time).  A pool of worker threads with their own contexts and streams
+ * takes jobs in submission order; a job starts when a worker is free and its
+ * estimated device footprint fits mem_budget (bytes) next to the running
+ * jobs (a job larger than the budget runs alone).  Each job is one
+ * multi-RHS solve; results do not depend on what else runs.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  int nx, ny, nz;          /* box                                             */
+  double d, k0;            /* lattice spacing [m], wave number [1/m]          */
+  int nquad;               /* Green near-field quadrature, as ifdda_green_build */
+  double tolinit;
+  const int32_t *tabdip;   /* nx*ny*nz box map (1-based dipole or 0), NULL: dense box */
+  int nd;                  /* dipoles                                         */
+  const double *alpha;     /* nd complex polarisabilities (isotropic)         */
+  int nrhs;                /* incidences                                      */
+  const double *E_inc;     /* nrhs * 3 nd complex, RHS r at r * 3 nd          */
+  ifdda_solve_opts opts;
+} ifdda_job;
+
+typedef struct ifdda_runner ifdda_runner;
+int ifdda_runner_create(int nworkers, double mem_budget, ifdda_runner **out);
+/* Copies the job's arrays; returns at once with a ticket. */
+int ifdda_runner_submit(ifdda_runner *r, const ifdda_job *job, long long *ticket);
+/* Blocks until the job has run; FFloc / FF (nrhs * 3 nd complex), infos
+ * (nrhs) may be NULL; times[3] (optional) = seconds queued, running, total. */
+int ifdda_runner_wait(ifdda_runner *r, long long ticket, double *FFloc, double *FF,
+                      ifdda_solve_info *infos, double *times);
+/* Runs what is queued, then stops the workers. */
+int ifdda_runner_destroy(ifdda_runner *r);
+
 #ifdef __cplusplus
 }
 #endif
