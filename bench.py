@@ -115,6 +115,14 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-inc", type=int, default=128, help="incidences of the batch (x2 RHS)")
     ap.add_argument("--batch-seq", type=int, default=16, help="sequential FP64 solves timed")
+    ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--uneven-count", type=int, default=512)
+    ap.add_argument("--uneven-seq", type=int, default=32, help="sequential FP64 solves timed")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-rates", default="2,4,8,16,32", help="requests per second")
+    ap.add_argument("--serving-requests", type=int, default=20, help="requests per rate")
+    ap.add_argument("--serving-seq", type=int, default=8, help="FP64 single solves timed per small object")
+    ap.add_argument("--workers", type=int, default=8, help="runner workers")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -313,7 +321,151 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_batch:
         out["batch"] = bench_batch(args)
+    if not args.no_uneven:
+        out["uneven"] = bench_uneven(args)
+    if not args.no_serving:
+        out["serving"] = bench_serving(args)
     print(json.dumps(out), flush=True)
+
+
+def bench_uneven(args):
+    """512 independent single-incidence solves of uneven shapes and sizes
+    (nx, ny, nz in [8, 48], each with its own wavelength, lattice and Green
+    tensor) through the runner (concurrent workers, mixed precision, tol
+    1e-4) vs plain FP64 solves one at a time on a sample."""
+    import ifdda
+    from ifdda import scene
+
+    probs = scene.config4(args.uneven_count, seed=4)
+    print(f"uneven: {len(probs)} problems", file=sys.stderr, flush=True)
+    with ifdda.Runner(nworkers=args.workers, mem_budget=240e9) as run:
+        warm = run.submit(probs[0][0].nx, probs[0][0].ny, probs[0][0].nz, probs[0][0].d, probs[0][1],
+                          probs[0][2], probs[0][3], tabdip=probs[0][0].tabdip)
+        run.wait(warm)
+        t0 = time.perf_counter()
+        tk = [run.submit(s.nx, s.ny, s.nz, s.d, k0, al, E, tabdip=s.tabdip, tol=1e-4, precision=32)
+              for (s, k0, al, E) in probs]
+        res = [run.wait(t) for t in tk]
+        t_all = time.perf_counter() - t0
+    assert all(r["infos"][0]["nstop"] == 0 and r["infos"][0]["tol1"] <= 1e-4 for r in res)
+    # the same problems as one pack (every matvec launch covers all of them)
+    batch = [dict(nx=s.nx, ny=s.ny, nz=s.nz, d=s.d, k0=k0, alpha=al, E_inc=E, tabdip=s.tabdip)
+             for (s, k0, al, E) in probs]
+    ifdda.solve_batch(batch[:8], "GPBICG1", 1e-4, precision=32)  # warm-up
+    t0 = time.perf_counter()
+    one = ifdda.solve_batch(batch, "GPBICG1", 1e-4, precision=32)
+    t_one = time.perf_counter() - t0
+    assert all(o["infos"][0]["nstop"] == 0 and o["infos"][0]["tol1"] <= 1e-4 for o in one)
+    ns = min(len(probs), args.uneven_seq)
+    pick = np.linspace(0, len(probs) - 1, ns).astype(int)
+    with ifdda.Runner(nworkers=1, mem_budget=240e9) as run:
+        t0 = time.perf_counter()
+        for i in pick:
+            s, k0, al, E = probs[i]
+            r = run.wait(run.submit(s.nx, s.ny, s.nz, s.d, k0, al, E, tabdip=s.tabdip, tol=1e-4,
+                                    precision=64))
+            assert r["infos"][0]["nstop"] == 0
+        t_seq = (time.perf_counter() - t0) / ns
+    its = [r["infos"][0]["nloop"] for r in res]
+    return {"workload": f"{len(probs)} solves: sphere / cube / ellipsoid / cylinder, nx, ny, nz in [8, 48], "
+                        "Re eps in [1.5, 4], Im eps in [0, 0.5], lambda in [400, 800] nm, d = lambda / (10 |n|), "
+                        "random plane wave; GPBICG tol 1e-4, Green tensor built per solve",
+            "workers": args.workers, "max_members": 64, "seconds": t_all, "solves_per_s": len(probs) / t_all,
+            "one_pack": {"seconds": t_one, "solves_per_s": len(probs) / t_one},
+            "iterations": {"mean": float(np.mean(its)), "max": int(np.max(its))},
+            "dipoles": {"mean": float(np.mean([p[0].nd for p in probs])),
+                        "max": int(np.max([p[0].nd for p in probs]))},
+            "sequential_fp64": {"sample": int(ns), "seconds_per_solve": t_seq, "solves_per_s": 1.0 / t_seq},
+            "speedup": (len(probs) / t_all) * t_seq}
+
+
+def bench_serving(args):
+    """Continuous batching: Poisson arrivals at several rates; 90% of the
+    requests are config-3-like objects (20 spheres, 64^3, 1-64 incidences),
+    10% are 96^3 inhomogeneous cubes (1-16 incidences); each request is one
+    multi-RHS job of the runner, admitted by device footprint (240 GB).
+    Latency from arrival to completion; throughput vs FP64 one at a time."""
+    import ifdda
+    from ifdda import scene
+
+    rng = np.random.default_rng(5)
+    lam = 632.8e-9
+    k0 = 2 * np.pi / lam
+    small = []
+    for i in range(4):
+        o = scene.random_spheres(64, 1.28e-6, 20, 150e-9, 2.0 + 0.001j, seed=100 + i)
+        small.append((o, scene.polarizability(o.eps, o.d, k0, "RR")))
+    large = []
+    for i in range(2):
+        o = scene.inhomogeneous_cube(96, 96 * 20e-9, 2.25 + 0.01j, 0.1, 100e-9, seed=200 + i)
+        large.append((o, scene.polarizability(o.eps, o.d, k0, "RR")))
+
+    def request():
+        big = rng.random() < 0.1
+        o, al = (large if big else small)[rng.integers(2 if big else 4)]
+        R = int(rng.integers(1, 17 if big else 65))
+        th, ph = scene.pupil_incidences(R, 0.95, int(rng.integers(1 << 30)))
+        pp = rng.random(R)
+        E = np.stack([scene.plane_wave(o.pos, k0, t, f, pp=q) for t, f, q in zip(th, ph, pp)])
+        return o, al, E, big
+
+    n_req = args.serving_requests
+    rates = [float(r) for r in args.serving_rates.split(",")]
+    reqs = [[request() for _ in range(n_req)] for _ in rates]
+    print(f"serving: {len(rates)} rates x {n_req} requests", file=sys.stderr, flush=True)
+    out = {"workload": "Poisson arrivals; 90% 20-sphere 64^3 objects with 1-64 incidences, 10% 96^3 "
+                       "inhomogeneous cubes with 1-16 incidences; mixed precision, tol 1e-4; admission "
+                       "budget 240 GB", "workers": args.workers, "rates": []}
+    with ifdda.Runner(nworkers=args.workers, mem_budget=240e9) as run:
+        o, al, E, _ = reqs[0][0]
+        run.wait(run.submit(o.nx, o.ny, o.nz, o.d, k0, al, E[:1], tabdip=o.tabdip))  # warm-up
+        for rate, rs in zip(rates, reqs):
+            arr = np.cumsum(rng.exponential(1.0 / rate, len(rs)))
+            t0 = time.perf_counter()
+            tickets = []
+            for a, (o, al, E, big) in zip(arr, rs):
+                dt = t0 + a - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                tickets.append(run.submit(o.nx, o.ny, o.nz, o.d, k0, al, E, tabdip=o.tabdip, tol=1e-4,
+                                          precision=32))
+            res = [run.wait(t) for t in tickets]
+            span = time.perf_counter() - t0
+            lat = np.array([r["latency_s"] for r in res])
+            nrhs = sum(E.shape[0] for (_, _, E, _) in rs)
+            assert all(i["nstop"] == 0 and i["tol1"] <= 1e-4 for r in res for i in r["infos"])
+            out["rates"].append({"rate_per_s": rate, "requests": len(rs), "incidences": nrhs,
+                                 "span_s": span, "solves_per_s": nrhs / span,
+                                 "latency_ms": {"p50": float(np.percentile(lat, 50)) * 1e3,
+                                                "p99": float(np.percentile(lat, 99)) * 1e3}})
+            print(f"serving rate {rate}: p50 {out['rates'][-1]['latency_ms']['p50']:.0f} ms", file=sys.stderr,
+                  flush=True)
+    # plain FP64 single solves one at a time (one incidence per solve, Green
+    # tensor resident): seconds per solve of each request kind on a sample,
+    # weighted by the incidences of the mix
+    per = {}
+    with ifdda.Runner(nworkers=1, mem_budget=240e9) as run:
+        for big, pool, ns in ((False, small, args.serving_seq), (True, large, max(2, args.serving_seq // 4))):
+            o, al = pool[0]
+            th, ph = scene.pupil_incidences(ns, 0.95, 77)
+            Es = [scene.plane_wave(o.pos, k0, t, f, pp=1.0) for t, f in zip(th, ph)]
+            run.wait(run.submit(o.nx, o.ny, o.nz, o.d, k0, al, Es[0], tabdip=o.tabdip, tol=1e-4,
+                                precision=64))  # Green tensor resident
+            t0 = time.perf_counter()
+            for E in Es:
+                r = run.wait(run.submit(o.nx, o.ny, o.nz, o.d, k0, al, E, tabdip=o.tabdip, tol=1e-4,
+                                        precision=64))
+                assert r["infos"][0]["nstop"] == 0 and r["infos"][0]["tol1"] <= 1e-4
+            per[big] = (time.perf_counter() - t0) / ns
+    n_small = sum(E.shape[0] for rs in reqs for (_, _, E, big) in rs if not big)
+    n_large = sum(E.shape[0] for rs in reqs for (_, _, E, big) in rs if big)
+    t_mix = (n_small * per[False] + n_large * per[True]) / (n_small + n_large)
+    out["sequential_fp64"] = {"seconds_per_solve_small": per[False], "seconds_per_solve_large": per[True],
+                              "incidence_share_large": n_large / (n_small + n_large),
+                              "solves_per_s": 1.0 / t_mix}
+    for r in out["rates"]:
+        r["speedup_vs_sequential_fp64"] = r["solves_per_s"] * t_mix
+    return out
 
 
 def bench_batch(args):

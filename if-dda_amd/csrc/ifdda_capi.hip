@@ -114,24 +114,10 @@ extern "C" int ifdda_create(int nx, int ny, int nz, ifdda_ctx** out) {
   if (!hip_ok(hipStreamCreate(&c->stream), "hipStreamCreate")) return fail(IFDDA_ERR_HIP);
   for (int a = 0; a < 3; ++a) {
     const int L = g.L[a];
-    std::vector<double2> t64(L);
-    std::vector<float2> t32(L);
-    for (int e = 0; e < L; ++e) {
-      // exp(+2 pi i e / L) from the host math library in FP64
-      const double ang = 6.283185307179586476925286766559 * (double)e / (double)L;
-      t64[e] = make_double2(std::cos(ang), std::sin(ang));
-      t32[e] = make_float2((float)t64[e].x, (float)t64[e].y);
-    }
-    std::vector<double> sre(L, 0.0), sim(L, 0.0);
-    stage_twiddles(g.ax[a], sre.data(), sim.data());
-    std::vector<double2> s64(L);
-    std::vector<float2> s32(L);
-    for (int e = 0; e < L; ++e) {
-      s64[e] = make_double2(sre[e], sim[e]);
-      s32[e] = make_float2((float)sre[e], (float)sim[e]);
-    }
+    std::vector<double2> t64(L), s64(L);
+    std::vector<float2> t32(L), s32(L);
     std::vector<int> pos(L);
-    axis_positions(g.ax[a], pos.data());
+    axis_tables(g.ax[a], t64.data(), t32.data(), s64.data(), s32.data(), pos.data());
     if (!hip_ok(hipMalloc(&c->pos[a], sizeof(int) * L), "hipMalloc positions") ||
         !hip_ok(hipMemcpy(c->pos[a], pos.data(), sizeof(int) * L, hipMemcpyHostToDevice),
                 "copy positions") ||

@@ -21,6 +21,8 @@
 #include "ifdda_internal.h"
 
 #include <algorithm>
+#include <cmath>
+#include <cstring>
 #include <vector>
 
 namespace ifdda {
@@ -121,6 +123,9 @@ constexpr int kBatch = 8;
 // Green-tensor items per thread in the convolution pass (k_zconv<V, NG> is
 // instantiated for NG = 1..kMaxG and launched with the smallest that fits).
 constexpr int kMaxG = 4;
+// Loop start of the threads of a packed launch that a tile does not use:
+// above every per-thread loop bound of the FFT tiles (L < 2048, B <= 64).
+constexpr int kIdle = 1 << 24;
 
 // R-point DFT of a half-zero input: a[q] = 0 for q >= R/2 (only the first R/2
 // are read).  Even outputs are the R/2-point DFT of a, odd outputs that of
@@ -315,7 +320,7 @@ __device__ __forceinline__ bool prunable(const AxisPlan& ax) {
 // Thread layout: line = tid % NL, jt = tid / NL, JT = blockDim / NL.
 // half_in: the signal is zero at n >= L/2 (caller checked 2 n_in <= L).
 template <int SGN, typename V>
-__device__ void tile_fft_dit(V* buf, const V* tw, const AxisPlan& ax, int NL, int line, int jt,
+__device__ __forceinline__ void tile_fft_dit(V* buf, const V* tw, const AxisPlan& ax, int NL, int line, int jt,
                              int JT, bool half_in) {
   int m = 1, s = 0;
   if (half_in && prunable(ax)) {
@@ -331,7 +336,7 @@ __device__ void tile_fft_dit(V* buf, const V* tw, const AxisPlan& ax, int NL, in
 
 // half_out: only transform indices < L/2 are read afterwards.
 template <int SGN, typename V>
-__device__ void tile_fft_dif(V* buf, const V* tw, const AxisPlan& ax, int NL, int line, int jt,
+__device__ __forceinline__ void tile_fft_dif(V* buf, const V* tw, const AxisPlan& ax, int NL, int line, int jt,
                              int JT, bool half_out) {
   int m = ax.L;
   const int last = half_out && prunable(ax) ? 1 : 0;
@@ -391,6 +396,9 @@ template <typename V> struct SArgs {
   long long zs_src, zs_dst;
   const int* skip;
   int skip_stride;
+  // packed launches: tiles along kx (the grid x extent) and threads of the
+  // tile shape
+  int gx, nthr;
 };
 
 __device__ __forceinline__ int fold_idx(int k, int L, int& sgn) {
@@ -440,24 +448,24 @@ __device__ __forceinline__ void load_strided(V* buf, const int* pos, const V* sr
 // Forward transforms (SGN > 0) run DIT from a permuted load with the
 // first-stage pruning; inverse transforms (SGN < 0) run DIF from a natural
 // load with the last-stage pruning and store through pos.
-template <typename V, int NC, int SGN>
-__global__ void __launch_bounds__(256) k_strided(SArgs<V> a) {
-  // a stays unmodified (kernel-argument memory); per-RHS offsets go to locals
-  if (a.skip && a.skip[blockIdx.z * a.skip_stride]) return;
-  const V* const zsrc = a.src + blockIdx.z * a.zs_src;
-  V* const zdst = a.dst + blockIdx.z * a.zs_dst;
+//
+// strided_tile computes logical tile lg (outer index lg / gx, kx tile
+// lg % gx) with nthr threads.  In a packed launch (PK) the block has more
+// threads than the tile uses: those at or above nthr take the loop start
+// kIdle, beyond every loop bound, so they only meet the barriers.
+template <typename V, int NC, int SGN, bool PK>
+__device__ __forceinline__ void strided_tile(const SArgs<V>& a, const V* zsrc, V* zdst, int lg,
+                                             int gx, int tid0, int nthr) {
   extern __shared__ double4 smem_d4[];
   V* tw = reinterpret_cast<V*>(smem_d4);
   const int L = a.ax.L;
   int* pos = reinterpret_cast<int*>(tw + L);
   V* buf = tile_base(tw, L);
   const int NL = a.B * NC;
-  const int tid = threadIdx.x;
-  const int nthr = blockDim.x;
-  const int line = tid % NL, jt = tid / NL, JT = nthr / NL;
-  const int lg = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-  const int o = lg / gridDim.x;
-  const int kx0 = (lg - o * gridDim.x) * a.B;
+  const int tid = PK && tid0 >= nthr ? kIdle : tid0;
+  const int line = tid0 % NL, jt = tid / NL, JT = nthr / NL;
+  const int o = lg / gx;
+  const int kx0 = (lg - o * gx) * a.B;
   const int nvalid = min(a.B, a.nbx - kx0) * NC;
 
   load_tables(tw, pos, a.tw, a.pos, L, tid, nthr);
@@ -485,6 +493,15 @@ __global__ void __launch_bounds__(256) k_strided(SArgs<V> a) {
   }
 }
 
+template <typename V, int NC, int SGN>
+__global__ void __launch_bounds__(256) k_strided(SArgs<V> a) {
+  // a stays unmodified (kernel-argument memory); per-RHS offsets go to locals
+  if (a.skip && a.skip[blockIdx.z * a.skip_stride]) return;
+  const int lg = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  strided_tile<V, NC, SGN, false>(a, a.src + blockIdx.z * a.zs_src, a.dst + blockIdx.z * a.zs_dst,
+                                  lg, gridDim.x, threadIdx.x, blockDim.x);
+}
+
 // ---------------------------------------------------------------------------
 // P3: FFT-z . G . IFFT-z on the y-transformed planes, in place.  One block
 // takes a tile of B consecutive kx and the two mirror rows ky' and L2-ky'
@@ -507,23 +524,21 @@ __device__ __forceinline__ void sym_product(V* v, const V (&g)[6], bool nxy, boo
   v[2] = cadd(cadd(cmul(gxz, ex), cmul(gyz, ey)), cmul(g[5], ez));
 }
 
-template <typename V, int NG>
-__global__ void __launch_bounds__(256) k_zconv(SArgs<V> a) {
-  if (a.skip && a.skip[blockIdx.z * a.skip_stride]) return;
-  const V* const zsrc = a.src + blockIdx.z * a.zs_src;
-  V* const zdst = a.dst + blockIdx.z * a.zs_dst;
+// zconv_tile: logical tile lg (ky' = lg / gx, kx tile lg % gx), nthr threads
+// (PK: as strided_tile).
+template <typename V, int NG, bool PK>
+__device__ __forceinline__ void zconv_tile(const SArgs<V>& a, const V* zsrc, V* zdst, int lg,
+                                           int gx, int tid0, int nthr) {
   extern __shared__ double4 smem_d4[];
   V* tw = reinterpret_cast<V*>(smem_d4);
   const int L = a.ax.L;  // L3
   int* pos = reinterpret_cast<int*>(tw + L);
   V* buf = tile_base(tw, L);
   const int NL = a.B * 3;
-  const int tid = threadIdx.x;
-  const int nthr = blockDim.x;
-  const int line = tid % NL, jt = tid / NL, JT = nthr / NL;
-  const int lg = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-  const int kyf = lg / gridDim.x;  // 0 .. L2/2
-  const int kx0 = (lg - kyf * gridDim.x) * a.B;
+  const int tid = PK && tid0 >= nthr ? kIdle : tid0;
+  const int line = tid0 % NL, jt = tid / NL, JT = nthr / NL;
+  const int kyf = lg / gx;  // 0 .. L2/2
+  const int kx0 = (lg - kyf * gx) * a.B;
   const int nvalid = min(a.B, a.nbx - kx0) * 3;
   const int F3 = L / 2 + 1;
 
@@ -583,6 +598,14 @@ __global__ void __launch_bounds__(256) k_zconv(SArgs<V> a) {
     }
     lds_sync();  // the next row reuses the LDS tile
   }
+}
+
+template <typename V, int NG>
+__global__ void __launch_bounds__(256) k_zconv(SArgs<V> a) {
+  if (a.skip && a.skip[blockIdx.z * a.skip_stride]) return;
+  const int lg = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  zconv_tile<V, NG, false>(a, a.src + blockIdx.z * a.zs_src, a.dst + blockIdx.z * a.zs_dst, lg,
+                           gridDim.x, threadIdx.x, blockDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -816,6 +839,7 @@ template <typename V> struct XArgs {
   long long zs_src, zs_dst, zs_x;
   const int* skip;
   int skip_stride;
+  int nthr;  // packed launches: threads of the tile shape
 };
 
 template <typename V>
@@ -867,14 +891,12 @@ struct DivStep {
 // Forward (SGN > 0): DIT from a permuted load, first stage pruned when
 // 2 n_in <= L.  Inverse (SGN < 0): DIF from a natural load, last stage pruned
 // when 2 n_out <= L, output read through pos.
-template <typename V, int NC, int SGN, int MODE>
-__global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
-  if (a.skip && a.skip[blockIdx.z * a.skip_stride]) return;
-  // a stays unmodified (kernel-argument memory); per-RHS offsets go to locals
-  const V* const zsrc = a.src ? a.src + blockIdx.z * a.zs_src : nullptr;
-  V* const zdst = a.dst ? a.dst + blockIdx.z * a.zs_dst : nullptr;
-  const V* const zxin = a.xin ? a.xin + blockIdx.z * a.zs_x : nullptr;
-  V* const zxout = a.xout ? a.xout + blockIdx.z * a.zs_x : nullptr;
+//
+// xpass_tile: the tile of box lines g0 .. g0 + B - 1, nthr threads (PK: as
+// strided_tile).
+template <typename V, int NC, int SGN, int MODE, bool PK>
+__device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* zdst,
+                                           const V* zxin, V* zxout, int g0, int tid0, int nthr) {
   constexpr bool DIT = SGN > 0;
   extern __shared__ double4 smem_d4[];
   V* tw = reinterpret_cast<V*>(smem_d4);
@@ -882,10 +904,8 @@ __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
   int* pos = reinterpret_cast<int*>(tw + L);
   V* buf = tile_base(tw, L);
   const int NL = a.B * NC;
-  const int tid = threadIdx.x;
-  const int nthr = blockDim.x;
-  const int line = tid % NL, jt = tid / NL, JT = nthr / NL;
-  const int g0 = xcd_remap(blockIdx.x, gridDim.x) * a.B;
+  const int tid = PK && tid0 >= nthr ? kIdle : tid0;
+  const int line = tid0 % NL, jt = tid / NL, JT = nthr / NL;
   const int nl = min(a.B, a.nlines - g0);
   const bool half_in = DIT && 2 * a.n_in <= L;
   const bool half_out = !DIT && 2 * a.n_out <= L;
@@ -905,7 +925,7 @@ __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
         ix.next();
       }
       V x[U][NC];
-      if (MODE == 1) {
+      if constexpr (MODE == 1) {
         long long dip[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -996,6 +1016,88 @@ __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
       for (int c = 0; c < NC; ++c) d[c] = v[c];
     }
   }
+}
+
+template <typename V, int NC, int SGN, int MODE>
+__global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
+  if (a.skip && a.skip[blockIdx.z * a.skip_stride]) return;
+  // a stays unmodified (kernel-argument memory); per-RHS offsets go to locals
+  xpass_tile<V, NC, SGN, MODE, false>(
+      a, a.src ? a.src + blockIdx.z * a.zs_src : nullptr,
+      a.dst ? a.dst + blockIdx.z * a.zs_dst : nullptr,
+      a.xin ? a.xin + blockIdx.z * a.zs_x : nullptr,
+      a.xout ? a.xout + blockIdx.z * a.zs_x : nullptr, xcd_remap(blockIdx.x, gridDim.x) * a.B,
+      threadIdx.x, blockDim.x);
+}
+
+// ---------------------------------------------------------------------------
+// Packed launches: one grid over the tiles of P independent problems
+// (members) of possibly different boxes, FFT lengths, Green tensors, dipole
+// maps and polarisabilities.  Member m owns blocks [boff[m], boff[m+1]) and
+// the argument block av[m] (built as for its own launch); its dipole vectors
+// start at element zs_x of its P1 / P5 blocks (P1: alpha x input; P5: b
+// input, y output), the members' vectors lying back to back.  The
+// block size is 256 for all members; a member whose tile shape uses fewer
+// threads leaves the rest idle.  A member's skip flag drops its blocks.
+// ---------------------------------------------------------------------------
+template <typename V> struct PkArgs {
+  const void* av;
+  const int* boff;
+  int P;
+  const V* x;
+  V* y;
+  const int* skip;
+  int skip_stride;
+};
+
+// m with boff[m] <= b < boff[m+1] (block-uniform: scalar loads)
+__device__ __forceinline__ int pk_member(const int* __restrict__ boff, int P, int b) {
+  int lo = 0, hi = P;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (boff[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Argument block m, read through the constant address space: the blob does
+// not change during a launch, so its fields are scalar loads (neither copied
+// to the stack nor reloaded into vector registers after the tile's stores).
+template <typename A> __device__ __forceinline__ const A& pk_args(const void* av, int m) {
+  using CA = __attribute__((address_space(4))) const A;
+  const CA* p = (const CA*)(av) + m;
+  return *(const A*)p;
+}
+
+template <typename V, int NC, int SGN, int MODE>
+__global__ void __launch_bounds__(256) k_xpass_pk(PkArgs<V> k) {
+  const int m = pk_member(k.boff, k.P, blockIdx.x);
+  if (k.skip && k.skip[m * k.skip_stride]) return;
+  const XArgs<V>& a = pk_args<XArgs<V>>(k.av, m);
+  const long long vo = a.zs_x;
+  xpass_tile<V, NC, SGN, MODE, true>(a, a.src, a.dst, MODE ? k.x + vo : nullptr,
+                                     MODE == 2 ? k.y + vo : nullptr,
+                                     (blockIdx.x - k.boff[m]) * a.B, threadIdx.x, a.nthr);
+}
+
+template <typename V, int NC, int SGN>
+__global__ void __launch_bounds__(256) k_strided_pk(PkArgs<V> k) {
+  const int m = pk_member(k.boff, k.P, blockIdx.x);
+  if (k.skip && k.skip[m * k.skip_stride]) return;
+  const SArgs<V>& a = pk_args<SArgs<V>>(k.av, m);
+  strided_tile<V, NC, SGN, true>(a, a.src, a.dst, blockIdx.x - k.boff[m], a.gx, threadIdx.x,
+                                 a.nthr);
+}
+
+template <typename V>
+__global__ void __launch_bounds__(256) k_zconv_pk(PkArgs<V> k) {
+  const int m = pk_member(k.boff, k.P, blockIdx.x);
+  if (k.skip && k.skip[m * k.skip_stride]) return;
+  const SArgs<V>& a = pk_args<SArgs<V>>(k.av, m);
+  zconv_tile<V, kMaxG, true>(a, a.src, a.dst, blockIdx.x - k.boff[m], a.gx, threadIdx.x, a.nthr);
 }
 
 // ---------------------------------------------------------------------------
@@ -1261,6 +1363,46 @@ void stage_twiddles(const AxisPlan& ax, double* re, double* im) {
   }
 }
 
+void axis_tables(const AxisPlan& ax, double2* t64, float2* t32, double2* s64, float2* s32,
+                 int* pos) {
+  const int L = ax.L;
+  for (int e = 0; e < L; ++e) {
+    // exp(+2 pi i e / L) from the host math library in FP64
+    const double ang = 6.283185307179586476925286766559 * (double)e / (double)L;
+    t64[e] = make_double2(std::cos(ang), std::sin(ang));
+    t32[e] = make_float2((float)t64[e].x, (float)t64[e].y);
+  }
+  std::vector<double> sre(L, 0.0), sim(L, 0.0);
+  stage_twiddles(ax, sre.data(), sim.data());
+  for (int e = 0; e < L; ++e) {
+    s64[e] = make_double2(sre[e], sim[e]);
+    s32[e] = make_float2((float)sre[e], (float)sim[e]);
+  }
+  axis_positions(ax, pos);
+}
+
+bool make_grid_plan(int nx, int ny, int nz, size_t lds_limit, GridPlan* out, std::string* why) {
+  GridPlan g{};
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  const int n[3] = {nx, ny, nz};
+  for (int a = 0; a < 3; ++a) {
+    g.L[a] = smooth235_at_least(2 * n[a]);
+    g.F[a] = g.L[a] / 2 + 1;
+    if (!make_axis_plan(g.L[a], &g.ax[a])) {
+      if (why) *why = "no radix plan for L=" + std::to_string(g.L[a]);
+      return false;
+    }
+  }
+  if (!make_tiles(g, lds_limit)) {
+    if (why) *why = "padded FFT length above the supported maximum";
+    return false;
+  }
+  *out = g;
+  return true;
+}
+
 bool make_tiles(GridPlan& g, size_t lds_limit) {
   const int L1 = g.L[0];
   // LDS budgets per block from a tile-width sweep on MI355X (200^3 grid):
@@ -1295,19 +1437,29 @@ template <typename K> static bool allow_lds(K kernel, size_t bytes) {
                 "hipFuncSetAttribute(MaxDynamicSharedMemorySize)");
 }
 
+// Argument blocks of the five passes of one problem: everything but the
+// vectors, the right-hand-side strides and the skip flags of a call.
+template <typename V> struct PassSet {
+  XArgs<V> p1, p5;
+  SArgs<V> p2, p3, p4;
+  const TileShape* t[5];
+  int gx[5], gy[5];  // tiles of one right-hand side: x and y extents
+};
+
 template <typename V>
-static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, const V* b, V* y,
-                        hipStream_t s, int pi, const int* skip, PassTimer* tm) {
-  const int R = std::max(1, m.nrhs);
-  const long long zA = (long long)g.nz * g.ny * g.L[0] * 3;  // A per RHS
-  const long long zB = (long long)g.nz * g.L[1] * g.L[0] * 3;  // B per RHS
+static void pass_set(const GridPlan& g, const MatvecDev<V>& m, int pi, PassSet<V>& o) {
   using S = decltype(V::x);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int L1 = g.L[0], L2 = g.L[1], L3 = g.L[2];
   const long long l1c = (long long)L1 * 3;
+  o.t[0] = &g.p1[pi];
+  o.t[1] = &g.p2[pi];
+  o.t[2] = &g.p3[pi];
+  o.t[3] = &g.p4[pi];
+  o.t[4] = &g.p5[pi];
   {  // P1: scatter(alpha x) + FFT-x
-    const TileShape& t = g.p1[pi];
-    XArgs<V> a{};
+    XArgs<V>& a = o.p1;
+    a = XArgs<V>{};
     a.dst = m.A;
     a.tw = m.tw[0];
     a.pos = m.pos[0];
@@ -1315,26 +1467,19 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.n_in = nx;
     a.n_out = L1;
     a.nlines = ny * nz;
-    a.B = t.B;
-    a.skip = skip;
-    a.skip_stride = m.skip_stride;
+    a.B = o.t[0]->B;
+    a.nthr = o.t[0]->threads;
     a.ld_dst = l1c;
     a.tab = m.tab;
     a.nx = nx;
-    a.xin = x;
-    a.zs_x = m.vstride;
-    a.zs_dst = zA;
     a.alpha = m.alpha;
     a.alpha_mode = m.alpha_mode;
-    auto k = k_xpass<V, 3, +1, 1>;
-    if (!allow_lds(k, t.lds)) return false;
-    if (tm) tm->begin(pi * 5 + 0, s);
-    hipLaunchKernelGGL(k, dim3((a.nlines + t.B - 1) / t.B, 1, R), dim3(t.threads), t.lds, s, a);
-    if (tm) tm->end(s);
+    o.gx[0] = (a.nlines + a.B - 1) / a.B;
+    o.gy[0] = 1;
   }
   {  // P2: FFT-y over the nz non-zero planes
-    const TileShape& t = g.p2[pi];
-    SArgs<V> a{};
+    SArgs<V>& a = o.p2;
+    a = SArgs<V>{};
     a.src = m.A;
     a.dst = m.B;
     a.tw = m.tw[1];
@@ -1343,24 +1488,18 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.n_in = ny;
     a.n_out = L2;
     a.nbx = L1;
-    a.B = t.B;
-    a.skip = skip;
-    a.skip_stride = m.skip_stride;
+    a.B = o.t[1]->B;
+    a.nthr = o.t[1]->threads;
     a.so_src = (long long)ny * l1c;
     a.sp_src = l1c;
     a.so_dst = (long long)L2 * l1c;
     a.sp_dst = l1c;
-    a.zs_src = zA;
-    a.zs_dst = zB;
-    auto k = k_strided<V, 3, +1>;
-    if (!allow_lds(k, t.lds)) return false;
-    if (tm) tm->begin(pi * 5 + 1, s);
-    hipLaunchKernelGGL(k, dim3((L1 + t.B - 1) / t.B, nz, R), dim3(t.threads), t.lds, s, a);
-    if (tm) tm->end(s);
+    o.gx[1] = a.gx = (L1 + a.B - 1) / a.B;
+    o.gy[1] = nz;
   }
   {  // P3: FFT-z . G . IFFT-z, in place, keeps kz < nz
-    const TileShape& t = g.p3[pi];
-    SArgs<V> a{};
+    SArgs<V>& a = o.p3;
+    a = SArgs<V>{};
     a.src = m.B;
     a.dst = m.B;
     a.tw = m.tw[2];
@@ -1369,32 +1508,22 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.n_in = nz;
     a.n_out = nz;
     a.nbx = L1;
-    a.B = t.B;
-    a.skip = skip;
-    a.skip_stride = m.skip_stride;
+    a.B = o.t[2]->B;
+    a.nthr = o.t[2]->threads;
     a.so_src = a.so_dst = l1c;
     a.sp_src = a.sp_dst = (long long)L2 * l1c;
-    a.zs_src = a.zs_dst = zB;
     a.G = m.G;
     a.F1 = g.F[0];
     a.F3 = g.F[2];
     a.L1 = L1;
     a.L2 = L2;
     a.tws = m.tws[2];
-    const int items = g.F[2] * t.B;
-    const int ng = (items + t.threads - 1) / t.threads;
-    ZconvKernel<V> k = m.specialized && 2 * nz <= g.L[2] ? zconv_ct_for<V>(g.L[2], t.B) : nullptr;
-    if (!k)
-      k = ng <= 1 ? k_zconv<V, 1> : ng == 2 ? k_zconv<V, 2> : ng == 3 ? k_zconv<V, 3>
-                                                                     : k_zconv<V, kMaxG>;
-    if (!allow_lds(k, t.lds)) return false;
-    if (tm) tm->begin(pi * 5 + 2, s);
-    hipLaunchKernelGGL(k, dim3((L1 + t.B - 1) / t.B, L2 / 2 + 1, R), dim3(t.threads), t.lds, s, a);
-    if (tm) tm->end(s);
+    o.gx[2] = a.gx = (L1 + a.B - 1) / a.B;
+    o.gy[2] = L2 / 2 + 1;
   }
   {  // P4: IFFT-y, keeps j < ny
-    const TileShape& t = g.p4[pi];
-    SArgs<V> a{};
+    SArgs<V>& a = o.p4;
+    a = SArgs<V>{};
     a.src = m.B;
     a.dst = m.A;
     a.tw = m.tw[1];
@@ -1403,24 +1532,18 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.n_in = L2;
     a.n_out = ny;
     a.nbx = L1;
-    a.B = t.B;
-    a.skip = skip;
-    a.skip_stride = m.skip_stride;
+    a.B = o.t[3]->B;
+    a.nthr = o.t[3]->threads;
     a.so_src = (long long)L2 * l1c;
     a.sp_src = l1c;
     a.so_dst = (long long)ny * l1c;
     a.sp_dst = l1c;
-    a.zs_src = zB;
-    a.zs_dst = zA;
-    auto k = k_strided<V, 3, -1>;
-    if (!allow_lds(k, t.lds)) return false;
-    if (tm) tm->begin(pi * 5 + 3, s);
-    hipLaunchKernelGGL(k, dim3((L1 + t.B - 1) / t.B, nz, R), dim3(t.threads), t.lds, s, a);
-    if (tm) tm->end(s);
+    o.gx[3] = a.gx = (L1 + a.B - 1) / a.B;
+    o.gy[3] = nz;
   }
   {  // P5: IFFT-x + gather, y = b - v/M
-    const TileShape& t = g.p5[pi];
-    XArgs<V> a{};
+    XArgs<V>& a = o.p5;
+    a = XArgs<V>{};
     a.src = m.A;
     a.tw = m.tw[0];
     a.pos = m.pos[0];
@@ -1428,24 +1551,188 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.n_in = L1;
     a.n_out = nx;
     a.nlines = ny * nz;
-    a.B = t.B;
-    a.skip = skip;
-    a.skip_stride = m.skip_stride;
+    a.B = o.t[4]->B;
+    a.nthr = o.t[4]->threads;
     a.ld_src = l1c;
     a.tab = m.tab;
     a.nx = nx;
+    a.inv_m = (S)(1.0 / ((double)L1 * (double)L2 * (double)L3));
+    o.gx[4] = (a.nlines + a.B - 1) / a.B;
+    o.gy[4] = 1;
+  }
+}
+
+template <typename V>
+static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, const V* b, V* y,
+                        hipStream_t s, int pi, const int* skip, PassTimer* tm) {
+  const int R = std::max(1, m.nrhs);
+  const long long zA = (long long)g.nz * g.ny * g.L[0] * 3;  // A per RHS
+  const long long zB = (long long)g.nz * g.L[1] * g.L[0] * 3;  // B per RHS
+  PassSet<V> ps;
+  pass_set(g, m, pi, ps);
+  auto grid = [&](int p) { return dim3(ps.gx[p], ps.gy[p], R); };
+  {  // P1: scatter(alpha x) + FFT-x
+    XArgs<V> a = ps.p1;
+    a.skip = skip;
+    a.skip_stride = m.skip_stride;
+    a.xin = x;
+    a.zs_x = m.vstride;
+    a.zs_dst = zA;
+    auto k = k_xpass<V, 3, +1, 1>;
+    if (!allow_lds(k, ps.t[0]->lds)) return false;
+    if (tm) tm->begin(pi * 5 + 0, s);
+    hipLaunchKernelGGL(k, grid(0), dim3(ps.t[0]->threads), ps.t[0]->lds, s, a);
+    if (tm) tm->end(s);
+  }
+  {  // P2: FFT-y over the nz non-zero planes
+    SArgs<V> a = ps.p2;
+    a.skip = skip;
+    a.skip_stride = m.skip_stride;
+    a.zs_src = zA;
+    a.zs_dst = zB;
+    auto k = k_strided<V, 3, +1>;
+    if (!allow_lds(k, ps.t[1]->lds)) return false;
+    if (tm) tm->begin(pi * 5 + 1, s);
+    hipLaunchKernelGGL(k, grid(1), dim3(ps.t[1]->threads), ps.t[1]->lds, s, a);
+    if (tm) tm->end(s);
+  }
+  {  // P3: FFT-z . G . IFFT-z, in place, keeps kz < nz
+    SArgs<V> a = ps.p3;
+    a.skip = skip;
+    a.skip_stride = m.skip_stride;
+    a.zs_src = a.zs_dst = zB;
+    const TileShape& t = *ps.t[2];
+    const int items = g.F[2] * t.B;
+    const int ng = (items + t.threads - 1) / t.threads;
+    ZconvKernel<V> k = m.specialized && 2 * g.nz <= g.L[2] ? zconv_ct_for<V>(g.L[2], t.B) : nullptr;
+    if (!k)
+      k = ng <= 1 ? k_zconv<V, 1> : ng == 2 ? k_zconv<V, 2> : ng == 3 ? k_zconv<V, 3>
+                                                                     : k_zconv<V, kMaxG>;
+    if (!allow_lds(k, t.lds)) return false;
+    if (tm) tm->begin(pi * 5 + 2, s);
+    hipLaunchKernelGGL(k, grid(2), dim3(t.threads), t.lds, s, a);
+    if (tm) tm->end(s);
+  }
+  {  // P4: IFFT-y, keeps j < ny
+    SArgs<V> a = ps.p4;
+    a.skip = skip;
+    a.skip_stride = m.skip_stride;
+    a.zs_src = zB;
+    a.zs_dst = zA;
+    auto k = k_strided<V, 3, -1>;
+    if (!allow_lds(k, ps.t[3]->lds)) return false;
+    if (tm) tm->begin(pi * 5 + 3, s);
+    hipLaunchKernelGGL(k, grid(3), dim3(ps.t[3]->threads), ps.t[3]->lds, s, a);
+    if (tm) tm->end(s);
+  }
+  {  // P5: IFFT-x + gather, y = b - v/M
+    XArgs<V> a = ps.p5;
+    a.skip = skip;
+    a.skip_stride = m.skip_stride;
     a.xin = b;
     a.xout = y;
     a.zs_x = m.vstride;
     a.zs_src = zA;
-    a.inv_m = (S)(1.0 / ((double)L1 * (double)L2 * (double)L3));
     auto k = k_xpass<V, 3, -1, 2>;
-    if (!allow_lds(k, t.lds)) return false;
+    if (!allow_lds(k, ps.t[4]->lds)) return false;
     if (tm) tm->begin(pi * 5 + 4, s);
-    hipLaunchKernelGGL(k, dim3((a.nlines + t.B - 1) / t.B, 1, R), dim3(t.threads), t.lds, s, a);
+    hipLaunchKernelGGL(k, grid(4), dim3(ps.t[4]->threads), ps.t[4]->lds, s, a);
     if (tm) tm->end(s);
   }
   return hip_ok(hipGetLastError(), "matvec launch");
+}
+
+// ----------------------------------------------------------- packed matvec
+template <typename V>
+static size_t pack_blob_impl(const std::vector<const GridPlan*>& g,
+                             const std::vector<MatvecDev<V>>& m, const long long* offs, int pi,
+                             PackLayout& lay, std::vector<unsigned char>& blob) {
+  const int P = (int)g.size();
+  const size_t asz[5] = {sizeof(XArgs<V>), sizeof(SArgs<V>), sizeof(SArgs<V>), sizeof(SArgs<V>),
+                         sizeof(XArgs<V>)};
+  auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  size_t off = 0;
+  for (int p = 0; p < 5; ++p) {
+    lay.av[p] = off;
+    off = al(off + asz[p] * P);
+    lay.boff[p] = off;
+    off = al(off + sizeof(int) * (P + 1));
+  }
+  blob.assign(off, 0);
+  for (int p = 0; p < 5; ++p) {
+    lay.nblk[p] = 0;
+    lay.lds[p] = 0;
+  }
+  for (int i = 0; i < P; ++i) {
+    PassSet<V> ps;
+    pass_set(*g[i], m[i], pi, ps);
+    ps.p1.zs_x = ps.p5.zs_x = offs[i];
+    const void* src[5] = {&ps.p1, &ps.p2, &ps.p3, &ps.p4, &ps.p5};
+    for (int p = 0; p < 5; ++p) {
+      std::memcpy(blob.data() + lay.av[p] + asz[p] * i, src[p], asz[p]);
+      int* bo = reinterpret_cast<int*>(blob.data() + lay.boff[p]);
+      bo[i] = lay.nblk[p];
+      lay.nblk[p] += ps.gx[p] * ps.gy[p];
+      bo[i + 1] = lay.nblk[p];
+      lay.lds[p] = std::max(lay.lds[p], ps.t[p]->lds);
+    }
+  }
+  lay.P = P;
+  return off;
+}
+
+size_t pack_blob_f64(const std::vector<const GridPlan*>& g,
+                     const std::vector<MatvecDev<double2>>& m, const long long* offs,
+                     PackLayout& lay, std::vector<unsigned char>& blob) {
+  return pack_blob_impl(g, m, offs, 0, lay, blob);
+}
+size_t pack_blob_f32(const std::vector<const GridPlan*>& g,
+                     const std::vector<MatvecDev<float2>>& m, const long long* offs,
+                     PackLayout& lay, std::vector<unsigned char>& blob) {
+  return pack_blob_impl(g, m, offs, 1, lay, blob);
+}
+
+template <typename V>
+static bool matvec_pack_impl(const PackLayout& lay, const void* dev, const V* x, const V* b, V* y,
+                             hipStream_t s, const int* skip, int skip_stride) {
+  const unsigned char* base = static_cast<const unsigned char*>(dev);
+  auto args = [&](int p, const V* in, V* out) {
+    PkArgs<V> k{};
+    k.av = base + lay.av[p];
+    k.boff = reinterpret_cast<const int*>(base + lay.boff[p]);
+    k.P = lay.P;
+    k.x = in;
+    k.y = out;
+    k.skip = skip;
+    k.skip_stride = skip_stride;
+    return k;
+  };
+  auto k1 = k_xpass_pk<V, 3, +1, 1>;
+  auto k2 = k_strided_pk<V, 3, +1>;
+  auto k3 = k_zconv_pk<V>;
+  auto k4 = k_strided_pk<V, 3, -1>;
+  auto k5 = k_xpass_pk<V, 3, -1, 2>;
+  if (!allow_lds(k1, lay.lds[0]) || !allow_lds(k2, lay.lds[1]) || !allow_lds(k3, lay.lds[2]) ||
+      !allow_lds(k4, lay.lds[3]) || !allow_lds(k5, lay.lds[4]))
+    return false;
+  const dim3 T(256);
+  hipLaunchKernelGGL(k1, dim3(lay.nblk[0]), T, lay.lds[0], s, args(0, x, nullptr));
+  hipLaunchKernelGGL(k2, dim3(lay.nblk[1]), T, lay.lds[1], s, args(1, nullptr, nullptr));
+  hipLaunchKernelGGL(k3, dim3(lay.nblk[2]), T, lay.lds[2], s, args(2, nullptr, nullptr));
+  hipLaunchKernelGGL(k4, dim3(lay.nblk[3]), T, lay.lds[3], s, args(3, nullptr, nullptr));
+  hipLaunchKernelGGL(k5, dim3(lay.nblk[4]), T, lay.lds[4], s, args(4, b, y));
+  return hip_ok(hipGetLastError(), "packed matvec launch");
+}
+
+bool launch_matvec_pack_f64(const PackLayout& lay, const void* dev, const double2* x,
+                            const double2* b, double2* y, hipStream_t s, const int* skip,
+                            int skip_stride) {
+  return matvec_pack_impl(lay, dev, x, b, y, s, skip, skip_stride);
+}
+bool launch_matvec_pack_f32(const PackLayout& lay, const void* dev, const float2* x,
+                            const float2* b, float2* y, hipStream_t s, const int* skip,
+                            int skip_stride) {
+  return matvec_pack_impl(lay, dev, x, b, y, s, skip, skip_stride);
 }
 
 bool launch_matvec_f64(const GridPlan& g, const MatvecDev<double2>& m, const double2* x,

@@ -2,8 +2,11 @@
 // translation units of libifdda_hip.so (not part of the C ABI).
 #pragma once
 
+#include <chrono>
 #include <string>
+#include <vector>
 
+#include "../../include/ifdda.h"
 #include "ifdda_common.h"
 
 namespace ifdda {
@@ -90,6 +93,13 @@ void axis_positions(const AxisPlan& ax, int* pos);
 void stage_twiddles(const AxisPlan& ax, double* re, double* im);
 // false when an axis is too long for the register-staged FFT tiles
 bool make_tiles(GridPlan& g, size_t lds_limit);
+// Plan of a box: padded lengths, radix plans and tile shapes; false (and the
+// reason) when a length has no plan or tile.
+bool make_grid_plan(int nx, int ny, int nz, size_t lds_limit, GridPlan* out, std::string* why);
+// Tables of one axis (L values each): twiddles exp(+2 pi i e / L) in FP64 and
+// FP32, stage-major twiddles (stage_twiddles), in-place FFT positions.
+void axis_tables(const AxisPlan& ax, double2* t64, float2* t32, double2* s64, float2* s32,
+                 int* pos);
 // whether the z pass of this grid has a compile-time kernel (prec 64 or 32)
 bool has_specialized_z(const GridPlan& g, int prec);
 
@@ -102,6 +112,33 @@ bool launch_matvec_f64(const GridPlan& g, const MatvecDev<double2>& m, const dou
 bool launch_matvec_f32(const GridPlan& g, const MatvecDev<float2>& m, const float2* x,
                        const float2* b, float2* y, hipStream_t s, const int* skip = nullptr,
                        PassTimer* timer = nullptr);
+
+// Packed matvec over P independent problems (members): the per-member
+// argument blocks and block offsets of the five passes live in one device
+// blob whose layout (byte offsets) is described here.
+struct PackLayout {
+  int P = 0;
+  size_t av[5] = {}, boff[5] = {};  // byte offsets in the blob
+  int nblk[5] = {};                 // blocks of each packed launch
+  size_t lds[5] = {};               // dynamic LDS of each packed launch
+};
+// Host image of the blob for members (g[i], m[i]) whose dipole vectors start
+// at element offs[i]; returns its size.
+size_t pack_blob_f64(const std::vector<const GridPlan*>& g,
+                     const std::vector<MatvecDev<double2>>& m, const long long* offs,
+                     PackLayout& lay, std::vector<unsigned char>& blob);
+size_t pack_blob_f32(const std::vector<const GridPlan*>& g,
+                     const std::vector<MatvecDev<float2>>& m, const long long* offs,
+                     PackLayout& lay, std::vector<unsigned char>& blob);
+// y_m = b_m - (1/M_m) IFFT[G_m . FFT[scatter(alpha_m . x_m)]] for every
+// member m (vectors at the blob's offsets); dev is the uploaded blob;
+// skip[m * skip_stride] != 0 drops member m.
+bool launch_matvec_pack_f64(const PackLayout& lay, const void* dev, const double2* x,
+                            const double2* b, double2* y, hipStream_t s, const int* skip,
+                            int skip_stride);
+bool launch_matvec_pack_f32(const PackLayout& lay, const void* dev, const float2* x,
+                            const float2* b, float2* y, hipStream_t s, const int* skip,
+                            int skip_stride);
 
 // Green tensor on the device: fills g64/g32 (folded, F2*F3*F1*6) using
 // `full` (L1*L2*L3 FP64 scratch).  nquad > 0 integrates near cells.
@@ -185,6 +222,63 @@ struct ifdda_ctx {
 namespace ifdda {
 // Release the Krylov workspace (ifdda_krylov.hip).
 void solver_free(ifdda_ctx* c);
+
+// The operator A = I - T alpha of a solve over R right-hand sides whose
+// vectors lie n apart: one context (all RHS share box, Green tensor, dipoles
+// and polarisability) or a pack of independent problems.
+struct MvOp {
+  virtual ~MvOp() = default;
+  virtual bool mv64(const double2* x, const double2* b, double2* y, hipStream_t s, const int* skip,
+                    int R, long long n, int skip_stride) const = 0;
+  virtual bool mv32(const float2* x, const float2* b, float2* y, hipStream_t s, const int* skip,
+                    int R, long long n, int skip_stride) const = 0;
+  // FF = alpha . x for every right-hand side
+  virtual bool alpha_apply(const double2* x, double2* FF, int R, long long n,
+                           hipStream_t s) const = 0;
+};
+// FF = alpha . x for one problem of nd dipoles.
+bool alpha_apply_one(const double2* x, double2* FF, const double2* al, int mode, long long nd,
+                     hipStream_t s);
+
+// Krylov workspace (ifdda_krylov.hip) for R right-hand sides in the ragged
+// layout of packed problems: RHS r holds lens[r] elements at offs[r] (host
+// arrays of R values).
+struct Solver;
+Solver* solver_new(int R, bool need32, const long long* lens, const long long* offs);
+// Reuse S for another ragged layout (lengths and offsets copied on stream s,
+// the host arrays must live until s has run them); false when it is too small.
+bool solver_refit(Solver* S, int R, bool need32, const long long* lens, const long long* offs,
+                  hipStream_t s);
+void solver_delete(Solver* S);
+// The solve of R right-hand sides on S (FP64, or FP32 inner solves with FP64
+// refinement): E_inc, FF, FFloc are device complex128 arrays in S's layout
+// (FF / FFloc may be null).
+int solve_core(const MvOp& op, Solver* S, const ifdda_solve_opts* o, int R, const double2* Einc,
+               double2* FF, double2* FFloc, ifdda_solve_info* infos, volatile int* nstop,
+               hipStream_t s, std::chrono::steady_clock::time_point t0);
+
+// Packs of independent problems (ifdda_pack.hip).  A workspace keeps its
+// stream, FFT tables, Green tensors and grow-only buffers between packs.
+struct PackWs;
+PackWs* pack_ws_new();
+void pack_ws_delete(PackWs* w);
+// Bytes of Green tensors kept between packs (least recently used out).
+void pack_ws_green_limit(PackWs* w, size_t bytes);
+// One job of a pack: its host arrays and where its results go (FFloc / FF
+// nrhs * 3 nd complex, may be null; infos nrhs entries).
+struct PackIn {
+  const ifdda_job* job;
+  double* FFloc;
+  double* FF;
+  ifdda_solve_info* infos;
+};
+// Solves every right-hand side of every job together (one set of options).
+int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts* o,
+               volatile int* nstop);
+// Device bytes a job adds to a pack (FFT scratch, vectors) and of its Green
+// tensor.
+size_t pack_member_bytes(const ifdda_job& j);
+size_t pack_green_bytes(const ifdda_job& j);
 // Grow the FFT scratch to hold R right-hand sides (ifdda_capi.hip).
 int ensure_scratch(ifdda_ctx* c, int R);
 }  // namespace ifdda

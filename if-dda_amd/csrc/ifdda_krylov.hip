@@ -164,6 +164,17 @@ __device__ __forceinline__ void finish_sum(KState* st, const double2* part, int 
   __syncthreads();
 }
 
+// Vector geometry of a launch over right-hand sides: RHS r starts at element
+// off(r) and holds len(r) elements -- r * n and n, or, for packed problems
+// of different sizes, offs[r] and lens[r] (ragged, back to back).
+struct VN {
+  long long n;
+  const long long* lens;
+  const long long* offs;
+  __device__ __forceinline__ long long len(int r) const { return lens ? lens[r] : n; }
+  __device__ __forceinline__ long long off(int r) const { return offs ? offs[r] : r * n; }
+};
+
 #define GRID_LOOP(i, n) \
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < (n); i += (long long)gridDim.x * blockDim.x)
 
@@ -180,15 +191,15 @@ __device__ __forceinline__ void push_trace(KState* st, double2* trace, double2 a
 
 // x = x0 (copy of src, or 0); partial |b|^2
 template <typename V>
-__global__ void __launch_bounds__(kThreads) k_init(const V* b, V* x, const V* src, long long n,
+__global__ void __launch_bounds__(kThreads) k_init(const V* b, V* x, const V* src, VN n,
                                                   double2* part, int nblk) {
   const int rh_ = blockIdx.y;  // right-hand side
   part += (long long)rh_ * kMaxDots * nblk;
-  b += rh_ * n;
-  x += rh_ * n;
-  if (src) src += rh_ * n;
+  b += n.off(rh_);
+  x += n.off(rh_);
+  if (src) src += n.off(rh_);
   double2 acc[1] = {make_double2(0.0, 0.0)};
-  GRID_LOOP(i, n) {
+  GRID_LOOP(i, n.len(rh_)) {
     const double2 bi = LD(b, i);
     acc[0].x += bi.x * bi.x + bi.y * bi.y;
     ST(x, i, src ? LD(src, i) : make_double2(0.0, 0.0));
@@ -221,14 +232,14 @@ template <typename V> struct GV {
 // r = b - Ax (Ax = 0 when x0 = 0); r0* = conj(r); p = t = told = u = z = 0;
 // partial sum r0* r  (gpbicg.f:133-148)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) g_start(GV<V> v, int x0zero, long long n,
+__global__ void __launch_bounds__(kThreads) g_start(GV<V> v, int x0zero, VN n,
                                                    double2* part, int nblk) {
   const int rh_ = blockIdx.y;  // right-hand side
-  v = v.shifted(rh_ * n);
+  v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
   double2 acc[1] = {make_double2(0.0, 0.0)};
   const double2 z0 = make_double2(0.0, 0.0);
-  GRID_LOOP(i, n) {
+  GRID_LOOP(i, n.len(rh_)) {
     double2 r = LD(v.b, i);
     if (!x0zero) r = zsub(r, LD(v.Ax, i));
     ST(v.r, i, r);
@@ -255,26 +266,26 @@ __global__ void __launch_bounds__(kThreads) g_fstart(KState* st, const double2* 
 
 // p = r + beta (p - u) (gpbicg.f:161-169)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) g_k1(const KState* st, GV<V> v, long long n) {
+__global__ void __launch_bounds__(kThreads) g_k1(const KState* st, GV<V> v, VN n) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
-  v = v.shifted(rh_ * n);
+  v = v.shifted(n.off(rh_));
   if (st->done) return;
   const double2 beta = st->beta;
-  GRID_LOOP(i, n) ST(v.p, i, zadd(LD(v.r, i), zmul(beta, zsub(LD(v.p, i), LD(v.u, i)))));
+  GRID_LOOP(i, n.len(rh_)) ST(v.p, i, zadd(LD(v.r, i), zmul(beta, zsub(LD(v.p, i), LD(v.u, i)))));
 }
 
 // partial sum r0* Ap (gpbicg.f:174-181)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) g_k2(const KState* st, GV<V> v, long long n,
+__global__ void __launch_bounds__(kThreads) g_k2(const KState* st, GV<V> v, VN n,
                                                 double2* part, int nblk) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
-  v = v.shifted(rh_ * n);
+  v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   double2 acc[1] = {make_double2(0.0, 0.0)};
-  GRID_LOOP(i, n) acc[0] = zadd(acc[0], zmul(LD(v.r0s, i), LD(v.Ap, i)));
+  GRID_LOOP(i, n.len(rh_)) acc[0] = zadd(acc[0], zmul(LD(v.r0s, i), LD(v.Ap, i)));
   block_partials<1>(acc, part, nblk);
 }
 
@@ -300,14 +311,14 @@ __global__ void __launch_bounds__(kThreads) g_f2(KState* st, const double2* part
 // previous iteration), y = told - r + alpha (Ap - w), t = r - alpha Ap
 // (gpbicg.f:192-200); told holds the previous t.
 template <typename V>
-__global__ void __launch_bounds__(kThreads) g_k3(const KState* st, GV<V> v, long long n) {
+__global__ void __launch_bounds__(kThreads) g_k3(const KState* st, GV<V> v, VN n) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
-  v = v.shifted(rh_ * n);
+  v = v.shifted(n.off(rh_));
   if (st->done) return;
   const double2 al = st->alpha, be = st->beta;
   const bool first = st->it == 0;
-  GRID_LOOP(i, n) {
+  GRID_LOOP(i, n.len(rh_)) {
     const double2 t = LD(v.told, i), r = LD(v.r, i), Ap = LD(v.Ap, i);
     const double2 w = first ? make_double2(0.0, 0.0) : zadd(LD(v.At, i), zmul(be, LD(v.Apold, i)));
     ST(v.y, i, zadd(zsub(t, r), zmul(al, zsub(Ap, w))));
@@ -317,16 +328,16 @@ __global__ void __launch_bounds__(kThreads) g_k3(const KState* st, GV<V> v, long
 
 // <At,At>, <y,y>, <At,y>, <At,t>, <y,t> (gpbicg.f:222-253)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) g_k4(const KState* st, GV<V> v, long long n,
+__global__ void __launch_bounds__(kThreads) g_k4(const KState* st, GV<V> v, VN n,
                                                 double2* part, int nblk) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
-  v = v.shifted(rh_ * n);
+  v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   double2 acc[5];
   for (int r = 0; r < 5; ++r) acc[r] = make_double2(0.0, 0.0);
-  GRID_LOOP(i, n) {
+  GRID_LOOP(i, n.len(rh_)) {
     const double2 At = LD(v.At, i), y = LD(v.y, i), t = LD(v.t, i);
     const double2 cAt = zconj(At), cy = zconj(y);
     acc[0] = zadd(acc[0], zmul(cAt, At));
@@ -372,16 +383,16 @@ __global__ void __launch_bounds__(kThreads) g_f4(KState* st, const double2* part
 
 // u, z, x, r updates; partial |r|^2 and r0* r (gpbicg.f:270-281, :300-307)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) g_k5(const KState* st, GV<V> v, long long n,
+__global__ void __launch_bounds__(kThreads) g_k5(const KState* st, GV<V> v, VN n,
                                                 double2* part, int nblk) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
-  v = v.shifted(rh_ * n);
+  v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   const double2 al = st->alpha, be = st->beta, et = st->eta, dz = st->dzeta;
   double2 acc[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
-  GRID_LOOP(i, n) {
+  GRID_LOOP(i, n.len(rh_)) {
     const double2 Ap = LD(v.Ap, i), r = LD(v.r, i), t = LD(v.t, i), y = LD(v.y, i),
                   At = LD(v.At, i);
     const double2 u = zadd(zmul(dz, Ap), zmul(et, zadd(zsub(LD(v.told, i), r), zmul(be, LD(v.u, i)))));
@@ -442,14 +453,14 @@ template <typename V> struct QV {  // qmrbicgstab.f:23-31 work columns
 
 // r = b - Ax; p = v = d = 0; rt = r; partial r.conj(r) (qmrbicgstab.f:50-62)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) q_start(QV<V> v, int x0zero, long long n,
+__global__ void __launch_bounds__(kThreads) q_start(QV<V> v, int x0zero, VN n,
                                                    double2* part, int nblk) {
   const int rh_ = blockIdx.y;  // right-hand side
-  v = v.shifted(rh_ * n);
+  v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
   double2 acc[1] = {make_double2(0.0, 0.0)};
   const double2 z0 = make_double2(0.0, 0.0);
-  GRID_LOOP(i, n) {
+  GRID_LOOP(i, n.len(rh_)) {
     double2 r = LD(v.b, i);
     if (!x0zero) r = zsub(r, LD(v.Ax, i));
     ST(v.r, i, r);
@@ -478,15 +489,15 @@ __global__ void __launch_bounds__(kThreads) q_fstart(KState* st, const double2* 
 
 // partial rt* r (qmrbicgstab.f:79-87)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) q_k1(const KState* st, QV<V> v, long long n,
+__global__ void __launch_bounds__(kThreads) q_k1(const KState* st, QV<V> v, VN n,
                                                 double2* part, int nblk) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
-  v = v.shifted(rh_ * n);
+  v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   double2 acc[1] = {make_double2(0.0, 0.0)};
-  GRID_LOOP(i, n) acc[0] = zadd(acc[0], zmul(zconj(LD(v.rt, i)), LD(v.r, i)));
+  GRID_LOOP(i, n.len(rh_)) acc[0] = zadd(acc[0], zmul(zconj(LD(v.rt, i)), LD(v.r, i)));
   block_partials<1>(acc, part, nblk);
 }
 
@@ -505,27 +516,27 @@ __global__ void __launch_bounds__(kThreads) q_f1(KState* st, const double2* part
 
 // p = r + beta (p - omega v) (qmrbicgstab.f:92-100)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) q_k2(const KState* st, QV<V> v, long long n) {
+__global__ void __launch_bounds__(kThreads) q_k2(const KState* st, QV<V> v, VN n) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
-  v = v.shifted(rh_ * n);
+  v = v.shifted(n.off(rh_));
   if (st->done) return;
   const double2 be = st->beta, om = st->omega;
-  GRID_LOOP(i, n)
+  GRID_LOOP(i, n.len(rh_))
   ST(v.p, i, zadd(LD(v.r, i), zmul(be, zsub(LD(v.p, i), zmul(om, LD(v.v, i))))));
 }
 
 // partial rt* v (qmrbicgstab.f:105-113)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) q_k3(const KState* st, QV<V> v, long long n,
+__global__ void __launch_bounds__(kThreads) q_k3(const KState* st, QV<V> v, VN n,
                                                 double2* part, int nblk) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
-  v = v.shifted(rh_ * n);
+  v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   double2 acc[1] = {make_double2(0.0, 0.0)};
-  GRID_LOOP(i, n) acc[0] = zadd(acc[0], zmul(zconj(LD(v.rt, i)), LD(v.v, i)));
+  GRID_LOOP(i, n.len(rh_)) acc[0] = zadd(acc[0], zmul(zconj(LD(v.rt, i)), LD(v.v, i)));
   block_partials<1>(acc, part, nblk);
 }
 
@@ -549,16 +560,16 @@ __global__ void __launch_bounds__(kThreads) q_f3(KState* st, const double2* part
 
 // s = r - alpha v; partial s* s (qmrbicgstab.f:123-131)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) q_k4(const KState* st, QV<V> v, long long n,
+__global__ void __launch_bounds__(kThreads) q_k4(const KState* st, QV<V> v, VN n,
                                                 double2* part, int nblk) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
-  v = v.shifted(rh_ * n);
+  v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   const double2 al = st->alpha;
   double2 acc[1] = {make_double2(0.0, 0.0)};
-  GRID_LOOP(i, n) {
+  GRID_LOOP(i, n.len(rh_)) {
     ST(v.s, i, zsub(LD(v.r, i), zmul(al, LD(v.v, i))));
     const double2 s = LD(v.s, i);
     acc[0] = zadd(acc[0], zmul(zconj(s), s));
@@ -589,14 +600,14 @@ __global__ void __launch_bounds__(kThreads) q_f4(KState* st, const double2* part
 
 // dt = p + theta^2 eta/alpha d; xt = x + etat dt (qmrbicgstab.f:151-159)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) q_k5(const KState* st, QV<V> v, long long n) {
+__global__ void __launch_bounds__(kThreads) q_k5(const KState* st, QV<V> v, VN n) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
-  v = v.shifted(rh_ * n);
+  v = v.shifted(n.off(rh_));
   if (st->done) return;
   const double2 f = zdiv(zmul(zmul(st->theta, st->theta), st->eta), st->alpha);
   const double2 et = st->etat;
-  GRID_LOOP(i, n) {
+  GRID_LOOP(i, n.len(rh_)) {
     const double2 dt = zadd(LD(v.p, i), zmul(f, LD(v.d, i)));
     ST(v.dt, i, dt);
     ST(v.xt, i, zadd(LD(v.x, i), zmul(et, LD(v.dt, i))));
@@ -605,16 +616,16 @@ __global__ void __launch_bounds__(kThreads) q_k5(const KState* st, QV<V> v, long
 
 // omega dots (qmrbicgstab.f:163-185)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) q_k6(const KState* st, QV<V> v, long long n,
+__global__ void __launch_bounds__(kThreads) q_k6(const KState* st, QV<V> v, VN n,
                                                 double2* part, int nblk) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
-  v = v.shifted(rh_ * n);
+  v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   double2 acc[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
   const bool m1 = st->methode == 1;
-  GRID_LOOP(i, n) {
+  GRID_LOOP(i, n.len(rh_)) {
     const double2 t = LD(v.t, i), s = LD(v.s, i);
     if (m1) {
       acc[0] = zadd(acc[0], zmul(zconj(t), t));
@@ -647,16 +658,16 @@ __global__ void __launch_bounds__(kThreads) q_f6(KState* st, const double2* part
 
 // r = s - omega t; partial r* r (qmrbicgstab.f:197-205)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) q_k7(const KState* st, QV<V> v, long long n,
+__global__ void __launch_bounds__(kThreads) q_k7(const KState* st, QV<V> v, VN n,
                                                 double2* part, int nblk) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
-  v = v.shifted(rh_ * n);
+  v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   const double2 om = st->omega;
   double2 acc[1] = {make_double2(0.0, 0.0)};
-  GRID_LOOP(i, n) {
+  GRID_LOOP(i, n.len(rh_)) {
     ST(v.r, i, zsub(LD(v.s, i), zmul(om, LD(v.t, i))));
     const double2 r = LD(v.r, i);
     acc[0] = zadd(acc[0], zmul(zconj(r), r));
@@ -693,14 +704,14 @@ __global__ void __launch_bounds__(kThreads) q_f7(KState* st, const double2* part
 
 // d = s + thetat^2 etat/omega dt; x = xt + eta d (qmrbicgstab.f:231-238)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) q_k8(const KState* st, QV<V> v, long long n) {
+__global__ void __launch_bounds__(kThreads) q_k8(const KState* st, QV<V> v, VN n) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
-  v = v.shifted(rh_ * n);
+  v = v.shifted(n.off(rh_));
   if (st->done) return;
   const double2 f = zdiv(zmul(zmul(st->thetat, st->thetat), st->etat), st->omega);
   const double2 et = st->eta;
-  GRID_LOOP(i, n) {
+  GRID_LOOP(i, n.len(rh_)) {
     ST(v.d, i, zadd(LD(v.s, i), zmul(f, LD(v.dt, i))));
     ST(v.x, i, zadd(LD(v.xt, i), zmul(et, LD(v.d, i))));
   }
@@ -738,10 +749,10 @@ template <typename V> struct TV {  // tfqmr.f:34-41 work columns
 
 // r = b - Ax; w = u = r (tfqmr.f:59-69)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) t_start(TV<V> v, int x0zero, long long n) {
+__global__ void __launch_bounds__(kThreads) t_start(TV<V> v, int x0zero, VN n) {
   const int rh_ = blockIdx.y;  // right-hand side
-  v = v.shifted(rh_ * n);
-  GRID_LOOP(i, n) {
+  v = v.shifted(n.off(rh_));
+  GRID_LOOP(i, n.len(rh_)) {
     double2 r = LD(v.b, i);
     if (!x0zero) r = zsub(r, LD(v.Ax, i));
     ST(v.r, i, r);
@@ -753,13 +764,13 @@ __global__ void __launch_bounds__(kThreads) t_start(TV<V> v, int x0zero, long lo
 // v = Au (given in v.v by the matvec); d = 0; Au = v; rt = r; partial rt* r
 // (tfqmr.f:73-84)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) t_start2(TV<V> v, long long n, double2* part,
+__global__ void __launch_bounds__(kThreads) t_start2(TV<V> v, VN n, double2* part,
                                                     int nblk) {
   const int rh_ = blockIdx.y;  // right-hand side
-  v = v.shifted(rh_ * n);
+  v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
   double2 acc[1] = {make_double2(0.0, 0.0)};
-  GRID_LOOP(i, n) {
+  GRID_LOOP(i, n.len(rh_)) {
     ST(v.d, i, make_double2(0.0, 0.0));
     ST(v.Au, i, LD(v.v, i));
     const double2 r = LD(v.r, i);
@@ -794,15 +805,15 @@ __global__ void __launch_bounds__(kThreads) t_fstart(KState* st, const double2* 
 
 // even step: partial rt* v (tfqmr.f:104-112)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) t_k1(const KState* st, TV<V> v, long long n,
+__global__ void __launch_bounds__(kThreads) t_k1(const KState* st, TV<V> v, VN n,
                                                 double2* part, int nblk) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
-  v = v.shifted(rh_ * n);
+  v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   double2 acc[1] = {make_double2(0.0, 0.0)};
-  GRID_LOOP(i, n) acc[0] = zadd(acc[0], zmul(zconj(LD(v.rt, i)), LD(v.v, i)));
+  GRID_LOOP(i, n.len(rh_)) acc[0] = zadd(acc[0], zmul(zconj(LD(v.rt, i)), LD(v.v, i)));
   block_partials<1>(acc, part, nblk);
 }
 
@@ -818,17 +829,17 @@ __global__ void __launch_bounds__(kThreads) t_f1(KState* st, const double2* part
 // (even) u1 = u - alpha v; w = w - alpha Au; d = u + theta^2/alpha eta d;
 // partial w* w (tfqmr.f:115-136)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) t_k2(const KState* st, TV<V> v, int even, long long n,
+__global__ void __launch_bounds__(kThreads) t_k2(const KState* st, TV<V> v, int even, VN n,
                                                 double2* part, int nblk) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
-  v = v.shifted(rh_ * n);
+  v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   const double2 al = st->alpha;
   const double2 f = zmul(zdiv(zmul(st->theta, st->theta), al), st->eta);
   double2 acc[1] = {make_double2(0.0, 0.0)};
-  GRID_LOOP(i, n) {
+  GRID_LOOP(i, n.len(rh_)) {
     const double2 u = LD(v.u, i);
     if (even) ST(v.u1, i, zsub(u, zmul(al, LD(v.v, i))));
     ST(v.w, i, zsub(LD(v.w, i), zmul(al, LD(v.Au, i))));
@@ -856,13 +867,13 @@ __global__ void __launch_bounds__(kThreads) t_f2(KState* st, const double2* part
 
 // x = x + eta d (tfqmr.f:147-153)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) t_k3(const KState* st, TV<V> v, long long n) {
+__global__ void __launch_bounds__(kThreads) t_k3(const KState* st, TV<V> v, VN n) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
-  v = v.shifted(rh_ * n);
+  v = v.shifted(n.off(rh_));
   if (st->done) return;
   const double2 et = st->eta;
-  GRID_LOOP(i, n) ST(v.x, i, zadd(LD(v.x, i), zmul(et, LD(v.d, i))));
+  GRID_LOOP(i, n.len(rh_)) ST(v.x, i, zadd(LD(v.x, i), zmul(et, LD(v.d, i))));
 }
 
 // residual bound (tfqmr.f:156-160)
@@ -881,15 +892,15 @@ __global__ void t_f3(KState* st, double2* trace) {
 
 // (odd) partial rt* w (tfqmr.f:163-173)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) t_k4(const KState* st, TV<V> v, long long n,
+__global__ void __launch_bounds__(kThreads) t_k4(const KState* st, TV<V> v, VN n,
                                                 double2* part, int nblk) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
-  v = v.shifted(rh_ * n);
+  v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   double2 acc[1] = {make_double2(0.0, 0.0)};
-  GRID_LOOP(i, n) acc[0] = zadd(acc[0], zmul(zconj(LD(v.rt, i)), LD(v.w, i)));
+  GRID_LOOP(i, n.len(rh_)) acc[0] = zadd(acc[0], zmul(zconj(LD(v.rt, i)), LD(v.w, i)));
   block_partials<1>(acc, part, nblk);
 }
 
@@ -908,13 +919,13 @@ __global__ void __launch_bounds__(kThreads) t_f4(KState* st, const double2* part
 
 // (odd) u1 = w + beta u; Auo = Au (tfqmr.f:176-183); then u = u1 (:187-194)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) t_k5(const KState* st, TV<V> v, int odd, long long n) {
+__global__ void __launch_bounds__(kThreads) t_k5(const KState* st, TV<V> v, int odd, VN n) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
-  v = v.shifted(rh_ * n);
+  v = v.shifted(n.off(rh_));
   if (st->done) return;
   const double2 be = st->beta;
-  GRID_LOOP(i, n) {
+  GRID_LOOP(i, n.len(rh_)) {
     if (odd) {
       ST(v.u, i, zadd(LD(v.w, i), zmul(be, LD(v.u, i))));
       ST(v.Auo, i, LD(v.Au, i));
@@ -926,13 +937,13 @@ __global__ void __launch_bounds__(kThreads) t_k5(const KState* st, TV<V> v, int 
 
 // (odd) v = Au + beta (Auo + beta v) (tfqmr.f:206-214)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) t_k6(const KState* st, TV<V> v, long long n) {
+__global__ void __launch_bounds__(kThreads) t_k6(const KState* st, TV<V> v, VN n) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
-  v = v.shifted(rh_ * n);
+  v = v.shifted(n.off(rh_));
   if (st->done) return;
   const double2 be = st->beta;
-  GRID_LOOP(i, n)
+  GRID_LOOP(i, n.len(rh_))
   ST(v.v, i, zadd(LD(v.Au, i), zmul(be, zadd(LD(v.Auo, i), zmul(be, LD(v.v, i))))));
 }
 
@@ -953,14 +964,14 @@ __global__ void t_f5(KState* st) {
 // ===================================================== driver / refinement
 // res = Ax - b, partial |res|^2 ; FF = alpha x (inverserig.f:95-105)
 template <typename V>
-__global__ void __launch_bounds__(kThreads) k_resid(const V* Ax, const V* b, long long n,
+__global__ void __launch_bounds__(kThreads) k_resid(const V* Ax, const V* b, VN n,
                                                    double2* part, int nblk) {
   const int rh_ = blockIdx.y;  // right-hand side
   part += (long long)rh_ * kMaxDots * nblk;
-  Ax += rh_ * n;
-  b += rh_ * n;
+  Ax += n.off(rh_);
+  b += n.off(rh_);
   double2 acc[1] = {make_double2(0.0, 0.0)};
-  GRID_LOOP(i, n) {
+  GRID_LOOP(i, n.len(rh_)) {
     const double2 e = zsub(LD(Ax, i), LD(b, i));
     acc[0].x += e.x * e.x + e.y * e.y;
   }
@@ -1006,15 +1017,15 @@ __global__ void __launch_bounds__(kThreads) k_alpha_apply(const double2* x, doub
 // r = b - Ax (FP64), cast to FP32 for the inner solve; partial |r|^2
 __global__ void __launch_bounds__(kThreads) k_refine_resid(const double2* b, const double2* Ax,
                                                           int have_ax, double2* r64, float2* r32,
-                                                          long long n, double2* part, int nblk) {
+                                                          VN n, double2* part, int nblk) {
   const int rh_ = blockIdx.y;  // right-hand side
   part += (long long)rh_ * kMaxDots * nblk;
-  b += rh_ * n;
-  Ax += rh_ * n;
-  r64 += rh_ * n;
-  r32 += rh_ * n;
+  b += n.off(rh_);
+  Ax += n.off(rh_);
+  r64 += n.off(rh_);
+  r32 += n.off(rh_);
   double2 acc[1] = {make_double2(0.0, 0.0)};
-  GRID_LOOP(i, n) {
+  GRID_LOOP(i, n.len(rh_)) {
     double2 r = b[i];
     if (have_ax) r = zsub(r, Ax[i]);
     r64[i] = r;
@@ -1025,13 +1036,13 @@ __global__ void __launch_bounds__(kThreads) k_refine_resid(const double2* b, con
 }
 
 // x += e (FP32 correction into the FP64 iterate)
-__global__ void __launch_bounds__(kThreads) k_refine_add(double2* x, const float2* e, long long n,
+__global__ void __launch_bounds__(kThreads) k_refine_add(double2* x, const float2* e, VN n,
                                                         const int* skip) {
   const int rh_ = blockIdx.y;  // right-hand side
   if (skip[rh_]) return;
-  x += rh_ * n;
-  e += rh_ * n;
-  GRID_LOOP(i, n) {
+  x += n.off(rh_);
+  e += n.off(rh_);
+  GRID_LOOP(i, n.len(rh_)) {
     const float2 v = e[i];
     x[i] = make_double2(x[i].x + (double)v.x, x[i].y + (double)v.y);
   }
@@ -1055,10 +1066,15 @@ struct Solver {
   double2* trace = nullptr;  // per-iteration scalars of RHS 0 (single-RHS solves)
   int tracecap = 0;
   double2* rr64 = nullptr;   // refinement residual (FP64), R * n
+  long long* lens = nullptr; // ragged layout (packed problems): per-RHS lengths
+  long long* offs = nullptr; // and offsets, else null (RHS r at r * n)
+  long long total = 0;       // elements per vector array (R * n, or the ragged sum)
+  long long cap_total = 0;   // capacity of the buffers: elements, right-hand sides
+  int cap_R = 0;
+  int part_blocks = 0;       // blocks per RHS the partial-sum array holds
 };
 
-void solver_free(ifdda_ctx* c) {
-  Solver* S = (Solver*)c->solver;
+void solver_delete(Solver* S) {
   if (!S) return;
   for (int i = 0; i < 14; ++i) {
     (void)hipFree(S->v64[i]);
@@ -1070,8 +1086,86 @@ void solver_free(ifdda_ctx* c) {
   (void)hipFree(S->act);
   (void)hipFree(S->trace);
   (void)hipFree(S->rr64);
+  (void)hipFree(S->lens);
+  (void)hipFree(S->offs);
   delete S;
+}
+
+void solver_free(ifdda_ctx* c) {
+  solver_delete((Solver*)c->solver);
   c->solver = nullptr;
+}
+
+// Buffers of a solver: vectors of `total` elements for R right-hand sides
+// of at most n elements each (R * n when the RHS lie n apart), FP32 copies
+// when need32.
+static int blocks_for(long long n) {
+  return (int)std::max<long long>(1, std::min<long long>(kMaxBlocks, (n + kThreads - 1) / kThreads));
+}
+
+static bool solver_alloc(Solver* S, long long n, int R, bool need32, long long total,
+                         int part_blocks) {
+  S->n = n;
+  S->R = R;
+  S->total = total;
+  S->nblk = blocks_for(n);
+  S->cap_total = total;
+  S->cap_R = R;
+  S->part_blocks = std::max(part_blocks, S->nblk);
+  const size_t nv = (size_t)std::max<long long>(total, 1);
+  bool ok = hip_ok(hipMalloc(&S->part, sizeof(double2) * kMaxDots * S->part_blocks * R), "hipMalloc partials") &&
+            hip_ok(hipMalloc(&S->st, sizeof(KState) * R), "hipMalloc state") &&
+            hip_ok(hipHostMalloc(&S->st_host, sizeof(KState) * R), "hipHostMalloc state") &&
+            hip_ok(hipMalloc(&S->act, sizeof(int) * R), "hipMalloc flags");
+  for (int i = 0; ok && i < 14; ++i)
+    ok = hip_ok(hipMalloc(&S->v64[i], sizeof(double2) * nv), "hipMalloc krylov");
+  if (ok && need32) {
+    ok = hip_ok(hipMalloc(&S->rr64, sizeof(double2) * nv), "hipMalloc resid");
+    for (int i = 0; ok && i < 14; ++i)
+      ok = hip_ok(hipMalloc(&S->v32[i], sizeof(float2) * nv), "hipMalloc krylov32");
+  }
+  return ok;
+}
+
+Solver* solver_new(int R, bool need32, const long long* lens, const long long* offs) {
+  long long n = 1, total = 0;
+  for (int r = 0; r < R; ++r) {
+    n = std::max(n, lens[r]);
+    total = std::max(total, offs[r] + lens[r]);
+  }
+  Solver* S = new Solver();
+  bool ok = solver_alloc(S, n, R, need32, total, kMaxBlocks) &&
+            hip_ok(hipMalloc(&S->lens, sizeof(long long) * R), "hipMalloc lengths") &&
+            hip_ok(hipMalloc(&S->offs, sizeof(long long) * R), "hipMalloc offsets") &&
+            hip_ok(hipMemcpy(S->lens, lens, sizeof(long long) * R, hipMemcpyHostToDevice),
+                   "copy lengths") &&
+            hip_ok(hipMemcpy(S->offs, offs, sizeof(long long) * R, hipMemcpyHostToDevice),
+                   "copy offsets");
+  if (!ok) {
+    solver_delete(S);
+    return nullptr;
+  }
+  return S;
+}
+
+bool solver_refit(Solver* S, int R, bool need32, const long long* lens, const long long* offs,
+                  hipStream_t s) {
+  long long n = 1, total = 0;
+  for (int r = 0; r < R; ++r) {
+    n = std::max(n, lens[r]);
+    total = std::max(total, offs[r] + lens[r]);
+  }
+  if (!S || total > S->cap_total || R > S->cap_R || (need32 && !S->v32[0]) ||
+      blocks_for(n) > S->part_blocks || !S->lens)
+    return false;
+  S->n = n;
+  S->R = R;
+  S->total = total;
+  S->nblk = blocks_for(n);
+  return hip_ok(hipMemcpyAsync(S->lens, lens, sizeof(long long) * R, hipMemcpyHostToDevice, s),
+                "copy lengths") &&
+         hip_ok(hipMemcpyAsync(S->offs, offs, sizeof(long long) * R, hipMemcpyHostToDevice, s),
+                "copy offsets");
 }
 
 static Solver* solver_get(ifdda_ctx* c, int R, bool need32) {
@@ -1096,17 +1190,7 @@ static Solver* solver_get(ifdda_ctx* c, int R, bool need32) {
       S = new Solver();
       c->solver = S;
     }
-    S->n = n;
-    S->R = R;
-    S->nblk = (int)std::max<long long>(1, std::min<long long>(kMaxBlocks, (n + kThreads - 1) / kThreads));
-    const size_t nv = (size_t)std::max<long long>(n, 1) * R;
-    bool ok = hip_ok(hipMalloc(&S->part, sizeof(double2) * kMaxDots * S->nblk * R), "hipMalloc partials") &&
-              hip_ok(hipMalloc(&S->st, sizeof(KState) * R), "hipMalloc state") &&
-              hip_ok(hipHostMalloc(&S->st_host, sizeof(KState) * R), "hipHostMalloc state") &&
-              hip_ok(hipMalloc(&S->act, sizeof(int) * R), "hipMalloc flags");
-    for (int i = 0; ok && i < 14; ++i)
-      ok = hip_ok(hipMalloc(&S->v64[i], sizeof(double2) * nv), "hipMalloc krylov");
-    if (!ok) {
+    if (!solver_alloc(S, n, R, false, n * R, 0)) {
       solver_free(c);
       return nullptr;
     }
@@ -1121,28 +1205,57 @@ static Solver* solver_get(ifdda_ctx* c, int R, bool need32) {
       return nullptr;
     }
   }
+  S->total = n * R;  // this call's right-hand sides (the buffers may hold more)
   return S;
 }
 
 // Batched matvec over R right-hand sides (vectors n apart); skip flags
 // skip[r * skip_stride] drop converged RHS on the device.
-template <typename V> static bool mv(ifdda_ctx* c, const V* x, const V* b, V* y, hipStream_t s,
+template <typename V> static bool mv(const MvOp& op, const V* x, const V* b, V* y, hipStream_t s,
                                      const int* skip, int R, long long n, int skip_stride);
-template <> bool mv<double2>(ifdda_ctx* c, const double2* x, const double2* b, double2* y,
+template <> bool mv<double2>(const MvOp& op, const double2* x, const double2* b, double2* y,
                              hipStream_t s, const int* skip, int R, long long n, int skip_stride) {
-  MatvecDev<double2> m = c->mv64();
-  m.nrhs = R;
-  m.vstride = n;
-  m.skip_stride = skip_stride;
-  return launch_matvec_f64(c->g, m, x, b, y, s, skip);
+  return op.mv64(x, b, y, s, skip, R, n, skip_stride);
 }
-template <> bool mv<float2>(ifdda_ctx* c, const float2* x, const float2* b, float2* y,
+template <> bool mv<float2>(const MvOp& op, const float2* x, const float2* b, float2* y,
                             hipStream_t s, const int* skip, int R, long long n, int skip_stride) {
-  MatvecDev<float2> m = c->mv32();
-  m.nrhs = R;
-  m.vstride = n;
-  m.skip_stride = skip_stride;
-  return launch_matvec_f32(c->g, m, x, b, y, s, skip);
+  return op.mv32(x, b, y, s, skip, R, n, skip_stride);
+}
+
+// The operator of one context: its box, Green tensor, dipoles and
+// polarisability, shared by all R right-hand sides.
+struct CtxOp : MvOp {
+  ifdda_ctx* c;
+  explicit CtxOp(ifdda_ctx* c_) : c(c_) {}
+  bool mv64(const double2* x, const double2* b, double2* y, hipStream_t s, const int* skip, int R,
+            long long n, int skip_stride) const override {
+    MatvecDev<double2> m = c->mv64();
+    m.nrhs = R;
+    m.vstride = n;
+    m.skip_stride = skip_stride;
+    return launch_matvec_f64(c->g, m, x, b, y, s, skip);
+  }
+  bool mv32(const float2* x, const float2* b, float2* y, hipStream_t s, const int* skip, int R,
+            long long n, int skip_stride) const override {
+    MatvecDev<float2> m = c->mv32();
+    m.nrhs = R;
+    m.vstride = n;
+    m.skip_stride = skip_stride;
+    return launch_matvec_f32(c->g, m, x, b, y, s, skip);
+  }
+  bool alpha_apply(const double2* x, double2* FF, int R, long long n, hipStream_t s) const override {
+    const int nb = (int)std::max<long long>(1, std::min<long long>(kMaxBlocks, (n + kThreads - 1) / kThreads));
+    hipLaunchKernelGGL(k_alpha_apply, dim3(nb, R), dim3(kThreads), 0, s, x, FF,
+                       (const double2*)c->al64, c->alpha_mode, c->nd);
+    return hip_ok(hipGetLastError(), "alpha apply");
+  }
+};
+
+bool alpha_apply_one(const double2* x, double2* FF, const double2* al, int mode, long long nd,
+                     hipStream_t s) {
+  const int nb = (int)std::max<long long>(1, std::min<long long>(kMaxBlocks, (3 * nd + kThreads - 1) / kThreads));
+  hipLaunchKernelGGL(k_alpha_apply, dim3(nb, 1), dim3(kThreads), 0, s, x, FF, al, mode, nd);
+  return hip_ok(hipGetLastError(), "alpha apply");
 }
 
 // Outcome of one Krylov run for one right-hand side.
@@ -1167,10 +1280,11 @@ static bool all_done(const Solver* S, int R) {
 // tols[r]: tolerance per RHS; active[r] == 0 starts RHS r as finished.
 // Returns false on a HIP error; *cancelled when nstop became -1.
 template <typename V>
-static bool krylov(ifdda_ctx* c, Solver* S, V** vv, int R, int method, const double* tols,
+static bool krylov(const MvOp& op, Solver* S, V** vv, int R, int method, const double* tols,
                    const char* active, int maxit, int x0zero, int check_every,
                    volatile int* nstop, hipStream_t s, KRun* out, bool* cancelled) {
   const long long n = S->n;
+  const VN vn{n, S->lens, S->offs};
   const int nb = S->nblk;
   const dim3 G(nb, R), T(kThreads), F(1, R), One(1);
   double2* trace = R == 1 ? S->trace : nullptr;
@@ -1197,10 +1311,10 @@ static bool krylov(ifdda_ctx* c, Solver* S, V** vv, int R, int method, const dou
   V* b = vv[1];
   V* Ax = vv[13];
   hipLaunchKernelGGL(k_init<V>, G, T, 0, s, (const V*)b, x, x0zero ? (const V*)nullptr : (const V*)x,
-                     n, S->part, nb);
+                     vn, S->part, nb);
   hipLaunchKernelGGL(f_init, F, T, 0, s, S->st, (const double2*)S->part, nb);
   if (!x0zero) {
-    if (!mv<V>(c, x, x, Ax, s, done, R, n, dstride)) return false;
+    if (!mv<V>(op, x, x, Ax, s, done, R, n, dstride)) return false;
     ++nmv0;
   }
   // Poll the device state every 8 iterations by default: work queued after
@@ -1223,22 +1337,22 @@ static bool krylov(ifdda_ctx* c, Solver* S, V** vv, int R, int method, const dou
     V* const tb[2] = {vv[6], vv[12]};  // t / told
     V* const ab[2] = {vv[5], vv[10]};  // Ap / Apold
     GV<V> v{x, b, vv[2], vv[3], vv[4], ab[0], tb[0], vv[7], vv[8], vv[9], ab[1], vv[11], tb[1], Ax};
-    hipLaunchKernelGGL(g_start<V>, G, T, 0, s, v, x0zero, n, S->part, nb);
+    hipLaunchKernelGGL(g_start<V>, G, T, 0, s, v, x0zero, vn, S->part, nb);
     hipLaunchKernelGGL(g_fstart, F, T, 0, s, S->st, (const double2*)S->part, nb);
     for (int it = 0;; ++it) {
       v.told = tb[it & 1];
       v.t = tb[(it + 1) & 1];
       v.Ap = ab[it & 1];
       v.Apold = ab[(it + 1) & 1];
-      hipLaunchKernelGGL(g_k1<V>, G, T, 0, s, (const KState*)S->st, v, n);
-      if (!mv<V>(c, v.p, v.p, v.Ap, s, done, R, n, dstride)) return false;
-      hipLaunchKernelGGL(g_k2<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
+      hipLaunchKernelGGL(g_k1<V>, G, T, 0, s, (const KState*)S->st, v, vn);
+      if (!mv<V>(op, v.p, v.p, v.Ap, s, done, R, n, dstride)) return false;
+      hipLaunchKernelGGL(g_k2<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
       hipLaunchKernelGGL(g_f2, F, T, 0, s, S->st, (const double2*)S->part, nb);
-      hipLaunchKernelGGL(g_k3<V>, G, T, 0, s, (const KState*)S->st, v, n);
-      if (!mv<V>(c, v.t, v.t, v.At, s, done, R, n, dstride)) return false;
-      hipLaunchKernelGGL(g_k4<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
+      hipLaunchKernelGGL(g_k3<V>, G, T, 0, s, (const KState*)S->st, v, vn);
+      if (!mv<V>(op, v.t, v.t, v.At, s, done, R, n, dstride)) return false;
+      hipLaunchKernelGGL(g_k4<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
       hipLaunchKernelGGL(g_f4, F, T, 0, s, S->st, (const double2*)S->part, nb);
-      hipLaunchKernelGGL(g_k5<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
+      hipLaunchKernelGGL(g_k5<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
       hipLaunchKernelGGL(g_f5, F, T, 0, s, S->st, (const double2*)S->part, nb, trace);
       if (!hip_ok(hipGetLastError(), "gpbicg launch")) return false;
       const int st = step_poll(it);
@@ -1258,24 +1372,24 @@ static bool krylov(ifdda_ctx* c, Solver* S, V** vv, int R, int method, const dou
     }
   } else if (method == IFDDA_QMRBICGSTAB1 || method == IFDDA_QMRBICGSTAB2) {
     QV<V> v{x, b, vv[2], vv[3], vv[4], vv[5], vv[6], vv[7], vv[8], vv[9], vv[10], Ax};
-    hipLaunchKernelGGL(q_start<V>, G, T, 0, s, v, x0zero, n, S->part, nb);
+    hipLaunchKernelGGL(q_start<V>, G, T, 0, s, v, x0zero, vn, S->part, nb);
     hipLaunchKernelGGL(q_fstart, F, T, 0, s, S->st, (const double2*)S->part, nb);
     for (int it = 0;; ++it) {
-      hipLaunchKernelGGL(q_k1<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
+      hipLaunchKernelGGL(q_k1<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
       hipLaunchKernelGGL(q_f1, F, T, 0, s, S->st, (const double2*)S->part, nb);
-      hipLaunchKernelGGL(q_k2<V>, G, T, 0, s, (const KState*)S->st, v, n);
-      if (!mv<V>(c, v.p, v.p, v.v, s, done, R, n, dstride)) return false;
-      hipLaunchKernelGGL(q_k3<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
+      hipLaunchKernelGGL(q_k2<V>, G, T, 0, s, (const KState*)S->st, v, vn);
+      if (!mv<V>(op, v.p, v.p, v.v, s, done, R, n, dstride)) return false;
+      hipLaunchKernelGGL(q_k3<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
       hipLaunchKernelGGL(q_f3, F, T, 0, s, S->st, (const double2*)S->part, nb);
-      hipLaunchKernelGGL(q_k4<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
+      hipLaunchKernelGGL(q_k4<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
       hipLaunchKernelGGL(q_f4, F, T, 0, s, S->st, (const double2*)S->part, nb);
-      hipLaunchKernelGGL(q_k5<V>, G, T, 0, s, (const KState*)S->st, v, n);
-      if (!mv<V>(c, v.s, v.s, v.t, s, done, R, n, dstride)) return false;
-      hipLaunchKernelGGL(q_k6<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
+      hipLaunchKernelGGL(q_k5<V>, G, T, 0, s, (const KState*)S->st, v, vn);
+      if (!mv<V>(op, v.s, v.s, v.t, s, done, R, n, dstride)) return false;
+      hipLaunchKernelGGL(q_k6<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
       hipLaunchKernelGGL(q_f6, F, T, 0, s, S->st, (const double2*)S->part, nb);
-      hipLaunchKernelGGL(q_k7<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
+      hipLaunchKernelGGL(q_k7<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
       hipLaunchKernelGGL(q_f7, F, T, 0, s, S->st, (const double2*)S->part, nb);
-      hipLaunchKernelGGL(q_k8<V>, G, T, 0, s, (const KState*)S->st, v, n);
+      hipLaunchKernelGGL(q_k8<V>, G, T, 0, s, (const KState*)S->st, v, vn);
       hipLaunchKernelGGL(q_f8, F, One, 0, s, S->st, trace);
       if (!hip_ok(hipGetLastError(), "qmrbicgstab launch")) return false;
       const int st = step_poll(it);
@@ -1295,27 +1409,27 @@ static bool krylov(ifdda_ctx* c, Solver* S, V** vv, int R, int method, const dou
     }
   } else if (method == IFDDA_TFQMR) {
     TV<V> v{x, b, vv[2], vv[3], vv[4], vv[5], vv[6], vv[7], vv[8], vv[9], vv[10], Ax};
-    hipLaunchKernelGGL(t_start<V>, G, T, 0, s, v, x0zero, n);
-    if (!mv<V>(c, v.u, v.u, v.v, s, done, R, n, dstride)) return false;
-    hipLaunchKernelGGL(t_start2<V>, G, T, 0, s, v, n, S->part, nb);
+    hipLaunchKernelGGL(t_start<V>, G, T, 0, s, v, x0zero, vn);
+    if (!mv<V>(op, v.u, v.u, v.v, s, done, R, n, dstride)) return false;
+    hipLaunchKernelGGL(t_start2<V>, G, T, 0, s, v, vn, S->part, nb);
     hipLaunchKernelGGL(t_fstart, F, T, 0, s, S->st, (const double2*)S->part, nb);
     for (int it = 0;; ++it) {
       const int even = (it % 2 == 0);
       if (even) {
-        hipLaunchKernelGGL(t_k1<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
+        hipLaunchKernelGGL(t_k1<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
         hipLaunchKernelGGL(t_f1, F, T, 0, s, S->st, (const double2*)S->part, nb);
       }
-      hipLaunchKernelGGL(t_k2<V>, G, T, 0, s, (const KState*)S->st, v, even, n, S->part, nb);
+      hipLaunchKernelGGL(t_k2<V>, G, T, 0, s, (const KState*)S->st, v, even, vn, S->part, nb);
       hipLaunchKernelGGL(t_f2, F, T, 0, s, S->st, (const double2*)S->part, nb);
-      hipLaunchKernelGGL(t_k3<V>, G, T, 0, s, (const KState*)S->st, v, n);
+      hipLaunchKernelGGL(t_k3<V>, G, T, 0, s, (const KState*)S->st, v, vn);
       hipLaunchKernelGGL(t_f3, F, One, 0, s, S->st, trace);
       if (!even) {
-        hipLaunchKernelGGL(t_k4<V>, G, T, 0, s, (const KState*)S->st, v, n, S->part, nb);
+        hipLaunchKernelGGL(t_k4<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
         hipLaunchKernelGGL(t_f4, F, T, 0, s, S->st, (const double2*)S->part, nb);
       }
-      hipLaunchKernelGGL(t_k5<V>, G, T, 0, s, (const KState*)S->st, v, !even, n);
-      if (!mv<V>(c, v.u, v.u, v.Au, s, done, R, n, dstride)) return false;
-      if (!even) hipLaunchKernelGGL(t_k6<V>, G, T, 0, s, (const KState*)S->st, v, n);
+      hipLaunchKernelGGL(t_k5<V>, G, T, 0, s, (const KState*)S->st, v, !even, vn);
+      if (!mv<V>(op, v.u, v.u, v.Au, s, done, R, n, dstride)) return false;
+      if (!even) hipLaunchKernelGGL(t_k6<V>, G, T, 0, s, (const KState*)S->st, v, vn);
       hipLaunchKernelGGL(t_f5, F, One, 0, s, S->st);
       if (!hip_ok(hipGetLastError(), "tfqmr launch")) return false;
       const int st = step_poll(it);
@@ -1379,20 +1493,27 @@ static int solve_dev(ifdda_ctx* c, const ifdda_solve_opts* o, int R, const doubl
     return IFDDA_ERR_STATE;
   }
   if (ensure_scratch(c, R) != IFDDA_OK) return IFDDA_ERR_HIP;
-  const int maxit = o->maxit > 0 ? o->maxit : 10000;
   Solver* S = solver_get(c, R, o->precision == 32);
   if (!S) return IFDDA_ERR_HIP;
+  return solve_core(CtxOp(c), S, o, R, Einc, FF, FFloc, infos, nstop, s, t0);
+}
+
+int solve_core(const MvOp& op, Solver* S, const ifdda_solve_opts* o, int R, const double2* Einc,
+               double2* FF, double2* FFloc, ifdda_solve_info* infos, volatile int* nstop,
+               hipStream_t s, std::chrono::steady_clock::time_point t0) {
+  const int maxit = o->maxit > 0 ? o->maxit : 10000;
   const long long n = S->n;
+  const VN vn{n, S->lens, S->offs};
   const int nb = S->nblk;
   const dim3 G(nb, R), T(kThreads);
   double2* x = S->v64[0];
   double2* b = S->v64[1];
   double2* Ax = S->v64[13];
-  if (!hip_ok(hipMemcpyAsync(b, Einc, sizeof(double2) * n * R, hipMemcpyDeviceToDevice, s),
+  if (!hip_ok(hipMemcpyAsync(b, Einc, sizeof(double2) * S->total, hipMemcpyDeviceToDevice, s),
               "copy E_inc"))
     return IFDDA_ERR_HIP;
   if (!o->x0_zero &&
-      !hip_ok(hipMemcpyAsync(x, Einc, sizeof(double2) * n * R, hipMemcpyDeviceToDevice, s),
+      !hip_ok(hipMemcpyAsync(x, Einc, sizeof(double2) * S->total, hipMemcpyDeviceToDevice, s),
               "copy x0"))
     return IFDDA_ERR_HIP;
   if (nstop && *nstop == -1) {
@@ -1407,7 +1528,7 @@ static int solve_dev(ifdda_ctx* c, const ifdda_solve_opts* o, int R, const doubl
   bool cancelled = false;
   if (o->precision == 64) {
     std::vector<double> tols(R, o->tol);
-    if (!krylov<double2>(c, S, S->v64, R, o->method, tols.data(), nullptr, maxit, o->x0_zero,
+    if (!krylov<double2>(op, S, S->v64, R, o->method, tols.data(), nullptr, maxit, o->x0_zero,
                          o->check_every, nstop, s, run.data(), &cancelled))
       return IFDDA_ERR_HIP;
     for (int r = 0; r < R; ++r) {
@@ -1422,7 +1543,7 @@ static int solve_dev(ifdda_ctx* c, const ifdda_solve_opts* o, int R, const doubl
     std::vector<char> active(R, 1);
     std::vector<int> act(R, 0);
     hipLaunchKernelGGL(k_init<double2>, G, T, 0, s, (const double2*)b, x,
-                       o->x0_zero ? (const double2*)nullptr : (const double2*)x, n, S->part, nb);
+                       o->x0_zero ? (const double2*)nullptr : (const double2*)x, vn, S->part, nb);
     if (!resid_norms(S, R, bnorm.data(), s)) return IFDDA_ERR_HIP;
     bool have_ax = !o->x0_zero;
     if (!hip_ok(hipMemcpyAsync(S->act, act.data(), sizeof(int) * R, hipMemcpyHostToDevice, s),
@@ -1430,11 +1551,11 @@ static int solve_dev(ifdda_ctx* c, const ifdda_solve_opts* o, int R, const doubl
       return IFDDA_ERR_HIP;
     for (int sweep = 0;; ++sweep) {
       if (have_ax) {
-        if (!mv<double2>(c, x, x, Ax, s, S->act, R, n, 1)) return IFDDA_ERR_HIP;
+        if (!mv<double2>(op, x, x, Ax, s, S->act, R, n, 1)) return IFDDA_ERR_HIP;
         for (int r = 0; r < R; ++r) ncompte[r] += active[r] ? 1 : 0;
       }
       hipLaunchKernelGGL(k_refine_resid, G, T, 0, s, (const double2*)b, (const double2*)Ax,
-                         (int)have_ax, r64, v32[1], n, S->part, nb);
+                         (int)have_ax, r64, v32[1], vn, S->part, nb);
       if (!resid_norms(S, R, rn.data(), s)) return IFDDA_ERR_HIP;
       have_ax = true;
       int nact = 0;
@@ -1464,7 +1585,7 @@ static int solve_dev(ifdda_ctx* c, const ifdda_solve_opts* o, int R, const doubl
         break;
       }
       std::vector<KRun> in(R);
-      if (!krylov<float2>(c, S, v32, R, o->method, tin.data(), active.data(), maxit, 1,
+      if (!krylov<float2>(op, S, v32, R, o->method, tin.data(), active.data(), maxit, 1,
                           o->check_every, nstop, s, in.data(), &cancelled))
         return IFDDA_ERR_HIP;
       bool broke = false;
@@ -1484,7 +1605,7 @@ static int solve_dev(ifdda_ctx* c, const ifdda_solve_opts* o, int R, const doubl
           !hip_ok(hipMemcpyAsync(S->act, act.data(), sizeof(int) * R, hipMemcpyHostToDevice, s),
                   "copy flags"))
         return IFDDA_ERR_HIP;
-      hipLaunchKernelGGL(k_refine_add, G, T, 0, s, x, (const float2*)v32[0], n, (const int*)S->act);
+      hipLaunchKernelGGL(k_refine_add, G, T, 0, s, x, (const float2*)v32[0], vn, (const int*)S->act);
     }
   }
   if (cancelled) {
@@ -1496,22 +1617,19 @@ static int solve_dev(ifdda_ctx* c, const ifdda_solve_opts* o, int R, const doubl
     return IFDDA_OK;
   }
   // Final true residual with the FP64 matvec (inverserig.f:95-105).
-  if (!mv<double2>(c, x, x, Ax, s, nullptr, R, n, 0)) return IFDDA_ERR_HIP;
-  hipLaunchKernelGGL(k_resid<double2>, G, T, 0, s, (const double2*)Ax, (const double2*)b, n,
+  if (!mv<double2>(op, x, x, Ax, s, nullptr, R, n, 0)) return IFDDA_ERR_HIP;
+  hipLaunchKernelGGL(k_resid<double2>, G, T, 0, s, (const double2*)Ax, (const double2*)b, vn,
                      S->part, nb);
   std::vector<double> e(R), bn(R);
   if (!resid_norms(S, R, e.data(), s)) return IFDDA_ERR_HIP;
-  hipLaunchKernelGGL(k_init<double2>, G, T, 0, s, (const double2*)b, Ax, (const double2*)Ax, n,
+  hipLaunchKernelGGL(k_init<double2>, G, T, 0, s, (const double2*)b, Ax, (const double2*)Ax, vn,
                      S->part, nb);
   if (!resid_norms(S, R, bn.data(), s)) return IFDDA_ERR_HIP;
   if (FFloc &&
-      !hip_ok(hipMemcpyAsync(FFloc, x, sizeof(double2) * n * R, hipMemcpyDeviceToDevice, s),
+      !hip_ok(hipMemcpyAsync(FFloc, x, sizeof(double2) * S->total, hipMemcpyDeviceToDevice, s),
               "copy E"))
     return IFDDA_ERR_HIP;
-  if (FF) {
-    hipLaunchKernelGGL(k_alpha_apply, G, T, 0, s, (const double2*)x, FF, (const double2*)c->al64,
-                       c->alpha_mode, c->nd);
-  }
+  if (FF && !op.alpha_apply(x, FF, R, n, s)) return IFDDA_ERR_HIP;
   if (!hip_ok(hipStreamSynchronize(s), "solve")) return IFDDA_ERR_HIP;
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   for (int r = 0; r < R; ++r) {

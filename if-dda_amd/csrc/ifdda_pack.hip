@@ -1,0 +1,502 @@
+// ifdda_pack.hip -- independent problems solved together.
+//
+// A pack holds P members; a member is one right-hand side (incidence) of one
+// problem (job), and every job brings its own box, padded FFT lengths, Green
+// tensor (wavelength, lattice spacing), dipole map and polarisability.  Each
+// matvec of the pack is five packed launches over the tiles of all members
+// (launch_matvec_pack_*), and the Krylov solve runs the members in lockstep
+// with per-member state, length and convergence test (solve_core on a ragged
+// vector layout), so a member's iterations and results are those of its own
+// solve: the member's tiles, reductions and their order are the same.
+// Members that have converged drop out of the launches.
+//
+// This stands for the reference's callers that solve many independent cases
+// one cdmlib call at a time (cdmlib.f:1-71; the GUI's cdmRun worker): here
+// their launches are shared.  A workspace keeps, between packs, the FFT
+// tables per length, the Green tensors per (box, d, k0, nquad) (least
+// recently used out past a byte limit), and grow-only device buffers, so that
+// a stream of packs allocates and frees nothing in steady state; every copy
+// is ordered on the workspace's own non-blocking stream.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/ifdda.h"
+#include "ifdda_internal.h"
+
+namespace ifdda {
+
+namespace {
+
+struct PackTable {
+  double2* tw64 = nullptr;
+  float2* tw32 = nullptr;
+  double2* tws64 = nullptr;
+  float2* tws32 = nullptr;
+  int* pos = nullptr;
+};
+
+struct GreenKey {
+  int nx, ny, nz, nquad;
+  double d, k0;
+  bool operator<(const GreenKey& o) const {
+    return std::tie(nx, ny, nz, nquad, d, k0) < std::tie(o.nx, o.ny, o.nz, o.nquad, o.d, o.k0);
+  }
+};
+
+struct GreenEntry {
+  GridPlan g{};
+  double2* G64 = nullptr;
+  float2* G32 = nullptr;
+  size_t bytes = 0;
+  unsigned long long used = 0;  // pack counter of the last use
+};
+
+size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+}  // namespace
+
+struct PackWs {
+  hipStream_t s = nullptr;
+  size_t lds_limit = 65536;
+  std::map<int, PackTable> tables;
+  std::map<GreenKey, GreenEntry> greens;
+  size_t green_bytes = 0;
+  size_t green_limit = (size_t)16 << 30;
+  unsigned long long tick = 0;
+  double2* full = nullptr;  // Green build scratch (L1 L2 L3 FP64)
+  size_t full_cap = 0;
+  char* arena = nullptr;  // per-pack data, grow-only
+  size_t arena_cap = 0;
+  char* blob = nullptr;  // packed-launch argument blocks, grow-only
+  size_t blob_cap = 0;
+  Solver* S = nullptr;
+  std::vector<long long> lens, offs;  // host layout of the last pack (read by s)
+};
+
+namespace {
+
+bool grow(char** p, size_t* cap, size_t need, const char* what) {
+  if (need <= *cap) return true;
+  (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  const size_t c = need + need / 4;
+  if (!hip_ok(hipMalloc(p, c), what)) return false;
+  *cap = c;
+  return true;
+}
+
+const PackTable* table_for(PackWs& w, const AxisPlan& ax) {
+  auto it = w.tables.find(ax.L);
+  if (it != w.tables.end()) return &it->second;
+  const int L = ax.L;
+  std::vector<double2> t64(L), s64(L);
+  std::vector<float2> t32(L), s32(L);
+  std::vector<int> pos(L);
+  axis_tables(ax, t64.data(), t32.data(), s64.data(), s32.data(), pos.data());
+  PackTable t;
+  bool ok = hip_ok(hipMalloc(&t.tw64, sizeof(double2) * L), "hipMalloc tables") &&
+            hip_ok(hipMalloc(&t.tw32, sizeof(float2) * L), "hipMalloc tables") &&
+            hip_ok(hipMalloc(&t.tws64, sizeof(double2) * L), "hipMalloc tables") &&
+            hip_ok(hipMalloc(&t.tws32, sizeof(float2) * L), "hipMalloc tables") &&
+            hip_ok(hipMalloc(&t.pos, sizeof(int) * L), "hipMalloc tables") &&
+            hip_ok(hipMemcpyAsync(t.tw64, t64.data(), sizeof(double2) * L, hipMemcpyHostToDevice, w.s), "copy tables") &&
+            hip_ok(hipMemcpyAsync(t.tw32, t32.data(), sizeof(float2) * L, hipMemcpyHostToDevice, w.s), "copy tables") &&
+            hip_ok(hipMemcpyAsync(t.tws64, s64.data(), sizeof(double2) * L, hipMemcpyHostToDevice, w.s), "copy tables") &&
+            hip_ok(hipMemcpyAsync(t.tws32, s32.data(), sizeof(float2) * L, hipMemcpyHostToDevice, w.s), "copy tables") &&
+            hip_ok(hipMemcpyAsync(t.pos, pos.data(), sizeof(int) * L, hipMemcpyHostToDevice, w.s), "copy tables") &&
+            hip_ok(hipStreamSynchronize(w.s), "copy tables");  // the host vectors end here
+  if (!ok) {
+    (void)hipFree(t.tw64), (void)hipFree(t.tw32), (void)hipFree(t.tws64), (void)hipFree(t.tws32);
+    (void)hipFree(t.pos);
+    return nullptr;
+  }
+  return &w.tables.emplace(L, t).first->second;
+}
+
+void green_free(GreenEntry& e) {
+  (void)hipFree(e.G64);
+  (void)hipFree(e.G32);
+  e.G64 = nullptr;
+  e.G32 = nullptr;
+}
+
+// The scalar polarisability of a job (nd complex) in FP64 and FP32.
+struct JobDev {
+  const GreenEntry* green = nullptr;
+  const PackTable* tab3[3] = {};
+  int32_t* tab = nullptr;  // null: dense box
+  double2* al64 = nullptr;
+  float2* al32 = nullptr;
+  long long nd = 0;
+};
+
+struct PackOp : MvOp {
+  PackLayout lay64, lay32;
+  const void* blob64 = nullptr;
+  const void* blob32 = nullptr;
+  std::vector<const double2*> al;  // per member
+  std::vector<long long> nd, off;
+  bool mv64(const double2* x, const double2* b, double2* y, hipStream_t s, const int* skip, int,
+            long long, int skip_stride) const override {
+    return launch_matvec_pack_f64(lay64, blob64, x, b, y, s, skip, skip_stride);
+  }
+  bool mv32(const float2* x, const float2* b, float2* y, hipStream_t s, const int* skip, int,
+            long long, int skip_stride) const override {
+    return launch_matvec_pack_f32(lay32, blob32, x, b, y, s, skip, skip_stride);
+  }
+  bool alpha_apply(const double2* x, double2* FF, int R, long long, hipStream_t s) const override {
+    for (int m = 0; m < R; ++m)
+      if (!alpha_apply_one(x + off[m], FF + off[m], al[m], kAlphaScalar, nd[m], s)) return false;
+    return true;
+  }
+};
+
+}  // namespace
+
+PackWs* pack_ws_new() {
+  PackWs* w = new PackWs();
+  int dev = 0, lds = 65536;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    lds = 65536;
+  w->lds_limit = (size_t)std::min(lds, 65536);
+  if (!hip_ok(hipStreamCreateWithFlags(&w->s, hipStreamNonBlocking), "hipStreamCreate")) {
+    delete w;
+    return nullptr;
+  }
+  return w;
+}
+
+void pack_ws_green_limit(PackWs* w, size_t bytes) { w->green_limit = bytes; }
+
+void pack_ws_delete(PackWs* w) {
+  if (!w) return;
+  if (w->s) (void)hipStreamSynchronize(w->s);
+  for (auto& t : w->tables) {
+    (void)hipFree(t.second.tw64), (void)hipFree(t.second.tw32);
+    (void)hipFree(t.second.tws64), (void)hipFree(t.second.tws32), (void)hipFree(t.second.pos);
+  }
+  for (auto& g : w->greens) green_free(g.second);
+  (void)hipFree(w->full);
+  (void)hipFree(w->arena);
+  (void)hipFree(w->blob);
+  solver_delete(w->S);
+  if (w->s) (void)hipStreamDestroy(w->s);
+  delete w;
+}
+
+size_t pack_member_bytes(const ifdda_job& j) {
+  const int L1 = smooth235_at_least(2 * j.nx), L2 = smooth235_at_least(2 * j.ny);
+  const size_t grid = ((size_t)j.nz * j.ny * L1 + (size_t)j.nz * L2 * L1) * 3 * sizeof(double2);
+  const size_t vec = (size_t)3 * j.nd * (14 * 24 + 16 + 3 * 16);  // Krylov, refinement, E/FF/FFloc
+  return grid + vec;
+}
+
+size_t pack_green_bytes(const ifdda_job& j) {
+  const size_t F = (size_t)(smooth235_at_least(2 * j.nx) / 2 + 1) *
+                   (smooth235_at_least(2 * j.ny) / 2 + 1) * (smooth235_at_least(2 * j.nz) / 2 + 1);
+  return F * 6 * (sizeof(double2) + sizeof(float2));
+}
+
+int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts* o,
+               volatile int* nstop) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const int J = (int)in.size();
+  if (J == 0) return IFDDA_OK;
+  if (o->method < IFDDA_GPBICG1 || o->method > IFDDA_TFQMR || (o->precision != 64 && o->precision != 32)) {
+    set_error("pack solve: unknown method or precision");
+    return IFDDA_ERR_ARG;
+  }
+  // ---- validate the jobs, plan their boxes
+  std::vector<GridPlan> plans(J);
+  std::vector<GreenKey> keys(J);
+  std::vector<char> dense(J, 0);
+  for (int j = 0; j < J; ++j) {
+    const ifdda_job& s = *in[j].job;
+    if (s.nx < 1 || s.ny < 1 || s.nz < 1 || s.nd < 1 || s.nrhs < 1 || !s.alpha || !s.E_inc ||
+        !(s.d > 0.0) || !(s.k0 > 0.0) || s.nquad < 0 || s.nquad > 10) {
+      set_error("pack solve: bad job " + std::to_string(j));
+      return IFDDA_ERR_ARG;
+    }
+    const size_t N = (size_t)s.nx * s.ny * s.nz;
+    if (!s.tabdip && (size_t)s.nd != N) {
+      set_error("pack solve: a dense box needs nd = nx ny nz (job " + std::to_string(j) + ")");
+      return IFDDA_ERR_ARG;
+    }
+    if (s.tabdip) {
+      std::vector<char> seen((size_t)s.nd + 1, 0);
+      bool identity = (size_t)s.nd == N;
+      size_t count = 0;
+      for (size_t i = 0; i < N; ++i) {
+        const int32_t t = s.tabdip[i];
+        if (t < 0 || t > s.nd || (t > 0 && seen[t])) {
+          set_error("pack solve: Tabdip of job " + std::to_string(j) +
+                    " out of range or mapping a dipole twice at cell " + std::to_string(i));
+          return IFDDA_ERR_ARG;
+        }
+        if (t > 0) seen[t] = 1, ++count;
+        identity = identity && t == (int32_t)(i + 1);
+      }
+      if (count != (size_t)s.nd) {
+        set_error("pack solve: Tabdip of job " + std::to_string(j) + " maps " +
+                  std::to_string(count) + " dipoles, nd = " + std::to_string(s.nd));
+        return IFDDA_ERR_ARG;
+      }
+      dense[j] = identity;
+    } else {
+      dense[j] = 1;
+    }
+    std::string why;
+    if (!make_grid_plan(s.nx, s.ny, s.nz, w.lds_limit, &plans[j], &why)) {
+      set_error("pack solve: job " + std::to_string(j) + ": " + why);
+      return IFDDA_ERR_ARG;
+    }
+    keys[j] = GreenKey{s.nx, s.ny, s.nz, s.nquad, s.d, s.k0};
+  }
+  ++w.tick;
+  // ---- Green tensors: cached per key; evict unused entries past the limit
+  std::vector<JobDev> jd(J);
+  {
+    size_t need = 0;
+    std::map<GreenKey, size_t> fresh;
+    for (int j = 0; j < J; ++j)
+      if (!w.greens.count(keys[j]) && !fresh.count(keys[j])) {
+        const GridPlan& g = plans[j];
+        fresh[keys[j]] = (size_t)g.F[0] * g.F[1] * g.F[2] * 6 * (sizeof(double2) + sizeof(float2));
+        need += fresh[keys[j]];
+      }
+    for (int j = 0; j < J; ++j)
+      if (w.greens.count(keys[j])) w.greens[keys[j]].used = w.tick;
+    while (w.green_bytes + need > w.green_limit) {
+      auto victim = w.greens.end();
+      for (auto it = w.greens.begin(); it != w.greens.end(); ++it)
+        if (it->second.used != w.tick && (victim == w.greens.end() || it->second.used < victim->second.used))
+          victim = it;
+      if (victim == w.greens.end()) break;
+      if (!hip_ok(hipStreamSynchronize(w.s), "green evict")) return IFDDA_ERR_HIP;
+      w.green_bytes -= victim->second.bytes;
+      green_free(victim->second);
+      w.greens.erase(victim);
+    }
+  }
+  for (int j = 0; j < J; ++j) {
+    const GridPlan& g = plans[j];
+    for (int a = 0; a < 3; ++a)
+      if (!(jd[j].tab3[a] = table_for(w, g.ax[a]))) return IFDDA_ERR_HIP;
+    auto it = w.greens.find(keys[j]);
+    if (it == w.greens.end()) {
+      GreenEntry e;
+      e.g = g;
+      const size_t nf = (size_t)g.F[0] * g.F[1] * g.F[2] * 6;
+      e.bytes = nf * (sizeof(double2) + sizeof(float2));
+      const size_t M = (size_t)g.L[0] * g.L[1] * g.L[2];
+      if (!hip_ok(hipMalloc(&e.G64, sizeof(double2) * nf), "hipMalloc green") ||
+          !hip_ok(hipMalloc(&e.G32, sizeof(float2) * nf), "hipMalloc green")) {
+        green_free(e);
+        return IFDDA_ERR_HIP;
+      }
+      if (M > w.full_cap) {
+        if (!hip_ok(hipStreamSynchronize(w.s), "green scratch")) return IFDDA_ERR_HIP;
+        char* p = reinterpret_cast<char*>(w.full);
+        size_t cap = w.full_cap * sizeof(double2);
+        if (!grow(&p, &cap, M * sizeof(double2), "hipMalloc green scratch")) {
+          green_free(e);
+          return IFDDA_ERR_HIP;
+        }
+        w.full = reinterpret_cast<double2*>(p);
+        w.full_cap = cap / sizeof(double2);
+      }
+      const double2* tw[3] = {jd[j].tab3[0]->tw64, jd[j].tab3[1]->tw64, jd[j].tab3[2]->tw64};
+      const int* pos[3] = {jd[j].tab3[0]->pos, jd[j].tab3[1]->pos, jd[j].tab3[2]->pos};
+      if (!green_build_device(g, tw, pos, keys[j].d, keys[j].k0, keys[j].nquad, w.full, e.G64,
+                              e.G32, w.s)) {
+        green_free(e);
+        return IFDDA_ERR_HIP;
+      }
+      e.used = w.tick;
+      w.green_bytes += e.bytes;
+      it = w.greens.emplace(keys[j], e).first;
+    }
+    jd[j].green = &it->second;
+    jd[j].nd = in[j].job->nd;
+  }
+  // ---- members and the ragged vector layout (offsets 64-element aligned)
+  std::vector<int> mjob, mrhs;
+  for (int j = 0; j < J; ++j)
+    for (int r = 0; r < in[j].job->nrhs; ++r) mjob.push_back(j), mrhs.push_back(r);
+  const int P = (int)mjob.size();
+  w.lens.assign(P, 0);
+  w.offs.assign(P, 0);
+  long long total = 0;
+  for (int m = 0; m < P; ++m) {
+    w.lens[m] = 3LL * jd[mjob[m]].nd;
+    w.offs[m] = total;
+    total += (w.lens[m] + 63) & ~63LL;
+  }
+  // ---- arena: per job map and polarisability, per member FFT scratch, E/FF/FFloc
+  std::vector<size_t> o_tab(J), o_al64(J), o_al32(J), o_A(P), o_B(P);
+  size_t used = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = used;
+    used = al256(used + bytes);
+    return at;
+  };
+  for (int j = 0; j < J; ++j) {
+    const ifdda_job& s = *in[j].job;
+    o_tab[j] = dense[j] ? 0 : take(sizeof(int32_t) * (size_t)s.nx * s.ny * s.nz);
+    o_al64[j] = take(sizeof(double2) * (size_t)s.nd);
+    o_al32[j] = take(sizeof(float2) * (size_t)s.nd);
+  }
+  for (int m = 0; m < P; ++m) {
+    const GridPlan& g = plans[mjob[m]];
+    o_A[m] = take(sizeof(double2) * (size_t)g.nz * g.ny * g.L[0] * 3);
+    o_B[m] = take(sizeof(double2) * (size_t)g.nz * g.L[1] * g.L[0] * 3);
+  }
+  const size_t o_E = take(sizeof(double2) * (size_t)total);
+  const size_t o_FF = take(sizeof(double2) * (size_t)total);
+  const size_t o_FL = take(sizeof(double2) * (size_t)total);
+  if (used > w.arena_cap && !hip_ok(hipStreamSynchronize(w.s), "arena")) return IFDDA_ERR_HIP;
+  if (!grow(&w.arena, &w.arena_cap, used, "hipMalloc pack arena")) return IFDDA_ERR_HIP;
+  char* A = w.arena;
+  // host staging of the polarisabilities (FP64 and FP32), alive until the sync below
+  std::vector<std::vector<double2>> h64(J);
+  std::vector<std::vector<float2>> h32(J);
+  for (int j = 0; j < J; ++j) {
+    const ifdda_job& s = *in[j].job;
+    if (!dense[j]) {
+      jd[j].tab = reinterpret_cast<int32_t*>(A + o_tab[j]);
+      if (!hip_ok(hipMemcpyAsync(jd[j].tab, s.tabdip, sizeof(int32_t) * (size_t)s.nx * s.ny * s.nz,
+                                 hipMemcpyHostToDevice, w.s), "copy tabdip"))
+        return IFDDA_ERR_HIP;
+    }
+    h64[j].resize(s.nd);
+    h32[j].resize(s.nd);
+    for (int i = 0; i < s.nd; ++i) {
+      h64[j][i] = make_double2(s.alpha[2 * i], s.alpha[2 * i + 1]);
+      h32[j][i] = make_float2((float)s.alpha[2 * i], (float)s.alpha[2 * i + 1]);
+    }
+    jd[j].al64 = reinterpret_cast<double2*>(A + o_al64[j]);
+    jd[j].al32 = reinterpret_cast<float2*>(A + o_al32[j]);
+    if (!hip_ok(hipMemcpyAsync(jd[j].al64, h64[j].data(), sizeof(double2) * s.nd, hipMemcpyHostToDevice, w.s), "copy alpha") ||
+        !hip_ok(hipMemcpyAsync(jd[j].al32, h32[j].data(), sizeof(float2) * s.nd, hipMemcpyHostToDevice, w.s), "copy alpha"))
+      return IFDDA_ERR_HIP;
+  }
+  double2* E = reinterpret_cast<double2*>(A + o_E);
+  double2* FF = reinterpret_cast<double2*>(A + o_FF);
+  double2* FL = reinterpret_cast<double2*>(A + o_FL);
+  for (int m = 0; m < P; ++m) {
+    const ifdda_job& s = *in[mjob[m]].job;
+    const double* src = s.E_inc + 2 * (size_t)w.lens[m] * mrhs[m];
+    if (!hip_ok(hipMemcpyAsync(E + w.offs[m], src, sizeof(double2) * w.lens[m], hipMemcpyHostToDevice, w.s),
+                "copy E_inc"))
+      return IFDDA_ERR_HIP;
+  }
+  // ---- packed-launch argument blocks
+  PackOp op;
+  {
+    std::vector<const GridPlan*> gp(P);
+    std::vector<MatvecDev<double2>> m64(P);
+    std::vector<MatvecDev<float2>> m32(P);
+    op.al.resize(P);
+    op.nd.resize(P);
+    op.off = w.offs;
+    for (int m = 0; m < P; ++m) {
+      const JobDev& d = jd[mjob[m]];
+      gp[m] = &plans[mjob[m]];
+      MatvecDev<double2>& a = m64[m];
+      MatvecDev<float2>& b = m32[m];
+      a.A = reinterpret_cast<double2*>(A + o_A[m]);
+      a.B = reinterpret_cast<double2*>(A + o_B[m]);
+      b.A = reinterpret_cast<float2*>(A + o_A[m]);  // the FP64-sized scratch holds either
+      b.B = reinterpret_cast<float2*>(A + o_B[m]);
+      a.G = d.green->G64;
+      b.G = d.green->G32;
+      for (int ax = 0; ax < 3; ++ax) {
+        a.tw[ax] = d.tab3[ax]->tw64, a.tws[ax] = d.tab3[ax]->tws64, a.pos[ax] = d.tab3[ax]->pos;
+        b.tw[ax] = d.tab3[ax]->tw32, b.tws[ax] = d.tab3[ax]->tws32, b.pos[ax] = d.tab3[ax]->pos;
+      }
+      a.tab = b.tab = d.tab;
+      a.alpha = d.al64;
+      b.alpha = d.al32;
+      a.alpha_mode = b.alpha_mode = kAlphaScalar;
+      op.al[m] = d.al64;
+      op.nd[m] = d.nd;
+    }
+    std::vector<unsigned char> b64, b32;
+    const size_t n64 = pack_blob_f64(gp, m64, w.offs.data(), op.lay64, b64);
+    const size_t n32 = pack_blob_f32(gp, m32, w.offs.data(), op.lay32, b32);
+    const size_t need = al256(n64) + al256(n32);
+    if (need > w.blob_cap && !hip_ok(hipStreamSynchronize(w.s), "blob")) return IFDDA_ERR_HIP;
+    if (!grow(&w.blob, &w.blob_cap, need, "hipMalloc pack blob") ||
+        !hip_ok(hipMemcpyAsync(w.blob, b64.data(), n64, hipMemcpyHostToDevice, w.s), "copy blob") ||
+        !hip_ok(hipMemcpyAsync(w.blob + al256(n64), b32.data(), n32, hipMemcpyHostToDevice, w.s), "copy blob") ||
+        !hip_ok(hipStreamSynchronize(w.s), "pack setup"))  // host staging ends here
+      return IFDDA_ERR_HIP;
+    op.blob64 = w.blob;
+    op.blob32 = w.blob + al256(n64);
+  }
+  // ---- Krylov workspace (reused when large enough) and the solve
+  const bool need32 = o->precision == 32;
+  if (!solver_refit(w.S, P, need32, w.lens.data(), w.offs.data(), w.s)) {
+    if (!hip_ok(hipStreamSynchronize(w.s), "solver")) return IFDDA_ERR_HIP;
+    solver_delete(w.S);
+    w.S = solver_new(P, need32, w.lens.data(), w.offs.data());
+    if (!w.S) return IFDDA_ERR_HIP;
+  }
+  std::vector<ifdda_solve_info> info(P);
+  const int rc = solve_core(op, w.S, o, P, E, FF, FL, info.data(), nstop, w.s, t0);
+  if (rc != IFDDA_OK) return rc;
+  // ---- outputs
+  for (int m = 0; m < P; ++m) {
+    const PackIn& p = in[mjob[m]];
+    const size_t at = 2 * (size_t)w.lens[m] * mrhs[m];
+    if (p.FFloc && !hip_ok(hipMemcpyAsync(p.FFloc + at, FL + w.offs[m], sizeof(double2) * w.lens[m],
+                                          hipMemcpyDeviceToHost, w.s), "copy FFloc"))
+      return IFDDA_ERR_HIP;
+    if (p.FF && !hip_ok(hipMemcpyAsync(p.FF + at, FF + w.offs[m], sizeof(double2) * w.lens[m],
+                                       hipMemcpyDeviceToHost, w.s), "copy FF"))
+      return IFDDA_ERR_HIP;
+    if (p.infos) p.infos[mrhs[m]] = info[m];
+  }
+  return hip_ok(hipStreamSynchronize(w.s), "pack outputs") ? IFDDA_OK : IFDDA_ERR_HIP;
+}
+
+}  // namespace ifdda
+
+using namespace ifdda;
+
+extern "C" int ifdda_solve_batch(int njobs, const ifdda_job* jobs, double* const* FFloc,
+                                 double* const* FF, ifdda_solve_info* const* infos,
+                                 volatile int* nstop) {
+  if (njobs < 0 || (njobs > 0 && (!jobs || !infos))) {
+    set_error("ifdda_solve_batch: NULL argument");
+    return IFDDA_ERR_ARG;
+  }
+  std::vector<PackIn> in(njobs);
+  for (int j = 0; j < njobs; ++j) {
+    const ifdda_solve_opts& a = jobs[j].opts;
+    const ifdda_solve_opts& b = jobs[0].opts;
+    if (a.method != b.method || a.precision != b.precision || a.x0_zero != b.x0_zero ||
+        a.maxit != b.maxit || a.tol != b.tol || a.check_every != b.check_every) {
+      set_error("ifdda_solve_batch: all jobs need the same solve options");
+      return IFDDA_ERR_ARG;
+    }
+    if (!infos[j]) {
+      set_error("ifdda_solve_batch: NULL infos");
+      return IFDDA_ERR_ARG;
+    }
+    in[j] = PackIn{&jobs[j], FFloc ? FFloc[j] : nullptr, FF ? FF[j] : nullptr, infos[j]};
+  }
+  if (njobs == 0) return IFDDA_OK;
+  PackWs* w = pack_ws_new();
+  if (!w) return IFDDA_ERR_HIP;
+  const int rc = pack_solve(*w, in, &jobs[0].opts, nstop);
+  pack_ws_delete(w);
+  return rc;
+}

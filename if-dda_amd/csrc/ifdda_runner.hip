@@ -1,16 +1,19 @@
 // ifdda_runner.hip -- many independent solves on one GPU at once.
 //
-// A pool of worker threads, each with its own contexts and HIP streams,
-// takes jobs (object, permittivity through alpha, wavelength, lattice,
-// incidences) from a queue in submission order.  A job is admitted when its
-// estimated device footprint fits the memory budget next to the jobs already
-// running (a job larger than the budget runs alone), so requests that arrive
-// while others are on the GPU start as soon as a worker and memory are free
-// and their work overlaps on the device.  Each job is one multi-RHS solve
-// (all its incidences share every matvec launch); a worker keeps the
-// contexts of the last grids it served and rebuilds the Green tensor only
-// when (d, k0, nquad) change.  Every kernel is deterministic, so a job's
-// result does not depend on what else runs.
+// A pool of worker threads takes jobs (object, permittivity through alpha,
+// wavelength, lattice, incidences) from a queue in submission order.  A free
+// worker takes the job at the head and, with it, every queued job with the
+// same solve options that fits the pack (at most max_members right-hand
+// sides) and the memory budget next to the packs already running (a job
+// larger than the budget runs alone); the group is solved as one pack
+// (ifdda_pack.hip): all its right-hand sides share every matvec launch and
+// one lockstep Krylov solve, while each keeps its own iterations and
+// convergence.  Requests that arrive while packs run queue up and start
+// together when a worker frees; packs of different workers overlap on the
+// device.  Each worker keeps its FFT tables, recent Green tensors and buffers
+// between packs.  Every kernel is deterministic and a member's arithmetic
+// does not depend on the rest of its pack, so a job's result does not depend
+// on what else runs.
 //
 // This stands for the reference's callers that loop over objects or
 // wavelengths and call cdmlib once per case (cdmlib.f:1-71, the GUI's
@@ -20,7 +23,6 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
-#include <list>
 #include <map>
 #include <mutex>
 #include <string>
@@ -49,23 +51,23 @@ struct Job {
   double t_submit = 0.0, t_start = 0.0, t_end = 0.0;
 };
 
-// Device bytes a job needs: Green tensor (FP64 + FP32 octant), FFT scratch
-// for R right-hand sides (FP64), Krylov vectors (14 per precision and RHS),
-// host-array staging.
+// Device bytes a job needs in a pack: FFT scratch and vectors per
+// right-hand side, and its Green tensor.
 double job_bytes(const ifdda_job& j) {
-  const int L1 = ifdda::smooth235_at_least(2 * j.nx), L2 = ifdda::smooth235_at_least(2 * j.ny),
-            L3 = ifdda::smooth235_at_least(2 * j.nz);
-  const double F = (double)(L1 / 2 + 1) * (L2 / 2 + 1) * (L3 / 2 + 1);
-  const double R = j.nrhs, n3 = 3.0 * j.nd;
-  const double grid = (double)j.nz * j.ny * L1 * 3 + (double)j.nz * L2 * L1 * 3;
-  const double kry = 14.0 * n3 * R * (16.0 + (j.opts.precision == 32 ? 8.0 : 0.0));
-  return F * 6 * 24 + grid * 16 * R + kry + 2.0 * n3 * R * 16 + 4.0 * j.nx * j.ny * j.nz + (1 << 20);
+  return (double)ifdda::pack_member_bytes(j) * j.nrhs + (double)ifdda::pack_green_bytes(j) +
+         (double)(1 << 20);
+}
+
+bool same_opts(const ifdda_solve_opts& a, const ifdda_solve_opts& b) {
+  return a.method == b.method && a.precision == b.precision && a.x0_zero == b.x0_zero &&
+         a.maxit == b.maxit && a.tol == b.tol && a.check_every == b.check_every;
 }
 
 }  // namespace
 
 struct ifdda_runner {
   int nworkers = 1;
+  int max_members = 64;
   double budget = 0.0;
   int device = 0;
   std::mutex mu;
@@ -81,61 +83,15 @@ struct ifdda_runner {
 
 namespace {
 
-struct CtxEntry {
-  std::array<int, 3> box;
-  ifdda_ctx* ctx;
-  double d, k0;
-  int nquad;
-  double tolinit;
-};
-
-// Run one job on this worker's contexts (at most kKeep kept, least recent
-// first out).
-constexpr size_t kKeep = 2;
-
-int run_job(std::list<CtxEntry>& cache, Job& jb) {
-  const ifdda_job& s = jb.spec;
-  const std::array<int, 3> box{s.nx, s.ny, s.nz};
-  auto it = std::find_if(cache.begin(), cache.end(), [&](const CtxEntry& e) { return e.box == box; });
-  if (it == cache.end()) {
-    ifdda_ctx* c = nullptr;
-    int rc = ifdda_create(s.nx, s.ny, s.nz, &c);
-    if (rc != IFDDA_OK) return rc;
-    cache.push_front({box, c, -1.0, -1.0, -1, -1.0});
-    while (cache.size() > kKeep) {
-      ifdda_destroy(cache.back().ctx);
-      cache.pop_back();
-    }
-    it = cache.begin();
-  } else {
-    cache.splice(cache.begin(), cache, it);
-    it = cache.begin();
-  }
-  CtxEntry& e = *it;
-  ifdda_ctx* c = e.ctx;
-  int rc = IFDDA_OK;
-  if (e.d != s.d || e.k0 != s.k0 || e.nquad != s.nquad || e.tolinit != s.tolinit) {
-    rc = ifdda_green_build(c, s.d, s.k0, s.nquad, s.tolinit);
-    if (rc != IFDDA_OK) return rc;
-    e.d = s.d, e.k0 = s.k0, e.nquad = s.nquad, e.tolinit = s.tolinit;
-  }
-  rc = ifdda_set_dipoles(c, s.tabdip ? jb.tab.data() : nullptr, s.nd);
-  if (rc == IFDDA_OK) rc = ifdda_set_polarizability(c, jb.alpha.data(), s.nd, 1);
-  if (rc != IFDDA_OK) return rc;
-  jb.FFloc.assign(2 * 3 * (size_t)s.nd * s.nrhs, 0.0);
-  jb.FF.assign(jb.FFloc.size(), 0.0);
-  jb.infos.assign(s.nrhs, ifdda_solve_info{});
-  volatile int stop = 0;
-  return ifdda_solve_multi(c, &s.opts, s.nrhs, jb.einc.data(), jb.FF.data(), jb.FFloc.data(),
-                           jb.infos.data(), &stop);
-}
-
 void worker(ifdda_runner* r) {
   (void)hipSetDevice(r->device);
-  std::list<CtxEntry> cache;
+  ifdda::PackWs* ws = ifdda::pack_ws_new();
+  // Green tensors kept between packs: a share of the budget per worker
+  if (ws)
+    ifdda::pack_ws_green_limit(
+        ws, (size_t)std::min(16e9, std::max(1e9, r->budget / (4.0 * r->nworkers))));
   for (;;) {
-    long long t = 0;
-    Job* jb = nullptr;
+    std::vector<Job*> group;
     {
       std::unique_lock<std::mutex> lk(r->mu);
       r->cv_work.wait(lk, [&] {
@@ -146,29 +102,58 @@ void worker(ifdda_runner* r) {
       });
       if (r->stop && r->queue.empty()) break;
       if (r->queue.empty()) continue;
-      t = r->queue.front();
+      Job* head = &r->jobs.at(r->queue.front());
       r->queue.pop_front();
-      jb = &r->jobs.at(t);
-      jb->state = 1;
-      jb->t_start = now_s();
-      r->used += jb->est;
+      group.push_back(head);
+      r->used += head->est;
+      int members = head->spec.nrhs;
+      for (auto it = r->queue.begin(); it != r->queue.end() && members < r->max_members;) {
+        Job* j = &r->jobs.at(*it);
+        if (same_opts(j->spec.opts, head->spec.opts) && members + j->spec.nrhs <= r->max_members &&
+            r->used + j->est <= r->budget) {
+          group.push_back(j);
+          r->used += j->est;
+          members += j->spec.nrhs;
+          it = r->queue.erase(it);
+        } else {
+          ++it;
+        }
+      }
+      const double t = now_s();
+      for (Job* j : group) j->state = 1, j->t_start = t;
       r->running++;
     }
-    const int rc = run_job(cache, *jb);
-    std::string err = rc == IFDDA_OK ? "" : ifdda_last_error();
+    std::vector<ifdda::PackIn> in;
+    for (Job* j : group) {
+      const size_t n = 2 * 3 * (size_t)j->spec.nd * j->spec.nrhs;
+      j->FFloc.assign(n, 0.0);
+      j->FF.assign(n, 0.0);
+      j->infos.assign(j->spec.nrhs, ifdda_solve_info{});
+      in.push_back(ifdda::PackIn{&j->spec, j->FFloc.data(), j->FF.data(), j->infos.data()});
+    }
+    int rc = IFDDA_ERR_HIP;
+    std::string err = "no pack workspace";
+    if (ws) {
+      volatile int stop = 0;
+      rc = ifdda::pack_solve(*ws, in, &group[0]->spec.opts, &stop);
+      err = rc == IFDDA_OK ? "" : ifdda_last_error();
+    }
     {
       std::lock_guard<std::mutex> lk(r->mu);
-      jb->rc = rc;
-      jb->err = err;
-      jb->state = 2;
-      jb->t_end = now_s();
-      r->used -= jb->est;
+      const double t = now_s();
+      for (Job* j : group) {
+        j->rc = rc;
+        j->err = err;
+        j->state = 2;
+        j->t_end = t;
+        r->used -= j->est;
+      }
       r->running--;
     }
     r->cv_done.notify_all();
     r->cv_work.notify_all();
   }
-  for (auto& e : cache) ifdda_destroy(e.ctx);
+  ifdda::pack_ws_delete(ws);
 }
 
 }  // namespace
@@ -193,23 +178,41 @@ extern "C" int ifdda_runner_submit(ifdda_runner* r, const ifdda_job* j, long lon
     ifdda::set_error("ifdda_runner_submit: bad job");
     return IFDDA_ERR_ARG;
   }
+  const size_t N = (size_t)j->nx * j->ny * j->nz;
+  if (!j->tabdip && (size_t)j->nd != N) {
+    ifdda::set_error("ifdda_runner_submit: bad job (a dense box needs nd = nx ny nz)");
+    return IFDDA_ERR_ARG;
+  }
+  if (j->tabdip) {  // the map must name each dipole 1..nd once (a bad job fails alone)
+    std::vector<char> seen((size_t)j->nd + 1, 0);
+    size_t count = 0;
+    for (size_t i = 0; i < N; ++i) {
+      const int32_t t = j->tabdip[i];
+      if (t < 0 || t > j->nd || (t > 0 && seen[t])) {
+        ifdda::set_error("ifdda_runner_submit: bad job (Tabdip entry out of range or repeated)");
+        return IFDDA_ERR_ARG;
+      }
+      if (t > 0) seen[t] = 1, ++count;
+    }
+    if (count != (size_t)j->nd) {
+      ifdda::set_error("ifdda_runner_submit: bad job (Tabdip maps a different number of dipoles)");
+      return IFDDA_ERR_ARG;
+    }
+  }
   Job jb;
   jb.spec = *j;
-  const size_t N = (size_t)j->nx * j->ny * j->nz;
   if (j->tabdip) jb.tab.assign(j->tabdip, j->tabdip + N);
   jb.alpha.assign(j->alpha, j->alpha + 2 * (size_t)j->nd);
   jb.einc.assign(j->E_inc, j->E_inc + 2 * 3 * (size_t)j->nd * j->nrhs);
-  jb.spec.tabdip = nullptr;  // the copies are used
-  if (j->tabdip) jb.spec.tabdip = jb.tab.data();
-  jb.spec.alpha = nullptr;
-  jb.spec.E_inc = nullptr;
   jb.est = job_bytes(*j);
   jb.t_submit = now_s();
   {
     std::lock_guard<std::mutex> lk(r->mu);
     const long long t = r->next++;
-    auto ins = r->jobs.emplace(t, std::move(jb));
-    if (j->tabdip) ins.first->second.spec.tabdip = ins.first->second.tab.data();
+    Job& q = r->jobs.emplace(t, std::move(jb)).first->second;
+    q.spec.tabdip = j->tabdip ? q.tab.data() : nullptr;  // the copies are used
+    q.spec.alpha = q.alpha.data();
+    q.spec.E_inc = q.einc.data();
     r->queue.push_back(t);
     *ticket = t;
   }
@@ -247,6 +250,16 @@ extern "C" int ifdda_runner_wait(ifdda_runner* r, long long ticket, double* FFlo
   if (FFloc) std::memcpy(FFloc, jb.FFloc.data(), sizeof(double) * jb.FFloc.size());
   if (FF) std::memcpy(FF, jb.FF.data(), sizeof(double) * jb.FF.size());
   if (infos) std::memcpy(infos, jb.infos.data(), sizeof(ifdda_solve_info) * jb.infos.size());
+  return IFDDA_OK;
+}
+
+extern "C" int ifdda_runner_pack_limit(ifdda_runner* r, int max_members) {
+  if (!r || max_members < 1) {
+    ifdda::set_error("ifdda_runner_pack_limit: bad argument");
+    return IFDDA_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(r->mu);
+  r->max_members = max_members;
   return IFDDA_OK;
 }
 

@@ -106,6 +106,9 @@ def lib():
     L.ifdda_runner_submit.argtypes = [vp, C.POINTER(Job), C.POINTER(C.c_longlong)]
     L.ifdda_runner_wait.argtypes = [vp, C.c_longlong, pd, pd, C.POINTER(SolveInfo), pd]
     L.ifdda_runner_destroy.argtypes = [vp]
+    L.ifdda_runner_pack_limit.argtypes = [vp, ip]
+    L.ifdda_solve_batch.argtypes = [ip, C.POINTER(Job), C.POINTER(pd), C.POINTER(pd),
+                                    C.POINTER(C.POINTER(SolveInfo)), C.POINTER(C.c_int)]
     L.ifdda_microscope_incidences.argtypes = [vp, C.POINTER(MicroscopeOpts), C.POINTER(C.c_int), pd,
                                               ip, pd]
     L.ifdda_microsbf_device.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(MicroscopeOpts), pd,
@@ -407,10 +410,11 @@ class Runner:
     """Many independent solves at once (ifdda_runner_*): a pool of workers
     with their own contexts and streams, admission by device footprint."""
 
-    def __init__(self, nworkers: int = 8, mem_budget: float = 240e9):
+    def __init__(self, nworkers: int = 8, mem_budget: float = 240e9, max_members: int = 64):
         self.h = C.c_void_p()
         _check(lib().ifdda_runner_create(int(nworkers), float(mem_budget), C.byref(self.h)),
                "ifdda_runner_create")
+        _check(lib().ifdda_runner_pack_limit(self.h, int(max_members)), "ifdda_runner_pack_limit")
         self._shape = {}
 
     def close(self):
@@ -464,3 +468,51 @@ class Runner:
                "ifdda_runner_wait")
         return {"FFloc": FL, "FF": FF, "infos": [i.as_dict() for i in infos],
                 "queued_s": times[0], "run_s": times[1], "latency_s": times[2]}
+
+
+def _job(nx, ny, nz, d, k0, alpha, E_inc, tabdip, opts, nquad=0, tolinit=1e-4):
+    """An ifdda_job and the arrays it points at (keep them alive)."""
+    al = _c128(alpha).reshape(-1)
+    E = _c128(E_inc)
+    E = E.reshape(1, -1) if E.ndim == 1 else E.reshape(E.shape[0], -1)
+    j = Job()
+    j.nx, j.ny, j.nz, j.d, j.k0 = int(nx), int(ny), int(nz), float(d), float(k0)
+    j.nquad, j.tolinit, j.nd, j.nrhs = int(nquad), float(tolinit), al.size, E.shape[0]
+    keep = [al, E]
+    if tabdip is not None:
+        tab = np.ascontiguousarray(tabdip, np.int32).reshape(-1)
+        j.tabdip = tab.ctypes.data
+        keep.append(tab)
+    j.alpha = al.ctypes.data
+    j.E_inc = E.ctypes.data
+    j.opts = opts
+    return j, keep, E.shape[0], al.size
+
+
+def solve_batch(problems, method="GPBICG1", tol=1e-4, precision=32, x0_zero=True, maxit=10000,
+                check_every=0):
+    """Independent problems solved together (ifdda_solve_batch): every
+    right-hand side of every problem shares the matvec launches and one
+    lockstep Krylov solve.  problems: iterable of dicts with keys nx, ny, nz,
+    d, k0, alpha, E_inc ([R, 3 nd] or [3 nd]) and optionally tabdip, nquad.
+    Returns one dict per problem: FFloc, FF ([R, 3 nd]) and infos."""
+    opts = Solver._opts(method, tol, precision, x0_zero, maxit, check_every)
+    jobs, keep, outs = [], [], []
+    for p in problems:
+        j, k, R, nd = _job(p["nx"], p["ny"], p["nz"], p["d"], p["k0"], p["alpha"], p["E_inc"],
+                           p.get("tabdip"), opts, p.get("nquad", 0))
+        jobs.append(j)
+        keep.append(k)
+        outs.append({"FFloc": np.zeros((R, 3 * nd), np.complex128),
+                     "FF": np.zeros((R, 3 * nd), np.complex128), "_info": (SolveInfo * R)()})
+    n = len(jobs)
+    D = C.POINTER(C.c_double)
+    arr = (Job * max(n, 1))(*jobs)
+    FL = (D * max(n, 1))(*[o["FFloc"].ctypes.data_as(D) for o in outs])
+    FF = (D * max(n, 1))(*[o["FF"].ctypes.data_as(D) for o in outs])
+    IN = (C.POINTER(SolveInfo) * max(n, 1))(*[C.cast(o["_info"], C.POINTER(SolveInfo)) for o in outs])
+    stop = C.c_int(0)
+    _check(lib().ifdda_solve_batch(n, arr, FL, FF, IN, C.byref(stop)), "ifdda_solve_batch")
+    for o in outs:
+        o["infos"] = [i.as_dict() for i in o.pop("_info")]
+    return outs

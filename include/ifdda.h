@@ -338,11 +338,13 @@ int ifdda_microsbf_device(ifdda_ctx *ctx, const ifdda_solve_opts *opts,
 /* ---------------------------------------------------------------------------
  * Runner: many independent solves at once (no counterpart in the reference,
  * whose callers loop over objects, wavelengths and requests one cdmlib call
- * at a time).  A pool of worker threads with their own contexts and streams
- * takes jobs in submission order; a job starts when a worker is free and its
- * estimated device footprint fits mem_budget (bytes) next to the running
- * jobs (a job larger than the budget runs alone).  Each job is one
- * multi-RHS solve; results do not depend on what else runs.
+ * at a time).  A pool of worker threads, each with its own stream, takes
+ * jobs in submission order: a free worker takes the head job and every
+ * queued job with the same solve options that fits the pack (at most
+ * max_members right-hand sides, 64 by default) and the device budget
+ * mem_budget (bytes) next to the packs already running (a job larger than
+ * the budget runs alone), and solves the group as one pack (as
+ * ifdda_solve_batch).  Results do not depend on what else runs.
  * ------------------------------------------------------------------------- */
 typedef struct {
   int nx, ny, nz;          /* box                                             */
@@ -357,6 +359,16 @@ typedef struct {
   ifdda_solve_opts opts;
 } ifdda_job;
 
+/* Independent problems solved together (no counterpart in the reference):
+ * every right-hand side of every job shares the matvec launches (packed
+ * launches over the members' own boxes, Green tensors, maps and
+ * polarisabilities) and one lockstep Krylov solve with per-RHS state and
+ * convergence, so each result equals the job's own solve.  All jobs use
+ * jobs[0].opts.  FFloc[j] / FF[j] (nrhs * 3 nd complex, or FFloc / FF NULL)
+ * and infos[j] (nrhs entries) receive job j's results. */
+int ifdda_solve_batch(int njobs, const ifdda_job *jobs, double *const *FFloc, double *const *FF,
+                      ifdda_solve_info *const *infos, volatile int *nstop);
+
 typedef struct ifdda_runner ifdda_runner;
 int ifdda_runner_create(int nworkers, double mem_budget, ifdda_runner **out);
 /* Copies the job's arrays; returns at once with a ticket. */
@@ -365,6 +377,8 @@ int ifdda_runner_submit(ifdda_runner *r, const ifdda_job *job, long long *ticket
  * (nrhs) may be NULL; times[3] (optional) = seconds queued, running, total. */
 int ifdda_runner_wait(ifdda_runner *r, long long ticket, double *FFloc, double *FF,
                       ifdda_solve_info *infos, double *times);
+/* Right-hand sides per pack (1: every job alone, apart from its own RHS). */
+int ifdda_runner_pack_limit(ifdda_runner *r, int max_members);
 /* Runs what is queued, then stops the workers. */
 int ifdda_runner_destroy(ifdda_runner *r);
 

@@ -1,4 +1,4 @@
-"""The runner (many independent solves at once): every job's result equals the
+"""The runner (many independent solves at once, grouped into packs): every job's result equals the
 same job solved alone through the handle API, bit for bit (all kernels are
 deterministic), whatever runs beside it; a budget too small for two jobs
 runs them one at a time."""
@@ -10,6 +10,7 @@ pytestmark = pytest.mark.gpu
 
 def _direct(gpu, s, k0, alpha, E, precision, tol):
     with gpu.Solver(s.nx, s.ny, s.nz) as sv:
+        sv.use_specialized(False)  # the generic z pass, as the runner's packed launches
         sv.greencalculfft(s.d, k0)
         sv.set_dipoles(s.tabdip)
         sv.set_polarizability(alpha)
