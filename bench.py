@@ -118,16 +118,25 @@ def main():
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--uneven-count", type=int, default=512)
     ap.add_argument("--uneven-seq", type=int, default=32, help="sequential FP64 solves timed")
+    ap.add_argument("--uneven-members", type=int, default=16, help="right-hand sides per pack")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-rates", default="2,4,8,16,32", help="requests per second")
-    ap.add_argument("--serving-requests", type=int, default=20, help="requests per rate")
+    ap.add_argument("--serving-requests", type=int, default=40, help="requests per rate")
     ap.add_argument("--serving-seq", type=int, default=8, help="FP64 single solves timed per small object")
     ap.add_argument("--workers", type=int, default=8, help="runner workers")
+    ap.add_argument("--section", choices=["batch", "uneven", "serving"],
+                    help="run only this section and print its result (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(independent solves shard trivially across GPUs; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.section:
+        import torch
+        torch.cuda.set_device(0)
+        fn = {"batch": bench_batch, "uneven": bench_uneven, "serving": bench_serving}[args.section]
+        print(json.dumps({args.section: fn(args)}), flush=True)
+        return
 
     import torch
 
@@ -338,7 +347,7 @@ def bench_uneven(args):
 
     probs = scene.config4(args.uneven_count, seed=4)
     print(f"uneven: {len(probs)} problems", file=sys.stderr, flush=True)
-    with ifdda.Runner(nworkers=args.workers, mem_budget=240e9) as run:
+    with ifdda.Runner(nworkers=args.workers, mem_budget=240e9, max_members=args.uneven_members) as run:
         warm = run.submit(probs[0][0].nx, probs[0][0].ny, probs[0][0].nz, probs[0][0].d, probs[0][1],
                           probs[0][2], probs[0][3], tabdip=probs[0][0].tabdip)
         run.wait(warm)
@@ -370,7 +379,7 @@ def bench_uneven(args):
     return {"workload": f"{len(probs)} solves: sphere / cube / ellipsoid / cylinder, nx, ny, nz in [8, 48], "
                         "Re eps in [1.5, 4], Im eps in [0, 0.5], lambda in [400, 800] nm, d = lambda / (10 |n|), "
                         "random plane wave; GPBICG tol 1e-4, Green tensor built per solve",
-            "workers": args.workers, "max_members": 64, "seconds": t_all, "solves_per_s": len(probs) / t_all,
+            "workers": args.workers, "max_members": args.uneven_members, "seconds": t_all, "solves_per_s": len(probs) / t_all,
             "one_pack": {"seconds": t_one, "solves_per_s": len(probs) / t_one},
             "iterations": {"mean": float(np.mean(its)), "max": int(np.max(its))},
             "dipoles": {"mean": float(np.mean([p[0].nd for p in probs])),

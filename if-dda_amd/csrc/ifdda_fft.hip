@@ -1092,12 +1092,12 @@ __global__ void __launch_bounds__(256) k_strided_pk(PkArgs<V> k) {
                                  a.nthr);
 }
 
-template <typename V>
+template <typename V, int NG>
 __global__ void __launch_bounds__(256) k_zconv_pk(PkArgs<V> k) {
   const int m = pk_member(k.boff, k.P, blockIdx.x);
   if (k.skip && k.skip[m * k.skip_stride]) return;
   const SArgs<V>& a = pk_args<SArgs<V>>(k.av, m);
-  zconv_tile<V, kMaxG, true>(a, a.src, a.dst, blockIdx.x - k.boff[m], a.gx, threadIdx.x, a.nthr);
+  zconv_tile<V, NG, true>(a, a.src, a.dst, blockIdx.x - k.boff[m], a.gx, threadIdx.x, a.nthr);
 }
 
 // ---------------------------------------------------------------------------
@@ -1663,6 +1663,7 @@ static size_t pack_blob_impl(const std::vector<const GridPlan*>& g,
     lay.nblk[p] = 0;
     lay.lds[p] = 0;
   }
+  lay.ng3 = 1;
   for (int i = 0; i < P; ++i) {
     PassSet<V> ps;
     pass_set(*g[i], m[i], pi, ps);
@@ -1676,6 +1677,9 @@ static size_t pack_blob_impl(const std::vector<const GridPlan*>& g,
       bo[i + 1] = lay.nblk[p];
       lay.lds[p] = std::max(lay.lds[p], ps.t[p]->lds);
     }
+    // Green items per thread of the z pass (as the single-problem launch)
+    const int items = g[i]->F[2] * ps.t[2]->B;
+    lay.ng3 = std::max(lay.ng3, (items + ps.t[2]->threads - 1) / ps.t[2]->threads);
   }
   lay.P = P;
   return off;
@@ -1709,7 +1713,8 @@ static bool matvec_pack_impl(const PackLayout& lay, const void* dev, const V* x,
   };
   auto k1 = k_xpass_pk<V, 3, +1, 1>;
   auto k2 = k_strided_pk<V, 3, +1>;
-  auto k3 = k_zconv_pk<V>;
+  auto k3 = lay.ng3 <= 1 ? k_zconv_pk<V, 1> : lay.ng3 == 2 ? k_zconv_pk<V, 2>
+                            : lay.ng3 == 3 ? k_zconv_pk<V, 3> : k_zconv_pk<V, kMaxG>;
   auto k4 = k_strided_pk<V, 3, -1>;
   auto k5 = k_xpass_pk<V, 3, -1, 2>;
   if (!allow_lds(k1, lay.lds[0]) || !allow_lds(k2, lay.lds[1]) || !allow_lds(k3, lay.lds[2]) ||

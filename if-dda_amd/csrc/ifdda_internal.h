@@ -121,6 +121,7 @@ struct PackLayout {
   size_t av[5] = {}, boff[5] = {};  // byte offsets in the blob
   int nblk[5] = {};                 // blocks of each packed launch
   size_t lds[5] = {};               // dynamic LDS of each packed launch
+  int ng3 = 1;                      // Green items per thread of the z pass
 };
 // Host image of the blob for members (g[i], m[i]) whose dipole vectors start
 // at element offs[i]; returns its size.
@@ -244,7 +245,10 @@ bool alpha_apply_one(const double2* x, double2* FF, const double2* al, int mode,
 // layout of packed problems: RHS r holds lens[r] elements at offs[r] (host
 // arrays of R values).
 struct Solver;
-Solver* solver_new(int R, bool need32, const long long* lens, const long long* offs);
+Solver* solver_new(int R, bool need32, const long long* lens, const long long* offs,
+                   long long min_total = 0, int min_R = 0);
+// Capacity of S's buffers (zeros for a null S).
+void solver_capacity(const Solver* S, long long* total, int* R);
 // Reuse S for another ragged layout (lengths and offsets copied on stream s,
 // the host arrays must live until s has run them); false when it is too small.
 bool solver_refit(Solver* S, int R, bool need32, const long long* lens, const long long* offs,

@@ -19,6 +19,8 @@
 // is ordered on the workspace's own non-blocking stream.
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -80,12 +82,21 @@ struct PackWs {
 
 namespace {
 
+// IFDDA_DEBUG set in the environment: one line on stderr per buffer growth.
+bool debug_on() {
+  static const bool on = std::getenv("IFDDA_DEBUG") != nullptr;
+  return on;
+}
+
+// Grow-only device buffer: at least doubles, so a worker meets O(log) growths
+// (each frees the old buffer, which waits for the device).
 bool grow(char** p, size_t* cap, size_t need, const char* what) {
   if (need <= *cap) return true;
+  const size_t c = std::max(need + need / 4, 2 * *cap);
+  if (debug_on()) std::fprintf(stderr, "ifdda pack: %s %zu -> %zu MB\n", what, *cap >> 20, c >> 20);
   (void)hipFree(*p);
   *p = nullptr;
   *cap = 0;
-  const size_t c = need + need / 4;
   if (!hip_ok(hipMalloc(p, c), what)) return false;
   *cap = c;
   return true;
@@ -445,9 +456,14 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
   const bool need32 = o->precision == 32;
   if (!solver_refit(w.S, P, need32, w.lens.data(), w.offs.data(), w.s)) {
     if (!hip_ok(hipStreamSynchronize(w.s), "solver")) return IFDDA_ERR_HIP;
+    // room for twice the elements and right-hand sides of the last workspace
+    long long cap_total = 0;
+    int cap_R = 0;
+    solver_capacity(w.S, &cap_total, &cap_R);
     solver_delete(w.S);
-    w.S = solver_new(P, need32, w.lens.data(), w.offs.data());
+    w.S = solver_new(P, need32, w.lens.data(), w.offs.data(), 2 * cap_total, 2 * cap_R);
     if (!w.S) return IFDDA_ERR_HIP;
+    if (debug_on()) std::fprintf(stderr, "ifdda pack: solver for %d RHS\n", P);
   }
   std::vector<ifdda_solve_info> info(P);
   const int rc = solve_core(op, w.S, o, P, E, FF, FL, info.data(), nstop, w.s, t0);
