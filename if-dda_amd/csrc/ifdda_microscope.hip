@@ -164,8 +164,12 @@ struct Incidence {
   double kx, ky, pp;
 };
 
+// dark: the darkfield microscope's ring of incidences (microsdf.f:159-221):
+// ideltam = max(int(2 pi / (deltak NA_inc / k0)) + 1, 8) directions at
+// phi = 2 pi idelta / ideltam on the condenser's aperture cone, both
+// polarisations (npolainc is 0 there), polarisation outer.
 static std::string incidence_list(const ifdda_ctx* c, const ifdda_microscope_opts& m,
-                                  std::vector<Incidence>* out, double* deltak) {
+                                  std::vector<Incidence>* out, double* deltak, bool dark) {
   const double k0 = m.k0, d = m.d;
   const double numaperk = k0 * m.numaperinc;
   const double xr = (c->cellbox[1] - c->cellbox[0]) * d, yr = (c->cellbox[3] - c->cellbox[2]) * d;
@@ -183,6 +187,17 @@ static std::string incidence_list(const ifdda_ctx* c, const ifdda_microscope_opt
     if (--imul == 0) return "In FFT lens nfft2d too small:Increase size of FFT 3";
   }
   *deltak = dk;
+  if (dark) {
+    const double dtheta = dk / k0 * m.numaperinc;
+    const int ideltam = std::max((int)(2.0 * kPi / dtheta) + 1, 8);
+    for (int ip = 0; ip < 2; ++ip)
+      for (int id = 0; id < ideltam; ++id) {
+        const double phi = (double)id * 2.0 * kPi / (double)ideltam;
+        out->push_back({id, 0, std::cos(phi) * k0 * m.numaperinc,
+                        std::sin(phi) * k0 * m.numaperinc, ip == 0 ? 0.0 : 1.0});
+      }
+    return "";
+  }
   const int npol = m.npolainc == 0 ? 2 : 1;
   for (int ip = 0; ip < npol; ++ip) {
     const double pp = m.npolainc == 1 ? 0.0 : m.npolainc == 2 ? 1.0 : (ip == 0 ? 0.0 : 1.0);
@@ -209,8 +224,8 @@ static bool mbuf_ensure(ifdda_ctx* c, size_t bytes) {
 
 using namespace ifdda;
 
-extern "C" int ifdda_microscope_incidences(ifdda_ctx* c, const ifdda_microscope_opts* m, int* ninc,
-                                           double* kxy, int cap, double* deltak) {
+static int incidences_api(ifdda_ctx* c, const ifdda_microscope_opts* m, int* ninc, double* kxy,
+                          int cap, double* deltak, bool dark) {
   if (!c || !m || !ninc || !deltak) {
     set_error("ifdda_microscope_incidences: NULL argument");
     return IFDDA_ERR_ARG;
@@ -220,7 +235,7 @@ extern "C" int ifdda_microscope_incidences(ifdda_ctx* c, const ifdda_microscope_
     return IFDDA_ERR_STATE;
   }
   std::vector<Incidence> inc;
-  const std::string msg = incidence_list(c, *m, &inc, deltak);
+  const std::string msg = incidence_list(c, *m, &inc, deltak, dark);
   if (!msg.empty()) {
     set_error("ifdda_microscope_incidences: " + msg);
     return IFDDA_ERR_ARG;
@@ -235,37 +250,50 @@ extern "C" int ifdda_microscope_incidences(ifdda_ctx* c, const ifdda_microscope_
   return IFDDA_OK;
 }
 
-extern "C" int ifdda_microsbf_device(ifdda_ctx* c, const ifdda_solve_opts* o,
-                                     const ifdda_microscope_opts* m, double* images, int* ninc,
-                                     ifdda_solve_info* infos, int infos_cap, volatile int* nstop) {
+extern "C" int ifdda_microscope_incidences(ifdda_ctx* c, const ifdda_microscope_opts* m, int* ninc,
+                                           double* kxy, int cap, double* deltak) {
+  return incidences_api(c, m, ninc, kxy, cap, deltak, false);
+}
+
+extern "C" int ifdda_microsdf_incidences(ifdda_ctx* c, const ifdda_microscope_opts* m, int* ninc,
+                                         double* kxy, int cap, double* deltak) {
+  return incidences_api(c, m, ninc, kxy, cap, deltak, true);
+}
+
+// The brightfield (microsbf.f) and darkfield (microsdf.f) microscopes: they
+// differ in the incidences (lattice in the condenser disk / ring on its
+// cone) and in how the direct wave enters the pupil (below).
+static int microscope_run(ifdda_ctx* c, const ifdda_solve_opts* o, const ifdda_microscope_opts* m,
+                          double* images, int* ninc, ifdda_solve_info* infos, int infos_cap,
+                          volatile int* nstop, bool dark) {
   if (!c || !o || !m || !images || !ninc) {
-    set_error("ifdda_microsbf_device: NULL argument");
+    set_error(std::string(dark ? "ifdda_microsdf_device: " : "ifdda_microsbf_device: ") + "NULL argument");
     return IFDDA_ERR_ARG;
   }
   if (!(m->numaperinc > 0.0 && m->numaperinc < 1.0) || !(m->numaper > 0.0 && m->numaper <= 1.0)) {
-    set_error("ifdda_microsbf_device: NA inc strictly between 0 and 1");
+    set_error(std::string(dark ? "ifdda_microsdf_device: " : "ifdda_microsbf_device: ") + "NA inc strictly between 0 and 1");
     return IFDDA_ERR_ARG;
   }
   if (m->nfft2d < 8 || m->nfft2d % 2 || !(m->k0 > 0.0) || !(m->d > 0.0) || !(m->gross > 0.0) ||
       (m->nside != 1 && m->nside != -1) || m->npolainc < 0 || m->npolainc > 2) {
-    set_error("ifdda_microsbf_device: bad option (nfft2d even >= 8, k0, d, gross > 0, nside +-1, "
+    set_error(std::string(dark ? "ifdda_microsdf_device: " : "ifdda_microsbf_device: ") + "bad option (nfft2d even >= 8, k0, d, gross > 0, nside +-1, "
               "npolainc 0..2)");
     return IFDDA_ERR_ARG;
   }
   if (!c->have_dipoles || !c->green_ready || c->alpha_mode == kAlphaNone) {
-    set_error("ifdda_microsbf_device: set the Green tensor, dipoles and polarisability first");
+    set_error(std::string(dark ? "ifdda_microsdf_device: " : "ifdda_microsbf_device: ") + "set the Green tensor, dipoles and polarisability first");
     return IFDDA_ERR_STATE;
   }
   const int N = m->nfft2d, h = N / 2;
   const double k0 = m->k0, d = m->d, nak = m->numaper * k0;
   std::vector<Incidence> inc;
   double dkinc = 0.0;
-  std::string msg = incidence_list(c, *m, &inc, &dkinc);
+  std::string msg = incidence_list(c, *m, &inc, &dkinc, dark);
   int imax = 0;
   double dk = 0.0;
   if (msg.empty()) msg = lens_geometry(N, k0, d, nak, &imax, &dk);
   if (!msg.empty()) {
-    set_error("ifdda_microsbf_device: " + msg);
+    set_error(std::string(dark ? "ifdda_microsdf_device: " : "ifdda_microsbf_device: ") + "" + msg);
     return IFDDA_ERR_ARG;
   }
   const int ni = (int)inc.size();
@@ -342,7 +370,7 @@ extern "C" int ifdda_microsbf_device(ifdda_ctx* c, const ifdda_solve_opts* o,
   double* dkv = (double*)take(sizeof(double) * 3 * B);
   int* dbin = (int*)take(sizeof(int) * B);
   if (w > wend) {  // the slices above must match the size computed for them
-    set_error("ifdda_microsbf_device: internal workspace layout error");
+    set_error(std::string(dark ? "ifdda_microsdf_device: " : "ifdda_microsbf_device: ") + "internal workspace layout error");
     return IFDDA_ERR_STATE;
   }
   hipStream_t s = c->stream;
@@ -364,10 +392,13 @@ extern "C" int ifdda_microsbf_device(ifdda_ctx* c, const ifdda_solve_opts* o,
       plane_wave_vector(k0, E0, q.pp, q.kx, q.ky, e, &kz);
       hk[3 * r] = q.kx, hk[3 * r + 1] = q.ky, hk[3 * r + 2] = kz;
       for (int cc = 0; cc < 3; ++cc) he[3 * r + cc] = e[cc];
-      // incident bin: deltakroutine(kxinc, kyinc, deltak, deltak, k0) (microsbf.f:689-690)
-      int ik = (int)std::lround(q.kx / dkinc), jk = (int)std::lround(q.ky / dkinc);
-      if ((dkinc * ik) * (dkinc * ik) + (dkinc * jk) * (dkinc * jk) > k0 * k0)
-        ik = (int)(q.kx / dkinc), jk = (int)(q.ky / dkinc);
+      // incident bin: deltakroutine(kxinc, kyinc, deltak, deltak, k0) with the
+      // condenser step (microsbf.f:689-690), or with the pupil step
+      // (microsdf.f:668-669)
+      const double dkb = dark ? dk : dkinc;
+      int ik = (int)std::lround(q.kx / dkb), jk = (int)std::lround(q.ky / dkb);
+      if ((dkb * ik) * (dkb * ik) + (dkb * jk) * (dkb * jk) > k0 * k0)
+        ik = (int)(q.kx / dkb), jk = (int)(q.ky / dkb);
       hbin[r] = -1;
       const double bx = dk * ik, by = dk * jk;
       if (std::abs(ik) <= imax && std::abs(jk) <= imax && bx * bx + by * by <= nak * nak) {
@@ -375,9 +406,14 @@ extern "C" int ifdda_microsbf_device(ifdda_ctx* c, const ifdda_solve_opts* o,
         double2 eb[3];
         double kzb;
         plane_wave_vector(k0, E0, q.pp, bx, by, eb, &kzb);  // at the bin's (kx, ky), the origin
-        hinc[3 * r] = make_double2(eb[0].x / (dk * dk), eb[0].y / (dk * dk));
-        hinc[3 * r + 1] = make_double2(eb[1].x / (dk * dk), eb[1].y / (dk * dk));
-        hinc[3 * r + 2] = make_double2(eb[2].x / dk, eb[2].y / dk);  // as microsbf.f:745-746
+        if (dark) {  // i E / (dkx dky) on every component (microsdf.f:712-717)
+          for (int cc = 0; cc < 3; ++cc)
+            hinc[3 * r + cc] = make_double2(-eb[cc].y / (dk * dk), eb[cc].x / (dk * dk));
+        } else {
+          hinc[3 * r] = make_double2(eb[0].x / (dk * dk), eb[0].y / (dk * dk));
+          hinc[3 * r + 1] = make_double2(eb[1].x / (dk * dk), eb[1].y / (dk * dk));
+          hinc[3 * r + 2] = make_double2(eb[2].x / dk, eb[2].y / dk);  // as microsbf.f:745-746
+        }
       } else {
         hinc[3 * r] = hinc[3 * r + 1] = hinc[3 * r + 2] = make_double2(0.0, 0.0);
       }
@@ -393,7 +429,7 @@ extern "C" int ifdda_microsbf_device(ifdda_ctx* c, const ifdda_solve_opts* o,
     for (int r = 0; r < R; ++r)
       if (infos && i0 + r < infos_cap) infos[i0 + r] = inf[r];
     if (nstop && *nstop == -1) {
-      set_error("ifdda_microsbf_device: Calculation cancelled during iterative method");
+      set_error(std::string(dark ? "ifdda_microsdf_device: " : "ifdda_microsbf_device: ") + "Calculation cancelled during iterative method");
       return IFDDA_OK;
     }
     int im2;
@@ -419,4 +455,16 @@ extern "C" int ifdda_microsbf_device(ifdda_ctx* c, const ifdda_solve_opts* o,
       !hip_ok(hipStreamSynchronize(s), "microscope images"))
     return IFDDA_ERR_HIP;
   return IFDDA_OK;
+}
+
+extern "C" int ifdda_microsbf_device(ifdda_ctx* c, const ifdda_solve_opts* o,
+                                     const ifdda_microscope_opts* m, double* images, int* ninc,
+                                     ifdda_solve_info* infos, int infos_cap, volatile int* nstop) {
+  return microscope_run(c, o, m, images, ninc, infos, infos_cap, nstop, false);
+}
+
+extern "C" int ifdda_microsdf_device(ifdda_ctx* c, const ifdda_solve_opts* o,
+                                     const ifdda_microscope_opts* m, double* images, int* ninc,
+                                     ifdda_solve_info* infos, int infos_cap, volatile int* nstop) {
+  return microscope_run(c, o, m, images, ninc, infos, infos_cap, nstop, true);
 }

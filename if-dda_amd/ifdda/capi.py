@@ -111,7 +111,11 @@ def lib():
                                     C.POINTER(C.POINTER(SolveInfo)), C.POINTER(C.c_int)]
     L.ifdda_microscope_incidences.argtypes = [vp, C.POINTER(MicroscopeOpts), C.POINTER(C.c_int), pd,
                                               ip, pd]
+    L.ifdda_microsdf_incidences.argtypes = [vp, C.POINTER(MicroscopeOpts), C.POINTER(C.c_int), pd,
+                                              ip, pd]
     L.ifdda_microsbf_device.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(MicroscopeOpts), pd,
+                                        C.POINTER(C.c_int), C.POINTER(SolveInfo), ip, C.POINTER(C.c_int)]
+    L.ifdda_microsdf_device.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(MicroscopeOpts), pd,
                                         C.POINTER(C.c_int), C.POINTER(SolveInfo), ip, C.POINTER(C.c_int)]
     L.ifdda_specialized_z.argtypes = [vp, ip, C.POINTER(C.c_int)]
     L.ifdda_timing_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
@@ -347,32 +351,40 @@ class Solver:
             m.origin[i] = float(origin[i])
         return m
 
-    def microscope_incidences(self, m):
-        """[(kx, ky, pp)] of the condenser lattice, and its step deltak."""
+    def microscope_incidences(self, m, dark=False):
+        """[(kx, ky, pp)] of the condenser (brightfield lattice, or the
+        darkfield ring), and the condenser step deltak."""
+        name = "ifdda_microsdf_incidences" if dark else "ifdda_microscope_incidences"
+        fn = getattr(lib(), name)
         n = C.c_int(0)
         dk = C.c_double(0.0)
-        _check(lib().ifdda_microscope_incidences(self.h, C.byref(m), C.byref(n), None, 0, C.byref(dk)),
-               "ifdda_microscope_incidences")
+        _check(fn(self.h, C.byref(m), C.byref(n), None, 0, C.byref(dk)), name)
         kxy = np.zeros(3 * n.value)
         D = C.POINTER(C.c_double)
-        _check(lib().ifdda_microscope_incidences(self.h, C.byref(m), C.byref(n), kxy.ctypes.data_as(D),
-                                                 n.value, C.byref(dk)), "ifdda_microscope_incidences")
+        _check(fn(self.h, C.byref(m), C.byref(n), kxy.ctypes.data_as(D), n.value, C.byref(dk)), name)
         return kxy.reshape(-1, 3), dk.value
 
     def microsbf(self, m, method="GPBICG1", tol=1e-4, precision=64, x0_zero=False, maxit=10000,
-                 check_every=0, nstop=None):
-        """Returns (images [6, N, N]: scattered x, y, z, total x, y, z; infos)."""
+                 check_every=0, nstop=None, dark=False):
+        """Brightfield (microsbf.f) or, dark=True, darkfield (microsdf.f)
+        microscope.  Returns (images [6, N, N]: scattered x, y, z, total x,
+        y, z; infos)."""
         o = self._opts(method, tol, precision, x0_zero, maxit, check_every)
-        ninc, _ = self.microscope_incidences(m)
+        ninc, _ = self.microscope_incidences(m, dark)
         infos = (SolveInfo * max(1, ninc.shape[0]))()
         N = m.nfft2d
         img = np.zeros(6 * N * N)
         n = C.c_int(0)
         stop = nstop if nstop is not None else C.c_int(0)
-        _check(lib().ifdda_microsbf_device(self.h, C.byref(o), C.byref(m),
-                                           img.ctypes.data_as(C.POINTER(C.c_double)), C.byref(n),
-                                           infos, len(infos), C.byref(stop)), "ifdda_microsbf_device")
+        name = "ifdda_microsdf_device" if dark else "ifdda_microsbf_device"
+        _check(getattr(lib(), name)(self.h, C.byref(o), C.byref(m),
+                                    img.ctypes.data_as(C.POINTER(C.c_double)), C.byref(n),
+                                    infos, len(infos), C.byref(stop)), name)
         return img.reshape(6, N, N), [infos[i].as_dict() for i in range(n.value)]
+
+    def microsdf(self, m, **kw):
+        """Darkfield microscope (microsdf.f), as microsbf."""
+        return self.microsbf(m, dark=True, **kw)
 
     def use_specialized(self, on: bool):
         """Select compile-time (default) or generic z-pass kernels."""

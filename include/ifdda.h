@@ -335,6 +335,22 @@ int ifdda_microsbf_device(ifdda_ctx *ctx, const ifdda_solve_opts *opts,
                           const ifdda_microscope_opts *m, double *images, int *ninc,
                           ifdda_solve_info *infos, int infos_cap, volatile int *nstop);
 
+/* Darkfield and phase microscope (microsdf.f:1-915, quick-lens path): as
+ * ifdda_microsbf_device with the incidences of microsdf.f:159-221 -- a ring
+ * of ideltam = max(int(2 pi k0 / (deltak NA_inc)) + 1, 8) plane waves on the
+ * condenser cone (kx, ky) = k0 NA_inc (cos phi, sin phi), both
+ * polarisations (npolainc is ignored, as there) -- and the direct wave added
+ * at its bin of the pupil grid as i E / (dkx dky) on every component
+ * (microsdf.f:668-717).  With NA_inc above the objective's NA the direct
+ * wave stays outside the pupil (dark field). */
+int ifdda_microsdf_device(ifdda_ctx *ctx, const ifdda_solve_opts *opts,
+                          const ifdda_microscope_opts *m, double *images, int *ninc,
+                          ifdda_solve_info *infos, int infos_cap, volatile int *nstop);
+/* The incidences of ifdda_microsdf_device (polarisation outer, phi inner);
+ * *deltak as ifdda_microscope_incidences. */
+int ifdda_microsdf_incidences(ifdda_ctx *ctx, const ifdda_microscope_opts *m, int *ninc,
+                              double *kxy, int cap, double *deltak);
+
 /* ---------------------------------------------------------------------------
  * Runner: many independent solves at once (no counterpart in the reference,
  * whose callers loop over objects, wavelengths and requests one cdmlib call

@@ -1,5 +1,6 @@
-"""TEST INFRASTRUCTURE ONLY: numpy restatement of the brightfield microscope
-of cdmlib (microsbf.f, quick-lens path nquicklens = 1) and its pieces:
+"""TEST INFRASTRUCTURE ONLY: numpy restatement of the brightfield and
+darkfield microscopes of cdmlib (microsbf.f, microsdf.f, quick-lens path
+nquicklens = 1) and their pieces:
 
   ondeplanekxky       plane wave of wave vector (kx, ky, kz)   ondeplanekxky.f
   deltakroutine       bin of the incident wave                 deltakroutine.f
@@ -7,7 +8,9 @@ of cdmlib (microsbf.f, quick-lens path nquicklens = 1) and its pieces:
   diffractefft2dlens  field in the lens pupil                  diffractefft2dlens.f
   fouriertoimage      pupil -> image plane                     fouriertoimage.f
   incidences          the condenser's (kx, ky) lattice         microsbf.f:90-172
-  image_assembly      pupil fill, lens rotation, |E|^2 sums    microsbf.f:676-845
+  incidences_df       the darkfield ring of incidences         microsdf.f:91-221
+  image_assembly      pupil fill, lens rotation, |E|^2 sums    microsbf.f:676-845,
+                      (dark=True)                               microsdf.f:655-790
 
 Only tests/ imports this module.
 """
@@ -151,16 +154,39 @@ def incidences(k0, numaperinc, d, nfft2d, xs, ys):
     return inc, dkinc, dkx
 
 
-def image_assembly(Elens_list, inc_list, k0, E0, numaper, zlens, gross, dkinc, imax, dk, nfft2d):
+def incidences_df(k0, numaperinc, d, nfft2d, xs, ys):
+    """microsdf.f:91-221 with nquicklens = 1: deltak as the brightfield's
+    (microsdf.f:91-137), then ideltam = max(int(2 pi / (deltak NA_inc / k0))
+    + 1, 8) incidences at phi = 2 pi idelta / ideltam on the cone of NA_inc
+    (kx, ky) = k0 NA_inc (cos phi, sin phi), for pp = 0 then pp = 1
+    (npolainc = 0 there).  Returns [(idelta, pp, kx, ky)], deltak, deltakx."""
+    _, dkinc, dkx = incidences(k0, numaperinc, d, nfft2d, xs, ys)
+    dtheta = dkinc / k0 * numaperinc
+    ideltam = max(int(2.0 * np.pi / dtheta) + 1, 8)
+    out = []
+    for pp in (0.0, 1.0):
+        for idel in range(ideltam):
+            phi = idel * 2.0 * np.pi / ideltam
+            out.append((idel, pp, np.cos(phi) * k0 * numaperinc, np.sin(phi) * k0 * numaperinc))
+    return out, dkinc, dkx
+
+
+def image_assembly(Elens_list, inc_list, k0, E0, numaper, zlens, gross, dkinc, imax, dk, nfft2d,
+                   dark=False):
     """microsbf.f:676-845: for each (pupil field, (idelta, jdelta, kxinc, kyinc,
     pp)) fill the pupil (focus phase; the incident wave at its bin), rotate by
     the lens, fouriertoimage, add |E|^2; returns the six sqrt-images
-    (scattered x, y, z, with incident x, y, z) as [y][x] arrays."""
+    (scattered x, y, z, with incident x, y, z) as [y][x] arrays.  dark: the
+    darkfield's incident wave (microsdf.f:668-717): its bin from the pupil
+    step dk, added as i E / dk^2 on every component."""
     N = nfft2d
     acc = [np.zeros((N, N)) for _ in range(6)]
     iv = np.arange(-imax, imax + 1)
     for El, (idel, jdel, kxinc, kyinc, pp) in zip(Elens_list, inc_list):
-        ikx, jky = deltakroutine(kxinc, kyinc, dkinc, dkinc, k0)
+        if dark:
+            ikx, jky = deltakroutine(kxinc, kyinc, dk, dk, k0)
+        else:
+            ikx, jky = deltakroutine(kxinc, kyinc, dkinc, dkinc, k0)
         planes = [np.zeros((N, N), np.complex128) for _ in range(6)]
         for a, i in enumerate(iv):
             for b, j in enumerate(iv):
@@ -175,7 +201,10 @@ def image_assembly(Elens_list, inc_list, k0, E0, numaper, zlens, gross, dkinc, i
                 w = s.copy()
                 if i == ikx and j == jky:
                     e = ondeplanekxky(np.zeros((1, 3)), k0, E0, pp, kx, ky)
-                    w = w + np.array([e[0] / (dk * dk), e[1] / (dk * dk), e[2] / dk]) * zf
+                    if dark:
+                        w = w + np.array([e[0], e[1], e[2]]) / (dk * dk) * zf * 1j
+                    else:
+                        w = w + np.array([e[0] / (dk * dk), e[1] / (dk * dk), e[2] / dk]) * zf
                 if t != 0.0:
                     u = np.array([kx / k0, ky / k0, 0.0])
                     u[2] = np.sqrt(1.0 - u[0] ** 2 - u[1] ** 2)

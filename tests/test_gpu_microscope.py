@@ -1,8 +1,9 @@
-"""Brightfield microscope (microsbf.f, quick lens) on the GPU against an
-independent CPU pipeline: the oracle's inverserig per incidence, the numpy
-restatements of diffractefft2dlens / ondeplanekxky / the image assembly of
-microsbf.f (whose pieces are pinned to the compiled reference in
-test_microscopy_oracle.py).  Images agree to the solver tolerance."""
+"""Brightfield and darkfield microscopes (microsbf.f, microsdf.f, quick lens)
+on the GPU against an independent CPU pipeline: the oracle's inverserig per
+incidence, the numpy restatements of diffractefft2dlens / ondeplanekxky /
+the image assembly of microsbf.f and microsdf.f (whose pieces are pinned to
+the compiled reference in test_microscopy_oracle.py).  Images agree to the
+solver tolerance."""
 import numpy as np
 import pytest
 
@@ -38,9 +39,14 @@ def test_incidence_lattice(small):
     assert np.allclose(kxy, np.array(ref), rtol=0, atol=1e-9 * K0)
 
 
-def _cpu_images(s, alpha, oracle, m, tol):
+def _cpu_images(s, alpha, oracle, m, tol, dark=False):
     import microscopy as mi
-    inc, dkinc, _ = mi.incidences(K0, m.numaperinc, s.d, m.nfft2d, s.pos[:, 0], s.pos[:, 1])
+    if dark:
+        ring, dkinc, _ = mi.incidences_df(K0, m.numaperinc, s.d, m.nfft2d, s.pos[:, 0], s.pos[:, 1])
+        waves = [(idl, 0, kx, ky, pp) for (idl, pp, kx, ky) in ring]
+    else:
+        inc, dkinc, _ = mi.incidences(K0, m.numaperinc, s.d, m.nfft2d, s.pos[:, 0], s.pos[:, 1])
+        waves = [(idl, jdl, kx, ky, pp) for pp in (0.0, 1.0) for (idl, jdl, kx, ky) in inc]
     G = oracle.green_fft(s.nx, s.ny, s.nz, (2 * s.nx, 2 * s.ny, 2 * s.nz), s.d, K0)
     pol = np.zeros((s.nx * s.ny * s.nz, 3, 3), np.complex128)
     for a in range(3):
@@ -48,20 +54,18 @@ def _cpu_images(s, alpha, oracle, m, tol):
     E0 = complex(m.E0[0], m.E0[1])
     zs = s.origin[2] + s.d * np.arange(s.nz)
     cells = np.nonzero(s.tabdip)[0]
-    lens, lst = [], []
-    for pp in (0.0, 1.0):
-        for (idl, jdl, kx, ky) in inc:
-            Einc = mi.ondeplanekxky(s.pos, K0, E0, pp, kx, ky)
-            o = oracle.inverserig(G, s.tabdip, s.nx, s.ny, s.nz, pol, Einc, "GPBICG1", tol)
-            p = (alpha[:, None] * o["FFloc"].reshape(-1, 3))
-            FFbox = np.zeros((s.nx * s.ny * s.nz, 3), np.complex128)
-            FFbox[cells] = p[s.tabdip[cells] - 1]
-            El, imax, dk = mi.diffractefft2dlens(FFbox.reshape(-1), s.nx, s.ny, s.nz, m.nfft2d, K0,
-                                                 s.origin[0], s.origin[1], zs, s.d, m.numaper * K0, m.nside)
-            lens.append(El)
-            lst.append((idl, jdl, kx, ky, pp))
-    return mi.image_assembly(lens, lst, K0, E0, m.numaper * K0, m.zlens, m.gross, dkinc, imax, dk,
-                             m.nfft2d)
+    lens = []
+    for (idl, jdl, kx, ky, pp) in waves:
+        Einc = mi.ondeplanekxky(s.pos, K0, E0, pp, kx, ky)
+        o = oracle.inverserig(G, s.tabdip, s.nx, s.ny, s.nz, pol, Einc, "GPBICG1", tol)
+        p = (alpha[:, None] * o["FFloc"].reshape(-1, 3))
+        FFbox = np.zeros((s.nx * s.ny * s.nz, 3), np.complex128)
+        FFbox[cells] = p[s.tabdip[cells] - 1]
+        El, imax, dk = mi.diffractefft2dlens(FFbox.reshape(-1), s.nx, s.ny, s.nz, m.nfft2d, K0,
+                                             s.origin[0], s.origin[1], zs, s.d, m.numaper * K0, m.nside)
+        lens.append(El)
+    return mi.image_assembly(lens, waves, K0, E0, m.numaper * K0, m.zlens, m.gross, dkinc, imax, dk,
+                             m.nfft2d, dark=dark)
 
 
 def test_brightfield_images_match_cpu_pipeline(small, oracle):
@@ -100,3 +104,31 @@ def test_brightfield_argument_errors(small, gpu):
     tiny = sv.microscope_opts(16, 0.8, 0.5, K0, s.d, s.origin)
     with pytest.raises(gpu.IfddaError, match="nfft2d too small"):
         sv.microsbf(tiny)
+
+
+@pytest.mark.parametrize("na,nainc", [(0.5, 0.8), (0.8, 0.5)])
+def test_darkfield_images_match_cpu_pipeline(small, oracle, na, nainc):
+    """Condenser NA above the objective's (dark field: the direct wave misses
+    the pupil, so total = scattered image) and below it (the direct wave
+    enters, phase-microscope case)."""
+    import microscopy as mi
+    s, alpha, sv, _ = small
+    m = sv.microscope_opts(128, na, nainc, K0, s.d, s.origin, gross=100.0, zlens=50e-9, nside=1,
+                           npolainc=0, E0=1.0 + 0.5j)
+    kxy, dk = sv.microscope_incidences(m, dark=True)
+    ring, dkinc, _ = mi.incidences_df(K0, nainc, s.d, 128, s.pos[:, 0], s.pos[:, 1])
+    assert abs(dk / dkinc - 1) < 1e-15
+    assert np.allclose(kxy, np.array([(kx, ky, pp) for (_, pp, kx, ky) in ring]), rtol=0,
+                       atol=1e-9 * K0)
+    print(f"darkfield NA {na} NA_inc {nainc}: {len(ring)} incidences", flush=True)
+    img, infos = sv.microsdf(m, method="GPBICG1", tol=1e-10, precision=64)
+    assert len(infos) == len(ring) and all(i["nstop"] == 0 and i["tol1"] <= 1e-10 for i in infos)
+    ref = _cpu_images(s, alpha, oracle, m, 1e-10, dark=True)
+    for c in range(6):
+        scale = np.abs(ref[c]).max()
+        assert scale > 0
+        assert np.abs(img[c] - ref[c]).max() < 1e-7 * scale, (c, np.abs(img[c] - ref[c]).max() / scale)
+    if nainc > na:
+        assert np.array_equal(img[:3], img[3:])
+    else:
+        assert np.abs(img[3:] - img[:3]).max() > 0.1 * np.abs(img[:3]).max()

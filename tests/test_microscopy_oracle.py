@@ -50,3 +50,27 @@ def test_pieces_match_compiled_reference(ref):
     planes = [rng.standard_normal((16, 16)) + 1j * rng.standard_normal((16, 16)) for _ in range(6)]
     got, want = mi.fouriertoimage(2e5, 40.0, planes), ref.fouriertoimage(2e5, 40.0, planes)
     assert relerr(np.stack(got), np.stack(want)) < 1e-12
+
+
+@pytest.mark.parametrize("nainc,N", [(0.8, 128), (0.3, 256), (0.95, 256)])
+def test_darkfield_ring(nainc, N):
+    """microsdf.f:159-221: the darkfield incidences lie on the cone of the
+    condenser NA, equally spaced in azimuth from phi = 0, ideltam =
+    max(int(2 pi / (deltak NA_inc / k0)) + 1, 8) of them per polarisation
+    (pp = 0 first), deltak the brightfield condenser step."""
+    import microscopy as mi
+    k0 = 2 * np.pi / 632.8e-9
+    d = 20e-9
+    xs = np.arange(6) * d
+    ring, dkinc, dkx = mi.incidences_df(k0, nainc, d, N, xs, xs)
+    _, dk_bf, _ = mi.incidences(k0, nainc, d, N, xs, xs)
+    assert dkinc == dk_bf and abs(dkinc / dkx - round(dkinc / dkx)) < 1e-12
+    m = len(ring) // 2
+    assert m == max(int(2 * np.pi / (dkinc / k0 * nainc)) + 1, 8) and len(ring) == 2 * m
+    kx = np.array([r[2] for r in ring])
+    ky = np.array([r[3] for r in ring])
+    assert np.allclose(np.hypot(kx, ky), k0 * nainc, rtol=1e-14)
+    phi = np.mod(np.arctan2(ky[:m], kx[:m]), 2 * np.pi)
+    assert np.allclose(phi, 2 * np.pi * np.arange(m) / m, atol=1e-12)
+    assert [r[1] for r in ring] == [0.0] * m + [1.0] * m
+    assert np.array_equal(kx[:m], kx[m:]) and np.array_equal(ky[:m], ky[m:])
