@@ -400,48 +400,50 @@ def bench_serving(args):
     rng = np.random.default_rng(5)
     lam = 632.8e-9
     k0 = 2 * np.pi / lam
-    small = []
-    for i in range(4):
-        o = scene.random_spheres(64, 1.28e-6, 20, 150e-9, 2.0 + 0.001j, seed=100 + i)
-        small.append((o, scene.polarizability(o.eps, o.d, k0, "RR")))
-    large = []
-    for i in range(2):
-        o = scene.inhomogeneous_cube(96, 96 * 20e-9, 2.25 + 0.01j, 0.1, 100e-9, seed=200 + i)
-        large.append((o, scene.polarizability(o.eps, o.d, k0, "RR")))
+    small = [scene.random_spheres(64, 1.28e-6, 20, 150e-9, 2.0 + 0.001j, seed=100 + i) for i in range(4)]
+    large = [scene.inhomogeneous_cube(96, 96 * 20e-9, 2.25 + 0.01j, 0.1, 100e-9, seed=200 + i)
+             for i in range(2)]
 
+    # a request: object (map, permittivity), wavelength and its incidences as
+    # (theta, phi, pp) -- the plane waves and the polarisability are formed on
+    # the device
     def request():
         big = rng.random() < 0.1
-        o, al = (large if big else small)[rng.integers(2 if big else 4)]
+        o = (large if big else small)[rng.integers(2 if big else 4)]
         R = int(rng.integers(1, 17 if big else 65))
         th, ph = scene.pupil_incidences(R, 0.95, int(rng.integers(1 << 30)))
-        pp = rng.random(R)
-        E = np.stack([scene.plane_wave(o.pos, k0, t, f, pp=q) for t, f, q in zip(th, ph, pp)])
-        return o, al, E, big
+        W = np.stack([th, ph, rng.random(R)], axis=1)
+        return o, W, big
+
+    def submit(run, o, W, precision):
+        return run.submit(o.nx, o.ny, o.nz, o.d, k0, tabdip=o.tabdip, eps=o.eps, pola="RR", waves=W,
+                          wave="plane", origin=o.origin, tol=1e-4, precision=precision)
 
     n_req = args.serving_requests
     rates = [float(r) for r in args.serving_rates.split(",")]
     reqs = [[request() for _ in range(n_req)] for _ in rates]
     print(f"serving: {len(rates)} rates x {n_req} requests", file=sys.stderr, flush=True)
     out = {"workload": "Poisson arrivals; 90% 20-sphere 64^3 objects with 1-64 incidences, 10% 96^3 "
-                       "inhomogeneous cubes with 1-16 incidences; mixed precision, tol 1e-4; admission "
+                       "inhomogeneous cubes with 1-16 incidences; a request is (object map, "
+                       "permittivity, wavelength, incidence angles and polarisations), its plane waves "
+                       "and polarisability formed on the device; mixed precision, tol 1e-4; admission "
                        "budget 240 GB", "workers": args.workers, "rates": []}
     with ifdda.Runner(nworkers=args.workers, mem_budget=240e9) as run:
-        o, al, E, _ = reqs[0][0]
-        run.wait(run.submit(o.nx, o.ny, o.nz, o.d, k0, al, E[:1], tabdip=o.tabdip))  # warm-up
+        o, W, _ = reqs[0][0]
+        run.wait(submit(run, o, W[:1], 32))  # warm-up
         for rate, rs in zip(rates, reqs):
             arr = np.cumsum(rng.exponential(1.0 / rate, len(rs)))
             t0 = time.perf_counter()
             tickets = []
-            for a, (o, al, E, big) in zip(arr, rs):
+            for a, (o, W, big) in zip(arr, rs):
                 dt = t0 + a - time.perf_counter()
                 if dt > 0:
                     time.sleep(dt)
-                tickets.append(run.submit(o.nx, o.ny, o.nz, o.d, k0, al, E, tabdip=o.tabdip, tol=1e-4,
-                                          precision=32))
+                tickets.append(submit(run, o, W, 32))
             res = [run.wait(t) for t in tickets]
             span = time.perf_counter() - t0
             lat = np.array([r["latency_s"] for r in res])
-            nrhs = sum(E.shape[0] for (_, _, E, _) in rs)
+            nrhs = sum(W.shape[0] for (_, W, _) in rs)
             assert all(i["nstop"] == 0 and i["tol1"] <= 1e-4 for r in res for i in r["infos"])
             out["rates"].append({"rate_per_s": rate, "requests": len(rs), "incidences": nrhs,
                                  "span_s": span, "solves_per_s": nrhs / span,
@@ -455,19 +457,17 @@ def bench_serving(args):
     per = {}
     with ifdda.Runner(nworkers=1, mem_budget=240e9) as run:
         for big, pool, ns in ((False, small, args.serving_seq), (True, large, max(2, args.serving_seq // 4))):
-            o, al = pool[0]
+            o = pool[0]
             th, ph = scene.pupil_incidences(ns, 0.95, 77)
-            Es = [scene.plane_wave(o.pos, k0, t, f, pp=1.0) for t, f in zip(th, ph)]
-            run.wait(run.submit(o.nx, o.ny, o.nz, o.d, k0, al, Es[0], tabdip=o.tabdip, tol=1e-4,
-                                precision=64))  # Green tensor resident
+            Ws = [np.array([[t, f, 1.0]]) for t, f in zip(th, ph)]
+            run.wait(submit(run, o, Ws[0], 64))  # Green tensor resident
             t0 = time.perf_counter()
-            for E in Es:
-                r = run.wait(run.submit(o.nx, o.ny, o.nz, o.d, k0, al, E, tabdip=o.tabdip, tol=1e-4,
-                                        precision=64))
+            for W in Ws:
+                r = run.wait(submit(run, o, W, 64))
                 assert r["infos"][0]["nstop"] == 0 and r["infos"][0]["tol1"] <= 1e-4
             per[big] = (time.perf_counter() - t0) / ns
-    n_small = sum(E.shape[0] for rs in reqs for (_, _, E, big) in rs if not big)
-    n_large = sum(E.shape[0] for rs in reqs for (_, _, E, big) in rs if big)
+    n_small = sum(W.shape[0] for rs in reqs for (_, W, big) in rs if not big)
+    n_large = sum(W.shape[0] for rs in reqs for (_, W, big) in rs if big)
     t_mix = (n_small * per[False] + n_large * per[True]) / (n_small + n_large)
     out["sequential_fp64"] = {"seconds_per_solve_small": per[False], "seconds_per_solve_large": per[True],
                               "incidence_share_large": n_large / (n_small + n_large),

@@ -261,6 +261,26 @@ int solve_core(const MvOp& op, Solver* S, const ifdda_solve_opts* o, int R, cons
                double2* FF, double2* FFloc, ifdda_solve_info* infos, volatile int* nstop,
                hipStream_t s, std::chrono::steady_clock::time_point t0);
 
+// Incident waves and polarisabilities (ifdda_beams.hip).
+// Polarisability of a cell of side d (poladiff, dddis = 1), IFDDA_POLA_*.
+double2 poladiff_host(double2 eps, double d, double k0, double eps0, int method);
+// ondeplanekxky.f polarisation vector (the wave's value at the origin) and kz.
+void plane_wave_vector(double k0, double2 E0, double pp, double kx, double ky, double2 e[3],
+                       double* kz_out);
+// Wave vector k and polarisation e of one IFDDA_WAVE_* wave with parameters
+// prm[3]; a non-empty message for invalid parameters.
+std::string wave_vectors(int kind, const double* prm, double2 E0, double k0, double k[3],
+                         double2 e[3]);
+// E_r = e_r exp(i k_r . x) at the dipoles' cell centres origin + d (i, j, k)
+// (dipcell: cell of each dipole, null for a dense box); RHS r at off[r] (or
+// r * 3 nd); kv, ev: device arrays of R (k, e) triples.
+bool launch_waves(double2* E, const long long* off, int R, long long nd, const int32_t* dipcell,
+                  int nx, int ny, const double origin[3], double d, const double* kv,
+                  const double2* ev, hipStream_t s);
+// al64 / al32 = poladiff(eps) per dipole on the device.
+bool launch_pola(const double2* eps, long long nd, double d, double k0, double eps0, int method,
+                 double2* al64, float2* al32, hipStream_t s);
+
 // Packs of independent problems (ifdda_pack.hip).  A workspace keeps its
 // stream, FFT tables, Green tensors and grow-only buffers between packs.
 struct PackWs;

@@ -38,31 +38,6 @@ namespace {
 
 constexpr double kPi = 3.141592653589793238462643383279502884;
 
-// E[r][3 dip + c] = e_r[c] exp(i k_r . x(dip)), x from the dipole's cell.
-__global__ void __launch_bounds__(256) k_onde(double2* E, long long nd, const int32_t* dipcell,
-                                              int nx, int ny, double x0, double y0, double z0,
-                                              double d, const double* kv, const double2* ev) {
-  const int r = blockIdx.y;
-  const double kx = kv[3 * r], ky = kv[3 * r + 1], kz = kv[3 * r + 2];
-  const double2 e0 = ev[3 * r], e1 = ev[3 * r + 1], e2 = ev[3 * r + 2];
-  double2* Er = E + (long long)r * 3 * nd;
-  for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < nd;
-       p += (long long)gridDim.x * blockDim.x) {
-    const long long cell = dipcell ? (long long)dipcell[p] : p;
-    double2 ph = make_double2(0.0, 0.0);
-    if (cell >= 0) {
-      const long long i = cell % nx, j = (cell / nx) % ny, k = cell / ((long long)nx * ny);
-      const double arg = kx * (x0 + i * d) + ky * (y0 + j * d) + kz * (z0 + k * d);
-      double sn, cs;
-      sincos(arg, &sn, &cs);
-      ph = make_double2(cs, sn);
-    }
-    Er[3 * p] = cmul(e0, ph);
-    Er[3 * p + 1] = cmul(e1, ph);
-    Er[3 * p + 2] = cmul(e2, ph);
-  }
-}
-
 // P[r][0..2][bin] = M_bin Elens[r][.][bin]; P[r][3..5][bin] = the same plus
 // M_bin einc_r at the incident bin of RHS r.  M_bin = focus phase x lens
 // rotation x sqrt(u_z / v_z) (zero outside the objective aperture).
@@ -130,29 +105,6 @@ __global__ void __launch_bounds__(256) k_zero(double* I, long long n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     I[i] = 0.0;
-}
-
-// ondeplanekxky.f polarisation vector (the wave's value at the origin).
-void plane_wave_vector(double k0, double2 E0, double pp, double kx, double ky, double2 e[3],
-                       double* kz_out) {
-  const double p = std::sqrt(pp), s = std::sqrt(1.0 - pp);
-  const double2 E0s = make_double2(E0.x * s, E0.y * s), E0p = make_double2(E0.x * p, E0.y * p);
-  const double kz = std::sqrt(k0 * k0 - kx * kx - ky * ky);
-  *kz_out = kz;
-  if (kz >= 0.99999 * k0) {
-    e[0] = E0p;
-    e[1] = E0s;
-    e[2] = make_double2(0.0, 0.0);
-    return;
-  }
-  const double r = std::sqrt(kx * kx + ky * ky);
-  const double nsx = -ky / r, nsy = kx / r;
-  double npx = kz * nsy, npy = -kz * nsx, npz = -kx * nsy + ky * nsx;
-  const double nn = std::sqrt(npx * npx + npy * npy + npz * npz);
-  npx /= nn, npy /= nn, npz /= nn;
-  e[0] = make_double2(E0s.x * nsx + E0p.x * npx, E0s.y * nsx + E0p.y * npx);
-  e[1] = make_double2(E0s.x * nsy + E0p.x * npy, E0s.y * nsy + E0p.y * npy);
-  e[2] = make_double2(E0p.x * npz, E0p.y * npz);
 }
 
 }  // namespace
@@ -421,9 +373,8 @@ static int microscope_run(ifdda_ctx* c, const ifdda_solve_opts* o, const ifdda_m
     if (!hip_ok(hipMemcpyAsync(dkv, hk.data(), sizeof(double) * 3 * R, hipMemcpyHostToDevice, s), "copy k") ||
         !hip_ok(hipMemcpyAsync(dev, he.data(), sizeof(double2) * 3 * R, hipMemcpyHostToDevice, s), "copy e"))
       return IFDDA_ERR_HIP;
-    hipLaunchKernelGGL(k_onde, dim3((unsigned)std::min<long long>(1024, (nd + 255) / 256), R), dim3(256), 0, s,
-                       dEinc, nd, (const int32_t*)c->dipcell, c->g.nx, c->g.ny, m->origin[0], m->origin[1],
-                       m->origin[2], d, (const double*)dkv, (const double2*)dev);
+    if (!launch_waves(dEinc, nullptr, R, nd, c->dipcell, c->g.nx, c->g.ny, m->origin, d, dkv, dev, s))
+      return IFDDA_ERR_HIP;
     int rc = ifdda_solve_multi_device(c, o, R, dEinc, dFF, dFL, inf.data(), nstop, nullptr);
     if (rc != IFDDA_OK) return rc;
     for (int r = 0; r < R; ++r)
@@ -437,7 +388,7 @@ static int microscope_run(ifdda_ctx* c, const ifdda_solve_opts* o, const ifdda_m
     rc = farfield_dev(c, R, dFF, c->tab, nd, N, k0, d, m->origin[0], m->origin[1], m->origin[2], dL,
                       nullptr, &im2, &dk2, s, 1, nak, m->nside);
     if (rc != IFDDA_OK) return rc;
-    // einc (double2) is staged in the ev slot after k_onde has consumed it
+    // einc (double2) is staged in the ev slot after the waves kernel has consumed it
     if (!hip_ok(hipMemcpyAsync(dev, hinc.data(), sizeof(double2) * 3 * R, hipMemcpyHostToDevice, s), "copy einc") ||
         !hip_ok(hipMemcpyAsync(dbin, hbin.data(), sizeof(int) * R, hipMemcpyHostToDevice, s), "copy bins"))
       return IFDDA_ERR_HIP;

@@ -230,9 +230,14 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
   std::vector<char> dense(J, 0);
   for (int j = 0; j < J; ++j) {
     const ifdda_job& s = *in[j].job;
-    if (s.nx < 1 || s.ny < 1 || s.nz < 1 || s.nd < 1 || s.nrhs < 1 || !s.alpha || !s.E_inc ||
-        !(s.d > 0.0) || !(s.k0 > 0.0) || s.nquad < 0 || s.nquad > 10) {
+    if (s.nx < 1 || s.ny < 1 || s.nz < 1 || s.nd < 1 || s.nrhs < 1 || !(s.d > 0.0) ||
+        !(s.k0 > 0.0) || s.nquad < 0 || s.nquad > 10 || (!s.alpha && !s.eps) ||
+        (!s.E_inc && !s.waves)) {
       set_error("pack solve: bad job " + std::to_string(j));
+      return IFDDA_ERR_ARG;
+    }
+    if (!s.alpha && (s.pola < IFDDA_POLA_CM || s.pola > IFDDA_POLA_LR || s.eps0 < 0.0)) {
+      set_error("pack solve: job " + std::to_string(j) + ": unknown polarisability method");
       return IFDDA_ERR_ARG;
     }
     const size_t N = (size_t)s.nx * s.ny * s.nz;
@@ -269,6 +274,24 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
       return IFDDA_ERR_ARG;
     }
     keys[j] = GreenKey{s.nx, s.ny, s.nz, s.nquad, s.d, s.k0};
+  }
+  // incident waves formed on the device: (k, e) per right-hand side
+  std::vector<std::vector<double>> wk(J);
+  std::vector<std::vector<double2>> we(J);
+  for (int j = 0; j < J; ++j) {
+    const ifdda_job& s = *in[j].job;
+    if (s.E_inc) continue;
+    wk[j].resize(3 * (size_t)s.nrhs);
+    we[j].resize(3 * (size_t)s.nrhs);
+    for (int r = 0; r < s.nrhs; ++r) {
+      const std::string msg = wave_vectors(s.wave_kind, s.waves + 3 * r,
+                                           make_double2(s.E0[0], s.E0[1]), s.k0, &wk[j][3 * r],
+                                           &we[j][3 * r]);
+      if (!msg.empty()) {
+        set_error("pack solve: job " + std::to_string(j) + " wave " + std::to_string(r) + ": " + msg);
+        return IFDDA_ERR_ARG;
+      }
+    }
   }
   ++w.tick;
   // ---- Green tensors: cached per key; evict unused entries past the limit
@@ -351,7 +374,7 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
     total += (w.lens[m] + 63) & ~63LL;
   }
   // ---- arena: per job map and polarisability, per member FFT scratch, E/FF/FFloc
-  std::vector<size_t> o_tab(J), o_al64(J), o_al32(J), o_A(P), o_B(P);
+  std::vector<size_t> o_tab(J), o_al64(J), o_al32(J), o_A(P), o_B(P), o_cell(J), o_wk(J), o_we(J);
   size_t used = 0;
   auto take = [&](size_t bytes) {
     const size_t at = used;
@@ -363,7 +386,13 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
     o_tab[j] = dense[j] ? 0 : take(sizeof(int32_t) * (size_t)s.nx * s.ny * s.nz);
     o_al64[j] = take(sizeof(double2) * (size_t)s.nd);
     o_al32[j] = take(sizeof(float2) * (size_t)s.nd);
+    if (!s.E_inc) {
+      o_cell[j] = dense[j] ? 0 : take(sizeof(int32_t) * (size_t)s.nd);
+      o_wk[j] = take(sizeof(double) * 3 * (size_t)s.nrhs);
+      o_we[j] = take(sizeof(double2) * 3 * (size_t)s.nrhs);
+    }
   }
+  const size_t o_off = take(sizeof(long long) * (size_t)P);
   for (int m = 0; m < P; ++m) {
     const GridPlan& g = plans[mjob[m]];
     o_A[m] = take(sizeof(double2) * (size_t)g.nz * g.ny * g.L[0] * 3);
@@ -389,8 +418,10 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
     h64[j].resize(s.nd);
     h32[j].resize(s.nd);
     for (int i = 0; i < s.nd; ++i) {
-      h64[j][i] = make_double2(s.alpha[2 * i], s.alpha[2 * i + 1]);
-      h32[j][i] = make_float2((float)s.alpha[2 * i], (float)s.alpha[2 * i + 1]);
+      h64[j][i] = s.alpha ? make_double2(s.alpha[2 * i], s.alpha[2 * i + 1])
+                          : poladiff_host(make_double2(s.eps[2 * i], s.eps[2 * i + 1]), s.d, s.k0,
+                                          s.eps0 > 0.0 ? s.eps0 : 1.0, s.pola);
+      h32[j][i] = make_float2((float)h64[j][i].x, (float)h64[j][i].y);
     }
     jd[j].al64 = reinterpret_cast<double2*>(A + o_al64[j]);
     jd[j].al32 = reinterpret_cast<float2*>(A + o_al32[j]);
@@ -403,9 +434,37 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
   double2* FL = reinterpret_cast<double2*>(A + o_FL);
   for (int m = 0; m < P; ++m) {
     const ifdda_job& s = *in[mjob[m]].job;
+    if (!s.E_inc) continue;
     const double* src = s.E_inc + 2 * (size_t)w.lens[m] * mrhs[m];
     if (!hip_ok(hipMemcpyAsync(E + w.offs[m], src, sizeof(double2) * w.lens[m], hipMemcpyHostToDevice, w.s),
                 "copy E_inc"))
+      return IFDDA_ERR_HIP;
+  }
+  // waves formed on the device at each job's dipoles (cells from its map)
+  long long* doff = reinterpret_cast<long long*>(A + o_off);
+  if (!hip_ok(hipMemcpyAsync(doff, w.offs.data(), sizeof(long long) * P, hipMemcpyHostToDevice, w.s),
+              "copy offsets"))
+    return IFDDA_ERR_HIP;
+  std::vector<std::vector<int32_t>> cells(J);
+  for (int j = 0, m0 = 0; j < J; m0 += in[j].job->nrhs, ++j) {
+    const ifdda_job& s = *in[j].job;
+    if (s.E_inc) continue;
+    int32_t* dcell = nullptr;
+    if (!dense[j]) {
+      const size_t N = (size_t)s.nx * s.ny * s.nz;
+      cells[j].assign(s.nd, -1);
+      for (size_t i = 0; i < N; ++i)
+        if (s.tabdip[i] > 0) cells[j][s.tabdip[i] - 1] = (int32_t)i;
+      dcell = reinterpret_cast<int32_t*>(A + o_cell[j]);
+      if (!hip_ok(hipMemcpyAsync(dcell, cells[j].data(), sizeof(int32_t) * s.nd, hipMemcpyHostToDevice, w.s),
+                  "copy dipole cells"))
+        return IFDDA_ERR_HIP;
+    }
+    double* dk = reinterpret_cast<double*>(A + o_wk[j]);
+    double2* de = reinterpret_cast<double2*>(A + o_we[j]);
+    if (!hip_ok(hipMemcpyAsync(dk, wk[j].data(), sizeof(double) * wk[j].size(), hipMemcpyHostToDevice, w.s), "copy waves") ||
+        !hip_ok(hipMemcpyAsync(de, we[j].data(), sizeof(double2) * we[j].size(), hipMemcpyHostToDevice, w.s), "copy waves") ||
+        !launch_waves(E, doff + m0, s.nrhs, s.nd, dcell, s.nx, s.ny, s.origin, s.d, dk, de, w.s))
       return IFDDA_ERR_HIP;
   }
   // ---- packed-launch argument blocks

@@ -3,8 +3,8 @@
 // A pool of worker threads takes jobs (object, permittivity through alpha,
 // wavelength, lattice, incidences) from a queue in submission order.  A free
 // worker takes the job at the head and, with it, every queued job with the
-// same solve options that fits the pack (at most max_members right-hand
-// sides) and the memory budget next to the packs already running (a job
+// same solve options and a box within a factor 2 in cells that fits the pack
+// (at most max_members right-hand sides) and the memory budget next to the packs already running (a job
 // larger than the budget runs alone); the group is solved as one pack
 // (ifdda_pack.hip): all its right-hand sides share every matvec launch and
 // one lockstep Krylov solve, while each keeps its own iterations and
@@ -41,7 +41,7 @@ double now_s() {
 struct Job {
   ifdda_job spec{};
   std::vector<int32_t> tab;
-  std::vector<double> alpha, einc;
+  std::vector<double> alpha, einc;  // or eps, wave parameters
   double est = 0.0;
   int state = 0;  // 0 queued, 1 running, 2 done
   int rc = IFDDA_OK;
@@ -61,6 +61,15 @@ double job_bytes(const ifdda_job& j) {
 bool same_opts(const ifdda_solve_opts& a, const ifdda_solve_opts& b) {
   return a.method == b.method && a.precision == b.precision && a.x0_zero == b.x0_zero &&
          a.maxit == b.maxit && a.tol == b.tol && a.check_every == b.check_every;
+}
+
+// Jobs that may share a pack: the same solve options and boxes within a
+// factor 2 in cells.  A pack runs in lockstep until its slowest member has
+// converged; a small job packed with a much larger one (more cells, most
+// often more iterations) would wait for it.
+bool packable(const ifdda_job& a, const ifdda_job& head) {
+  const double na = (double)a.nx * a.ny * a.nz, nh = (double)head.nx * head.ny * head.nz;
+  return same_opts(a.opts, head.opts) && na <= 2.0 * nh && nh <= 2.0 * na;
 }
 
 }  // namespace
@@ -109,7 +118,7 @@ void worker(ifdda_runner* r) {
       int members = head->spec.nrhs;
       for (auto it = r->queue.begin(); it != r->queue.end() && members < r->max_members;) {
         Job* j = &r->jobs.at(*it);
-        if (same_opts(j->spec.opts, head->spec.opts) && members + j->spec.nrhs <= r->max_members &&
+        if (packable(j->spec, head->spec) && members + j->spec.nrhs <= r->max_members &&
             r->used + j->est <= r->budget) {
           group.push_back(j);
           r->used += j->est;
@@ -174,7 +183,7 @@ extern "C" int ifdda_runner_create(int nworkers, double mem_budget, ifdda_runner
 
 extern "C" int ifdda_runner_submit(ifdda_runner* r, const ifdda_job* j, long long* ticket) {
   if (!r || !j || !ticket || j->nx < 1 || j->ny < 1 || j->nz < 1 || j->nd < 1 || j->nrhs < 1 ||
-      !j->alpha || !j->E_inc || !(j->d > 0.0) || !(j->k0 > 0.0)) {
+      (!j->alpha && !j->eps) || (!j->E_inc && !j->waves) || !(j->d > 0.0) || !(j->k0 > 0.0)) {
     ifdda::set_error("ifdda_runner_submit: bad job");
     return IFDDA_ERR_ARG;
   }
@@ -202,8 +211,14 @@ extern "C" int ifdda_runner_submit(ifdda_runner* r, const ifdda_job* j, long lon
   Job jb;
   jb.spec = *j;
   if (j->tabdip) jb.tab.assign(j->tabdip, j->tabdip + N);
-  jb.alpha.assign(j->alpha, j->alpha + 2 * (size_t)j->nd);
-  jb.einc.assign(j->E_inc, j->E_inc + 2 * 3 * (size_t)j->nd * j->nrhs);
+  if (j->alpha)
+    jb.alpha.assign(j->alpha, j->alpha + 2 * (size_t)j->nd);
+  else
+    jb.alpha.assign(j->eps, j->eps + 2 * (size_t)j->nd);
+  if (j->E_inc)
+    jb.einc.assign(j->E_inc, j->E_inc + 2 * 3 * (size_t)j->nd * j->nrhs);
+  else
+    jb.einc.assign(j->waves, j->waves + 3 * (size_t)j->nrhs);
   jb.est = job_bytes(*j);
   jb.t_submit = now_s();
   {
@@ -211,8 +226,8 @@ extern "C" int ifdda_runner_submit(ifdda_runner* r, const ifdda_job* j, long lon
     const long long t = r->next++;
     Job& q = r->jobs.emplace(t, std::move(jb)).first->second;
     q.spec.tabdip = j->tabdip ? q.tab.data() : nullptr;  // the copies are used
-    q.spec.alpha = q.alpha.data();
-    q.spec.E_inc = q.einc.data();
+    (j->alpha ? q.spec.alpha : q.spec.eps) = q.alpha.data();
+    (j->E_inc ? q.spec.E_inc : q.spec.waves) = q.einc.data();
     r->queue.push_back(t);
     *ticket = t;
   }

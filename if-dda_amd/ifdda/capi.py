@@ -59,7 +59,14 @@ class Job(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("d", C.c_double),
                 ("k0", C.c_double), ("nquad", C.c_int), ("tolinit", C.c_double),
                 ("tabdip", C.c_void_p), ("nd", C.c_int), ("alpha", C.c_void_p),
-                ("nrhs", C.c_int), ("E_inc", C.c_void_p), ("opts", SolveOpts)]
+                ("nrhs", C.c_int), ("E_inc", C.c_void_p), ("opts", SolveOpts),
+                ("wave_kind", C.c_int), ("waves", C.c_void_p), ("E0", C.c_double * 2),
+                ("origin", C.c_double * 3), ("eps", C.c_void_p), ("pola", C.c_int),
+                ("eps0", C.c_double)]
+
+
+WAVES = {"plane": 1, "circular": 2, "kxky": 3}
+POLA = {"CM": 0, "RR": 1, "PS": 1, "GB": 2, "LA": 3, "LR": 4}
 
 
 def lib():
@@ -107,6 +114,8 @@ def lib():
     L.ifdda_runner_wait.argtypes = [vp, C.c_longlong, pd, pd, C.POINTER(SolveInfo), pd]
     L.ifdda_runner_destroy.argtypes = [vp]
     L.ifdda_runner_pack_limit.argtypes = [vp, ip]
+    L.ifdda_incident_device.argtypes = [vp, ip, ip, pd, pd, dp, dp, pd, vp, vp]
+    L.ifdda_polarizability_device.argtypes = [vp, vp, dp, dp, dp, ip]
     L.ifdda_solve_batch.argtypes = [ip, C.POINTER(Job), C.POINTER(pd), C.POINTER(pd),
                                     C.POINTER(C.POINTER(SolveInfo)), C.POINTER(C.c_int)]
     L.ifdda_microscope_incidences.argtypes = [vp, C.POINTER(MicroscopeOpts), C.POINTER(C.c_int), pd,
@@ -386,6 +395,28 @@ class Solver:
         """Darkfield microscope (microsdf.f), as microsbf."""
         return self.microsbf(m, dark=True, **kw)
 
+    def incident_device(self, waves, k0, d, origin, wave="plane", E0=1.0, out=None):
+        """R plane waves at the dipoles, formed on the device
+        (ifdda_incident_device); returns a torch complex128 device tensor
+        [R, 3 nd] viewed as float64 pairs (or fills `out`)."""
+        import torch
+        W = np.ascontiguousarray(waves, np.float64).reshape(-1, 3)
+        R = W.shape[0]
+        if out is None:
+            out = torch.empty(R * 3 * self.nd * 2, dtype=torch.float64, device="cuda")
+        e0 = (C.c_double * 2)(complex(E0).real, complex(E0).imag)
+        org = (C.c_double * 3)(*[float(v) for v in origin])
+        _check(lib().ifdda_incident_device(self.h, WAVES[wave], R, W.ctypes.data_as(C.POINTER(C.c_double)),
+                                           e0, float(k0), float(d), org, C.c_void_p(out.data_ptr()), None),
+               "ifdda_incident_device")
+        return out
+
+    def polarizability_device(self, eps, d, k0, method="RR", eps0=1.0):
+        """alpha from eps per dipole on the device (ifdda_polarizability_device)."""
+        e = _c128(eps).reshape(-1)
+        _check(lib().ifdda_polarizability_device(self.h, _p(e), float(d), float(k0), float(eps0),
+                                                 POLA[method]), "ifdda_polarizability_device")
+
     def use_specialized(self, on: bool):
         """Select compile-time (default) or generic z-pass kernels."""
         _check(lib().ifdda_use_specialized(self.h, int(bool(on))), "ifdda_use_specialized")
@@ -446,26 +477,17 @@ class Runner:
         except Exception:
             pass
 
-    def submit(self, nx, ny, nz, d, k0, alpha, E_inc, tabdip=None, method="GPBICG1", tol=1e-4,
-               precision=32, x0_zero=True, maxit=10000, nquad=0, tolinit=1e-4):
-        """E_inc: [R, 3 nd] or [3 nd] complex; returns a ticket (the arrays
-        are copied before this returns)."""
-        al = _c128(alpha).reshape(-1)
-        E = _c128(E_inc)
-        E = E.reshape(1, -1) if E.ndim == 1 else E.reshape(E.shape[0], -1)
-        j = Job()
-        j.nx, j.ny, j.nz, j.d, j.k0 = int(nx), int(ny), int(nz), float(d), float(k0)
-        j.nquad, j.tolinit, j.nd, j.nrhs = int(nquad), float(tolinit), al.size, E.shape[0]
-        tab = None
-        if tabdip is not None:
-            tab = np.ascontiguousarray(tabdip, np.int32).reshape(-1)
-            j.tabdip = tab.ctypes.data
-        j.alpha = al.ctypes.data
-        j.E_inc = E.ctypes.data
-        j.opts = Solver._opts(method, tol, precision, x0_zero, maxit, 0)
+    def submit(self, nx, ny, nz, d, k0, alpha=None, E_inc=None, tabdip=None, method="GPBICG1",
+               tol=1e-4, precision=32, x0_zero=True, maxit=10000, nquad=0, tolinit=1e-4, **kw):
+        """E_inc: [R, 3 nd] or [3 nd] complex, or waves=[R, 3] parameters
+        (wave="plane" | "circular" | "kxky", E0, origin) formed on the
+        device; alpha: nd complex, or eps=nd complex (pola="RR", eps0).
+        Returns a ticket (the arrays are copied before this returns)."""
+        j, keep, R, nd = _job(nx, ny, nz, d, k0, Solver._opts(method, tol, precision, x0_zero, maxit, 0),
+                              alpha, E_inc, tabdip, nquad, tolinit, **kw)
         t = C.c_longlong(0)
         _check(lib().ifdda_runner_submit(self.h, C.byref(j), C.byref(t)), "ifdda_runner_submit")
-        self._shape[t.value] = (E.shape[0], al.size)
+        self._shape[t.value] = (R, nd)
         return t.value
 
     def wait(self, ticket):
@@ -482,21 +504,40 @@ class Runner:
                 "queued_s": times[0], "run_s": times[1], "latency_s": times[2]}
 
 
-def _job(nx, ny, nz, d, k0, alpha, E_inc, tabdip, opts, nquad=0, tolinit=1e-4):
-    """An ifdda_job and the arrays it points at (keep them alive)."""
-    al = _c128(alpha).reshape(-1)
-    E = _c128(E_inc)
-    E = E.reshape(1, -1) if E.ndim == 1 else E.reshape(E.shape[0], -1)
+def _job(nx, ny, nz, d, k0, opts, alpha=None, E_inc=None, tabdip=None, nquad=0, tolinit=1e-4,
+         eps=None, pola="RR", eps0=1.0, waves=None, wave="plane", E0=1.0, origin=(0.0, 0.0, 0.0)):
+    """An ifdda_job and the arrays it points at (keep them alive): alpha or
+    eps (+ pola, eps0); E_inc or waves (+ wave, E0, origin)."""
     j = Job()
     j.nx, j.ny, j.nz, j.d, j.k0 = int(nx), int(ny), int(nz), float(d), float(k0)
-    j.nquad, j.tolinit, j.nd, j.nrhs = int(nquad), float(tolinit), al.size, E.shape[0]
-    keep = [al, E]
+    j.nquad, j.tolinit = int(nquad), float(tolinit)
+    keep = []
+    if alpha is not None:
+        al = _c128(alpha).reshape(-1)
+        j.alpha = al.ctypes.data
+    else:
+        al = _c128(eps).reshape(-1)
+        j.eps = al.ctypes.data
+        j.pola, j.eps0 = POLA[pola], float(eps0)
+    keep.append(al)
+    if E_inc is not None:
+        E = _c128(E_inc)
+        E = E.reshape(1, -1) if E.ndim == 1 else E.reshape(E.shape[0], -1)
+        j.E_inc = E.ctypes.data
+    else:
+        E = np.ascontiguousarray(waves, np.float64).reshape(-1, 3)
+        j.waves = E.ctypes.data
+        j.wave_kind = WAVES[wave]
+        E0 = complex(E0)
+        j.E0[0], j.E0[1] = E0.real, E0.imag
+        for a in range(3):
+            j.origin[a] = float(origin[a])
+    keep.append(E)
+    j.nd, j.nrhs = al.size, E.shape[0]
     if tabdip is not None:
         tab = np.ascontiguousarray(tabdip, np.int32).reshape(-1)
         j.tabdip = tab.ctypes.data
         keep.append(tab)
-    j.alpha = al.ctypes.data
-    j.E_inc = E.ctypes.data
     j.opts = opts
     return j, keep, E.shape[0], al.size
 
@@ -506,13 +547,15 @@ def solve_batch(problems, method="GPBICG1", tol=1e-4, precision=32, x0_zero=True
     """Independent problems solved together (ifdda_solve_batch): every
     right-hand side of every problem shares the matvec launches and one
     lockstep Krylov solve.  problems: iterable of dicts with keys nx, ny, nz,
-    d, k0, alpha, E_inc ([R, 3 nd] or [3 nd]) and optionally tabdip, nquad.
+    d, k0, alpha (or eps, pola, eps0), E_inc ([R, 3 nd] or [3 nd]; or waves
+    [R, 3], wave, E0, origin) and optionally tabdip, nquad.
     Returns one dict per problem: FFloc, FF ([R, 3 nd]) and infos."""
     opts = Solver._opts(method, tol, precision, x0_zero, maxit, check_every)
     jobs, keep, outs = [], [], []
     for p in problems:
-        j, k, R, nd = _job(p["nx"], p["ny"], p["nz"], p["d"], p["k0"], p["alpha"], p["E_inc"],
-                           p.get("tabdip"), opts, p.get("nquad", 0))
+        extra = {k: p[k] for k in ("eps", "pola", "eps0", "waves", "wave", "E0", "origin") if k in p}
+        j, k, R, nd = _job(p["nx"], p["ny"], p["nz"], p["d"], p["k0"], opts, p.get("alpha"),
+                           p.get("E_inc"), p.get("tabdip"), p.get("nquad", 0), **extra)
         jobs.append(j)
         keep.append(k)
         outs.append({"FFloc": np.zeros((R, 3 * nd), np.complex128),

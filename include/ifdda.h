@@ -356,7 +356,8 @@ int ifdda_microsdf_incidences(ifdda_ctx *ctx, const ifdda_microscope_opts *m, in
  * whose callers loop over objects, wavelengths and requests one cdmlib call
  * at a time).  A pool of worker threads, each with its own stream, takes
  * jobs in submission order: a free worker takes the head job and every
- * queued job with the same solve options that fits the pack (at most
+ * queued job with the same solve options and a box within a factor 2 in
+ * cells that fits the pack (at most
  * max_members right-hand sides, 64 by default) and the device budget
  * mem_budget (bytes) next to the packs already running (a job larger than
  * the budget runs alone), and solves the group as one pack (as
@@ -373,7 +374,43 @@ typedef struct {
   int nrhs;                /* incidences                                      */
   const double *E_inc;     /* nrhs * 3 nd complex, RHS r at r * 3 nd          */
   ifdda_solve_opts opts;
+  /* Optional, when E_inc is NULL: the nrhs incident plane waves formed on
+   * the device (as ifdda_incident_device) */
+  int wave_kind;           /* IFDDA_WAVE_*                                    */
+  const double *waves;     /* nrhs * 3 parameters of that kind                */
+  double E0[2];            /* amplitude (re, im)                              */
+  double origin[3];        /* centre of box cell (1,1,1) [m]                  */
+  /* Optional, when alpha is NULL: the polarisability from the permittivity
+   * (as ifdda_polarizability_device) */
+  const double *eps;       /* nd complex                                      */
+  int pola;                /* IFDDA_POLA_*                                    */
+  double eps0;             /* background permittivity (0 means 1)             */
 } ifdda_job;
+
+/* ---------------------------------------------------------------------------
+ * Incident waves and polarisabilities on the device (the step before the
+ * solve; the reference forms them per dipole in host loops).
+ * ------------------------------------------------------------------------- */
+enum {
+  IFDDA_WAVE_PLANE = 1,    /* ondeplane.f: params (theta, phi [deg], pp), ss = 1 - pp  */
+  IFDDA_WAVE_CIRCULAR = 2, /* ondecirce.f: params (theta, phi [deg], helicity sign)     */
+  IFDDA_WAVE_KXKY = 3      /* ondeplanekxky.f: params (kx, ky [1/m], pp)              */
+};
+enum { IFDDA_POLA_CM = 0, IFDDA_POLA_RR = 1, IFDDA_POLA_GB = 2, IFDDA_POLA_LA = 3, IFDDA_POLA_LR = 4 };
+
+/* R incident plane waves of amplitude E0 (re, im) at the context's dipoles,
+ * whose cell (i, j, k) sits at origin + d (i, j, k) (origin: centre of cell
+ * (1,1,1)): E (device complex128, R * 3 nd, RHS r at r * 3 nd, the layout of
+ * ifdda_solve_multi_device); params R * 3 as the kind says.  Replaces the
+ * loops over dipoles around ondeplane / ondecirce / ondeplanekxky
+ * (cdmlib.f incident-field set-up, microsbf.f:226-230). */
+int ifdda_incident_device(ifdda_ctx *ctx, int kind, int R, const double *params, const double E0[2],
+                          double k0, double d, const double origin[3], void *E, void *stream);
+/* The context's polarisability from the permittivity of each dipole's cell
+ * (eps: host, nd complex): poladiff (polarisabilite.f:1-62, dddis = 1, cell
+ * of side d) with method CM, RR (= PS), GB, LA or LR, on the device. */
+int ifdda_polarizability_device(ifdda_ctx *ctx, const double *eps, double d, double k0, double eps0,
+                                int method);
 
 /* Independent problems solved together (no counterpart in the reference):
  * every right-hand side of every job shares the matvec launches (packed

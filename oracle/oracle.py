@@ -322,6 +322,31 @@ class Ref:
             out[n] = [complex(ex[0], ex[1]), complex(ey[0], ey[1]), complex(ez[0], ez[1])]
         return out.reshape(-1)
 
+    def ondeplane(self, pos, k0, E0, pp, theta_deg, phi_deg):
+        """ondeplane_ (ondeplane.f:1-50) at each position; ss = 1 - pp."""
+        out = np.zeros((len(pos), 3), np.complex128)
+        e0 = (C.c_double * 2)(E0.real, E0.imag)
+        nstop = C.c_int(0)
+        info = C.create_string_buffer(b" " * 64, 64)
+        for n, (x, y, z) in enumerate(pos):
+            ex, ey, ez = ((C.c_double * 2)() for _ in range(3))
+            self.lib.ondeplane_(_rd(x), _rd(y), _rd(z), _rd(k0), C.byref(e0), _rd(1.0 - pp), _rd(pp),
+                                _rd(theta_deg), _rd(phi_deg), C.byref(ex), C.byref(ey), C.byref(ez),
+                                C.byref(nstop), info, C.c_size_t(64))
+            out[n] = [complex(ex[0], ex[1]), complex(ey[0], ey[1]), complex(ez[0], ez[1])]
+        return out.reshape(-1)
+
+    def ondecirce(self, pos, k0, E0, helicity, theta_deg, phi_deg):
+        """ondecirce_ (ondecirce.f:1-55) at each position."""
+        out = np.zeros((len(pos), 3), np.complex128)
+        e0 = (C.c_double * 2)(E0.real, E0.imag)
+        for n, (x, y, z) in enumerate(pos):
+            ex, ey, ez = ((C.c_double * 2)() for _ in range(3))
+            self.lib.ondecirce_(_rd(x), _rd(y), _rd(z), _rd(k0), C.byref(e0), _rd(helicity),
+                                _rd(theta_deg), _rd(phi_deg), C.byref(ex), C.byref(ey), C.byref(ez))
+            out[n] = [complex(ex[0], ex[1]), complex(ey[0], ey[1]), complex(ez[0], ez[1])]
+        return out.reshape(-1)
+
     def deltakroutine(self, kxinc, kyinc, dkx, dky, numaper):
         i, j = C.c_int(0), C.c_int(0)
         self.lib.deltakroutine_(_rd(kxinc), _rd(kyinc), _rd(dkx), _rd(dky), _rd(numaper), C.byref(i),

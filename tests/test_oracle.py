@@ -333,3 +333,26 @@ def test_farfield_config1_physics():
     assert abs(cs / Cext - 1) < 0.03
     _, _, gmie = bhmie(k0 * 200e-9, np.sqrt(2.25 + 0j), 200)
     assert abs(gg / gmie - 1) < 0.02
+
+
+def test_inputs_match_compiled_reference(ref):
+    """The restatements that form every solve's inputs here -- scene.plane_wave
+    (ondeplane.f), scene.circular_wave (ondecirce.f), scene.polarizability
+    (poladiff, polarisabilite.f) -- against the compiled reference."""
+    from ifdda import scene
+    k0 = 2 * np.pi / 632.8e-9
+    pos = np.random.default_rng(3).uniform(-1e-6, 1e-6, (6, 3))
+    E0 = 0.7 - 0.2j
+    for th, ph, pp in ((30.0, 10.0, 0.5), (-20.0, 200.0, 0.0), (0.0, 45.0, 1.0), (75.0, -30.0, 0.3)):
+        a = scene.plane_wave(pos, k0, th, ph, pp=pp, E0=E0)
+        b = ref.ondeplane(pos, k0, E0, pp, th, ph)
+        assert np.abs(a - b).max() < 1e-14 * np.abs(b).max(), (th, ph, pp)
+    for th, ph, h in ((30.0, 10.0, 1.0), (-20.0, 200.0, -1.0), (0.0, 45.0, 1.0)):
+        a = scene.circular_wave(pos, k0, th, ph, helicity=h, E0=E0)
+        b = ref.ondecirce(pos, k0, E0, h, th, ph)
+        assert np.abs(a - b).max() < 1e-14 * np.abs(b).max(), (th, ph, h)
+    for method in ("CM", "RR"):
+        for eps in (2.25 + 0j, 4.0 - 0.5j, 1.5 + 0.3j):
+            a = scene.polarizability(np.array([eps]), 20e-9, k0, method)[0]
+            b = ref.poladiff(20e-9, eps, k0, method)
+            assert abs(a - b) <= 1e-15 * abs(b), (method, eps)
