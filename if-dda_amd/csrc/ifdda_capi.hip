@@ -700,13 +700,27 @@ static void fill_info(char* infostr, size_t len, const char* msg) {
   if (len > n) std::memset(infostr + n, ' ', len - n);
 }
 
+// inverserig.f:44-1076 dispatches on methodeit(1:k) .eq. 'NAME' in this
+// order (Fortran compares blank-padded to k characters; the first match
+// wins, so e.g. 'CG' must be followed by blanks).  Methods without a device
+// implementation return 0 ('Iterative method not correct!').
 static int method_from_name(const char* m, size_t len) {
   std::string s(m, len);
-  auto starts = [&](const char* p) { return s.compare(0, std::strlen(p), p) == 0; };
-  if (starts("GPBICG1")) return IFDDA_GPBICG1;
-  if (starts("QMRBICGSTAB1")) return IFDDA_QMRBICGSTAB1;
-  if (starts("QMRBICGSTAB2")) return IFDDA_QMRBICGSTAB2;
-  if (starts("TFQMR")) return IFDDA_TFQMR;
+  auto eq = [&](size_t k, const char* name) {
+    std::string a = s.substr(0, std::min(k, s.size())), b(name);
+    a.resize(k, ' ');
+    b.resize(k, ' ');
+    return a == b;
+  };
+  if (eq(7, "GPBICG1")) return IFDDA_GPBICG1;
+  if (eq(7, "GPBICG2") || eq(10, "GPBICGsafe") || eq(10, "GPBICGplus") || eq(8, "GPBICGAR") ||
+      eq(12, "BICGSTARPLUS") || eq(6, "QMRCLA"))
+    return 0;
+  if (eq(5, "TFQMR")) return IFDDA_TFQMR;
+  if (eq(5, "CG")) return IFDDA_CG;
+  if (eq(8, "BICGSTAB")) return IFDDA_BICGSTAB;
+  if (eq(12, "QMRBICGSTAB1")) return IFDDA_QMRBICGSTAB1;
+  if (eq(12, "QMRBICGSTAB2")) return IFDDA_QMRBICGSTAB2;
   return 0;
 }
 

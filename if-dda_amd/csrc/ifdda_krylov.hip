@@ -37,6 +37,8 @@ struct KState {
   double tol;
   double2 alpha, beta, eta, dzeta, r0rn;    // GPBICG (eta/alpha shared by all)
   double2 rho, omega, tau, taut, theta, thetat, etat;  // QMRBICGSTAB / TFQMR
+  double2 rho0, xxi;  // CG (zcg.f), BICGSTAB (pimzbicgstab.f)
+  double rr;          // BICGSTAB: sum conj(r) r of the latest residual
   double2 dots[kMaxDots];                   // last finished reduction
   int it;          // ITNO of the iteration in progress
   int done;        // solve ended (converged, breakdown or iteration cap)
@@ -1009,6 +1011,444 @@ __global__ void t_f5(KState* st) {
 
 // ===================================================== driver / refinement
 // res = Ax - b, partial |res|^2 ; FF = alpha x (inverserig.f:95-105)
+// ================================================================ CG kernels
+// zcg.f: conjugate gradients for the complex-symmetric A = I - T alpha with
+// bilinear products (r.r, p.w, r.s are not conjugated); the stopping test
+// uses |r| / |b|.  Work columns: x (WRK1), r (WRK2), p (WRK3), s (WRK4),
+// w (WRK5).
+template <typename V> struct CV {
+  V *x, *b, *r, *p, *w, *s, *Ax;
+  __device__ CV shifted(long long o) const { return CV{x + o, b + o, r + o, p + o, w + o, s + o, Ax + o}; }
+};
+
+// r = b - Ax, p = r; partial |b|^2, r.r (zcg.f:53-74)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) c_start(CV<V> v, int x0zero, VN n, double2* part,
+                                                   int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  v = v.shifted(n.off(rh_));
+  part += (long long)rh_ * kMaxDots * nblk;
+  double2 acc[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+  RHS_BLOCKS(n, rh_, 2);
+  GRID_LOOP(i, n.len(rh_)) {
+    const double2 b = LD(v.b, i);
+    double2 r = b;
+    if (!x0zero) r = zsub(r, LD(v.Ax, i));
+    ST(v.r, i, r);
+    ST(v.p, i, r);
+    const double2 rs = Vt<V>::rnd(r);
+    acc[0].x += b.x * b.x + b.y * b.y;
+    acc[1] = zadd(acc[1], zmul(rs, rs));
+  }
+  block_partials<2>(acc, part, nblk);
+}
+
+__global__ void __launch_bounds__(kThreads) c_fstart(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
+  finish_sum<2>(st, part, nblk);
+  if (threadIdx.x == 0) {
+    st->norm = sqrt(st->dots[0].x);
+    st->rho0 = st->dots[1];
+    st->it = 0;
+  }
+}
+
+// partial p.w (zcg.f:78-84)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) c_k1(const KState* st, CV<V> v, VN n, double2* part,
+                                                int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(n.off(rh_));
+  part += (long long)rh_ * kMaxDots * nblk;
+  if (st->done) return;
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  RHS_BLOCKS(n, rh_, 1);
+  GRID_LOOP(i, n.len(rh_)) acc[0] = zadd(acc[0], zmul(LD(v.p, i), LD(v.w, i)));
+  block_partials<1>(acc, part, nblk);
+}
+
+// xi = p.w (zcg.f:85-88; the reference stops the program on xi = 0), then
+// the first loop head: nloop = 1, the iteration cap, alpha = rho0 / xi
+// (zcg.f:92-107)
+__global__ void __launch_bounds__(kThreads) c_f1(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
+  if (st->done) return;
+  finish_sum<1>(st, part, nblk);
+  if (threadIdx.x == 0) {
+    st->xxi = st->dots[0];
+    if (zzero(st->xxi)) {
+      st->status = -1;
+      st->steperr = 1;
+      st->done = 1;
+      return;
+    }
+    st->it = 1;
+    if (st->it >= st->maxit) {  // NSTAT = -4
+      st->status = -1;
+      st->steperr = 0;
+      st->done = 1;
+      return;
+    }
+    st->alpha = zdiv(st->rho0, st->xxi);
+  }
+}
+
+// (after the first iteration) p = r + beta p, w = s + beta w (zcg.f:159-165);
+// x += alpha p, r -= alpha w; partial |r|^2 (zcg.f:110-120)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) c_k2(const KState* st, CV<V> v, VN n, double2* part,
+                                                int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(n.off(rh_));
+  part += (long long)rh_ * kMaxDots * nblk;
+  if (st->done) return;
+  const double2 al = st->alpha, be = st->beta;
+  const bool first = st->it == 1;
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  RHS_BLOCKS(n, rh_, 1);
+  GRID_LOOP(i, n.len(rh_)) {
+    double2 p = LD(v.p, i), w = LD(v.w, i);
+    if (!first) {
+      p = Vt<V>::rnd(zadd(LD(v.r, i), zmul(be, p)));
+      w = Vt<V>::rnd(zadd(LD(v.s, i), zmul(be, w)));
+      ST(v.p, i, p);
+      ST(v.w, i, w);
+    }
+    ST(v.x, i, zadd(LD(v.x, i), zmul(al, p)));
+    const double2 r = zsub(LD(v.r, i), zmul(al, w));
+    ST(v.r, i, r);
+    const double2 rs = Vt<V>::rnd(r);
+    acc[0].x += rs.x * rs.x + rs.y * rs.y;
+  }
+  block_partials<1>(acc, part, nblk);
+}
+
+// tole = |r| / |b|; converged when tole <= tol (zcg.f:121-138)
+__global__ void __launch_bounds__(kThreads) c_f2(KState* st, const double2* part, int nblk,
+                                                double2* trace) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
+  if (st->done) return;
+  finish_sum<1>(st, part, nblk);
+  if (threadIdx.x != 0) return;
+  const double res = sqrt(fabs(st->dots[0].x)) / st->norm;
+  st->residu = res;
+  push_trace(st, trace, st->alpha, st->beta, st->xxi, st->rho0, make_double2(0.0, 0.0), res);
+  if (res <= st->tol) {
+    st->status = 1;
+    st->done = 1;
+  }
+}
+
+// partial r.r, r.s (zcg.f:150-156)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) c_k3(const KState* st, CV<V> v, VN n, double2* part,
+                                                int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(n.off(rh_));
+  part += (long long)rh_ * kMaxDots * nblk;
+  if (st->done) return;
+  double2 acc[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+  RHS_BLOCKS(n, rh_, 2);
+  GRID_LOOP(i, n.len(rh_)) {
+    const double2 r = LD(v.r, i);
+    acc[0] = zadd(acc[0], zmul(r, r));
+    acc[1] = zadd(acc[1], zmul(r, LD(v.s, i)));
+  }
+  block_partials<2>(acc, part, nblk);
+}
+
+// beta = rho / rho0, xi = delta - beta^2 xi (zcg.f:158-169), then the loop
+// head: nloop += 1, the iteration cap, alpha = rho0 / xi (zcg.f:92-107)
+__global__ void __launch_bounds__(kThreads) c_f3(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
+  if (st->done) return;
+  finish_sum<2>(st, part, nblk);
+  if (threadIdx.x == 0) {
+    const double2 rho = st->dots[0], delta = st->dots[1];
+    const double2 be = zdiv(rho, st->rho0);
+    st->beta = be;
+    st->rho0 = rho;
+    st->xxi = zsub(delta, zmul(zmul(be, be), st->xxi));
+    st->it++;
+    if (st->it >= st->maxit) {  // NSTAT = -4
+      st->status = -1;
+      st->steperr = 0;
+      st->done = 1;
+      return;
+    }
+    if (zzero(st->xxi)) {  // (no test in the reference: a division by zero there)
+      st->status = -1;
+      st->steperr = 1;
+      st->done = 1;
+      return;
+    }
+    st->alpha = zdiv(st->rho0, st->xxi);
+  }
+}
+
+// ========================================================== BICGSTAB kernels
+// pimzbicgstab.f, as the reference runs it: rho = sum conj(r) r of the
+// current residual (pimzbicgstab.f:71-74; the shadow residual only enters
+// xi), stopping test |r|^2 / |r0|^2 <= tol / 10 (pimzbicgstab.f:168-181).
+// Work columns: x, r (1), rt (2), p (3), s (5), t (6), v (7).
+template <typename V> struct BV {
+  V *x, *b, *r, *rt, *p, *v, *s, *t, *Ax;
+  __device__ BV shifted(long long o) const {
+    return BV{x + o, b + o, r + o, rt + o, p + o, v + o, s + o, t + o, Ax + o};
+  }
+};
+
+// r = b - Ax, rt = r, p = v = 0; partial |r|^2 (pimzbicgstab.f:48-58)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) s_start(BV<V> v, int x0zero, VN n, double2* part,
+                                                   int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  v = v.shifted(n.off(rh_));
+  part += (long long)rh_ * kMaxDots * nblk;
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  const double2 z0 = make_double2(0.0, 0.0);
+  RHS_BLOCKS(n, rh_, 1);
+  GRID_LOOP(i, n.len(rh_)) {
+    double2 r = LD(v.b, i);
+    if (!x0zero) r = zsub(r, LD(v.Ax, i));
+    ST(v.r, i, r);
+    ST(v.rt, i, r);
+    ST(v.p, i, z0);
+    ST(v.v, i, z0);
+    const double2 rs = Vt<V>::rnd(r);
+    acc[0].x += rs.x * rs.x + rs.y * rs.y;
+  }
+  block_partials<1>(acc, part, nblk);
+}
+
+// The head of iteration it + 1 (pimzbicgstab.f:69-89): rho0 = rho, rho =
+// conj(r).r (the latest residual's, already summed), kappa = rho0 omega,
+// beta = rho alpha / kappa.
+__device__ __forceinline__ void s_head(KState* st) {
+  st->it++;
+  const double2 rho0 = st->rho;
+  st->rho0 = rho0;
+  st->rho = make_double2(st->rr, 0.0);
+  const double2 kappa = zmul(rho0, st->omega);
+  if (zabs(kappa) == 0.0) {
+    st->status = -1;
+    st->steperr = 6;
+    st->done = 1;
+    return;
+  }
+  st->beta = zdiv(zmul(st->rho, st->alpha), kappa);
+}
+
+__global__ void __launch_bounds__(kThreads) s_fstart(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
+  finish_sum<1>(st, part, nblk);
+  if (threadIdx.x == 0) {
+    st->norm = st->dots[0].x;  // the squared norm, as the reference keeps NORM
+    st->rr = st->dots[0].x;
+    st->rho = st->alpha = st->omega = make_double2(1.0, 0.0);
+    st->it = 0;
+    s_head(st);
+  }
+}
+
+// p = r + beta (p - omega v) (pimzbicgstab.f:91-100)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) s_k1(const KState* st, BV<V> v, VN n) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(n.off(rh_));
+  if (st->done) return;
+  const double2 be = st->beta, om = st->omega;
+  RHS_BLOCKS_NOSUM(n, rh_);
+  GRID_LOOP(i, n.len(rh_)) {
+    const double2 q = Vt<V>::rnd(zsub(LD(v.p, i), zmul(om, LD(v.v, i))));
+    ST(v.p, i, zadd(LD(v.r, i), zmul(be, q)));
+  }
+}
+
+// partial conj(rt).v (pimzbicgstab.f:104-108)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) s_k2(const KState* st, BV<V> v, VN n, double2* part,
+                                                int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(n.off(rh_));
+  part += (long long)rh_ * kMaxDots * nblk;
+  if (st->done) return;
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  RHS_BLOCKS(n, rh_, 1);
+  GRID_LOOP(i, n.len(rh_)) acc[0] = zadd(acc[0], zmul(zconj(LD(v.rt, i)), LD(v.v, i)));
+  block_partials<1>(acc, part, nblk);
+}
+
+// alpha = rho / xi (pimzbicgstab.f:111-117)
+__global__ void __launch_bounds__(kThreads) s_f2(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
+  if (st->done) return;
+  finish_sum<1>(st, part, nblk);
+  if (threadIdx.x == 0) {
+    st->xxi = st->dots[0];
+    if (zabs(st->xxi) == 0.0) {
+      st->status = -1;
+      st->steperr = 10;
+      st->done = 1;
+      return;
+    }
+    st->alpha = zdiv(st->rho, st->xxi);
+  }
+}
+
+// s = r - alpha v; partial conj(s).s (pimzbicgstab.f:118-124)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) s_k3(const KState* st, BV<V> v, VN n, double2* part,
+                                                int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(n.off(rh_));
+  part += (long long)rh_ * kMaxDots * nblk;
+  if (st->done) return;
+  const double2 al = st->alpha;
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  RHS_BLOCKS(n, rh_, 1);
+  GRID_LOOP(i, n.len(rh_)) {
+    const double2 sv = zsub(LD(v.r, i), zmul(al, LD(v.v, i)));
+    ST(v.s, i, sv);
+    const double2 ss = Vt<V>::rnd(sv);
+    acc[0] = zadd(acc[0], zmul(zconj(ss), ss));
+  }
+  block_partials<1>(acc, part, nblk);
+}
+
+// |s| < 1e-15: soft breakdown (pimzbicgstab.f:125-134)
+__global__ void __launch_bounds__(kThreads) s_f3(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
+  if (st->done) return;
+  finish_sum<1>(st, part, nblk);
+  if (threadIdx.x == 0 && zabs(zsqrt(st->dots[0])) < 1.0e-15) {
+    st->status = -1;
+    st->steperr = 12;
+    st->done = 1;
+  }
+}
+
+// partial conj(t).t, conj(t).s (pimzbicgstab.f:144-148)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) s_k4(const KState* st, BV<V> v, VN n, double2* part,
+                                                int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(n.off(rh_));
+  part += (long long)rh_ * kMaxDots * nblk;
+  if (st->done) return;
+  double2 acc[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+  RHS_BLOCKS(n, rh_, 2);
+  GRID_LOOP(i, n.len(rh_)) {
+    const double2 ct = zconj(LD(v.t, i));
+    acc[0] = zadd(acc[0], zmul(ct, LD(v.t, i)));
+    acc[1] = zadd(acc[1], zmul(ct, LD(v.s, i)));
+  }
+  block_partials<2>(acc, part, nblk);
+}
+
+// omega = conj(t).s / conj(t).t (pimzbicgstab.f:150-156)
+__global__ void __launch_bounds__(kThreads) s_f4(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
+  if (st->done) return;
+  finish_sum<2>(st, part, nblk);
+  if (threadIdx.x == 0) {
+    if (zabs(st->dots[0]) == 0.0) {
+      st->status = -1;
+      st->steperr = 14;
+      st->done = 1;
+      return;
+    }
+    st->omega = zdiv(st->dots[1], st->dots[0]);
+  }
+}
+
+// x += alpha p + omega s, r = s - omega t; partial conj(r).r
+// (pimzbicgstab.f:158-166)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) s_k5(const KState* st, BV<V> v, VN n, double2* part,
+                                                int nblk) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(n.off(rh_));
+  part += (long long)rh_ * kMaxDots * nblk;
+  if (st->done) return;
+  const double2 al = st->alpha, om = st->omega;
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  RHS_BLOCKS(n, rh_, 1);
+  GRID_LOOP(i, n.len(rh_)) {
+    const double2 sv = LD(v.s, i);
+    ST(v.x, i, zadd(zadd(LD(v.x, i), zmul(al, LD(v.p, i))), zmul(om, sv)));
+    const double2 r = zsub(sv, zmul(om, LD(v.t, i)));
+    ST(v.r, i, r);
+    const double2 rs = Vt<V>::rnd(r);
+    acc[0].x += rs.x * rs.x + rs.y * rs.y;
+  }
+  block_partials<1>(acc, part, nblk);
+}
+
+// |r|^2 / |r0|^2 against tol / 10, the iteration cap (pimzbicgstab.f:167-181),
+// then the next iteration's head
+__global__ void __launch_bounds__(kThreads) s_f5(KState* st, const double2* part, int nblk,
+                                                double2* trace) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
+  if (st->done) return;
+  finish_sum<1>(st, part, nblk);
+  if (threadIdx.x != 0) return;
+  st->rr = st->dots[0].x;
+  const double c1 = st->rr / st->norm;
+  st->residu = c1;
+  push_trace(st, trace, st->alpha, st->beta, st->omega, st->rho, st->xxi, c1);
+  if (st->it > st->maxit) {  // STATUS = -1 (the reference then reports ITNO = MAXIT)
+    st->status = -1;
+    st->steperr = 0;
+    st->done = 1;
+    return;
+  }
+  if (fabs(c1) <= st->tol / 10.0) {
+    st->status = 1;
+    st->done = 1;
+    return;
+  }
+  s_head(st);
+}
+
+// x = p for the right-hand sides that ended in a soft breakdown
+// (pimzbicgstab.f:126-133)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) s_soft(const KState* st, BV<V> v, VN n) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  v = v.shifted(n.off(rh_));
+  if (!(st->status == -1 && st->steperr == 12)) return;
+  RHS_BLOCKS_NOSUM(n, rh_);
+  GRID_LOOP(i, n.len(rh_)) ST(v.x, i, LD(v.p, i));
+}
+
 template <typename V>
 __global__ void __launch_bounds__(kThreads) k_resid(const V* Ax, const V* b, VN n,
                                                    double2* part, int nblk) {
@@ -1504,6 +1944,71 @@ static bool krylov(const MvOp& op, Solver* S, V** vv, int R, int method, const d
       o.nloop = h.it + 1;        // ITNO starts at -1 (tfqmr.f:96), the driver adds one
       o.nmv = nmv0 + 1 + h.it;   // A u0, then one matvec per half step that did not converge
     }
+  } else if (method == IFDDA_CG) {
+    CV<V> v{x, b, vv[2], vv[3], vv[4], vv[5], Ax};
+    hipLaunchKernelGGL(c_start<V>, G, T, 0, s, v, x0zero, vn, S->part, nb);
+    hipLaunchKernelGGL(c_fstart, F, T, 0, s, S->st, (const double2*)S->part, nb);
+    if (!mv<V>(op, v.p, v.p, v.w, s, done, R, n, dstride)) return false;
+    hipLaunchKernelGGL(c_k1<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
+    hipLaunchKernelGGL(c_f1, F, T, 0, s, S->st, (const double2*)S->part, nb);
+    for (int it = 0;; ++it) {
+      hipLaunchKernelGGL(c_k2<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
+      hipLaunchKernelGGL(c_f2, F, T, 0, s, S->st, (const double2*)S->part, nb, trace);
+      if (!mv<V>(op, v.r, v.r, v.s, s, done, R, n, dstride)) return false;
+      hipLaunchKernelGGL(c_k3<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
+      hipLaunchKernelGGL(c_f3, F, T, 0, s, S->st, (const double2*)S->part, nb);
+      if (!hip_ok(hipGetLastError(), "cg launch")) return false;
+      const int st = step_poll(it);
+      if (st < 0) return false;
+      if (st > 0) break;
+    }
+    for (int r = 0; r < R; ++r) {
+      const KState& h = S->st_host[r];
+      KRun& o = out[r];
+      o.status = h.status;
+      o.steperr = h.steperr;
+      if (active && !active[r]) continue;
+      o.nloop = h.it;  // NLOOP (zcg.f:92)
+      // A p0, then one A r per iteration that went on (zcg.f:140-148)
+      o.nmv = nmv0 + std::max(h.it, 1);
+    }
+  } else if (method == IFDDA_BICGSTAB) {
+    BV<V> v{x, b, vv[2], vv[3], vv[4], vv[5], vv[6], vv[7], Ax};
+    hipLaunchKernelGGL(s_start<V>, G, T, 0, s, v, x0zero, vn, S->part, nb);
+    hipLaunchKernelGGL(s_fstart, F, T, 0, s, S->st, (const double2*)S->part, nb);
+    for (int it = 0;; ++it) {
+      hipLaunchKernelGGL(s_k1<V>, G, T, 0, s, (const KState*)S->st, v, vn);
+      if (!mv<V>(op, v.p, v.p, v.v, s, done, R, n, dstride)) return false;
+      hipLaunchKernelGGL(s_k2<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
+      hipLaunchKernelGGL(s_f2, F, T, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(s_k3<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
+      hipLaunchKernelGGL(s_f3, F, T, 0, s, S->st, (const double2*)S->part, nb);
+      if (!mv<V>(op, v.s, v.s, v.t, s, done, R, n, dstride)) return false;
+      hipLaunchKernelGGL(s_k4<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
+      hipLaunchKernelGGL(s_f4, F, T, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(s_k5<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
+      hipLaunchKernelGGL(s_f5, F, T, 0, s, S->st, (const double2*)S->part, nb, trace);
+      if (!hip_ok(hipGetLastError(), "bicgstab launch")) return false;
+      const int st = step_poll(it);
+      if (st < 0) return false;
+      if (st > 0) break;
+    }
+    hipLaunchKernelGGL(s_soft<V>, G, T, 0, s, (const KState*)S->st, v, vn);
+    for (int r = 0; r < R; ++r) {
+      const KState& h = S->st_host[r];
+      KRun& o = out[r];
+      o.status = h.status;
+      o.steperr = h.steperr;
+      if (active && !active[r]) continue;
+      const int k = h.it;  // ITNO (pimzbicgstab.f:69)
+      o.nloop = std::min(k, maxit);
+      // A p and A s per iteration; a breakdown ends after the first (10,
+      // 12) or before either (6) of them (pimzbicgstab.f:79-134)
+      int nmv = 2 * k;
+      if (h.status == -1 && (h.steperr == 10 || h.steperr == 12)) nmv = 2 * k - 1;
+      if (h.status == -1 && h.steperr == 6) nmv = 2 * (k - 1);
+      o.nmv = nmv0 + nmv;
+    }
   } else {
     set_error("unknown method");
     return false;
@@ -1535,7 +2040,7 @@ static int solve_dev(ifdda_ctx* c, const ifdda_solve_opts* o, int R, const doubl
     std::memset(&infos[r], 0, sizeof(ifdda_solve_info));
     set_info_str(&infos[r], "");
   }
-  if (o->method < IFDDA_GPBICG1 || o->method > IFDDA_TFQMR) {
+  if (o->method < IFDDA_GPBICG1 || o->method > IFDDA_BICGSTAB) {
     set_error("ifdda_solve: unknown method");
     return IFDDA_ERR_ARG;
   }
@@ -1710,6 +2215,10 @@ int solve_core(const MvOp& op, Solver* S, const ifdda_solve_opts* o, int R, cons
     inf.outer = o->precision == 32 ? outer[r] : 0;
     inf.seconds = secs;
     inf.tol1 = bn[r] > 0.0 ? e[r] / bn[r] : 0.0;
+    // inverserig.f:796-808 divides BICGSTAB's residual by the NORM the solver
+    // returns, the squared initial residual norm (pimzbicgstab.f:50-53)
+    if (o->method == IFDDA_BICGSTAB && o->precision == 64 && S->st_host[r].norm > 0.0)
+      inf.tol1 = e[r] / S->st_host[r].norm;
     if (run[r].status < 0) {  // inverserig.f:47-52
       inf.nstop = 1;
       inf.ncompte = ncompte[r];

@@ -20,7 +20,7 @@ lib_path = os.path.join(PKG, "libifdda_hip.so")
 
 GPBICG1, QMRBICGSTAB1, QMRBICGSTAB2, TFQMR = 1, 2, 3, 4
 METHODS = {"GPBICG1": GPBICG1, "QMRBICGSTAB1": QMRBICGSTAB1, "QMRBICGSTAB2": QMRBICGSTAB2,
-           "TFQMR": TFQMR}
+           "TFQMR": TFQMR, "CG": 5, "BICGSTAB": 6}
 
 
 class IfddaError(RuntimeError):

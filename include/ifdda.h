@@ -36,6 +36,8 @@ extern "C" {
 #define IFDDA_QMRBICGSTAB1 2
 #define IFDDA_QMRBICGSTAB2 3
 #define IFDDA_TFQMR 4
+#define IFDDA_CG 5        /* zcg.f: conjugate gradients, bilinear products   */
+#define IFDDA_BICGSTAB 6  /* pimzbicgstab.f                                 */
 
 typedef struct ifdda_ctx ifdda_ctx;
 
@@ -92,7 +94,7 @@ int ifdda_matvec_device(ifdda_ctx *ctx, int prec, int apply_alpha, const void *x
 
 /* Solve options (the reference's methodeit / tol / nlim, plus precision). */
 typedef struct {
-  int method;      /* IFDDA_GPBICG1 ... IFDDA_TFQMR                        */
+  int method;      /* IFDDA_GPBICG1 ... IFDDA_BICGSTAB                     */
   int precision;   /* 64: plain FP64; 32: FP32 matvec + FP64 refinement    */
   int x0_zero;     /* 0: x0 = E_inc as inverserig.f:36-42; 1: x0 = 0        */
   int maxit;       /* iteration cap; <= 0 means 10000 (inverserig.f:31)     */

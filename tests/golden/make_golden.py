@@ -16,6 +16,9 @@ Fixtures (inputs and the reference's outputs, nothing else):
                    dipoles, eps 2.25, RR, x-polarised plane wave), tol 1e-4,
                    for GPBICG1, QMRBICGSTAB1, QMRBICGSTAB2, TFQMR: E (FFloc),
                    tol1, nloop, ncompte
+  solve_more.npz   the same for CG (zcg.f) and BICGSTAB (pimzbicgstab.f), at
+                   tol 1e-4 and 1e-6 (written by `make_golden.py more`, which
+                   leaves the other fixtures as they are)
   microscope.npz   diffractefft2dlens_ (nside +1 and -1, two NA) on the seeded
                    6x5x4 box and fouriertoimage_ on seeded 32 x 32 planes
   farfield.npz     diffractefft2d_ and computegcfft2d_: a seeded box-ordered
@@ -146,5 +149,29 @@ def farfield(r, s, k0, alpha, Eloc):
     np.savez_compressed(os.path.join(HERE, "farfield.npz"), **out)
 
 
+def solve_more(r):
+    s, k0, alpha, E = scene.config1()
+    Gs = r.green_fft(s.nx, s.ny, s.nz, s.d, k0)
+    pol = np.zeros((s.nx * s.ny * s.nz, 3, 3), np.complex128)
+    for a in range(3):
+        pol[: s.nd, a, a] = alpha
+    res = {}
+    for m in ["CG", "BICGSTAB"]:
+        for tag, tol in (("4", 1e-4), ("6", 1e-6)):
+            o = r.inverserig(Gs, s.tabdip, s.nx, s.ny, s.nz, pol, E, m, tol)
+            assert o["nstop"] == 0, o
+            key = f"{m}_{tag}"
+            res[f"{key}_E"] = o["FFloc"]
+            res[f"{key}_FF"] = o["FF"]
+            res[f"{key}_tol1"] = o["tol1"]
+            res[f"{key}_nloop"] = o["nloop"]
+            res[f"{key}_ncompte"] = o["ncompte"]
+            print(key, o["nloop"], o["ncompte"], o["tol1"])
+    np.savez_compressed(os.path.join(HERE, "solve_more.npz"), tabdip=s.tabdip, E_inc=E, **res)
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "more":
+        solve_more(Ref())
+    else:
+        main()

@@ -41,6 +41,26 @@ def test_config1_matches_reference(cfg1, method):
     assert relerr(r["FF"], ff) < 1e-14
 
 
+@pytest.mark.parametrize("method,tag,tol", [("CG", "4", 1e-4), ("CG", "6", 1e-6),
+                                            ("BICGSTAB", "4", 1e-4), ("BICGSTAB", "6", 1e-6)])
+def test_cg_bicgstab_match_reference(cfg1, method, tag, tol):
+    """zcg.f and pimzbicgstab.f as inverserig runs them (x0 = E_inc): the
+    reference's iteration and matvec counts, its tol1 (BICGSTAB's divided by
+    the squared initial residual norm, inverserig.f:796-808), its fields."""
+    s, k0, alpha, E, sv = cfg1
+    g = golden("solve_more.npz")
+    assert np.array_equal(g["tabdip"], s.tabdip) and relerr(g["E_inc"], E) < 1e-15
+    key = f"{method}_{tag}"
+    print(f"solve config1 {key}", flush=True)
+    r = sv.inverserig(E, method, tol)
+    assert r["nstop"] == 0, r
+    assert r["nloop"] == int(g[f"{key}_nloop"])
+    assert r["ncompte"] == int(g[f"{key}_ncompte"])
+    assert abs(r["tol1"] / float(g[f"{key}_tol1"]) - 1) < 1e-8
+    assert relerr(r["FFloc"], g[f"{key}_E"]) < 1e-10
+    assert relerr(r["FF"], g[f"{key}_FF"]) < 1e-10
+
+
 def test_gpbicg_trajectory_matches_oracle(cfg1, oracle):
     s, k0, alpha, E, sv = cfg1
     sv.trace_enable(64)
@@ -59,12 +79,14 @@ def test_gpbicg_trajectory_matches_oracle(cfg1, oracle):
     sv.trace_enable(0)
 
 
-@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("method", METHODS + ["CG", "BICGSTAB"])
 def test_mixed_precision_refinement(cfg1, method):
     """FP32 matvec + FP64 refinement: true residual <= tol (FP64 matvec) and
     the field within the tolerance of the FP64 solve."""
     s, k0, alpha, E, sv = cfg1
-    ref = sv.inverserig(E, method, 1e-8)
+    # an accurate field (BICGSTAB's own criterion, |r|^2/|r0|^2 <= tol/10, stops
+    # far above tol; the refinement tests the true residual)
+    ref = sv.inverserig(E, "GPBICG1", 1e-10)
     for tol in (1e-4, 1e-8):
         r = sv.inverserig(E, method, tol, precision=32, x0_zero=True)
         assert r["nstop"] == 0, r
@@ -92,10 +114,13 @@ def test_x0_zero_and_dense_solution(gpu, oracle):
         sv.greencalculfft(s.d, k0)
         sv.set_dipoles(s.tabdip)
         sv.set_polarizability(alpha)
-        for m in METHODS:
+        for m in METHODS + ["CG"]:
             r = sv.inverserig(E, m, 1e-10, x0_zero=True)
             assert r["nstop"] == 0 and r["tol1"] < 1e-10
             assert relerr(r["FFloc"], xs) < 1e-8
+        # BICGSTAB stops at |r|^2 / |r0|^2 <= tol / 10 (pimzbicgstab.f:168-177)
+        r = sv.inverserig(E, "BICGSTAB", 1e-10, x0_zero=True)
+        assert r["nstop"] == 0 and relerr(r["FFloc"], xs) < 1e-4
 
 
 def test_zero_rhs_breakdown(cfg1):
