@@ -713,14 +713,19 @@ static int method_from_name(const char* m, size_t len) {
     return a == b;
   };
   if (eq(7, "GPBICG1")) return IFDDA_GPBICG1;
-  if (eq(7, "GPBICG2") || eq(10, "GPBICGsafe") || eq(10, "GPBICGplus") || eq(8, "GPBICGAR") ||
-      eq(12, "BICGSTARPLUS") || eq(6, "QMRCLA"))
-    return 0;
+  if (eq(7, "GPBICG2")) return IFDDA_GPBICG2;
+  if (eq(10, "GPBICGsafe")) return IFDDA_GPBICGSAFE;
+  if (eq(10, "GPBICGplus")) return IFDDA_GPBICGPLUS;
+  if (eq(8, "GPBICGAR")) return IFDDA_GPBICGAR;  // 'GPBICGAR2' too (inverserig.f:306 comes first)
+  if (eq(12, "BICGSTARPLUS")) return IFDDA_BICGSTARPLUS;
+  if (eq(6, "QMRCLA")) return IFDDA_QMRCLA;
   if (eq(5, "TFQMR")) return IFDDA_TFQMR;
   if (eq(5, "CG")) return IFDDA_CG;
   if (eq(8, "BICGSTAB")) return IFDDA_BICGSTAB;
   if (eq(12, "QMRBICGSTAB1")) return IFDDA_QMRBICGSTAB1;
   if (eq(12, "QMRBICGSTAB2")) return IFDDA_QMRBICGSTAB2;
+  if (eq(7, "GPBICOR")) return IFDDA_GPBICOR;
+  if (eq(7, "CORS")) return IFDDA_CORS;
   return 0;
 }
 

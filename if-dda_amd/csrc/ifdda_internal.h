@@ -236,6 +236,12 @@ struct MvOp {
   // FF = alpha . x for every right-hand side
   virtual bool alpha_apply(const double2* x, double2* FF, int R, long long n,
                            hipStream_t s) const = 0;
+  // The transposed operator y = x - alpha^T T x (what inverserig.f:559-590
+  // forms for QMRCLA); zero: an all-zero vector of the layout of x.
+  virtual bool mvt64(const double2* x, const double2* zero, double2* y, hipStream_t s,
+                     const int* skip, int R, long long n, int skip_stride) const;
+  virtual bool mvt32(const float2* x, const float2* zero, float2* y, hipStream_t s,
+                     const int* skip, int R, long long n, int skip_stride) const;
 };
 // FF = alpha . x for one problem of nd dipoles.
 bool alpha_apply_one(const double2* x, double2* FF, const double2* al, int mode, long long nd,

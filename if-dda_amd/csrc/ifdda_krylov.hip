@@ -20,196 +20,13 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 #include "../../include/ifdda.h"
 #include "ifdda_internal.h"
+#include "ifdda_krylov.h"
 
 namespace ifdda {
-
-constexpr int kThreads = 256;
-constexpr int kMaxBlocks = 1024;
-constexpr int kMaxDots = 5;
-static_assert(kMaxBlocks % kThreads == 0, "finish_sum splits the partials evenly");
-
-struct KState {
-  double norm;    // |b|
-  double residu;  // recurrence residual estimate
-  double tol;
-  double2 alpha, beta, eta, dzeta, r0rn;    // GPBICG (eta/alpha shared by all)
-  double2 rho, omega, tau, taut, theta, thetat, etat;  // QMRBICGSTAB / TFQMR
-  double2 rho0, xxi;  // CG (zcg.f), BICGSTAB (pimzbicgstab.f)
-  double rr;          // BICGSTAB: sum conj(r) r of the latest residual
-  double2 dots[kMaxDots];                   // last finished reduction
-  int it;          // ITNO of the iteration in progress
-  int done;        // solve ended (converged, breakdown or iteration cap)
-  int status;      // 1 converged / cap, -1 breakdown
-  int steperr;
-  int maxit;
-  int methode;     // QMRBICGSTAB omega formula
-  int ntrace, tracecap;
-};
-
-// ---------------------------------------------------------------- complex FP64
-__device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 zsub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 zmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ double2 zconj(double2 a) { return make_double2(a.x, -a.y); }
-__device__ __forceinline__ bool zzero(double2 a) { return a.x == 0.0 && a.y == 0.0; }
-__device__ __forceinline__ double zabs(double2 a) { return hypot(a.x, a.y); }
-// Smith's division
-__device__ __forceinline__ double2 zdiv(double2 a, double2 b) {
-  if (fabs(b.x) >= fabs(b.y)) {
-    const double r = b.y / b.x, d = b.x + b.y * r;
-    return make_double2((a.x + a.y * r) / d, (a.y - a.x * r) / d);
-  }
-  const double r = b.x / b.y, d = b.y + b.x * r;
-  return make_double2((a.x * r + a.y) / d, (a.y * r - a.x) / d);
-}
-// principal square root
-__device__ __forceinline__ double2 zsqrt(double2 z) {
-  if (z.x == 0.0 && z.y == 0.0) return make_double2(0.0, 0.0);
-  const double r = hypot(z.x, z.y);
-  if (z.x >= 0.0) {
-    const double t = sqrt(0.5 * (r + z.x));
-    return make_double2(t, z.y / (2.0 * t));
-  }
-  const double t = sqrt(0.5 * (r - z.x));
-  return make_double2(fabs(z.y) / (2.0 * t), copysign(t, z.y));
-}
-__device__ __forceinline__ double2 zone() { return make_double2(1.0, 0.0); }
-
-// ------------------------------------------------------- vector element access
-template <typename V> struct Vt;
-template <> struct Vt<double2> {
-  static __device__ __forceinline__ double2 ld(const double2* p, long long i) { return p[i]; }
-  static __device__ __forceinline__ void st(double2* p, long long i, double2 v) { p[i] = v; }
-  static __device__ __forceinline__ double2 rnd(double2 v) { return v; }
-};
-template <> struct Vt<float2> {
-  static __device__ __forceinline__ double2 ld(const float2* p, long long i) {
-    const float2 v = p[i];
-    return make_double2((double)v.x, (double)v.y);
-  }
-  static __device__ __forceinline__ void st(float2* p, long long i, double2 v) {
-    p[i] = make_float2((float)v.x, (float)v.y);
-  }
-  // the value a store followed by a load returns
-  static __device__ __forceinline__ double2 rnd(double2 v) {
-    return make_double2((double)(float)v.x, (double)(float)v.y);
-  }
-};
-// Vector updates are evaluated in FP64 registers and stored in V.
-#define LD(p, i) Vt<V>::ld(p, i)
-#define ST(p, i, v) Vt<V>::st(p, i, v)
-
-// -------------------------------------------------------------- reductions
-template <int NR>
-__device__ __forceinline__ void block_partials(double2 (&v)[NR], double2* part, int nblk) {
-  for (int off = 32; off > 0; off >>= 1)
-    for (int r = 0; r < NR; ++r) {
-      v[r].x += __shfl_down(v[r].x, off, 64);
-      v[r].y += __shfl_down(v[r].y, off, 64);
-    }
-  __shared__ double2 sh[NR][kThreads / 64];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0)
-    for (int r = 0; r < NR; ++r) sh[r][wid] = v[r];
-  __syncthreads();
-  if (threadIdx.x == 0)
-    for (int r = 0; r < NR; ++r) {
-      double2 s = make_double2(0.0, 0.0);
-      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s = zadd(s, sh[r][w]);
-      part[r * nblk + blockIdx.x] = s;
-    }
-}
-
-// Sum the partials of NR reductions into st->dots.  Launched as one block of
-// kThreads: every thread issues its (at most kMaxBlocks / kThreads) partial
-// loads back to back, then a fixed-order reduction (thread, LDS, first
-// wavefront shuffles) so the result does not depend on timing.
-template <int NR>
-__device__ __forceinline__ void finish_sum(KState* st, const double2* part, int nblk) {
-  constexpr int kPer = kMaxBlocks / kThreads;
-  __shared__ double2 red[NR][kThreads];
-  const int tid = threadIdx.x;
-  double2 v[NR][kPer];
-#pragma unroll
-  for (int r = 0; r < NR; ++r)
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const int b = tid + k * kThreads;
-      v[r][k] = b < nblk ? part[r * nblk + b] : make_double2(0.0, 0.0);
-    }
-#pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    double2 s = v[r][0];
-#pragma unroll
-    for (int k = 1; k < kPer; ++k) s = zadd(s, v[r][k]);
-    red[r][tid] = s;
-  }
-  __syncthreads();
-  if (tid < 64) {
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      double2 s = red[r][tid];
-#pragma unroll
-      for (int k = 1; k < kThreads / 64; ++k) s = zadd(s, red[r][tid + 64 * k]);
-      for (int off = 32; off > 0; off >>= 1) {
-        s.x += __shfl_down(s.x, off, 64);
-        s.y += __shfl_down(s.y, off, 64);
-      }
-      if (tid == 0) st->dots[r] = s;
-    }
-  }
-  __syncthreads();
-}
-
-// Vector geometry of a launch over right-hand sides: RHS r starts at element
-// off(r) and holds len(r) elements -- r * n and n, or, for packed problems
-// of different sizes, offs[r] and lens[r] (ragged, back to back).
-struct VN {
-  long long n;
-  const long long* lens;
-  const long long* offs;
-  __device__ __forceinline__ long long len(int r) const { return lens ? lens[r] : n; }
-  __device__ __forceinline__ long long off(int r) const { return offs ? offs[r] : r * n; }
-};
-
-// Blocks that serve right-hand side r: the whole grid when all RHS have the
-// same length, else the blocks a solve of RHS r alone would launch.  Either
-// way element i goes to the same block as in that solve, so the partial sums
-// and their order are the same.  The grid's other blocks leave at once
-// (reductions write a zero partial, which the fixed-order finish adds
-// exactly).
-__device__ __forceinline__ int rhs_blocks(const VN& n, int r) {
-  if (!n.lens) return gridDim.x;
-  const long long b = (n.lens[r] + kThreads - 1) / kThreads;
-  return (int)(b < 1 ? 1 : (b > kMaxBlocks ? kMaxBlocks : b));
-}
-#define RHS_BLOCKS(n, r, NR)                                                   \
-  const int nbr_ = rhs_blocks(n, r);                                           \
-  if ((int)blockIdx.x >= nbr_) {                                               \
-    if (threadIdx.x == 0)                                                      \
-      for (int q_ = 0; q_ < NR; ++q_) part[q_ * nblk + blockIdx.x] = make_double2(0.0, 0.0); \
-    return;                                                                    \
-  }
-#define RHS_BLOCKS_NOSUM(n, r)          \
-  const int nbr_ = rhs_blocks(n, r);    \
-  if ((int)blockIdx.x >= nbr_) return;
-// Grid-stride loop over the elements of the block's RHS (nbr_ blocks).
-#define GRID_LOOP(i, n) \
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < (n); i += (long long)nbr_ * blockDim.x)
-
-__device__ __forceinline__ void push_trace(KState* st, double2* trace, double2 a, double2 b,
-                                           double2 c, double2 d, double2 e, double res) {
-  if (trace && st->ntrace < st->tracecap) {
-    double2* t = trace + 6 * st->ntrace;
-    t[0] = a; t[1] = b; t[2] = c; t[3] = d; t[4] = e; t[5] = make_double2(res, 0.0);
-    st->ntrace++;
-  }
-}
 
 // ======================================================= shared set-up kernels
 
@@ -337,9 +154,10 @@ __global__ void __launch_bounds__(kThreads) g_f2(KState* st, const double2* part
 
 // w = At + beta Apold (0 in the first iteration; gpbicg.f:319-325 of the
 // previous iteration), y = told - r + alpha (Ap - w), t = r - alpha Ap
-// (gpbicg.f:192-200); told holds the previous t.
+// (gpbicg.f:192-200); told holds the previous t.  gp2: y summed as
+// gpbicg2.f:61 writes it, told - r - alpha w + alpha Ap.
 template <typename V>
-__global__ void __launch_bounds__(kThreads) g_k3(const KState* st, GV<V> v, VN n) {
+__global__ void __launch_bounds__(kThreads) g_k3(const KState* st, GV<V> v, VN n, int gp2) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
@@ -350,7 +168,8 @@ __global__ void __launch_bounds__(kThreads) g_k3(const KState* st, GV<V> v, VN n
   GRID_LOOP(i, n.len(rh_)) {
     const double2 t = LD(v.told, i), r = LD(v.r, i), Ap = LD(v.Ap, i);
     const double2 w = first ? make_double2(0.0, 0.0) : zadd(LD(v.At, i), zmul(be, LD(v.Apold, i)));
-    ST(v.y, i, zadd(zsub(t, r), zmul(al, zsub(Ap, w))));
+    ST(v.y, i, gp2 ? zadd(zsub(zsub(t, r), zmul(al, w)), zmul(al, Ap))
+                   : zadd(zsub(t, r), zmul(al, zsub(Ap, w))));
     ST(v.t, i, zsub(r, zmul(al, Ap)));
   }
 }
@@ -379,7 +198,9 @@ __global__ void __launch_bounds__(kThreads) g_k4(const KState* st, GV<V> v, VN n
   block_partials<5>(acc, part, nblk);
 }
 
-__global__ void __launch_bounds__(kThreads) g_f4(KState* st, const double2* part, int nblk) {
+// gp2: the steepest-descent step (eta = 0) on every even iteration
+// (gpbicg2.f:75-88), not only the first
+__global__ void __launch_bounds__(kThreads) g_f4(KState* st, const double2* part, int nblk, int gp2) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
   part += (long long)rh_ * kMaxDots * nblk;
@@ -388,7 +209,7 @@ __global__ void __launch_bounds__(kThreads) g_f4(KState* st, const double2* part
   if (threadIdx.x == 0) {
     const double2 c1 = st->dots[0], c2 = st->dots[1], c3 = st->dots[2], c4 = st->dots[3],
                   c5 = st->dots[4];
-    if (st->it == 0) {  // gpbicg.f:217-235
+    if (gp2 ? st->it % 2 == 0 : st->it == 0) {  // gpbicg.f:217-235
       st->eta = make_double2(0.0, 0.0);
       if (zzero(c1)) {
         st->status = -1;
@@ -1502,6 +1323,35 @@ __global__ void __launch_bounds__(kThreads) k_alpha_apply(const double2* x, doub
   }
 }
 
+// y = alpha^T y + x per dipole, y holding -T x on entry: the transposed
+// operator A^T x = x - alpha^T T x as inverserig.f:576-589 forms it (the
+// transpose of the 3x3 alpha, summed over its first index in order).
+// Right-hand sides whose skip flag is set are left alone.
+template <typename V>
+__global__ void __launch_bounds__(kThreads) k_alpha_t(const V* x, V* y, const V* al, int mode,
+                                                     long long nd, const int* skip,
+                                                     int skip_stride) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  if (skip && skip[(long long)rh_ * skip_stride]) return;
+  x += rh_ * 3 * nd;
+  y += rh_ * 3 * nd;
+  const int nbr_ = gridDim.x;
+  GRID_LOOP(i, nd) {
+    double2 e[3], f[3];
+    for (int a = 0; a < 3; ++a) e[a] = Vt<V>::ld(y, 3 * i + a);
+    if (mode == kAlphaScalar) {
+      const double2 s = Vt<V>::ld(al, i);
+      for (int a = 0; a < 3; ++a) f[a] = zmul(s, e[a]);
+    } else {
+      const V* m = al + 9 * i;
+      for (int a = 0; a < 3; ++a)
+        f[a] = zadd(zadd(zmul(Vt<V>::ld(m, a), e[0]), zmul(Vt<V>::ld(m, 3 + a), e[1])),
+                    zmul(Vt<V>::ld(m, 6 + a), e[2]));
+    }
+    for (int a = 0; a < 3; ++a) Vt<V>::st(y, 3 * i + a, zadd(f[a], Vt<V>::ld(x, 3 * i + a)));
+  }
+}
+
 // r = b - Ax (FP64), cast to FP32 for the inner solve; partial |r|^2
 __global__ void __launch_bounds__(kThreads) k_refine_resid(const double2* b, const double2* Ax,
                                                           int have_ax, double2* r64, float2* r32,
@@ -1542,28 +1392,6 @@ __global__ void __launch_bounds__(kThreads) k_refine_add(double2* x, const float
 #undef ST
 
 // ------------------------------------------------------------- host driver
-// Workspace for up to R right-hand sides of n = 3*nd complex values each.
-struct Solver {
-  long long n = 0;  // 3 * dipoles per RHS
-  int R = 0;        // right-hand sides the buffers hold
-  int nblk = 1;     // blocks per RHS in vector kernels
-  double2* v64[14] = {};
-  float2* v32[14] = {};
-  double2* part = nullptr;   // R * kMaxDots * nblk partial sums
-  KState* st = nullptr;      // R states
-  KState* st_host = nullptr; // pinned copy
-  int* act = nullptr;        // R refinement flags: 1 = converged (skip)
-  double2* trace = nullptr;  // per-iteration scalars of RHS 0 (single-RHS solves)
-  int tracecap = 0;
-  double2* rr64 = nullptr;   // refinement residual (FP64), R * n
-  long long* lens = nullptr; // ragged layout (packed problems): per-RHS lengths
-  long long* offs = nullptr; // and offsets, else null (RHS r at r * n)
-  long long total = 0;       // elements per vector array (R * n, or the ragged sum)
-  long long cap_total = 0;   // capacity of the buffers: elements, right-hand sides
-  int cap_R = 0;
-  int part_blocks = 0;       // blocks per RHS the partial-sum array holds
-};
-
 void solver_delete(Solver* S) {
   if (!S) return;
   for (int i = 0; i < 14; ++i) {
@@ -1708,19 +1536,6 @@ static Solver* solver_get(ifdda_ctx* c, int R, bool need32) {
   return S;
 }
 
-// Batched matvec over R right-hand sides (vectors n apart); skip flags
-// skip[r * skip_stride] drop converged RHS on the device.
-template <typename V> static bool mv(const MvOp& op, const V* x, const V* b, V* y, hipStream_t s,
-                                     const int* skip, int R, long long n, int skip_stride);
-template <> bool mv<double2>(const MvOp& op, const double2* x, const double2* b, double2* y,
-                             hipStream_t s, const int* skip, int R, long long n, int skip_stride) {
-  return op.mv64(x, b, y, s, skip, R, n, skip_stride);
-}
-template <> bool mv<float2>(const MvOp& op, const float2* x, const float2* b, float2* y,
-                            hipStream_t s, const int* skip, int R, long long n, int skip_stride) {
-  return op.mv32(x, b, y, s, skip, R, n, skip_stride);
-}
-
 // The operator of one context: its box, Green tensor, dipoles and
 // polarisability, shared by all R right-hand sides.
 struct CtxOp : MvOp {
@@ -1748,7 +1563,47 @@ struct CtxOp : MvOp {
                        (const double2*)c->al64, c->alpha_mode, c->nd);
     return hip_ok(hipGetLastError(), "alpha apply");
   }
+  // -T x by the matvec without alpha (b = 0), then alpha^T (-T x) + x
+  template <typename V>
+  bool transposed(MatvecDev<V> m, const V* al, const V* x, const V* zero, V* y, hipStream_t s,
+                  const int* skip, int R, long long n, int skip_stride) const {
+    m.nrhs = R;
+    m.vstride = n;
+    m.skip_stride = skip_stride;
+    m.alpha_mode = kAlphaNone;
+    bool ok;
+    if constexpr (std::is_same<V, double2>::value)
+      ok = launch_matvec_f64(c->g, m, x, zero, y, s, skip);
+    else
+      ok = launch_matvec_f32(c->g, m, x, zero, y, s, skip);
+    if (!ok) return false;
+    const int nb = (int)std::max<long long>(1, std::min<long long>(kMaxBlocks, (c->nd + kThreads - 1) / kThreads));
+    hipLaunchKernelGGL(k_alpha_t<V>, dim3(nb, R), dim3(kThreads), 0, s, x, y, al, c->alpha_mode,
+                       c->nd, skip, skip_stride);
+    return hip_ok(hipGetLastError(), "transposed alpha");
+  }
+  bool mvt64(const double2* x, const double2* zero, double2* y, hipStream_t s, const int* skip,
+             int R, long long n, int skip_stride) const override {
+    return transposed<double2>(c->mv64(), (const double2*)c->al64, x, zero, y, s, skip, R, n,
+                               skip_stride);
+  }
+  bool mvt32(const float2* x, const float2* zero, float2* y, hipStream_t s, const int* skip, int R,
+             long long n, int skip_stride) const override {
+    return transposed<float2>(c->mv32(), (const float2*)c->al32, x, zero, y, s, skip, R, n,
+                              skip_stride);
+  }
 };
+
+bool MvOp::mvt64(const double2*, const double2*, double2*, hipStream_t, const int*, int, long long,
+                 int) const {
+  set_error("the transposed operator (QMRCLA) is not available for this solve");
+  return false;
+}
+bool MvOp::mvt32(const float2*, const float2*, float2*, hipStream_t, const int*, int, long long,
+                 int) const {
+  set_error("the transposed operator (QMRCLA) is not available for this solve");
+  return false;
+}
 
 bool alpha_apply_one(const double2* x, double2* FF, const double2* al, int mode, long long nd,
                      hipStream_t s) {
@@ -1757,10 +1612,6 @@ bool alpha_apply_one(const double2* x, double2* FF, const double2* al, int mode,
   return hip_ok(hipGetLastError(), "alpha apply");
 }
 
-// Outcome of one Krylov run for one right-hand side.
-struct KRun {
-  int status = 0, steperr = -1, nloop = 0, nmv = 0;
-};
 
 static bool poll(Solver* S, int R, hipStream_t s) {
   return hip_ok(hipMemcpyAsync(S->st_host, S->st, sizeof(KState) * R, hipMemcpyDeviceToHost, s),
@@ -1832,7 +1683,8 @@ static bool krylov(const MvOp& op, Solver* S, V** vv, int R, int method, const d
     return 0;
   };
 
-  if (method == IFDDA_GPBICG1) {
+  if (method == IFDDA_GPBICG1 || method == IFDDA_GPBICG2) {
+    const int gp2 = method == IFDDA_GPBICG2;
     V* const tb[2] = {vv[6], vv[12]};  // t / told
     V* const ab[2] = {vv[5], vv[10]};  // Ap / Apold
     GV<V> v{x, b, vv[2], vv[3], vv[4], ab[0], tb[0], vv[7], vv[8], vv[9], ab[1], vv[11], tb[1], Ax};
@@ -1847,10 +1699,10 @@ static bool krylov(const MvOp& op, Solver* S, V** vv, int R, int method, const d
       if (!mv<V>(op, v.p, v.p, v.Ap, s, done, R, n, dstride)) return false;
       hipLaunchKernelGGL(g_k2<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
       hipLaunchKernelGGL(g_f2, F, T, 0, s, S->st, (const double2*)S->part, nb);
-      hipLaunchKernelGGL(g_k3<V>, G, T, 0, s, (const KState*)S->st, v, vn);
+      hipLaunchKernelGGL(g_k3<V>, G, T, 0, s, (const KState*)S->st, v, vn, gp2);
       if (!mv<V>(op, v.t, v.t, v.At, s, done, R, n, dstride)) return false;
       hipLaunchKernelGGL(g_k4<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
-      hipLaunchKernelGGL(g_f4, F, T, 0, s, S->st, (const double2*)S->part, nb);
+      hipLaunchKernelGGL(g_f4, F, T, 0, s, S->st, (const double2*)S->part, nb, gp2);
       hipLaunchKernelGGL(g_k5<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
       hipLaunchKernelGGL(g_f5, F, T, 0, s, S->st, (const double2*)S->part, nb, trace);
       if (!hip_ok(hipGetLastError(), "gpbicg launch")) return false;
@@ -2009,6 +1861,9 @@ static bool krylov(const MvOp& op, Solver* S, V** vv, int R, int method, const d
       if (h.status == -1 && h.steperr == 6) nmv = 2 * (k - 1);
       o.nmv = nmv0 + nmv;
     }
+  } else if (krylov_gp_method(method)) {
+    KLoop L{op, S, vn, nb, R, n, s, trace, done, dstride, nmv0, x0zero, active, step_poll};
+    if (!krylov_gp<V>(L, vv, method, out)) return false;
   } else {
     set_error("unknown method");
     return false;
@@ -2040,7 +1895,7 @@ static int solve_dev(ifdda_ctx* c, const ifdda_solve_opts* o, int R, const doubl
     std::memset(&infos[r], 0, sizeof(ifdda_solve_info));
     set_info_str(&infos[r], "");
   }
-  if (o->method < IFDDA_GPBICG1 || o->method > IFDDA_BICGSTAB) {
+  if (o->method < IFDDA_GPBICG1 || o->method > IFDDA_QMRCLA) {
     set_error("ifdda_solve: unknown method");
     return IFDDA_ERR_ARG;
   }

@@ -220,7 +220,11 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
   const auto t0 = std::chrono::steady_clock::now();
   const int J = (int)in.size();
   if (J == 0) return IFDDA_OK;
-  if (o->method < IFDDA_GPBICG1 || o->method > IFDDA_TFQMR || (o->precision != 64 && o->precision != 32)) {
+  if (o->method == IFDDA_QMRCLA) {  // A^T per member is not built for packed launches
+    set_error("pack solve: QMRCLA needs the transposed operator; solve it on a context");
+    return IFDDA_ERR_ARG;
+  }
+  if (o->method < IFDDA_GPBICG1 || o->method > IFDDA_CORS || (o->precision != 64 && o->precision != 32)) {
     set_error("pack solve: unknown method or precision");
     return IFDDA_ERR_ARG;
   }

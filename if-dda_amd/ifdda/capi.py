@@ -20,7 +20,9 @@ lib_path = os.path.join(PKG, "libifdda_hip.so")
 
 GPBICG1, QMRBICGSTAB1, QMRBICGSTAB2, TFQMR = 1, 2, 3, 4
 METHODS = {"GPBICG1": GPBICG1, "QMRBICGSTAB1": QMRBICGSTAB1, "QMRBICGSTAB2": QMRBICGSTAB2,
-           "TFQMR": TFQMR, "CG": 5, "BICGSTAB": 6}
+           "TFQMR": TFQMR, "CG": 5, "BICGSTAB": 6, "GPBICG2": 7, "GPBICGsafe": 8, "GPBICGplus": 9,
+           "GPBICGAR": 10, "GPBICGAR2": 10, "BICGSTARPLUS": 11, "GPBICOR": 12, "CORS": 13,
+           "QMRCLA": 14}
 
 
 class IfddaError(RuntimeError):

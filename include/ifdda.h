@@ -38,6 +38,14 @@ extern "C" {
 #define IFDDA_TFQMR 4
 #define IFDDA_CG 5        /* zcg.f: conjugate gradients, bilinear products   */
 #define IFDDA_BICGSTAB 6  /* pimzbicgstab.f                                 */
+#define IFDDA_GPBICG2 7       /* gpbicg2.f: GPBICG, BiCGSTAB step every other iteration */
+#define IFDDA_GPBICGSAFE 8    /* gpbicgsafe.f                                    */
+#define IFDDA_GPBICGPLUS 9    /* gpbicgplus.f                                    */
+#define IFDDA_GPBICGAR 10     /* gpbicgar.f (also what 'GPBICGAR2' runs)         */
+#define IFDDA_BICGSTARPLUS 11 /* bicgstarplus.f                                  */
+#define IFDDA_GPBICOR 12      /* gpbicor.f                                       */
+#define IFDDA_CORS 13         /* cors.f                                          */
+#define IFDDA_QMRCLA 14       /* qmrpim.f (PIMZQMR): needs A^T, contexts only     */
 
 typedef struct ifdda_ctx ifdda_ctx;
 
@@ -94,7 +102,7 @@ int ifdda_matvec_device(ifdda_ctx *ctx, int prec, int apply_alpha, const void *x
 
 /* Solve options (the reference's methodeit / tol / nlim, plus precision). */
 typedef struct {
-  int method;      /* IFDDA_GPBICG1 ... IFDDA_BICGSTAB                     */
+  int method;      /* IFDDA_GPBICG1 ... IFDDA_QMRCLA                       */
   int precision;   /* 64: plain FP64; 32: FP32 matvec + FP64 refinement    */
   int x0_zero;     /* 0: x0 = E_inc as inverserig.f:36-42; 1: x0 = 0        */
   int maxit;       /* iteration cap; <= 0 means 10000 (inverserig.f:31)     */

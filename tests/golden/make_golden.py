@@ -19,6 +19,13 @@ Fixtures (inputs and the reference's outputs, nothing else):
   solve_more.npz   the same for CG (zcg.f) and BICGSTAB (pimzbicgstab.f), at
                    tol 1e-4 and 1e-6 (written by `make_golden.py more`, which
                    leaves the other fixtures as they are)
+  solve_gp.npz     the same for GPBICG2, GPBICGsafe, GPBICGplus, GPBICGAR,
+                   BICGSTARPLUS, GPBICOR, CORS and QMRCLA at tol 1e-4 and 1e-6, the
+                   fields as 32 projections on seeded complex weights
+                   (projection_weights) to keep the file small
+                   (`make_golden.py gp`; run with OMP_NUM_THREADS=1: the
+                   reference's GPBICOR gives run-to-run differences of ~1e-5
+                   on several OpenMP threads, deterministic on one)
   microscope.npz   diffractefft2dlens_ (nside +1 and -1, two NA) on the seeded
                    6x5x4 box and fouriertoimage_ on seeded 32 x 32 planes
   farfield.npz     diffractefft2d_ and computegcfft2d_: a seeded box-ordered
@@ -149,29 +156,45 @@ def farfield(r, s, k0, alpha, Eloc):
     np.savez_compressed(os.path.join(HERE, "farfield.npz"), **out)
 
 
-def solve_more(r):
+def projection_weights(n, k=32, seed=2024):
+    """k seeded complex weight rows of length n: a field E is kept as W @ E."""
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n))
+
+
+def solve_more(r, methods=("CG", "BICGSTAB"), name="solve_more.npz", project=False):
     s, k0, alpha, E = scene.config1()
     Gs = r.green_fft(s.nx, s.ny, s.nz, s.d, k0)
     pol = np.zeros((s.nx * s.ny * s.nz, 3, 3), np.complex128)
     for a in range(3):
         pol[: s.nd, a, a] = alpha
     res = {}
-    for m in ["CG", "BICGSTAB"]:
+    for m in methods:
         for tag, tol in (("4", 1e-4), ("6", 1e-6)):
             o = r.inverserig(Gs, s.tabdip, s.nx, s.ny, s.nz, pol, E, m, tol)
             assert o["nstop"] == 0, o
             key = f"{m}_{tag}"
-            res[f"{key}_E"] = o["FFloc"]
-            res[f"{key}_FF"] = o["FF"]
+            if project:
+                res[f"{key}_Eproj"] = projection_weights(E.size) @ o["FFloc"]
+            else:
+                res[f"{key}_E"] = o["FFloc"]
+                res[f"{key}_FF"] = o["FF"]
             res[f"{key}_tol1"] = o["tol1"]
             res[f"{key}_nloop"] = o["nloop"]
             res[f"{key}_ncompte"] = o["ncompte"]
             print(key, o["nloop"], o["ncompte"], o["tol1"])
-    np.savez_compressed(os.path.join(HERE, "solve_more.npz"), tabdip=s.tabdip, E_inc=E, **res)
+    if project:  # the inputs are those of solve_more.npz; keep a checksum of them
+        res.update(E_inc_proj=projection_weights(E.size) @ E, tabdip_sum=int(s.tabdip.sum()))
+        np.savez_compressed(os.path.join(HERE, name), **res)
+    else:
+        np.savez_compressed(os.path.join(HERE, name), tabdip=s.tabdip, E_inc=E, **res)
 
 
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "more":
         solve_more(Ref())
+    elif len(sys.argv) > 1 and sys.argv[1] == "gp":
+        solve_more(Ref(), ("GPBICG2", "GPBICGsafe", "GPBICGplus", "GPBICGAR", "BICGSTARPLUS",
+                           "GPBICOR", "CORS", "QMRCLA"), "solve_gp.npz", project=True)
     else:
         main()

@@ -98,9 +98,11 @@ def test_jobs_with_descriptors_equal_explicit_arrays(gpu):
     for x, y, z in zip(a, b, c):
         # inputs equal to rounding (device sincos vs numpy exp, C++ vs numpy
         # poladiff): both converge to the tolerance, to the same fields within
-        # it; an ill-conditioned case may take one iteration more or less
+        # it; a long ill-conditioned solve (200+ GPBiCG iterations) may end a
+        # few iterations apart on rounding-level input differences
         assert x["infos"][0]["tol1"] <= 1e-6 and y["infos"][0]["tol1"] <= 1e-6
-        assert abs(x["infos"][0]["nloop"] - y["infos"][0]["nloop"]) <= 2
+        nx_, ny_ = x["infos"][0]["nloop"], y["infos"][0]["nloop"]
+        assert abs(nx_ - ny_) <= max(2, 0.03 * nx_), (nx_, ny_)
         ref = np.abs(x["FFloc"]).max()
         assert np.abs(x["FFloc"] - y["FFloc"]).max() < 1e-4 * ref
         # the same descriptors through the runner: bit for bit

@@ -61,6 +61,124 @@ def test_cg_bicgstab_match_reference(cfg1, method, tag, tol):
     assert relerr(r["FF"], g[f"{key}_FF"]) < 1e-10
 
 
+GP_METHODS = ["GPBICG2", "GPBICGsafe", "GPBICGplus", "GPBICGAR", "BICGSTARPLUS", "GPBICOR", "CORS",
+              "QMRCLA"]
+
+
+def projection_weights(n, k=32, seed=2024):
+    """The seeded weights the golden solve_gp.npz projects its fields on
+    (tests/golden/make_golden.py)."""
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n))
+
+
+@pytest.mark.parametrize("tag,tol", [("4", 1e-4), ("6", 1e-6)])
+@pytest.mark.parametrize("method", GP_METHODS)
+def test_gp_methods_match_reference(cfg1, method, tag, tol):
+    """gpbicg2.f, gpbicgsafe.f, gpbicgplus.f, gpbicgar.f, bicgstarplus.f,
+    gpbicor.f, cors.f and qmrpim.f (QMRCLA, with the transposed operator
+    I - alpha^T T of inverserig.f:559-590) as inverserig runs them (x0 = E_inc): the
+    reference's iteration and matvec counts, its tol1 and its field (as 32
+    seeded projections).  GPBICOR forms u~ = zeta (Ar - At) / alpha from a
+    difference of nearly equal vectors: the reference itself moves by ~1e-5
+    between OpenMP thread counts, so its field and tol1 are compared at that
+    level; the others at 1e-9 / 1e-7."""
+    s, k0, alpha, E, sv = cfg1
+    g = golden("solve_gp.npz")
+    W = projection_weights(E.size)
+    assert int(g["tabdip_sum"]) == int(s.tabdip.sum())
+    assert relerr(W @ E, g["E_inc_proj"]) < 1e-13
+    key = f"{method}_{tag}"
+    print(f"solve config1 {key}", flush=True)
+    r = sv.inverserig(E, method, tol)
+    assert r["nstop"] == 0, r
+    assert r["nloop"] == int(g[f"{key}_nloop"])
+    assert r["ncompte"] == int(g[f"{key}_ncompte"])
+    assert r["tol1"] <= tol
+    loose = method == "GPBICOR"
+    assert abs(r["tol1"] / float(g[f"{key}_tol1"]) - 1) < (5e-2 if loose else 1e-7)
+    assert relerr(W @ r["FFloc"], g[f"{key}_Eproj"]) < (1e-4 if loose else 1e-9)
+    ff = (alpha[:, None] * r["FFloc"].reshape(-1, 3)).reshape(-1)
+    assert relerr(r["FF"], ff) < 1e-14
+
+
+def _dropin_inverserig(L, Gs, s, pol, E, method, tol):
+    """inverserig_ of libifdda_dropin.so with the reference's 40 arguments
+    (inverserig.f:1-6) and the two hidden character lengths."""
+    import ctypes as C
+    nx, ny, nz = s.nx, s.ny, s.nz
+    nx2, ny2, nz2 = 2 * nx, 2 * ny, 2 * nz
+    M = nx2 * ny2 * nz2
+    nmax, nb = nx * ny * nz, E.size // 3
+    ri = lambda v: C.byref(C.c_int(int(v)))  # noqa: E731
+    P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    V = [np.zeros(M, np.complex128) for _ in range(3)]
+    XI, XR = np.zeros(3 * nmax, np.complex128), np.zeros(3 * nmax, np.complex128)
+    wrk = np.zeros(3 * nmax * 12, np.complex128)
+    FF, FFloc = np.zeros(3 * nmax, np.complex128), np.zeros(3 * nmax, np.complex128)
+    FF0 = np.zeros(3 * nmax, np.complex128)
+    FF0[: 3 * nb] = E
+    tab = np.ascontiguousarray(s.tabdip, np.int32)
+    info = C.create_string_buffer(b" " * 64, 64)
+    tol1, nloop, ncompte, nstop = C.c_double(0), C.c_int(0), C.c_int(0), C.c_int(0)
+    plan = C.c_int64(0)
+    L.inverserig_(*[P(g) for g in Gs], *[P(v) for v in V], P(tab), ri(M), ri(M), ri(3 * nmax), ri(12),
+                  ri(nmax), ri(nmax), ri(nx), ri(ny), ri(nz), ri(nx), ri(ny), ri(nz), ri(nx2), ri(ny2),
+                  ri(nx2 * ny2), ri(nz2), ri(nb), ri(3 * nb), P(XI), P(XR), P(wrk), P(FF), P(FF0),
+                  P(FFloc), P(pol), C.c_char_p(method.ljust(12).encode()),
+                  C.byref(C.c_double(tol)), C.byref(tol1), C.byref(nloop), C.byref(ncompte),
+                  C.byref(plan), C.byref(plan), C.byref(nstop), info, C.c_size_t(12), C.c_size_t(64))
+    return dict(FFloc=FFloc[: 3 * nb].copy(), nloop=nloop.value, ncompte=ncompte.value,
+                nstop=nstop.value, tol1=tol1.value, infostr=info.raw.decode().rstrip())
+
+
+def test_dropin_method_names_as_inverserig_compares_them(cfg1):
+    """Through the Fortran drop-in: 'GPBICGAR2' runs GPBICGAR (inverserig.f
+    compares methodeit(1:8) with 'GPBICGAR' first), 'CORS' and 'GPBICOR'
+    reach their solvers, an unknown name sets nstop = 1 with the reference's
+    message."""
+    import ctypes as C
+    import os
+    s, k0, alpha, E, sv = cfg1
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    L = C.CDLL(os.path.join(root, "if-dda_amd", "libifdda_dropin.so"))
+    nx, ny, nz = s.nx, s.ny, s.nz
+    M = 8 * nx * ny * nz
+    Gs = [np.zeros(M, np.complex128) for _ in range(6)]
+    ri = lambda v: C.byref(C.c_int(int(v)))  # noqa: E731
+    P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    L.greencalculfft_(ri(nx), ri(ny), ri(nz), ri(2 * nx), ri(2 * ny), ri(2 * nz), ri(4 * nx * ny),
+                      ri(nx), ri(ny), ri(nz), C.byref(C.c_double(s.d)), C.byref(C.c_double(k0)),
+                      *[P(g) for g in Gs], C.byref(C.c_int64(0)))
+    pol = np.zeros((nx * ny * nz, 3, 3), np.complex128)
+    for a in range(3):
+        pol[: s.nd, a, a] = alpha
+    pol = np.asfortranarray(pol)
+    g = golden("solve_gp.npz")
+    a = _dropin_inverserig(L, Gs, s, pol, E, "GPBICGAR", 1e-4)
+    b = _dropin_inverserig(L, Gs, s, pol, E, "GPBICGAR2", 1e-4)
+    assert a["nstop"] == 0 and b["nstop"] == 0
+    assert np.array_equal(a["FFloc"], b["FFloc"]) and a["nloop"] == b["nloop"]
+    assert a["nloop"] == int(g["GPBICGAR_4_nloop"]) and a["ncompte"] == int(g["GPBICGAR_4_ncompte"])
+    for m in ("CORS", "GPBICOR", "QMRCLA"):
+        c = _dropin_inverserig(L, Gs, s, pol, E, m, 1e-4)
+        assert c["nstop"] == 0 and c["nloop"] == int(g[f"{m}_4_nloop"])
+        assert c["ncompte"] == int(g[f"{m}_4_ncompte"])
+    bad = _dropin_inverserig(L, Gs, s, pol, E, "GPBICG3", 1e-4)
+    assert bad["nstop"] == 1 and bad["infostr"] == "Iterative method not correct!"
+
+
+def test_qmrcla_refused_in_packs(gpu):
+    """Packed launches have no transposed operator: QMRCLA is refused with a
+    clear error, on a context it runs (above)."""
+    from ifdda import scene
+    probs = scene.config4(2, seed=3)
+    batch = [dict(nx=p.nx, ny=p.ny, nz=p.nz, d=p.d, k0=k0, alpha=al, E_inc=E, tabdip=p.tabdip)
+             for (p, k0, al, E) in probs]
+    with pytest.raises(gpu.IfddaError, match="transposed"):
+        gpu.solve_batch(batch, "QMRCLA", 1e-4, precision=64)
+
+
 def test_gpbicg_trajectory_matches_oracle(cfg1, oracle):
     s, k0, alpha, E, sv = cfg1
     sv.trace_enable(64)
@@ -79,7 +197,7 @@ def test_gpbicg_trajectory_matches_oracle(cfg1, oracle):
     sv.trace_enable(0)
 
 
-@pytest.mark.parametrize("method", METHODS + ["CG", "BICGSTAB"])
+@pytest.mark.parametrize("method", METHODS + ["CG", "BICGSTAB"] + GP_METHODS)
 def test_mixed_precision_refinement(cfg1, method):
     """FP32 matvec + FP64 refinement: true residual <= tol (FP64 matvec) and
     the field within the tolerance of the FP64 solve."""
