@@ -60,6 +60,7 @@ extern "C" void ifdda_destroy(ifdda_ctx* c) {
   }
   (void)hipFree(c->G64);
   (void)hipFree(c->G32);
+  (void)hipFree(c->Gd64);
   (void)hipFree(c->tab);
   (void)hipFree(c->al64);
   (void)hipFree(c->al32);
@@ -181,6 +182,7 @@ extern "C" int ifdda_green_build(ifdda_ctx* c, double aretecube, double k0, int 
   const double2* tw[3] = {c->tw64[0], c->tw64[1], c->tw64[2]};
   const int* pos[3] = {c->pos[0], c->pos[1], c->pos[2]};
   c->green_ready = false;
+  c->gd_ready = false;
   bool ok = green_build_device(c->g, tw, pos, aretecube, k0, nquad, full, c->G64, c->G32,
                                c->stream);
   ok = ok && hip_ok(hipStreamSynchronize(c->stream), "green build");
@@ -244,6 +246,7 @@ extern "C" int ifdda_green_set(ifdda_ctx* c, const double* const G[6]) {
          hip_ok(hipStreamSynchronize(c->stream), "green fold");
   }
   (void)hipFree(full);
+  c->gd_ready = false;
   if (!ok) return IFDDA_ERR_HIP;
   c->green_ready = true;
   c->green_host = nullptr;

@@ -399,6 +399,9 @@ template <typename V> struct SArgs {
   // packed launches: tiles along kx (the grid x extent) and threads of the
   // tile shape
   int gx, nthr;
+  // k_zconv_der only: the tensor is a derivative along axis dpar (1 x, 2 y,
+  // 3 z), odd under that axis's mirror on top of the Green tensor's parity
+  int dpar;
 };
 
 __device__ __forceinline__ int fold_idx(int k, int L, int& sgn) {
@@ -525,8 +528,8 @@ __device__ __forceinline__ void sym_product(V* v, const V (&g)[6], bool nxy, boo
 }
 
 // zconv_tile: logical tile lg (ky' = lg / gx, kx tile lg % gx), nthr threads
-// (PK: as strided_tile).
-template <typename V, int NG, bool PK>
+// (PK: as strided_tile; DER: the tensor of a.dpar, see SArgs).
+template <typename V, int NG, bool PK, bool DER = false>
 __device__ __forceinline__ void zconv_tile(const SArgs<V>& a, const V* zsrc, V* zdst, int lg,
                                            int gx, int tid0, int nthr) {
   extern __shared__ double4 smem_d4[];
@@ -586,7 +589,12 @@ __device__ __forceinline__ void zconv_tile(const SArgs<V>& a, const V* zsrc, V* 
       for (int zz = 0; zz < nz2; ++zz) {
         const int kz = zz == 0 ? kzf : L - kzf;
         const bool szn = zz == 1;
-        sym_product(buf + kz * NL + kxl * 3, gr[it], sxn != sy, sxn != szn, sy != szn);
+        V* v = buf + kz * NL + kxl * 3;
+        sym_product(v, gr[it], sxn != sy, sxn != szn, sy != szn);
+        if constexpr (DER) {
+          if ((a.dpar == 1 && sxn) || (a.dpar == 2 && sy) || (a.dpar == 3 && szn))
+            for (int c = 0; c < 3; ++c) v[c] = mk(-v[c].x, -v[c].y);
+        }
       }
     }
     lds_sync();
@@ -606,6 +614,15 @@ __global__ void __launch_bounds__(256) k_zconv(SArgs<V> a) {
   const int lg = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
   zconv_tile<V, NG, false>(a, a.src + blockIdx.z * a.zs_src, a.dst + blockIdx.z * a.zs_dst, lg,
                            gridDim.x, threadIdx.x, blockDim.x);
+}
+
+// P3 for the derivative tensors of the optical force (derivativefield2.f)
+template <typename V, int NG>
+__global__ void __launch_bounds__(256) k_zconv_der(SArgs<V> a) {
+  if (a.skip && a.skip[blockIdx.z * a.skip_stride]) return;
+  const int lg = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  zconv_tile<V, NG, false, true>(a, a.src + blockIdx.z * a.zs_src, a.dst + blockIdx.z * a.zs_dst,
+                                 lg, gridDim.x, threadIdx.x, blockDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1132,6 +1149,42 @@ __device__ __forceinline__ void green_T(double rx, double ry, double rz, double 
   }
 }
 
+// d T / d r_dd at r (the observation minus the source: propesplibdermat.f
+// restated in Cartesian form).  With T = A(R) I + B(R) r r^T,
+//   A = e^{ikR} (k^2/R - 1/R^3 + ik/R^2), B = e^{ikR} (3/R^5 - 3ik/R^4 - k^2/R^3),
+//   d T_ab = delta_ab A' r_dd / R + (delta_a,dd r_b + delta_b,dd r_a) B + r_a r_b B' r_dd / R,
+//   A' = e^{ikR} (ik^3/R - 2k^2/R^2 - 3ik/R^3 + 3/R^4),
+//   B' = e^{ikR} (-15/R^6 + 15ik/R^5 + 6k^2/R^4 - ik^3/R^3);  0 at r = 0.
+__device__ __forceinline__ void green_dT(double rx, double ry, double rz, double k, int dd,
+                                         double2 T[6]) {
+  const double r2 = rx * rx + ry * ry + rz * rz;
+  if (r2 == 0.0) {
+    for (int c = 0; c < 6; ++c) T[c] = make_double2(0.0, 0.0);
+    return;
+  }
+  const double R = sqrt(r2), iR = 1.0 / R, k2 = k * k;
+  const double iR2 = iR * iR, iR3 = iR2 * iR, iR4 = iR2 * iR2, iR5 = iR4 * iR, iR6 = iR3 * iR3;
+  double sn, cs;
+  sincos(k * R, &sn, &cs);
+  const double2 e = make_double2(cs, sn);
+  auto cm = [](double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); };
+  const double2 A = cm(e, make_double2(k2 * iR - iR3, k * iR2));
+  const double2 B = cm(e, make_double2(3.0 * iR5 - k2 * iR3, -3.0 * k * iR4));
+  const double2 Ap = cm(e, make_double2(-2.0 * k2 * iR2 + 3.0 * iR4, k2 * k * iR - 3.0 * k * iR3));
+  const double2 Bp = cm(e, make_double2(-15.0 * iR6 + 6.0 * k2 * iR4, 15.0 * k * iR5 - k2 * k * iR3));
+  const double rv[3] = {rx, ry, rz};
+  const double rd = rv[dd] * iR;
+  const int ia[6] = {0, 0, 0, 1, 1, 2}, ib[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const int a = ia[c], b = ib[c];
+    const double fa = (a == b ? rd : 0.0);
+    const double fb = (a == dd ? rv[b] : 0.0) + (b == dd ? rv[a] : 0.0);
+    const double fc = rv[a] * rv[b] * rd;
+    T[c] = make_double2(fa * Ap.x + fb * B.x + fc * Bp.x, fa * Ap.y + fb * B.y + fc * Bp.y);
+  }
+}
+
 __device__ __forceinline__ int circ_off(int idx, int n, int L, bool& ok) {
   if (idx < n) { ok = true; return idx; }
   if (idx > L - n) { ok = true; return idx - L; }
@@ -1161,6 +1214,30 @@ __global__ void k_green_fill(double2* __restrict__ out, int comp, int nx, int ny
         green_T(-(double)i * d, -(double)j * d, -(double)k * d, k0, T);
         v = T[comp];
       }
+    }
+    out[idx] = v;
+  }
+}
+
+// One component of d T / d r_dd on the circulant grid at r = +offset * d
+// (derivativefield2.f:22-60: the observation minus the source, point values
+// only, zero on the planes at index n).
+__global__ void k_green_dfill(double2* __restrict__ out, int comp, int dd, int nx, int ny, int nz,
+                              int L1, int L2, int L3, double d, double k0) {
+  const long long M = (long long)L1 * L2 * L3;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < M;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int ii = idx % L1;
+    const int jj = (idx / L1) % L2;
+    const int kk = idx / ((long long)L1 * L2);
+    bool okx, oky, okz;
+    const int i = circ_off(ii, nx, L1, okx), j = circ_off(jj, ny, L2, oky),
+              k = circ_off(kk, nz, L3, okz);
+    double2 v = make_double2(0.0, 0.0);
+    if (okx && oky && okz) {
+      double2 T[6];
+      green_dT((double)i * d, (double)j * d, (double)k * d, k0, dd, T);
+      v = T[comp];
     }
     out[idx] = v;
   }
@@ -1262,7 +1339,7 @@ __global__ void k_green_fold(double2* __restrict__ g64, float2* __restrict__ g32
     const int ky = t / ((long long)F1 * F3);
     const double2 v = full[((long long)kz * L2 + ky) * L1 + kx];
     g64[t * 6 + comp] = v;
-    g32[t * 6 + comp] = make_float2((float)v.x, (float)v.y);
+    if (g32) g32[t * 6 + comp] = make_float2((float)v.x, (float)v.y);
   }
 }
 
@@ -1604,7 +1681,13 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     const TileShape& t = *ps.t[2];
     const int items = g.F[2] * t.B;
     const int ng = (items + t.threads - 1) / t.threads;
-    ZconvKernel<V> k = m.specialized && 2 * g.nz <= g.L[2] ? zconv_ct_for<V>(g.L[2], t.B) : nullptr;
+    ZconvKernel<V> k = m.specialized && !m.dpar && 2 * g.nz <= g.L[2]
+                           ? zconv_ct_for<V>(g.L[2], t.B) : nullptr;
+    if (m.dpar) {
+      a.dpar = m.dpar;
+      k = ng <= 1 ? k_zconv_der<V, 1> : ng == 2 ? k_zconv_der<V, 2> : ng == 3 ? k_zconv_der<V, 3>
+                                                                             : k_zconv_der<V, kMaxG>;
+    }
     if (!k)
       k = ng <= 1 ? k_zconv<V, 1> : ng == 2 ? k_zconv<V, 2> : ng == 3 ? k_zconv<V, 3>
                                                                      : k_zconv<V, kMaxG>;
@@ -1812,6 +1895,20 @@ bool green_fold_component(const GridPlan& g, const double2* full, int comp, doub
   hipLaunchKernelGGL(k_green_fold, dim3(blocks), dim3(256), 0, s, g64, g32, full, comp, g.L[0],
                      g.L[1], g.F[0], g.F[1], g.F[2]);
   return hip_ok(hipGetLastError(), "green fold");
+}
+
+bool green_dbuild_device(const GridPlan& g, const double2* tw64[3], const int* pos[3], double d,
+                         double k0, int dd, double2* full, double2* g64, hipStream_t s) {
+  const long long M = (long long)g.L[0] * g.L[1] * g.L[2];
+  bool ok = true;
+  for (int c = 0; ok && c < 6; ++c) {
+    const int blocks = (int)std::min<long long>((M + 255) / 256, 65535);
+    hipLaunchKernelGGL(k_green_dfill, dim3(blocks), dim3(256), 0, s, full, c, dd, g.nx, g.ny, g.nz,
+                       g.L[0], g.L[1], g.L[2], d, k0);
+    ok = hip_ok(hipGetLastError(), "green derivative fill") && fft3d_full_f64(g, tw64, pos, full, s) &&
+         green_fold_component(g, full, c, g64, nullptr, s);
+  }
+  return ok;
 }
 
 bool green_build_device(const GridPlan& g, const double2* tw64[3], const int* pos[3], double d,

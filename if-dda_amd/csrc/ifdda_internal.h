@@ -44,6 +44,7 @@ template <typename V> struct MatvecDev {
   int alpha_mode;
   PassTimer* timer = nullptr;
   bool specialized = true;  // use compile-time kernels where built
+  int dpar = 0;             // G is d T / d r_(dpar-1) (odd along that axis), 0: T
   // batch of right-hand sides: nrhs vectors vstride elements apart, A and B
   // sized for nrhs, skip flags skip_stride ints apart
   int nrhs = 1;
@@ -149,6 +150,10 @@ bool green_build_device(const GridPlan& g, const double2* tw64[3], const int* po
 // Fold one caller-supplied FFT'd component already in `full` into g64/g32.
 bool green_fold_component(const GridPlan& g, const double2* full, int comp, double2* g64,
                           float2* g32, hipStream_t s);
+// The folded FFT of d T / d r_dd (dd = 0, 1, 2) for the optical force
+// (derivativefield2.f:22-76), FP64 only; full: L-grid scratch.
+bool green_dbuild_device(const GridPlan& g, const double2* tw64[3], const int* pos[3], double d,
+                         double k0, int dd, double2* full, double2* g64, hipStream_t s);
 
 }  // namespace ifdda
 
@@ -189,6 +194,9 @@ struct ifdda_ctx {
   double green_sample[6][2 * 16] = {};
   uint64_t tab_hash = 0;
   bool tab_dense = false;
+  double2* Gd64 = nullptr;  // folded d T / d x, d y, d z (force), 3 * F2*F3*F1*6
+  double gd_d = 0.0, gd_k0 = 0.0;  // the lattice and wavenumber Gd64 was built for
+  bool gd_ready = false;            // cleared with every new Green tensor
   ifdda::PassTimer timer;  // enabled by ifdda_timing_enable
   bool specialized = true;  // ifdda_use_specialized
 

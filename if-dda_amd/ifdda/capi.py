@@ -130,6 +130,7 @@ def lib():
                                         C.POINTER(C.c_int), C.POINTER(SolveInfo), ip, C.POINTER(C.c_int)]
     L.ifdda_specialized_z.argtypes = [vp, ip, C.POINTER(C.c_int)]
     L.ifdda_timing_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.ifdda_force_device.argtypes = [vp, ip, vp, vp, vp, pd, dp, dp, pd, pd, vp, vp, vp]
     _lib = L
     return L
 
@@ -314,6 +315,36 @@ class Solver:
                                                  C.c_void_p(stream) if stream else None),
                "ifdda_cross_sections_device")
         return out.reshape(R, 3)
+
+    def force(self, E_inc, FF, FFloc, kvec, d, k0, per_dipole=False):
+        """Optical force and torque (ifdda_force_device) for R solved RHS given
+        as host arrays [R, 3 nd] (or [3 nd]); kvec [R, 3].  Returns dict force
+        [R, 3] (N), torque [R, 3] (N m) and, with per_dipole, fdip / tdip
+        [R, nd, 3]."""
+        import torch
+        E = np.atleast_2d(_c128(E_inc))
+        R = E.shape[0]
+        dev = [torch.from_numpy(np.ascontiguousarray(np.atleast_2d(_c128(a))).view(np.float64).copy()).cuda()
+               for a in (E_inc, FF, FFloc)]
+        kv = np.ascontiguousarray(np.broadcast_to(np.asarray(kvec, np.float64).reshape(-1, 3), (R, 3)))
+        fo, to = np.zeros(3 * R), np.zeros(3 * R)
+        fd = td = None
+        if per_dipole:
+            fd = torch.zeros(R * 3 * self.nd, dtype=torch.float64, device="cuda")
+            td = torch.zeros_like(fd)
+        D = C.POINTER(C.c_double)
+        _check(lib().ifdda_force_device(self.h, R, *[C.c_void_p(t.data_ptr()) for t in dev],
+                                        kv.ctypes.data_as(D), float(d), float(k0), fo.ctypes.data_as(D),
+                                        to.ctypes.data_as(D),
+                                        C.c_void_p(fd.data_ptr()) if fd is not None else None,
+                                        C.c_void_p(td.data_ptr()) if td is not None else None, None),
+               "ifdda_force_device")
+        out = dict(force=fo.reshape(R, 3), torque=to.reshape(R, 3))
+        if per_dipole:
+            torch.cuda.synchronize()
+            out["fdip"] = fd.cpu().numpy().reshape(R, self.nd, 3)
+            out["tdip"] = td.cpu().numpy().reshape(R, self.nd, 3)
+        return out
 
     @staticmethod
     def farfield_size(nfft2d, k0, d):

@@ -263,6 +263,20 @@ int ifdda_cross_sections_device(ifdda_ctx *ctx, int nrhs, const void *FF0, const
                                 const void *FFloc, double k0, const double *I0, double *out,
                                 void *stream);
 
+/* Optical force and torque on every dipole (cdmlib.f:3102-3560, nforce /
+ * ntorque) for R solved right-hand sides in the layout of
+ * ifdda_solve_multi_device (device complex128: E_inc, FF = alpha E, FFloc =
+ * E), each a plane or circular wave of wave vector kvec[3r..3r+2] (1/m, host):
+ * the incident derivative is i k E_inc (ondeplaned.f, ondecirced.f), the
+ * scattered one the FFT convolution of derivativefield2.f with d T / dx
+ * (built on first use for this d, k0 and kept with the context).
+ * force[3r..3r+2], torque[3r..3r+2] (host): totals in N and N m, the torque
+ * about the mean dipole position; fdip, tdip: optional device double arrays
+ * of R * 3 * nd per-dipole values (tdip needs fdip), or NULL. */
+int ifdda_force_device(ifdda_ctx *ctx, int nrhs, const void *E_inc, const void *FF,
+                       const void *FFloc, const double *kvec, double d, double k0, double *force,
+                       double *torque, void *fdip, void *tdip, void *stream);
+
 /* Far field by the quick 2D-FFT method of diffractefft2d.f:1-183 for the
  * dipole moments FF (device, dipole order; the context's dipole map places
  * them in the box): lattice spacing d, (x0, y0, z0) = centre of box cell

@@ -179,6 +179,44 @@ class Ref:
             self.lib.greencalculfft_(*common, _rd(d), _rd(k0), *gp, _ri8())
         return np.stack(G).reshape(6, nz2, ny2, nx2)
 
+    def propder(self, r_obs, r_src, k0, test):
+        """propesplibdermat_: d T / d x_test at r_obs for a source at r_src."""
+        S = np.zeros((3, 3, 3), np.complex128, order="F")
+        a = [C.c_double(float(v)) for v in (*r_obs, *r_src, k0)]
+        self.lib.propesplibdermat_(*[C.byref(x) for x in a], _ri(test), S.ctypes.data_as(C.c_void_p))
+        return np.array(S[:, :, test - 1])
+
+    def dfield(self, FF, tabdip, nx, ny, nz, d, k0, test):
+        """calculforcefft_ then derivativefield2_: the x_test derivative of the
+        field radiated by the moments FF (dipole order), on every box cell
+        (3 nx ny nz values, box order)."""
+        nx2, ny2, nz2 = 2 * nx, 2 * ny, 2 * nz
+        M = nx2 * ny2 * nz2
+        nmax = nx * ny * nz
+        V = self._vect(M)
+        D = self._vect(M) + self._vect(M)
+        FFp = np.zeros(3 * nmax, np.complex128)
+        FFp[: FF.size] = FF
+        tab = np.ascontiguousarray(tabdip, np.int32)
+        P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+        dims = [_ri(nx), _ri(ny), _ri(nz), _ri(nx2), _ri(ny2), _ri(nz2), _ri(nx2 * ny2)]
+        self.lib.calculforcefft_(*dims, _ri(nx), _ri(ny), _ri(nz), P(tab), *[P(v) for v in V],
+                                 P(FFp), _ri8())
+        out = np.zeros(3 * nmax, np.complex128)
+        self.lib.derivativefield2_(_rd(d), _rd(k0), _ri(nx), _ri(ny), _ri(nz), _ri(nx2), _ri(ny2),
+                                   _ri(nz2), _ri(nx2 * ny2), _ri(M), _ri(M), _ri(nmax),
+                                   *[P(x) for x in D], *[P(v) for v in V], _ri(test), P(out),
+                                   _ri8(), _ri8())
+        return out
+
+    def ondeplaned(self, r, k0, E0, ss, pp, theta, phi):
+        """ondeplaned_ (test 4): Eder[c, d] = d E_c / d x_d of the plane wave."""
+        Eder = np.zeros((3, 3), np.complex128, order="F")
+        e0 = (C.c_double * 2)(E0.real, E0.imag)
+        self.lib.ondeplaned_(*[_rd(v) for v in r], _rd(k0), C.byref(e0), _rd(ss), _rd(pp),
+                             _rd(theta), _rd(phi), _ri(4), Eder.ctypes.data_as(C.c_void_p))
+        return np.array(Eder)
+
     def _vect(self, M):
         return [np.zeros(M, np.complex128) for _ in range(3)]
 

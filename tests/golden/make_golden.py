@@ -26,6 +26,12 @@ Fixtures (inputs and the reference's outputs, nothing else):
                    (`make_golden.py gp`; run with OMP_NUM_THREADS=1: the
                    reference's GPBICOR gives run-to-run differences of ~1e-5
                    on several OpenMP threads, deterministic on one)
+  force.npz        optical force and torque: calculforcefft_ + derivativefield2_
+                   (d/dx, d/dy, d/dz of the scattered field) on a seeded 6x5x4
+                   box with seeded moments, incident and local fields, and on
+                   config 1 solved by inverserig_ (GPBICG1, tol 1e-8); the
+                   force and torque of cdmlib.f:3102-3560 restated in
+                   oracle/force.py on those derivatives (`make_golden.py force`)
   microscope.npz   diffractefft2dlens_ (nside +1 and -1, two NA) on the seeded
                    6x5x4 box and fouriertoimage_ on seeded 32 x 32 planes
   farfield.npz     diffractefft2d_ and computegcfft2d_: a seeded box-ordered
@@ -190,9 +196,47 @@ def solve_more(r, methods=("CG", "BICGSTAB"), name="solve_more.npz", project=Fal
         np.savez_compressed(os.path.join(HERE, name), tabdip=s.tabdip, E_inc=E, **res)
 
 
+def force(r):
+    import force as fo
+    out = {}
+    nx, ny, nz = 6, 5, 4
+    tab, ffx, ffb, _, _ = seeded_case(nx, ny, nz, 21)
+    nd = int(tab.max())
+    rng = np.random.default_rng(22)
+    Einc = rng.standard_normal(3 * nd) + 1j * rng.standard_normal(3 * nd)
+    th, ph = np.radians(20.0), np.radians(40.0)
+    kvec = K0 * np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+    cellidx = np.nonzero(tab)[0][np.argsort(tab[tab > 0])]
+    cells = np.stack([cellidx % nx, (cellidx // nx) % ny, cellidx // (nx * ny)], 1)
+    dEs = [r.dfield(ffx, tab, nx, ny, nz, D, K0, t + 1).reshape(-1, 3)[cellidx] for t in range(3)]
+    fdip, tdip, F, tau = fo.force_torque(dEs, Einc, ffx, ffb, kvec, K0, cells, D)
+    out.update(box_tab=tab, box_FF=ffx, box_FFloc=ffb, box_Einc=Einc, box_kvec=kvec,
+               box_dE=np.stack(dEs), box_fdip=fdip, box_tdip=tdip, box_F=F, box_tau=tau)
+    # config 1, solved by the reference
+    s, k0, alpha, E = scene.config1()
+    Gs = r.green_fft(s.nx, s.ny, s.nz, s.d, k0)
+    pol = np.zeros((s.nx * s.ny * s.nz, 3, 3), np.complex128)
+    for a in range(3):
+        pol[: s.nd, a, a] = alpha
+    o = r.inverserig(Gs, s.tabdip, s.nx, s.ny, s.nz, pol, E, "GPBICG1", 1e-8)
+    assert o["nstop"] == 0, o
+    ci = np.nonzero(s.tabdip)[0][np.argsort(s.tabdip[s.tabdip > 0])]
+    cells = np.stack([ci % s.nx, (ci // s.nx) % s.ny, ci // (s.nx * s.ny)], 1)
+    dEs = [r.dfield(o["FF"], s.tabdip, s.nx, s.ny, s.nz, s.d, k0, t + 1).reshape(-1, 3)[ci]
+           for t in range(3)]
+    fdip, tdip, F, tau = fo.force_torque(dEs, E, o["FF"], o["FFloc"], np.array([0.0, 0.0, k0]), k0,
+                                         cells, s.d)
+    print("config 1 force", F, "torque", tau, "nloop", o["nloop"])
+    out.update(cfg1_F=F, cfg1_tau=tau, cfg1_fz_dip_sum_abs=np.abs(fdip[:, 2]).sum(),
+               cfg1_nloop=o["nloop"], cfg1_tol1=o["tol1"])
+    np.savez_compressed(os.path.join(HERE, "force.npz"), **out)
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "more":
         solve_more(Ref())
+    elif len(sys.argv) > 1 and sys.argv[1] == "force":
+        force(Ref())
     elif len(sys.argv) > 1 and sys.argv[1] == "gp":
         solve_more(Ref(), ("GPBICG2", "GPBICGsafe", "GPBICGplus", "GPBICGAR", "BICGSTARPLUS",
                            "GPBICOR", "CORS", "QMRCLA"), "solve_gp.npz", project=True)
