@@ -214,7 +214,10 @@ extern "C" int ifdda_microsdf_incidences(ifdda_ctx* c, const ifdda_microscope_op
 
 // The brightfield (microsbf.f) and darkfield (microsdf.f) microscopes: they
 // differ in the incidences (lattice in the condenser disk / ring on its
-// cone) and in how the direct wave enters the pupil (below).
+// cone) and in how the direct wave enters the pupil (below).  nside = -1
+// runs their reflection forms microsbfneg / microsdfneg (microssansinc.f,
+// what cdmlib.f:3737-3796 calls for that side): the same incidences, the
+// lens on z < 0, no incident wave, so images 3..5 equal 0..2.
 static int microscope_run(ifdda_ctx* c, const ifdda_solve_opts* o, const ifdda_microscope_opts* m,
                           double* images, int* ninc, ifdda_solve_info* infos, int infos_cap,
                           volatile int* nstop, bool dark) {
@@ -263,9 +266,12 @@ static int microscope_run(ifdda_ctx* c, const ifdda_solve_opts* o, const ifdda_m
       const double2 zf = make_double2(std::cos(kz * m->zlens), std::sin(kz * m->zlens));
       double Rm[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, sc = 1.0;
       if (t != 0.0) {
-        const double u0 = kx / k0, u1v = ky / k0, u2 = std::sqrt(1.0 - u0 * u0 - u1v * u1v);
+        // nside = -1 (microsbfneg / microsdfneg): u_z and v_z point to z < 0
+        // (microssansinc.f:494-497)
+        const double u0 = kx / k0, u1v = ky / k0,
+                     u2 = std::sqrt(1.0 - u0 * u0 - u1v * u1v) * m->nside;
         const double v0 = -kx / k0 / m->gross, v1 = -ky / k0 / m->gross,
-                     v2 = std::sqrt(1.0 - v0 * v0 - v1 * v1);
+                     v2 = std::sqrt(1.0 - v0 * v0 - v1 * v1) * m->nside;
         double a1 = u1v * v2 - u2 * v1, a2 = -u0 * v2 + u2 * v0;
         const double co = u0 * v0 + u1v * v1 + u2 * v2, si = std::sqrt(a1 * a1 + a2 * a2);
         a1 /= si, a2 /= si;
@@ -353,7 +359,10 @@ static int microscope_run(ifdda_ctx* c, const ifdda_solve_opts* o, const ifdda_m
         ik = (int)(q.kx / dkb), jk = (int)(q.ky / dkb);
       hbin[r] = -1;
       const double bx = dk * ik, by = dk * jk;
-      if (std::abs(ik) <= imax && std::abs(jk) <= imax && bx * bx + by * by <= nak * nak) {
+      // the objective on the far side (nside = -1) sees no incident wave:
+      // microsbfneg / microsdfneg form the scattered images only
+      if (m->nside == 1 && std::abs(ik) <= imax && std::abs(jk) <= imax &&
+          bx * bx + by * by <= nak * nak) {
         hbin[r] = (jk + imax) * n + (ik + imax);
         double2 eb[3];
         double kzb;

@@ -330,7 +330,10 @@ typedef struct {
   double numaperinc;  /* condenser NA, 0 < NA < 1                              */
   double gross;       /* magnification                                         */
   double zlens;       /* position of the focal plane [m]                       */
-  int nside;          /* 1: objective on the z > 0 side, -1: z < 0             */
+  int nside;          /* 1: objective on the z > 0 side (microsbf.f,
+                         microsdf.f); -1: z < 0, the reflection microscopes
+                         microsbfneg / microsdfneg of microssansinc.f
+                         (scattered light only: images 3..5 equal 0..2)    */
   int npolainc;       /* 0: both polarisations (pp = 0 then 1), 1: pp = 0, 2: pp = 1 */
   double E0[2];       /* amplitude of each illumination (re, im), as microsbf.f:205-209 */
   double k0;          /* wave number [1/m]                                     */

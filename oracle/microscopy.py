@@ -172,13 +172,17 @@ def incidences_df(k0, numaperinc, d, nfft2d, xs, ys):
 
 
 def image_assembly(Elens_list, inc_list, k0, E0, numaper, zlens, gross, dkinc, imax, dk, nfft2d,
-                   dark=False):
+                   dark=False, nside=1):
     """microsbf.f:676-845: for each (pupil field, (idelta, jdelta, kxinc, kyinc,
     pp)) fill the pupil (focus phase; the incident wave at its bin), rotate by
     the lens, fouriertoimage, add |E|^2; returns the six sqrt-images
     (scattered x, y, z, with incident x, y, z) as [y][x] arrays.  dark: the
     darkfield's incident wave (microsdf.f:668-717): its bin from the pupil
-    step dk, added as i E / dk^2 on every component."""
+    step dk, added as i E / dk^2 on every component.  nside = -1: the
+    objective on the z < 0 side (microssansinc.f, microsbfneg / microsdfneg:
+    no incident wave in the pupil, so the with-incident images equal the
+    scattered ones; the lens rotation takes u_z, v_z times nside,
+    microssansinc.f:494-497, 1397-1400)."""
     N = nfft2d
     acc = [np.zeros((N, N)) for _ in range(6)]
     iv = np.arange(-imax, imax + 1)
@@ -199,7 +203,7 @@ def image_assembly(Elens_list, inc_list, k0, E0, numaper, zlens, gross, dkinc, i
                 ix, iy = i % N, j % N
                 s = El[a, b] * zf
                 w = s.copy()
-                if i == ikx and j == jky:
+                if i == ikx and j == jky and nside == 1:
                     e = ondeplanekxky(np.zeros((1, 3)), k0, E0, pp, kx, ky)
                     if dark:
                         w = w + np.array([e[0], e[1], e[2]]) / (dk * dk) * zf * 1j
@@ -207,9 +211,9 @@ def image_assembly(Elens_list, inc_list, k0, E0, numaper, zlens, gross, dkinc, i
                         w = w + np.array([e[0] / (dk * dk), e[1] / (dk * dk), e[2] / dk]) * zf
                 if t != 0.0:
                     u = np.array([kx / k0, ky / k0, 0.0])
-                    u[2] = np.sqrt(1.0 - u[0] ** 2 - u[1] ** 2)
+                    u[2] = np.sqrt(1.0 - u[0] ** 2 - u[1] ** 2) * nside
                     v = np.array([-kx / k0 / gross, -ky / k0 / gross, 0.0])
-                    v[2] = np.sqrt(1.0 - v[0] ** 2 - v[1] ** 2)
+                    v[2] = np.sqrt(1.0 - v[0] ** 2 - v[1] ** 2) * nside
                     u1 = u[1] * v[2] - u[2] * v[1]
                     u2 = -u[0] * v[2] + u[2] * v[0]
                     co = u @ v

@@ -65,7 +65,7 @@ def _cpu_images(s, alpha, oracle, m, tol, dark=False):
                                              s.origin[0], s.origin[1], zs, s.d, m.numaper * K0, m.nside)
         lens.append(El)
     return mi.image_assembly(lens, waves, K0, E0, m.numaper * K0, m.zlens, m.gross, dkinc, imax, dk,
-                             m.nfft2d, dark=dark)
+                             m.nfft2d, dark=dark, nside=m.nside)
 
 
 def test_brightfield_images_match_cpu_pipeline(small, oracle):
@@ -132,3 +132,28 @@ def test_darkfield_images_match_cpu_pipeline(small, oracle, na, nainc):
         assert np.array_equal(img[:3], img[3:])
     else:
         assert np.abs(img[3:] - img[:3]).max() > 0.1 * np.abs(img[:3]).max()
+
+
+@pytest.mark.parametrize("dark", [False, True])
+def test_reflection_microscopes_match_cpu_pipeline(small, oracle, dark):
+    """nside = -1: microsbfneg / microsdfneg (microssansinc.f), the objective
+    on the z < 0 side: the lens on that side, u_z and v_z of the lens
+    rotation negative, no incident wave (total image = scattered image)."""
+    s, alpha, sv, _ = small
+    m = sv.microscope_opts(128, 0.8, 0.5, K0, s.d, s.origin, gross=100.0, zlens=-50e-9, nside=-1,
+                           npolainc=0, E0=1.0 + 0.5j)
+    print(f"reflection microscope dark={dark}", flush=True)
+    img, infos = (sv.microsdf(m, method="GPBICG1", tol=1e-10, precision=64) if dark else
+                  sv.microsbf(m, "GPBICG1", 1e-10, precision=64))
+    assert all(i["nstop"] == 0 for i in infos)
+    ref = _cpu_images(s, alpha, oracle, m, 1e-10, dark=dark)
+    for c in range(3):
+        scale = np.abs(ref[c]).max()
+        assert scale > 0
+        assert np.abs(img[c] - ref[c]).max() < 1e-7 * scale, (c, np.abs(img[c] - ref[c]).max() / scale)
+    assert np.array_equal(img[:3], img[3:])
+    # the light scattered back differs from the light scattered forward
+    fwd = _cpu_images(s, alpha, oracle, sv.microscope_opts(128, 0.8, 0.5, K0, s.d, s.origin, gross=100.0,
+                                                           zlens=50e-9, nside=1, npolainc=0,
+                                                           E0=1.0 + 0.5j), 1e-10, dark=dark)
+    assert np.abs(fwd[0] - ref[0]).max() > 0.01 * np.abs(fwd[0]).max()

@@ -74,3 +74,30 @@ def test_darkfield_ring(nainc, N):
     assert np.allclose(phi, 2 * np.pi * np.arange(m) / m, atol=1e-12)
     assert [r[1] for r in ring] == [0.0] * m + [1.0] * m
     assert np.array_equal(kx[:m], kx[m:]) and np.array_equal(ky[:m], ky[m:])
+
+
+def test_reflection_image_mirror_symmetry():
+    """A dipole in the z = 0 plane radiates mirror images into z > 0 and
+    z < 0 (E_x, E_y equal, E_z opposite).  The reflection microscope
+    (nside = -1, lens rotation with u_z, v_z < 0) must then give the
+    transmission microscope's scattered images exactly, component by
+    component, with the focal plane on the dipole (zlens = 0: the focus
+    phase exp(i kz zlens) is not mirrored, microssansinc.f:487); a lens
+    rotation that ignored the side would not."""
+    import microscopy as mi
+    N, d = 64, 20e-9
+    p = np.array([1.0 + 0.3j, -0.4 + 0.2j, 0.0])
+    FFbox = p.copy()  # one cell at the origin
+    imgs = {}
+    for side in (1, -1):
+        El, imax, dk = mi.diffractefft2dlens(FFbox, 1, 1, 1, N, K0, 0.0, 0.0, np.array([0.0]), d,
+                                             0.9 * K0, side)
+        inc = [(0, 0, 0.0, 0.0, 0.0)]
+        imgs[side] = mi.image_assembly([El], inc, K0, 0.0, 0.9 * K0, 0.0, 50.0, dk, imax, dk, N,
+                                       nside=side)
+    for c in range(3):
+        scale = np.abs(imgs[1][c]).max()
+        assert scale > 0
+        assert np.abs(imgs[1][c] - imgs[-1][c]).max() < 1e-12 * scale
+    # without the incident wave (E0 = 0) both keep total = scattered
+    assert np.array_equal(imgs[-1][0], imgs[-1][3])
