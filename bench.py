@@ -238,6 +238,22 @@ def main():
                            "min_bytes": mins, "GB_per_s_min_bytes": mins / tm / 1e9,
                            "frac_min_bytes": mins / tm / 1e9 / HBM_PEAK_GBS}
 
+    # optical force and torque of the last (FP64) field: the first call also
+    # builds the three derivative Green tensors, the second reuses them
+    th = np.radians(30.0)
+    kvec = [k0 * np.sin(th), 0.0, k0 * np.cos(th)]
+    ft = []
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        Fz, Tq = sv.force_device(1, Ed.data_ptr(), FFd.data_ptr(), FLd.data_ptr(), kvec, s.d, k0)
+        torch.cuda.synchronize()
+        ft.append(time.perf_counter() - t1)
+    force = {"first_call_s": ft[0], "cached_s": ft[1], "force_N": Fz[0].tolist(),
+             "torque_Nm": Tq[0].tolist(),
+             "what": "cdmlib nforce/ntorque on the config-2 FP64 field: 3 derivative convolutions "
+                     "(200^3 grid) + per-dipole force and torque"}
+
     # dominant kernel of the timed region (FP32 passes dominate mixed solves)
     names = ["P1 scatter+FFT-x", "P2 FFT-y", "P3 FFT-z.G.IFFT-z", "P4 IFFT-y", "P5 IFFT-x+gather"]
     zk = (f"ifdda::k_zconv_ct<float2, {sv.L[2]}," if sv.specialized_z(32) else "ifdda::k_zconv<float2,")
@@ -312,6 +328,7 @@ def main():
         "field_reference": {"method": "GPBICG1", "precision": "fp64", "tol": 1e-8,
                             "tol1": info_ref["tol1"], "iterations": info_ref["nloop"]},
         "green_setup_s": green_s,
+        "force": force,
         "passes": passes,
         "roofline": {"bound": "hbm", "kernel": f"{names[p_dom]} ({kern[p_dom]})",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -316,6 +316,21 @@ class Solver:
                "ifdda_cross_sections_device")
         return out.reshape(R, 3)
 
+    def force_device(self, R, Einc_ptr, FF_ptr, FFloc_ptr, kvec, d, k0, fdip_ptr=None, tdip_ptr=None,
+                     stream=None):
+        """ifdda_force_device on device arrays; returns (force [R, 3], torque [R, 3])."""
+        kv = np.ascontiguousarray(np.broadcast_to(np.asarray(kvec, np.float64).reshape(-1, 3), (R, 3)))
+        fo, to = np.zeros(3 * R), np.zeros(3 * R)
+        D = C.POINTER(C.c_double)
+        _check(lib().ifdda_force_device(self.h, int(R), C.c_void_p(Einc_ptr), C.c_void_p(FF_ptr),
+                                        C.c_void_p(FFloc_ptr), kv.ctypes.data_as(D), float(d), float(k0),
+                                        fo.ctypes.data_as(D), to.ctypes.data_as(D),
+                                        C.c_void_p(fdip_ptr) if fdip_ptr else None,
+                                        C.c_void_p(tdip_ptr) if tdip_ptr else None,
+                                        C.c_void_p(stream) if stream else None),
+               "ifdda_force_device")
+        return fo.reshape(R, 3), to.reshape(R, 3)
+
     def force(self, E_inc, FF, FFloc, kvec, d, k0, per_dipole=False):
         """Optical force and torque (ifdda_force_device) for R solved RHS given
         as host arrays [R, 3 nd] (or [3 nd]); kvec [R, 3].  Returns dict force
