@@ -254,6 +254,16 @@ def main():
              "what": "cdmlib nforce/ntorque on the config-2 FP64 field: 3 derivative convolutions "
                      "(200^3 grid) + per-dipole force and torque"}
 
+    # effective solve bandwidth: the matvecs' algorithmic bytes plus the
+    # GPBICG vector streams (31 streams of 3 Nd per iteration, each written
+    # or read once); the mixed solve's FP64 checks are its outer sweeps
+    mvb64 = sum(alg_pass_bytes(s.nx, s.ny, s.nz, sv.L, nd, 16))
+    mvb32 = sum(alg_pass_bytes(s.nx, s.ny, s.nz, sv.L, nd, 8))
+    solve_bytes64 = info64["ncompte"] * mvb64 + info64["nloop"] * 31 * 3 * nd * 16
+    n64_mixed = max(1, info32["outer"])
+    solve_bytes32 = ((info32["ncompte"] - n64_mixed) * mvb32 + n64_mixed * mvb64
+                     + info32["nloop"] * 31 * 3 * nd * 8)
+
     # dominant kernel of the timed region (FP32 passes dominate mixed solves)
     names = ["P1 scatter+FFT-x", "P2 FFT-y", "P3 FFT-z.G.IFFT-z", "P4 IFFT-y", "P5 IFFT-x+gather"]
     zk = (f"ifdda::k_zconv_ct<float2, {sv.L[2]}," if sv.specialized_z(32) else "ifdda::k_zconv<float2,")
@@ -317,12 +327,14 @@ def main():
                   "field_rel_err_vs_fp64_1e-4": field_err,
                   "field_rel_err_vs_converged": err32,
                   "latency_ms": {"p50": float(np.percentile(lat, 50)) * 1e3,
-                                 "p99": float(np.percentile(lat, 99)) * 1e3}},
+                                 "p99": float(np.percentile(lat, 99)) * 1e3},
+                  "effective_GB_per_s": solve_bytes32 / (elapsed / args.steps) / 1e9},
         "fp64": {"solves_per_s": 1.0 / float(np.median(lat64)), "iterations": info64["nloop"],
                  "matvecs": info64["ncompte"], "tol1": info64["tol1"],
                  "field_rel_err_vs_converged": err64,
                  "latency_ms": {"p50": float(np.percentile(lat64, 50)) * 1e3,
-                                "p99": float(np.percentile(lat64, 99)) * 1e3}},
+                                "p99": float(np.percentile(lat64, 99)) * 1e3},
+                 "effective_GB_per_s": solve_bytes64 / float(np.median(lat64)) / 1e9},
         "mixed_speedup_vs_fp64": (args.steps / elapsed) * float(np.median(lat64)),
         "matvec": mv,
         "field_reference": {"method": "GPBICG1", "precision": "fp64", "tol": 1e-8,
