@@ -650,6 +650,76 @@ extern "C" void ifdda_produitfftmatvectopt(
   matvec_entry("produitfftmatvectopt", G, nullptr, *nx, *ny, *nz, *nx2, *ny2, *nz2, FFX, FFB);
 }
 
+// The optical-force FFTs (calculforcefft.f / calculforcefftopt.f, then
+// derivativefield2.f, as cdmlib.f:3159-3300 calls them): the first keeps the
+// moments, placed in box order, for the box; the second forms one derivative
+// of their field at every box cell on the device.  The caller's vect and
+// DFFTTENSOR arrays are scratch there and are left alone here.
+static std::mutex g_mom_mu;
+static std::map<ifdda_ctx*, std::vector<double>> g_mom;
+
+static void force_moments(const char* who, int nx, int ny, int nz, const int32_t* Tabdip,
+                          const double* FF) {
+  ifdda_ctx* c = ctx_for(nx, ny, nz);
+  if (!c) return report(who);
+  const size_t N = (size_t)nx * ny * nz;
+  std::vector<double> m(6 * N, 0.0);
+  for (size_t i = 0; i < N; ++i) {
+    const long long j = Tabdip ? (long long)Tabdip[i] - 1 : (long long)i;
+    if (j < 0) continue;
+    for (int q = 0; q < 6; ++q) m[6 * i + q] = FF[6 * j + q];
+  }
+  std::lock_guard<std::mutex> lk(g_mom_mu);
+  g_mom[c] = std::move(m);
+}
+
+extern "C" void ifdda_calculforcefft(const int32_t* nx, const int32_t* ny, const int32_t* nz,
+                                     const int32_t* nx2, const int32_t* ny2, const int32_t* nz2,
+                                     const int32_t* nxy2, const int32_t* nxm, const int32_t* nym,
+                                     const int32_t* nzm, const int32_t* Tabdip, double* vectx,
+                                     double* vecty, double* vectz, const double* FF,
+                                     const int64_t* planb) {
+  (void)nx2; (void)ny2; (void)nz2; (void)nxy2; (void)nxm; (void)nym; (void)nzm;
+  (void)vectx; (void)vecty; (void)vectz; (void)planb;
+  force_moments("calculforcefft", *nx, *ny, *nz, Tabdip, FF);
+}
+
+extern "C" void ifdda_calculforcefftopt(const int32_t* nx, const int32_t* ny, const int32_t* nz,
+                                        const int32_t* nx2, const int32_t* ny2, const int32_t* nz2,
+                                        const int32_t* nxy2, const int32_t* nxm, const int32_t* nym,
+                                        const int32_t* nzm, double* vectx, double* vecty,
+                                        double* vectz, const double* FF, const int64_t* planb) {
+  (void)nx2; (void)ny2; (void)nz2; (void)nxy2; (void)nxm; (void)nym; (void)nzm;
+  (void)vectx; (void)vecty; (void)vectz; (void)planb;
+  force_moments("calculforcefftopt", *nx, *ny, *nz, nullptr, FF);
+}
+
+extern "C" void ifdda_derivativefield2(
+    const double* aretecube, const double* k0, const int32_t* nx, const int32_t* ny,
+    const int32_t* nz, const int32_t* nx2, const int32_t* ny2, const int32_t* nz2,
+    const int32_t* nxy2, const int32_t* ntotal, const int32_t* ntotalm, const int32_t* nmax,
+    double* Dxx, double* Dxy, double* Dxz, double* Dyy, double* Dyz, double* Dzz, double* vectx,
+    double* vecty, double* vectz, const int32_t* test, double* FF0, const int64_t* planb,
+    const int64_t* planf) {
+  (void)nx2; (void)ny2; (void)nz2; (void)nxy2; (void)ntotal; (void)ntotalm; (void)nmax;
+  (void)Dxx; (void)Dxy; (void)Dxz; (void)Dyy; (void)Dyz; (void)Dzz;
+  (void)vectx; (void)vecty; (void)vectz; (void)planb; (void)planf;
+  ifdda_ctx* c = ctx_for(*nx, *ny, *nz);
+  if (!c) return report("derivativefield2");
+  std::vector<double> m;
+  {
+    std::lock_guard<std::mutex> lk(g_mom_mu);
+    auto it = g_mom.find(c);
+    if (it == g_mom.end()) {
+      set_error("derivativefield2: call calculforcefft for this box first");
+      return report("derivativefield2");
+    }
+    m = it->second;
+  }
+  if (derivative_field_box(c, m.data(), *aretecube, *k0, *test - 1, FF0))
+    report("derivativefield2");
+}
+
 static void green_entry(const char* who, int nx, int ny, int nz, int nx2, int ny2, int nz2,
                         int nquad, double tolinit, double d, double k0, double* const G[6]) {
   ifdda_ctx* c = ctx_for(nx, ny, nz);

@@ -167,6 +167,42 @@ bool ensure_dgreen(ifdda_ctx* c, double d, double k0, hipStream_t s) {
 }
 
 }  // namespace
+
+int derivative_field_box(ifdda_ctx* c, const double* mom, double d, double k0, int dd, double* out) {
+  if (!(d > 0.0) || !(k0 > 0.0) || dd < 0 || dd > 2) {
+    set_error("derivativefield2: need d > 0, k0 > 0 and test in 1..3");
+    return IFDDA_ERR_ARG;
+  }
+  hipStream_t s = c->stream;
+  if (!ensure_dgreen(c, d, k0, s)) return IFDDA_ERR_HIP;
+  if (ensure_scratch(c, 1) != IFDDA_OK) return IFDDA_ERR_HIP;
+  const long long n = 3LL * c->g.nx * c->g.ny * c->g.nz;
+  const size_t nf = (size_t)c->g.F[0] * c->g.F[1] * c->g.F[2] * 6;
+  double2* buf = nullptr;  // moments, zero, result
+  bool ok = hip_ok(hipMalloc(&buf, sizeof(double2) * n * 3), "hipMalloc derivative field") &&
+            hip_ok(hipMemcpyAsync(buf, mom, sizeof(double2) * n, hipMemcpyHostToDevice, s),
+                   "copy moments") &&
+            hip_ok(hipMemsetAsync(buf + n, 0, sizeof(double2) * n, s), "zero rhs");
+  if (ok) {
+    MatvecDev<double2> m = c->mv64();
+    m.tab = nullptr;  // every box cell: the moments come in box order
+    m.G = c->Gd64 + nf * dd;
+    m.alpha_mode = kAlphaNone;
+    m.dpar = dd + 1;
+    m.nrhs = 1;
+    m.vstride = n;
+    m.timer = nullptr;
+    ok = launch_matvec_f64(c->g, m, buf, buf + n, buf + 2 * n, s, nullptr) &&
+         hip_ok(hipMemcpyAsync(out, buf + 2 * n, sizeof(double2) * n, hipMemcpyDeviceToHost, s),
+                "copy derivative field") &&
+         hip_ok(hipStreamSynchronize(s), "derivative field");
+  }
+  (void)hipFree(buf);
+  if (!ok) return IFDDA_ERR_HIP;
+  for (long long i = 0; i < 2 * n; ++i) out[i] = -out[i];  // y = 0 - conv / M
+  return IFDDA_OK;
+}
+
 }  // namespace ifdda
 
 using namespace ifdda;

@@ -319,4 +319,7 @@ size_t pack_member_bytes(const ifdda_job& j);
 size_t pack_green_bytes(const ifdda_job& j);
 // Grow the FFT scratch to hold R right-hand sides (ifdda_capi.hip).
 int ensure_scratch(ifdda_ctx* c, int R);
+// d E / d x_(dd) at every box cell of the field radiated by box-ordered
+// moments (host, 3 N complex each; derivativefield2.f on the device).
+int derivative_field_box(ifdda_ctx* c, const double* mom, double d, double k0, int dd, double* out);
 }  // namespace ifdda

@@ -2066,7 +2066,9 @@ int solve_core(const MvOp& op, Solver* S, const ifdda_solve_opts* o, int R, cons
     inf.status = run[r].status;
     inf.steperr = run[r].steperr;
     inf.nloop = nloop[r];
-    inf.ncompte = ncompte[r] + (final_mv ? 1 : 0);
+    // the final matvec runs for every RHS once one needs it; it counts only
+    // for the RHS whose last refinement check did not already give A x
+    inf.ncompte = ncompte[r] + (ax_ok[r] ? 0 : 1);
     inf.outer = o->precision == 32 ? outer[r] : 0;
     inf.seconds = secs;
     inf.tol1 = bn[r] > 0.0 ? e[r] / bn[r] : 0.0;

@@ -10,6 +10,9 @@
 !   greencalculquadfft   calculgreenquadfft.f:1-3
 !   inverserig           inverserig.f:1-6
 !   inverserigopt        inverserigopt.f:1-6
+!   calculforcefft       calculforcefft.f:1-2     (optical force, cdmlib.f:3159)
+!   calculforcefftopt    calculforcefftopt.f:1-2
+!   derivativefield2     derivativefield2.f:1-4
 !
 ! Each forwards its arguments by reference to the C entry point of the same
 ! name prefixed with ifdda_ (include/ifdda.h), which runs on the GPU.  The
@@ -39,6 +42,33 @@ module ifdda_capi
       integer(c_int32_t) :: ntotalm, ntotal, nmax, nxm, nym, nzm, nx, ny, nz, nx2, ny2, nxy2, &
            nz2
       integer(c_int64_t) :: planf, planb
+    end subroutine
+
+    subroutine c_forcefft(nx, ny, nz, nx2, ny2, nz2, nxy2, nxm, nym, nzm, tabdip, vx, vy, vz, &
+         ff, planb) bind(C, name="ifdda_calculforcefft")
+      import :: c_double_complex, c_int32_t, c_int64_t
+      integer(c_int32_t) :: nx, ny, nz, nx2, ny2, nz2, nxy2, nxm, nym, nzm, tabdip(*)
+      complex(c_double_complex) :: vx(*), vy(*), vz(*), ff(*)
+      integer(c_int64_t) :: planb
+    end subroutine
+
+    subroutine c_forcefftopt(nx, ny, nz, nx2, ny2, nz2, nxy2, nxm, nym, nzm, vx, vy, vz, ff, &
+         planb) bind(C, name="ifdda_calculforcefftopt")
+      import :: c_double_complex, c_int32_t, c_int64_t
+      integer(c_int32_t) :: nx, ny, nz, nx2, ny2, nz2, nxy2, nxm, nym, nzm
+      complex(c_double_complex) :: vx(*), vy(*), vz(*), ff(*)
+      integer(c_int64_t) :: planb
+    end subroutine
+
+    subroutine c_dfield(d, k0, nx, ny, nz, nx2, ny2, nz2, nxy2, ntotal, ntotalm, nmax, dxx, dxy, &
+         dxz, dyy, dyz, dzz, vx, vy, vz, test, ff0, planb, planf) &
+         bind(C, name="ifdda_derivativefield2")
+      import :: c_double, c_double_complex, c_int32_t, c_int64_t
+      real(c_double) :: d, k0
+      integer(c_int32_t) :: nx, ny, nz, nx2, ny2, nz2, nxy2, ntotal, ntotalm, nmax, test
+      complex(c_double_complex) :: dxx(*), dxy(*), dxz(*), dyy(*), dyz(*), dzz(*)
+      complex(c_double_complex) :: vx(*), vy(*), vz(*), ff0(*)
+      integer(c_int64_t) :: planb, planf
     end subroutine
 
     subroutine c_green(nx, ny, nz, nx2, ny2, nz2, nxy2, nxm, nym, nzm, d, k0, gxx, gxy, gxz, &
@@ -247,3 +277,44 @@ subroutine computegcfft2d(imax, deltakx, deltaky, k0, I0, theta, phi, nfft2d, Ed
   call c_computegcfft2d(imax, deltakx, deltaky, k0, I0, theta, phi, nfft2d, Ediffkzpos, &
        Ediffkzneg, gasym, Cscai)
 end subroutine computegcfft2d
+
+subroutine calculforcefft(nx, ny, nz, nx2, ny2, nz2, nxy2, nxm, nym, nzm, tabdip, vectx, vecty, &
+     vectz, FF, planb)
+  use ifdda_capi
+  implicit none
+  integer :: nx, ny, nz, nx2, ny2, nz2, nxy2, nxm, nym, nzm
+  integer, dimension(nxm*nym*nzm) :: tabdip
+  double complex, dimension(8*nxm*nym*nzm) :: vectx, vecty, vectz
+  double complex, dimension(3*nxm*nym*nzm) :: FF
+  integer*8 :: planb
+  call c_forcefft(nx, ny, nz, nx2, ny2, nz2, nxy2, nxm, nym, nzm, tabdip, vectx, vecty, vectz, &
+       FF, planb)
+end subroutine calculforcefft
+
+subroutine calculforcefftopt(nx, ny, nz, nx2, ny2, nz2, nxy2, nxm, nym, nzm, vectx, vecty, &
+     vectz, FF, planb)
+  use ifdda_capi
+  implicit none
+  integer :: nx, ny, nz, nx2, ny2, nz2, nxy2, nxm, nym, nzm
+  double complex, dimension(8*nxm*nym*nzm) :: vectx, vecty, vectz
+  double complex, dimension(3*nxm*nym*nzm) :: FF
+  integer*8 :: planb
+  call c_forcefftopt(nx, ny, nz, nx2, ny2, nz2, nxy2, nxm, nym, nzm, vectx, vecty, vectz, FF, &
+       planb)
+end subroutine calculforcefftopt
+
+subroutine derivativefield2(aretecube, k0, nx, ny, nz, nx2, ny2, nz2, nxy2, ntotal, ntotalm, &
+     nmax, DFFTTENSORxx, DFFTTENSORxy, DFFTTENSORxz, DFFTTENSORyy, DFFTTENSORyz, DFFTTENSORzz, &
+     vectx, vecty, vectz, test, FF0, planb, planf)
+  use ifdda_capi
+  implicit none
+  integer :: nx, ny, nz, nx2, ny2, nz2, nxy2, ntotal, ntotalm, nmax, test
+  double precision :: aretecube, k0
+  double complex :: DFFTTENSORxx(ntotalm), DFFTTENSORxy(ntotalm), DFFTTENSORxz(ntotalm), &
+       DFFTTENSORyy(ntotalm), DFFTTENSORyz(ntotalm), DFFTTENSORzz(ntotalm), vectx(ntotalm), &
+       vecty(ntotalm), vectz(ntotalm), FF0(3*nmax)
+  integer*8 :: planb, planf
+  call c_dfield(aretecube, k0, nx, ny, nz, nx2, ny2, nz2, nxy2, ntotal, ntotalm, nmax, &
+       DFFTTENSORxx, DFFTTENSORxy, DFFTTENSORxz, DFFTTENSORyy, DFFTTENSORyz, DFFTTENSORzz, vectx, &
+       vecty, vectz, test, FF0, planb, planf)
+end subroutine derivativefield2

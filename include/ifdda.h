@@ -263,6 +263,32 @@ int ifdda_cross_sections_device(ifdda_ctx *ctx, int nrhs, const void *FF0, const
                                 const void *FFloc, double k0, const double *I0, double *out,
                                 void *stream);
 
+/* Reference-shaped entries of the optical-force FFTs (the Fortran drop-in
+ * binds calculforcefft_, calculforcefftopt_ and derivativefield2_ to them):
+ * calculforcefft.f:1-2 / calculforcefftopt.f:1-2 keep the moments FF (dipole
+ * order with Tabdip, box order without) for the box; derivativefield2.f:1-4
+ * then writes FF0 (3 nx ny nz complex, box order) = the x_test derivative of
+ * their field at every box cell.  vect* and DFFTTENSOR* are accepted and left
+ * alone (scratch in the reference). */
+void ifdda_calculforcefft(const int32_t *nx, const int32_t *ny, const int32_t *nz,
+                          const int32_t *nx2, const int32_t *ny2, const int32_t *nz2,
+                          const int32_t *nxy2, const int32_t *nxm, const int32_t *nym,
+                          const int32_t *nzm, const int32_t *Tabdip, double *vectx, double *vecty,
+                          double *vectz, const double *FF, const int64_t *planb);
+void ifdda_calculforcefftopt(const int32_t *nx, const int32_t *ny, const int32_t *nz,
+                             const int32_t *nx2, const int32_t *ny2, const int32_t *nz2,
+                             const int32_t *nxy2, const int32_t *nxm, const int32_t *nym,
+                             const int32_t *nzm, double *vectx, double *vecty, double *vectz,
+                             const double *FF, const int64_t *planb);
+void ifdda_derivativefield2(const double *aretecube, const double *k0, const int32_t *nx,
+                            const int32_t *ny, const int32_t *nz, const int32_t *nx2,
+                            const int32_t *ny2, const int32_t *nz2, const int32_t *nxy2,
+                            const int32_t *ntotal, const int32_t *ntotalm, const int32_t *nmax,
+                            double *Dxx, double *Dxy, double *Dxz, double *Dyy, double *Dyz,
+                            double *Dzz, double *vectx, double *vecty, double *vectz,
+                            const int32_t *test, double *FF0, const int64_t *planb,
+                            const int64_t *planf);
+
 /* Optical force and torque on every dipole (cdmlib.f:3102-3560, nforce /
  * ntorque) for R solved right-hand sides in the layout of
  * ifdda_solve_multi_device (device complex128: E_inc, FF = alpha E, FFloc =

@@ -48,7 +48,7 @@ def test_fortran_dropin_exports_reference_names():
     syms = exported(FLIB)
     for n in ("produitfftmatvect3_", "produitfftmatvectopt_", "greencalculfft_",
               "greencalculquadfft_", "inverserig_", "inverserigopt_", "diffractefft2d_",
-              "computegcfft2d_"):
+              "computegcfft2d_", "calculforcefft_", "calculforcefftopt_", "derivativefield2_"):
         assert n in syms
 
 

@@ -90,3 +90,42 @@ def test_force_many_rhs_equal_single_calls(gpu):
             assert np.array_equal(one["fdip"][0], both["fdip"][r])
             assert np.array_equal(one["force"][0], both["force"][r])
             assert np.array_equal(one["torque"][0], both["torque"][r])
+
+
+def test_dropin_derivativefield2_matches_reference(gpu):
+    """calculforcefft_ then derivativefield2_ through libifdda_dropin.so, as
+    cdmlib.f:3159-3300 calls them: the derivative field at the dipole cells
+    equals the compiled reference's (force.npz), and at the other cells the
+    direct sum of oracle/force.py."""
+    import ctypes as C
+    import os
+    import force as fo
+    g = golden("force.npz")
+    tab = np.ascontiguousarray(g["box_tab"], np.int32)
+    nx, ny, nz = 6, 5, 4
+    N = nx * ny * nz
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    L = C.CDLL(os.path.join(root, "if-dda_amd", "libifdda_dropin.so"))
+    ri = lambda v: C.byref(C.c_int(int(v)))  # noqa: E731
+    P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    M = 8 * N
+    V = [np.zeros(M, np.complex128) for _ in range(3)]
+    Dg = [np.zeros(M, np.complex128) for _ in range(6)]
+    FF = np.zeros(3 * N, np.complex128)
+    FF[: g["box_FF"].size] = g["box_FF"]
+    plan = C.c_int64(0)
+    dims = [ri(nx), ri(ny), ri(nz), ri(2 * nx), ri(2 * ny), ri(2 * nz), ri(4 * nx * ny)]
+    L.calculforcefft_(*dims, ri(nx), ri(ny), ri(nz), P(tab), *[P(v) for v in V], P(FF), C.byref(plan))
+    ci = np.nonzero(tab)[0][np.argsort(tab[tab > 0])]
+    cells = np.stack([np.arange(N) % nx, (np.arange(N) // nx) % ny, np.arange(N) // (nx * ny)], 1)
+    for t in (1, 2, 3):
+        print(f"drop-in derivativefield2 test={t}", flush=True)
+        out = np.zeros(3 * N, np.complex128)
+        L.derivativefield2_(C.byref(C.c_double(D)), C.byref(C.c_double(K0)), ri(nx), ri(ny), ri(nz),
+                            ri(2 * nx), ri(2 * ny), ri(2 * nz), ri(4 * nx * ny), ri(M), ri(M), ri(N),
+                            *[P(x) for x in Dg], *[P(v) for v in V], ri(t), P(out), C.byref(plan),
+                            C.byref(plan))
+        out = out.reshape(N, 3)
+        assert relerr(out[ci], g["box_dE"][t - 1]) < 1e-12
+        direct = fo.dfield_direct(cells * D, cells[ci] * D, g["box_FF"], K0, t - 1)
+        assert relerr(out, direct) < 1e-12
