@@ -217,6 +217,16 @@ class Ref:
                              _rd(theta), _rd(phi), _ri(4), Eder.ctypes.data_as(C.c_void_p))
         return np.array(Eder)
 
+    def bhmie(self, x, m, nang=2):
+        """BHMIE_ of module_mie.f: (qext, qsca, qback, gsca)."""
+        S1 = np.zeros(2 * 1000 - 1, np.complex128)
+        S2 = np.zeros(2 * 1000 - 1, np.complex128)
+        mm = (C.c_double * 2)(m.real, m.imag)
+        out = [C.c_double(0) for _ in range(4)]
+        self.lib.bhmie_(_rd(x), C.byref(mm), _ri(nang), S1.ctypes.data_as(C.c_void_p),
+                        S2.ctypes.data_as(C.c_void_p), *[C.byref(o) for o in out])
+        return tuple(o.value for o in out)
+
     def _vect(self, M):
         return [np.zeros(M, np.complex128) for _ in range(3)]
 

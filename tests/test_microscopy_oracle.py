@@ -101,3 +101,22 @@ def test_reflection_image_mirror_symmetry():
         assert np.abs(imgs[1][c] - imgs[-1][c]).max() < 1e-12 * scale
     # without the incident wave (E0 = 0) both keep total = scattered
     assert np.array_equal(imgs[-1][0], imgs[-1][3])
+
+
+@pytest.mark.parametrize("dark", [False, True])
+def test_image_assembly_of_the_bare_incident_wave(dark):
+    """No scattered light, one normal incidence: the pupil holds the incident
+    wave alone at bin (0, 0) and the image is flat.  Brightfield adds E/dk^2
+    (microsbf.f:736-746), darkfield i E/dk^2 (microsdf.f:712-717); the image
+    plane scales by dk^2/gross, so every pixel reads |E_c|/gross."""
+    import microscopy as mi
+    N, imax, dk, gross = 64, 5, 3e5, 40.0
+    E0 = 0.7 - 0.2j
+    El = np.zeros((2 * imax + 1, 2 * imax + 1, 3), np.complex128)
+    for pp in (0.0, 1.0):
+        img = mi.image_assembly([El], [(0, 0, 0.0, 0.0, pp)], K0, E0, 0.5 * K0, 0.0, gross, dk, imax,
+                                dk, N, dark=dark)
+        e = mi.ondeplanekxky(np.zeros((1, 3)), K0, E0, pp, 0.0, 0.0)
+        for c in range(3):
+            assert np.abs(img[c]).max() == 0.0
+            assert np.allclose(img[3 + c], abs(e[c]) / gross, rtol=1e-12, atol=1e-300)

@@ -356,3 +356,14 @@ def test_inputs_match_compiled_reference(ref):
             a = scene.polarizability(np.array([eps]), 20e-9, k0, method)[0]
             b = ref.poladiff(20e-9, eps, k0, method)
             assert abs(a - b) <= 1e-15 * abs(b), (method, eps)
+
+
+@pytest.mark.parametrize("x,m", [(0.5, 1.5 + 0j), (1.99, 1.5 + 0j), (4.0, 1.33 + 0.01j),
+                                 (7.5, 2.0 + 0.5j)])
+def test_mie_matches_compiled_bhmie(ref, x, m):
+    """oracle/mie.py against module_mie.f's BHMIE, compiled: Qext, Qsca, g."""
+    from mie import bhmie
+    qext, qsca, g = bhmie(x, m)
+    rqext, rqsca, _, rg = ref.bhmie(x, m)
+    assert abs(qext / rqext - 1) < 1e-12 and abs(qsca / rqsca - 1) < 1e-12
+    assert abs(g - rg) < 1e-12 * max(1.0, abs(rg))
