@@ -251,6 +251,11 @@ struct MvOp {
   virtual bool mvt32(const float2* x, const float2* zero, float2* y, hipStream_t s,
                      const int* skip, int R, long long n, int skip_stride) const;
 };
+// y = x + alpha^T y for one right-hand side of nd dipoles (y holding -T x:
+// the transposed operator's last step), unless *skip is set.
+template <typename V>
+bool alpha_t_launch(const V* x, V* y, const V* al, int mode, long long nd, hipStream_t s,
+                    const int* skip);
 // FF = alpha . x for one problem of nd dipoles.
 bool alpha_apply_one(const double2* x, double2* FF, const double2* al, int mode, long long nd,
                      hipStream_t s);

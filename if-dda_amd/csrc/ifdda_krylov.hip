@@ -1594,6 +1594,18 @@ struct CtxOp : MvOp {
   }
 };
 
+template <typename V>
+bool alpha_t_launch(const V* x, V* y, const V* al, int mode, long long nd, hipStream_t s,
+                    const int* skip) {
+  const int nb = (int)std::max<long long>(1, std::min<long long>(kMaxBlocks, (nd + kThreads - 1) / kThreads));
+  hipLaunchKernelGGL(k_alpha_t<V>, dim3(nb, 1), dim3(kThreads), 0, s, x, y, al, mode, nd, skip, 0);
+  return hip_ok(hipGetLastError(), "transposed alpha");
+}
+template bool alpha_t_launch<double2>(const double2*, double2*, const double2*, int, long long,
+                                      hipStream_t, const int*);
+template bool alpha_t_launch<float2>(const float2*, float2*, const float2*, int, long long,
+                                     hipStream_t, const int*);
+
 bool MvOp::mvt64(const double2*, const double2*, double2*, hipStream_t, const int*, int, long long,
                  int) const {
   set_error("the transposed operator (QMRCLA) is not available for this solve");

@@ -25,6 +25,7 @@
 #include <map>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/ifdda.h"
@@ -151,7 +152,12 @@ struct PackOp : MvOp {
   PackLayout lay64, lay32;
   const void* blob64 = nullptr;
   const void* blob32 = nullptr;
+  // the same launches without alpha, for QMRCLA's A^T = I - alpha^T T
+  // (inverserig.f:559-590; alpha is one scalar per dipole in a pack)
+  const void* blobt64 = nullptr;
+  const void* blobt32 = nullptr;
   std::vector<const double2*> al;  // per member
+  std::vector<const float2*> al32;
   std::vector<long long> nd, off;
   bool mv64(const double2* x, const double2* b, double2* y, hipStream_t s, const int* skip, int,
             long long, int skip_stride) const override {
@@ -165,6 +171,31 @@ struct PackOp : MvOp {
     for (int m = 0; m < R; ++m)
       if (!alpha_apply_one(x + off[m], FF + off[m], al[m], kAlphaScalar, nd[m], s)) return false;
     return true;
+  }
+  template <typename V>
+  bool transposed(const V* x, const V* zero, V* y, hipStream_t s, const int* skip, int R,
+                  int skip_stride) const {
+    bool ok;
+    if constexpr (std::is_same<V, double2>::value)
+      ok = launch_matvec_pack_f64(lay64, blobt64, x, zero, y, s, skip, skip_stride);
+    else
+      ok = launch_matvec_pack_f32(lay32, blobt32, x, zero, y, s, skip, skip_stride);
+    for (int m = 0; ok && m < R; ++m) {
+      const V* a;
+      if constexpr (std::is_same<V, double2>::value) a = al[m];
+      else a = al32[m];
+      ok = alpha_t_launch<V>(x + off[m], y + off[m], a, kAlphaScalar, nd[m], s,
+                             skip ? skip + (long long)m * skip_stride : nullptr);
+    }
+    return ok;
+  }
+  bool mvt64(const double2* x, const double2* zero, double2* y, hipStream_t s, const int* skip,
+             int R, long long, int skip_stride) const override {
+    return transposed<double2>(x, zero, y, s, skip, R, skip_stride);
+  }
+  bool mvt32(const float2* x, const float2* zero, float2* y, hipStream_t s, const int* skip, int R,
+             long long, int skip_stride) const override {
+    return transposed<float2>(x, zero, y, s, skip, R, skip_stride);
   }
 };
 
@@ -220,11 +251,7 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
   const auto t0 = std::chrono::steady_clock::now();
   const int J = (int)in.size();
   if (J == 0) return IFDDA_OK;
-  if (o->method == IFDDA_QMRCLA) {  // A^T per member is not built for packed launches
-    set_error("pack solve: QMRCLA needs the transposed operator; solve it on a context");
-    return IFDDA_ERR_ARG;
-  }
-  if (o->method < IFDDA_GPBICG1 || o->method > IFDDA_CORS || (o->precision != 64 && o->precision != 32)) {
+  if (o->method < IFDDA_GPBICG1 || o->method > IFDDA_QMRCLA || (o->precision != 64 && o->precision != 32)) {
     set_error("pack solve: unknown method or precision");
     return IFDDA_ERR_ARG;
   }
@@ -478,6 +505,7 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
     std::vector<MatvecDev<double2>> m64(P);
     std::vector<MatvecDev<float2>> m32(P);
     op.al.resize(P);
+    op.al32.resize(P);
     op.nd.resize(P);
     op.off = w.offs;
     for (int m = 0; m < P; ++m) {
@@ -500,20 +528,36 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
       b.alpha = d.al32;
       a.alpha_mode = b.alpha_mode = kAlphaScalar;
       op.al[m] = d.al64;
+      op.al32[m] = d.al32;
       op.nd[m] = d.nd;
     }
-    std::vector<unsigned char> b64, b32;
+    std::vector<unsigned char> b64, b32, t64, t32;
     const size_t n64 = pack_blob_f64(gp, m64, w.offs.data(), op.lay64, b64);
     const size_t n32 = pack_blob_f32(gp, m32, w.offs.data(), op.lay32, b32);
-    const size_t need = al256(n64) + al256(n32);
+    size_t nt64 = 0, nt32 = 0;
+    if (o->method == IFDDA_QMRCLA) {
+      for (int m = 0; m < P; ++m) m64[m].alpha_mode = m32[m].alpha_mode = kAlphaNone;
+      PackLayout l64, l32;
+      nt64 = pack_blob_f64(gp, m64, w.offs.data(), l64, t64);
+      nt32 = pack_blob_f32(gp, m32, w.offs.data(), l32, t32);
+    }
+    const size_t need = al256(n64) + al256(n32) + al256(nt64) + al256(nt32);
     if (need > w.blob_cap && !hip_ok(hipStreamSynchronize(w.s), "blob")) return IFDDA_ERR_HIP;
     if (!grow(&w.blob, &w.blob_cap, need, "hipMalloc pack blob") ||
         !hip_ok(hipMemcpyAsync(w.blob, b64.data(), n64, hipMemcpyHostToDevice, w.s), "copy blob") ||
         !hip_ok(hipMemcpyAsync(w.blob + al256(n64), b32.data(), n32, hipMemcpyHostToDevice, w.s), "copy blob") ||
+        (nt64 && !hip_ok(hipMemcpyAsync(w.blob + al256(n64) + al256(n32), t64.data(), nt64,
+                                        hipMemcpyHostToDevice, w.s), "copy blob")) ||
+        (nt32 && !hip_ok(hipMemcpyAsync(w.blob + al256(n64) + al256(n32) + al256(nt64), t32.data(),
+                                        nt32, hipMemcpyHostToDevice, w.s), "copy blob")) ||
         !hip_ok(hipStreamSynchronize(w.s), "pack setup"))  // host staging ends here
       return IFDDA_ERR_HIP;
     op.blob64 = w.blob;
     op.blob32 = w.blob + al256(n64);
+    if (nt64) {
+      op.blobt64 = w.blob + al256(n64) + al256(n32);
+      op.blobt32 = w.blob + al256(n64) + al256(n32) + al256(nt64);
+    }
   }
   // ---- Krylov workspace (reused when large enough) and the solve
   const bool need32 = o->precision == 32;

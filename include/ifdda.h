@@ -45,7 +45,7 @@ extern "C" {
 #define IFDDA_BICGSTARPLUS 11 /* bicgstarplus.f                                  */
 #define IFDDA_GPBICOR 12      /* gpbicor.f                                       */
 #define IFDDA_CORS 13         /* cors.f                                          */
-#define IFDDA_QMRCLA 14       /* qmrpim.f (PIMZQMR): needs A^T, contexts only     */
+#define IFDDA_QMRCLA 14       /* qmrpim.f (PIMZQMR), with A^T = I - alpha^T T    */
 
 typedef struct ifdda_ctx ifdda_ctx;
 

@@ -88,10 +88,13 @@ def test_batch_rejects_bad_jobs(gpu):
 
 
 @pytest.mark.parametrize("method", ["GPBICG2", "GPBICGsafe", "GPBICGplus", "GPBICGAR", "BICGSTARPLUS",
-                                    "GPBICOR", "CORS", "CG", "BICGSTAB", "TFQMR", "QMRBICGSTAB2"])
+                                    "GPBICOR", "CORS", "CG", "BICGSTAB", "TFQMR", "QMRBICGSTAB2",
+                                    "QMRCLA"])
 def test_pack_methods_equal_own_solves(gpu, method):
-    """Every Krylov method a pack runs (all but QMRCLA) gives each member the
-    result of its own context solve, bit for bit, in both precisions; a
+    """Every Krylov method gives each member of a pack the result of its own
+    context solve, bit for bit, in both precisions (QMRCLA's transposed
+    operator included: the pack's launches without alpha, then alpha^T per
+    member); a
     member whose method stalls (BICGSTAB's stopping test of
     pimzbicgstab.f:168-181 can run to the iteration cap) stalls the same way
     alone, with the same counts and fields."""

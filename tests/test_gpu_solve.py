@@ -168,17 +168,6 @@ def test_dropin_method_names_as_inverserig_compares_them(cfg1):
     assert bad["nstop"] == 1 and bad["infostr"] == "Iterative method not correct!"
 
 
-def test_qmrcla_refused_in_packs(gpu):
-    """Packed launches have no transposed operator: QMRCLA is refused with a
-    clear error, on a context it runs (above)."""
-    from ifdda import scene
-    probs = scene.config4(2, seed=3)
-    batch = [dict(nx=p.nx, ny=p.ny, nz=p.nz, d=p.d, k0=k0, alpha=al, E_inc=E, tabdip=p.tabdip)
-             for (p, k0, al, E) in probs]
-    with pytest.raises(gpu.IfddaError, match="transposed"):
-        gpu.solve_batch(batch, "QMRCLA", 1e-4, precision=64)
-
-
 def test_gpbicg_trajectory_matches_oracle(cfg1, oracle):
     s, k0, alpha, E, sv = cfg1
     sv.trace_enable(64)
