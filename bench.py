@@ -124,6 +124,8 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=40, help="requests per rate")
     ap.add_argument("--serving-seq", type=int, default=8, help="FP64 single solves timed per small object")
     ap.add_argument("--workers", type=int, default=8, help="runner workers")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default tile widths (skip ifdda_autotune)")
     ap.add_argument("--section", choices=["batch", "uneven", "serving"],
                     help="run only this section and print its result (for profiling)")
     args = ap.parse_args()
@@ -152,6 +154,11 @@ def main():
     green_s = time.perf_counter() - t0
     sv.set_dipoles(s.tabdip)
     sv.set_polarizability(alpha)
+    tiles = {"default": sv.tile_widths()}
+    if not args.no_tune:  # measured tile widths for this grid (untimed setup)
+        t0 = time.perf_counter()
+        tiles["tuned"] = sv.autotune()
+        tiles["tune_s"] = time.perf_counter() - t0
     nd = s.nd
     Ed = torch.from_numpy(E.view(np.float64).copy()).cuda()  # inputs resident in HBM
     FFd = torch.empty_like(Ed)
@@ -340,6 +347,7 @@ def main():
         "field_reference": {"method": "GPBICG1", "precision": "fp64", "tol": 1e-8,
                             "tol1": info_ref["tol1"], "iterations": info_ref["nloop"]},
         "green_setup_s": green_s,
+        "tile_widths": tiles,
         "force": force,
         "passes": passes,
         "roofline": {"bound": "hbm", "kernel": f"{names[p_dom]} ({kern[p_dom]})",
@@ -520,6 +528,8 @@ def bench_batch(args):
     sv.greencalculfft(s.d, k0)
     sv.set_dipoles(s.tabdip)
     sv.set_polarizability(alpha)
+    if not args.no_tune:  # both the batch and its FP64 baseline use the tuned widths
+        sv.autotune()
     n3 = 3 * s.nd
     Ed = torch.from_numpy(E.view(np.float64).copy()).cuda()
     FFd = torch.empty_like(Ed)

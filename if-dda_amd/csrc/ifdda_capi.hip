@@ -103,7 +103,8 @@ extern "C" int ifdda_create(int nx, int ny, int nz, ifdda_ctx** out) {
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) !=
       hipSuccess)
     lds = 65536;
-  if (!make_tiles(g, (size_t)std::min(lds, 65536))) {
+  c->lds_limit = (size_t)std::min(lds, 65536);
+  if (!make_tiles(g, c->lds_limit)) {
     delete c;
     return arg_error("ifdda_create: padded FFT length above the supported maximum");
   }

@@ -818,12 +818,14 @@ __global__ void __launch_bounds__(256) k_zconv_ct(SArgs<V> a) {
 }
 
 // The compile-time z convolutions built into the library: (L, B) of the
-// 40^3, 128^3, 192^3 and 200^3 grids at the default LDS budgets.
+// 40^3, 128^3, 192^3 and 200^3 grids at their default tile widths (L = 40:
+// width 5 for one right-hand side, 27 for batches).
 template <typename V> using ZconvKernel = void (*)(SArgs<V>);
 template <typename V> ZconvKernel<V> zconv_ct_for(int L, int B) {
   if (L == 200 && B == 5) return k_zconv_ct<V, 200, 5>;
   if (L == 128 && B == 8) return k_zconv_ct<V, 128, 8>;
   if (L == 192 && B == 5) return k_zconv_ct<V, 192, 5>;
+  if (L == 40 && B == 5) return k_zconv_ct<V, 40, 5>;
   if (L == 40 && B == 27) return k_zconv_ct<V, 40, 27>;
   return nullptr;
 }
@@ -1384,8 +1386,8 @@ bool make_axis_plan(int L, AxisPlan* out) {
 bool has_specialized_z(const GridPlan& g, int prec) {
   const int pi = prec == 32 ? 1 : 0;
   if (2 * g.nz > g.L[2]) return false;
-  return prec == 32 ? zconv_ct_for<float2>(g.L[2], g.p3[pi].B) != nullptr
-                    : zconv_ct_for<double2>(g.L[2], g.p3[pi].B) != nullptr;
+  return prec == 32 ? zconv_ct_for<float2>(g.L[2], g.one[pi][2].B) != nullptr
+                    : zconv_ct_for<double2>(g.L[2], g.one[pi][2].B) != nullptr;
 }
 
 // Largest batch width B (lines per tile = B*NC) such that the tile fits the
@@ -1406,6 +1408,18 @@ static TileShape tile_for(const AxisPlan& ax, int NC, int esz, int nbatch, size_
     t = TileShape{B, threads, lds};
   }
   return t;
+}
+
+TileShape pass_tile_shape(const GridPlan& g, int pi, int p, int B, size_t lds_limit) {
+  static const int axis[5] = {0, 1, 2, 1, 0};  // P1..P5: x, y, z, y, x
+  const int esz = pi == 0 ? 16 : 8;
+  const int nbatch = (p == 0 || p == 4) ? g.ny * g.nz : g.L[0];
+  const bool conv = p == 2;
+  if (B == 0) return tile_for(g.ax[axis[p]], 3, esz, nbatch, lds_limit, conv);
+  if (B < 0 || B > std::max(1, nbatch)) return TileShape{0, 0, 0};
+  // feasibility is monotone in B: the widest tile up to B is B when B fits
+  const TileShape t = tile_for(g.ax[axis[p]], 3, esz, B, lds_limit, conv);
+  return t.B == B ? t : TileShape{0, 0, 0};
 }
 
 void axis_positions(const AxisPlan& ax, int* pos) {
@@ -1486,8 +1500,21 @@ bool make_tiles(GridPlan& g, size_t lds_limit) {
   // FP32 passes are fastest at ~26 KB (B = 5 for L = 200), FP64 y/x passes
   // at ~33 KB (B = 3) and the FP64 z convolution at ~52 KB (B = 5); smaller
   // tiles leave more workgroups per CU to overlap load, FFT and store phases.
+  // On short lines the budgets allow wide tiles (B = 17..64 at L = 16..48),
+  // which a sweep of 8^3..100^3 boxes measured 10-35% slower per matvec than
+  // narrow ones: widths are further capped at 4 lines (x passes) and 5 (y
+  // and z passes), which came within ~2% of the best width per pass on
+  // average.  A z pass keeps its budget width where a compile-time kernel is
+  // built for it.
+  // Batched launches (several right-hand sides, packs) keep the budget
+  // widths: the right-hand sides already give the grid its workgroups, and
+  // narrower tiles there only add per-workgroup table loads (measured: the
+  // capped widths made the 256-RHS batch and the packed uneven batch 8-15%
+  // slower).
+  constexpr int kCapX = 4, kCapYZ = 5;
   const size_t b32 = std::min<size_t>(lds_limit, 26624), b64 = std::min<size_t>(lds_limit, 33792),
                b32c = std::min<size_t>(lds_limit, 26624), b64c = std::min<size_t>(lds_limit, 53248);
+  const int L3 = g.L[2];
   for (int p = 0; p < 2; ++p) {
     const int esz = p == 0 ? 16 : 8;
     const size_t bs = p == 0 ? b64 : b32, bc = p == 0 ? b64c : b32c;
@@ -1496,6 +1523,22 @@ bool make_tiles(GridPlan& g, size_t lds_limit) {
     g.p3[p] = tile_for(g.ax[2], 3, esz, L1, bc, true);
     g.p4[p] = tile_for(g.ax[1], 3, esz, L1, bs, false);
     g.p5[p] = tile_for(g.ax[0], 3, esz, g.ny * g.nz, bs, false);
+    // single right-hand side: capped widths
+    TileShape* o = g.one[p];
+    o[0] = tile_for(g.ax[0], 3, esz, std::min(g.ny * g.nz, kCapX), bs, false);
+    o[1] = tile_for(g.ax[1], 3, esz, std::min(L1, kCapYZ), bs, false);
+    o[2] = tile_for(g.ax[2], 3, esz, std::min(L1, kCapYZ), bc, true);
+    if (!zconv_ct_for<float2>(L3, o[2].B) && zconv_ct_for<float2>(L3, g.p3[p].B)) o[2] = g.p3[p];
+    o[3] = tile_for(g.ax[1], 3, esz, std::min(L1, kCapYZ), bs, false);
+    o[4] = tile_for(g.ax[0], 3, esz, std::min(g.ny * g.nz, kCapX), bs, false);
+    // widths measured by the autotuner for this box, where they fit
+    int tb[5];
+    if (tuned_widths(g.nx, g.ny, g.nz, p == 0 ? 64 : 32, tb)) {
+      for (int q = 0; q < 5; ++q) {
+        const TileShape u = pass_tile_shape(g, p, q, tb[q], lds_limit);
+        if (u.B == tb[q]) o[q] = u;
+      }
+    }
   }
   g.gx = tile_for(g.ax[0], 1, 16, g.L[1] * g.L[2], lds_limit, false);
   g.gy = tile_for(g.ax[1], 1, 16, L1, lds_limit, false);
@@ -1503,6 +1546,9 @@ bool make_tiles(GridPlan& g, size_t lds_limit) {
   for (const TileShape* t : {&g.p1[0], &g.p2[0], &g.p3[0], &g.p4[0], &g.p5[0], &g.p1[1], &g.p2[1],
                              &g.p3[1], &g.p4[1], &g.p5[1], &g.gx, &g.gy, &g.gz})
     if (t->B == 0) return false;
+  for (int p = 0; p < 2; ++p)
+    for (int q = 0; q < 5; ++q)
+      if (g.one[p][q].B == 0) return false;
   return true;
 }
 
@@ -1524,16 +1570,17 @@ template <typename V> struct PassSet {
 };
 
 template <typename V>
-static void pass_set(const GridPlan& g, const MatvecDev<V>& m, int pi, PassSet<V>& o) {
+static void pass_set(const GridPlan& g, const MatvecDev<V>& m, int pi, bool single,
+                     PassSet<V>& o) {
   using S = decltype(V::x);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int L1 = g.L[0], L2 = g.L[1], L3 = g.L[2];
   const long long l1c = (long long)L1 * 3;
-  o.t[0] = &g.p1[pi];
-  o.t[1] = &g.p2[pi];
-  o.t[2] = &g.p3[pi];
-  o.t[3] = &g.p4[pi];
-  o.t[4] = &g.p5[pi];
+  o.t[0] = single ? &g.one[pi][0] : &g.p1[pi];
+  o.t[1] = single ? &g.one[pi][1] : &g.p2[pi];
+  o.t[2] = single ? &g.one[pi][2] : &g.p3[pi];
+  o.t[3] = single ? &g.one[pi][3] : &g.p4[pi];
+  o.t[4] = single ? &g.one[pi][4] : &g.p5[pi];
   {  // P1: scatter(alpha x) + FFT-x
     XArgs<V>& a = o.p1;
     a = XArgs<V>{};
@@ -1646,7 +1693,7 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
   const long long zA = (long long)g.nz * g.ny * g.L[0] * 3;  // A per RHS
   const long long zB = (long long)g.nz * g.L[1] * g.L[0] * 3;  // B per RHS
   PassSet<V> ps;
-  pass_set(g, m, pi, ps);
+  pass_set(g, m, pi, R == 1, ps);
   auto grid = [&](int p) { return dim3(ps.gx[p], ps.gy[p], R); };
   {  // P1: scatter(alpha x) + FFT-x
     XArgs<V> a = ps.p1;
@@ -1749,7 +1796,7 @@ static size_t pack_blob_impl(const std::vector<const GridPlan*>& g,
   lay.ng3 = 1;
   for (int i = 0; i < P; ++i) {
     PassSet<V> ps;
-    pass_set(*g[i], m[i], pi, ps);
+    pass_set(*g[i], m[i], pi, false, ps);
     ps.p1.zs_x = ps.p5.zs_x = offs[i];
     const void* src[5] = {&ps.p1, &ps.p2, &ps.p3, &ps.p4, &ps.p5};
     for (int p = 0; p < 5; ++p) {

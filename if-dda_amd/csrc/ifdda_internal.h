@@ -25,7 +25,8 @@ struct GridPlan {
   int F[3];         // folded lengths L/2 + 1
   AxisPlan ax[3];
   // tile shapes per pass and precision (index 0 = FP64, 1 = FP32)
-  TileShape p1[2], p2[2], p3[2], p4[2], p5[2];
+  TileShape p1[2], p2[2], p3[2], p4[2], p5[2];  // batched launches and packs
+  TileShape one[2][5];  // launches of one right-hand side: P1..P5 per precision
   TileShape gx, gy, gz;  // Green full-grid passes (FP64, 1 component)
 };
 
@@ -101,6 +102,14 @@ bool make_grid_plan(int nx, int ny, int nz, size_t lds_limit, GridPlan* out, std
 // FP32, stage-major twiddles (stage_twiddles), in-place FFT positions.
 void axis_tables(const AxisPlan& ax, double2* t64, float2* t32, double2* s64, float2* s32,
                  int* pos);
+// Tile shape of pass p (0..4 = P1..P5) of precision index pi (0 = FP64,
+// 1 = FP32): width exactly B, or the widest that fits lds_limit when B = 0;
+// B = 0 in the result when the width does not fit.  (The autotuner's widths
+// are those of one-right-hand-side launches, GridPlan::one.)
+TileShape pass_tile_shape(const GridPlan& g, int pi, int p, int B, size_t lds_limit);
+// Widths of P1..P5 recorded by the autotuner for a box and precision (32 or
+// 64); false when none.
+bool tuned_widths(int nx, int ny, int nz, int prec, int B[5]);
 // whether the z pass of this grid has a compile-time kernel (prec 64 or 32)
 bool has_specialized_z(const GridPlan& g, int prec);
 
@@ -197,6 +206,7 @@ struct ifdda_ctx {
   double2* Gd64 = nullptr;  // folded d T / d x, d y, d z (force), 3 * F2*F3*F1*6
   double gd_d = 0.0, gd_k0 = 0.0;  // the lattice and wavenumber Gd64 was built for
   bool gd_ready = false;            // cleared with every new Green tensor
+  size_t lds_limit = 65536;  // dynamic LDS per workgroup the tiles may use
   ifdda::PassTimer timer;  // enabled by ifdda_timing_enable
   bool specialized = true;  // ifdda_use_specialized
 

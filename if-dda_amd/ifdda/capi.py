@@ -131,6 +131,11 @@ def lib():
     L.ifdda_specialized_z.argtypes = [vp, ip, C.POINTER(C.c_int)]
     L.ifdda_timing_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.ifdda_force_device.argtypes = [vp, ip, vp, vp, vp, pd, dp, dp, pd, pd, vp, vp, vp]
+    pi = C.POINTER(C.c_int)
+    L.ifdda_autotune.argtypes = [vp, ip, ip, pi]
+    L.ifdda_tile_widths.argtypes = [vp, pi]
+    L.ifdda_tune_get.argtypes = [ip, ip, ip, ip, pi]
+    L.ifdda_tune_set.argtypes = [ip, ip, ip, ip, pi]
     _lib = L
     return L
 
@@ -475,6 +480,20 @@ class Solver:
         _check(lib().ifdda_specialized_z(self.h, int(prec), C.byref(has)), "ifdda_specialized_z")
         return bool(has.value)
 
+    def autotune(self, prec: int = 0, reps: int = 0):
+        """Time the matvec passes at every candidate tile width and keep the
+        fastest per pass (ifdda_autotune).  Returns the widths now in use,
+        {64: [P1..P5], 32: [P1..P5]}."""
+        w = (C.c_int * 10)()
+        _check(lib().ifdda_autotune(self.h, int(prec), int(reps), w), "ifdda_autotune")
+        return {64: list(w[:5]), 32: list(w[5:])}
+
+    def tile_widths(self):
+        """The tile widths of this context, {64: [P1..P5], 32: [P1..P5]}."""
+        w = (C.c_int * 10)()
+        _check(lib().ifdda_tile_widths(self.h, w), "ifdda_tile_widths")
+        return {64: list(w[:5]), 32: list(w[5:])}
+
     def timing_passes(self, mask: int, every: int = 1):
         """Bracket only the passes whose bit is set (bit p = P(p+1)), and of
         those every every-th launch."""
@@ -495,6 +514,23 @@ class Solver:
         n = C.c_int(0)
         _check(lib().ifdda_solve_trace(self.h, int(cap), _p(out), C.byref(n)), "ifdda_solve_trace")
         return out.reshape(-1, 6)[: n.value].copy()
+
+
+def tune_get(nx, ny, nz, prec):
+    """The autotuner's widths (P1..P5) recorded for a box and precision, or
+    None."""
+    w = (C.c_int * 5)()
+    rc = lib().ifdda_tune_get(int(nx), int(ny), int(nz), int(prec), w)
+    if rc not in (0, 1):
+        _check(rc, "ifdda_tune_get")
+    return list(w[:]) if rc == 1 else None
+
+
+def tune_set(nx, ny, nz, prec, widths):
+    """Record tile widths (P1..P5) for a box and precision, e.g. saved from an
+    earlier run; contexts created afterwards use them."""
+    w = (C.c_int * 5)(*[int(v) for v in widths])
+    _check(lib().ifdda_tune_set(int(nx), int(ny), int(nz), int(prec), w), "ifdda_tune_set")
 
 
 class Runner:

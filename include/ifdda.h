@@ -166,6 +166,34 @@ int ifdda_use_specialized(ifdda_ctx *ctx, int on);
 /* *has = 1 when this grid's z pass has a compile-time kernel for prec 64/32. */
 int ifdda_specialized_z(ifdda_ctx *ctx, int prec, int *has);
 
+/* Tile-width autotuner.  It replaces the reference's untimed FFT planning
+ * (FFTW_ESTIMATE plans, cdmlib.f:1153-1158) with a measured choice of the
+ * matvec passes' tile widths (box lines per workgroup) in launches of one
+ * right-hand side (solves and matvecs of one vector; batched right-hand
+ * sides and packs keep the widths of fixed LDS budgets).  ifdda_autotune
+ * times P1..P5 of this context's grid at every candidate width (the widths
+ * that LDS budgets of 6..64 KB give, plus the current one), `reps` timed
+ * matvecs each (0: 5), for prec 64, 32 or 0 (both).  It keeps the fastest
+ * width per pass in the context and records it in a process-wide table
+ * keyed by box size and precision; later plans of the same box (contexts,
+ * pack members) start from that table.  widths (optional, 10 ints) receives
+ * the widths now in use: P1..P5 in FP64, then P1..P5 in FP32.  Needs a Green
+ * tensor and dipoles; synchronises the context's stream.  A width changes
+ * how lines are grouped, not the arithmetic of a line: the generic kernels
+ * give bit-identical results at every width, and a z-pass width without a
+ * compile-time kernel runs the generic kernel (ifdda_use_specialized). */
+int ifdda_autotune(ifdda_ctx *ctx, int prec, int reps, int widths[10]);
+/* The tile widths of this context, laid out as in ifdda_autotune. */
+int ifdda_tile_widths(ifdda_ctx *ctx, int widths[10]);
+/* Read the table entry of a box (prec 64 or 32) into widths[5] (P1..P5):
+ * returns 1 when there is one, 0 when not, or an error code.  Also usable
+ * to save the table between runs. */
+int ifdda_tune_get(int nx, int ny, int nz, int prec, int widths[5]);
+/* Write the table entry of a box, e.g. one saved by a previous run;
+ * IFDDA_ERR_ARG when a width does not fit that box.  Contexts created
+ * afterwards use it. */
+int ifdda_tune_set(int nx, int ny, int nz, int prec, const int widths[5]);
+
 /* Per-iteration scalars of the last solve: rows of six complex values
  * (GPBICG: alpha, beta, eta, dzeta, R0RN; QMRBICGSTAB: alpha, beta, omega,
  * tau, eta; TFQMR: alpha, beta, rho, tau, eta; then the residual), recorded
