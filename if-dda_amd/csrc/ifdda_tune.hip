@@ -19,10 +19,10 @@
 // (the widths that the LDS budgets 6..64 KB give, plus the current one) with
 // HIP events around the pass launches, keeps the fastest width per pass, and
 // records the choice in a process-wide table keyed by box size and precision.
-// Plans made afterwards for the same box (new contexts, pack members, the
-// reference-shaped entries' contexts) start from the table; the table can be
-// read and written through ifdda_tune_get / ifdda_tune_set, so a caller can
-// keep it between runs.
+// Contexts created afterwards for the same box (including those of the
+// reference-shaped entries) start from the table; the table can be read and
+// written through ifdda_tune_get / ifdda_tune_set, so a caller can keep it
+// between runs.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

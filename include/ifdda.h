@@ -175,8 +175,8 @@ int ifdda_specialized_z(ifdda_ctx *ctx, int prec, int *has);
  * that LDS budgets of 6..64 KB give, plus the current one), `reps` timed
  * matvecs each (0: 5), for prec 64, 32 or 0 (both).  It keeps the fastest
  * width per pass in the context and records it in a process-wide table
- * keyed by box size and precision; later plans of the same box (contexts,
- * pack members) start from that table.  widths (optional, 10 ints) receives
+ * keyed by box size and precision; contexts created later for the same box
+ * start from that table.  widths (optional, 10 ints) receives
  * the widths now in use: P1..P5 in FP64, then P1..P5 in FP32.  Needs a Green
  * tensor and dipoles; synchronises the context's stream.  A width changes
  * how lines are grouped, not the arithmetic of a line: the generic kernels
