@@ -112,6 +112,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=100, help="cube edge in dipoles")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-config1", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-inc", type=int, default=128, help="incidences of the batch (x2 RHS)")
     ap.add_argument("--batch-seq", type=int, default=16, help="sequential FP64 solves timed")
@@ -126,7 +127,7 @@ def main():
     ap.add_argument("--workers", type=int, default=8, help="runner workers")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default tile widths (skip ifdda_autotune)")
-    ap.add_argument("--section", choices=["batch", "uneven", "serving"],
+    ap.add_argument("--section", choices=["config1", "batch", "uneven", "serving"],
                     help="run only this section and print its result (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -136,7 +137,8 @@ def main():
     if args.section:
         import torch
         torch.cuda.set_device(0)
-        fn = {"batch": bench_batch, "uneven": bench_uneven, "serving": bench_serving}[args.section]
+        fn = {"config1": bench_config1, "batch": bench_batch, "uneven": bench_uneven,
+              "serving": bench_serving}[args.section]
         print(json.dumps({args.section: fn(args)}), flush=True)
         return
 
@@ -365,6 +367,8 @@ def main():
     sv.close()
     del Ed, FFd, FLd
     torch.cuda.empty_cache()
+    if not args.no_config1:
+        out["config1"] = bench_config1(args)
     if not args.no_batch:
         out["batch"] = bench_batch(args)
     if not args.no_uneven:
@@ -512,6 +516,96 @@ def bench_serving(args):
     for r in out["rates"]:
         r["speedup_vs_sequential_fp64"] = r["solves_per_s"] * t_mix
     return out
+
+
+class _StdoutToStderr:
+    """fd 1 goes to fd 2 while the compiled reference runs: its Fortran
+    runtime reports on unit 6 (flushed by `flush` before fd 1 is restored),
+    and stdout carries only the JSON line."""
+
+    def __init__(self, flush):
+        self.flush = flush
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+
+    def __exit__(self, *exc):
+        self.flush()
+        sys.stdout.flush()
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+
+
+def bench_config1(args):
+    """The reference's own CPU-runnable case end to end: Green tensor and
+    FP64 GPBICG solve (x0 = E_inc, as inverserig.f:36-42 starts) through the
+    host-array entry points on the GPU, beside the compiled reference's
+    greencalculfft + inverserig on this host's cores (median of runs)."""
+    import ifdda
+    from ifdda import scene
+
+    s, k0, alpha, E = scene.config1()
+    tol = 1e-4
+
+    def gpu_once():
+        t0 = time.perf_counter()
+        with ifdda.Solver(s.nx, s.ny, s.nz) as sv:
+            sv.greencalculfft(s.d, k0)
+            t1 = time.perf_counter()
+            sv.set_dipoles(s.tabdip)
+            sv.set_polarizability(alpha)
+            o = sv.inverserig(E, "GPBICG1", tol, precision=64, x0_zero=False)
+            t2 = time.perf_counter()
+        return t1 - t0, t2 - t1, o
+
+    gpu_once()  # first launches of every kernel
+    g = [gpu_once() for _ in range(7)]
+    og = g[-1][2]
+    res = {"workload": "config 1: sphere of diameter 400 nm, eps 2.25, 20^3 lattice (d = 20 nm), "
+                       "RR polarisability, x-polarised plane wave along +z, lambda 632.8 nm; "
+                       "GPBICG1 tol 1e-4 from x0 = E_inc; host arrays in and out",
+           "dipoles": int(s.nd),
+           "gpu_fp64": {"green_s": float(np.median([x[0] for x in g])),
+                        "solve_s": float(np.median([x[1] for x in g])),
+                        "total_s": float(np.median([x[0] + x[1] for x in g])),
+                        "nloop": og["nloop"], "ncompte": og["ncompte"], "tol1": og["tol1"],
+                        "runs": len(g)}}
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Ref  # the checker, timed only as the CPU baseline
+    if not Ref.available():
+        res["reference_cpu"] = None
+        return res
+    r = Ref()
+    al = np.broadcast_to(np.asarray(alpha, np.complex128), (s.nd,))
+    pol = np.zeros((s.nd, 3, 3), np.complex128)
+    for c in range(3):
+        pol[:, c, c] = al
+
+    def ref_once():
+        t0 = time.perf_counter()
+        G = r.green_fft(s.nx, s.ny, s.nz, s.d, k0)
+        t1 = time.perf_counter()
+        o = r.inverserig(G, s.tabdip, s.nx, s.ny, s.nz, pol, E, "GPBICG1", tol)
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1, o
+
+    with _StdoutToStderr(r.flush_stdout):
+        c = [ref_once() for _ in range(3)]
+    oc = c[-1][2]
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    res["reference_cpu"] = {"green_s": float(np.median([x[0] for x in c])),
+                            "solve_s": float(np.median([x[1] for x in c])),
+                            "total_s": float(np.median([x[0] + x[1] for x in c])),
+                            "nloop": oc["nloop"], "ncompte": oc["ncompte"], "tol1": oc["tol1"],
+                            "cores": threads, "runs": len(c),
+                            "kind": "reference (cdmlib Fortran compiled from source, Singleton FFT, "
+                                    "OpenMP)"}
+    res["speedup_total"] = res["reference_cpu"]["total_s"] / res["gpu_fp64"]["total_s"]
+    res["field_rel_err_gpu_vs_reference"] = float(
+        np.linalg.norm(og["FFloc"] - oc["FFloc"]) / np.linalg.norm(oc["FFloc"]))
+    return res
 
 
 def bench_batch(args):

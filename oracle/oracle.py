@@ -152,6 +152,12 @@ class Ref:
     def available(path: str = REF_SO) -> bool:
         return os.path.exists(path)
 
+    def flush_stdout(self):
+        """Flush the Fortran runtime's buffered unit 6 (oracle/ref_stdout.f90)."""
+        f = getattr(self.lib, "ref_flush_stdout", None)
+        if f is not None:
+            f()
+
     def propa(self, r_obs, r_src, k0):
         T = np.zeros((3, 3), np.complex128, order="F")
         a = [C.c_double(float(v)) for v in (*r_obs, *r_src, k0)]
