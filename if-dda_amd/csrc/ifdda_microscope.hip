@@ -1,11 +1,14 @@
-// ifdda_microscope.hip -- the brightfield microscope of microsbf.f (the quick
-// lens path, nquicklens = 1) on the device, many incidences per solve:
+// ifdda_microscope.hip -- the brightfield microscope of microsbf.f on the
+// device, many incidences per solve (quick lens path, nquicklens = 1; the
+// direct-sum pupil of nquicklens = 0 with direct_lens):
 //
 //   condenser lattice of incidences        microsbf.f:90-172
 //   incident plane waves at the dipoles    ondeplanekxky.f (k_onde)
 //   local field, all incidences at once    ifdda_solve_multi_device
 //   field in the lens pupil                diffractefft2dlens.f (far-field
-//                                          kernels, lens mode)
+//                                          kernels, lens mode), or the direct
+//                                          sum of microsbf.f:629-668
+//                                          (k_lens_direct)
 //   pupil: focus phase, incident wave,     microsbf.f:676-790 (k_pupil)
 //   lens rotation
 //   image plane                            fouriertoimage.f (k_img_x, k_img_acc)
@@ -109,8 +112,79 @@ __global__ void __launch_bounds__(256) k_zero(double* I, long long n) {
 
 }  // namespace
 
-// The condenser's incidences (microsbf.f:90-172, nquicklens = 1), in the
-// reference's order: polarisation outer, idelta, jdelta inner.
+// Pupil field of the nquicklens = 0 path (microsbf.f:629-668,
+// microssansinc.f:600-640): for every bin (ii, jj) of the (2 imax + 1)^2
+// pupil with kx^2 + ky^2 <= (NA k0)^2, n = (kx, ky, nside kz) / k0 and
+//   E = i k0^2 / (2 pi kz) sum_dipoles exp(-i k0 n.r) (p - (n.p) n)
+// (zero elsewhere).  One block per (bin, right-hand side); the dipoles are
+// summed per thread in a fixed stride order, then over threads in a fixed
+// tree, so the result does not depend on timing.
+__global__ void __launch_bounds__(256) k_lens_direct(const double2* FF, long long nd,
+                                                     const int32_t* dipcell, int nx, int ny,
+                                                     double ox, double oy, double oz, double d,
+                                                     int imax, double dk, double k0, double nak,
+                                                     double nside, double2* E) {
+  const int n = 2 * imax + 1, nb = n * n;
+  const int bin = blockIdx.x, r = blockIdx.y;
+  const int ii = bin % n, jj = bin / n;
+  const double kx = dk * (ii - imax), ky = dk * (jj - imax), t = kx * kx + ky * ky;
+  double2* o = E + (long long)r * 3 * nb;
+  if (t > nak * nak) {
+    if (threadIdx.x < 3) o[(long long)threadIdx.x * nb + bin] = make_double2(0.0, 0.0);
+    return;
+  }
+  const double kz = sqrt(k0 * k0 - t);
+  const double n0 = kx / k0, n1 = ky / k0, n2 = nside * sqrt(1.0 - n0 * n0 - n1 * n1);
+  const double2* p = FF + (long long)r * 3 * nd;
+  double2 acc[3] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+  for (long long i = threadIdx.x; i < nd; i += blockDim.x) {
+    const long long cell = dipcell ? (long long)dipcell[i] : i;
+    const long long a = cell % nx, b = (cell / nx) % ny, c = cell / ((long long)nx * ny);
+    const double x = ox + d * a, y = oy + d * b, z = oz + d * c;
+    double sn, cs;
+    sincos(-k0 * (n0 * x + n1 * y + n2 * z), &sn, &cs);
+    const double2 ph = make_double2(cs, sn);
+    const double2 p0 = p[3 * i], p1 = p[3 * i + 1], p2 = p[3 * i + 2];
+    const double2 np = make_double2(n0 * p0.x + n1 * p1.x + n2 * p2.x,
+                                    n0 * p0.y + n1 * p1.y + n2 * p2.y);
+    acc[0] = cadd(acc[0], cmul(ph, make_double2(p0.x - np.x * n0, p0.y - np.y * n0)));
+    acc[1] = cadd(acc[1], cmul(ph, make_double2(p1.x - np.x * n1, p1.y - np.y * n1)));
+    acc[2] = cadd(acc[2], cmul(ph, make_double2(p2.x - np.x * n2, p2.y - np.y * n2)));
+  }
+  __shared__ double2 red[3][256];
+  for (int c = 0; c < 3; ++c) red[c][threadIdx.x] = acc[c];
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int c = 0; c < 3; ++c) red[c][threadIdx.x] = cadd(red[c][threadIdx.x], red[c][threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) {
+    // k0^2 / (-2 pi i kz) = i k0^2 / (2 pi kz)
+    const double f = k0 * k0 / (2.0 * kPi * kz);
+    const double2 v = red[threadIdx.x][0];
+    o[(long long)threadIdx.x * nb + bin] = make_double2(-f * v.y, f * v.x);
+  }
+}
+
+// Pupil step and size of the nquicklens = 0 path (microsbf.f:139-146,
+// microsdf.f:140-147, microssansinc.f: the same): deltakx = 2 pi / (nfft2d
+// d) / 2^k for the first k with imaxk0 = nint(k0 / deltakx) + 1 > 5.  The
+// pupil must fit the image grid (2 imaxk0 + 1 <= nfft2d), which the
+// reference does not check.
+static std::string direct_geometry(int nfft2d, double k0, double d, int* imax, double* dk) {
+  if (nfft2d > 16384) return "nfft2d too large";
+  for (int k = 0; k < 60; ++k) {
+    *dk = 2.0 * kPi / ((double)nfft2d * d) / std::ldexp(1.0, k);
+    *imax = (int)std::lround(k0 / *dk) + 1;
+    if (*imax > 5) break;
+  }
+  if (2 * *imax + 1 > nfft2d) return "In FFT diffraction for lens nfft2d too small!";
+  return "";
+}
+
+// The condenser's incidences (microsbf.f:90-172), in the reference's order:
+// polarisation outer, idelta, jdelta inner.
 struct Incidence {
   int idelta, jdelta;
   double kx, ky, pp;
@@ -126,17 +200,34 @@ static std::string incidence_list(const ifdda_ctx* c, const ifdda_microscope_opt
   const double numaperk = k0 * m.numaperinc;
   const double xr = (c->cellbox[1] - c->cellbox[0]) * d, yr = (c->cellbox[3] - c->cellbox[2]) * d;
   const double deltakm = kPi / std::max(xr, yr);
-  const double dkx = 2.0 * kPi / (d * m.nfft2d);
-  if (dkx >= numaperk) return "In FFT lens nfft2d too small:Increase size of FFT 1";
-  int imul = (int)(deltakm / dkx);
-  if (imul == 0) return "In FFT lens nfft2d too small:Increase size of FFT 2";
   double dk;
   int imaxinc;
-  for (;;) {
-    dk = dkx * imul;
-    imaxinc = (int)std::lround(k0 / dk) + 1;
-    if (imaxinc > 3) break;
-    if (--imul == 0) return "In FFT lens nfft2d too small:Increase size of FFT 3";
+  if (m.direct_lens) {
+    // nquicklens = 0 (microsbf.f:148-157, 179-183): deltak = deltakx
+    // dnint(deltakm / deltakx) / 2^k for the first k with imaxinc > 2
+    int imaxk0;
+    double dkx;
+    const std::string g = direct_geometry(m.nfft2d, k0, d, &imaxk0, &dkx);
+    if (!g.empty()) return g;
+    const double q = std::round(deltakm / dkx);
+    if (q == 0.0) return "In FFT lens nfft2d too small";
+    for (int k = 0;; ++k) {
+      dk = dkx * q / std::ldexp(1.0, k);
+      imaxinc = (int)std::lround(k0 / dk) + 1;
+      if (imaxinc > 2) break;
+    }
+    if (dk >= numaperk) return "In FFT lens nfft2d too small";
+  } else {
+    const double dkx = 2.0 * kPi / (d * m.nfft2d);
+    if (dkx >= numaperk) return "In FFT lens nfft2d too small:Increase size of FFT 1";
+    int imul = (int)(deltakm / dkx);
+    if (imul == 0) return "In FFT lens nfft2d too small:Increase size of FFT 2";
+    for (;;) {
+      dk = dkx * imul;
+      imaxinc = (int)std::lround(k0 / dk) + 1;
+      if (imaxinc > 3) break;
+      if (--imul == 0) return "In FFT lens nfft2d too small:Increase size of FFT 3";
+    }
   }
   *deltak = dk;
   if (dark) {
@@ -246,7 +337,8 @@ static int microscope_run(ifdda_ctx* c, const ifdda_solve_opts* o, const ifdda_m
   std::string msg = incidence_list(c, *m, &inc, &dkinc, dark);
   int imax = 0;
   double dk = 0.0;
-  if (msg.empty()) msg = lens_geometry(N, k0, d, nak, &imax, &dk);
+  if (msg.empty())
+    msg = m->direct_lens ? direct_geometry(N, k0, d, &imax, &dk) : lens_geometry(N, k0, d, nak, &imax, &dk);
   if (!msg.empty()) {
     set_error(std::string(dark ? "ifdda_microsdf_device: " : "ifdda_microsbf_device: ") + "" + msg);
     return IFDDA_ERR_ARG;
@@ -392,11 +484,17 @@ static int microscope_run(ifdda_ctx* c, const ifdda_solve_opts* o, const ifdda_m
       set_error(std::string(dark ? "ifdda_microsdf_device: " : "ifdda_microsbf_device: ") + "Calculation cancelled during iterative method");
       return IFDDA_OK;
     }
-    int im2;
-    double dk2;
-    rc = farfield_dev(c, R, dFF, c->tab, nd, N, k0, d, m->origin[0], m->origin[1], m->origin[2], dL,
-                      nullptr, &im2, &dk2, s, 1, nak, m->nside);
-    if (rc != IFDDA_OK) return rc;
+    if (m->direct_lens) {
+      hipLaunchKernelGGL(k_lens_direct, dim3(nb, R), dim3(256), 0, s, (const double2*)dFF, nd,
+                         (const int32_t*)c->dipcell, c->g.nx, c->g.ny, m->origin[0], m->origin[1],
+                         m->origin[2], d, imax, dk, k0, nak, (double)m->nside, dL);
+    } else {
+      int im2;
+      double dk2;
+      rc = farfield_dev(c, R, dFF, c->tab, nd, N, k0, d, m->origin[0], m->origin[1], m->origin[2],
+                        dL, nullptr, &im2, &dk2, s, 1, nak, m->nside);
+      if (rc != IFDDA_OK) return rc;
+    }
     // einc (double2) is staged in the ev slot after the waves kernel has consumed it
     if (!hip_ok(hipMemcpyAsync(dev, hinc.data(), sizeof(double2) * 3 * R, hipMemcpyHostToDevice, s), "copy einc") ||
         !hip_ok(hipMemcpyAsync(dbin, hbin.data(), sizeof(int) * R, hipMemcpyHostToDevice, s), "copy bins"))

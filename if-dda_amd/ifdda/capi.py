@@ -53,7 +53,8 @@ class MicroscopeOpts(C.Structure):
     _fields_ = [("nfft2d", C.c_int), ("numaper", C.c_double), ("numaperinc", C.c_double),
                 ("gross", C.c_double), ("zlens", C.c_double), ("nside", C.c_int),
                 ("npolainc", C.c_int), ("E0", C.c_double * 2), ("k0", C.c_double),
-                ("d", C.c_double), ("origin", C.c_double * 3), ("max_batch", C.c_int)]
+                ("d", C.c_double), ("origin", C.c_double * 3), ("max_batch", C.c_int),
+                ("direct_lens", C.c_int)]
 
 
 class Job(C.Structure):
@@ -404,10 +405,13 @@ class Solver:
     # -- brightfield microscope (microsbf.f, quick lens) ---------------------
     @staticmethod
     def microscope_opts(nfft2d, numaper, numaperinc, k0, d, origin, gross=100.0, zlens=0.0,
-                        nside=1, npolainc=0, E0=1.0, max_batch=0):
+                        nside=1, npolainc=0, E0=1.0, max_batch=0, direct_lens=False):
+        """direct_lens: the nquicklens = 0 path (pupil field as a direct sum
+        over dipoles, with its own pupil and condenser steps)."""
         m = MicroscopeOpts()
         m.nfft2d, m.numaper, m.numaperinc, m.gross, m.zlens = int(nfft2d), numaper, numaperinc, gross, zlens
         m.nside, m.npolainc, m.k0, m.d, m.max_batch = int(nside), int(npolainc), k0, d, int(max_batch)
+        m.direct_lens = int(bool(direct_lens))
         m.E0[0], m.E0[1] = complex(E0).real, complex(E0).imag
         for i in range(3):
             m.origin[i] = float(origin[i])

@@ -375,8 +375,9 @@ void ifdda_computegcfft2d(const int32_t *imax, const double *deltakx, const doub
                           const double *Ediffkzneg, double *gasym, double *Cscai);
 
 /* ---------------------------------------------------------------------------
- * Brightfield microscope (microsbf.f:1-965, quick-lens path nquicklens = 1):
- * the condenser's incidences solved together, lens, image.
+ * Brightfield microscope (microsbf.f:1-965): the condenser's incidences
+ * solved together, lens, image.  Quick-lens path (nquicklens = 1) by
+ * default; direct_lens = 1 selects the nquicklens = 0 path.
  * ------------------------------------------------------------------------- */
 typedef struct {
   int nfft2d;         /* FFT size of the pupil / image grid (even)             */
@@ -394,6 +395,12 @@ typedef struct {
   double d;           /* lattice spacing [m]                                   */
   double origin[3];   /* centre of box cell (1,1,1) [m]                         */
   int max_batch;      /* incidences per multi-RHS solve (0: all at once)        */
+  int direct_lens;    /* 0: quick lens, the pupil field by FFT
+                         (diffractefft2dlens.f; nquicklens = 1).  1: the
+                         nquicklens = 0 path: pupil step and condenser step
+                         of microsbf.f:139-157 and the pupil field as the
+                         direct sum over dipoles of microsbf.f:629-668
+                         (microssansinc.f:600-640 for nside = -1)          */
 } ifdda_microscope_opts;
 
 /* The incidences microsbf.f:90-172 takes (polarisation outer, idelta,

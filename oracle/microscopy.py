@@ -11,6 +11,12 @@ nquicklens = 1) and their pieces:
   incidences_df       the darkfield ring of incidences         microsdf.f:91-221
   image_assembly      pupil fill, lens rotation, |E|^2 sums    microsbf.f:676-845,
                       (dark=True)                               microsdf.f:655-790
+  direct_geometry     pupil step of nquicklens = 0             microsbf.f:139-146
+                      (parity unpinned, as the direct=True steps of
+                      incidences: microsbf.f needs HDF5 and is not built,
+                      and no fixture holds these steps)
+  lens_field_direct   pupil field as a direct sum (nquicklens  microsbf.f:629-668,
+                      = 0)                                      microssansinc.f:600-640
 
 Only tests/ imports this module.
 """
@@ -126,25 +132,75 @@ def fouriertoimage_even(dk, gross, planes):
     return [np.fft.fftshift(np.fft.fft2(P)) * tmp for P in planes]
 
 
-def incidences(k0, numaperinc, d, nfft2d, xs, ys):
-    """microsbf.f:90-172 with nquicklens = 1: the condenser's (kx, ky) list
-    (idelta outer, jdelta inner), deltak, deltakx."""
-    deltakm = np.pi / max(np.max(xs) - np.min(xs), np.max(ys) - np.min(ys))
-    dkx = 2.0 * np.pi / (d * nfft2d)
-    numaperk = k0 * numaperinc
-    if dkx >= numaperk:
-        raise ValueError("In FFT lens nfft2d too small:Increase size of FFT 1")
-    imul = int(deltakm / dkx)
-    if imul == 0:
-        raise ValueError("In FFT lens nfft2d too small:Increase size of FFT 2")
+def direct_geometry(nfft2d, k0, d):
+    """microsbf.f:139-146 (nquicklens = 0): deltakx = 2 pi / (nfft2d d) / 2^k
+    for the first k with imaxk0 = nint(k0 / deltakx) + 1 > 5.  Returns
+    imaxk0, deltakx."""
+    k = 0
     while True:
-        dkinc = dkx * imul
-        imaxinc = nint((k0 / dkinc)) + 1
-        if imaxinc > 3:
-            break
-        imul -= 1
+        dkx = 2.0 * np.pi / (nfft2d * d) / 2.0 ** k
+        imax = nint(k0 / dkx) + 1
+        if imax > 5:
+            return imax, dkx
+        k += 1
+
+
+def lens_field_direct(pos, FF, k0, dk, imax, numaper, nside=1):
+    """microsbf.f:629-668 (microssansinc.f:600-640 for nside = -1): at every
+    pupil bin (i, j) with |k| <= numaper (k units), n = (kx, ky, nside kz) /
+    k0 and E[i, j] = k0^2 / (-2 pi i kz) sum_dipoles exp(-i k0 n.r) (p -
+    (n.p) n); zero elsewhere.  pos [nd, 3], FF [3 nd] (p per dipole)."""
+    P = np.asarray(FF).reshape(-1, 3)
+    n = 2 * imax + 1
+    E = np.zeros((n, n, 3), np.complex128)
+    for a, i in enumerate(range(-imax, imax + 1)):
+        for b, j in enumerate(range(-imax, imax + 1)):
+            kx, ky = dk * i, dk * j
+            if np.sqrt(kx * kx + ky * ky) > numaper:
+                continue
+            kz = np.sqrt(k0 * k0 - kx * kx - ky * ky)
+            nv = np.array([kx / k0, ky / k0,
+                           nside * np.sqrt(1.0 - (kx / k0) ** 2 - (ky / k0) ** 2)])
+            ph = np.exp(-1j * k0 * (pos @ nv))
+            pn = P @ nv
+            Em = (ph[:, None] * (P - pn[:, None] * nv[None, :])).sum(0)
+            E[a, b] = Em * k0 * k0 / (-2.0 * np.pi * 1j * kz)
+    return E
+
+
+def incidences(k0, numaperinc, d, nfft2d, xs, ys, direct=False):
+    """microsbf.f:90-172: the condenser's (kx, ky) list (idelta outer,
+    jdelta inner), deltak, deltakx.  nquicklens = 1 by default; direct: the
+    nquicklens = 0 steps (microsbf.f:139-157, 179-183)."""
+    deltakm = np.pi / max(np.max(xs) - np.min(xs), np.max(ys) - np.min(ys))
+    numaperk = k0 * numaperinc
+    if direct:
+        _, dkx = direct_geometry(nfft2d, k0, d)
+        q = float(nint(deltakm / dkx))  # dnint
+        k = 0
+        while True:
+            dkinc = dkx * q / 2.0 ** k
+            imaxinc = nint(k0 / dkinc) + 1
+            if imaxinc > 2:
+                break
+            k += 1
+        if dkinc >= numaperk:
+            raise ValueError("In FFT lens nfft2d too small")
+    else:
+        dkx = 2.0 * np.pi / (d * nfft2d)
+        if dkx >= numaperk:
+            raise ValueError("In FFT lens nfft2d too small:Increase size of FFT 1")
+        imul = int(deltakm / dkx)
         if imul == 0:
-            raise ValueError("In FFT lens nfft2d too small:Increase size of FFT 3")
+            raise ValueError("In FFT lens nfft2d too small:Increase size of FFT 2")
+        while True:
+            dkinc = dkx * imul
+            imaxinc = nint((k0 / dkinc)) + 1
+            if imaxinc > 3:
+                break
+            imul -= 1
+            if imul == 0:
+                raise ValueError("In FFT lens nfft2d too small:Increase size of FFT 3")
     inc = []
     for idel in range(-imaxinc, imaxinc + 1):
         for jdel in range(-imaxinc, imaxinc + 1):
@@ -154,13 +210,13 @@ def incidences(k0, numaperinc, d, nfft2d, xs, ys):
     return inc, dkinc, dkx
 
 
-def incidences_df(k0, numaperinc, d, nfft2d, xs, ys):
+def incidences_df(k0, numaperinc, d, nfft2d, xs, ys, direct=False):
     """microsdf.f:91-221 with nquicklens = 1: deltak as the brightfield's
     (microsdf.f:91-137), then ideltam = max(int(2 pi / (deltak NA_inc / k0))
     + 1, 8) incidences at phi = 2 pi idelta / ideltam on the cone of NA_inc
     (kx, ky) = k0 NA_inc (cos phi, sin phi), for pp = 0 then pp = 1
     (npolainc = 0 there).  Returns [(idelta, pp, kx, ky)], deltak, deltakx."""
-    _, dkinc, dkx = incidences(k0, numaperinc, d, nfft2d, xs, ys)
+    _, dkinc, dkx = incidences(k0, numaperinc, d, nfft2d, xs, ys, direct=direct)
     dtheta = dkinc / k0 * numaperinc
     ideltam = max(int(2.0 * np.pi / dtheta) + 1, 8)
     out = []

@@ -41,11 +41,14 @@ def test_incidence_lattice(small):
 
 def _cpu_images(s, alpha, oracle, m, tol, dark=False):
     import microscopy as mi
+    direct = bool(m.direct_lens)
     if dark:
-        ring, dkinc, _ = mi.incidences_df(K0, m.numaperinc, s.d, m.nfft2d, s.pos[:, 0], s.pos[:, 1])
+        ring, dkinc, _ = mi.incidences_df(K0, m.numaperinc, s.d, m.nfft2d, s.pos[:, 0], s.pos[:, 1],
+                                          direct=direct)
         waves = [(idl, 0, kx, ky, pp) for (idl, pp, kx, ky) in ring]
     else:
-        inc, dkinc, _ = mi.incidences(K0, m.numaperinc, s.d, m.nfft2d, s.pos[:, 0], s.pos[:, 1])
+        inc, dkinc, _ = mi.incidences(K0, m.numaperinc, s.d, m.nfft2d, s.pos[:, 0], s.pos[:, 1],
+                                      direct=direct)
         waves = [(idl, jdl, kx, ky, pp) for pp in (0.0, 1.0) for (idl, jdl, kx, ky) in inc]
     G = oracle.green_fft(s.nx, s.ny, s.nz, (2 * s.nx, 2 * s.ny, 2 * s.nz), s.d, K0)
     pol = np.zeros((s.nx * s.ny * s.nz, 3, 3), np.complex128)
@@ -61,8 +64,13 @@ def _cpu_images(s, alpha, oracle, m, tol, dark=False):
         p = (alpha[:, None] * o["FFloc"].reshape(-1, 3))
         FFbox = np.zeros((s.nx * s.ny * s.nz, 3), np.complex128)
         FFbox[cells] = p[s.tabdip[cells] - 1]
-        El, imax, dk = mi.diffractefft2dlens(FFbox.reshape(-1), s.nx, s.ny, s.nz, m.nfft2d, K0,
-                                             s.origin[0], s.origin[1], zs, s.d, m.numaper * K0, m.nside)
+        if direct:  # nquicklens = 0: the pupil field as the direct sum over dipoles
+            imax, dk = mi.direct_geometry(m.nfft2d, K0, s.d)
+            El = mi.lens_field_direct(s.pos, p, K0, dk, imax, m.numaper * K0, m.nside)
+        else:
+            El, imax, dk = mi.diffractefft2dlens(FFbox.reshape(-1), s.nx, s.ny, s.nz, m.nfft2d, K0,
+                                                 s.origin[0], s.origin[1], zs, s.d, m.numaper * K0,
+                                                 m.nside)
         lens.append(El)
     return mi.image_assembly(lens, waves, K0, E0, m.numaper * K0, m.zlens, m.gross, dkinc, imax, dk,
                              m.nfft2d, dark=dark, nside=m.nside)
@@ -157,3 +165,34 @@ def test_reflection_microscopes_match_cpu_pipeline(small, oracle, dark):
                                                            zlens=50e-9, nside=1, npolainc=0,
                                                            E0=1.0 + 0.5j), 1e-10, dark=dark)
     assert np.abs(fwd[0] - ref[0]).max() > 0.01 * np.abs(fwd[0]).max()
+
+
+@pytest.mark.parametrize("dark,nside", [(False, 1), (True, 1), (False, -1)])
+def test_direct_lens_matches_cpu_pipeline(small, oracle, dark, nside):
+    """nquicklens = 0 (direct_lens): the pupil and condenser steps of
+    microsbf.f:139-157 and the pupil field as the direct sum over dipoles
+    (microsbf.f:629-668; its restatement equals the reference's FFT lens at
+    common bins, test_microscopy_oracle.py), then the same pupil and image
+    steps as the quick lens."""
+    import microscopy as mi
+    s, alpha, sv, _ = small
+    m = sv.microscope_opts(128, 0.8, 0.5, K0, s.d, s.origin, gross=100.0, zlens=50e-9, nside=nside,
+                           npolainc=0, E0=1.0 + 0.5j, direct_lens=True)
+    print(f"direct lens dark={dark} nside={nside}", flush=True)
+    kxy, dk = sv.microscope_incidences(m, dark=dark)
+    if dark:
+        ring, dkinc, _ = mi.incidences_df(K0, 0.5, s.d, 128, s.pos[:, 0], s.pos[:, 1], direct=True)
+        ref_inc = [(kx, ky, pp) for (_, pp, kx, ky) in ring]
+    else:
+        inc, dkinc, _ = mi.incidences(K0, 0.5, s.d, 128, s.pos[:, 0], s.pos[:, 1], direct=True)
+        ref_inc = [(kx, ky, pp) for pp in (0.0, 1.0) for (_, _, kx, ky) in inc]
+    assert abs(dk / dkinc - 1) < 1e-15
+    assert kxy.shape == (len(ref_inc), 3)
+    assert np.allclose(kxy, np.array(ref_inc), rtol=0, atol=1e-9 * K0)
+    img, infos = sv.microsbf(m, "GPBICG1", 1e-10, precision=64, dark=dark)
+    assert all(i["nstop"] == 0 and i["tol1"] <= 1e-10 for i in infos)
+    ref = _cpu_images(s, alpha, oracle, m, 1e-10, dark=dark)
+    for c in range(6):
+        scale = np.abs(ref[c]).max()
+        assert scale > 0
+        assert np.abs(img[c] - ref[c]).max() < 1e-7 * scale, (c, np.abs(img[c] - ref[c]).max() / scale)

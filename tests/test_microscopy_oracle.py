@@ -23,6 +23,29 @@ def test_lens_matches_reference_fixture(tag, N, na, side):
     assert relerr(E, f[f"lens_{tag}"]) < 1e-12
 
 
+@pytest.mark.parametrize("tag,N,na,side", [("a", 64, 0.9, 1), ("b", 96, 0.5, -1)])
+def test_direct_lens_sum_matches_reference_lens_fixture(tag, N, na, side):
+    """The nquicklens = 0 pupil field (microsbf.f:629-668, a direct sum over
+    the dipoles; microssansinc.f:600-640 for nside = -1) samples the sum that
+    the reference's FFT lens (diffractefft2dlens.f) evaluates: at the same
+    step and size it equals the reference's own output at every bin inside
+    the aperture."""
+    import microscopy as mi
+    f = golden("microscope.npz")
+    imax, dk = int(f[f"lens_{tag}_imax"]), float(f[f"lens_{tag}_dk"])
+    nx, ny, nz = 6, 5, 4
+    F = np.asarray(f["box_FF"]).reshape(nz, ny, nx, 3)
+    c, b, a = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    pos = np.stack([-50e-9 + a.ravel() * D, -40e-9 + b.ravel() * D,
+                    np.asarray(f["box_zs"])[c.ravel()]], 1)
+    E = mi.lens_field_direct(pos, F.reshape(-1, 3), K0, dk, imax, na * K0, side)
+    ref = f[f"lens_{tag}"]
+    k = np.arange(-imax, imax + 1) * dk
+    inside = k[:, None] ** 2 + k[None, :] ** 2 < (na * K0) ** 2
+    assert inside.sum() >= 9
+    assert relerr(E[inside], ref[inside]) < 1e-12, relerr(E[inside], ref[inside])
+
+
 def test_fouriertoimage_matches_reference_fixture():
     import microscopy as mi
     f = golden("microscope.npz")
