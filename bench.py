@@ -314,6 +314,7 @@ def main():
                 traffic = v["fetch_bytes"] + v["write_bytes"]
                 traffic_src = f"profiles/matvec_config2_hbm_traffic.json ({name})"
 
+    copy_gbs = copy_bandwidth()  # measured ceiling beside the 8 TB/s spec peak
     out = {
         "metric": METRIC,
         "value": args.steps / elapsed,
@@ -360,7 +361,11 @@ def main():
                      "duration_source": dom_src,
                      "min_bytes_per_launch": m32[p_dom], "achieved_min_bytes": achieved_min,
                      "frac_min_bytes": achieved_min / HBM_PEAK_GBS,
-                     "traffic_GB_per_s": (traffic / (avg_dom * 1e-3) / 1e9) if traffic else None},
+                     "traffic_GB_per_s": (traffic / (avg_dom * 1e-3) / 1e9) if traffic else None,
+                     "copy_GB_per_s": copy_gbs, "frac_of_copy": achieved / copy_gbs,
+                     "frac_min_bytes_of_copy": achieved_min / copy_gbs,
+                     "traffic_frac_of_copy": ((traffic / (avg_dom * 1e-3) / 1e9) / copy_gbs)
+                                             if traffic else None},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sv, s.nx, s.ny, s.nz, s.tabdip, nd, info64["ncompte"])
@@ -516,6 +521,26 @@ def bench_serving(args):
     for r in out["rates"]:
         r["speedup_vs_sequential_fp64"] = r["solves_per_s"] * t_mix
     return out
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """Measured device-to-device copy rate (bytes read + written per second)
+    of a plain torch copy of nbytes, median of reps, HIP events."""
+    import torch
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        ts.append(e0.elapsed_time(e1) * 1e-3)
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2.0 * nbytes / float(np.median(ts)) / 1e9
 
 
 class _StdoutToStderr:
