@@ -1170,7 +1170,7 @@ __device__ __forceinline__ void green_dT(double rx, double ry, double rz, double
   sincos(k * R, &sn, &cs);
   const double2 e = make_double2(cs, sn);
   auto cm = [](double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); };
-  const double2 A = cm(e, make_double2(k2 * iR - iR3, k * iR2));
+  // T = A I + B r r^T; its derivative needs B, A' and B' (A itself drops out)
   const double2 B = cm(e, make_double2(3.0 * iR5 - k2 * iR3, -3.0 * k * iR4));
   const double2 Ap = cm(e, make_double2(-2.0 * k2 * iR2 + 3.0 * iR4, k2 * k * iR - 3.0 * k * iR3));
   const double2 Bp = cm(e, make_double2(-15.0 * iR6 + 6.0 * k2 * iR4, 15.0 * k * iR5 - k2 * k * iR3));
