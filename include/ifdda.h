@@ -423,7 +423,8 @@ int ifdda_microsbf_device(ifdda_ctx *ctx, const ifdda_solve_opts *opts,
                           const ifdda_microscope_opts *m, double *images, int *ninc,
                           ifdda_solve_info *infos, int infos_cap, volatile int *nstop);
 
-/* Darkfield and phase microscope (microsdf.f:1-915, quick-lens path): as
+/* Darkfield and phase microscope (microsdf.f:1-915; quick lens, or
+ * direct_lens for its nquicklens = 0 path): as
  * ifdda_microsbf_device with the incidences of microsdf.f:159-221 -- a ring
  * of ideltam = max(int(2 pi k0 / (deltak NA_inc)) + 1, 8) plane waves on the
  * condenser cone (kx, ky) = k0 NA_inc (cos phi, sin phi), both
