@@ -66,6 +66,7 @@ extern "C" void ifdda_destroy(ifdda_ctx* c) {
   (void)hipFree(c->al32);
   (void)hipFree(c->scrA);
   (void)hipFree(c->scrB);
+  (void)hipFree(c->sstage);
   (void)hipFree(c->hx);
   (void)hipFree(c->hb);
   (void)hipFree(c->ffbuf);

@@ -191,6 +191,8 @@ struct ifdda_ctx {
   double2* hx = nullptr;  // device staging for host-array entry points
   double2* hb = nullptr;
   size_t hcap = 0;        // complex elements in hx / hb
+  double2* sstage = nullptr;  // host-array solves: E_inc, FF, FFloc (grow-only)
+  size_t scap = 0;            // complex elements in sstage
   void* solver = nullptr; // Krylov workspace (ifdda_krylov.hip)
   void* ffbuf = nullptr;  // far-field / cross-section workspace (ifdda_farfield.hip)
   size_t ffcap = 0;

@@ -2139,24 +2139,31 @@ extern "C" int ifdda_solve_multi(ifdda_ctx* c, const ifdda_solve_opts* o, int nr
     for (int r = 0; r < nrhs; ++r) std::memset(&infos[r], 0, sizeof(ifdda_solve_info));
     return IFDDA_OK;
   }
-  double2 *dE = nullptr, *dFF = nullptr, *dL = nullptr;
+  // device copies of E_inc, FF, FFloc in the context's grow-only staging
+  // (a solve per call would otherwise pay three allocations and three
+  // device-synchronising frees)
+  if (c->scap < 3 * n) {
+    (void)hipFree(c->sstage);
+    c->sstage = nullptr;
+    c->scap = 0;
+    if (!hip_ok(hipMalloc(&c->sstage, sizeof(double2) * 3 * n), "hipMalloc solve staging"))
+      return IFDDA_ERR_HIP;
+    c->scap = 3 * n;
+  }
+  double2 *dE = c->sstage, *dFF = dE + n, *dL = dFF + n;
+  hipStream_t s = c->stream;
   int rc = IFDDA_OK;
-  if (!hip_ok(hipMalloc(&dE, sizeof(double2) * n), "hipMalloc E_inc") ||
-      !hip_ok(hipMalloc(&dFF, sizeof(double2) * n), "hipMalloc FF") ||
-      !hip_ok(hipMalloc(&dL, sizeof(double2) * n), "hipMalloc FFloc") ||
-      !hip_ok(hipMemcpy(dE, E_inc, sizeof(double2) * n, hipMemcpyHostToDevice), "copy E_inc"))
+  if (!hip_ok(hipMemcpyAsync(dE, E_inc, sizeof(double2) * n, hipMemcpyHostToDevice, s), "copy E_inc"))
     rc = IFDDA_ERR_HIP;
-  if (!rc) rc = solve_dev(c, o, nrhs, dE, dFF, dL, infos, nstop, c->stream);
+  if (!rc) rc = solve_dev(c, o, nrhs, dE, dFF, dL, infos, nstop, s);
   if (!rc) {
-    if (FF && !hip_ok(hipMemcpy(FF, dFF, sizeof(double2) * n, hipMemcpyDeviceToHost), "copy FF"))
+    if (FF && !hip_ok(hipMemcpyAsync(FF, dFF, sizeof(double2) * n, hipMemcpyDeviceToHost, s), "copy FF"))
       rc = IFDDA_ERR_HIP;
-    if (FFloc &&
-        !hip_ok(hipMemcpy(FFloc, dL, sizeof(double2) * n, hipMemcpyDeviceToHost), "copy FFloc"))
+    if (!rc && FFloc &&
+        !hip_ok(hipMemcpyAsync(FFloc, dL, sizeof(double2) * n, hipMemcpyDeviceToHost, s), "copy FFloc"))
       rc = IFDDA_ERR_HIP;
   }
-  (void)hipFree(dE);
-  (void)hipFree(dFF);
-  (void)hipFree(dL);
+  if (!hip_ok(hipStreamSynchronize(s), "ifdda_solve")) rc = rc ? rc : IFDDA_ERR_HIP;
   return rc;
 }
 
