@@ -713,6 +713,15 @@ def bench_batch(args):
     img, minfo = sv.microsbf(m, "GPBICG1", 1e-4, precision=32, x0_zero=True)
     torch.cuda.synchronize()
     t_mic = time.perf_counter() - t1
+    # the same microscope on the nquicklens = 0 path (direct-sum pupil field,
+    # its own pupil and condenser steps)
+    md = sv.microscope_opts(256, 0.95, 0.95, k0, s.d, s.origin, gross=100.0, direct_lens=True)
+    kxyd, dkincd = sv.microscope_incidences(md)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _, dinfo = sv.microsbf(md, "GPBICG1", 1e-4, precision=32, x0_zero=True)
+    torch.cuda.synchronize()
+    t_micd = time.perf_counter() - t1
     sv.close()
     per_rhs_batch = R / t_batch
     per_rhs_seq = 1.0 / t_seq
@@ -739,7 +748,12 @@ def bench_batch(args):
                             "iterations_mean": float(np.mean([i["nloop"] for i in minfo])),
                             "tol1_max": max(i["tol1"] for i in minfo),
                             "what": "microsbf quick-lens path: all incidences in one multi-RHS mixed "
-                                    "solve, lens, image, incoherent sum"}}
+                                    "solve, lens, image, incoherent sum"},
+            "brightfield_direct_lens": {"incidences": int(kxyd.shape[0]), "deltak_inc": dkincd,
+                                        "seconds": t_micd,
+                                        "tol1_max": max(i["tol1"] for i in dinfo),
+                                        "what": "microsbf nquicklens = 0 path: the pupil field as a "
+                                                "direct sum over the dipoles"}}
 
 
 if __name__ == "__main__":
