@@ -132,7 +132,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(independent solves shard trivially across GPUs; see DESIGN.md)", file=sys.stderr)
+              "(independent solves shard trivially across GPUs, one process per GPU)", file=sys.stderr)
         sys.exit(2)
     if args.section:
         import torch
