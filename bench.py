@@ -131,7 +131,7 @@ def main():
                     help="run only this section and print its result (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the benchmark runs on one GPU; --gpus {args.gpus} is not supported "
               "(independent solves shard trivially across GPUs, one process per GPU)", file=sys.stderr)
         sys.exit(2)
     if args.section:
