@@ -307,6 +307,40 @@ class Ref:
                     nloop=nloop.value, ncompte=ncompte.value, nstop=nstop.value,
                     infostr=info.raw.decode(errors="replace").rstrip())
 
+    def inverserigopt(self, G, nx, ny, nz, polarisa, FF0, method="GPBICG1", tol=1e-4):
+        """inverserigopt_ (inverserigopt.f:1-6): the dense-box driver, every
+        box cell a dipole (nbsphere = nx ny nz), x0 = E_inc; its 39 arguments
+        (planf before planb) and the two hidden character lengths."""
+        nz2, ny2, nx2 = G.shape[1:]
+        M = nx2 * ny2 * nz2
+        nmax = nx * ny * nz
+        Gs = [np.ascontiguousarray(G[c]).reshape(-1) for c in range(6)]
+        V = self._vect(M)
+        XI = np.zeros(3 * nmax, np.complex128)
+        XR = np.zeros(3 * nmax, np.complex128)
+        wrk = np.zeros(3 * nmax * 12, np.complex128)
+        FF = np.zeros(3 * nmax, np.complex128)
+        FF0p = np.ascontiguousarray(FF0, np.complex128).copy()
+        FFloc = np.zeros(3 * nmax, np.complex128)
+        pol_f = np.asfortranarray(np.asarray(polarisa, np.complex128))
+        meth = method.ljust(12).encode()
+        info = C.create_string_buffer(b" " * 64, 64)
+        tol1 = C.c_double(0)
+        nloop = C.c_int(0)
+        ncompte = C.c_int(0)
+        nstop = C.c_int(0)
+        P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+        self.lib.inverserigopt_(*[P(g) for g in Gs], *[P(v) for v in V], _ri(M), _ri(M),
+                                _ri(3 * nmax), _ri(12), _ri(nmax), _ri(nx), _ri(ny), _ri(nz),
+                                _ri(nx), _ri(ny), _ri(nz), _ri(nx2), _ri(ny2), _ri(nx2 * ny2),
+                                _ri(nz2), _ri(nmax), _ri(3 * nmax), P(XI), P(XR), P(wrk), P(FF),
+                                P(FF0p), P(FFloc), P(pol_f), C.c_char_p(meth), _rd(tol),
+                                C.byref(tol1), C.byref(nloop), C.byref(ncompte), _ri8(), _ri8(),
+                                C.byref(nstop), info, C.c_size_t(12), C.c_size_t(64))
+        return dict(FF=FF, FFloc=FFloc, tol1=tol1.value, nloop=nloop.value,
+                    ncompte=ncompte.value, nstop=nstop.value,
+                    infostr=info.raw.decode(errors="replace").rstrip())
+
     def poladiff(self, d, eps, k0, method="RR", eps0=1.0):
         out = C.c_double * 2
         res = out()

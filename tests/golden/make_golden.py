@@ -34,6 +34,10 @@ Fixtures (inputs and the reference's outputs, nothing else):
                    oracle/force.py on those derivatives (`make_golden.py force`)
   microscope.npz   diffractefft2dlens_ (nside +1 and -1, two NA) on the seeded
                    6x5x4 box and fouriertoimage_ on seeded 32 x 32 planes
+  solve_opt.npz    inverserigopt_ (the dense-box driver, every box cell a
+                   dipole, alpha = 0 outside a sphere) on a 12 x 10 x 9 box for
+                   GPBICG1, QMRBICGSTAB1 and TFQMR at tol 1e-4 and 1e-6: E
+                   (FFloc), FF, tol1, nloop, ncompte (`make_golden.py opt`)
   farfield.npz     diffractefft2d_ and computegcfft2d_: a seeded box-ordered
                    FF on 6x5x4 at nfft2d 64 and 65 (odd), and the config-1
                    GPBICG1 moments (alpha E) at nfft2d 128; Epos / Eneg cut to
@@ -232,8 +236,47 @@ def force(r):
     np.savez_compressed(os.path.join(HERE, "force.npz"), **out)
 
 
+def opt_case():
+    """The dense-box problem of solve_opt.npz: a sphere of radius 100 nm
+    (eps 4+0.5i, RR) centred in a 12 x 10 x 9 box of 20 nm cells; every box
+    cell is a dipole (inverserigopt.f), alpha = 0 outside the sphere; a plane
+    wave at theta 20, phi 35, pp 0.4 over every cell."""
+    nx, ny, nz = 12, 10, 9
+    d = D
+    cx = (np.arange(nx) - (nx - 1) / 2) * d
+    cy = (np.arange(ny) - (ny - 1) / 2) * d
+    cz = (np.arange(nz) - (nz - 1) / 2) * d
+    Z, Y, X = np.meshgrid(cz, cy, cx, indexing="ij")
+    pos = np.stack([X.reshape(-1), Y.reshape(-1), Z.reshape(-1)], 1)
+    inside = np.linalg.norm(pos, axis=1) < 100e-9
+    alpha = np.where(inside, scene.polarizability(np.full(pos.shape[0], 4.0 + 0.5j), d, K0, "RR"), 0)
+    E = scene.plane_wave(pos, K0, 20.0, 35.0, pp=0.4)
+    return nx, ny, nz, d, alpha.astype(np.complex128), E
+
+
+def solve_opt(r):
+    nx, ny, nz, d, alpha, E = opt_case()
+    G = r.green_fft(nx, ny, nz, d, K0)
+    pol = np.zeros((nx * ny * nz, 3, 3), np.complex128)
+    for a in range(3):
+        pol[:, a, a] = alpha
+    res = {}
+    for m in ["GPBICG1", "QMRBICGSTAB1", "TFQMR"]:
+        for tag, tol in (("4", 1e-4), ("6", 1e-6)):
+            o = r.inverserigopt(G, nx, ny, nz, pol, E, m, tol)
+            assert o["nstop"] == 0, o
+            k = f"{m}_{tag}"
+            res.update({f"{k}_E": o["FFloc"], f"{k}_FF": o["FF"], f"{k}_tol1": o["tol1"],
+                        f"{k}_nloop": o["nloop"], f"{k}_ncompte": o["ncompte"]})
+            print("inverserigopt", k, o["nloop"], o["ncompte"], o["tol1"])
+    np.savez_compressed(os.path.join(HERE, "solve_opt.npz"), nx=nx, ny=ny, nz=nz, d=d, k0=K0,
+                        alpha=alpha, E_inc=E, **res)
+
+
 if __name__ == "__main__":
-    if len(sys.argv) > 1 and sys.argv[1] == "more":
+    if len(sys.argv) > 1 and sys.argv[1] == "opt":
+        solve_opt(Ref())
+    elif len(sys.argv) > 1 and sys.argv[1] == "more":
         solve_more(Ref())
     elif len(sys.argv) > 1 and sys.argv[1] == "force":
         force(Ref())

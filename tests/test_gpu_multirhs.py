@@ -48,9 +48,3 @@ def test_batch_mixed_precision(sph):
         assert infos[r]["nstop"] == 0 and infos[r]["tol1"] <= 1e-6
         assert relerr(FL[r], one["FFloc"]) < 1e-5
 
-
-def test_config3_scene():
-    from ifdda import scene
-    s, k0, alpha, E = scene.config3(n_inc=4)
-    assert E.shape == (8, 3 * s.nd)
-    assert 30000 < s.nd < 40000  # 20 spheres of ~1760 cells

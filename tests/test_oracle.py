@@ -211,6 +211,28 @@ def test_solvers_match_reference_golden(oracle, method):
     assert relerr(o["FFloc"], g[f"{method}_E"]) < 1e-11
 
 
+@pytest.mark.parametrize("method", ["GPBICG1", "QMRBICGSTAB1", "TFQMR"])
+@pytest.mark.parametrize("tag,tol", [("4", 1e-4), ("6", 1e-6)])
+def test_dense_box_solvers_match_inverserigopt_golden(oracle, method, tag, tol):
+    """inverserigopt.f: every box cell is a dipole (no map), alpha = 0 outside
+    the object; the oracle's dense-box driver reproduces the compiled
+    reference's iteration and matvec counts, tol1 and field."""
+    g = golden("solve_opt.npz")
+    nx, ny, nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
+    G = oracle.green_fft(nx, ny, nz, (2 * nx, 2 * ny, 2 * nz), float(g["d"]), float(g["k0"]))
+    pol = np.zeros((nx * ny * nz, 3, 3), complex)
+    for a in range(3):
+        pol[:, a, a] = g["alpha"]
+    o = oracle.inverserig(G, None, nx, ny, nz, pol, g["E_inc"], method, tol)
+    key = f"{method}_{tag}"
+    assert o["nstop"] == 0
+    assert o["nloop"] == int(g[f"{key}_nloop"])
+    assert o["ncompte"] == int(g[f"{key}_ncompte"])
+    assert abs(o["tol1"] / float(g[f"{key}_tol1"]) - 1) < 1e-8
+    assert relerr(o["FFloc"], g[f"{key}_E"]) < 1e-11
+    assert relerr(o["FF"], g[f"{key}_FF"]) < 1e-11
+
+
 def test_solver_vs_dense_solve(oracle):
     import ifdda.scene as scene
     s = scene.sphere(4, 40e-9, 4.0 + 1.0j)
@@ -367,3 +389,12 @@ def test_mie_matches_compiled_bhmie(ref, x, m):
     rqext, rqsca, _, rg = ref.bhmie(x, m)
     assert abs(qext / rqext - 1) < 1e-12 and abs(qsca / rqsca - 1) < 1e-12
     assert abs(g - rg) < 1e-12 * max(1.0, abs(rg))
+
+
+def test_config3_scene():
+    """The brightfield batch object: 20 spheres on a 64^3 lattice, 2 RHS per
+    incidence (p then s)."""
+    from ifdda import scene
+    s, k0, alpha, E = scene.config3(n_inc=4)
+    assert E.shape == (8, 3 * s.nd)
+    assert 30000 < s.nd < 40000  # 20 spheres of ~1760 cells
