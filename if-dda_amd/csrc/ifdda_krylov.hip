@@ -19,6 +19,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -2211,4 +2212,138 @@ extern "C" int ifdda_solve_trace_enable(ifdda_ctx* c, int cap) {
     S->tracecap = cap;
   }
   return IFDDA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ifdda_solve_device_noalloc: the device solve of one right-hand side with
+// every Krylov work vector, partial sum and state taken from `scratch`, a
+// fixed array of IFDDA_NOALLOC_BYTES in device memory that each GPU holds
+// for the life of the process.  The context's own buffers (FFT scratch,
+// Green tensor, map, polarisability) are made when the context is set up;
+// after that this call makes no device allocation.  The host never uses the
+// array's name as an address: hipGetSymbolAddress gives each device's copy
+// once, and that pointer is what the kernels receive.
+// ---------------------------------------------------------------------------
+namespace ifdda {
+
+static __device__ __align__(256) unsigned char scratch[IFDDA_NOALLOC_BYTES];
+
+namespace {
+
+constexpr int kNoAllocDevices = 64;
+std::mutex g_noalloc_mu[kNoAllocDevices];   // one solve at a time per device
+unsigned char* g_noalloc_ptr[kNoAllocDevices] = {};
+std::mutex g_noalloc_init;
+
+// The current device's copy of `scratch` (looked up on its first use).
+int noalloc_base(int dev, unsigned char** out) {
+  std::lock_guard<std::mutex> lk(g_noalloc_init);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("ifdda_solve_device_noalloc: hipGetSymbolAddress: ") +
+                hipGetErrorString(e));
+      return IFDDA_ERR_HIP;
+    }
+    g_noalloc_ptr[dev] = (unsigned char*)p;
+  }
+  *out = g_noalloc_ptr[dev];
+  return IFDDA_OK;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Bytes of the work of one right-hand side of n elements; with base set,
+// also points the solver's buffers into base.
+size_t noalloc_layout(Solver* S, long long n, bool need32, unsigned char* base) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) -> unsigned char* {
+    unsigned char* p = base ? base + off : nullptr;
+    off += align256(bytes);
+    return p;
+  };
+  const int nblk = blocks_for(n);
+  S->part = (double2*)take(sizeof(double2) * kMaxDots * nblk);
+  S->st = (KState*)take(sizeof(KState));
+  S->act = (int*)take(sizeof(int));
+  for (int i = 0; i < 14; ++i) S->v64[i] = (double2*)take(sizeof(double2) * n);
+  if (need32) {
+    S->rr64 = (double2*)take(sizeof(double2) * n);
+    for (int i = 0; i < 14; ++i) S->v32[i] = (float2*)take(sizeof(float2) * n);
+  }
+  return off;
+}
+
+}  // namespace
+}  // namespace ifdda
+
+extern "C" int ifdda_solve_device_noalloc(ifdda_ctx* c, const ifdda_solve_opts* o,
+                                          const void* E_inc, void* FF, void* FFloc,
+                                          ifdda_solve_info* info, volatile int* nstop,
+                                          void* stream) {
+  const auto t0 = std::chrono::steady_clock::now();
+  if (!c || !o || !E_inc || !info) {
+    set_error("ifdda_solve_device_noalloc: NULL argument");
+    return IFDDA_ERR_ARG;
+  }
+  std::memset(info, 0, sizeof(*info));
+  set_info_str(info, "");
+  if (o->method < IFDDA_GPBICG1 || o->method > IFDDA_QMRCLA || (o->precision != 64 && o->precision != 32)) {
+    set_error("ifdda_solve_device_noalloc: unknown method or precision");
+    return IFDDA_ERR_ARG;
+  }
+  if (!c->green_ready || !c->have_dipoles || c->alpha_mode == kAlphaNone || c->scr_rhs < 1 ||
+      !c->scrA || !c->scrB) {
+    set_error("ifdda_solve_device_noalloc: Green tensor, dipoles and polarisability must be set");
+    return IFDDA_ERR_STATE;
+  }
+  const long long n = 3 * (long long)c->nd;
+  if (n == 0) return IFDDA_OK;
+  Solver S;
+  const bool need32 = o->precision == 32;
+  const size_t need = noalloc_layout(&S, n, need32, nullptr);
+  if (need > (size_t)IFDDA_NOALLOC_BYTES) {
+    set_error("ifdda_solve_device_noalloc: " + std::to_string(c->nd) + " dipoles need " +
+              std::to_string(need) + " bytes of work at precision " +
+              std::to_string(o->precision) + ", above the fixed " +
+              std::to_string((long long)IFDDA_NOALLOC_BYTES) + " (use ifdda_solve_device)");
+    return IFDDA_ERR_ARG;
+  }
+  int dev = 0;
+  if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return IFDDA_ERR_HIP;
+  if (dev < 0 || dev >= kNoAllocDevices) {
+    set_error("ifdda_solve_device_noalloc: device index out of range");
+    return IFDDA_ERR_ARG;
+  }
+  unsigned char* base = nullptr;
+  int rc = noalloc_base(dev, &base);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu[dev]);
+  noalloc_layout(&S, n, need32, base);
+  KState host_state;  // polled copy of the device state (pageable)
+  S.st_host = &host_state;
+  S.n = n;
+  S.R = 1;
+  S.nblk = blocks_for(n);
+  S.part_blocks = S.nblk;
+  S.total = n;
+  S.cap_total = n;
+  S.cap_R = 1;
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  // every buffer is written before it is read (init kernels, state upload),
+  // so nothing a previous call left in the array leaks into this one
+  rc = solve_core(CtxOp(c), &S, o, 1, (const double2*)E_inc, (double2*)FF, (double2*)FFloc, info,
+                  nstop, s, t0);
+  // S only borrows the array: clear its pointers so nothing frees them
+  for (int i = 0; i < 14; ++i) {
+    S.v64[i] = nullptr;
+    S.v32[i] = nullptr;
+  }
+  S.part = nullptr;
+  S.st = nullptr;
+  S.st_host = nullptr;
+  S.act = nullptr;
+  S.rr64 = nullptr;
+  return rc;
 }

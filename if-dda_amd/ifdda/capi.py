@@ -97,6 +97,8 @@ def lib():
                               C.POINTER(C.c_int)]
     L.ifdda_solve_device.argtypes = [vp, C.POINTER(SolveOpts), vp, vp, vp, C.POINTER(SolveInfo),
                                      C.POINTER(C.c_int), vp]
+    L.ifdda_solve_device_noalloc.argtypes = [vp, C.POINTER(SolveOpts), vp, vp, vp,
+                                             C.POINTER(SolveInfo), C.POINTER(C.c_int), vp]
     L.ifdda_solve_multi.argtypes = [vp, C.POINTER(SolveOpts), ip, vp, vp, vp, C.POINTER(SolveInfo),
                                     C.POINTER(C.c_int)]
     L.ifdda_solve_multi_device.argtypes = [vp, C.POINTER(SolveOpts), ip, vp, vp, vp,
@@ -303,6 +305,21 @@ class Solver:
                                         C.byref(info), C.byref(stop),
                                         C.c_void_p(stream) if stream else None),
                "ifdda_solve_device")
+        return info.as_dict()
+
+    def solve_device_noalloc(self, Einc_ptr, FF_ptr, FFloc_ptr, method="GPBICG1", tol=1e-4,
+                             precision=64, x0_zero=False, maxit=10000, check_every=0, stream=None):
+        """solve_device with the work in the library's fixed device array: no
+        device allocation in the call (one RHS, up to IFDDA_NOALLOC_BYTES of work)."""
+        o = self._opts(method, tol, precision, x0_zero, maxit, check_every)
+        info = SolveInfo()
+        stop = C.c_int(0)
+        _check(lib().ifdda_solve_device_noalloc(self.h, C.byref(o), C.c_void_p(Einc_ptr),
+                                                C.c_void_p(FF_ptr) if FF_ptr else None,
+                                                C.c_void_p(FFloc_ptr) if FFloc_ptr else None,
+                                                C.byref(info), C.byref(stop),
+                                                C.c_void_p(stream) if stream else None),
+               "ifdda_solve_device_noalloc")
         return info.as_dict()
 
     def timing_enable(self, cap: int):

@@ -133,6 +133,24 @@ int ifdda_solve(ifdda_ctx *ctx, const ifdda_solve_opts *opts, const double *E_in
 int ifdda_solve_device(ifdda_ctx *ctx, const ifdda_solve_opts *opts, const void *E_inc, void *FF,
                        void *FFloc, ifdda_solve_info *info, volatile int *nstop, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory inside their loop (tight serving loops, code that forbids
+ * hipMalloc): ifdda_solve_device with the Krylov work vectors, partial sums
+ * and solver state in a fixed array of IFDDA_NOALLOC_BYTES that each GPU
+ * holds from the first call until the process ends.  The call itself
+ * allocates no device memory; the context must be set up (Green tensor,
+ * dipoles, polarisability) beforehand.  The work of one right-hand side is
+ * about 224 bytes per vector element (3 per dipole) at precision 64 and 352
+ * at precision 32, so up to ~99,000 dipoles (FP64) or ~63,000 (mixed) fit;
+ * larger problems return IFDDA_ERR_ARG with the sizes in ifdda_last_error().
+ * Results equal ifdda_solve_device's bit for bit.  Calls in one process on
+ * one GPU share the array and run one at a time (a lock); the array keeps
+ * the last call's vectors, and each call overwrites what it reads. */
+#define IFDDA_NOALLOC_BYTES (64ull << 20)
+int ifdda_solve_device_noalloc(ifdda_ctx *ctx, const ifdda_solve_opts *opts, const void *E_inc,
+                               void *FF, void *FFloc, ifdda_solve_info *info, volatile int *nstop,
+                               void *stream);
+
 /* Several right-hand sides (incidences) of one object in one batched solve:
  * E_inc, FF, FFloc hold nrhs vectors of 3*ndipole complex128, one after the
  * other; infos has nrhs entries.  Each RHS runs the same recurrence as a
