@@ -127,8 +127,12 @@ def main():
     ap.add_argument("--workers", type=int, default=8, help="runner workers")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default tile widths (skip ifdda_autotune)")
-    ap.add_argument("--section", choices=["config1", "batch", "uneven", "serving"],
+    ap.add_argument("--section", choices=["config1", "batch", "uneven", "serving", "profile"],
                     help="run only this section and print its result (for profiling)")
+    ap.add_argument("--profile-reps", type=int, default=5,
+                    help="profile section: device matvecs per precision")
+    ap.add_argument("--profile-solves", type=int, default=1,
+                    help="profile section: GPBICG tol 1e-4 solves per precision")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: the benchmark runs on one GPU; --gpus {args.gpus} is not supported "
@@ -138,7 +142,7 @@ def main():
         import torch
         torch.cuda.set_device(0)
         fn = {"config1": bench_config1, "batch": bench_batch, "uneven": bench_uneven,
-              "serving": bench_serving}[args.section]
+              "serving": bench_serving, "profile": bench_profile}[args.section]
         print(json.dumps({args.section: fn(args)}), flush=True)
         return
 
@@ -520,6 +524,59 @@ def bench_serving(args):
                               "solves_per_s": 1.0 / t_mix}
     for r in out["rates"]:
         r["speedup_vs_sequential_fp64"] = r["solves_per_s"] * t_mix
+    return out
+
+
+def bench_profile(args):
+    """A short, fixed workload for rocprofv3 runs on the config-2 box (100^3
+    inhomogeneous cube, 200^3 FFT grid): `--profile-reps` device matvecs per
+    precision, `--profile-solves` GPBICG solves to 1e-4 per precision (from
+    x0 = 0, tuned tile widths), then a plain device copy of a known size
+    (torch's copy kernel, three times, and one fill) that calibrates the
+    FETCH_SIZE / WRITE_SIZE counters per dispatch:
+
+        rocprofv3 --kernel-trace --stats -d OUT -o run -- python3 bench.py --section profile
+        rocprofv3 --pmc FETCH_SIZE -d OUT -o fetch -- python3 bench.py --section profile
+        rocprofv3 --pmc WRITE_SIZE -d OUT -o write -- python3 bench.py --section profile
+    """
+    import torch
+
+    import ifdda
+    from ifdda import scene
+
+    s, k0, alpha, E = scene.config2(args.n)
+    sv = ifdda.Solver(s.nx, s.ny, s.nz)
+    sv.greencalculfft(s.d, k0)
+    sv.set_dipoles(s.tabdip)
+    sv.set_polarizability(alpha)
+    out = {"tiles": sv.autotune() if not args.no_tune else sv.tile_widths()}
+    for prec in (64, 32):
+        cdt = np.complex128 if prec == 64 else np.complex64
+        rdt = np.float64 if prec == 64 else np.float32
+        x = torch.from_numpy(E.astype(cdt).view(rdt).copy()).cuda()
+        y = torch.empty_like(x)
+        for _ in range(args.profile_reps):
+            sv.matvec_device(x.data_ptr(), x.data_ptr(), y.data_ptr(), prec=prec)
+        torch.cuda.synchronize()
+        print(f"profile: matvec fp{prec} x{args.profile_reps} done", file=sys.stderr, flush=True)
+    Ed = torch.from_numpy(E.view(np.float64).copy()).cuda()
+    FFd = torch.empty_like(Ed)
+    FLd = torch.empty_like(Ed)
+    for prec in (64, 32):
+        for _ in range(args.profile_solves):
+            info = sv.solve_device(Ed.data_ptr(), FFd.data_ptr(), FLd.data_ptr(), "GPBICG1", 1e-4,
+                                   precision=prec, x0_zero=True)
+            assert info["nstop"] == 0 and info["tol1"] <= 1e-4, info
+            out[f"solve_fp{prec}"] = {"nloop": info["nloop"], "seconds": info["seconds"]}
+        print(f"profile: solves precision {prec} done", file=sys.stderr, flush=True)
+    nbytes = 512 << 20
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    out["calibration"] = f"copy of {nbytes} bytes x3 and a fill of {nbytes} bytes"
+    sv.close()
     return out
 
 
