@@ -312,6 +312,12 @@ extern "C" int ifdda_use_specialized(ifdda_ctx* c, int on) {
   return IFDDA_OK;
 }
 
+extern "C" int ifdda_fuse_krylov(ifdda_ctx* c, int mask) {
+  if (!c || mask < 0 || mask > 3) return arg_error("ifdda_fuse_krylov: bad argument");
+  c->fuse_mask = mask;
+  return IFDDA_OK;
+}
+
 extern "C" int ifdda_specialized_z(ifdda_ctx* c, int prec, int* has) {
   if (!c || !has || (prec != 32 && prec != 64)) return arg_error("ifdda_specialized_z: bad argument");
   *has = has_specialized_z(c->g, prec) ? 1 : 0;

@@ -19,6 +19,7 @@
 // lines of a tile are B consecutive kx values (contiguous in HBM), so every
 // global row access is a contiguous B*NC-element segment.
 #include "ifdda_internal.h"
+#include "ifdda_krylov.h"
 
 #include <algorithm>
 #include <cmath>
@@ -819,12 +820,21 @@ __global__ void __launch_bounds__(256) k_zconv_ct(SArgs<V> a) {
 
 // The compile-time z convolutions built into the library: (L, B) of the
 // 40^3, 128^3, 192^3 and 200^3 grids at their default tile widths (L = 40:
-// width 5 for one right-hand side, 27 for batches).
+// width 5 for one right-hand side, 27 for batches) and at the neighbouring
+// widths the autotuner may pick (4 to 8 lines).
 template <typename V> using ZconvKernel = void (*)(SArgs<V>);
 template <typename V> ZconvKernel<V> zconv_ct_for(int L, int B) {
+  if (L == 200 && B == 4) return k_zconv_ct<V, 200, 4>;
   if (L == 200 && B == 5) return k_zconv_ct<V, 200, 5>;
+  if (L == 200 && B == 6) return k_zconv_ct<V, 200, 6>;
+  if (L == 200 && B == 8) return k_zconv_ct<V, 200, 8>;
+  if (L == 128 && B == 4) return k_zconv_ct<V, 128, 4>;
+  if (L == 128 && B == 5) return k_zconv_ct<V, 128, 5>;
+  if (L == 128 && B == 6) return k_zconv_ct<V, 128, 6>;
   if (L == 128 && B == 8) return k_zconv_ct<V, 128, 8>;
+  if (L == 192 && B == 4) return k_zconv_ct<V, 192, 4>;
   if (L == 192 && B == 5) return k_zconv_ct<V, 192, 5>;
+  if (L == 192 && B == 6) return k_zconv_ct<V, 192, 6>;
   if (L == 40 && B == 5) return k_zconv_ct<V, 40, 5>;
   if (L == 40 && B == 27) return k_zconv_ct<V, 40, 27>;
   return nullptr;
@@ -907,15 +917,27 @@ struct DivStep {
   }
 };
 
+// A complex value of V in FP64 (the fused inner products are summed in FP64).
+template <typename V> __device__ __forceinline__ double2 to_d(V v) {
+  return make_double2((double)v.x, (double)v.y);
+}
+
 // Forward (SGN > 0): DIT from a permuted load, first stage pruned when
 // 2 n_in <= L.  Inverse (SGN < 0): DIF from a natural load, last stage pruned
 // when 2 n_out <= L, output read through pos.
 //
 // xpass_tile: the tile of box lines g0 .. g0 + B - 1, nthr threads (PK: as
 // strided_tile).
+// MODE 1: P1 (scatter alpha x, FFT-x).  MODE 2 / 5 / 6: P5 (IFFT-x, gather
+// y = b - v/M); 5 and 6 also form the inner products of y (KFuse p5 = 1, 2).
+// MODE 0: plain x-FFT of full lines.  f and zo (the RHS's element offset in
+// the Krylov vectors) are used by modes 5 and 6 only.
 template <typename V, int NC, int SGN, int MODE, bool PK>
 __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* zdst,
-                                           const V* zxin, V* zxout, int g0, int tid0, int nthr) {
+                                           const V* zxin, V* zxout, int g0, int tid0, int nthr,
+                                           const KFuse<V>* f = nullptr, long long zo = 0) {
+  constexpr bool kP5 = MODE == 2 || MODE == 5 || MODE == 6;
+  constexpr int kND = MODE == 5 ? 1 : MODE == 6 ? 5 : 0;  // inner products formed
   constexpr bool DIT = SGN > 0;
   extern __shared__ double4 smem_d4[];
   V* tw = reinterpret_cast<V*>(smem_d4);
@@ -992,8 +1014,11 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
   else
     tile_fft_dif<SGN>(buf, tw, a.ax, NL, line, jt, JT, half_out);
 
-  if (MODE == 2) {
+  if constexpr (kP5) {
     constexpr int U = kBatch / 2;
+    double2 acc[kND > 0 ? kND : 1];
+#pragma unroll
+    for (int q = 0; q < (kND > 0 ? kND : 1); ++q) acc[q] = make_double2(0.0, 0.0);
     DivStep ix(tid, nthr, a.n_out);
     for (int e0 = tid; e0 < a.B * a.n_out; e0 += U * nthr) {
       int ll[U], ii[U];
@@ -1020,10 +1045,52 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
         if (dip[u] >= 0) {
           const V* v = buf + (DIT ? ii[u] : pos[ii[u]]) * NL + ll[u] * 3;
           V* y = zxout + dip[u] * 3;
-          y[0] = csub(b[u][0], cscal(v[0], a.inv_m));
-          y[1] = csub(b[u][1], cscal(v[1], a.inv_m));
-          y[2] = csub(b[u][2], cscal(v[2], a.inv_m));
+          V yo[3];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            yo[c] = csub(b[u][c], cscal(v[c], a.inv_m));
+            y[c] = yo[c];
+          }
+          if constexpr (MODE == 5) {  // <r0*, Ap> (gpbicg.f:174-181)
+            const long long e = zo + dip[u] * 3;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[0] = zadd(acc[0], zmul(to_d(f->r0s[e + c]), to_d(yo[c])));
+          } else if constexpr (MODE == 6) {  // gpbicg.f:222-253
+            const long long e = zo + dip[u] * 3;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              const double2 At = to_d(yo[c]), yv = to_d(f->y[e + c]), t = to_d(b[u][c]);
+              const double2 cAt = zconj(At), cy = zconj(yv);
+              acc[0] = zadd(acc[0], zmul(cAt, At));
+              acc[1] = zadd(acc[1], zmul(cy, yv));
+              acc[2] = zadd(acc[2], zmul(cAt, yv));
+              acc[3] = zadd(acc[3], zmul(cAt, t));
+              acc[4] = zadd(acc[4], zmul(cy, t));
+            }
+          }
         }
+    }
+    if constexpr (kND > 0) {
+      // one partial per block and inner product, in a fixed order: threads
+      // in LDS, the first wavefront sums four strided values, then shuffles
+      __shared__ double2 red[256];
+      double2* part = f->part + (long long)blockIdx.z * kND * f->pstride + blockIdx.x;
+#pragma unroll
+      for (int q = 0; q < kND; ++q) {
+        __syncthreads();
+        red[tid0] = acc[q];
+        __syncthreads();
+        if (tid0 < 64) {
+          double2 sum = red[tid0];
+          for (int k = 1; k < 4; ++k)
+            if (tid0 + 64 * k < (int)blockDim.x) sum = zadd(sum, red[tid0 + 64 * k]);
+          for (int off = 32; off > 0; off >>= 1) {
+            sum.x += __shfl_down(sum.x, off, 64);
+            sum.y += __shfl_down(sum.y, off, 64);
+          }
+          if (tid0 == 0) part[(long long)q * f->pstride] = sum;
+        }
+      }
     }
   } else {
     DivStep ix(tid, nthr, a.n_out);
@@ -1047,6 +1114,19 @@ __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
       a.xin ? a.xin + blockIdx.z * a.zs_x : nullptr,
       a.xout ? a.xout + blockIdx.z * a.zs_x : nullptr, xcd_remap(blockIdx.x, gridDim.x) * a.B,
       threadIdx.x, blockDim.x);
+}
+
+// P5 with GPBICG inner products fused in (KFuse; one context, R right-hand
+// sides zs_x elements apart).
+template <typename V, int NC, int SGN, int MODE>
+__global__ void __launch_bounds__(256) k_xpass_f(XArgs<V> a, KFuse<V> f) {
+  if (a.skip && a.skip[blockIdx.z * a.skip_stride]) return;
+  xpass_tile<V, NC, SGN, MODE, false>(
+      a, a.src ? a.src + blockIdx.z * a.zs_src : nullptr,
+      a.dst ? a.dst + blockIdx.z * a.zs_dst : nullptr,
+      a.xin ? a.xin + blockIdx.z * a.zs_x : nullptr,
+      a.xout ? a.xout + blockIdx.z * a.zs_x : nullptr, xcd_remap(blockIdx.x, gridDim.x) * a.B,
+      threadIdx.x, blockDim.x, &f, (long long)blockIdx.z * a.zs_x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1688,7 +1768,8 @@ static void pass_set(const GridPlan& g, const MatvecDev<V>& m, int pi, bool sing
 
 template <typename V>
 static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, const V* b, V* y,
-                        hipStream_t s, int pi, const int* skip, PassTimer* tm) {
+                        hipStream_t s, int pi, const int* skip, PassTimer* tm,
+                        const KFuse<V>* f = nullptr) {
   const int R = std::max(1, m.nrhs);
   const long long zA = (long long)g.nz * g.ny * g.L[0] * 3;  // A per RHS
   const long long zB = (long long)g.nz * g.L[1] * g.L[0] * 3;  // B per RHS
@@ -1763,11 +1844,21 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.xout = y;
     a.zs_x = m.vstride;
     a.zs_src = zA;
-    auto k = k_xpass<V, 3, -1, 2>;
-    if (!allow_lds(k, ps.t[4]->lds)) return false;
-    if (tm) tm->begin(pi * 5 + 4, s);
-    hipLaunchKernelGGL(k, grid(4), dim3(ps.t[4]->threads), ps.t[4]->lds, s, a);
-    if (tm) tm->end(s);
+    if (f && f->p5) {
+      // the block sum of the fused inner products needs a full first wavefront
+      if (ps.t[4]->threads < 64 || f->pstride < ps.gx[4]) return false;
+      auto k = f->p5 == 1 ? k_xpass_f<V, 3, -1, 5> : k_xpass_f<V, 3, -1, 6>;
+      if (!allow_lds(k, ps.t[4]->lds)) return false;
+      if (tm) tm->begin(pi * 5 + 4, s);
+      hipLaunchKernelGGL(k, grid(4), dim3(ps.t[4]->threads), ps.t[4]->lds, s, a, *f);
+      if (tm) tm->end(s);
+    } else {
+      auto k = k_xpass<V, 3, -1, 2>;
+      if (!allow_lds(k, ps.t[4]->lds)) return false;
+      if (tm) tm->begin(pi * 5 + 4, s);
+      hipLaunchKernelGGL(k, grid(4), dim3(ps.t[4]->threads), ps.t[4]->lds, s, a);
+      if (tm) tm->end(s);
+    }
   }
   return hip_ok(hipGetLastError(), "matvec launch");
 }
@@ -1879,6 +1970,23 @@ bool launch_matvec_f32(const GridPlan& g, const MatvecDev<float2>& m, const floa
                        const float2* b, float2* y, hipStream_t s, const int* skip,
                        PassTimer* timer) {
   return matvec_impl(g, m, x, b, y, s, 1, skip, timer ? timer : m.timer);
+}
+
+int matvec_p5_blocks(const GridPlan& g, int prec, int R) {
+  const int pi = prec == 64 ? 0 : 1;
+  const TileShape& t = R <= 1 ? g.one[pi][4] : g.p5[pi];
+  if (t.B < 1 || t.threads < 64) return 0;
+  return (g.ny * g.nz + t.B - 1) / t.B;
+}
+bool launch_matvec_fused_f64(const GridPlan& g, const MatvecDev<double2>& m, const double2* x,
+                             const double2* b, double2* y, hipStream_t s, const int* skip,
+                             const KFuse<double2>& f) {
+  return matvec_impl(g, m, x, b, y, s, 0, skip, m.timer, &f);
+}
+bool launch_matvec_fused_f32(const GridPlan& g, const MatvecDev<float2>& m, const float2* x,
+                             const float2* b, float2* y, hipStream_t s, const int* skip,
+                             const KFuse<float2>& f) {
+  return matvec_impl(g, m, x, b, y, s, 1, skip, m.timer, &f);
 }
 
 // In-place forward 3D FFT (sign +1) of one FP64 component on the full grid.

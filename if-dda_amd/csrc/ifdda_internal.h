@@ -123,6 +123,33 @@ bool launch_matvec_f32(const GridPlan& g, const MatvecDev<float2>& m, const floa
                        const float2* b, float2* y, hipStream_t s, const int* skip = nullptr,
                        PassTimer* timer = nullptr);
 
+struct KState;
+// GPBICG inner products fused into the last pass of a matvec (gpbicg.f:174-181,
+// :222-253), for launches of one context: P5 forms the products of the
+// vector it gathers, y = b - v/M, as it writes it.  Every vector holds R
+// right-hand sides vstride apart.
+//   p5 = 1: <r0*, Ap>, Ap = y of the matvec
+//   p5 = 2: <At,At>, <y,y>, <At,y>, <At,t>, <y,t>, At = y of the matvec,
+//           t = b of the matvec
+// The products are the unfused kernels' (conj where they conjugate), in
+// FP64, one partial per P5 block: part[(rhs * nd + q) * pstride + block].
+template <typename V> struct KFuse {
+  int p5 = 0;
+  const V* r0s = nullptr;  // p5 = 1
+  const V* y = nullptr;    // p5 = 2
+  double2* part = nullptr;
+  int pstride = 0;
+};
+// Blocks of the P5 pass of a launch over R right-hand sides (the partials a
+// fused P5 writes per inner product and RHS).
+int matvec_p5_blocks(const GridPlan& g, int prec, int R);
+bool launch_matvec_fused_f64(const GridPlan& g, const MatvecDev<double2>& m, const double2* x,
+                             const double2* b, double2* y, hipStream_t s, const int* skip,
+                             const KFuse<double2>& f);
+bool launch_matvec_fused_f32(const GridPlan& g, const MatvecDev<float2>& m, const float2* x,
+                             const float2* b, float2* y, hipStream_t s, const int* skip,
+                             const KFuse<float2>& f);
+
 // Packed matvec over P independent problems (members): the per-member
 // argument blocks and block offsets of the five passes live in one device
 // blob whose layout (byte offsets) is described here.
@@ -211,6 +238,9 @@ struct ifdda_ctx {
   size_t lds_limit = 65536;  // dynamic LDS per workgroup the tiles may use
   ifdda::PassTimer timer;  // enabled by ifdda_timing_enable
   bool specialized = true;  // ifdda_use_specialized
+  // GPBICG inner products fused into the matvec ends (ifdda_fuse_krylov):
+  // bit 0 FP64 solves, bit 1 FP32 (the inner solves of mixed precision)
+  int fuse_mask = 2;
 
   ifdda::MatvecDev<double2> mv64() const {
     ifdda::MatvecDev<double2> m;
@@ -262,6 +292,22 @@ struct MvOp {
                      const int* skip, int R, long long n, int skip_stride) const;
   virtual bool mvt32(const float2* x, const float2* zero, float2* y, hipStream_t s,
                      const int* skip, int R, long long n, int skip_stride) const;
+  // The matvec with GPBICG inner products fused into its end (KFuse).  fuse_blocks:
+  // P5 blocks per RHS of such a launch, 0 when this operator does not fuse
+  // at precision prec (the solver then runs the separate vector kernels).
+  virtual int fuse_blocks(int prec, int R) const { (void)prec; (void)R; return 0; }
+  virtual bool mvf64(const double2* x, const double2* b, double2* y, hipStream_t s,
+                     const int* skip, int R, long long n, int skip_stride,
+                     const KFuse<double2>& f) const {
+    (void)x; (void)b; (void)y; (void)s; (void)skip; (void)R; (void)n; (void)skip_stride; (void)f;
+    return false;
+  }
+  virtual bool mvf32(const float2* x, const float2* b, float2* y, hipStream_t s,
+                     const int* skip, int R, long long n, int skip_stride,
+                     const KFuse<float2>& f) const {
+    (void)x; (void)b; (void)y; (void)s; (void)skip; (void)R; (void)n; (void)skip_stride; (void)f;
+    return false;
+  }
 };
 // y = x + alpha^T y for one right-hand side of nd dipoles (y holding -T x:
 // the transposed operator's last step), unless *skip is set.

@@ -150,6 +150,50 @@ __device__ __forceinline__ void finish_sum(KState* st, const double2* part, int 
   __syncthreads();
 }
 
+// As finish_sum for any number of partials (one per block of a matvec pass):
+// each thread sums its strided share in order, then the same fixed-order
+// reduction over threads.
+template <int NR>
+__device__ __forceinline__ void finish_sum_any(KState* st, const double2* part, int nblk) {
+  __shared__ double2 red[NR][kThreads];
+  const int tid = threadIdx.x;
+  constexpr int kU = 8;  // loads issued before their sums
+  double2 s[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) s[r] = make_double2(0.0, 0.0);
+  for (int b0 = tid; b0 < nblk; b0 += kU * kThreads) {
+    double2 v[NR][kU];
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int k = 0; k < kU; ++k) {
+        const int b = b0 + k * kThreads;
+        v[r][k] = b < nblk ? part[r * nblk + b] : make_double2(0.0, 0.0);
+      }
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int k = 0; k < kU; ++k) s[r] = zadd(s[r], v[r][k]);
+  }
+#pragma unroll
+  for (int r = 0; r < NR; ++r) red[r][tid] = s[r];
+  __syncthreads();
+  if (tid < 64) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      double2 s = red[r][tid];
+#pragma unroll
+      for (int k = 1; k < kThreads / 64; ++k) s = zadd(s, red[r][tid + 64 * k]);
+      for (int off = 32; off > 0; off >>= 1) {
+        s.x += __shfl_down(s.x, off, 64);
+        s.y += __shfl_down(s.y, off, 64);
+      }
+      if (tid == 0) st->dots[r] = s;
+    }
+  }
+  __syncthreads();
+}
+
 // Vector geometry of a launch over right-hand sides: RHS r starts at element
 // off(r) and holds len(r) elements -- r * n and n, or, for packed problems
 // of different sizes, offs[r] and lens[r] (ragged, back to back).
@@ -215,6 +259,8 @@ struct Solver {
   long long cap_total = 0;   // capacity of the buffers: elements, right-hand sides
   int cap_R = 0;
   int part_blocks = 0;       // blocks per RHS the partial-sum array holds
+  double2* part5 = nullptr;  // partials of inner products formed in fused matvec ends
+  long long part5_cap = 0;   // elements of part5
 };
 
 // Outcome of one Krylov run for one right-hand side.
@@ -233,6 +279,21 @@ template <> inline bool mv<double2>(const MvOp& op, const double2* x, const doub
 template <> inline bool mv<float2>(const MvOp& op, const float2* x, const float2* b, float2* y,
                             hipStream_t s, const int* skip, int R, long long n, int skip_stride) {
   return op.mv32(x, b, y, s, skip, R, n, skip_stride);
+}
+
+// The matvec with GPBICG work fused into its ends (MvOp::mvf64 / mvf32).
+template <typename V> bool mvf(const MvOp& op, const V* x, const V* b, V* y, hipStream_t s,
+                               const int* skip, int R, long long n, int skip_stride,
+                               const KFuse<V>& f);
+template <> inline bool mvf<double2>(const MvOp& op, const double2* x, const double2* b, double2* y,
+                                     hipStream_t s, const int* skip, int R, long long n,
+                                     int skip_stride, const KFuse<double2>& f) {
+  return op.mvf64(x, b, y, s, skip, R, n, skip_stride, f);
+}
+template <> inline bool mvf<float2>(const MvOp& op, const float2* x, const float2* b, float2* y,
+                                    hipStream_t s, const int* skip, int R, long long n,
+                                    int skip_stride, const KFuse<float2>& f) {
+  return op.mvf32(x, b, y, s, skip, R, n, skip_stride, f);
 }
 
 // The transposed operator over R right-hand sides (MvOp::mvt64 / mvt32).

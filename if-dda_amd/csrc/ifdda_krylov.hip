@@ -135,22 +135,35 @@ __global__ void __launch_bounds__(kThreads) g_k2(const KState* st, GV<V> v, VN n
   block_partials<1>(acc, part, nblk);
 }
 
+// alpha = r0rn / <r0*, Ap>, or breakdown (gpbicg.f:183-187)
+__device__ __forceinline__ void gp_alpha(KState* st) {
+  const double2 c = st->dots[0];
+  if (zzero(c)) {
+    st->status = -1;
+    st->steperr = 1;
+    st->done = 1;
+  } else {
+    st->alpha = zdiv(st->r0rn, c);
+  }
+}
+
 __global__ void __launch_bounds__(kThreads) g_f2(KState* st, const double2* part, int nblk) {
   const int rh_ = blockIdx.y;  // right-hand side
   st += rh_;
   part += (long long)rh_ * kMaxDots * nblk;
   if (st->done) return;
   finish_sum<1>(st, part, nblk);
-  if (threadIdx.x == 0) {
-    const double2 c = st->dots[0];
-    if (zzero(c)) {  // gpbicg.f:183-187
-      st->status = -1;
-      st->steperr = 1;
-      st->done = 1;
-    } else {
-      st->alpha = zdiv(st->r0rn, c);
-    }
-  }
+  if (threadIdx.x == 0) gp_alpha(st);
+}
+
+// g_f2 on the partials of a fused P5 pass (nblk per RHS)
+__global__ void __launch_bounds__(kThreads) g_f2x(KState* st, const double2* part, int nblk) {
+  const int rh_ = blockIdx.y;
+  st += rh_;
+  part += (long long)rh_ * nblk;
+  if (st->done) return;
+  finish_sum_any<1>(st, part, nblk);
+  if (threadIdx.x == 0) gp_alpha(st);
 }
 
 // w = At + beta Apold (0 in the first iteration; gpbicg.f:319-325 of the
@@ -199,15 +212,11 @@ __global__ void __launch_bounds__(kThreads) g_k4(const KState* st, GV<V> v, VN n
   block_partials<5>(acc, part, nblk);
 }
 
+// zeta and eta from the five products, or breakdown (gpbicg.f:217-264);
 // gp2: the steepest-descent step (eta = 0) on every even iteration
 // (gpbicg2.f:75-88), not only the first
-__global__ void __launch_bounds__(kThreads) g_f4(KState* st, const double2* part, int nblk, int gp2) {
-  const int rh_ = blockIdx.y;  // right-hand side
-  st += rh_;
-  part += (long long)rh_ * kMaxDots * nblk;
-  if (st->done) return;
-  finish_sum<5>(st, part, nblk);
-  if (threadIdx.x == 0) {
+__device__ __forceinline__ void gp_zeta(KState* st, int gp2) {
+  {
     const double2 c1 = st->dots[0], c2 = st->dots[1], c3 = st->dots[2], c4 = st->dots[3],
                   c5 = st->dots[4];
     if (gp2 ? st->it % 2 == 0 : st->it == 0) {  // gpbicg.f:217-235
@@ -231,6 +240,25 @@ __global__ void __launch_bounds__(kThreads) g_f4(KState* st, const double2* part
       st->eta = zdiv(zsub(zmul(c1, c5), zmul(zconj(c3), c4)), den);
     }
   }
+}
+
+__global__ void __launch_bounds__(kThreads) g_f4(KState* st, const double2* part, int nblk, int gp2) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
+  if (st->done) return;
+  finish_sum<5>(st, part, nblk);
+  if (threadIdx.x == 0) gp_zeta(st, gp2);
+}
+
+// g_f4 on the partials of a fused P5 pass (5 x nblk per RHS)
+__global__ void __launch_bounds__(kThreads) g_f4x(KState* st, const double2* part, int nblk, int gp2) {
+  const int rh_ = blockIdx.y;
+  st += rh_;
+  part += (long long)rh_ * 5 * nblk;
+  if (st->done) return;
+  finish_sum_any<5>(st, part, nblk);
+  if (threadIdx.x == 0) gp_zeta(st, gp2);
 }
 
 // u, z, x, r updates; partial |r|^2 and r0* r (gpbicg.f:270-281, :300-307)
@@ -1407,6 +1435,7 @@ void solver_delete(Solver* S) {
   (void)hipFree(S->rr64);
   (void)hipFree(S->lens);
   (void)hipFree(S->offs);
+  (void)hipFree(S->part5);
   delete S;
 }
 
@@ -1564,6 +1593,26 @@ struct CtxOp : MvOp {
                        (const double2*)c->al64, c->alpha_mode, c->nd);
     return hip_ok(hipGetLastError(), "alpha apply");
   }
+  int fuse_blocks(int prec, int R) const override {
+    if (!c->specialized || !((c->fuse_mask >> (prec == 64 ? 0 : 1)) & 1)) return 0;
+    return matvec_p5_blocks(c->g, prec, R);
+  }
+  bool mvf64(const double2* x, const double2* b, double2* y, hipStream_t s, const int* skip, int R,
+             long long n, int skip_stride, const KFuse<double2>& f) const override {
+    MatvecDev<double2> m = c->mv64();
+    m.nrhs = R;
+    m.vstride = n;
+    m.skip_stride = skip_stride;
+    return launch_matvec_fused_f64(c->g, m, x, b, y, s, skip, f);
+  }
+  bool mvf32(const float2* x, const float2* b, float2* y, hipStream_t s, const int* skip, int R,
+             long long n, int skip_stride, const KFuse<float2>& f) const override {
+    MatvecDev<float2> m = c->mv32();
+    m.nrhs = R;
+    m.vstride = n;
+    m.skip_stride = skip_stride;
+    return launch_matvec_fused_f32(c->g, m, x, b, y, s, skip, f);
+  }
   // -T x by the matvec without alpha (b = 0), then alpha^T (-T x) + x
   template <typename V>
   bool transposed(MatvecDev<V> m, const V* al, const V* x, const V* zero, V* y, hipStream_t s,
@@ -1703,7 +1752,41 @@ static bool krylov(const MvOp& op, Solver* S, V** vv, int R, int method, const d
     GV<V> v{x, b, vv[2], vv[3], vv[4], ab[0], tb[0], vv[7], vv[8], vv[9], ab[1], vv[11], tb[1], Ax};
     hipLaunchKernelGGL(g_start<V>, G, T, 0, s, v, x0zero, vn, S->part, nb);
     hipLaunchKernelGGL(g_fstart, F, T, 0, s, S->st, (const double2*)S->part, nb);
-    for (int it = 0;; ++it) {
+    // Fused matvec ends (one context): <r0*, Ap> inside the first matvec's
+    // last pass, the five products of step 4 inside the second's.  The
+    // vectors are those of the separate kernels; only the products are
+    // summed in another order.
+    const int nb5 = S->lens ? 0 : op.fuse_blocks(std::is_same<V, double2>::value ? 64 : 32, R);
+    const bool fuse = nb5 > 0 && S->part5 && S->part5_cap >= 5LL * R * nb5;
+    for (int it = 0; fuse; ++it) {
+      v.told = tb[it & 1];
+      v.t = tb[(it + 1) & 1];
+      v.Ap = ab[it & 1];
+      v.Apold = ab[(it + 1) & 1];
+      hipLaunchKernelGGL(g_k1<V>, G, T, 0, s, (const KState*)S->st, v, vn);
+      KFuse<V> f1;
+      f1.p5 = 1;
+      f1.r0s = v.r0s;
+      f1.part = S->part5;
+      f1.pstride = nb5;
+      if (!mvf<V>(op, v.p, v.p, v.Ap, s, done, R, n, dstride, f1)) return false;
+      hipLaunchKernelGGL(g_f2x, F, T, 0, s, S->st, (const double2*)S->part5, nb5);
+      hipLaunchKernelGGL(g_k3<V>, G, T, 0, s, (const KState*)S->st, v, vn, gp2);
+      KFuse<V> f2;
+      f2.p5 = 2;
+      f2.y = v.y;
+      f2.part = S->part5;
+      f2.pstride = nb5;
+      if (!mvf<V>(op, v.t, v.t, v.At, s, done, R, n, dstride, f2)) return false;
+      hipLaunchKernelGGL(g_f4x, F, T, 0, s, S->st, (const double2*)S->part5, nb5, gp2);
+      hipLaunchKernelGGL(g_k5<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
+      hipLaunchKernelGGL(g_f5, F, T, 0, s, S->st, (const double2*)S->part, nb, trace);
+      if (!hip_ok(hipGetLastError(), "gpbicg launch")) return false;
+      const int st = step_poll(it);
+      if (st < 0) return false;
+      if (st > 0) break;
+    }
+    for (int it = 0; !fuse; ++it) {
       v.told = tb[it & 1];
       v.t = tb[(it + 1) & 1];
       v.Ap = ab[it & 1];
@@ -1927,6 +2010,18 @@ static int solve_dev(ifdda_ctx* c, const ifdda_solve_opts* o, int R, const doubl
   if (ensure_scratch(c, R) != IFDDA_OK) return IFDDA_ERR_HIP;
   Solver* S = solver_get(c, R, o->precision == 32);
   if (!S) return IFDDA_ERR_HIP;
+  if (o->method == IFDDA_GPBICG1 || o->method == IFDDA_GPBICG2) {
+    // partial sums of the inner products formed in fused matvec ends
+    const long long need = 5LL * R * CtxOp(c).fuse_blocks(o->precision, R);
+    if (need > S->part5_cap) {
+      (void)hipFree(S->part5);
+      S->part5 = nullptr;
+      S->part5_cap = 0;
+      if (!hip_ok(hipMalloc(&S->part5, sizeof(double2) * need), "hipMalloc fused partials"))
+        return IFDDA_ERR_HIP;
+      S->part5_cap = need;
+    }
+  }
   return solve_core(CtxOp(c), S, o, R, Einc, FF, FFloc, infos, nstop, s, t0);
 }
 
@@ -2256,7 +2351,7 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Bytes of the work of one right-hand side of n elements; with base set,
 // also points the solver's buffers into base.
-size_t noalloc_layout(Solver* S, long long n, bool need32, unsigned char* base) {
+size_t noalloc_layout(Solver* S, long long n, bool need32, long long part5, unsigned char* base) {
   size_t off = 0;
   auto take = [&](size_t bytes) -> unsigned char* {
     unsigned char* p = base ? base + off : nullptr;
@@ -2267,6 +2362,8 @@ size_t noalloc_layout(Solver* S, long long n, bool need32, unsigned char* base) 
   S->part = (double2*)take(sizeof(double2) * kMaxDots * nblk);
   S->st = (KState*)take(sizeof(KState));
   S->act = (int*)take(sizeof(int));
+  S->part5 = part5 > 0 ? (double2*)take(sizeof(double2) * part5) : nullptr;
+  S->part5_cap = part5;
   for (int i = 0; i < 14; ++i) S->v64[i] = (double2*)take(sizeof(double2) * n);
   if (need32) {
     S->rr64 = (double2*)take(sizeof(double2) * n);
@@ -2302,7 +2399,10 @@ extern "C" int ifdda_solve_device_noalloc(ifdda_ctx* c, const ifdda_solve_opts* 
   if (n == 0) return IFDDA_OK;
   Solver S;
   const bool need32 = o->precision == 32;
-  const size_t need = noalloc_layout(&S, n, need32, nullptr);
+  const long long part5 = o->method == IFDDA_GPBICG1 || o->method == IFDDA_GPBICG2
+                              ? 5LL * CtxOp(c).fuse_blocks(o->precision, 1)
+                              : 0;
+  const size_t need = noalloc_layout(&S, n, need32, part5, nullptr);
   if (need > (size_t)IFDDA_NOALLOC_BYTES) {
     set_error("ifdda_solve_device_noalloc: " + std::to_string(c->nd) + " dipoles need " +
               std::to_string(need) + " bytes of work at precision " +
@@ -2320,7 +2420,7 @@ extern "C" int ifdda_solve_device_noalloc(ifdda_ctx* c, const ifdda_solve_opts* 
   int rc = noalloc_base(dev, &base);
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(g_noalloc_mu[dev]);
-  noalloc_layout(&S, n, need32, base);
+  noalloc_layout(&S, n, need32, part5, base);
   KState host_state;  // polled copy of the device state (pageable)
   S.st_host = &host_state;
   S.n = n;
@@ -2341,6 +2441,7 @@ extern "C" int ifdda_solve_device_noalloc(ifdda_ctx* c, const ifdda_solve_opts* 
     S.v32[i] = nullptr;
   }
   S.part = nullptr;
+  S.part5 = nullptr;
   S.st = nullptr;
   S.st_host = nullptr;
   S.act = nullptr;

@@ -108,6 +108,7 @@ def lib():
     L.ifdda_timing_enable.argtypes = [vp, ip]
     L.ifdda_timing_passes.argtypes = [vp, ip, ip]
     L.ifdda_use_specialized.argtypes = [vp, ip]
+    L.ifdda_fuse_krylov.argtypes = [vp, ip]
     pd = C.POINTER(C.c_double)
     L.ifdda_cross_sections_device.argtypes = [vp, ip, vp, vp, vp, dp, pd, pd, vp]
     L.ifdda_farfield_device.argtypes = [vp, ip, vp, ip, dp, dp, dp, dp, dp, vp, vp,
@@ -494,6 +495,10 @@ class Solver:
     def use_specialized(self, on: bool):
         """Select compile-time (default) or generic z-pass kernels."""
         _check(lib().ifdda_use_specialized(self.h, int(bool(on))), "ifdda_use_specialized")
+
+    def fuse_krylov(self, mask: int):
+        """GPBICG work fused into the matvec ends: bit 0 FP64, bit 1 FP32 (default 2)."""
+        _check(lib().ifdda_fuse_krylov(self.h, int(mask)), "ifdda_fuse_krylov")
 
     def specialized_z(self, prec: int = 64) -> bool:
         """Whether this grid's z pass has a compile-time kernel."""

@@ -181,6 +181,18 @@ int ifdda_timing_passes(ifdda_ctx *ctx, int mask, int every);
  * library has one for this grid, 0 always runs the generic kernels.  Both
  * compute the same products; tests compare them. */
 int ifdda_use_specialized(ifdda_ctx *ctx, int on);
+
+/* GPBICG inner products fused into its matvecs (no counterpart in the
+ * reference, whose gpbicg.f forms each product in a loop of its own): the
+ * products of gpbicg.f:174-181 and :222-253 are summed by the pass that
+ * gathers the matvec's result, as it writes it, instead of by separate
+ * kernels that read the vectors again.  mask bit 0: FP64 solves, bit 1: FP32
+ * (the inner solves of mixed precision; the default is 2).  The vectors are
+ * the same; the products are summed in another fixed order, so fused and
+ * unfused trajectories agree to rounding.  Single-context solves with the
+ * specialized kernels on (ifdda_use_specialized) only; packs always run the
+ * separate kernels. */
+int ifdda_fuse_krylov(ifdda_ctx *ctx, int mask);
 /* *has = 1 when this grid's z pass has a compile-time kernel for prec 64/32. */
 int ifdda_specialized_z(ifdda_ctx *ctx, int prec, int *has);
 
