@@ -1879,7 +1879,10 @@ static size_t pack_blob_impl(const std::vector<const GridPlan*>& g,
     lay.boff[p] = off;
     off = al(off + sizeof(int) * (P + 1));
   }
+  lay.mb = off;
+  off = al(off + sizeof(int) * 5 * P);
   blob.assign(off, 0);
+  lay.mbh.assign(5 * (size_t)P, 0);
   for (int p = 0; p < 5; ++p) {
     lay.nblk[p] = 0;
     lay.lds[p] = 0;
@@ -1896,6 +1899,8 @@ static size_t pack_blob_impl(const std::vector<const GridPlan*>& g,
       bo[i] = lay.nblk[p];
       lay.nblk[p] += ps.gx[p] * ps.gy[p];
       bo[i + 1] = lay.nblk[p];
+      lay.mbh[(size_t)p * P + i] = ps.gx[p] * ps.gy[p];
+      reinterpret_cast<int*>(blob.data() + lay.mb)[(size_t)p * P + i] = ps.gx[p] * ps.gy[p];
       lay.lds[p] = std::max(lay.lds[p], ps.t[p]->lds);
     }
     // Green items per thread of the z pass (as the single-problem launch)
@@ -1942,12 +1947,49 @@ static bool matvec_pack_impl(const PackLayout& lay, const void* dev, const V* x,
       !allow_lds(k4, lay.lds[3]) || !allow_lds(k5, lay.lds[4]))
     return false;
   const dim3 T(256);
+  if (lay.nblk[0] == 0) return true;  // every member has finished (compacted away)
   hipLaunchKernelGGL(k1, dim3(lay.nblk[0]), T, lay.lds[0], s, args(0, x, nullptr));
   hipLaunchKernelGGL(k2, dim3(lay.nblk[1]), T, lay.lds[1], s, args(1, nullptr, nullptr));
   hipLaunchKernelGGL(k3, dim3(lay.nblk[2]), T, lay.lds[2], s, args(2, nullptr, nullptr));
   hipLaunchKernelGGL(k4, dim3(lay.nblk[3]), T, lay.lds[3], s, args(3, nullptr, nullptr));
   hipLaunchKernelGGL(k5, dim3(lay.nblk[4]), T, lay.lds[4], s, args(4, b, y));
   return hip_ok(hipGetLastError(), "packed matvec launch");
+}
+
+// Block offsets of the five packed passes from the blocks per member (mb,
+// pass-major) and the members' solver state: members that have finished get
+// none (unless all).  One thread per pass, members in order.
+struct BoffOut {
+  int* bo[5];
+};
+__global__ void k_pack_boff(const int* __restrict__ mb, int P, const KState* __restrict__ st,
+                            int all, BoffOut o) {
+  const int p = threadIdx.x;
+  if (p >= 5) return;
+  int* bo = o.bo[p];
+  int acc = 0;
+  bo[0] = 0;
+  for (int m = 0; m < P; ++m) {
+    if (all || !st[m].done) acc += mb[(size_t)p * P + m];
+    bo[m + 1] = acc;
+  }
+}
+
+bool pack_compact(PackLayout& lay, void* dev, const KState* st, const int* host_done, bool all,
+                  hipStream_t s) {
+  if (!dev || lay.P == 0) return true;
+  unsigned char* base = static_cast<unsigned char*>(dev);
+  BoffOut o;
+  for (int p = 0; p < 5; ++p) {
+    o.bo[p] = reinterpret_cast<int*>(base + lay.boff[p]);
+    int n = 0;
+    for (int m = 0; m < lay.P; ++m)
+      if (all || !host_done[m]) n += lay.mbh[(size_t)p * lay.P + m];
+    lay.nblk[p] = n;
+  }
+  hipLaunchKernelGGL(k_pack_boff, dim3(1), dim3(64), 0, s,
+                     reinterpret_cast<const int*>(base + lay.mb), lay.P, st, all ? 1 : 0, o);
+  return hip_ok(hipGetLastError(), "pack compaction");
 }
 
 bool launch_matvec_pack_f64(const PackLayout& lay, const void* dev, const double2* x,

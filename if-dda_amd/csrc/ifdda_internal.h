@@ -156,10 +156,17 @@ bool launch_matvec_fused_f32(const GridPlan& g, const MatvecDev<float2>& m, cons
 struct PackLayout {
   int P = 0;
   size_t av[5] = {}, boff[5] = {};  // byte offsets in the blob
+  size_t mb = 0;                    // byte offset of the blocks per member and pass (5 x P ints)
   int nblk[5] = {};                 // blocks of each packed launch
   size_t lds[5] = {};               // dynamic LDS of each packed launch
   int ng3 = 1;                      // Green items per thread of the z pass
+  std::vector<int> mbh;             // host copy: blocks of member m in pass p at [p * P + m]
 };
+// Rewrite the five block-offset tables of an uploaded blob (dev) so that only
+// members with st[m].done == 0 (or every member, all) get blocks; lay.nblk
+// follows (host_done: the same flags on the host, or null with all).
+bool pack_compact(PackLayout& lay, void* dev, const KState* st, const int* host_done, bool all,
+                  hipStream_t s);
 // Host image of the blob for members (g[i], m[i]) whose dipole vectors start
 // at element offs[i]; returns its size.
 size_t pack_blob_f64(const std::vector<const GridPlan*>& g,
@@ -292,6 +299,15 @@ struct MvOp {
                      const int* skip, int R, long long n, int skip_stride) const;
   virtual bool mvt32(const float2* x, const float2* zero, float2* y, hipStream_t s,
                      const int* skip, int R, long long n, int skip_stride) const;
+  // Launch only the blocks of the RHS that are still running (st[r].done ==
+  // 0, device state; host_done: the same flags on the host) from now on, or
+  // of every RHS again (all = true).  Operators whose launches cover every
+  // RHS in any case do nothing.
+  virtual bool compact(const KState* st, const int* host_done, int R, bool all,
+                       hipStream_t s) const {
+    (void)st; (void)host_done; (void)R; (void)all; (void)s;
+    return true;
+  }
   // The matvec with GPBICG inner products fused into its end (KFuse).  fuse_blocks:
   // P5 blocks per RHS of such a launch, 0 when this operator does not fuse
   // at precision prec (the solver then runs the separate vector kernels).

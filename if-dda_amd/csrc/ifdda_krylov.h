@@ -4,6 +4,7 @@
 // workspace and the loop context the host drivers of the solvers run in.
 #pragma once
 #include <functional>
+#include <vector>
 
 #include "ifdda_internal.h"
 
@@ -90,7 +91,7 @@ template <> struct Vt<float2> {
 
 // -------------------------------------------------------------- reductions
 template <int NR>
-__device__ __forceinline__ void block_partials(double2 (&v)[NR], double2* part, int nblk) {
+__device__ __forceinline__ void block_partials(double2 (&v)[NR], double2* part, int nblk, int bx) {
   for (int off = 32; off > 0; off >>= 1)
     for (int r = 0; r < NR; ++r) {
       v[r].x += __shfl_down(v[r].x, off, 64);
@@ -105,7 +106,7 @@ __device__ __forceinline__ void block_partials(double2 (&v)[NR], double2* part, 
     for (int r = 0; r < NR; ++r) {
       double2 s = make_double2(0.0, 0.0);
       for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s = zadd(s, sh[r][w]);
-      part[r * nblk + blockIdx.x] = s;
+      part[r * nblk + bx] = s;
     }
 }
 
@@ -197,38 +198,67 @@ __device__ __forceinline__ void finish_sum_any(KState* st, const double2* part, 
 // Vector geometry of a launch over right-hand sides: RHS r starts at element
 // off(r) and holds len(r) elements -- r * n and n, or, for packed problems
 // of different sizes, offs[r] and lens[r] (ragged, back to back).
+//
+// Blocks of a vector kernel over n elements (kThreads each, at most kMaxBlocks).
+__host__ __device__ __forceinline__ int blocks_of(long long n) {
+  const long long b = (n + kThreads - 1) / kThreads;
+  return (int)(b < 1 ? 1 : (b > kMaxBlocks ? kMaxBlocks : b));
+}
+//
+// Ragged layouts launch a flat grid: kb (R + 1 ints) gives each RHS its
+// blocks [kb[r], kb[r + 1]), as many as a solve of that RHS alone launches
+// (or none once it has converged), so no block of the grid idles.  Equal
+// lengths launch (nblk, R) with the RHS in blockIdx.y.
 struct VN {
   long long n;
   const long long* lens;
   const long long* offs;
+  const int* kb = nullptr;
+  int R = 0;
   __device__ __forceinline__ long long len(int r) const { return lens ? lens[r] : n; }
   __device__ __forceinline__ long long off(int r) const { return offs ? offs[r] : r * n; }
+  // the RHS of this block (block-uniform: scalar loads)
+  __device__ __forceinline__ int rhs() const {
+    if (!kb) return blockIdx.y;
+    int lo = 0, hi = R;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (kb[mid] <= (int)blockIdx.x)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    return lo;
+  }
+  // the block's index among the blocks of RHS r
+  __device__ __forceinline__ int bx(int r) const { return kb ? (int)blockIdx.x - kb[r] : blockIdx.x; }
 };
 
-// Blocks that serve right-hand side r: the whole grid when all RHS have the
-// same length, else the blocks a solve of RHS r alone would launch.  Either
-// way element i goes to the same block as in that solve, so the partial sums
-// and their order are the same.  The grid's other blocks leave at once
-// (reductions write a zero partial, which the fixed-order finish adds
-// exactly).
+// Blocks that serve right-hand side r: the whole grid row when all RHS have
+// the same length, else the blocks a solve of RHS r alone would launch
+// (flat grid, VN::kb).  Either way element i goes to the same block as in
+// that solve, so the partial sums and their order are the same.  Partial
+// slots past an RHS's block count hold zeros (written when the workspace is
+// laid out), which the fixed-order finish adds exactly.
 __device__ __forceinline__ int rhs_blocks(const VN& n, int r) {
   if (!n.lens) return gridDim.x;
-  const long long b = (n.lens[r] + kThreads - 1) / kThreads;
-  return (int)(b < 1 ? 1 : (b > kMaxBlocks ? kMaxBlocks : b));
+  return n.kb ? n.kb[r + 1] - n.kb[r] : blocks_of(n.lens[r]);
 }
 #define RHS_BLOCKS(n, r, NR)                                                   \
   const int nbr_ = rhs_blocks(n, r);                                           \
-  if ((int)blockIdx.x >= nbr_) {                                               \
+  const int bx_ = (n).bx(r);                                                   \
+  if (bx_ >= nbr_) {                                                           \
     if (threadIdx.x == 0)                                                      \
-      for (int q_ = 0; q_ < NR; ++q_) part[q_ * nblk + blockIdx.x] = make_double2(0.0, 0.0); \
+      for (int q_ = 0; q_ < NR; ++q_) part[q_ * nblk + bx_] = make_double2(0.0, 0.0); \
     return;                                                                    \
   }
 #define RHS_BLOCKS_NOSUM(n, r)          \
   const int nbr_ = rhs_blocks(n, r);    \
-  if ((int)blockIdx.x >= nbr_) return;
+  const int bx_ = (n).bx(r);            \
+  if (bx_ >= nbr_) return;
 // Grid-stride loop over the elements of the block's RHS (nbr_ blocks).
 #define GRID_LOOP(i, n) \
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < (n); i += (long long)nbr_ * blockDim.x)
+  for (long long i = bx_ * (long long)blockDim.x + threadIdx.x; i < (n); i += (long long)nbr_ * blockDim.x)
 
 __device__ __forceinline__ void push_trace(KState* st, double2* trace, double2 a, double2 b,
                                            double2 c, double2 d, double2 e, double res) {
@@ -261,6 +291,9 @@ struct Solver {
   int part_blocks = 0;       // blocks per RHS the partial-sum array holds
   double2* part5 = nullptr;  // partials of inner products formed in fused matvec ends
   long long part5_cap = 0;   // elements of part5
+  int* kb = nullptr;         // ragged layouts: block offsets of the flat vector grid (cap_R + 1)
+  int kb_full = 0;           // blocks of that grid with every RHS in
+  std::vector<long long> lens_h;  // ragged layouts: the lengths on the host
 };
 
 // Outcome of one Krylov run for one right-hand side.
@@ -317,6 +350,7 @@ struct KLoop {
   const MvOp& op;
   Solver* S;
   VN vn;
+  const dim3* grid;  // vector kernels (the flat grid of a ragged layout shrinks as RHS finish)
   int nb, R;
   long long n;
   hipStream_t s;

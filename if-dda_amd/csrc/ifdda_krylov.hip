@@ -35,7 +35,7 @@ namespace ifdda {
 template <typename V>
 __global__ void __launch_bounds__(kThreads) k_init(const V* b, V* x, const V* src, VN n,
                                                   double2* part, int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   part += (long long)rh_ * kMaxDots * nblk;
   b += n.off(rh_);
   x += n.off(rh_);
@@ -47,7 +47,7 @@ __global__ void __launch_bounds__(kThreads) k_init(const V* b, V* x, const V* sr
     acc[0].x += bi.x * bi.x + bi.y * bi.y;
     ST(x, i, src ? LD(src, i) : make_double2(0.0, 0.0));
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 __global__ void __launch_bounds__(kThreads) f_init(KState* st, const double2* part, int nblk) {
@@ -77,7 +77,7 @@ template <typename V> struct GV {
 template <typename V>
 __global__ void __launch_bounds__(kThreads) g_start(GV<V> v, int x0zero, VN n,
                                                    double2* part, int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
   double2 acc[1] = {make_double2(0.0, 0.0)};
@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(kThreads) g_start(GV<V> v, int x0zero, VN n,
     const double2 rs = zconj(r);
     acc[0] = zadd(acc[0], zmul(rs, r));
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 __global__ void __launch_bounds__(kThreads) g_fstart(KState* st, const double2* part, int nblk) {
@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(kThreads) g_fstart(KState* st, const double2* 
 // p = r + beta (p - u) (gpbicg.f:161-169)
 template <typename V>
 __global__ void __launch_bounds__(kThreads) g_k1(const KState* st, GV<V> v, VN n) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   if (st->done) return;
@@ -124,7 +124,7 @@ __global__ void __launch_bounds__(kThreads) g_k1(const KState* st, GV<V> v, VN n
 template <typename V>
 __global__ void __launch_bounds__(kThreads) g_k2(const KState* st, GV<V> v, VN n,
                                                 double2* part, int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
@@ -132,7 +132,7 @@ __global__ void __launch_bounds__(kThreads) g_k2(const KState* st, GV<V> v, VN n
   double2 acc[1] = {make_double2(0.0, 0.0)};
   RHS_BLOCKS(n, rh_, 1);
   GRID_LOOP(i, n.len(rh_)) acc[0] = zadd(acc[0], zmul(LD(v.r0s, i), LD(v.Ap, i)));
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 // alpha = r0rn / <r0*, Ap>, or breakdown (gpbicg.f:183-187)
@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(kThreads) g_f2x(KState* st, const double2* par
 // gpbicg2.f:61 writes it, told - r - alpha w + alpha Ap.
 template <typename V>
 __global__ void __launch_bounds__(kThreads) g_k3(const KState* st, GV<V> v, VN n, int gp2) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   if (st->done) return;
@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(kThreads) g_k3(const KState* st, GV<V> v, VN n
 template <typename V>
 __global__ void __launch_bounds__(kThreads) g_k4(const KState* st, GV<V> v, VN n,
                                                 double2* part, int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
@@ -209,7 +209,7 @@ __global__ void __launch_bounds__(kThreads) g_k4(const KState* st, GV<V> v, VN n
     acc[3] = zadd(acc[3], zmul(cAt, t));
     acc[4] = zadd(acc[4], zmul(cy, t));
   }
-  block_partials<5>(acc, part, nblk);
+  block_partials<5>(acc, part, nblk, bx_);
 }
 
 // zeta and eta from the five products, or breakdown (gpbicg.f:217-264);
@@ -265,7 +265,7 @@ __global__ void __launch_bounds__(kThreads) g_f4x(KState* st, const double2* par
 template <typename V>
 __global__ void __launch_bounds__(kThreads) g_k5(const KState* st, GV<V> v, VN n,
                                                 double2* part, int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(kThreads) g_k5(const KState* st, GV<V> v, VN n
     acc[0].x += rs.x * rs.x + rs.y * rs.y;
     acc[1] = zadd(acc[1], zmul(LD(v.r0s, i), rs));
   }
-  block_partials<2>(acc, part, nblk);
+  block_partials<2>(acc, part, nblk, bx_);
 }
 
 __global__ void __launch_bounds__(kThreads) g_f5(KState* st, const double2* part, int nblk,
@@ -336,7 +336,7 @@ template <typename V> struct QV {  // qmrbicgstab.f:23-31 work columns
 template <typename V>
 __global__ void __launch_bounds__(kThreads) q_start(QV<V> v, int x0zero, VN n,
                                                    double2* part, int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
   double2 acc[1] = {make_double2(0.0, 0.0)};
@@ -350,7 +350,7 @@ __global__ void __launch_bounds__(kThreads) q_start(QV<V> v, int x0zero, VN n,
     ST(v.p, i, z0); ST(v.v, i, z0); ST(v.d, i, z0);
     acc[0] = zadd(acc[0], zmul(r, zconj(r)));
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 __global__ void __launch_bounds__(kThreads) q_fstart(KState* st, const double2* part, int nblk) {
@@ -373,7 +373,7 @@ __global__ void __launch_bounds__(kThreads) q_fstart(KState* st, const double2* 
 template <typename V>
 __global__ void __launch_bounds__(kThreads) q_k1(const KState* st, QV<V> v, VN n,
                                                 double2* part, int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(kThreads) q_k1(const KState* st, QV<V> v, VN n
   double2 acc[1] = {make_double2(0.0, 0.0)};
   RHS_BLOCKS(n, rh_, 1);
   GRID_LOOP(i, n.len(rh_)) acc[0] = zadd(acc[0], zmul(zconj(LD(v.rt, i)), LD(v.r, i)));
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 __global__ void __launch_bounds__(kThreads) q_f1(KState* st, const double2* part, int nblk) {
@@ -400,7 +400,7 @@ __global__ void __launch_bounds__(kThreads) q_f1(KState* st, const double2* part
 // p = r + beta (p - omega v) (qmrbicgstab.f:92-100)
 template <typename V>
 __global__ void __launch_bounds__(kThreads) q_k2(const KState* st, QV<V> v, VN n) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   if (st->done) return;
@@ -414,7 +414,7 @@ __global__ void __launch_bounds__(kThreads) q_k2(const KState* st, QV<V> v, VN n
 template <typename V>
 __global__ void __launch_bounds__(kThreads) q_k3(const KState* st, QV<V> v, VN n,
                                                 double2* part, int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
@@ -422,7 +422,7 @@ __global__ void __launch_bounds__(kThreads) q_k3(const KState* st, QV<V> v, VN n
   double2 acc[1] = {make_double2(0.0, 0.0)};
   RHS_BLOCKS(n, rh_, 1);
   GRID_LOOP(i, n.len(rh_)) acc[0] = zadd(acc[0], zmul(zconj(LD(v.rt, i)), LD(v.v, i)));
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 __global__ void __launch_bounds__(kThreads) q_f3(KState* st, const double2* part, int nblk) {
@@ -447,7 +447,7 @@ __global__ void __launch_bounds__(kThreads) q_f3(KState* st, const double2* part
 template <typename V>
 __global__ void __launch_bounds__(kThreads) q_k4(const KState* st, QV<V> v, VN n,
                                                 double2* part, int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
@@ -460,7 +460,7 @@ __global__ void __launch_bounds__(kThreads) q_k4(const KState* st, QV<V> v, VN n
     const double2 s = LD(v.s, i);
     acc[0] = zadd(acc[0], zmul(zconj(s), s));
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 // thetat, c, taut, etat (qmrbicgstab.f:132-146)
@@ -487,7 +487,7 @@ __global__ void __launch_bounds__(kThreads) q_f4(KState* st, const double2* part
 // dt = p + theta^2 eta/alpha d; xt = x + etat dt (qmrbicgstab.f:151-159)
 template <typename V>
 __global__ void __launch_bounds__(kThreads) q_k5(const KState* st, QV<V> v, VN n) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   if (st->done) return;
@@ -505,7 +505,7 @@ __global__ void __launch_bounds__(kThreads) q_k5(const KState* st, QV<V> v, VN n
 template <typename V>
 __global__ void __launch_bounds__(kThreads) q_k6(const KState* st, QV<V> v, VN n,
                                                 double2* part, int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
@@ -523,7 +523,7 @@ __global__ void __launch_bounds__(kThreads) q_k6(const KState* st, QV<V> v, VN n
       acc[1] = zadd(acc[1], zmul(zconj(s), s));
     }
   }
-  block_partials<2>(acc, part, nblk);
+  block_partials<2>(acc, part, nblk, bx_);
 }
 
 __global__ void __launch_bounds__(kThreads) q_f6(KState* st, const double2* part, int nblk) {
@@ -548,7 +548,7 @@ __global__ void __launch_bounds__(kThreads) q_f6(KState* st, const double2* part
 template <typename V>
 __global__ void __launch_bounds__(kThreads) q_k7(const KState* st, QV<V> v, VN n,
                                                 double2* part, int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
@@ -561,7 +561,7 @@ __global__ void __launch_bounds__(kThreads) q_k7(const KState* st, QV<V> v, VN n
     const double2 r = LD(v.r, i);
     acc[0] = zadd(acc[0], zmul(zconj(r), r));
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 // theta, c, tau, eta (qmrbicgstab.f:206-229); the x update of this iteration
@@ -594,7 +594,7 @@ __global__ void __launch_bounds__(kThreads) q_f7(KState* st, const double2* part
 // d = s + thetat^2 etat/omega dt; x = xt + eta d (qmrbicgstab.f:231-238)
 template <typename V>
 __global__ void __launch_bounds__(kThreads) q_k8(const KState* st, QV<V> v, VN n) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   if (st->done) return;
@@ -640,7 +640,7 @@ template <typename V> struct TV {  // tfqmr.f:34-41 work columns
 // r = b - Ax; w = u = r (tfqmr.f:59-69)
 template <typename V>
 __global__ void __launch_bounds__(kThreads) t_start(TV<V> v, int x0zero, VN n) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   v = v.shifted(n.off(rh_));
   RHS_BLOCKS_NOSUM(n, rh_);
   GRID_LOOP(i, n.len(rh_)) {
@@ -657,7 +657,7 @@ __global__ void __launch_bounds__(kThreads) t_start(TV<V> v, int x0zero, VN n) {
 template <typename V>
 __global__ void __launch_bounds__(kThreads) t_start2(TV<V> v, VN n, double2* part,
                                                     int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
   double2 acc[1] = {make_double2(0.0, 0.0)};
@@ -669,7 +669,7 @@ __global__ void __launch_bounds__(kThreads) t_start2(TV<V> v, VN n, double2* par
     ST(v.rt, i, r);
     acc[0] = zadd(acc[0], zmul(zconj(r), r));
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 __global__ void __launch_bounds__(kThreads) t_fstart(KState* st, const double2* part, int nblk) {
@@ -699,7 +699,7 @@ __global__ void __launch_bounds__(kThreads) t_fstart(KState* st, const double2* 
 template <typename V>
 __global__ void __launch_bounds__(kThreads) t_k1(const KState* st, TV<V> v, VN n,
                                                 double2* part, int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
@@ -707,7 +707,7 @@ __global__ void __launch_bounds__(kThreads) t_k1(const KState* st, TV<V> v, VN n
   double2 acc[1] = {make_double2(0.0, 0.0)};
   RHS_BLOCKS(n, rh_, 1);
   GRID_LOOP(i, n.len(rh_)) acc[0] = zadd(acc[0], zmul(zconj(LD(v.rt, i)), LD(v.v, i)));
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 __global__ void __launch_bounds__(kThreads) t_f1(KState* st, const double2* part, int nblk) {
@@ -724,7 +724,7 @@ __global__ void __launch_bounds__(kThreads) t_f1(KState* st, const double2* part
 template <typename V>
 __global__ void __launch_bounds__(kThreads) t_k2(const KState* st, TV<V> v, int even, VN n,
                                                 double2* part, int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
@@ -741,7 +741,7 @@ __global__ void __launch_bounds__(kThreads) t_k2(const KState* st, TV<V> v, int 
     const double2 w = LD(v.w, i);
     acc[0] = zadd(acc[0], zmul(zconj(w), w));
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 // theta, c, tau, eta (tfqmr.f:139-144)
@@ -762,7 +762,7 @@ __global__ void __launch_bounds__(kThreads) t_f2(KState* st, const double2* part
 // x = x + eta d (tfqmr.f:147-153)
 template <typename V>
 __global__ void __launch_bounds__(kThreads) t_k3(const KState* st, TV<V> v, VN n) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   if (st->done) return;
@@ -789,7 +789,7 @@ __global__ void t_f3(KState* st, double2* trace) {
 template <typename V>
 __global__ void __launch_bounds__(kThreads) t_k4(const KState* st, TV<V> v, VN n,
                                                 double2* part, int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
@@ -797,7 +797,7 @@ __global__ void __launch_bounds__(kThreads) t_k4(const KState* st, TV<V> v, VN n
   double2 acc[1] = {make_double2(0.0, 0.0)};
   RHS_BLOCKS(n, rh_, 1);
   GRID_LOOP(i, n.len(rh_)) acc[0] = zadd(acc[0], zmul(zconj(LD(v.rt, i)), LD(v.w, i)));
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 __global__ void __launch_bounds__(kThreads) t_f4(KState* st, const double2* part, int nblk) {
@@ -816,7 +816,7 @@ __global__ void __launch_bounds__(kThreads) t_f4(KState* st, const double2* part
 // (odd) u1 = w + beta u; Auo = Au (tfqmr.f:176-183); then u = u1 (:187-194)
 template <typename V>
 __global__ void __launch_bounds__(kThreads) t_k5(const KState* st, TV<V> v, int odd, VN n) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   if (st->done) return;
@@ -835,7 +835,7 @@ __global__ void __launch_bounds__(kThreads) t_k5(const KState* st, TV<V> v, int 
 // (odd) v = Au + beta (Auo + beta v) (tfqmr.f:206-214)
 template <typename V>
 __global__ void __launch_bounds__(kThreads) t_k6(const KState* st, TV<V> v, VN n) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   if (st->done) return;
@@ -875,7 +875,7 @@ template <typename V> struct CV {
 template <typename V>
 __global__ void __launch_bounds__(kThreads) c_start(CV<V> v, int x0zero, VN n, double2* part,
                                                    int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
   double2 acc[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
@@ -890,7 +890,7 @@ __global__ void __launch_bounds__(kThreads) c_start(CV<V> v, int x0zero, VN n, d
     acc[0].x += b.x * b.x + b.y * b.y;
     acc[1] = zadd(acc[1], zmul(rs, rs));
   }
-  block_partials<2>(acc, part, nblk);
+  block_partials<2>(acc, part, nblk, bx_);
 }
 
 __global__ void __launch_bounds__(kThreads) c_fstart(KState* st, const double2* part, int nblk) {
@@ -909,7 +909,7 @@ __global__ void __launch_bounds__(kThreads) c_fstart(KState* st, const double2* 
 template <typename V>
 __global__ void __launch_bounds__(kThreads) c_k1(const KState* st, CV<V> v, VN n, double2* part,
                                                 int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
@@ -917,7 +917,7 @@ __global__ void __launch_bounds__(kThreads) c_k1(const KState* st, CV<V> v, VN n
   double2 acc[1] = {make_double2(0.0, 0.0)};
   RHS_BLOCKS(n, rh_, 1);
   GRID_LOOP(i, n.len(rh_)) acc[0] = zadd(acc[0], zmul(LD(v.p, i), LD(v.w, i)));
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 // xi = p.w (zcg.f:85-88; the reference stops the program on xi = 0), then
@@ -953,7 +953,7 @@ __global__ void __launch_bounds__(kThreads) c_f1(KState* st, const double2* part
 template <typename V>
 __global__ void __launch_bounds__(kThreads) c_k2(const KState* st, CV<V> v, VN n, double2* part,
                                                 int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
@@ -976,7 +976,7 @@ __global__ void __launch_bounds__(kThreads) c_k2(const KState* st, CV<V> v, VN n
     const double2 rs = Vt<V>::rnd(r);
     acc[0].x += rs.x * rs.x + rs.y * rs.y;
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 // tole = |r| / |b|; converged when tole <= tol (zcg.f:121-138)
@@ -1001,7 +1001,7 @@ __global__ void __launch_bounds__(kThreads) c_f2(KState* st, const double2* part
 template <typename V>
 __global__ void __launch_bounds__(kThreads) c_k3(const KState* st, CV<V> v, VN n, double2* part,
                                                 int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
@@ -1013,7 +1013,7 @@ __global__ void __launch_bounds__(kThreads) c_k3(const KState* st, CV<V> v, VN n
     acc[0] = zadd(acc[0], zmul(r, r));
     acc[1] = zadd(acc[1], zmul(r, LD(v.s, i)));
   }
-  block_partials<2>(acc, part, nblk);
+  block_partials<2>(acc, part, nblk, bx_);
 }
 
 // beta = rho / rho0, xi = delta - beta^2 xi (zcg.f:158-169), then the loop
@@ -1063,7 +1063,7 @@ template <typename V> struct BV {
 template <typename V>
 __global__ void __launch_bounds__(kThreads) s_start(BV<V> v, int x0zero, VN n, double2* part,
                                                    int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
   double2 acc[1] = {make_double2(0.0, 0.0)};
@@ -1079,7 +1079,7 @@ __global__ void __launch_bounds__(kThreads) s_start(BV<V> v, int x0zero, VN n, d
     const double2 rs = Vt<V>::rnd(r);
     acc[0].x += rs.x * rs.x + rs.y * rs.y;
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 // The head of iteration it + 1 (pimzbicgstab.f:69-89): rho0 = rho, rho =
@@ -1117,7 +1117,7 @@ __global__ void __launch_bounds__(kThreads) s_fstart(KState* st, const double2* 
 // p = r + beta (p - omega v) (pimzbicgstab.f:91-100)
 template <typename V>
 __global__ void __launch_bounds__(kThreads) s_k1(const KState* st, BV<V> v, VN n) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   if (st->done) return;
@@ -1133,7 +1133,7 @@ __global__ void __launch_bounds__(kThreads) s_k1(const KState* st, BV<V> v, VN n
 template <typename V>
 __global__ void __launch_bounds__(kThreads) s_k2(const KState* st, BV<V> v, VN n, double2* part,
                                                 int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
@@ -1141,7 +1141,7 @@ __global__ void __launch_bounds__(kThreads) s_k2(const KState* st, BV<V> v, VN n
   double2 acc[1] = {make_double2(0.0, 0.0)};
   RHS_BLOCKS(n, rh_, 1);
   GRID_LOOP(i, n.len(rh_)) acc[0] = zadd(acc[0], zmul(zconj(LD(v.rt, i)), LD(v.v, i)));
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 // alpha = rho / xi (pimzbicgstab.f:111-117)
@@ -1167,7 +1167,7 @@ __global__ void __launch_bounds__(kThreads) s_f2(KState* st, const double2* part
 template <typename V>
 __global__ void __launch_bounds__(kThreads) s_k3(const KState* st, BV<V> v, VN n, double2* part,
                                                 int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
@@ -1181,7 +1181,7 @@ __global__ void __launch_bounds__(kThreads) s_k3(const KState* st, BV<V> v, VN n
     const double2 ss = Vt<V>::rnd(sv);
     acc[0] = zadd(acc[0], zmul(zconj(ss), ss));
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 // |s| < 1e-15: soft breakdown (pimzbicgstab.f:125-134)
@@ -1202,7 +1202,7 @@ __global__ void __launch_bounds__(kThreads) s_f3(KState* st, const double2* part
 template <typename V>
 __global__ void __launch_bounds__(kThreads) s_k4(const KState* st, BV<V> v, VN n, double2* part,
                                                 int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
@@ -1214,7 +1214,7 @@ __global__ void __launch_bounds__(kThreads) s_k4(const KState* st, BV<V> v, VN n
     acc[0] = zadd(acc[0], zmul(ct, LD(v.t, i)));
     acc[1] = zadd(acc[1], zmul(ct, LD(v.s, i)));
   }
-  block_partials<2>(acc, part, nblk);
+  block_partials<2>(acc, part, nblk, bx_);
 }
 
 // omega = conj(t).s / conj(t).t (pimzbicgstab.f:150-156)
@@ -1240,7 +1240,7 @@ __global__ void __launch_bounds__(kThreads) s_f4(KState* st, const double2* part
 template <typename V>
 __global__ void __launch_bounds__(kThreads) s_k5(const KState* st, BV<V> v, VN n, double2* part,
                                                 int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
@@ -1256,7 +1256,7 @@ __global__ void __launch_bounds__(kThreads) s_k5(const KState* st, BV<V> v, VN n
     const double2 rs = Vt<V>::rnd(r);
     acc[0].x += rs.x * rs.x + rs.y * rs.y;
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 // |r|^2 / |r0|^2 against tol / 10, the iteration cap (pimzbicgstab.f:167-181),
@@ -1291,7 +1291,7 @@ __global__ void __launch_bounds__(kThreads) s_f5(KState* st, const double2* part
 // (pimzbicgstab.f:126-133)
 template <typename V>
 __global__ void __launch_bounds__(kThreads) s_soft(const KState* st, BV<V> v, VN n) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   st += rh_;
   v = v.shifted(n.off(rh_));
   if (!(st->status == -1 && st->steperr == 12)) return;
@@ -1302,7 +1302,7 @@ __global__ void __launch_bounds__(kThreads) s_soft(const KState* st, BV<V> v, VN
 template <typename V>
 __global__ void __launch_bounds__(kThreads) k_resid(const V* Ax, const V* b, VN n,
                                                    double2* part, int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   part += (long long)rh_ * kMaxDots * nblk;
   Ax += n.off(rh_);
   b += n.off(rh_);
@@ -1312,7 +1312,7 @@ __global__ void __launch_bounds__(kThreads) k_resid(const V* Ax, const V* b, VN 
     const double2 e = zsub(LD(Ax, i), LD(b, i));
     acc[0].x += e.x * e.x + e.y * e.y;
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 __global__ void __launch_bounds__(kThreads) f_resid(KState* st, const double2* part, int nblk) {
@@ -1329,7 +1329,7 @@ __global__ void __launch_bounds__(kThreads) k_alpha_apply(const double2* x, doub
   const int rh_ = blockIdx.y;  // right-hand side
   x += rh_ * 3 * nd;
   out += rh_ * 3 * nd;
-  const int nbr_ = gridDim.x;
+  const int nbr_ = gridDim.x, bx_ = blockIdx.x;
   GRID_LOOP(i, nd) {
     double2 e0 = x[3 * i], e1 = x[3 * i + 1], e2 = x[3 * i + 2];
     if (mode == kAlphaScalar) {
@@ -1364,7 +1364,7 @@ __global__ void __launch_bounds__(kThreads) k_alpha_t(const V* x, V* y, const V*
   if (skip && skip[(long long)rh_ * skip_stride]) return;
   x += rh_ * 3 * nd;
   y += rh_ * 3 * nd;
-  const int nbr_ = gridDim.x;
+  const int nbr_ = gridDim.x, bx_ = blockIdx.x;
   GRID_LOOP(i, nd) {
     double2 e[3], f[3];
     for (int a = 0; a < 3; ++a) e[a] = Vt<V>::ld(y, 3 * i + a);
@@ -1385,7 +1385,7 @@ __global__ void __launch_bounds__(kThreads) k_alpha_t(const V* x, V* y, const V*
 __global__ void __launch_bounds__(kThreads) k_refine_resid(const double2* b, const double2* Ax,
                                                           int have_ax, double2* r64, float2* r32,
                                                           VN n, double2* part, int nblk) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   part += (long long)rh_ * kMaxDots * nblk;
   b += n.off(rh_);
   Ax += n.off(rh_);
@@ -1400,13 +1400,13 @@ __global__ void __launch_bounds__(kThreads) k_refine_resid(const double2* b, con
     r32[i] = make_float2((float)r.x, (float)r.y);
     acc[0].x += r.x * r.x + r.y * r.y;
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 // x += e (FP32 correction into the FP64 iterate)
 __global__ void __launch_bounds__(kThreads) k_refine_add(double2* x, const float2* e, VN n,
                                                         const int* skip) {
-  const int rh_ = blockIdx.y;  // right-hand side
+  const int rh_ = n.rhs();  // right-hand side
   if (skip[rh_]) return;
   x += n.off(rh_);
   e += n.off(rh_);
@@ -1421,6 +1421,26 @@ __global__ void __launch_bounds__(kThreads) k_refine_add(double2* x, const float
 #undef ST
 
 // ------------------------------------------------------------- host driver
+// p[0 .. n) = 0
+__global__ void __launch_bounds__(kThreads) k_zero2(double2* p, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = make_double2(0.0, 0.0);
+}
+
+// Block offsets of the flat vector grid of a ragged layout: RHS r gets the
+// blocks its own solve launches, none once it has finished (unless all).
+__global__ void k_ragged_blocks(const long long* __restrict__ lens, const KState* __restrict__ st,
+                                int R, int all, int* __restrict__ kb) {
+  if (threadIdx.x != 0) return;
+  int acc = 0;
+  kb[0] = 0;
+  for (int r = 0; r < R; ++r) {
+    if (all || !st[r].done) acc += blocks_of(lens[r]);
+    kb[r + 1] = acc;
+  }
+}
+
 void solver_delete(Solver* S) {
   if (!S) return;
   for (int i = 0; i < 14; ++i) {
@@ -1436,6 +1456,7 @@ void solver_delete(Solver* S) {
   (void)hipFree(S->lens);
   (void)hipFree(S->offs);
   (void)hipFree(S->part5);
+  (void)hipFree(S->kb);
   delete S;
 }
 
@@ -1490,13 +1511,17 @@ Solver* solver_new(int R, bool need32, const long long* lens, const long long* o
             hip_ok(hipMemcpy(S->lens, lens, sizeof(long long) * R, hipMemcpyHostToDevice),
                    "copy lengths") &&
             hip_ok(hipMemcpy(S->offs, offs, sizeof(long long) * R, hipMemcpyHostToDevice),
-                   "copy offsets");
+                   "copy offsets") &&
+            hip_ok(hipMalloc(&S->kb, sizeof(int) * (cap_R + 1)), "hipMalloc block offsets");
   if (!ok) {
     solver_delete(S);
     return nullptr;
   }
   S->R = R;  // this layout; the buffers hold cap_R right-hand sides
   S->total = total;
+  S->lens_h.assign(lens, lens + R);
+  S->kb_full = 0;
+  for (int r = 0; r < R; ++r) S->kb_full += blocks_of(lens[r]);
   return S;
 }
 
@@ -1513,12 +1538,15 @@ bool solver_refit(Solver* S, int R, bool need32, const long long* lens, const lo
     total = std::max(total, offs[r] + lens[r]);
   }
   if (!S || total > S->cap_total || R > S->cap_R || (need32 && !S->v32[0]) ||
-      blocks_for(n) > S->part_blocks || !S->lens)
+      blocks_for(n) > S->part_blocks || !S->lens || !S->kb)
     return false;
   S->n = n;
   S->R = R;
   S->total = total;
   S->nblk = blocks_for(n);
+  S->lens_h.assign(lens, lens + R);
+  S->kb_full = 0;
+  for (int r = 0; r < R; ++r) S->kb_full += blocks_of(lens[r]);
   return hip_ok(hipMemcpyAsync(S->lens, lens, sizeof(long long) * R, hipMemcpyHostToDevice, s),
                 "copy lengths") &&
          hip_ok(hipMemcpyAsync(S->offs, offs, sizeof(long long) * R, hipMemcpyHostToDevice, s),
@@ -1675,6 +1703,26 @@ bool alpha_apply_one(const double2* x, double2* FF, const double2* al, int mode,
 }
 
 
+// Ragged layouts: point the flat vector grid (S->kb, grid G) and the
+// operator's launches at the RHS still running (per S->st_host / S->st done
+// flags), or at every RHS (all).  The block tables are rewritten on the
+// device, in stream order.
+static bool ragged_grid(const MvOp& op, Solver* S, int R, bool all, hipStream_t s, dim3& G) {
+  std::vector<int> hd(R, 0);
+  int total = 0;
+  for (int r = 0; r < R; ++r) {
+    hd[r] = all ? 0 : S->st_host[r].done;
+    if (!hd[r]) total += blocks_of(S->lens_h[r]);
+  }
+  if (total == 0) return true;  // nothing runs: keep the tables as they are
+  hipLaunchKernelGGL(k_ragged_blocks, dim3(1), dim3(64), 0, s, (const long long*)S->lens,
+                     (const KState*)S->st, R, all ? 1 : 0, S->kb);
+  if (!hip_ok(hipGetLastError(), "ragged grid") || !op.compact(S->st, hd.data(), R, all, s))
+    return false;
+  G = dim3(total);
+  return true;
+}
+
 static bool poll(Solver* S, int R, hipStream_t s) {
   return hip_ok(hipMemcpyAsync(S->st_host, S->st, sizeof(KState) * R, hipMemcpyDeviceToHost, s),
                 "copy state") &&
@@ -1696,9 +1744,11 @@ static bool krylov(const MvOp& op, Solver* S, V** vv, int R, int method, const d
                    const char* active, int maxit, int x0zero, int check_every,
                    volatile int* nstop, hipStream_t s, KRun* out, bool* cancelled) {
   const long long n = S->n;
-  const VN vn{n, S->lens, S->offs};
+  const bool ragged = S->lens != nullptr;
+  const VN vn{n, S->lens, S->offs, ragged ? S->kb : nullptr, R};
   const int nb = S->nblk;
-  const dim3 G(nb, R), T(kThreads), F(1, R), One(1);
+  dim3 G = ragged ? dim3(S->kb_full) : dim3(nb, R);
+  const dim3 T(kThreads), F(1, R), One(1);
   double2* trace = R == 1 ? S->trace : nullptr;
   for (int r = 0; r < R; ++r) {
     KState& k = S->st_host[r];
@@ -1716,6 +1766,10 @@ static bool krylov(const MvOp& op, Solver* S, V** vv, int R, int method, const d
   if (!hip_ok(hipMemcpyAsync(S->st, S->st_host, sizeof(KState) * R, hipMemcpyHostToDevice, s),
               "state init"))
     return false;
+  // RHS that start finished (refinement: converged) launch nothing
+  int nlive = 0;
+  for (int r = 0; r < R; ++r) nlive += S->st_host[r].done ? 0 : 1;
+  if (ragged && nlive < R && !ragged_grid(op, S, R, false, s, G)) return false;
   const int* done = &S->st->done;
   const int dstride = (int)(sizeof(KState) / sizeof(int));
   int nmv0 = 0;
@@ -1740,6 +1794,14 @@ static bool krylov(const MvOp& op, Solver* S, V** vv, int R, int method, const d
       if (nstop && *nstop == -1) {
         *cancelled = true;
         return 1;
+      }
+      if (ragged) {  // RHS that finished since the last poll leave the launches
+        int live = 0;
+        for (int r = 0; r < R; ++r) live += S->st_host[r].done ? 0 : 1;
+        if (live < nlive) {
+          nlive = live;
+          if (!ragged_grid(op, S, R, false, s, G)) return -1;
+        }
       }
     }
     return 0;
@@ -1958,13 +2020,14 @@ static bool krylov(const MvOp& op, Solver* S, V** vv, int R, int method, const d
       o.nmv = nmv0 + nmv;
     }
   } else if (krylov_gp_method(method)) {
-    KLoop L{op, S, vn, nb, R, n, s, trace, done, dstride, nmv0, x0zero, active, step_poll};
+    KLoop L{op, S, vn, &G, nb, R, n, s, trace, done, dstride, nmv0, x0zero, active, step_poll};
     if (!krylov_gp<V>(L, vv, method, out)) return false;
   } else {
     set_error("unknown method");
     return false;
   }
-  return true;
+  // every RHS back in the launches (the caller's kernels cover them all)
+  return !ragged || nlive == R || ragged_grid(op, S, R, true, s, G);
 }
 
 // Norms of R residual vectors whose partial sums k_resid / k_init /
@@ -2030,9 +2093,19 @@ int solve_core(const MvOp& op, Solver* S, const ifdda_solve_opts* o, int R, cons
                hipStream_t s, std::chrono::steady_clock::time_point t0) {
   const int maxit = o->maxit > 0 ? o->maxit : 10000;
   const long long n = S->n;
-  const VN vn{n, S->lens, S->offs};
+  const bool ragged = S->lens != nullptr;
+  const VN vn{n, S->lens, S->offs, ragged ? S->kb : nullptr, R};
   const int nb = S->nblk;
-  const dim3 G(nb, R), T(kThreads);
+  const dim3 G = ragged ? dim3(S->kb_full) : dim3(nb, R), T(kThreads);
+  if (ragged) {
+    // the flat grid with every RHS in; partial slots past an RHS's block
+    // count stay zero (no block writes them)
+    hipLaunchKernelGGL(k_zero2, dim3(kMaxBlocks), T, 0, s, S->part,
+                       (long long)kMaxDots * S->part_blocks * S->cap_R);
+    hipLaunchKernelGGL(k_ragged_blocks, dim3(1), dim3(64), 0, s, (const long long*)S->lens,
+                       (const KState*)S->st, R, 1, S->kb);
+    if (!hip_ok(hipGetLastError(), "ragged layout")) return IFDDA_ERR_HIP;
+  }
   double2* x = S->v64[0];
   double2* b = S->v64[1];
   double2* Ax = S->v64[13];

@@ -34,10 +34,10 @@ __device__ __forceinline__ void kbreak(KState* st, int steperr) {
   st->done = 1;
 }
 
-// Vector kernels: RHS blockIdx.y, its state, its slice of the vectors and of
+// Vector kernels: the block's RHS (VN::rhs), its state, its slice of the vectors and of
 // the partial sums; leave when the solve of this RHS has ended.
 #define VK_ENTER()                                   \
-  const int rh_ = blockIdx.y;                        \
+  const int rh_ = n.rhs();                           \
   st += rh_;                                         \
   v = v.shifted(n.off(rh_));                         \
   part += (long long)rh_ * kMaxDots * nblk;          \
@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(kThreads) sf_start(const KState* st, SfV<V> v,
     ST(v.p, i, z0); ST(v.u, i, z0); ST(v.z, i, z0); ST(v.y, i, z0); ST(v.Ap, i, z0); ST(v.Au, i, z0);
     acc[0] = zadd(acc[0], zmul(r, zconj(r)));
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 // R0RN, beta = 0, ITNO = 0 (gpbicgsafe.f:107-127)
@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(kThreads) sf_k2(KState* st, SfV<V> v, VN n, do
     acc[4] = zadd(acc[4], zcdot(Ar, y));
     acc[5] = zadd(acc[5], zcdot(Ar, Ar));
   }
-  block_partials<6>(acc, part, nblk);
+  block_partials<6>(acc, part, nblk, bx_);
 }
 
 // alpha = R0RN / (r0,Ap), then zeta and eta (gpbicgsafe.f:139-170);
@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(kThreads) sf_k4(KState* st, SfV<V> v, VN n, do
     acc[0].x += rn.x * rn.x + rn.y * rn.y;
     acc[1] = zadd(acc[1], zcdot(LD(v.r0, i), rn));
   }
-  block_partials<2>(acc, part, nblk);
+  block_partials<2>(acc, part, nblk, bx_);
 }
 
 // residual test, beta = alpha (r0,r) / zeta / R0RN (gpbicgsafe.f:192-219)
@@ -256,7 +256,7 @@ __global__ void __launch_bounds__(kThreads) pl_start(const KState* st, PlV<V> v,
     ST(v.p, i, z0); ST(v.Ap, i, z0); ST(v.Au, i, z0); ST(v.u, i, z0); ST(v.z, i, z0); ST(v.y, i, z0);
     acc[0] = zadd(acc[0], zmul(r, zconj(r)));
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 // After Ar = A r: p = r + beta (p - u), Ap = Ar + beta (Ap - Au); partial
@@ -280,7 +280,7 @@ __global__ void __launch_bounds__(kThreads) pl_k2(KState* st, PlV<V> v, VN n, do
     acc[4] = zadd(acc[4], zcdot(y, r));
     acc[5] = zadd(acc[5], zcdot(y, Ar));
   }
-  block_partials<6>(acc, part, nblk);
+  block_partials<6>(acc, part, nblk, bx_);
 }
 
 // gpbicgplus.f:118-163: c1 = (y,y), c2 = (Ar,Ar), c3 = (Ar,r), c4 = (y,r),
@@ -345,7 +345,7 @@ __global__ void __launch_bounds__(kThreads) pl_k4(KState* st, PlV<V> v, VN n, do
     acc[0].x += rn.x * rn.x + rn.y * rn.y;
     acc[1] = zadd(acc[1], zcdot(LD(v.r0, i), rn));
   }
-  block_partials<2>(acc, part, nblk);
+  block_partials<2>(acc, part, nblk, bx_);
 }
 
 // ================================================================ GPBICGAR
@@ -375,7 +375,7 @@ __global__ void __launch_bounds__(kThreads) ar_start(const KState* st, ArV<V> v,
     ST(v.Au, i, z0);
     acc[0] = zadd(acc[0], zmul(r, zconj(r)));
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 // p = r + beta (p - u), Ap = Ar + beta (Ap - Au); partial (r0,Ap), (y,y),
@@ -400,7 +400,7 @@ __global__ void __launch_bounds__(kThreads) ar_k1(KState* st, ArV<V> v, VN n, do
     acc[4] = zadd(acc[4], zcdot(Ar, y));
     acc[5] = zadd(acc[5], zcdot(Ar, Ar));
   }
-  block_partials<6>(acc, part, nblk);
+  block_partials<6>(acc, part, nblk, bx_);
 }
 
 // u = zeta Ap + eta (t - r + beta u) (gpbicgar.f:156-160)
@@ -437,7 +437,7 @@ __global__ void __launch_bounds__(kThreads) ar_k3(KState* st, ArV<V> v, VN n, do
     acc[0].x += rn.x * rn.x + rn.y * rn.y;
     acc[1] = zadd(acc[1], zcdot(LD(v.r0, i), rn));
   }
-  block_partials<2>(acc, part, nblk);
+  block_partials<2>(acc, part, nblk, bx_);
 }
 
 // residual test, then (after the reference's A r) beta (gpbicgar.f:178-213)
@@ -474,7 +474,7 @@ __global__ void __launch_bounds__(kThreads) bp_start(const KState* st, BpV<V> v,
     ST(v.e8, i, z0); ST(v.u, i, z0); ST(v.g9, i, z0); ST(v.s5, i, z0); ST(v.h12, i, z0);
     acc[0] = zadd(acc[0], zmul(rs, r));
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 // After Ar = A r: c1 = r0*.r, c2 = r0*.Ar, c3 = (s,s), c4 = (Ar,r),
@@ -497,7 +497,7 @@ __global__ void __launch_bounds__(kThreads) bp_k1(KState* st, BpV<V> v, VN n, do
     acc[6] = zadd(acc[6], zcdot(Ar, Ar));
     acc[7] = zadd(acc[7], zmul(r0s, LD(v.h12, i)));
   }
-  block_partials<8>(acc, part, nblk);
+  block_partials<8>(acc, part, nblk, bx_);
 }
 
 // alpha, beta, zeta, eta (bicgstarplus.f:86-160)
@@ -589,7 +589,7 @@ __global__ void __launch_bounds__(kThreads) bp_k3(KState* st, BpV<V> v, VN n, do
     ST(v.r, i, rn);
     acc[0].x += rn.x * rn.x + rn.y * rn.y;
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 // residual test and the iteration cap (bicgstarplus.f:184-198)
@@ -645,7 +645,7 @@ __global__ void __launch_bounds__(kThreads) co_start(const KState* st, CoV<V> v,
     ST(v.u, i, z0); ST(v.ut, i, z0); ST(v.z, i, z0);
     acc[0].x += r.x * r.x + r.y * r.y;
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 __global__ void __launch_bounds__(kThreads) co_fstart(KState* st, const double2* part, int nblk) {
@@ -661,7 +661,7 @@ __global__ void __launch_bounds__(kThreads) co_k1(KState* st, CoV<V> v, VN n, do
   double2 acc[1] = {make_double2(0.0, 0.0)};
   RHS_BLOCKS(n, rh_, 1);
   GRID_LOOP(i, n.len(rh_)) acc[0] = zadd(acc[0], zcdot(LD(v.r0, i), LD(v.Ar, i)));
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 // rho, beta = rho / rhoa alpha / zeta (gpbicor.f:172-198)
@@ -707,7 +707,7 @@ __global__ void __launch_bounds__(kThreads) co_k3(KState* st, CoV<V> v, VN n, do
   double2 acc[1] = {make_double2(0.0, 0.0)};
   RHS_BLOCKS(n, rh_, 1);
   GRID_LOOP(i, n.len(rh_)) acc[0] = zadd(acc[0], zcdot(LD(v.r0, i), LD(v.At, i)));
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 __global__ void __launch_bounds__(kThreads) co_f3(KState* st, const double2* part, int nblk) {
@@ -747,7 +747,7 @@ __global__ void __launch_bounds__(kThreads) co_k4(KState* st, CoV<V> v, VN n, do
     acc[3] = zadd(acc[3], zcdot(At, t));
     acc[4] = zadd(acc[4], zcdot(y, y));
   }
-  block_partials<5>(acc, part, nblk);
+  block_partials<5>(acc, part, nblk, bx_);
 }
 
 // zeta, eta (gpbicor.f:250-299)
@@ -798,7 +798,7 @@ __global__ void __launch_bounds__(kThreads) co_k5(KState* st, CoV<V> v, VN n, do
     ST(v.x, i, zadd(zadd(LD(v.x, i), zmul(al, LD(v.p, i))), z));
     ST(v.tp, i, t);
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 // residual test, rhoa = rho, the iteration cap (gpbicor.f:333-354)
@@ -844,7 +844,7 @@ __global__ void __launch_bounds__(kThreads) cs_start(const KState* st, CsV<V> v,
     ST(v.r, i, r);
     acc[0].x += r.x * r.x + r.y * r.y;
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 __global__ void __launch_bounds__(kThreads) cs_fstart(KState* st, const double2* part, int nblk) {
@@ -871,7 +871,7 @@ __global__ void __launch_bounds__(kThreads) cs_k1(KState* st, CsV<V> v, VN n, do
     }
     acc[0] = zadd(acc[0], zmul(rs, Ar));
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 __global__ void __launch_bounds__(kThreads) cs_f1(KState* st, const double2* part, int nblk) {
@@ -914,7 +914,7 @@ __global__ void __launch_bounds__(kThreads) cs_k3(KState* st, CsV<V> v, VN n, do
   double2 acc[1] = {make_double2(0.0, 0.0)};
   RHS_BLOCKS(n, rh_, 1);
   GRID_LOOP(i, n.len(rh_)) acc[0] = zadd(acc[0], zmul(LD(v.r0s, i), LD(v.Ad, i)));
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 __global__ void __launch_bounds__(kThreads) cs_f3(KState* st, const double2* part, int nblk) {
@@ -942,7 +942,7 @@ __global__ void __launch_bounds__(kThreads) cs_k4(KState* st, CsV<V> v, VN n, do
     ST(v.r, i, rn);
     acc[0].x += rn.x * rn.x + rn.y * rn.y;
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 // residual test, rhoa = rho, the iteration cap (cors.f:167-181)
@@ -998,7 +998,7 @@ __global__ void __launch_bounds__(kThreads) qc_start(const KState* st, QcV<V> v,
     acc[1] = zadd(acc[1], zmul(r, zconj(r)));
     acc[2] = zadd(acc[2], zmul(r, r));
   }
-  block_partials<3>(acc, part, nblk);
+  block_partials<3>(acc, part, nblk, bx_);
 }
 
 // gamma = sqrt(v.conj(v)), ksi = sqrt(w.conj(w)), rho = v.w; the initial
@@ -1021,7 +1021,7 @@ __global__ void __launch_bounds__(kThreads) qc_kd4(KState* st, QcV<V> v, VN n, d
   double2 acc[1] = {make_double2(0.0, 0.0)};
   RHS_BLOCKS(n, rh_, 1);
   GRID_LOOP(i, n.len(rh_)) acc[0] = zadd(acc[0], zmul(LD(v.v, i), LD(v.Atw, i)));
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 // The head of an iteration (qmrpim.f:92-104): ITNO + 1, gamma and ksi
@@ -1085,7 +1085,7 @@ __global__ void __launch_bounds__(kThreads) qc_k2(KState* st, QcV<V> v, VN n, do
     acc[1] = zadd(acc[1], zmul(ww, zconj(ww)));
     acc[2] = zadd(acc[2], zmul(vv, ww));
   }
-  block_partials<3>(acc, part, nblk);
+  block_partials<3>(acc, part, nblk, bx_);
 }
 
 // gamma, ksi, rho of the new v and w, keeping the previous ones
@@ -1154,7 +1154,7 @@ __global__ void __launch_bounds__(kThreads) qc_k4(KState* st, QcV<V> v, VN n, do
     ST(v.r, i, rn);
     acc[0] = zadd(acc[0], zmul(rn, zconj(rn)));
   }
-  block_partials<1>(acc, part, nblk);
+  block_partials<1>(acc, part, nblk, bx_);
 }
 
 // TOLE = sqrt|r.conj(r)| / |b| against tol; past MAXIT a failure
@@ -1198,7 +1198,8 @@ bool krylov_gp(KLoop& L, V** vv, int method, KRun* out) {
   const int nb = L.nb, R = L.R;
   const long long n = L.n;
   hipStream_t s = L.s;
-  const dim3 G(nb, R), T(kThreads), F(1, R);
+  const dim3& G = *L.grid;  // shrinks as right-hand sides of a ragged layout finish
+  const dim3 T(kThreads), F(1, R);
   KState* st = S->st;
   double2* part = S->part;
   const double2* cpart = part;

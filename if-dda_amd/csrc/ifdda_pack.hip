@@ -18,12 +18,15 @@
 // a stream of packs allocates and frees nothing in steady state; every copy
 // is ordered on the workspace's own non-blocking stream.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <type_traits>
 #include <vector>
@@ -71,14 +74,21 @@ struct PackWs {
   size_t green_bytes = 0;
   size_t green_limit = (size_t)16 << 30;
   unsigned long long tick = 0;
-  double2* full = nullptr;  // Green build scratch (L1 L2 L3 FP64)
-  size_t full_cap = 0;
   char* arena = nullptr;  // per-pack data, grow-only
   size_t arena_cap = 0;
   char* blob = nullptr;  // packed-launch argument blocks, grow-only
   size_t blob_cap = 0;
+  char* hstage = nullptr;  // pinned host image of a pack's inputs / outputs, grow-only
+  size_t hstage_cap = 0;
   Solver* S = nullptr;
   std::vector<long long> lens, offs;  // host layout of the last pack (read by s)
+  // Green tensors of a pack's new (box, d, k0) keys are built side by side on
+  // these streams, each with its own full-grid scratch; s waits for them.
+  static constexpr int kAux = 4;
+  hipStream_t aux[kAux] = {};
+  hipEvent_t ev_aux[kAux] = {}, ev_main = nullptr;
+  char* afull[kAux] = {};
+  size_t afull_cap[kAux] = {};
 };
 
 namespace {
@@ -88,6 +98,24 @@ bool debug_on() {
   static const bool on = std::getenv("IFDDA_DEBUG") != nullptr;
   return on;
 }
+
+// IFDDA_DEBUG: the seconds each phase of a pack took (the stream is drained
+// at each mark, so the marks cost time of their own).
+struct PhaseClock {
+  hipStream_t s;
+  std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+  std::string line;
+  void mark(const char* what) {
+    if (!debug_on()) return;
+    (void)hipStreamSynchronize(s);
+    const auto n = std::chrono::steady_clock::now();
+    char b[64];
+    std::snprintf(b, sizeof(b), " %s %.1f ms", what,
+                  std::chrono::duration<double, std::milli>(n - t).count());
+    line += b;
+    t = n;
+  }
+};
 
 // Grow-only device buffer: at least doubles, so a worker meets O(log) growths
 // (each frees the old buffer, which waits for the device).
@@ -101,6 +129,39 @@ bool grow(char** p, size_t* cap, size_t need, const char* what) {
   if (!hip_ok(hipMalloc(p, c), what)) return false;
   *cap = c;
   return true;
+}
+
+// Grow-only pinned host buffer (the old one is no longer read: callers
+// drain the stream first).
+bool grow_host(char** p, size_t* cap, size_t need) {
+  if (need <= *cap) return true;
+  const size_t c = std::max(need + need / 4, 2 * *cap);
+  if (debug_on()) std::fprintf(stderr, "ifdda pack: pinned staging %zu -> %zu MB\n", *cap >> 20, c >> 20);
+  (void)hipHostFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  if (!hip_ok(hipHostMalloc(reinterpret_cast<void**>(p), c, hipHostMallocDefault),
+              "hipHostMalloc pack staging"))
+    return false;
+  *cap = c;
+  return true;
+}
+
+// f(0) .. f(n - 1) on up to 8 host threads when the work moves at least
+// 32 MB, else in order on this thread.
+template <typename F> void par_for(int n, size_t bytes, F&& f) {
+  const int nt = bytes >= ((size_t)32 << 20) ? std::min(8, n) : 1;
+  if (nt <= 1) {
+    for (int i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<int> next{0};
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([&] {
+      for (int i; (i = next.fetch_add(1)) < n;) f(i);
+    });
+  for (auto& t : th) t.join();
 }
 
 const PackTable* table_for(PackWs& w, const AxisPlan& ax) {
@@ -131,9 +192,9 @@ const PackTable* table_for(PackWs& w, const AxisPlan& ax) {
   return &w.tables.emplace(L, t).first->second;
 }
 
+// G32 lives in the same allocation as G64, after it.
 void green_free(GreenEntry& e) {
   (void)hipFree(e.G64);
-  (void)hipFree(e.G32);
   e.G64 = nullptr;
   e.G32 = nullptr;
 }
@@ -149,7 +210,7 @@ struct JobDev {
 };
 
 struct PackOp : MvOp {
-  PackLayout lay64, lay32;
+  mutable PackLayout lay64, lay32;  // block tables follow compaction
   const void* blob64 = nullptr;
   const void* blob32 = nullptr;
   // the same launches without alpha, for QMRCLA's A^T = I - alpha^T T
@@ -193,6 +254,14 @@ struct PackOp : MvOp {
              int R, long long, int skip_stride) const override {
     return transposed<double2>(x, zero, y, s, skip, R, skip_stride);
   }
+  // Finished members dispatch no blocks: every blob's tables are rewritten
+  // (the transposed blobs share the layouts of the plain ones).
+  bool compact(const KState* st, const int* host_done, int, bool all, hipStream_t s) const override {
+    return pack_compact(lay64, const_cast<void*>(blob64), st, host_done, all, s) &&
+           pack_compact(lay32, const_cast<void*>(blob32), st, host_done, all, s) &&
+           (!blobt64 || pack_compact(lay64, const_cast<void*>(blobt64), st, host_done, all, s)) &&
+           (!blobt32 || pack_compact(lay32, const_cast<void*>(blobt32), st, host_done, all, s));
+  }
   bool mvt32(const float2* x, const float2* zero, float2* y, hipStream_t s, const int* skip, int R,
              long long, int skip_stride) const override {
     return transposed<float2>(x, zero, y, s, skip, R, skip_stride);
@@ -208,8 +277,13 @@ PackWs* pack_ws_new() {
       hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
     lds = 65536;
   w->lds_limit = (size_t)std::min(lds, 65536);
-  if (!hip_ok(hipStreamCreateWithFlags(&w->s, hipStreamNonBlocking), "hipStreamCreate")) {
-    delete w;
+  bool ok = hip_ok(hipStreamCreateWithFlags(&w->s, hipStreamNonBlocking), "hipStreamCreate") &&
+            hip_ok(hipEventCreateWithFlags(&w->ev_main, hipEventDisableTiming), "hipEventCreate");
+  for (int k = 0; ok && k < PackWs::kAux; ++k)
+    ok = hip_ok(hipStreamCreateWithFlags(&w->aux[k], hipStreamNonBlocking), "hipStreamCreate") &&
+         hip_ok(hipEventCreateWithFlags(&w->ev_aux[k], hipEventDisableTiming), "hipEventCreate");
+  if (!ok) {
+    pack_ws_delete(w);
     return nullptr;
   }
   return w;
@@ -225,9 +299,16 @@ void pack_ws_delete(PackWs* w) {
     (void)hipFree(t.second.tws64), (void)hipFree(t.second.tws32), (void)hipFree(t.second.pos);
   }
   for (auto& g : w->greens) green_free(g.second);
-  (void)hipFree(w->full);
+  for (int k = 0; k < PackWs::kAux; ++k) {
+    if (w->aux[k]) (void)hipStreamSynchronize(w->aux[k]);
+    (void)hipFree(w->afull[k]);
+    if (w->aux[k]) (void)hipStreamDestroy(w->aux[k]);
+    if (w->ev_aux[k]) (void)hipEventDestroy(w->ev_aux[k]);
+  }
+  if (w->ev_main) (void)hipEventDestroy(w->ev_main);
   (void)hipFree(w->arena);
   (void)hipFree(w->blob);
+  (void)hipHostFree(w->hstage);
   solver_delete(w->S);
   if (w->s) (void)hipStreamDestroy(w->s);
   delete w;
@@ -249,6 +330,7 @@ size_t pack_green_bytes(const ifdda_job& j) {
 int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts* o,
                volatile int* nstop) {
   const auto t0 = std::chrono::steady_clock::now();
+  PhaseClock clk{w.s};
   const int J = (int)in.size();
   if (J == 0) return IFDDA_OK;
   if (o->method < IFDDA_GPBICG1 || o->method > IFDDA_QMRCLA || (o->precision != 64 && o->precision != 32)) {
@@ -325,6 +407,7 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
     }
   }
   ++w.tick;
+  clk.mark("plan");
   // ---- Green tensors: cached per key; evict unused entries past the limit
   std::vector<JobDev> jd(J);
   {
@@ -350,47 +433,68 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
       w.greens.erase(victim);
     }
   }
-  for (int j = 0; j < J; ++j) {
-    const GridPlan& g = plans[j];
+  // FFT tables of every length the pack uses (uploaded and synchronised)
+  for (int j = 0; j < J; ++j)
     for (int a = 0; a < 3; ++a)
-      if (!(jd[j].tab3[a] = table_for(w, g.ax[a]))) return IFDDA_ERR_HIP;
-    auto it = w.greens.find(keys[j]);
-    if (it == w.greens.end()) {
+      if (!(jd[j].tab3[a] = table_for(w, plans[j].ax[a]))) return IFDDA_ERR_HIP;
+  // new keys: built round-robin on the auxiliary streams, each with its own
+  // full-grid scratch, after everything already queued on s
+  std::vector<int> build;  // first job of each new key
+  {
+    std::map<GreenKey, int> seen;
+    for (int j = 0; j < J; ++j)
+      if (!w.greens.count(keys[j]) && !seen.count(keys[j])) seen[keys[j]] = j, build.push_back(j);
+  }
+  if (!build.empty()) {
+    size_t need[PackWs::kAux] = {};
+    for (size_t i = 0; i < build.size(); ++i) {
+      const GridPlan& g = plans[build[i]];
+      need[i % PackWs::kAux] = std::max(need[i % PackWs::kAux],
+                                        sizeof(double2) * (size_t)g.L[0] * g.L[1] * g.L[2]);
+    }
+    if (!hip_ok(hipEventRecord(w.ev_main, w.s), "green event")) return IFDDA_ERR_HIP;
+    for (int k = 0; k < PackWs::kAux; ++k) {
+      if (!need[k]) continue;
+      // a growth frees the old scratch: the stream's earlier builds end first
+      if (need[k] > w.afull_cap[k] && !hip_ok(hipStreamSynchronize(w.aux[k]), "green scratch"))
+        return IFDDA_ERR_HIP;
+      if (!grow(&w.afull[k], &w.afull_cap[k], need[k], "hipMalloc green scratch") ||
+          !hip_ok(hipStreamWaitEvent(w.aux[k], w.ev_main, 0), "green wait"))
+        return IFDDA_ERR_HIP;
+    }
+    for (size_t i = 0; i < build.size(); ++i) {
+      const int j = build[i], k = (int)(i % PackWs::kAux);
+      const GridPlan& g = plans[j];
       GreenEntry e;
       e.g = g;
       const size_t nf = (size_t)g.F[0] * g.F[1] * g.F[2] * 6;
       e.bytes = nf * (sizeof(double2) + sizeof(float2));
-      const size_t M = (size_t)g.L[0] * g.L[1] * g.L[2];
-      if (!hip_ok(hipMalloc(&e.G64, sizeof(double2) * nf), "hipMalloc green") ||
-          !hip_ok(hipMalloc(&e.G32, sizeof(float2) * nf), "hipMalloc green")) {
+      if (!hip_ok(hipMalloc(&e.G64, sizeof(double2) * nf + sizeof(float2) * nf), "hipMalloc green")) {
         green_free(e);
         return IFDDA_ERR_HIP;
       }
-      if (M > w.full_cap) {
-        if (!hip_ok(hipStreamSynchronize(w.s), "green scratch")) return IFDDA_ERR_HIP;
-        char* p = reinterpret_cast<char*>(w.full);
-        size_t cap = w.full_cap * sizeof(double2);
-        if (!grow(&p, &cap, M * sizeof(double2), "hipMalloc green scratch")) {
-          green_free(e);
-          return IFDDA_ERR_HIP;
-        }
-        w.full = reinterpret_cast<double2*>(p);
-        w.full_cap = cap / sizeof(double2);
-      }
+      e.G32 = reinterpret_cast<float2*>(e.G64 + nf);
       const double2* tw[3] = {jd[j].tab3[0]->tw64, jd[j].tab3[1]->tw64, jd[j].tab3[2]->tw64};
       const int* pos[3] = {jd[j].tab3[0]->pos, jd[j].tab3[1]->pos, jd[j].tab3[2]->pos};
-      if (!green_build_device(g, tw, pos, keys[j].d, keys[j].k0, keys[j].nquad, w.full, e.G64,
-                              e.G32, w.s)) {
+      if (!green_build_device(g, tw, pos, keys[j].d, keys[j].k0, keys[j].nquad,
+                              reinterpret_cast<double2*>(w.afull[k]), e.G64, e.G32, w.aux[k])) {
         green_free(e);
         return IFDDA_ERR_HIP;
       }
       e.used = w.tick;
       w.green_bytes += e.bytes;
-      it = w.greens.emplace(keys[j], e).first;
+      w.greens.emplace(keys[j], e);
     }
-    jd[j].green = &it->second;
+    for (int k = 0; k < PackWs::kAux; ++k)
+      if (need[k] && (!hip_ok(hipEventRecord(w.ev_aux[k], w.aux[k]), "green event") ||
+                      !hip_ok(hipStreamWaitEvent(w.s, w.ev_aux[k], 0), "green wait")))
+        return IFDDA_ERR_HIP;
+  }
+  for (int j = 0; j < J; ++j) {
+    jd[j].green = &w.greens.find(keys[j])->second;
     jd[j].nd = in[j].job->nd;
   }
+  clk.mark("green");
   // ---- members and the ragged vector layout (offsets 64-element aligned)
   std::vector<int> mjob, mrhs;
   for (int j = 0; j < J; ++j)
@@ -435,69 +539,70 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
   if (used > w.arena_cap && !hip_ok(hipStreamSynchronize(w.s), "arena")) return IFDDA_ERR_HIP;
   if (!grow(&w.arena, &w.arena_cap, used, "hipMalloc pack arena")) return IFDDA_ERR_HIP;
   char* A = w.arena;
-  // host staging of the polarisabilities (FP64 and FP32), alive until the sync below
-  std::vector<std::vector<double2>> h64(J);
-  std::vector<std::vector<float2>> h32(J);
-  for (int j = 0; j < J; ++j) {
+  // Inputs: a host image of the arena's input part (maps, polarisabilities,
+  // wave parameters, dipole cells, member offsets) and of the incident
+  // fields, filled in pinned memory by several threads, then two copies.
+  const size_t in_bytes = o_off + sizeof(long long) * (size_t)P;
+  const size_t e_bytes = sizeof(double2) * (size_t)total;
+  const size_t out_bytes = (o_FL - o_FF) + e_bytes;  // the FF and FFloc blocks
+  if (std::max(in_bytes + e_bytes, out_bytes) > w.hstage_cap &&
+      (!hip_ok(hipStreamSynchronize(w.s), "staging") ||
+       !grow_host(&w.hstage, &w.hstage_cap, std::max(in_bytes + e_bytes, out_bytes))))
+    return IFDDA_ERR_HIP;
+  char* H = w.hstage;
+  par_for(J, in_bytes, [&](int j) {
     const ifdda_job& s = *in[j].job;
-    if (!dense[j]) {
-      jd[j].tab = reinterpret_cast<int32_t*>(A + o_tab[j]);
-      if (!hip_ok(hipMemcpyAsync(jd[j].tab, s.tabdip, sizeof(int32_t) * (size_t)s.nx * s.ny * s.nz,
-                                 hipMemcpyHostToDevice, w.s), "copy tabdip"))
-        return IFDDA_ERR_HIP;
-    }
-    h64[j].resize(s.nd);
-    h32[j].resize(s.nd);
+    const size_t N = (size_t)s.nx * s.ny * s.nz;
+    if (!dense[j]) std::memcpy(H + o_tab[j], s.tabdip, sizeof(int32_t) * N);
+    double2* h64 = reinterpret_cast<double2*>(H + o_al64[j]);
+    float2* h32 = reinterpret_cast<float2*>(H + o_al32[j]);
     for (int i = 0; i < s.nd; ++i) {
-      h64[j][i] = s.alpha ? make_double2(s.alpha[2 * i], s.alpha[2 * i + 1])
-                          : poladiff_host(make_double2(s.eps[2 * i], s.eps[2 * i + 1]), s.d, s.k0,
-                                          s.eps0 > 0.0 ? s.eps0 : 1.0, s.pola);
-      h32[j][i] = make_float2((float)h64[j][i].x, (float)h64[j][i].y);
+      h64[i] = s.alpha ? make_double2(s.alpha[2 * i], s.alpha[2 * i + 1])
+                       : poladiff_host(make_double2(s.eps[2 * i], s.eps[2 * i + 1]), s.d, s.k0,
+                                       s.eps0 > 0.0 ? s.eps0 : 1.0, s.pola);
+      h32[i] = make_float2((float)h64[i].x, (float)h64[i].y);
     }
+    if (!s.E_inc) {  // plane waves: k and e per RHS, the cell of each dipole
+      std::memcpy(H + o_wk[j], wk[j].data(), sizeof(double) * wk[j].size());
+      std::memcpy(H + o_we[j], we[j].data(), sizeof(double2) * we[j].size());
+      if (!dense[j]) {
+        int32_t* cell = reinterpret_cast<int32_t*>(H + o_cell[j]);
+        for (size_t i = 0; i < N; ++i)
+          if (s.tabdip[i] > 0) cell[s.tabdip[i] - 1] = (int32_t)i;
+      }
+    }
+  });
+  std::memcpy(H + o_off, w.offs.data(), sizeof(long long) * P);
+  par_for(P, e_bytes, [&](int m) {
+    const ifdda_job& s = *in[mjob[m]].job;
+    if (s.E_inc)
+      std::memcpy(H + in_bytes + sizeof(double2) * w.offs[m],
+                  s.E_inc + 2 * (size_t)w.lens[m] * mrhs[m], sizeof(double2) * w.lens[m]);
+  });
+  if (!hip_ok(hipMemcpyAsync(A, H, in_bytes, hipMemcpyHostToDevice, w.s), "copy inputs") ||
+      !hip_ok(hipMemcpyAsync(A + o_E, H + in_bytes, e_bytes, hipMemcpyHostToDevice, w.s),
+              "copy E_inc"))
+    return IFDDA_ERR_HIP;
+  for (int j = 0; j < J; ++j) {
+    jd[j].tab = dense[j] ? nullptr : reinterpret_cast<int32_t*>(A + o_tab[j]);
     jd[j].al64 = reinterpret_cast<double2*>(A + o_al64[j]);
     jd[j].al32 = reinterpret_cast<float2*>(A + o_al32[j]);
-    if (!hip_ok(hipMemcpyAsync(jd[j].al64, h64[j].data(), sizeof(double2) * s.nd, hipMemcpyHostToDevice, w.s), "copy alpha") ||
-        !hip_ok(hipMemcpyAsync(jd[j].al32, h32[j].data(), sizeof(float2) * s.nd, hipMemcpyHostToDevice, w.s), "copy alpha"))
-      return IFDDA_ERR_HIP;
   }
   double2* E = reinterpret_cast<double2*>(A + o_E);
   double2* FF = reinterpret_cast<double2*>(A + o_FF);
   double2* FL = reinterpret_cast<double2*>(A + o_FL);
-  for (int m = 0; m < P; ++m) {
-    const ifdda_job& s = *in[mjob[m]].job;
-    if (!s.E_inc) continue;
-    const double* src = s.E_inc + 2 * (size_t)w.lens[m] * mrhs[m];
-    if (!hip_ok(hipMemcpyAsync(E + w.offs[m], src, sizeof(double2) * w.lens[m], hipMemcpyHostToDevice, w.s),
-                "copy E_inc"))
-      return IFDDA_ERR_HIP;
-  }
-  // waves formed on the device at each job's dipoles (cells from its map)
-  long long* doff = reinterpret_cast<long long*>(A + o_off);
-  if (!hip_ok(hipMemcpyAsync(doff, w.offs.data(), sizeof(long long) * P, hipMemcpyHostToDevice, w.s),
-              "copy offsets"))
-    return IFDDA_ERR_HIP;
-  std::vector<std::vector<int32_t>> cells(J);
+  // waves formed on the device at each job's dipoles
+  const long long* doff = reinterpret_cast<const long long*>(A + o_off);
   for (int j = 0, m0 = 0; j < J; m0 += in[j].job->nrhs, ++j) {
     const ifdda_job& s = *in[j].job;
     if (s.E_inc) continue;
-    int32_t* dcell = nullptr;
-    if (!dense[j]) {
-      const size_t N = (size_t)s.nx * s.ny * s.nz;
-      cells[j].assign(s.nd, -1);
-      for (size_t i = 0; i < N; ++i)
-        if (s.tabdip[i] > 0) cells[j][s.tabdip[i] - 1] = (int32_t)i;
-      dcell = reinterpret_cast<int32_t*>(A + o_cell[j]);
-      if (!hip_ok(hipMemcpyAsync(dcell, cells[j].data(), sizeof(int32_t) * s.nd, hipMemcpyHostToDevice, w.s),
-                  "copy dipole cells"))
-        return IFDDA_ERR_HIP;
-    }
-    double* dk = reinterpret_cast<double*>(A + o_wk[j]);
-    double2* de = reinterpret_cast<double2*>(A + o_we[j]);
-    if (!hip_ok(hipMemcpyAsync(dk, wk[j].data(), sizeof(double) * wk[j].size(), hipMemcpyHostToDevice, w.s), "copy waves") ||
-        !hip_ok(hipMemcpyAsync(de, we[j].data(), sizeof(double2) * we[j].size(), hipMemcpyHostToDevice, w.s), "copy waves") ||
-        !launch_waves(E, doff + m0, s.nrhs, s.nd, dcell, s.nx, s.ny, s.origin, s.d, dk, de, w.s))
+    const int32_t* dcell = dense[j] ? nullptr : reinterpret_cast<const int32_t*>(A + o_cell[j]);
+    if (!launch_waves(E, doff + m0, s.nrhs, s.nd, dcell, s.nx, s.ny, s.origin, s.d,
+                      reinterpret_cast<const double*>(A + o_wk[j]),
+                      reinterpret_cast<const double2*>(A + o_we[j]), w.s))
       return IFDDA_ERR_HIP;
   }
+  clk.mark("inputs");
   // ---- packed-launch argument blocks
   PackOp op;
   {
@@ -573,26 +678,40 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
     if (debug_on()) std::fprintf(stderr, "ifdda pack: solver for %d RHS\n", P);
   }
   std::vector<ifdda_solve_info> info(P);
+  clk.mark("blob");
   const int rc = solve_core(op, w.S, o, P, E, FF, FL, info.data(), nstop, w.s, t0);
   if (rc != IFDDA_OK) return rc;
-  // ---- outputs
-  for (int m = 0; m < P; ++m) {
+  clk.mark("solve");
+  // ---- outputs: one copy of the FF and FFloc blocks into the pinned image,
+  // then each member's slices to its job's arrays on several threads
+  (void)FF;
+  (void)FL;
+  if (!hip_ok(hipMemcpyAsync(H, A + o_FF, out_bytes, hipMemcpyDeviceToHost, w.s), "copy outputs") ||
+      !hip_ok(hipStreamSynchronize(w.s), "pack outputs"))
+    return IFDDA_ERR_HIP;
+  par_for(P, out_bytes, [&](int m) {
     const PackIn& p = in[mjob[m]];
     const size_t at = 2 * (size_t)w.lens[m] * mrhs[m];
-    if (p.FFloc && !hip_ok(hipMemcpyAsync(p.FFloc + at, FL + w.offs[m], sizeof(double2) * w.lens[m],
-                                          hipMemcpyDeviceToHost, w.s), "copy FFloc"))
-      return IFDDA_ERR_HIP;
-    if (p.FF && !hip_ok(hipMemcpyAsync(p.FF + at, FF + w.offs[m], sizeof(double2) * w.lens[m],
-                                       hipMemcpyDeviceToHost, w.s), "copy FF"))
-      return IFDDA_ERR_HIP;
-    if (p.infos) p.infos[mrhs[m]] = info[m];
-  }
-  return hip_ok(hipStreamSynchronize(w.s), "pack outputs") ? IFDDA_OK : IFDDA_ERR_HIP;
+    const size_t bytes = sizeof(double2) * w.lens[m];
+    if (p.FF) std::memcpy(p.FF + at, H + sizeof(double2) * w.offs[m], bytes);
+    if (p.FFloc) std::memcpy(p.FFloc + at, H + (o_FL - o_FF) + sizeof(double2) * w.offs[m], bytes);
+  });
+  for (int m = 0; m < P; ++m)
+    if (in[mjob[m]].infos) in[mjob[m]].infos[mrhs[m]] = info[m];
+  clk.mark("outputs");
+  if (debug_on()) std::fprintf(stderr, "ifdda pack: %d jobs %d RHS:%s\n", J, P, clk.line.c_str());
+  return IFDDA_OK;
 }
 
 }  // namespace ifdda
 
 using namespace ifdda;
+
+namespace {
+constexpr int kBatchDevices = 64;
+std::mutex g_batch_mu[kBatchDevices];
+PackWs* g_batch_ws[kBatchDevices] = {};
+}  // namespace
 
 extern "C" int ifdda_solve_batch(int njobs, const ifdda_job* jobs, double* const* FFloc,
                                  double* const* FF, ifdda_solve_info* const* infos,
@@ -617,9 +736,29 @@ extern "C" int ifdda_solve_batch(int njobs, const ifdda_job* jobs, double* const
     in[j] = PackIn{&jobs[j], FFloc ? FFloc[j] : nullptr, FF ? FF[j] : nullptr, infos[j]};
   }
   if (njobs == 0) return IFDDA_OK;
-  PackWs* w = pack_ws_new();
-  if (!w) return IFDDA_ERR_HIP;
-  const int rc = pack_solve(*w, in, &jobs[0].opts, nstop);
-  pack_ws_delete(w);
-  return rc;
+  int dev = 0;
+  if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return IFDDA_ERR_HIP;
+  if (dev < 0 || dev >= kBatchDevices) {
+    set_error("ifdda_solve_batch: device index out of range");
+    return IFDDA_ERR_ARG;
+  }
+  // one workspace per device, kept between calls (FFT tables, recent Green
+  // tensors, grow-only buffers), one batch at a time on it
+  std::lock_guard<std::mutex> lk(g_batch_mu[dev]);
+  if (!g_batch_ws[dev] && !(g_batch_ws[dev] = pack_ws_new())) return IFDDA_ERR_HIP;
+  return pack_solve(*g_batch_ws[dev], in, &jobs[0].opts, nstop);
+}
+
+extern "C" int ifdda_solve_batch_release(void) {
+  for (int dev = 0; dev < kBatchDevices; ++dev) {
+    std::lock_guard<std::mutex> lk(g_batch_mu[dev]);
+    if (!g_batch_ws[dev]) continue;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(dev);
+    pack_ws_delete(g_batch_ws[dev]);
+    g_batch_ws[dev] = nullptr;
+    (void)hipSetDevice(cur);
+  }
+  return IFDDA_OK;
 }

@@ -8,5 +8,5 @@ library is missing.
 """
 from .capi import (  # noqa: F401
     GPBICG1, QMRBICGSTAB1, QMRBICGSTAB2, TFQMR, METHODS, IfddaError, Runner, Solver, lib, lib_path,
-    solve_batch,
+    solve_batch, solve_batch_release,
 )

@@ -124,6 +124,7 @@ def lib():
     L.ifdda_polarizability_device.argtypes = [vp, vp, dp, dp, dp, ip]
     L.ifdda_solve_batch.argtypes = [ip, C.POINTER(Job), C.POINTER(pd), C.POINTER(pd),
                                     C.POINTER(C.POINTER(SolveInfo)), C.POINTER(C.c_int)]
+    L.ifdda_solve_batch_release.argtypes = []
     L.ifdda_microscope_incidences.argtypes = [vp, C.POINTER(MicroscopeOpts), C.POINTER(C.c_int), pd,
                                               ip, pd]
     L.ifdda_microsdf_incidences.argtypes = [vp, C.POINTER(MicroscopeOpts), C.POINTER(C.c_int), pd,
@@ -652,6 +653,11 @@ def _job(nx, ny, nz, d, k0, opts, alpha=None, E_inc=None, tabdip=None, nquad=0, 
     return j, keep, E.shape[0], al.size
 
 
+def solve_batch_release():
+    """Free the workspace solve_batch keeps between calls."""
+    _check(lib().ifdda_solve_batch_release(), "ifdda_solve_batch_release")
+
+
 def solve_batch(problems, method="GPBICG1", tol=1e-4, precision=32, x0_zero=True, maxit=10000,
                 check_every=0):
     """Independent problems solved together (ifdda_solve_batch): every
@@ -668,8 +674,8 @@ def solve_batch(problems, method="GPBICG1", tol=1e-4, precision=32, x0_zero=True
                            p.get("E_inc"), p.get("tabdip"), p.get("nquad", 0), **extra)
         jobs.append(j)
         keep.append(k)
-        outs.append({"FFloc": np.zeros((R, 3 * nd), np.complex128),
-                     "FF": np.zeros((R, 3 * nd), np.complex128), "_info": (SolveInfo * R)()})
+        outs.append({"FFloc": np.empty((R, 3 * nd), np.complex128),
+                     "FF": np.empty((R, 3 * nd), np.complex128), "_info": (SolveInfo * R)()})
     n = len(jobs)
     D = C.POINTER(C.c_double)
     arr = (Job * max(n, 1))(*jobs)

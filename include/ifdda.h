@@ -537,9 +537,15 @@ int ifdda_polarizability_device(ifdda_ctx *ctx, const double *eps, double d, dou
  * polarisabilities) and one lockstep Krylov solve with per-RHS state and
  * convergence, so each result equals the job's own solve.  All jobs use
  * jobs[0].opts.  FFloc[j] / FF[j] (nrhs * 3 nd complex, or FFloc / FF NULL)
- * and infos[j] (nrhs entries) receive job j's results. */
+ * and infos[j] (nrhs entries) receive job j's results.  Right-hand sides
+ * that have converged drop out of every launch.  The workspace of the
+ * current device (FFT tables, the Green tensors of recent (box, d, k0)
+ * keys, grow-only buffers) is kept between calls; calls on one device run
+ * one at a time. */
 int ifdda_solve_batch(int njobs, const ifdda_job *jobs, double *const *FFloc, double *const *FF,
                       ifdda_solve_info *const *infos, volatile int *nstop);
+/* Free the workspaces ifdda_solve_batch keeps (every device). */
+int ifdda_solve_batch_release(void);
 
 typedef struct ifdda_runner ifdda_runner;
 int ifdda_runner_create(int nworkers, double mem_budget, ifdda_runner **out);
