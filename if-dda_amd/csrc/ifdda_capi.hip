@@ -72,6 +72,9 @@ extern "C" void ifdda_destroy(ifdda_ctx* c) {
   (void)hipFree(c->ffbuf);
   (void)hipFree(c->dipcell);
   (void)hipFree(c->mbuf);
+  (void)hipFree(c->gfull);
+  (void)hipFree(c->xbuf);
+  green_scratch_free(c->gscr);
   ifdda_timing_enable(c, 0);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -172,6 +175,21 @@ extern "C" int ifdda_fft_lengths(const ifdda_ctx* c, int L[3]) {
   return IFDDA_OK;
 }
 
+// The context's full-grid Green scratch, grown to M complex values (the
+// stream drains before a smaller buffer is freed).
+double2* ifdda::green_full(ifdda_ctx* c, size_t M) {
+  const size_t bytes = sizeof(double2) * M;
+  if (bytes > c->gfull_cap) {
+    if (c->gfull && !hip_ok(hipStreamSynchronize(c->stream), "green scratch")) return nullptr;
+    (void)hipFree(c->gfull);
+    c->gfull = nullptr;
+    c->gfull_cap = 0;
+    if (!hip_ok(hipMalloc(&c->gfull, bytes), "hipMalloc green scratch")) return nullptr;
+    c->gfull_cap = bytes;
+  }
+  return c->gfull;
+}
+
 extern "C" int ifdda_green_build(ifdda_ctx* c, double aretecube, double k0, int nquad,
                                  double tolinit) {
   (void)tolinit;  // RELREQ is unused by the reference quadrature (propaesplibreintsim.f:2-107)
@@ -179,19 +197,18 @@ extern "C" int ifdda_green_build(ifdda_ctx* c, double aretecube, double k0, int 
   if (!(aretecube > 0.0) || !(k0 > 0.0)) return arg_error("ifdda_green_build: need d > 0, k0 > 0");
   if (nquad < 0 || nquad > 10) return arg_error("ifdda_green_build: nquad must be in [0,10]");
   const size_t M = (size_t)c->g.L[0] * c->g.L[1] * c->g.L[2];
-  double2* full = nullptr;
-  CHECK_HIP(hipMalloc(&full, sizeof(double2) * M), "hipMalloc green scratch");
+  double2* full = green_full(c, M);
+  if (!full) return IFDDA_ERR_HIP;
   const double2* tw[3] = {c->tw64[0], c->tw64[1], c->tw64[2]};
   const int* pos[3] = {c->pos[0], c->pos[1], c->pos[2]};
   c->green_ready = false;
   c->gd_ready = false;
   bool ok = green_build_device(c->g, tw, pos, aretecube, k0, nquad, full, c->G64, c->G32,
-                               c->stream);
+                               &c->gscr, c->stream);
   ok = ok && hip_ok(hipStreamSynchronize(c->stream), "green build");
-  (void)hipFree(full);
   if (!ok) return IFDDA_ERR_HIP;
   c->green_ready = true;
-  c->green_host = nullptr;
+  for (auto& h : c->green_host) h = nullptr;
   return IFDDA_OK;
 }
 
@@ -237,8 +254,8 @@ extern "C" int ifdda_green_set(ifdda_ctx* c, const double* const G[6]) {
       }
     }
   }
-  double2* full = nullptr;
-  CHECK_HIP(hipMalloc(&full, sizeof(double2) * M), "hipMalloc green scratch");
+  double2* full = green_full(c, M);
+  if (!full) return IFDDA_ERR_HIP;
   bool ok = true;
   for (int comp = 0; ok && comp < 6; ++comp) {
     ok = hip_ok(hipMemcpyAsync(full, G[comp], sizeof(double2) * M, hipMemcpyHostToDevice,
@@ -247,11 +264,10 @@ extern "C" int ifdda_green_set(ifdda_ctx* c, const double* const G[6]) {
          green_fold_component(g, full, comp, c->G64, c->G32, c->stream) &&
          hip_ok(hipStreamSynchronize(c->stream), "green fold");
   }
-  (void)hipFree(full);
   c->gd_ready = false;
   if (!ok) return IFDDA_ERR_HIP;
   c->green_ready = true;
-  c->green_host = nullptr;
+  for (auto& h : c->green_host) h = nullptr;
   return IFDDA_OK;
 }
 
@@ -551,6 +567,19 @@ extern "C" int ifdda_matvec(ifdda_ctx* c, const double* FFX, double* FFB) {
 // solves (cdmlib.f:3682-3730) are seen.
 // ---------------------------------------------------------------------------
 
+// Grow-only device staging of the host-array far-field entries (these run
+// synchronously, so a growth frees a buffer no work still reads).
+static void* x_stage(ifdda_ctx* c, size_t bytes) {
+  if (bytes > c->xcap) {
+    (void)hipFree(c->xbuf);
+    c->xbuf = nullptr;
+    c->xcap = 0;
+    if (!hip_ok(hipMalloc(&c->xbuf, bytes), "hipMalloc far-field staging")) return nullptr;
+    c->xcap = bytes;
+  }
+  return c->xbuf;
+}
+
 static std::mutex g_reg_mu;
 static std::map<std::array<int, 3>, ifdda_ctx*> g_reg;
 
@@ -565,25 +594,34 @@ static ifdda_ctx* ctx_for(int nx, int ny, int nz) {
   return c;
 }
 
+// A hash of every 61st complex value of a component (and of its last one):
+// a Green tensor rebuilt for another wavelength, lattice step or box changes
+// every value, so any rebuild between calls changes it.
+static uint64_t green_hash(const double* G, size_t M) {
+  uint64_t h = 1469598103934665603ULL;
+  auto mix = [&](size_t i) {
+    uint64_t a, b;
+    std::memcpy(&a, G + 2 * i, sizeof(a));
+    std::memcpy(&b, G + 2 * i + 1, sizeof(b));
+    h = (h ^ a) * 1099511628211ULL;
+    h = (h ^ b) * 1099511628211ULL;
+  };
+  for (size_t i = 0; i < M; i += 61) mix(i);
+  if (M) mix(M - 1);
+  return h;
+}
+
 static void take_fingerprint(ifdda_ctx* c, const double* const G[6], size_t M) {
-  for (int comp = 0; comp < 6; ++comp)
-    for (int s = 0; s < 16; ++s) {
-      const size_t idx = (size_t)(((unsigned long long)(s + 7) * 40503ULL * 65537ULL) % M);
-      c->green_sample[comp][2 * s] = G[comp][2 * idx];
-      c->green_sample[comp][2 * s + 1] = G[comp][2 * idx + 1];
-    }
-  c->green_host = G[0];
+  for (int comp = 0; comp < 6; ++comp) {
+    c->green_host[comp] = G[comp];
+    c->green_sig[comp] = green_hash(G[comp], M);
+  }
 }
 
 static bool fingerprint_matches(const ifdda_ctx* c, const double* const G[6], size_t M) {
-  if (!c->green_ready || c->green_host != G[0]) return false;
+  if (!c->green_ready) return false;
   for (int comp = 0; comp < 6; ++comp)
-    for (int s = 0; s < 16; ++s) {
-      const size_t idx = (size_t)(((unsigned long long)(s + 7) * 40503ULL * 65537ULL) % M);
-      if (c->green_sample[comp][2 * s] != G[comp][2 * idx] ||
-          c->green_sample[comp][2 * s + 1] != G[comp][2 * idx + 1])
-        return false;
-    }
+    if (c->green_host[comp] != G[comp] || c->green_sig[comp] != green_hash(G[comp], M)) return false;
   return true;
 }
 
@@ -838,10 +876,10 @@ static void driver_entry(const double* const G[6], const int32_t* Tabdip, int nx
     o.tol = tol;
     rc = ifdda_solve(c, &o, FF0, FF, FFloc, &info, (volatile int*)nstop);
   }
-  if (rc) {
+  if (rc) {  // a device or argument failure, not a solver breakdown
     report("inverserig");
     *nstop = 1;
-    fill_info(infostr, len_infostr, "Problem to solve Ax=b!");
+    fill_info(infostr, len_infostr, ("GPU: " + g_err).c_str());
     return;
   }
   *tol1 = info.tol1;
@@ -944,10 +982,11 @@ extern "C" void ifdda_diffractefft2d(
   ifdda_ctx* c = ctx_for(*nx, *ny, *nz);
   const long long ncell = (long long)(*nx) * (*ny) * (*nz);
   const int n = 2 * im + 1;
-  double2 *dF = nullptr, *dE = nullptr;
-  bool ok = c && hip_ok(hipMalloc(&dF, sizeof(double2) * 3 * ncell), "hipMalloc FF") &&
-            hip_ok(hipMalloc(&dE, sizeof(double2) * 6 * (size_t)n * n), "hipMalloc Ediff") &&
-            hip_ok(hipMemcpy(dF, FF, sizeof(double2) * 3 * ncell, hipMemcpyHostToDevice), "copy FF");
+  double2* dF = c ? (double2*)x_stage(c, sizeof(double2) * (3 * (size_t)ncell + 6 * (size_t)n * n))
+                  : nullptr;
+  double2* dE = dF ? dF + 3 * ncell : nullptr;
+  bool ok = dF && hip_ok(hipMemcpy(dF, FF, sizeof(double2) * 3 * ncell, hipMemcpyHostToDevice),
+                         "copy FF");
   if (ok)
     ok = farfield_dev(c, 1, dF, nullptr, ncell, N, *k0, *aretecube, xs[0], ys[0], zs[0], dE,
                       dE + 3 * (size_t)n * n, &im, &dk, c->stream, 0, 0.0, 1) == IFDDA_OK;
@@ -955,8 +994,6 @@ extern "C" void ifdda_diffractefft2d(
   if (ok)
     ok = hip_ok(hipMemcpy(hE.data(), dE, sizeof(double2) * hE.size(), hipMemcpyDeviceToHost),
                 "copy Ediff");
-  (void)hipFree(dF);
-  (void)hipFree(dE);
   if (!ok) {
     report("diffractefft2d");
     *nstop = 1;
@@ -999,14 +1036,13 @@ extern "C" void ifdda_computegcfft2d(const int32_t* imax, const double* deltakx,
   *gasym = 0.0;
   *Cscai = 0.0;
   ifdda_ctx* c = ctx_for(1, 1, 1);
-  double2* dE = nullptr;
+  double2* dE = c ? (double2*)x_stage(c, sizeof(double2) * h.size()) : nullptr;
   double out[2] = {0.0, 0.0};
-  bool ok = c && hip_ok(hipMalloc(&dE, sizeof(double2) * h.size()), "hipMalloc Ediff") &&
+  bool ok = dE &&
             hip_ok(hipMemcpy(dE, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice),
                    "copy Ediff") &&
             gc_dev(c, 1, dE, dE + 3 * (size_t)n * n, *imax, *deltakx, *k0, I0, theta, phi, out,
                    c->stream) == IFDDA_OK;
-  (void)hipFree(dE);
   if (!ok) {
     report("computegcfft2d");
     return;

@@ -2108,13 +2108,38 @@ bool green_dbuild_device(const GridPlan& g, const double2* tw64[3], const int* p
   return ok;
 }
 
+__global__ void k_fill_zero(double2* p, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = make_double2(0.0, 0.0);
+}
+
+void green_scratch_free(GreenScratch& gs) {
+  (void)hipFree(gs.near);
+  (void)hipFree(gs.part);
+  (void)hipFree(gs.offs);
+  gs = GreenScratch{};
+}
+
+// Grow-only device buffer for a build on stream s (the stream's earlier work
+// ends before the old buffer is freed).
+template <typename T>
+static bool scratch_fit(T** p, size_t* cap, size_t bytes, hipStream_t s, const char* what) {
+  if (bytes <= *cap) return true;
+  if (*p && !hip_ok(hipStreamSynchronize(s), what)) return false;
+  (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  if (!hip_ok(hipMalloc(reinterpret_cast<void**>(p), bytes), what)) return false;
+  *cap = bytes;
+  return true;
+}
+
 bool green_build_device(const GridPlan& g, const double2* tw64[3], const int* pos[3], double d,
                         double k0, int nquad, double2* full, double2* g64, float2* g32,
-                        hipStream_t s) {
+                        GreenScratch* gs, hipStream_t s) {
   const long long M = (long long)g.L[0] * g.L[1] * g.L[2];
   double2* near = nullptr;
-  double2* part = nullptr;
-  int* offs = nullptr;
   bool ok = true;
   if (nquad > 0) {
     // Near cells: |offset|^2 <= nquad^2 inside the box reach (calculgreenquadfft.f:50).
@@ -2129,17 +2154,23 @@ bool green_build_device(const GridPlan& g, const double2* tw64[3], const int* po
           }
     const int ncell = (int)h.size() / 3;
     const int w = 2 * nquad + 1;
-    ok = hip_ok(hipMalloc(&near, sizeof(double2) * 6 * w * w * w), "hipMalloc near") &&
-         hip_ok(hipMalloc(&part, sizeof(double2) * 6 * 129 * (size_t)ncell), "hipMalloc part") &&
-         hip_ok(hipMalloc(&offs, sizeof(int) * h.size()), "hipMalloc offs") &&
-         hip_ok(hipMemcpyAsync(offs, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, s),
-                "copy offs") &&
-         hip_ok(hipMemsetAsync(near, 0, sizeof(double2) * 6 * w * w * w, s), "memset near");
+    const long long nnear = 6LL * w * w * w;
+    ok = gs && scratch_fit(&gs->near, &gs->near_cap, sizeof(double2) * nnear, s, "green near") &&
+         scratch_fit(&gs->part, &gs->part_cap, sizeof(double2) * 6 * 129 * (size_t)ncell, s,
+                     "green near") &&
+         scratch_fit(&gs->offs, &gs->offs_cap, sizeof(int) * h.size(), s, "green near") &&
+         hip_ok(hipMemcpyAsync(gs->offs, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, s),
+                "copy offs");
     if (ok) {
-      hipLaunchKernelGGL(k_green_near_partial, dim3(129, ncell), dim3(256), 0, s, part, offs, d, k0);
-      hipLaunchKernelGGL(k_green_near_final, dim3((ncell + 63) / 64), dim3(64), 0, s, near, part,
-                         offs, ncell, nquad, d, k0, 1.0);
-      ok = hip_ok(hipGetLastError(), "green near");
+      near = gs->near;
+      hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((nnear + 255) / 256)), dim3(256), 0, s, near,
+                         nnear);
+      hipLaunchKernelGGL(k_green_near_partial, dim3(129, ncell), dim3(256), 0, s, gs->part,
+                         gs->offs, d, k0);
+      hipLaunchKernelGGL(k_green_near_final, dim3((ncell + 63) / 64), dim3(64), 0, s, near,
+                         gs->part, gs->offs, ncell, nquad, d, k0, 1.0);
+      ok = hip_ok(hipGetLastError(), "green near") &&
+           hip_ok(hipStreamSynchronize(s), "green near");  // h (copied from) ends here
     }
   }
   for (int c = 0; ok && c < 6; ++c) {
@@ -2148,12 +2179,6 @@ bool green_build_device(const GridPlan& g, const double2* tw64[3], const int* po
                        g.L[0], g.L[1], g.L[2], d, k0, near, nquad);
     ok = hip_ok(hipGetLastError(), "green fill") && fft3d_full_f64(g, tw64, pos, full, s) &&
          green_fold_component(g, full, c, g64, g32, s);
-  }
-  if (near || part || offs) {
-    hip_ok(hipStreamSynchronize(s), "green sync");
-    (void)hipFree(near);
-    (void)hipFree(part);
-    (void)hipFree(offs);
   }
   return ok;
 }

@@ -150,15 +150,14 @@ bool ensure_dgreen(ifdda_ctx* c, double d, double k0, hipStream_t s) {
   if (!c->Gd64 && !hip_ok(hipMalloc(&c->Gd64, sizeof(double2) * nf * 3), "hipMalloc force Green"))
     return false;
   const size_t M = (size_t)c->g.L[0] * c->g.L[1] * c->g.L[2];
-  double2* full = nullptr;
-  if (!hip_ok(hipMalloc(&full, sizeof(double2) * M), "hipMalloc green scratch")) return false;
+  double2* full = green_full(c, M);
+  if (!full) return false;
   const double2* tw[3] = {c->tw64[0], c->tw64[1], c->tw64[2]};
   const int* pos[3] = {c->pos[0], c->pos[1], c->pos[2]};
   bool ok = true;
   for (int dd = 0; ok && dd < 3; ++dd)
     ok = green_dbuild_device(c->g, tw, pos, d, k0, dd, full, c->Gd64 + nf * dd, s);
   ok = ok && hip_ok(hipStreamSynchronize(s), "force Green build");
-  (void)hipFree(full);
   if (!ok) return false;
   c->gd_ready = true;
   c->gd_d = d;

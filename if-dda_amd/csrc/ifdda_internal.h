@@ -187,9 +187,20 @@ bool launch_matvec_pack_f32(const PackLayout& lay, const void* dev, const float2
 
 // Green tensor on the device: fills g64/g32 (folded, F2*F3*F1*6) using
 // `full` (L1*L2*L3 FP64 scratch).  nquad > 0 integrates near cells.
+// Grow-only device buffers of the near-cell quadrature (nquad > 0), kept by
+// the owner of the builds (a context or a pack workspace) between builds.
+struct GreenScratch {
+  double2* near = nullptr;
+  size_t near_cap = 0;  // bytes
+  double2* part = nullptr;
+  size_t part_cap = 0;
+  int* offs = nullptr;
+  size_t offs_cap = 0;
+};
+void green_scratch_free(GreenScratch& gs);
 bool green_build_device(const GridPlan& g, const double2* tw64[3], const int* pos[3], double d,
                         double k0, int nquad, double2* full, double2* g64, float2* g32,
-                        hipStream_t s);
+                        GreenScratch* gs, hipStream_t s);
 // Fold one caller-supplied FFT'd component already in `full` into g64/g32.
 bool green_fold_component(const GridPlan& g, const double2* full, int comp, double2* g64,
                           float2* g32, hipStream_t s);
@@ -234,9 +245,15 @@ struct ifdda_ctx {
   int cellbox[6] = {0, 0, 0, 0, 0, 0};  // occupied cells: i, j, k min / max (0-based)
   void* mbuf = nullptr;   // microscope workspace (ifdda_microscope.hip)
   size_t mcap = 0;
-  // provenance of the device Green tensor for the reference-shaped entries
-  const void* green_host = nullptr;
-  double green_sample[6][2 * 16] = {};
+  double2* gfull = nullptr;  // full-grid scratch of the Green builds (grow-only)
+  size_t gfull_cap = 0;      // bytes
+  ifdda::GreenScratch gscr;  // near-cell quadrature buffers
+  void* xbuf = nullptr;      // device staging of the host-array far-field entries (grow-only)
+  size_t xcap = 0;
+  // provenance of the device Green tensor for the reference-shaped entries:
+  // the six host arrays and a hash of every 61st value of each
+  const void* green_host[6] = {};
+  uint64_t green_sig[6] = {};
   uint64_t tab_hash = 0;
   bool tab_dense = false;
   double2* Gd64 = nullptr;  // folded d T / d x, d y, d z (force), 3 * F2*F3*F1*6
@@ -280,6 +297,9 @@ struct ifdda_ctx {
 namespace ifdda {
 // Release the Krylov workspace (ifdda_krylov.hip).
 void solver_free(ifdda_ctx* c);
+// The context's grow-only full-grid Green scratch of M complex values (null
+// on a HIP failure); a build synchronises before it returns.
+double2* green_full(ifdda_ctx* c, size_t M);
 
 // The operator A = I - T alpha of a solve over R right-hand sides whose
 // vectors lie n apart: one context (all RHS share box, Green tensor, dipoles

@@ -2257,10 +2257,10 @@ int solve_core(const MvOp& op, Solver* S, const ifdda_solve_opts* o, int R, cons
     // returns, the squared initial residual norm (pimzbicgstab.f:50-53)
     if (o->method == IFDDA_BICGSTAB && o->precision == 64 && S->st_host[r].norm > 0.0)
       inf.tol1 = e[r] / S->st_host[r].norm;
-    if (run[r].status < 0) {  // inverserig.f:47-52
+    if (run[r].status < 0) {  // inverserig.f:47-52; the TFQMR branch has no '!' (:614)
       inf.nstop = 1;
       inf.ncompte = ncompte[r];
-      set_info_str(&inf, "Problem to solve Ax=b!");
+      set_info_str(&inf, o->method == IFDDA_TFQMR ? "Problem to solve Ax=b" : "Problem to solve Ax=b!");
     }
   }
   return IFDDA_OK;

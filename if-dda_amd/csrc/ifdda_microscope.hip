@@ -334,7 +334,25 @@ static int microscope_run(ifdda_ctx* c, const ifdda_solve_opts* o, const ifdda_m
   const double k0 = m->k0, d = m->d, nak = m->numaper * k0;
   std::vector<Incidence> inc;
   double dkinc = 0.0;
-  std::string msg = incidence_list(c, *m, &inc, &dkinc, dark);
+  std::string msg;
+  const bool user = !dark && m->ninc_user > 0;
+  if (user) {  // given plane waves inside the light cone
+    if (!m->kxy_user) {
+      set_error("ifdda_microsbf_device: ninc_user > 0 needs kxy_user");
+      return IFDDA_ERR_ARG;
+    }
+    for (int i = 0; i < m->ninc_user; ++i) {
+      const double kx = m->kxy_user[3 * i], ky = m->kxy_user[3 * i + 1], pp = m->kxy_user[3 * i + 2];
+      if (!(kx * kx + ky * ky < k0 * k0) || !(pp >= 0.0 && pp <= 1.0)) {
+        set_error("ifdda_microsbf_device: incidence " + std::to_string(i) +
+                  " outside the light cone or pp outside [0, 1]");
+        return IFDDA_ERR_ARG;
+      }
+      inc.push_back({0, 0, kx, ky, pp});
+    }
+  } else {
+    msg = incidence_list(c, *m, &inc, &dkinc, dark);
+  }
   int imax = 0;
   double dk = 0.0;
   if (msg.empty())
@@ -445,7 +463,7 @@ static int microscope_run(ifdda_ctx* c, const ifdda_solve_opts* o, const ifdda_m
       // incident bin: deltakroutine(kxinc, kyinc, deltak, deltak, k0) with the
       // condenser step (microsbf.f:689-690), or with the pupil step
       // (microsdf.f:668-669)
-      const double dkb = dark ? dk : dkinc;
+      const double dkb = dark || user ? dk : dkinc;
       int ik = (int)std::lround(q.kx / dkb), jk = (int)std::lround(q.ky / dkb);
       if ((dkb * ik) * (dkb * ik) + (dkb * jk) * (dkb * jk) > k0 * k0)
         ik = (int)(q.kx / dkb), jk = (int)(q.ky / dkb);

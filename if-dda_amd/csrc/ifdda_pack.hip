@@ -89,6 +89,7 @@ struct PackWs {
   hipEvent_t ev_aux[kAux] = {}, ev_main = nullptr;
   char* afull[kAux] = {};
   size_t afull_cap[kAux] = {};
+  GreenScratch gs[kAux];  // near-cell quadrature buffers per stream
 };
 
 namespace {
@@ -302,6 +303,7 @@ void pack_ws_delete(PackWs* w) {
   for (int k = 0; k < PackWs::kAux; ++k) {
     if (w->aux[k]) (void)hipStreamSynchronize(w->aux[k]);
     (void)hipFree(w->afull[k]);
+    green_scratch_free(w->gs[k]);
     if (w->aux[k]) (void)hipStreamDestroy(w->aux[k]);
     if (w->ev_aux[k]) (void)hipEventDestroy(w->ev_aux[k]);
   }
@@ -477,7 +479,8 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
       const double2* tw[3] = {jd[j].tab3[0]->tw64, jd[j].tab3[1]->tw64, jd[j].tab3[2]->tw64};
       const int* pos[3] = {jd[j].tab3[0]->pos, jd[j].tab3[1]->pos, jd[j].tab3[2]->pos};
       if (!green_build_device(g, tw, pos, keys[j].d, keys[j].k0, keys[j].nquad,
-                              reinterpret_cast<double2*>(w.afull[k]), e.G64, e.G32, w.aux[k])) {
+                              reinterpret_cast<double2*>(w.afull[k]), e.G64, e.G32, &w.gs[k],
+                              w.aux[k])) {
         green_free(e);
         return IFDDA_ERR_HIP;
       }

@@ -54,7 +54,7 @@ class MicroscopeOpts(C.Structure):
                 ("gross", C.c_double), ("zlens", C.c_double), ("nside", C.c_int),
                 ("npolainc", C.c_int), ("E0", C.c_double * 2), ("k0", C.c_double),
                 ("d", C.c_double), ("origin", C.c_double * 3), ("max_batch", C.c_int),
-                ("direct_lens", C.c_int)]
+                ("direct_lens", C.c_int), ("ninc_user", C.c_int), ("kxy_user", C.c_void_p)]
 
 
 class Job(C.Structure):
@@ -424,10 +424,18 @@ class Solver:
     # -- brightfield microscope (microsbf.f, quick lens) ---------------------
     @staticmethod
     def microscope_opts(nfft2d, numaper, numaperinc, k0, d, origin, gross=100.0, zlens=0.0,
-                        nside=1, npolainc=0, E0=1.0, max_batch=0, direct_lens=False):
+                        nside=1, npolainc=0, E0=1.0, max_batch=0, direct_lens=False,
+                        incidences=None):
         """direct_lens: the nquicklens = 0 path (pupil field as a direct sum
-        over dipoles, with its own pupil and condenser steps)."""
+        over dipoles, with its own pupil and condenser steps).  incidences:
+        [(kx, ky, pp)] plane waves to illuminate with instead of the
+        condenser lattice (brightfield)."""
         m = MicroscopeOpts()
+        if incidences is not None:
+            kxy = np.ascontiguousarray(incidences, np.float64).reshape(-1, 3)
+            m._kxy = kxy  # kept alive with the options
+            m.ninc_user = kxy.shape[0]
+            m.kxy_user = kxy.ctypes.data
         m.nfft2d, m.numaper, m.numaperinc, m.gross, m.zlens = int(nfft2d), numaper, numaperinc, gross, zlens
         m.nside, m.npolainc, m.k0, m.d, m.max_batch = int(nside), int(npolainc), k0, d, int(max_batch)
         m.direct_lens = int(bool(direct_lens))
@@ -455,8 +463,9 @@ class Solver:
         microscope.  Returns (images [6, N, N]: scattered x, y, z, total x,
         y, z; infos)."""
         o = self._opts(method, tol, precision, x0_zero, maxit, check_every)
-        ninc, _ = self.microscope_incidences(m, dark)
-        infos = (SolveInfo * max(1, ninc.shape[0]))()
+        count = m.ninc_user if (m.ninc_user > 0 and not dark) else \
+            self.microscope_incidences(m, dark)[0].shape[0]
+        infos = (SolveInfo * max(1, count))()
         N = m.nfft2d
         img = np.zeros(6 * N * N)
         n = C.c_int(0)

@@ -431,6 +431,12 @@ typedef struct {
                          of microsbf.f:139-157 and the pupil field as the
                          direct sum over dipoles of microsbf.f:629-668
                          (microssansinc.f:600-640 for nside = -1)          */
+  int ninc_user;      /* > 0 (brightfield only): illuminate with these plane
+                         waves instead of the condenser lattice of
+                         microsbf.f:90-172 -- kxy_user holds (kx, ky, pp) per
+                         incidence (host); the direct wave goes to its
+                         nearest pupil bin                                   */
+  const double *kxy_user;
 } ifdda_microscope_opts;
 
 /* The incidences microsbf.f:90-172 takes (polarisation outer, idelta,

@@ -196,3 +196,30 @@ def test_direct_lens_matches_cpu_pipeline(small, oracle, dark, nside):
         scale = np.abs(ref[c]).max()
         assert scale > 0
         assert np.abs(img[c] - ref[c]).max() < 1e-7 * scale, (c, np.abs(img[c] - ref[c]).max() / scale)
+
+
+def test_given_incidences_equal_the_condenser_lattice(small):
+    """The condenser's own incidences passed as an explicit list give the
+    same scattered images (same solves, pupil fields and sums); only the
+    direct wave's pupil bin follows the given-incidence rule (nearest pupil
+    bin), so the total images are compared only where no incident wave
+    enters: an off-lattice list with the direct waves outside the objective
+    NA equals the scattered image."""
+    s, alpha, sv, m = small
+    kxy, _ = sv.microscope_incidences(m)
+    img, infos = sv.microsbf(m, tol=1e-8)
+    mu = sv.microscope_opts(128, 0.8, 0.5, K0, s.d, s.origin, gross=100.0, zlens=50e-9, nside=1,
+                            npolainc=0, E0=1.0 + 0.5j, incidences=kxy)
+    imu, infu = sv.microsbf(mu, tol=1e-8)
+    assert len(infu) == len(infos) == kxy.shape[0]
+    assert np.array_equal(img[:3], imu[:3])
+    # oblique waves (sin theta = 0.87) far outside an NA 0.3 objective: even
+    # rounded to the pupil grid (step 0.25 k0 here) no direct wave enters
+    th = np.radians(60.0)
+    far = np.array([[K0 * np.sin(th) * np.cos(p), K0 * np.sin(th) * np.sin(p), pp]
+                    for p in (0.3, 2.0) for pp in (0.0, 1.0)])
+    mf = sv.microscope_opts(128, 0.3, 0.5, K0, s.d, s.origin, gross=100.0, zlens=50e-9, nside=1,
+                            npolainc=0, E0=1.0, incidences=far)
+    imf, inff = sv.microsbf(mf, tol=1e-8)
+    assert len(inff) == 4 and all(i["nstop"] == 0 for i in inff)
+    assert np.allclose(imf[3:], imf[:3], rtol=1e-12, atol=0)

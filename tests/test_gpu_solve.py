@@ -254,3 +254,90 @@ def test_iteration_cap(cfg1):
     # ITNO runs 0..maxit+1 (gpbicg.f:327), then the true residual is reported
     assert r["nstop"] == 0 and r["nloop"] == 3 and r["steperr"] == 0
     assert r["tol1"] > 1e-12
+
+
+def test_tfqmr_breakdown_message(cfg1):
+    """E_inc = 0: the TFQMR branch of inverserig.f reports 'Problem to solve
+    Ax=b' without the '!' of the other branches (inverserig.f:614)."""
+    s, k0, alpha, E, sv = cfg1
+    r = sv.inverserig(np.zeros_like(E), "TFQMR", 1e-4)
+    assert r["nstop"] == 1 and r["status"] == -1
+    assert r["infostr"] == "Problem to solve Ax=b"
+
+
+def test_dropin_argument_failure_is_not_a_breakdown(cfg1):
+    """A dipole count that disagrees with Tabdip is an argument error: nstop
+    = 1 with the library's own message, not the solver-breakdown text."""
+    import ctypes as C
+    import os
+    s, k0, alpha, E, sv = cfg1
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    L = C.CDLL(os.path.join(root, "if-dda_amd", "libifdda_dropin.so"))
+    nx, ny, nz = s.nx, s.ny, s.nz
+    M = 8 * nx * ny * nz
+    Gs = [np.zeros(M, np.complex128) for _ in range(6)]
+    ri = lambda v: C.byref(C.c_int(int(v)))  # noqa: E731
+    P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    L.greencalculfft_(ri(nx), ri(ny), ri(nz), ri(2 * nx), ri(2 * ny), ri(2 * nz), ri(4 * nx * ny),
+                      ri(nx), ri(ny), ri(nz), C.byref(C.c_double(s.d)), C.byref(C.c_double(k0)),
+                      *[P(g) for g in Gs], C.byref(C.c_int64(0)))
+    pol = np.zeros((nx * ny * nz, 3, 3), np.complex128)
+    for a in range(3):
+        pol[: s.nd, a, a] = alpha
+    pol = np.asfortranarray(pol)
+    E1 = np.concatenate([E, np.ones(3, complex)])  # nbsphere = nd + 1
+    r = _dropin_inverserig(L, Gs, s, pol, E1, "GPBICG1", 1e-4)
+    assert r["nstop"] == 1
+    assert r["infostr"].startswith("GPU: inverserig: nbsphere differs"), r["infostr"]
+
+
+def _dropin_matvec(L, Gs, s, FFX, FFB):
+    import ctypes as C
+    nx, ny, nz = s.nx, s.ny, s.nz
+    M = 8 * nx * ny * nz
+    ri = lambda v: C.byref(C.c_int(int(v)))  # noqa: E731
+    P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    V = [np.zeros(M, np.complex128) for _ in range(3)]
+    tab = np.ascontiguousarray(s.tabdip, np.int32)
+    out = np.ascontiguousarray(FFB, np.complex128).copy()
+    L.produitfftmatvect3_(*[P(g) for g in Gs], *[P(v) for v in V], P(tab), ri(M), ri(M), ri(s.nd),
+                          ri(nx * ny * nz), ri(nx), ri(ny), ri(nz), ri(nx), ri(ny), ri(nz),
+                          ri(2 * nx), ri(2 * ny), ri(4 * nx * ny), ri(2 * nz),
+                          P(np.ascontiguousarray(FFX, np.complex128)), P(out),
+                          C.byref(C.c_int64(0)), C.byref(C.c_int64(0)))
+    return out
+
+
+def test_dropin_sees_a_green_tensor_rewritten_in_place(cfg1):
+    """The caller recomputes FFTTENSOR* in the same arrays for another
+    wavelength (as cdmlib.f:3682-3730 does between solves): the next matvec
+    uses the new tensor, equal to a matvec on fresh arrays holding it."""
+    import ctypes as C
+    import os
+    s, k0, alpha, E, sv = cfg1
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    L = C.CDLL(os.path.join(root, "if-dda_amd", "libifdda_dropin.so"))
+    nx, ny, nz = s.nx, s.ny, s.nz
+    M = 8 * nx * ny * nz
+    ri = lambda v: C.byref(C.c_int(int(v)))  # noqa: E731
+    P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+
+    def green(Gs, kk):
+        L.greencalculfft_(ri(nx), ri(ny), ri(nz), ri(2 * nx), ri(2 * ny), ri(2 * nz),
+                          ri(4 * nx * ny), ri(nx), ri(ny), ri(nz), C.byref(C.c_double(s.d)),
+                          C.byref(C.c_double(kk)), *[P(g) for g in Gs], C.byref(C.c_int64(0)))
+
+    rng = np.random.default_rng(31)
+    x = rng.standard_normal(3 * s.nd) + 1j * rng.standard_normal(3 * s.nd)
+    Ga = [np.zeros(M, np.complex128) for _ in range(6)]
+    green(Ga, k0)
+    y1 = _dropin_matvec(L, Ga, s, x, x)
+    Gb = [np.zeros(M, np.complex128) for _ in range(6)]
+    green(Gb, 1.1 * k0)
+    for a, b in zip(Ga, Gb):
+        a[:] = b  # the same host arrays, rewritten outside the library
+    y2 = _dropin_matvec(L, Ga, s, x, x)
+    Gc = [g.copy() for g in Gb]
+    y3 = _dropin_matvec(L, Gc, s, x, x)
+    assert relerr(y1, y3) > 1e-3  # the two wavelengths differ
+    assert np.array_equal(y2, y3)
