@@ -67,9 +67,88 @@ bool same_opts(const ifdda_solve_opts& a, const ifdda_solve_opts& b) {
 // factor 2 in cells.  A pack runs in lockstep until its slowest member has
 // converged; a small job packed with a much larger one (more cells, most
 // often more iterations) would wait for it.
+// Jobs of at least this many box cells (an 80^3 box) run alone through a
+// context of their own (ifdda_solve_multi_device: the compile-time z pass
+// of their grid, the tuned tile widths, the fused GPBICG products) rather
+// than in a pack, whose launches are generic; they never share a pack, so a
+// large job's result is the same whatever else is queued.
+constexpr double kLargeCells = 512000.0;
+bool is_large(const ifdda_job& j) { return (double)j.nx * j.ny * j.nz >= kLargeCells; }
+
 bool packable(const ifdda_job& a, const ifdda_job& head) {
   const double na = (double)a.nx * a.ny * a.nz, nh = (double)head.nx * head.ny * head.nz;
-  return same_opts(a.opts, head.opts) && na <= 2.0 * nh && nh <= 2.0 * na;
+  return same_opts(a.opts, head.opts) && na <= 2.0 * nh && nh <= 2.0 * na && !is_large(a) &&
+         !is_large(head);
+}
+
+// A worker's context for large jobs: kept for the last box, with the Green
+// tensor of the last (d, k0, nquad) and grow-only device vectors.
+struct CtxSlot {
+  ifdda_ctx* c = nullptr;
+  int nx = 0, ny = 0, nz = 0;
+  bool green = false;
+  double d = 0.0, k0 = 0.0;
+  int nquad = -1;
+  double2* buf = nullptr;  // E_inc, FF, FFloc
+  size_t cap = 0;          // complex values
+  void release() {
+    if (c) ifdda_destroy(c);
+    (void)hipFree(buf);
+    *this = CtxSlot{};
+  }
+};
+
+// One large job through its context: Green tensor, map, polarisability and
+// incident waves on the device, one multi-RHS solve, results to the job.
+int ctx_solve(CtxSlot& w, Job& j) {
+  const ifdda_job& s = j.spec;
+  if (!w.c || w.nx != s.nx || w.ny != s.ny || w.nz != s.nz) {
+    w.release();
+    int rc = ifdda_create(s.nx, s.ny, s.nz, &w.c);
+    if (rc) return rc;
+    w.nx = s.nx, w.ny = s.ny, w.nz = s.nz;
+  }
+  ifdda_ctx* c = w.c;
+  int rc = IFDDA_OK;
+  if (!w.green || w.d != s.d || w.k0 != s.k0 || w.nquad != s.nquad) {
+    w.green = false;
+    if ((rc = ifdda_green_build(c, s.d, s.k0, s.nquad, s.tolinit))) return rc;
+    w.green = true, w.d = s.d, w.k0 = s.k0, w.nquad = s.nquad;
+  }
+  if ((rc = ifdda_set_dipoles(c, s.tabdip, s.nd))) return rc;
+  rc = s.alpha ? ifdda_set_polarizability(c, s.alpha, s.nd, 1)
+               : ifdda_polarizability_device(c, s.eps, s.d, s.k0, s.eps0 > 0.0 ? s.eps0 : 1.0, s.pola);
+  if (rc) return rc;
+  const size_t n = 3 * (size_t)s.nd * s.nrhs;
+  if (3 * n > w.cap) {
+    (void)hipFree(w.buf);
+    w.buf = nullptr;
+    w.cap = 0;
+    if (hipMalloc(&w.buf, sizeof(double2) * 3 * n) != hipSuccess) {
+      ifdda::set_error("runner: hipMalloc of a large job's vectors failed");
+      return IFDDA_ERR_HIP;
+    }
+    w.cap = 3 * n;
+  }
+  double2 *E = w.buf, *FF = w.buf + n, *FL = w.buf + 2 * n;
+  if (s.E_inc) {
+    if (hipMemcpy(E, s.E_inc, sizeof(double2) * n, hipMemcpyHostToDevice) != hipSuccess) {
+      ifdda::set_error("runner: copy of a large job's incident field failed");
+      return IFDDA_ERR_HIP;
+    }
+  } else if ((rc = ifdda_incident_device(c, s.wave_kind, s.nrhs, s.waves, s.E0, s.k0, s.d, s.origin,
+                                         E, nullptr))) {
+    return rc;
+  }
+  volatile int stop = 0;
+  if ((rc = ifdda_solve_multi_device(c, &s.opts, s.nrhs, E, FF, FL, j.infos.data(), &stop, nullptr)))
+    return rc;
+  if (hipMemcpy(j.FFloc.data(), FL, sizeof(double2) * n, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(j.FF.data(), FF, sizeof(double2) * n, hipMemcpyDeviceToHost) != hipSuccess) {
+    ifdda::set_error("runner: copy of a large job's results failed");
+    return IFDDA_ERR_HIP;
+  }
+  return IFDDA_OK;
 }
 
 }  // namespace
@@ -95,6 +174,7 @@ namespace {
 void worker(ifdda_runner* r) {
   (void)hipSetDevice(r->device);
   ifdda::PackWs* ws = ifdda::pack_ws_new();
+  CtxSlot cs;
   // Green tensors kept between packs: a share of the budget per worker
   if (ws)
     ifdda::pack_ws_green_limit(
@@ -115,7 +195,7 @@ void worker(ifdda_runner* r) {
       r->queue.pop_front();
       group.push_back(head);
       r->used += head->est;
-      int members = head->spec.nrhs;
+      int members = is_large(head->spec) ? r->max_members : head->spec.nrhs;
       for (auto it = r->queue.begin(); it != r->queue.end() && members < r->max_members;) {
         Job* j = &r->jobs.at(*it);
         if (packable(j->spec, head->spec) && members + j->spec.nrhs <= r->max_members &&
@@ -142,7 +222,10 @@ void worker(ifdda_runner* r) {
     }
     int rc = IFDDA_ERR_HIP;
     std::string err = "no pack workspace";
-    if (ws) {
+    if (is_large(group[0]->spec)) {
+      rc = ctx_solve(cs, *group[0]);
+      err = rc == IFDDA_OK ? "" : ifdda_last_error();
+    } else if (ws) {
       volatile int stop = 0;
       rc = ifdda::pack_solve(*ws, in, &group[0]->spec.opts, &stop);
       err = rc == IFDDA_OK ? "" : ifdda_last_error();
@@ -162,6 +245,7 @@ void worker(ifdda_runner* r) {
     r->cv_done.notify_all();
     r->cv_work.notify_all();
   }
+  cs.release();
   ifdda::pack_ws_delete(ws);
 }
 
