@@ -329,7 +329,7 @@ extern "C" int ifdda_use_specialized(ifdda_ctx* c, int on) {
 }
 
 extern "C" int ifdda_fuse_krylov(ifdda_ctx* c, int mask) {
-  if (!c || mask < 0 || mask > 3) return arg_error("ifdda_fuse_krylov: bad argument");
+  if (!c || mask < 0 || mask > 7) return arg_error("ifdda_fuse_krylov: bad argument");
   c->fuse_mask = mask;
   return IFDDA_OK;
 }

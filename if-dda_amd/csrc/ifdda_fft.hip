@@ -741,6 +741,22 @@ __device__ __forceinline__ void ct_dif(V* buf, const V* tws, int line, int jt) {
   }
 }
 
+// Tile lg (over gridDim.x * gridDim.y) and right-hand side rhs (over
+// gridDim.z) of this block.  The right-hand sides are taken in groups of RG
+// (8, 4, 2 or 1, dividing gridDim.z) and the blocks are ordered (group,
+// tile, rhs in group), in XCD order: the RG blocks of one tile run
+// back to back on one XCD and read that tile's Green columns from its L2,
+// and neighbouring tiles of one right-hand side stay RG blocks apart.  With
+// one right-hand side this is xcd_remap over the tiles.
+__device__ __forceinline__ void rhs_group_tile(int& lg, int& rhs) {
+  const int T = gridDim.x * gridDim.y, R = gridDim.z;
+  const int g = xcd_remap(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), T * R);
+  const int RG = R % 8 == 0 ? 8 : R % 4 == 0 ? 4 : R % 2 == 0 ? 2 : 1;
+  const int q = g / RG;
+  lg = q % T;
+  rhs = (q / T) * RG + (g - q * RG);
+}
+
 // Green items per thread of k_zconv_ct<V, L, B>.
 constexpr int zct_ng(int L, int B) {
   return ((L / 2 + 1) * B + (256 / (3 * B)) * 3 * B - 1) / ((256 / (3 * B)) * 3 * B);
@@ -755,18 +771,22 @@ __global__ void __launch_bounds__(256) k_zconv_ct(SArgs<V> a) {
   constexpr PlanC P = plan_c(L);
   static_assert(P.nst > 0 && NG >= 1 && NG <= kMaxG, "unsupported compile-time z plan");
   constexpr int FILL = P.rad[0] % 2 == 0 ? L / 2 : L;
-  if (a.skip && a.skip[blockIdx.z * a.skip_stride]) return;
-  const V* const zsrc = a.src + blockIdx.z * a.zs_src;
-  V* const zdst = a.dst + blockIdx.z * a.zs_dst;
+  int lg, rhs;
+  rhs_group_tile(lg, rhs);
+  if (a.skip && a.skip[rhs * a.skip_stride]) return;
+  const V* const zsrc = a.src + rhs * a.zs_src;
+  V* const zdst = a.dst + rhs * a.zs_dst;
   extern __shared__ double4 smem_d4[];
   V* tws = reinterpret_cast<V*>(smem_d4);
   int* pos = reinterpret_cast<int*>(tws + L);
   V* buf = tile_base(tws, L);
   const int tid = threadIdx.x;
   const int line = tid % NL, jt = tid / NL;
-  const int lg = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
   const int kyf = lg / gridDim.x;  // 0 .. L2/2
-  const int kx0 = (lg - kyf * gridDim.x) * B;
+  // tiles of a row in the order 0, gx-1, 1, gx-2, ...: kx and its mirror
+  // L1 - kx read the same folded Green column and now run side by side
+  const int u = lg - kyf * gridDim.x;
+  const int kx0 = ((u & 1) ? (int)gridDim.x - 1 - (u >> 1) : (u >> 1)) * B;
   const int nvalid = min(B, a.nbx - kx0) * 3;
 
   load_tables(tws, pos, a.tws, a.pos, L, tid, NTHR);
@@ -921,6 +941,13 @@ struct DivStep {
 template <typename V> __device__ __forceinline__ double2 to_d(V v) {
   return make_double2((double)v.x, (double)v.y);
 }
+// An FP64 complex value rounded to V (as the Krylov kernels store it).
+template <typename V> __device__ __forceinline__ V from_d(double2 v) {
+  V o;
+  o.x = (decltype(o.x))v.x;
+  o.y = (decltype(o.y))v.y;
+  return o;
+}
 
 // Forward (SGN > 0): DIT from a permuted load, first stage pruned when
 // 2 n_in <= L.  Inverse (SGN < 0): DIF from a natural load, last stage pruned
@@ -928,10 +955,12 @@ template <typename V> __device__ __forceinline__ double2 to_d(V v) {
 //
 // xpass_tile: the tile of box lines g0 .. g0 + B - 1, nthr threads (PK: as
 // strided_tile).
-// MODE 1: P1 (scatter alpha x, FFT-x).  MODE 2 / 5 / 6: P5 (IFFT-x, gather
-// y = b - v/M); 5 and 6 also form the inner products of y (KFuse p5 = 1, 2).
-// MODE 0: plain x-FFT of full lines.  f and zo (the RHS's element offset in
-// the Krylov vectors) are used by modes 5 and 6 only.
+// MODE 1: P1 (scatter alpha x, FFT-x); MODE 3: P1 that first forms
+// x = p = r + beta (p - u) and stores it over p (KFuse p1 = 1, gpbicg.f:161-169).
+// MODE 2 / 5 / 6: P5 (IFFT-x, gather y = b - v/M); 5 and 6 also form the
+// inner products of y (KFuse p5 = 1, 2).  MODE 0: plain x-FFT of full lines.
+// f and zo (the RHS's element offset in the Krylov vectors) are used by
+// modes 3, 5 and 6 only.
 template <typename V, int NC, int SGN, int MODE, bool PK>
 __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* zdst,
                                            const V* zxin, V* zxout, int g0, int tid0, int nthr,
@@ -955,7 +984,9 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
   load_tables(tw, pos, a.tw, a.pos, L, tid, nthr);
   lds_sync();
   {
-    constexpr int U = MODE == 1 ? kBatch / 2 : kBatch / NC;
+    constexpr int U = MODE == 1 || MODE == 3 ? kBatch / 2 : kBatch / NC;
+    double2 beta = make_double2(0.0, 0.0);
+    if constexpr (MODE == 3) beta = f->st[blockIdx.z].beta;
     DivStep ix(tid, nthr, fill);  // (tile line l, point i)
     for (int e0 = tid; e0 < a.B * fill; e0 += U * nthr) {
       int ll[U], ii[U];
@@ -966,23 +997,53 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
         ix.next();
       }
       V x[U][NC];
-      if constexpr (MODE == 1) {
+      if constexpr (MODE == 1 || MODE == 3) {
         long long dip[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           dip[u] = -1;
           if (ll[u] < nl && ii[u] < a.n_in) dip[u] = dip_of(a.tab, a.nx, g0 + ll[u], ii[u]);
         }
+        if constexpr (MODE == 3) {
+          // every load of the batch before the stores over p
+          V rv[U][3], uv[U][3];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
+          for (int u = 0; u < U; ++u) {
 #pragma unroll
-          for (int c = 0; c < NC; ++c) x[u][c] = czero<V>();
-          if (dip[u] >= 0) {
-            const V* xp = zxin + dip[u] * 3;
-            x[u][0] = xp[0];
-            x[u][1] = xp[1];
-            x[u][2] = xp[2];
-            alpha_apply(a.alpha, a.alpha_mode, dip[u], x[u][0], x[u][1], x[u][2]);
+            for (int c = 0; c < NC; ++c) x[u][c] = czero<V>();
+            if (dip[u] >= 0) {
+              const long long e = zo + dip[u] * 3;
+#pragma unroll
+              for (int c = 0; c < 3; ++c) {
+                x[u][c] = zxin[dip[u] * 3 + c];
+                rv[u][c] = f->r[e + c];
+                uv[u][c] = f->u[e + c];
+              }
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+            if (dip[u] >= 0) {
+              const long long e = zo + dip[u] * 3;
+#pragma unroll
+              for (int c = 0; c < 3; ++c) {
+                x[u][c] = from_d<V>(zadd(to_d(rv[u][c]), zmul(beta, zsub(to_d(x[u][c]), to_d(uv[u][c])))));
+                f->p[e + c] = x[u][c];
+              }
+              alpha_apply(a.alpha, a.alpha_mode, dip[u], x[u][0], x[u][1], x[u][2]);
+            }
+        } else {
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) x[u][c] = czero<V>();
+            if (dip[u] >= 0) {
+              const V* xp = zxin + dip[u] * 3;
+              x[u][0] = xp[0];
+              x[u][1] = xp[1];
+              x[u][2] = xp[2];
+              alpha_apply(a.alpha, a.alpha_mode, dip[u], x[u][0], x[u][1], x[u][2]);
+            }
           }
         }
       } else {
@@ -1032,6 +1093,7 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
         if (ll[u] < nl) dip[u] = dip_of(a.tab, a.nx, g0 + ll[u], ii[u]);
       }
       V b[U][3];
+      V q[kND > 0 ? U : 1][3];  // r0* (MODE 5) or y (MODE 6), loaded before the stores of y
 #pragma unroll
       for (int u = 0; u < U; ++u)
         if (dip[u] >= 0) {
@@ -1039,6 +1101,11 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
           b[u][0] = bp[0];
           b[u][1] = bp[1];
           b[u][2] = bp[2];
+          if constexpr (kND > 0) {
+            const V* qp = (MODE == 5 ? f->r0s : f->y) + zo + dip[u] * 3;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) q[u][c] = qp[c];
+          }
         }
 #pragma unroll
       for (int u = 0; u < U; ++u)
@@ -1052,14 +1119,12 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
             y[c] = yo[c];
           }
           if constexpr (MODE == 5) {  // <r0*, Ap> (gpbicg.f:174-181)
-            const long long e = zo + dip[u] * 3;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) acc[0] = zadd(acc[0], zmul(to_d(f->r0s[e + c]), to_d(yo[c])));
+            for (int c = 0; c < 3; ++c) acc[0] = zadd(acc[0], zmul(to_d(q[u][c]), to_d(yo[c])));
           } else if constexpr (MODE == 6) {  // gpbicg.f:222-253
-            const long long e = zo + dip[u] * 3;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-              const double2 At = to_d(yo[c]), yv = to_d(f->y[e + c]), t = to_d(b[u][c]);
+              const double2 At = to_d(yo[c]), yv = to_d(q[u][c]), t = to_d(b[u][c]);
               const double2 cAt = zconj(At), cy = zconj(yv);
               acc[0] = zadd(acc[0], zmul(cAt, At));
               acc[1] = zadd(acc[1], zmul(cy, yv));
@@ -1116,8 +1181,9 @@ __global__ void __launch_bounds__(256) k_xpass(XArgs<V> a) {
       threadIdx.x, blockDim.x);
 }
 
-// P5 with GPBICG inner products fused in (KFuse; one context, R right-hand
-// sides zs_x elements apart).
+// P1 with the GPBICG p update (MODE 3) or P5 with GPBICG inner products
+// (MODE 5, 6) fused in (KFuse; one context, R right-hand sides zs_x elements
+// apart).
 template <typename V, int NC, int SGN, int MODE>
 __global__ void __launch_bounds__(256) k_xpass_f(XArgs<V> a, KFuse<V> f) {
   if (a.skip && a.skip[blockIdx.z * a.skip_stride]) return;
@@ -1783,11 +1849,20 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.xin = x;
     a.zs_x = m.vstride;
     a.zs_dst = zA;
-    auto k = k_xpass<V, 3, +1, 1>;
-    if (!allow_lds(k, ps.t[0]->lds)) return false;
-    if (tm) tm->begin(pi * 5 + 0, s);
-    hipLaunchKernelGGL(k, grid(0), dim3(ps.t[0]->threads), ps.t[0]->lds, s, a);
-    if (tm) tm->end(s);
+    if (f && f->p1) {
+      if (f->p != x || !f->r || !f->u || !f->st) return false;
+      auto k = k_xpass_f<V, 3, +1, 3>;
+      if (!allow_lds(k, ps.t[0]->lds)) return false;
+      if (tm) tm->begin(pi * 5 + 0, s);
+      hipLaunchKernelGGL(k, grid(0), dim3(ps.t[0]->threads), ps.t[0]->lds, s, a, *f);
+      if (tm) tm->end(s);
+    } else {
+      auto k = k_xpass<V, 3, +1, 1>;
+      if (!allow_lds(k, ps.t[0]->lds)) return false;
+      if (tm) tm->begin(pi * 5 + 0, s);
+      hipLaunchKernelGGL(k, grid(0), dim3(ps.t[0]->threads), ps.t[0]->lds, s, a);
+      if (tm) tm->end(s);
+    }
   }
   {  // P2: FFT-y over the nz non-zero planes
     SArgs<V> a = ps.p2;

@@ -139,6 +139,13 @@ template <typename V> struct KFuse {
   const V* y = nullptr;    // p5 = 2
   double2* part = nullptr;
   int pstride = 0;
+  // p1 = 1: P1 first forms p = r + beta (p - u) (beta of each RHS's solver
+  // state), stores it over p (the matvec input x) and scatters alpha p
+  int p1 = 0;
+  V* p = nullptr;
+  const V* r = nullptr;
+  const V* u = nullptr;
+  const KState* st = nullptr;
 };
 // Blocks of the P5 pass of a launch over R right-hand sides (the partials a
 // fused P5 writes per inner product and RHS).
@@ -332,6 +339,7 @@ struct MvOp {
   // P5 blocks per RHS of such a launch, 0 when this operator does not fuse
   // at precision prec (the solver then runs the separate vector kernels).
   virtual int fuse_blocks(int prec, int R) const { (void)prec; (void)R; return 0; }
+  virtual bool fuse_p1() const { return false; }  // p update inside P1 as well
   virtual bool mvf64(const double2* x, const double2* b, double2* y, hipStream_t s,
                      const int* skip, int R, long long n, int skip_stride,
                      const KFuse<double2>& f) const {

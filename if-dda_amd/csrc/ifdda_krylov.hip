@@ -1625,6 +1625,7 @@ struct CtxOp : MvOp {
     if (!c->specialized || !((c->fuse_mask >> (prec == 64 ? 0 : 1)) & 1)) return 0;
     return matvec_p5_blocks(c->g, prec, R);
   }
+  bool fuse_p1() const override { return (c->fuse_mask >> 2) & 1; }
   bool mvf64(const double2* x, const double2* b, double2* y, hipStream_t s, const int* skip, int R,
              long long n, int skip_stride, const KFuse<double2>& f) const override {
     MatvecDev<double2> m = c->mv64();
@@ -1815,18 +1816,26 @@ static bool krylov(const MvOp& op, Solver* S, V** vv, int R, int method, const d
     hipLaunchKernelGGL(g_start<V>, G, T, 0, s, v, x0zero, vn, S->part, nb);
     hipLaunchKernelGGL(g_fstart, F, T, 0, s, S->st, (const double2*)S->part, nb);
     // Fused matvec ends (one context): <r0*, Ap> inside the first matvec's
-    // last pass, the five products of step 4 inside the second's.  The
-    // vectors are those of the separate kernels; only the products are
-    // summed in another order.
+    // last pass, the five products of step 4 inside the second's, and on
+    // request p = r + beta (p - u) inside the first matvec's scatter (it
+    // replaces g_k1 but reads r, u and p through the dipole map, slower than
+    // g_k1's stream on the 100^3 cube).  The vectors are those of the
+    // separate kernels; only the products are summed in another order.
     const int nb5 = S->lens ? 0 : op.fuse_blocks(std::is_same<V, double2>::value ? 64 : 32, R);
     const bool fuse = nb5 > 0 && S->part5 && S->part5_cap >= 5LL * R * nb5;
+    const bool p1f = fuse && op.fuse_p1();
     for (int it = 0; fuse; ++it) {
       v.told = tb[it & 1];
       v.t = tb[(it + 1) & 1];
       v.Ap = ab[it & 1];
       v.Apold = ab[(it + 1) & 1];
-      hipLaunchKernelGGL(g_k1<V>, G, T, 0, s, (const KState*)S->st, v, vn);
+      if (!p1f) hipLaunchKernelGGL(g_k1<V>, G, T, 0, s, (const KState*)S->st, v, vn);
       KFuse<V> f1;
+      f1.p1 = p1f ? 1 : 0;
+      f1.p = v.p;
+      f1.r = v.r;
+      f1.u = v.u;
+      f1.st = S->st;
       f1.p5 = 1;
       f1.r0s = v.r0s;
       f1.part = S->part5;

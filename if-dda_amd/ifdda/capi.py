@@ -507,7 +507,8 @@ class Solver:
         _check(lib().ifdda_use_specialized(self.h, int(bool(on))), "ifdda_use_specialized")
 
     def fuse_krylov(self, mask: int):
-        """GPBICG work fused into the matvec ends: bit 0 FP64, bit 1 FP32 (default 2)."""
+        """GPBICG work fused into the matvec ends: bit 0 FP64, bit 1 FP32 (default 2);
+        bit 2 also forms p = r + beta (p - u) inside the scatter pass."""
         _check(lib().ifdda_fuse_krylov(self.h, int(mask)), "ifdda_fuse_krylov")
 
     def specialized_z(self, prec: int = 64) -> bool:

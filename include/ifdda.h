@@ -187,7 +187,11 @@ int ifdda_use_specialized(ifdda_ctx *ctx, int on);
  * products of gpbicg.f:174-181 and :222-253 are summed by the pass that
  * gathers the matvec's result, as it writes it, instead of by separate
  * kernels that read the vectors again.  mask bit 0: FP64 solves, bit 1: FP32
- * (the inner solves of mixed precision; the default is 2).  The vectors are
+ * (the inner solves of mixed precision; the default is 2); bit 2 also moves
+ * the update p = r + beta (p - u) (gpbicg.f:161-169) into the matvec's
+ * scatter pass, in place of its own kernel (off by default: slower on the
+ * 100^3 cube, where the scatter reads r, u and p through the dipole map
+ * less efficiently than the separate kernel streams them).  The vectors are
  * the same; the products are summed in another fixed order, so fused and
  * unfused trajectories agree to rounding.  Single-context solves with the
  * specialized kernels on (ifdda_use_specialized) only; packs always run the
