@@ -123,6 +123,8 @@ def main():
     ap.add_argument("--uneven-count", type=int, default=512)
     ap.add_argument("--uneven-seq", type=int, default=32, help="sequential FP64 solves timed")
     ap.add_argument("--uneven-members", type=int, default=16, help="right-hand sides per pack")
+    ap.add_argument("--uneven-no-runner", action="store_true",
+                    help="uneven section: only the one-pack ifdda_solve_batch leg")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-rates", default="2,4,8,16,32", help="requests per second")
     ap.add_argument("--serving-requests", type=int, default=40, help="requests per rate")
@@ -427,6 +429,15 @@ def bench_uneven(args):
 
     probs = scene.config4(args.uneven_count, seed=4)
     print(f"uneven: {len(probs)} problems", file=sys.stderr, flush=True)
+    if args.uneven_no_runner:  # the one-pack leg alone, from this thread (profiling)
+        batch = [dict(nx=s.nx, ny=s.ny, nz=s.nz, d=s.d, k0=k0, alpha=al, E_inc=E, tabdip=s.tabdip)
+                 for (s, k0, al, E) in probs]
+        ifdda.solve_batch(batch[:8], "GPBICG1", 1e-4, precision=32)  # warm-up
+        t0 = time.perf_counter()
+        one = ifdda.solve_batch(batch, "GPBICG1", 1e-4, precision=32)
+        t_one = time.perf_counter() - t0
+        assert all(o["infos"][0]["nstop"] == 0 and o["infos"][0]["tol1"] <= 1e-4 for o in one)
+        return {"one_pack": {"seconds": t_one, "solves_per_s": len(probs) / t_one}}
     with ifdda.Runner(nworkers=args.workers, mem_budget=240e9, max_members=args.uneven_members) as run:
         warm = run.submit(probs[0][0].nx, probs[0][0].ny, probs[0][0].nz, probs[0][0].d, probs[0][1],
                           probs[0][2], probs[0][3], tabdip=probs[0][0].tabdip)
