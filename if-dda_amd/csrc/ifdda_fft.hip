@@ -657,6 +657,9 @@ constexpr PlanC plan_c(int L) {
   a.nst = r == 1 ? n : -1;
   return a;
 }
+// Rows a compile-time DIT pass loads: L/2 when its first (pruned) stage has
+// an even radix, else L.
+constexpr int ct_fill(int L) { return plan_c(L).rad[0] % 2 == 0 ? L / 2 : L; }
 constexpr int plan_m(const PlanC& p, int s) {
   int m = 1;
   for (int i = 0; i < s; ++i) m *= p.rad[i];
@@ -741,6 +744,30 @@ __device__ __forceinline__ void ct_dif(V* buf, const V* tws, int line, int jt) {
   }
 }
 
+// Rows p = jt, jt + JT, ... < FILL of one tile line at element stride sp into
+// registers (zero for p >= n_in or an invalid line), and from registers to
+// LDS at their permuted positions: load_strided in two halves, so that the
+// loads of one row can be in flight while another is transformed.
+template <int FILL, int JT, typename V>
+__device__ __forceinline__ void fetch_rows(V (&v)[(FILL + JT - 1) / JT], const V* src,
+                                           long long sp, int n_in, int line, int nvalid, int jt) {
+#pragma unroll
+  for (int u = 0; u < (FILL + JT - 1) / JT; ++u) {
+    const int p = jt + u * JT;
+    v[u] = czero<V>();
+    if (p < n_in && line < nvalid) v[u] = src[(long long)p * sp + line];
+  }
+}
+template <int FILL, int JT, int NL, typename V>
+__device__ __forceinline__ void put_rows(V* buf, const int* pos,
+                                         const V (&v)[(FILL + JT - 1) / JT], int line, int jt) {
+#pragma unroll
+  for (int u = 0; u < (FILL + JT - 1) / JT; ++u) {
+    const int p = jt + u * JT;
+    if (p < FILL) buf[pos[p] * NL + line] = v[u];
+  }
+}
+
 // Tile lg (over gridDim.x * gridDim.y) and right-hand side rhs (over
 // gridDim.z) of this block.  The right-hand sides are taken in groups of RG
 // (8, 4, 2 or 1, dividing gridDim.z) and the blocks are ordered (group,
@@ -789,14 +816,35 @@ __global__ void __launch_bounds__(256) k_zconv_ct(SArgs<V> a) {
   const int kx0 = ((u & 1) ? (int)gridDim.x - 1 - (u >> 1) : (u >> 1)) * B;
   const int nvalid = min(B, a.nbx - kx0) * 3;
 
+  // FP64 (latency-bound, 3 blocks per CU by LDS): each thread's rows of a
+  // tile line go through registers; the first row's loads are issued before
+  // the tables', the mirror row's while the first row is transformed (rows
+  // ky and L2 - ky are disjoint, so reading the second before the first is
+  // stored in place is safe).  200^3: 158.7 -> 155.2 us.  FP32 (VALU-bound)
+  // loads each row straight into LDS: the registers cost it a wave per SIMD
+  // and took it from 83.6 to 88.0 us.
+  constexpr bool PREF = sizeof(V) == 16;
+  constexpr int NR = PREF ? (FILL + JT - 1) / JT : 1;
+  [[maybe_unused]] V nxt[NR];
+  const int nrow = (kyf == 0 || 2 * kyf == a.L2) ? 1 : 2;
+  if constexpr (PREF)
+    fetch_rows<FILL, JT>(nxt, zsrc + (long long)kyf * a.so_src + (long long)kx0 * 3, a.sp_src,
+                         a.n_in, line, nvalid, jt);
   load_tables(tws, pos, a.tws, a.pos, L, tid, NTHR);
   lds_sync();
-  const int nrow = (kyf == 0 || 2 * kyf == a.L2) ? 1 : 2;
   for (int rr = 0; rr < nrow; ++rr) {
     const int ky = rr == 0 ? kyf : a.L2 - kyf;
     const bool sy = rr == 1;  // y odd-parity sign for the mirror row
-    load_strided<true>(buf, pos, zsrc + (long long)ky * a.so_src + (long long)kx0 * 3, a.sp_src,
-                       FILL, a.n_in, NL, line, nvalid, jt, JT);
+    if constexpr (PREF) {
+      put_rows<FILL, JT, NL>(buf, pos, nxt, line, jt);
+      lds_sync();
+      if (rr + 1 < nrow)
+        fetch_rows<FILL, JT>(nxt, zsrc + (long long)(a.L2 - kyf) * a.so_src + (long long)kx0 * 3,
+                             a.sp_src, a.n_in, line, nvalid, jt);
+    } else {
+      load_strided<true>(buf, pos, zsrc + (long long)ky * a.so_src + (long long)kx0 * 3, a.sp_src,
+                         FILL, a.n_in, NL, line, nvalid, jt, JT);
+    }
     ct_dit<L, NL, JT, +1, 0>(buf, tws, line, jt);
     V gr[NG][6];
     int gx[NG];  // bit0 valid, bit1 x odd-parity sign
@@ -869,6 +917,7 @@ template <typename V> struct XArgs {
   const V* src;
   V* dst;
   const V* tw;
+  const V* tws;  // stage-major twiddles (k_xpass_ct)
   const int* pos;
   AxisPlan ax;
   int n_in, n_out;
@@ -961,34 +1010,39 @@ template <typename V> __device__ __forceinline__ V from_d(double2 v) {
 // inner products of y (KFuse p5 = 1, 2).  MODE 0: plain x-FFT of full lines.
 // f and zo (the RHS's element offset in the Krylov vectors) are used by
 // modes 3, 5 and 6 only.
-template <typename V, int NC, int SGN, int MODE, bool PK>
+template <typename V, int NC, int SGN, int MODE, bool PK, int LC = 0, int BC = 0>
 __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* zdst,
                                            const V* zxin, V* zxout, int g0, int tid0, int nthr,
                                            const KFuse<V>* f = nullptr, long long zo = 0) {
+  // LC > 0: compile-time line length LC and tile width BC (k_xpass_ct), FFT
+  // stages of the z pass on the axis's stage-major twiddles; launched with
+  // (256 / (BC NC)) * BC NC threads, 2 n_in <= LC (P1) / 2 n_out <= LC (P5)
+  constexpr bool CT = LC > 0;
   constexpr bool kP5 = MODE == 2 || MODE == 5 || MODE == 6;
   constexpr int kND = MODE == 5 ? 1 : MODE == 6 ? 5 : 0;  // inner products formed
   constexpr bool DIT = SGN > 0;
   extern __shared__ double4 smem_d4[];
   V* tw = reinterpret_cast<V*>(smem_d4);
-  const int L = a.ax.L;
+  const int L = CT ? LC : a.ax.L;
+  const int BT = CT ? BC : a.B;
   int* pos = reinterpret_cast<int*>(tw + L);
   V* buf = tile_base(tw, L);
-  const int NL = a.B * NC;
+  const int NL = BT * NC;
   const int tid = PK && tid0 >= nthr ? kIdle : tid0;
   const int line = tid0 % NL, jt = tid / NL, JT = nthr / NL;
-  const int nl = min(a.B, a.nlines - g0);
+  const int nl = min(BT, a.nlines - g0);
   const bool half_in = DIT && 2 * a.n_in <= L;
   const bool half_out = !DIT && 2 * a.n_out <= L;
-  const int fill = DIT ? rows_to_fill(a.ax, half_in) : L;
+  const int fill = DIT ? (CT ? ct_fill(LC) : rows_to_fill(a.ax, half_in)) : L;
 
-  load_tables(tw, pos, a.tw, a.pos, L, tid, nthr);
+  load_tables(tw, pos, CT ? a.tws : a.tw, a.pos, L, tid, nthr);
   lds_sync();
   {
     constexpr int U = MODE == 1 || MODE == 3 ? kBatch / 2 : kBatch / NC;
     double2 beta = make_double2(0.0, 0.0);
     if constexpr (MODE == 3) beta = f->st[blockIdx.z].beta;
     DivStep ix(tid, nthr, fill);  // (tile line l, point i)
-    for (int e0 = tid; e0 < a.B * fill; e0 += U * nthr) {
+    for (int e0 = tid; e0 < BT * fill; e0 += U * nthr) {
       int ll[U], ii[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -1060,7 +1114,7 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        if (ll[u] < a.B) {
+        if (ll[u] < BT) {
           V* d = buf + (DIT ? pos[ii[u]] : ii[u]) * NL + ll[u] * NC;
 #pragma unroll
           for (int c = 0; c < NC; ++c) d[c] = x[u][c];
@@ -1070,10 +1124,17 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
   }
   lds_sync();
 
-  if (DIT)
+  if constexpr (CT) {
+    constexpr int NLC = BC * NC, JTC = 256 / NLC;
+    if constexpr (DIT)
+      ct_dit<LC, NLC, JTC, SGN, 0>(buf, tw, line, jt);
+    else
+      ct_dif<LC, NLC, JTC, SGN, plan_c(LC).nst - 1>(buf, tw, line, jt);
+  } else if (DIT) {
     tile_fft_dit<SGN>(buf, tw, a.ax, NL, line, jt, JT, half_in);
-  else
+  } else {
     tile_fft_dif<SGN>(buf, tw, a.ax, NL, line, jt, JT, half_out);
+  }
 
   if constexpr (kP5) {
     constexpr int U = kBatch / 2;
@@ -1081,7 +1142,7 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
 #pragma unroll
     for (int q = 0; q < (kND > 0 ? kND : 1); ++q) acc[q] = make_double2(0.0, 0.0);
     DivStep ix(tid, nthr, a.n_out);
-    for (int e0 = tid; e0 < a.B * a.n_out; e0 += U * nthr) {
+    for (int e0 = tid; e0 < BT * a.n_out; e0 += U * nthr) {
       int ll[U], ii[U];
       long long dip[U];
 #pragma unroll
@@ -1159,7 +1220,7 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
     }
   } else {
     DivStep ix(tid, nthr, a.n_out);
-    for (int e = tid; e < a.B * a.n_out; e += nthr, ix.next()) {
+    for (int e = tid; e < BT * a.n_out; e += nthr, ix.next()) {
       if (ix.l >= nl) continue;
       const V* v = buf + (DIT ? ix.i : pos[ix.i]) * NL + ix.l * NC;
       V* d = zdst + (long long)(g0 + ix.l) * a.ld_dst + (long long)ix.i * NC;
@@ -1193,6 +1254,42 @@ __global__ void __launch_bounds__(256) k_xpass_f(XArgs<V> a, KFuse<V> f) {
       a.xin ? a.xin + blockIdx.z * a.zs_x : nullptr,
       a.xout ? a.xout + blockIdx.z * a.zs_x : nullptr, xcd_remap(blockIdx.x, gridDim.x) * a.B,
       threadIdx.x, blockDim.x, &f, (long long)blockIdx.z * a.zs_x);
+}
+
+// x passes with a compile-time line length L and tile width B (P1: MODE 1;
+// P5: MODE 2, and 5 / 6 with the GPBICG products): the z pass's FFT stages,
+// twiddle offsets and trip counts as constants.  f is used by modes 5 and 6.
+template <typename V, int NC, int SGN, int MODE, int L, int B>
+__global__ void __launch_bounds__(256) k_xpass_ct(XArgs<V> a, KFuse<V> f) {
+  if (a.skip && a.skip[blockIdx.z * a.skip_stride]) return;
+  xpass_tile<V, NC, SGN, MODE, false, L, B>(
+      a, a.src ? a.src + blockIdx.z * a.zs_src : nullptr,
+      a.dst ? a.dst + blockIdx.z * a.zs_dst : nullptr,
+      a.xin ? a.xin + blockIdx.z * a.zs_x : nullptr,
+      a.xout ? a.xout + blockIdx.z * a.zs_x : nullptr, xcd_remap(blockIdx.x, gridDim.x) * B,
+      threadIdx.x, blockDim.x, &f, (long long)blockIdx.z * a.zs_x);
+}
+
+template <typename V> using XpassKernel = void (*)(XArgs<V>, KFuse<V>);
+template <typename V, int SGN, int MODE, int L>
+static XpassKernel<V> xpass_ct_width(int B) {
+  switch (B) {
+    case 2: return k_xpass_ct<V, 3, SGN, MODE, L, 2>;
+    case 3: return k_xpass_ct<V, 3, SGN, MODE, L, 3>;
+    case 4: return k_xpass_ct<V, 3, SGN, MODE, L, 4>;
+    case 5: return k_xpass_ct<V, 3, SGN, MODE, L, 5>;
+    case 8: if constexpr (L == 128) return k_xpass_ct<V, 3, SGN, MODE, L, 8>; else return nullptr;
+    default: return nullptr;
+  }
+}
+// The compile-time x passes built into the library: the x lengths of the
+// 128^3, 192^3 and 200^3 grids at widths 2 to 5 (and 8 at 128, the width of
+// batched launches); nullptr elsewhere.
+template <typename V, int SGN, int MODE> XpassKernel<V> xpass_ct_for(int L, int B) {
+  if (L == 200) return xpass_ct_width<V, SGN, MODE, 200>(B);
+  if (L == 128) return xpass_ct_width<V, SGN, MODE, 128>(B);
+  if (L == 192) return xpass_ct_width<V, SGN, MODE, 192>(B);
+  return nullptr;
 }
 
 // ---------------------------------------------------------------------------
@@ -1732,6 +1829,7 @@ static void pass_set(const GridPlan& g, const MatvecDev<V>& m, int pi, bool sing
     a = XArgs<V>{};
     a.dst = m.A;
     a.tw = m.tw[0];
+    a.tws = m.tws[0];
     a.pos = m.pos[0];
     a.ax = g.ax[0];
     a.n_in = nx;
@@ -1816,6 +1914,7 @@ static void pass_set(const GridPlan& g, const MatvecDev<V>& m, int pi, bool sing
     a = XArgs<V>{};
     a.src = m.A;
     a.tw = m.tw[0];
+    a.tws = m.tws[0];
     a.pos = m.pos[0];
     a.ax = g.ax[0];
     a.n_in = L1;
@@ -1842,6 +1941,11 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
   PassSet<V> ps;
   pass_set(g, m, pi, R == 1, ps);
   auto grid = [&](int p) { return dim3(ps.gx[p], ps.gy[p], R); };
+  // compile-time x passes (both need 2 nx <= L1, which every plan keeps), in
+  // FP64: they took config 2's P1 / P5 from 49.5 / 42.5 to 44.6 / 38.7 us,
+  // while in FP32 P1 stayed at 26.8 us and the fused P5 (more VGPRs than the
+  // runtime kernel, fewer waves) went from 33.0 to 36.5 us
+  const bool ct_x = pi == 0 && m.specialized && m.tws[0] && 2 * g.nx <= g.L[0];
   {  // P1: scatter(alpha x) + FFT-x
     XArgs<V> a = ps.p1;
     a.skip = skip;
@@ -1855,6 +1959,11 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
       if (!allow_lds(k, ps.t[0]->lds)) return false;
       if (tm) tm->begin(pi * 5 + 0, s);
       hipLaunchKernelGGL(k, grid(0), dim3(ps.t[0]->threads), ps.t[0]->lds, s, a, *f);
+      if (tm) tm->end(s);
+    } else if (XpassKernel<V> kc = ct_x ? xpass_ct_for<V, +1, 1>(g.L[0], ps.t[0]->B) : nullptr) {
+      if (!allow_lds(kc, ps.t[0]->lds)) return false;
+      if (tm) tm->begin(pi * 5 + 0, s);
+      hipLaunchKernelGGL(kc, grid(0), dim3(ps.t[0]->threads), ps.t[0]->lds, s, a, KFuse<V>{});
       if (tm) tm->end(s);
     } else {
       auto k = k_xpass<V, 3, +1, 1>;
@@ -1919,10 +2028,22 @@ static bool matvec_impl(const GridPlan& g, const MatvecDev<V>& m, const V* x, co
     a.xout = y;
     a.zs_x = m.vstride;
     a.zs_src = zA;
-    if (f && f->p5) {
-      // the block sum of the fused inner products needs a full first wavefront
-      if (ps.t[4]->threads < 64 || f->pstride < ps.gx[4]) return false;
-      auto k = f->p5 == 1 ? k_xpass_f<V, 3, -1, 5> : k_xpass_f<V, 3, -1, 6>;
+    const int mode5 = f && f->p5 ? (f->p5 == 1 ? 5 : 6) : 2;
+    XpassKernel<V> kc = nullptr;
+    if (ct_x)
+      kc = mode5 == 2   ? xpass_ct_for<V, -1, 2>(g.L[0], ps.t[4]->B)
+           : mode5 == 5 ? xpass_ct_for<V, -1, 5>(g.L[0], ps.t[4]->B)
+                        : xpass_ct_for<V, -1, 6>(g.L[0], ps.t[4]->B);
+    if (mode5 != 2 && (ps.t[4]->threads < 64 || f->pstride < ps.gx[4]))
+      return false;  // the block sum of the fused inner products needs a full first wavefront
+    if (kc) {
+      if (!allow_lds(kc, ps.t[4]->lds)) return false;
+      if (tm) tm->begin(pi * 5 + 4, s);
+      hipLaunchKernelGGL(kc, grid(4), dim3(ps.t[4]->threads), ps.t[4]->lds, s, a,
+                         mode5 == 2 ? KFuse<V>{} : *f);
+      if (tm) tm->end(s);
+    } else if (mode5 != 2) {
+      auto k = mode5 == 5 ? k_xpass_f<V, 3, -1, 5> : k_xpass_f<V, 3, -1, 6>;
       if (!allow_lds(k, ps.t[4]->lds)) return false;
       if (tm) tm->begin(pi * 5 + 4, s);
       hipLaunchKernelGGL(k, grid(4), dim3(ps.t[4]->threads), ps.t[4]->lds, s, a, *f);
