@@ -249,6 +249,33 @@ def main():
         torch.cuda.synchronize()
         lat64.append(time.perf_counter() - t1)
 
+    # speed-up at equal field error: the loosest mixed tolerance of a few at
+    # or above 1e-4 whose field is no further from the converged FP64 field
+    # than the FP64 solve's at tol 1e-4, timed (median of 3) against the FP64
+    # solves above; untimed by the headline
+    tried, pick = [], None
+    for tl in (1e-4, 1.25e-4, 1.5e-4, 2e-4, 2.5e-4, 3e-4):
+        inf = sv.solve_device(Ed.data_ptr(), FFd.data_ptr(), FLd.data_ptr(), "GPBICG1", tl,
+                              precision=32, x0_zero=True)
+        assert inf["nstop"] == 0 and inf["tol1"] <= tl, inf
+        e = ferr()
+        tried.append({"tol": tl, "field_rel_err_vs_converged": e, "iterations": inf["nloop"]})
+        if e <= err64:
+            pick = tried[-1]
+    eqe = {"fp64": {"tol": 1e-4, "field_rel_err_vs_converged": err64,
+                    "solves_per_s": 1.0 / float(np.median(lat64))}, "tried": tried}
+    if pick:
+        te = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            sv.solve_device(Ed.data_ptr(), FFd.data_ptr(), FLd.data_ptr(), "GPBICG1", pick["tol"],
+                            precision=32, x0_zero=True)
+            torch.cuda.synchronize()
+            te.append(time.perf_counter() - t1)
+        eqe["mixed"] = dict(pick, solves_per_s=1.0 / float(np.median(te)))
+        eqe["mixed_speedup_vs_fp64"] = float(np.median(lat64)) / float(np.median(te))
+
     # per-pass durations: one more mixed and one FP64 solve (untimed), all
     # five passes bracketed
     sv.timing_passes(31, 1)
@@ -379,6 +406,7 @@ def main():
                                 "p99": float(np.percentile(lat64, 99)) * 1e3},
                  "effective_GB_per_s": solve_bytes64 / float(np.median(lat64)) / 1e9},
         "mixed_speedup_vs_fp64": (args.steps / elapsed) * float(np.median(lat64)),
+        "equal_error": eqe,
         "tol_1e-8": tol8,
         "matvec": mv,
         "field_reference": {"method": "GPBICG1", "precision": "fp64", "tol": 1e-8,
