@@ -360,6 +360,15 @@ template <typename V> __device__ __forceinline__ V* tile_base(V* tw, int L) {
   const size_t off = ((size_t)L * sizeof(V) + (size_t)L * sizeof(int) + 15) & ~(size_t)15;
   return reinterpret_cast<V*>(reinterpret_cast<char*>(tw) + off);
 }
+// Row stride (in values) of the matvec x passes' LDS tile of NL lines: one
+// value of padding for wide tiles (16 lines or more: batched launches), so
+// that the rows a wavefront's butterflies touch in one access do not all
+// start on the same bank (a 24-line FP32 tile, 192 B rows, put every row of
+// a radix-8 butterfly on one bank group: 44% of LDS cycles in conflict).  On
+// the 256-RHS batch (128^3) P1 went from 1299 to 1119 us and the fused P5
+// from 1394 / 2051 to 1198 / 1860 us.  Narrow single-RHS tiles (B <= 5)
+// keep NL: padded, they measured no faster.  Other tiles: NL.
+__host__ __device__ constexpr int x_stride(int NL) { return NL >= 16 ? NL + 1 : NL; }
 template <typename V> size_t tile_lds_bytes(int L, int NL) {
   return (((size_t)L * sizeof(V) + (size_t)L * sizeof(int) + 15) & ~(size_t)15) +
          (size_t)L * NL * sizeof(V);
@@ -1028,6 +1037,7 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
   int* pos = reinterpret_cast<int*>(tw + L);
   V* buf = tile_base(tw, L);
   const int NL = BT * NC;
+  const int NLS = NC == 3 ? x_stride(NL) : NL;  // LDS row stride
   const int tid = PK && tid0 >= nthr ? kIdle : tid0;
   const int line = tid0 % NL, jt = tid / NL, JT = nthr / NL;
   const int nl = min(BT, a.nlines - g0);
@@ -1115,7 +1125,7 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         if (ll[u] < BT) {
-          V* d = buf + (DIT ? pos[ii[u]] : ii[u]) * NL + ll[u] * NC;
+          V* d = buf + (DIT ? pos[ii[u]] : ii[u]) * NLS + ll[u] * NC;
 #pragma unroll
           for (int c = 0; c < NC; ++c) d[c] = x[u][c];
         }
@@ -1125,15 +1135,16 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
   lds_sync();
 
   if constexpr (CT) {
-    constexpr int NLC = BC * NC, JTC = 256 / NLC;
+    // the stages take the row stride as their NL (it only places rows)
+    constexpr int NLC = BC * NC, JTC = 256 / NLC, NLSC = NC == 3 ? x_stride(NLC) : NLC;
     if constexpr (DIT)
-      ct_dit<LC, NLC, JTC, SGN, 0>(buf, tw, line, jt);
+      ct_dit<LC, NLSC, JTC, SGN, 0>(buf, tw, line, jt);
     else
-      ct_dif<LC, NLC, JTC, SGN, plan_c(LC).nst - 1>(buf, tw, line, jt);
+      ct_dif<LC, NLSC, JTC, SGN, plan_c(LC).nst - 1>(buf, tw, line, jt);
   } else if (DIT) {
-    tile_fft_dit<SGN>(buf, tw, a.ax, NL, line, jt, JT, half_in);
+    tile_fft_dit<SGN>(buf, tw, a.ax, NLS, line, jt, JT, half_in);
   } else {
-    tile_fft_dif<SGN>(buf, tw, a.ax, NL, line, jt, JT, half_out);
+    tile_fft_dif<SGN>(buf, tw, a.ax, NLS, line, jt, JT, half_out);
   }
 
   if constexpr (kP5) {
@@ -1171,7 +1182,7 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
 #pragma unroll
       for (int u = 0; u < U; ++u)
         if (dip[u] >= 0) {
-          const V* v = buf + (DIT ? ii[u] : pos[ii[u]]) * NL + ll[u] * 3;
+          const V* v = buf + (DIT ? ii[u] : pos[ii[u]]) * NLS + ll[u] * 3;
           V* y = zxout + dip[u] * 3;
           V yo[3];
 #pragma unroll
@@ -1222,7 +1233,7 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
     DivStep ix(tid, nthr, a.n_out);
     for (int e = tid; e < BT * a.n_out; e += nthr, ix.next()) {
       if (ix.l >= nl) continue;
-      const V* v = buf + (DIT ? ix.i : pos[ix.i]) * NL + ix.l * NC;
+      const V* v = buf + (DIT ? ix.i : pos[ix.i]) * NLS + ix.l * NC;
       V* d = zdst + (long long)(g0 + ix.l) * a.ld_dst + (long long)ix.i * NC;
 #pragma unroll
       for (int c = 0; c < NC; ++c) d[c] = v[c];
@@ -1637,7 +1648,7 @@ bool has_specialized_z(const GridPlan& g, int prec) {
 // LDS budget and, for the convolution pass, every thread holds at most kMaxG
 // Green-tensor items ((L/2+1)*B <= kMaxG*threads).
 static TileShape tile_for(const AxisPlan& ax, int NC, int esz, int nbatch, size_t lds_limit,
-                          bool conv) {
+                          bool conv, bool xpass = false) {
   const int L = ax.L;
   TileShape t{0, 0, 0};  // B = 0: no feasible tile
   for (int B = 1; B <= 64 && B <= std::max(1, nbatch); ++B) {
@@ -1645,7 +1656,8 @@ static TileShape tile_for(const AxisPlan& ax, int NC, int esz, int nbatch, size_
     if (NL > 256) break;
     const int JT = 256 / NL;
     const int threads = NL * JT;
-    const size_t lds = esz == 16 ? tile_lds_bytes<double2>(L, NL) : tile_lds_bytes<float2>(L, NL);
+    const int NLS = xpass ? x_stride(NL) : NL;  // LDS row stride
+    const size_t lds = esz == 16 ? tile_lds_bytes<double2>(L, NLS) : tile_lds_bytes<float2>(L, NLS);
     if (lds > lds_limit) break;
     if (conv && (L / 2 + 1) * B > kMaxG * threads) break;
     t = TileShape{B, threads, lds};
@@ -1658,10 +1670,11 @@ TileShape pass_tile_shape(const GridPlan& g, int pi, int p, int B, size_t lds_li
   const int esz = pi == 0 ? 16 : 8;
   const int nbatch = (p == 0 || p == 4) ? g.ny * g.nz : g.L[0];
   const bool conv = p == 2;
-  if (B == 0) return tile_for(g.ax[axis[p]], 3, esz, nbatch, lds_limit, conv);
+  const bool xp = p == 0 || p == 4;
+  if (B == 0) return tile_for(g.ax[axis[p]], 3, esz, nbatch, lds_limit, conv, xp);
   if (B < 0 || B > std::max(1, nbatch)) return TileShape{0, 0, 0};
   // feasibility is monotone in B: the widest tile up to B is B when B fits
-  const TileShape t = tile_for(g.ax[axis[p]], 3, esz, B, lds_limit, conv);
+  const TileShape t = tile_for(g.ax[axis[p]], 3, esz, B, lds_limit, conv, xp);
   return t.B == B ? t : TileShape{0, 0, 0};
 }
 
@@ -1761,19 +1774,19 @@ bool make_tiles(GridPlan& g, size_t lds_limit) {
   for (int p = 0; p < 2; ++p) {
     const int esz = p == 0 ? 16 : 8;
     const size_t bs = p == 0 ? b64 : b32, bc = p == 0 ? b64c : b32c;
-    g.p1[p] = tile_for(g.ax[0], 3, esz, g.ny * g.nz, bs, false);
+    g.p1[p] = tile_for(g.ax[0], 3, esz, g.ny * g.nz, bs, false, true);
     g.p2[p] = tile_for(g.ax[1], 3, esz, L1, bs, false);
     g.p3[p] = tile_for(g.ax[2], 3, esz, L1, bc, true);
     g.p4[p] = tile_for(g.ax[1], 3, esz, L1, bs, false);
-    g.p5[p] = tile_for(g.ax[0], 3, esz, g.ny * g.nz, bs, false);
+    g.p5[p] = tile_for(g.ax[0], 3, esz, g.ny * g.nz, bs, false, true);
     // single right-hand side: capped widths
     TileShape* o = g.one[p];
-    o[0] = tile_for(g.ax[0], 3, esz, std::min(g.ny * g.nz, kCapX), bs, false);
+    o[0] = tile_for(g.ax[0], 3, esz, std::min(g.ny * g.nz, kCapX), bs, false, true);
     o[1] = tile_for(g.ax[1], 3, esz, std::min(L1, kCapYZ), bs, false);
     o[2] = tile_for(g.ax[2], 3, esz, std::min(L1, kCapYZ), bc, true);
     if (!zconv_ct_for<float2>(L3, o[2].B) && zconv_ct_for<float2>(L3, g.p3[p].B)) o[2] = g.p3[p];
     o[3] = tile_for(g.ax[1], 3, esz, std::min(L1, kCapYZ), bs, false);
-    o[4] = tile_for(g.ax[0], 3, esz, std::min(g.ny * g.nz, kCapX), bs, false);
+    o[4] = tile_for(g.ax[0], 3, esz, std::min(g.ny * g.nz, kCapX), bs, false, true);
     // widths measured by the autotuner for this box, where they fit
     int tb[5];
     if (tuned_widths(g.nx, g.ny, g.nz, p == 0 ? 64 : 32, tb)) {
