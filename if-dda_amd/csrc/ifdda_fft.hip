@@ -75,8 +75,10 @@ template <int SGN, typename V> struct Dft<5, SGN, V> {
     using S = decltype(V::x);
     V t1 = cadd(a[1], a[4]), t2 = cadd(a[2], a[3]);
     V u1 = csub(a[1], a[4]), u2 = csub(a[2], a[3]);
-    V m1 = cadd(a[0], cadd(cscal(t1, S(IFDDA_C51)), cscal(t2, S(IFDDA_C52))));
-    V m2 = cadd(a[0], cadd(cscal(t1, S(IFDDA_C52)), cscal(t2, S(IFDDA_C51))));
+    // (a0 + c t1) + c' t2: two fused multiply-adds per component (a0 added
+    // last took a multiply, a multiply-add and an add)
+    V m1 = cadd(cadd(a[0], cscal(t1, S(IFDDA_C51))), cscal(t2, S(IFDDA_C52)));
+    V m2 = cadd(cadd(a[0], cscal(t1, S(IFDDA_C52))), cscal(t2, S(IFDDA_C51)));
     V v1 = mul_si<SGN>(cadd(cscal(u1, S(IFDDA_S51)), cscal(u2, S(IFDDA_S52))));
     V v2 = mul_si<SGN>(csub(cscal(u1, S(IFDDA_S52)), cscal(u2, S(IFDDA_S51))));
     a[0] = cadd(a[0], cadd(t1, t2));
