@@ -204,9 +204,9 @@ def test_fortran_dropin_matvec(gpu):
 
 @pytest.mark.parametrize("n", [20, 64, 96, 100])
 def test_specialized_z_pass_equals_generic(gpu, n):
-    """The z-pass kernels compiled for a fixed FFT length (the 40^3, 128^3,
-    192^3 and 200^3 grids) give the generic kernel's product: FP64 to 1e-14,
-    FP32 to 1e-6 relative to |T alpha x|."""
+    """The kernels compiled for a fixed FFT length (z pass: the 40^3, 128^3,
+    192^3 and 200^3 grids; FP64 x passes: 128, 192, 200) give the generic
+    kernels' product: FP64 to 1e-14, FP32 to 1e-6 relative to |T alpha x|."""
     import torch
     tab, x, _ = _case(n, n, n, 10 + n, frac=0.5)
     nd = int(tab.max())
