@@ -22,8 +22,8 @@ sides one after the other (a bounded sample, per-RHS throughput).
 Byte accounting: the headline figures ("bytes", "achieved", "frac") count
 what these kernels must move: vector grids in and out of each pass and the
 Green tensor as the folded octant ((L/2+1)^3 x 6) the z pass reads.  The
-"survey" figures use the per-unit bytes of the five-pass pruned matvec with
-the Green tensor counted as 6 full (L1 L2 L3) grids read once (not
+"full_tensor" figures use the per-unit bytes of the five-pass pruned matvec
+with the Green tensor counted as 6 full (L1 L2 L3) grids read once (not
 symmetry-compressed), which no kernel here moves.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-batch] [--no-cpu]
@@ -306,8 +306,8 @@ def main():
         mins = sum(pass_bytes(s.nx, s.ny, s.nz, sv.L, nd, sz))
         mv[f"fp{prec}"] = {"matvec_per_s": 1.0 / tm, "ms": tm * 1e3, "bytes": mins,
                            "GB_per_s": mins / tm / 1e9, "frac_hbm_peak": mins / tm / 1e9 / HBM_PEAK_GBS,
-                           "survey_bytes": byts, "GB_per_s_survey_basis": byts / tm / 1e9,
-                           "frac_survey_basis": byts / tm / 1e9 / HBM_PEAK_GBS}
+                           "full_tensor_bytes": byts, "GB_per_s_full_tensor_basis": byts / tm / 1e9,
+                           "frac_full_tensor_basis": byts / tm / 1e9 / HBM_PEAK_GBS}
 
     # optical force and torque of the last (FP64) field: the first call also
     # builds the three derivative Green tensors, the second reuses them
@@ -418,8 +418,8 @@ def main():
         "force": force,
         "passes": passes,
         # headline: the bytes this kernel must move (vector grids in and out,
-        # the folded Green octant it reads); the survey's per-unit figure,
-        # which counts the Green tensor as six full grids, is kept beside it
+        # the folded Green octant it reads); the per-unit figure that counts
+        # the Green tensor as six full grids is kept beside it
         "roofline": {"bound": "hbm", "kernel": f"{names[p_dom]} ({kern[p_dom]})",
                      "achieved": achieved_min, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_min / HBM_PEAK_GBS, "traffic": traffic,
@@ -427,8 +427,8 @@ def main():
                      "bytes_per_launch": m32[p_dom], "avg_us": avg_dom * 1e3,
                      "duration_source": dom_src,
                      "traffic_GB_per_s": (traffic / (avg_dom * 1e-3) / 1e9) if traffic else None,
-                     "survey_bytes_per_launch": b32[p_dom], "achieved_survey_basis": achieved,
-                     "frac_survey_basis": achieved / HBM_PEAK_GBS,
+                     "full_tensor_bytes_per_launch": b32[p_dom], "achieved_full_tensor_basis": achieved,
+                     "frac_full_tensor_basis": achieved / HBM_PEAK_GBS,
                      "copy_GB_per_s": copy_gbs, "frac_of_copy": achieved_min / copy_gbs,
                      "traffic_frac_of_copy": ((traffic / (avg_dom * 1e-3) / 1e9) / copy_gbs)
                                              if traffic else None},
