@@ -827,14 +827,15 @@ __global__ void __launch_bounds__(256) k_zconv_ct(SArgs<V> a) {
   const int kx0 = ((u & 1) ? (int)gridDim.x - 1 - (u >> 1) : (u >> 1)) * B;
   const int nvalid = min(B, a.nbx - kx0) * 3;
 
-  // FP64 (latency-bound, 3 blocks per CU by LDS): each thread's rows of a
-  // tile line go through registers; the first row's loads are issued before
-  // the tables', the mirror row's while the first row is transformed (rows
-  // ky and L2 - ky are disjoint, so reading the second before the first is
-  // stored in place is safe).  200^3: 158.7 -> 155.2 us.  FP32 (VALU-bound)
-  // loads each row straight into LDS: the registers cost it a wave per SIMD
-  // and took it from 83.6 to 88.0 us.
-  constexpr bool PREF = sizeof(V) == 16;
+  // FP64 at widths up to 5 (latency-bound, 3 blocks per CU by LDS): each
+  // thread's rows of a tile line go through registers; the first row's loads
+  // are issued before the tables', the mirror row's while the first row is
+  // transformed (rows ky and L2 - ky are disjoint, so reading the second
+  // before the first is stored in place is safe).  200^3, width 5: 158.7 ->
+  // 155.2 us.  Wider FP64 tiles and FP32 load each row straight into LDS:
+  // there the registers cost a wave per SIMD (128^3 FP64 at width 8, the
+  // batch's: 9.9 -> 11.4 ms per 256-RHS launch; FP32 200^3: 83.6 -> 88.0 us).
+  constexpr bool PREF = sizeof(V) == 16 && B <= 5;
   constexpr int NR = PREF ? (FILL + JT - 1) / JT : 1;
   [[maybe_unused]] V nxt[NR];
   const int nrow = (kyf == 0 || 2 * kyf == a.L2) ? 1 : 2;
