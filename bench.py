@@ -130,12 +130,16 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=40, help="requests per rate")
     ap.add_argument("--serving-seq", type=int, default=8, help="FP64 single solves timed per small object")
     ap.add_argument("--workers", type=int, default=8, help="runner workers")
+    ap.add_argument("--zreg", type=int, default=None,
+                    help="register-staged z pass mask (bit 0 FP64, bit 1 FP32); default: the library's")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default tile widths (skip ifdda_autotune)")
     ap.add_argument("--section", choices=["config1", "batch", "uneven", "serving", "profile"],
                     help="run only this section and print its result (for profiling)")
     ap.add_argument("--profile-reps", type=int, default=5,
                     help="profile section: device matvecs per precision")
+    ap.add_argument("--profile-fuse", default="",
+                    help="profile section: comma-separated fuse masks of extra mixed solves")
     ap.add_argument("--profile-solves", type=int, default=1,
                     help="profile section: GPBICG tol 1e-4 solves per precision")
     args = ap.parse_args()
@@ -165,6 +169,8 @@ def main():
     green_s = time.perf_counter() - t0
     sv.set_dipoles(s.tabdip)
     sv.set_polarizability(alpha)
+    if args.zreg is not None:
+        sv.zpass_registers(args.zreg)
     tiles = {"default": sv.tile_widths()}
     if not args.no_tune:  # measured tile widths for this grid (untimed setup)
         t0 = time.perf_counter()
@@ -248,6 +254,35 @@ def main():
         info64 = solve(64)
         torch.cuda.synchronize()
         lat64.append(time.perf_counter() - t1)
+
+    # FP64 with the FP32 inner solves' loop structure: inner products fused
+    # into the matvec ends (mask bit 0), and with the look-ahead recurrence
+    # (bit 4) as well; the plain FP64 leg above keeps the separate kernels
+    fp64_variants = {}
+    mixed_variants = {}
+    for name, mask in (("fused_no_lookahead", 2), ("separate", 0)):
+        sv.fuse_krylov(mask)
+        tv = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            iv = solve(32)
+            torch.cuda.synchronize()
+            tv.append(time.perf_counter() - t1)
+        mixed_variants[name] = {"mask": mask, "solves_per_s": 1.0 / float(np.median(tv)),
+                                "iterations": iv["nloop"], "tol1": iv["tol1"]}
+    for name, mask in (("fused", 1 | 2 | 8), ("fused_lookahead", 1 | 2 | 8 | 16)):
+        sv.fuse_krylov(mask)
+        tv = []
+        for _ in range(2):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            iv = solve(64)
+            torch.cuda.synchronize()
+            tv.append(time.perf_counter() - t1)
+        fp64_variants[name] = {"mask": mask, "solves_per_s": 1.0 / float(np.median(tv)),
+                               "iterations": iv["nloop"], "tol1": iv["tol1"]}
+    sv.fuse_krylov(2 | 8)
 
     # speed-up at equal field error: the loosest mixed tolerance of a few at
     # or above 1e-4 whose field is no further from the converged FP64 field
@@ -408,6 +443,8 @@ def main():
                                 "p99": float(np.percentile(lat64, 99)) * 1e3},
                  "effective_GB_per_s": solve_bytes64 / float(np.median(lat64)) / 1e9},
         "mixed_speedup_vs_fp64": (args.steps / elapsed) * float(np.median(lat64)),
+        "fp64_variants": fp64_variants,
+        "mixed_variants": mixed_variants,
         "equal_error": eqe,
         "tol_1e-8": tol8,
         "matvec": mv,
@@ -640,6 +677,13 @@ def bench_profile(args):
             assert info["nstop"] == 0 and info["tol1"] <= 1e-4, info
             out[f"solve_fp{prec}"] = {"nloop": info["nloop"], "seconds": info["seconds"]}
         print(f"profile: solves precision {prec} done", file=sys.stderr, flush=True)
+    # mixed solves with other GPBICG loop forms (ifdda_fuse_krylov masks)
+    for mask in [int(m) for m in args.profile_fuse.split(",") if m]:
+        sv.fuse_krylov(mask)
+        info = sv.solve_device(Ed.data_ptr(), FFd.data_ptr(), FLd.data_ptr(), "GPBICG1", 1e-4,
+                               precision=32, x0_zero=True)
+        out[f"solve_fp32_mask{mask}"] = {"nloop": info["nloop"], "seconds": info["seconds"]}
+        print(f"profile: mixed solve with fuse mask {mask} done", file=sys.stderr, flush=True)
     nbytes = 512 << 20
     a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
@@ -775,6 +819,8 @@ def bench_batch(args):
     sv.greencalculfft(s.d, k0)
     sv.set_dipoles(s.tabdip)
     sv.set_polarizability(alpha)
+    if args.zreg is not None:
+        sv.zpass_registers(args.zreg)
     if not args.no_tune:  # both the batch and its FP64 baseline use the tuned widths
         sv.autotune()
     n3 = 3 * s.nd

@@ -328,8 +328,14 @@ extern "C" int ifdda_use_specialized(ifdda_ctx* c, int on) {
   return IFDDA_OK;
 }
 
+extern "C" int ifdda_zpass_registers(ifdda_ctx* c, int mask) {
+  if (!c || mask < 0 || mask > 3) return arg_error("ifdda_zpass_registers: bad argument");
+  c->zreg_mask = mask;
+  return IFDDA_OK;
+}
+
 extern "C" int ifdda_fuse_krylov(ifdda_ctx* c, int mask) {
-  if (!c || mask < 0 || mask > 7) return arg_error("ifdda_fuse_krylov: bad argument");
+  if (!c || mask < 0 || mask > 31) return arg_error("ifdda_fuse_krylov: bad argument");
   c->fuse_mask = mask;
   return IFDDA_OK;
 }

@@ -45,6 +45,7 @@ template <typename V> struct MatvecDev {
   int alpha_mode;
   PassTimer* timer = nullptr;
   bool specialized = true;  // use compile-time kernels where built
+  bool zreg = false;        // P3 as the register-staged kernel where built
   int dpar = 0;             // G is d T / d r_(dpar-1) (odd along that axis), 0: T
   // batch of right-hand sides: nrhs vectors vstride elements apart, A and B
   // sized for nrhs, skip flags skip_stride ints apart
@@ -135,8 +136,8 @@ struct KState;
 // FP64, one partial per P5 block: part[(rhs * nd + q) * pstride + block].
 template <typename V> struct KFuse {
   int p5 = 0;
-  const V* r0s = nullptr;  // p5 = 1
-  const V* y = nullptr;    // p5 = 2
+  const V* r0s = nullptr;  // p5 = 1, 3
+  const V* y = nullptr;    // p5 = 2, 3
   double2* part = nullptr;
   int pstride = 0;
   // p1 = 1: P1 first forms p = r + beta (p - u) (beta of each RHS's solver
@@ -150,6 +151,21 @@ template <typename V> struct KFuse {
 // Blocks of the P5 pass of a launch over R right-hand sides (the partials a
 // fused P5 writes per inner product and RHS).
 int matvec_p5_blocks(const GridPlan& g, int prec, int R);
+// The register-staged z pass (ifdda_zreg.hip): whether it is built for
+// (L3, B) at prec 64 / 32, and its launch over a z pass's argument block and
+// grid (24 B threads).  False when not built or not launchable.
+template <typename V> struct SArgs;
+bool has_zreg(int L, int B, int prec);
+bool launch_zreg(const SArgs<double2>& a, int L, int B, dim3 grid, hipStream_t s);
+bool launch_zreg(const SArgs<float2>& a, int L, int B, dim3 grid, hipStream_t s);
+// The y passes with compile-time length and width (ifdda_ystride.hip): built
+// for (L2, B) at prec and direction sgn (+1 P2, -1 P4); launch with the
+// stage-major twiddles in a.tws and lds bytes of LDS.
+bool has_strided_ct(int L, int B, int prec, int sgn);
+bool launch_strided_ct(const SArgs<double2>& a, int L, int B, int sgn, dim3 grid, size_t lds,
+                       hipStream_t s);
+bool launch_strided_ct(const SArgs<float2>& a, int L, int B, int sgn, dim3 grid, size_t lds,
+                       hipStream_t s);
 bool launch_matvec_fused_f64(const GridPlan& g, const MatvecDev<double2>& m, const double2* x,
                              const double2* b, double2* y, hipStream_t s, const int* skip,
                              const KFuse<double2>& f);
@@ -271,7 +287,8 @@ struct ifdda_ctx {
   bool specialized = true;  // ifdda_use_specialized
   // GPBICG inner products fused into the matvec ends (ifdda_fuse_krylov):
   // bit 0 FP64 solves, bit 1 FP32 (the inner solves of mixed precision)
-  int fuse_mask = 2;
+  int fuse_mask = 2 | 8;  // ifdda_fuse_krylov
+  int zreg_mask = 0;      // ifdda_zpass_registers: bit 0 FP64, bit 1 FP32
 
   ifdda::MatvecDev<double2> mv64() const {
     ifdda::MatvecDev<double2> m;
@@ -284,6 +301,7 @@ struct ifdda_ctx {
     m.alpha_mode = alpha_mode;
     m.timer = timer.cap ? const_cast<ifdda::PassTimer*>(&timer) : nullptr;
     m.specialized = specialized;
+    m.zreg = zreg_mask & 1;
     return m;
   }
   ifdda::MatvecDev<float2> mv32() const {
@@ -297,6 +315,7 @@ struct ifdda_ctx {
     m.alpha_mode = alpha_mode;
     m.timer = timer.cap ? const_cast<ifdda::PassTimer*>(&timer) : nullptr;
     m.specialized = specialized;
+    m.zreg = (zreg_mask >> 1) & 1;
     return m;
   }
 };
@@ -340,6 +359,8 @@ struct MvOp {
   // at precision prec (the solver then runs the separate vector kernels).
   virtual int fuse_blocks(int prec, int R) const { (void)prec; (void)R; return 0; }
   virtual bool fuse_p1() const { return false; }  // p update inside P1 as well
+  // the look-ahead GPBICG loop (three more fused products, no g_k1) for prec
+  virtual bool fuse_la(int prec) const { (void)prec; return false; }
   virtual bool mvf64(const double2* x, const double2* b, double2* y, hipStream_t s,
                      const int* skip, int R, long long n, int skip_stride,
                      const KFuse<double2>& f) const {

@@ -20,6 +20,7 @@ struct KState {
   double residu;  // recurrence residual estimate
   double tol;
   double2 alpha, beta, eta, dzeta, r0rn;    // GPBICG (eta/alpha shared by all)
+  double2 ct_la, beta_la;  // GPBICG look-ahead: <r0*, r_new> and the next beta
   double2 rho, omega, tau, taut, theta, thetat, etat;  // QMRBICGSTAB / TFQMR
   double2 rho0, xxi;  // CG (zcg.f), BICGSTAB (pimzbicgstab.f)
   double rr;          // BICGSTAB: sum conj(r) r of the latest residual

@@ -73,10 +73,12 @@ template <typename V> struct GV {
 };
 
 // r = b - Ax (Ax = 0 when x0 = 0); r0* = conj(r); p = t = told = u = z = 0;
-// partial sum r0* r  (gpbicg.f:133-148)
+// partial sum r0* r  (gpbicg.f:133-148).  pinit: p = r instead, the first
+// iteration's p = r + beta (p - u) with beta = 0, for the look-ahead loop,
+// which forms p at the end of the iteration before.
 template <typename V>
 __global__ void __launch_bounds__(kThreads) g_start(GV<V> v, int x0zero, VN n,
-                                                   double2* part, int nblk) {
+                                                   double2* part, int nblk, int pinit = 0) {
   const int rh_ = n.rhs();  // right-hand side
   v = v.shifted(n.off(rh_));
   part += (long long)rh_ * kMaxDots * nblk;
@@ -88,7 +90,8 @@ __global__ void __launch_bounds__(kThreads) g_start(GV<V> v, int x0zero, VN n,
     if (!x0zero) r = zsub(r, LD(v.Ax, i));
     ST(v.r, i, r);
     ST(v.r0s, i, zconj(r));
-    ST(v.p, i, z0); ST(v.t, i, z0); ST(v.told, i, z0); ST(v.u, i, z0); ST(v.z, i, z0);
+    ST(v.p, i, pinit ? r : z0);
+    ST(v.t, i, z0); ST(v.told, i, z0); ST(v.u, i, z0); ST(v.z, i, z0);
     ST(v.Ap, i, z0); ST(v.Apold, i, z0); ST(v.At, i, z0);
     const double2 rs = zconj(r);
     acc[0] = zadd(acc[0], zmul(rs, r));
@@ -315,6 +318,107 @@ __global__ void __launch_bounds__(kThreads) g_f5(KState* st, const double2* part
   }
   st->beta = zdiv(zdiv(zmul(st->alpha, ct), st->dzeta), st->r0rn);
   st->r0rn = ct;
+  if (st->it > st->maxit) {  // gpbicg.f:327-330
+    st->status = 1;
+    st->steperr = 0;
+    st->done = 1;
+    return;
+  }
+  st->it++;
+}
+
+// ------------------------------------------------ GPBICG, look-ahead form
+// The same recurrence with its last reduction moved ahead, so that the step
+// that updates u, z, x and r also forms the next iteration's p and no kernel
+// of its own does (g_k1):
+//   r_new = t - eta y - zeta At, so <r0*, r_new> = <r0*, t> - eta <r0*, y>
+//   - zeta <r0*, At>, whose three products the second matvec's gathering
+//   pass sums with the five of gpbicg.f:222-253 (MODE 7);
+//   beta_new = alpha <r0*, r_new> / (zeta <r0*, r>)   (gpbicg.f:314)
+// is then known before the update, which writes p = r_new + beta_new
+// (p - u_new) (gpbicg.f:161-169) over p once x has read the old p.  t is
+// re-formed from r and Ap as g_k3 stored it, and r0* is not read.  Vector
+// traffic per iteration falls from 29 to 25 streams of 3 Nd values.  The
+// stopping test, the breakdown tests and their order are gpbicg.f's; only
+// <r0*, r_new> is a sum of three products instead of one (equal in exact
+// arithmetic, and in rounding to ~1e-16 of |r0*| |t| in FP64).
+
+// zeta and eta (g_f4x), then <r0*, r_new> and the next beta
+__global__ void __launch_bounds__(kThreads) g_f4xL(KState* st, const double2* part, int nblk,
+                                                  int gp2) {
+  const int rh_ = blockIdx.y;
+  st += rh_;
+  part += (long long)rh_ * 8 * nblk;
+  if (st->done) return;
+  finish_sum_any<8>(st, part, nblk);
+  if (threadIdx.x != 0) return;
+  gp_zeta(st, gp2);
+  if (st->done) return;
+  const double2* d = st->dots;
+  const double2 ct = zsub(zsub(d[5], zmul(st->eta, d[6])), zmul(st->dzeta, d[7]));
+  st->ct_la = ct;
+  // gpbicg.f:308-316: with r0rn = 0 the iteration ends in a breakdown (g_f5L)
+  st->beta_la = zzero(st->r0rn) ? make_double2(0.0, 0.0)
+                                : zdiv(zdiv(zmul(st->alpha, ct), st->dzeta), st->r0rn);
+}
+
+// u, z, x, r updates and the next p; partial |r|^2 (gpbicg.f:270-281,
+// :300-307, :161-169)
+template <typename V>
+__global__ void __launch_bounds__(kThreads) g_k5L(const KState* st, GV<V> v, VN n,
+                                                 double2* part, int nblk) {
+  const int rh_ = n.rhs();  // right-hand side
+  st += rh_;
+  v = v.shifted(n.off(rh_));
+  part += (long long)rh_ * kMaxDots * nblk;
+  if (st->done) return;
+  const double2 al = st->alpha, be = st->beta, et = st->eta, dz = st->dzeta, bn = st->beta_la;
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  RHS_BLOCKS(n, rh_, 1);
+  GRID_LOOP(i, n.len(rh_)) {
+    const double2 Ap = LD(v.Ap, i), r = LD(v.r, i), y = LD(v.y, i), At = LD(v.At, i),
+                  uo = LD(v.u, i), p = LD(v.p, i);
+    const double2 t = Vt<V>::rnd(zsub(r, zmul(al, Ap)));  // as g_k3 stored it
+    const double2 u = zadd(zmul(dz, Ap), zmul(et, zadd(zsub(LD(v.told, i), r), zmul(be, uo))));
+    const double2 z = zsub(zadd(zmul(dz, r), zmul(et, LD(v.z, i))), zmul(al, u));
+    ST(v.u, i, u);
+    ST(v.z, i, z);
+    ST(v.x, i, zadd(zadd(LD(v.x, i), zmul(al, p)), z));
+    const double2 rn = zsub(zsub(t, zmul(et, y)), zmul(dz, At));
+    ST(v.r, i, rn);
+    // the next p from the stored r and u, as g_k1 would read them
+    const double2 rs = Vt<V>::rnd(rn), us = Vt<V>::rnd(u);
+    ST(v.p, i, zadd(rs, zmul(bn, zsub(p, us))));
+    acc[0].x += rs.x * rs.x + rs.y * rs.y;
+  }
+  block_partials<1>(acc, part, nblk, bx_);
+}
+
+// g_f5 with <r0*, r_new> and beta from g_f4xL
+__global__ void __launch_bounds__(kThreads) g_f5L(KState* st, const double2* part, int nblk,
+                                                 double2* trace) {
+  const int rh_ = blockIdx.y;  // right-hand side
+  st += rh_;
+  part += (long long)rh_ * kMaxDots * nblk;
+  if (st->done) return;
+  finish_sum<1>(st, part, nblk);
+  if (threadIdx.x != 0) return;
+  const double res = sqrt(st->dots[0].x) / st->norm;
+  st->residu = res;
+  push_trace(st, trace, st->alpha, st->beta, st->eta, st->dzeta, st->r0rn, res);
+  if (res <= st->tol) {  // gpbicg.f:285-296
+    st->status = 1;
+    st->done = 1;
+    return;
+  }
+  if (zzero(st->r0rn)) {  // gpbicg.f:308-312
+    st->status = -1;
+    st->steperr = 4;
+    st->done = 1;
+    return;
+  }
+  st->beta = st->beta_la;
+  st->r0rn = st->ct_la;
   if (st->it > st->maxit) {  // gpbicg.f:327-330
     st->status = 1;
     st->steperr = 0;
@@ -1626,6 +1730,7 @@ struct CtxOp : MvOp {
     return matvec_p5_blocks(c->g, prec, R);
   }
   bool fuse_p1() const override { return (c->fuse_mask >> 2) & 1; }
+  bool fuse_la(int prec) const override { return (c->fuse_mask >> (prec == 64 ? 4 : 3)) & 1; }
   bool mvf64(const double2* x, const double2* b, double2* y, hipStream_t s, const int* skip, int R,
              long long n, int skip_stride, const KFuse<double2>& f) const override {
     MatvecDev<double2> m = c->mv64();
@@ -1813,18 +1918,50 @@ static bool krylov(const MvOp& op, Solver* S, V** vv, int R, int method, const d
     V* const tb[2] = {vv[6], vv[12]};  // t / told
     V* const ab[2] = {vv[5], vv[10]};  // Ap / Apold
     GV<V> v{x, b, vv[2], vv[3], vv[4], ab[0], tb[0], vv[7], vv[8], vv[9], ab[1], vv[11], tb[1], Ax};
-    hipLaunchKernelGGL(g_start<V>, G, T, 0, s, v, x0zero, vn, S->part, nb);
-    hipLaunchKernelGGL(g_fstart, F, T, 0, s, S->st, (const double2*)S->part, nb);
     // Fused matvec ends (one context): <r0*, Ap> inside the first matvec's
     // last pass, the five products of step 4 inside the second's, and on
     // request p = r + beta (p - u) inside the first matvec's scatter (it
     // replaces g_k1 but reads r, u and p through the dipole map, slower than
     // g_k1's stream on the 100^3 cube).  The vectors are those of the
     // separate kernels; only the products are summed in another order.
-    const int nb5 = S->lens ? 0 : op.fuse_blocks(std::is_same<V, double2>::value ? 64 : 32, R);
-    const bool fuse = nb5 > 0 && S->part5 && S->part5_cap >= 5LL * R * nb5;
-    const bool p1f = fuse && op.fuse_p1();
-    for (int it = 0; fuse; ++it) {
+    // Look-ahead (la): the second matvec's last pass also sums <r0*, t>,
+    // <r0*, y> and <r0*, At>, and g_k5L forms the next p (no g_k1).
+    const int prec = std::is_same<V, double2>::value ? 64 : 32;
+    const int nb5 = S->lens ? 0 : op.fuse_blocks(prec, R);
+    const bool la = nb5 > 0 && op.fuse_la(prec) && S->part5 && S->part5_cap >= 8LL * R * nb5;
+    const bool fuse = la || (nb5 > 0 && S->part5 && S->part5_cap >= 5LL * R * nb5);
+    const bool p1f = fuse && !la && op.fuse_p1();
+    hipLaunchKernelGGL(g_start<V>, G, T, 0, s, v, x0zero, vn, S->part, nb, la ? 1 : 0);
+    hipLaunchKernelGGL(g_fstart, F, T, 0, s, S->st, (const double2*)S->part, nb);
+    for (int it = 0; la; ++it) {
+      v.told = tb[it & 1];
+      v.t = tb[(it + 1) & 1];
+      v.Ap = ab[it & 1];
+      v.Apold = ab[(it + 1) & 1];
+      KFuse<V> f1;
+      f1.p5 = 1;
+      f1.r0s = v.r0s;
+      f1.part = S->part5;
+      f1.pstride = nb5;
+      if (!mvf<V>(op, v.p, v.p, v.Ap, s, done, R, n, dstride, f1)) return false;
+      hipLaunchKernelGGL(g_f2x, F, T, 0, s, S->st, (const double2*)S->part5, nb5);
+      hipLaunchKernelGGL(g_k3<V>, G, T, 0, s, (const KState*)S->st, v, vn, gp2);
+      KFuse<V> f2;
+      f2.p5 = 3;
+      f2.y = v.y;
+      f2.r0s = v.r0s;
+      f2.part = S->part5;
+      f2.pstride = nb5;
+      if (!mvf<V>(op, v.t, v.t, v.At, s, done, R, n, dstride, f2)) return false;
+      hipLaunchKernelGGL(g_f4xL, F, T, 0, s, S->st, (const double2*)S->part5, nb5, gp2);
+      hipLaunchKernelGGL(g_k5L<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
+      hipLaunchKernelGGL(g_f5L, F, T, 0, s, S->st, (const double2*)S->part, nb, trace);
+      if (!hip_ok(hipGetLastError(), "gpbicg launch")) return false;
+      const int st = step_poll(it);
+      if (st < 0) return false;
+      if (st > 0) break;
+    }
+    for (int it = 0; fuse && !la; ++it) {
       v.told = tb[it & 1];
       v.t = tb[(it + 1) & 1];
       v.Ap = ab[it & 1];
@@ -2084,7 +2221,7 @@ static int solve_dev(ifdda_ctx* c, const ifdda_solve_opts* o, int R, const doubl
   if (!S) return IFDDA_ERR_HIP;
   if (o->method == IFDDA_GPBICG1 || o->method == IFDDA_GPBICG2) {
     // partial sums of the inner products formed in fused matvec ends
-    const long long need = 5LL * R * CtxOp(c).fuse_blocks(o->precision, R);
+    const long long need = 8LL * R * CtxOp(c).fuse_blocks(o->precision, R);
     if (need > S->part5_cap) {
       (void)hipFree(S->part5);
       S->part5 = nullptr;
@@ -2482,7 +2619,7 @@ extern "C" int ifdda_solve_device_noalloc(ifdda_ctx* c, const ifdda_solve_opts* 
   Solver S;
   const bool need32 = o->precision == 32;
   const long long part5 = o->method == IFDDA_GPBICG1 || o->method == IFDDA_GPBICG2
-                              ? 5LL * CtxOp(c).fuse_blocks(o->precision, 1)
+                              ? 8LL * CtxOp(c).fuse_blocks(o->precision, 1)
                               : 0;
   const size_t need = noalloc_layout(&S, n, need32, part5, nullptr);
   if (need > (size_t)IFDDA_NOALLOC_BYTES) {

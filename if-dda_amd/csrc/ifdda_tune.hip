@@ -97,7 +97,7 @@ static int tune_one(ifdda_ctx* c, int pi, int reps, size_t lds_limit, int chosen
   }
   V *x = nullptr, *y = nullptr;
   PassTimer tm;
-  tm.cap = 5 * reps;
+  tm.cap = 10 * reps;  // two matvecs per repetition when P5 is timed in its fused forms
   std::vector<hipEvent_t> ev(2 * (size_t)tm.cap, nullptr);
   std::vector<int> tag(tm.cap, 0);
   auto cleanup = [&]() {
@@ -120,11 +120,44 @@ static int tune_one(ifdda_ctx* c, int pi, int reps, size_t lds_limit, int chosen
   MatvecDev<V> m;
   if constexpr (sizeof(V) == 16) m = c->mv64(); else m = c->mv32();
   m.timer = nullptr;
+  // Solves of this precision with GPBICG products fused into P5
+  // (ifdda_fuse_krylov) run P5 as its fused forms, one gathering <r0*, Ap>,
+  // the other the step-4 products (and the look-ahead's three): time those,
+  // one after the other in every repetition, in place of the plain gather.
+  const int pbit = sizeof(V) == 16 ? 0 : 1;
+  const bool fz = c->specialized && ((c->fuse_mask >> pbit) & 1);
+  const bool la = fz && ((c->fuse_mask >> (pbit == 0 ? 4 : 3)) & 1);
+  const long long np5 = (long long)g.ny * g.nz;  // P5 blocks at width 1, an upper bound
+  double2* part = nullptr;
+  KFuse<V> f1, f2;
+  if (fz) {
+    if (!hip_ok(hipMalloc(&part, sizeof(double2) * 8 * np5), "hipMalloc tune partials"))
+      return cleanup(), IFDDA_ERR_HIP;
+    f1.p5 = 1;
+    f1.r0s = x;
+    f2.p5 = la ? 3 : 2;
+    f2.y = x;
+    f2.r0s = x;
+    f1.part = f2.part = part;
+    f1.pstride = f2.pstride = (int)np5;
+  }
   const TileShape keep[5] = {*pass_tile(g, pi, 0), *pass_tile(g, pi, 1), *pass_tile(g, pi, 2),
                              *pass_tile(g, pi, 3), *pass_tile(g, pi, 4)};
   std::vector<double> best(5, 1e300);
   for (int p = 0; p < 5; ++p) chosen[p] = keep[p].B;
   auto run = [&](PassTimer* t) {
+    if (fz) {
+      m.timer = t;
+      bool ok;
+      if constexpr (sizeof(V) == 16)
+        ok = launch_matvec_fused_f64(g, m, x, x, y, c->stream, nullptr, f1) &&
+             launch_matvec_fused_f64(g, m, x, x, y, c->stream, nullptr, f2);
+      else
+        ok = launch_matvec_fused_f32(g, m, x, x, y, c->stream, nullptr, f1) &&
+             launch_matvec_fused_f32(g, m, x, x, y, c->stream, nullptr, f2);
+      m.timer = nullptr;
+      return ok;
+    }
     if constexpr (sizeof(V) == 16)
       return launch_matvec_f64(g, m, x, x, y, c->stream, nullptr, t);
     else
@@ -170,6 +203,8 @@ static int tune_one(ifdda_ctx* c, int pi, int reps, size_t lds_limit, int chosen
     for (const TileShape& t : cand[p])
       if (t.B == chosen[p]) *pass_tile(g, pi, p) = t;
   }
+  if (part) (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(part);
   cleanup();
   if (rc == IFDDA_OK) store_widths(g.nx, g.ny, g.nz, pi == 0 ? 64 : 32, chosen);
   return rc;

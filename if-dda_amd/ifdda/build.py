@@ -34,10 +34,26 @@ def _run(cmd):
 
 
 def build_hip():
+    """One object per source, compiled in parallel (each source is its own
+    translation unit either way), then one link."""
+    from concurrent.futures import ThreadPoolExecutor
+
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    objdir = os.path.join(PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    cflags = [f for f in HIPCC_FLAGS if f != "-shared"]
+
+    def obj(src):
+        o = os.path.join(objdir, os.path.basename(src)[:-4] + ".o")
+        _run([hipcc, *cflags, "-c", "-o", o, src])
+        return o
+
+    jobs = max(1, min(len(srcs), os.cpu_count() or 1, 16))
+    with ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(obj, srcs))
     tmp = LIB + ".tmp"
-    _run([hipcc, *HIPCC_FLAGS, "-o", tmp, *srcs])
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     return LIB
 

@@ -109,6 +109,7 @@ def lib():
     L.ifdda_timing_passes.argtypes = [vp, ip, ip]
     L.ifdda_use_specialized.argtypes = [vp, ip]
     L.ifdda_fuse_krylov.argtypes = [vp, ip]
+    L.ifdda_zpass_registers.argtypes = [vp, ip]
     pd = C.POINTER(C.c_double)
     L.ifdda_cross_sections_device.argtypes = [vp, ip, vp, vp, vp, dp, pd, pd, vp]
     L.ifdda_farfield_device.argtypes = [vp, ip, vp, ip, dp, dp, dp, dp, dp, vp, vp,
@@ -507,9 +508,14 @@ class Solver:
         _check(lib().ifdda_use_specialized(self.h, int(bool(on))), "ifdda_use_specialized")
 
     def fuse_krylov(self, mask: int):
-        """GPBICG work fused into the matvec ends: bit 0 FP64, bit 1 FP32 (default 2);
-        bit 2 also forms p = r + beta (p - u) inside the scatter pass."""
+        """GPBICG work fused into the matvec ends: bit 0 FP64, bit 1 FP32; bit 2
+        also forms p = r + beta (p - u) inside the scatter pass; bits 3 (FP32) and
+        4 (FP64) the look-ahead loop (default 10: FP32 fused, look-ahead)."""
         _check(lib().ifdda_fuse_krylov(self.h, int(mask)), "ifdda_fuse_krylov")
+
+    def zpass_registers(self, mask: int):
+        """The register-staged z pass: bit 0 FP64 matvecs, bit 1 FP32."""
+        _check(lib().ifdda_zpass_registers(self.h, int(mask)), "ifdda_zpass_registers")
 
     def specialized_z(self, prec: int = 64) -> bool:
         """Whether this grid's z pass has a compile-time kernel."""

@@ -191,12 +191,27 @@ int ifdda_use_specialized(ifdda_ctx *ctx, int on);
  * the update p = r + beta (p - u) (gpbicg.f:161-169) into the matvec's
  * scatter pass, in place of its own kernel (off by default: slower on the
  * 100^3 cube, where the scatter reads r, u and p through the dipole map
- * less efficiently than the separate kernel streams them).  The vectors are
- * the same; the products are summed in another fixed order, so fused and
- * unfused trajectories agree to rounding.  Single-context solves with the
- * specialized kernels on (ifdda_use_specialized) only; packs always run the
- * separate kernels. */
+ * less efficiently than the separate kernel streams them).  Bits 3 (FP32,
+ * on by default: mask 10) and 4 (FP64) select the look-ahead form of the
+ * fused loop: the second matvec's gathering pass also sums <r0*,t>, <r0*,y>
+ * and <r0*,At>, which give <r0*, r_new> and the next beta (gpbicg.f:300-316)
+ * before the u/z/x/r update, and that update writes the next p itself
+ * (25 vector streams per iteration instead of 29; bit 2 is then ignored).
+ * The vectors are the same; the products are summed in another fixed order,
+ * so fused and unfused trajectories agree to rounding.  Single-context
+ * solves with the specialized kernels on (ifdda_use_specialized) only; packs
+ * always run the separate kernels. */
 int ifdda_fuse_krylov(ifdda_ctx *ctx, int mask);
+
+/* The z pass (FFT-z . G . IFFT-z, produitfftmatvect3.f:62-107) as the
+ * register-staged kernel (no counterpart in the reference, whose FFTs run
+ * per component through fftsingletonz): the length-L3 transforms split into
+ * 8-point steps across threads and L3/8-point steps inside one thread's
+ * registers, for L3 = 128, 192, 200 at the tile widths it is built for.
+ * mask bit 0: FP64 matvecs, bit 1: FP32.  Same products as the other z
+ * kernels to rounding; single-context matvecs with the specialized kernels
+ * on only. */
+int ifdda_zpass_registers(ifdda_ctx *ctx, int mask);
 /* *has = 1 when this grid's z pass has a compile-time kernel for prec 64/32. */
 int ifdda_specialized_z(ifdda_ctx *ctx, int prec, int *has);
 

@@ -205,7 +205,7 @@ def test_fortran_dropin_matvec(gpu):
 @pytest.mark.parametrize("n", [20, 64, 96, 100])
 def test_specialized_z_pass_equals_generic(gpu, n):
     """The kernels compiled for a fixed FFT length (z pass: the 40^3, 128^3,
-    192^3 and 200^3 grids; FP64 x passes: 128, 192, 200) give the generic
+    192^3 and 200^3 grids; FP64 x passes and y passes: 128, 192, 200) give the generic
     kernels' product: FP64 to 1e-14, FP32 to 1e-6 relative to |T alpha x|."""
     import torch
     tab, x, _ = _case(n, n, n, 10 + n, frac=0.5)
@@ -231,3 +231,60 @@ def test_specialized_z_pass_equals_generic(gpu, n):
                 outs.append(y - x.astype(cdt).astype(np.complex128))  # -T alpha x
             sv.use_specialized(True)
             assert relerr(outs[0], outs[1]) < tol, (prec, relerr(outs[0], outs[1]))
+
+
+@pytest.mark.parametrize("n", [64, 96, 100])
+def test_register_z_pass_equals_lds_z_pass(gpu, n):
+    """The register-staged z pass (8-point steps across threads, L/8-point
+    steps in one thread's registers) gives the LDS-staged compile-time z
+    pass's product on the 128^3, 192^3 and 200^3 grids: FP64 to 1e-13, FP32
+    to 1e-6 relative to |T alpha x|."""
+    import torch
+    tab, x, _ = _case(n, n, n, 20 + n, frac=0.5)
+    nd = int(tab.max())
+    rng = np.random.default_rng(n + 1)
+    alpha = (1 + rng.random(nd)) * 1e-24 + 1j * rng.random(nd) * 1e-26
+    with gpu.Solver(n, n, n) as sv:
+        print(f"register z pass {n}^3 L={sv.L}", flush=True)
+        sv.greencalculfft(D, K0)
+        sv.set_dipoles(tab)
+        sv.set_polarizability(alpha)
+        for prec, tol in ((64, 1e-13), (32, 1e-6)):
+            cdt, rdt = (np.complex128, np.float64) if prec == 64 else (np.complex64, np.float32)
+            xd = torch.from_numpy(x.astype(cdt).view(rdt).copy()).cuda()
+            outs = []
+            for mask in (3, 0):
+                sv.zpass_registers(mask)
+                yd = torch.zeros_like(xd)
+                sv.matvec_device(_dev(xd), _dev(xd), _dev(yd), prec=prec)
+                torch.cuda.synchronize()
+                y = yd.cpu().numpy().view(cdt).astype(np.complex128)
+                outs.append(y - x.astype(cdt).astype(np.complex128))  # -T alpha x
+            sv.zpass_registers(0)
+            err = relerr(outs[0], outs[1])
+            assert err < tol, (prec, err)
+
+
+def test_register_z_pass_batch_solve(gpu):
+    """Four right-hand sides of the 20-sphere brightfield object solved at
+    once on its 128^3 grid (the batched z pass, right-hand sides in groups per
+    tile) with the register-staged z pass: FP64 iteration counts and fields
+    as with the LDS-staged one."""
+    from ifdda import scene
+    s, k0, alpha, E = scene.config3(2)
+    with gpu.Solver(s.nx, s.ny, s.nz) as sv:
+        sv.greencalculfft(s.d, k0)
+        sv.set_dipoles(s.tabdip)
+        sv.set_polarizability(alpha)
+        res = {}
+        for mask in (3, 0):
+            sv.zpass_registers(mask)
+            print(f"register z pass batch solve mask {mask}", flush=True)
+            res[mask] = sv.inverserig_multi(E, "GPBICG1", 1e-6, precision=64, x0_zero=True)
+        sv.zpass_registers(0)
+        _, FLa, ia = res[3]
+        _, FLb, ib = res[0]
+        for r in range(E.shape[0]):
+            assert ia[r]["nstop"] == 0 and ia[r]["tol1"] <= 1e-6
+            assert abs(ia[r]["nloop"] - ib[r]["nloop"]) <= 1
+            assert relerr(FLa[r], FLb[r]) < 1e-9
