@@ -376,7 +376,6 @@ __global__ void __launch_bounds__(256) k_zconv_ct(SArgs<V> a) {
   constexpr int NG = zct_ng(L, B);
   constexpr PlanC P = plan_c(L);
   static_assert(P.nst > 0 && NG >= 1 && NG <= kMaxG, "unsupported compile-time z plan");
-  constexpr int FILL = P.rad[0] % 2 == 0 ? L / 2 : L;
   int lg, rhs;
   rhs_group_tile(lg, rhs);
   if (a.skip && a.skip[rhs * a.skip_stride]) return;
@@ -400,10 +399,18 @@ __global__ void __launch_bounds__(256) k_zconv_ct(SArgs<V> a) {
   // are issued before the tables', the mirror row's while the first row is
   // transformed (rows ky and L2 - ky are disjoint, so reading the second
   // before the first is stored in place is safe).  200^3, width 5: 158.7 ->
-  // 155.2 us.  Wider FP64 tiles and FP32 load each row straight into LDS:
-  // there the registers cost a wave per SIMD (128^3 FP64 at width 8, the
-  // batch's: 9.9 -> 11.4 ms per 256-RHS launch; FP32 200^3: 83.6 -> 88.0 us).
+  // 155.2 us.  These rows keep the load / LDS / first-stage order and the
+  // store from LDS: fed from registers to the first stage and stored from the
+  // last, the FP64 pass measured 166.8 against 159.7 us.  Wider FP64 tiles
+  // and FP32 load each row when they transform it (holding a row in
+  // registers cost a wave per SIMD: 128^3 FP64 at width 8, the batch's: 9.9
+  // -> 11.4 ms per 256-RHS launch; FP32 200^3: 83.6 -> 88.0 us), straight
+  // into the first stage's registers (ct_load_first), and store from the
+  // last stage's registers (ct_last_store): FP32 200^3 88.3 -> 85.4 us.  The
+  // arithmetic is the same either way (the build contracts multiply-adds
+  // only within an expression), so every width gives the same bits.
   constexpr bool PREF = sizeof(V) == 16 && B <= 5;
+  constexpr int FILL = P.rad[0] % 2 == 0 ? L / 2 : L;
   constexpr int NR = PREF ? (FILL + JT - 1) / JT : 1;
   [[maybe_unused]] V nxt[NR];
   const int nrow = (kyf == 0 || 2 * kyf == a.L2) ? 1 : 2;
@@ -421,11 +428,12 @@ __global__ void __launch_bounds__(256) k_zconv_ct(SArgs<V> a) {
       if (rr + 1 < nrow)
         fetch_rows<FILL, JT>(nxt, zsrc + (long long)(a.L2 - kyf) * a.so_src + (long long)kx0 * 3,
                              a.sp_src, a.n_in, line, nvalid, jt);
+      ct_dit<L, NL, JT, +1, 0>(buf, tws, line, jt);
     } else {
-      load_strided<true>(buf, pos, zsrc + (long long)ky * a.so_src + (long long)kx0 * 3, a.sp_src,
-                         FILL, a.n_in, NL, line, nvalid, jt, JT);
+      ct_load_first<L, NL, JT, +1>(buf, pos, zsrc + (long long)ky * a.so_src + (long long)kx0 * 3,
+                                   a.sp_src, a.n_in, line, nvalid, jt);
+      ct_dit<L, NL, JT, +1, 1>(buf, tws, line, jt);
     }
-    ct_dit<L, NL, JT, +1, 0>(buf, tws, line, jt);
     V gr[NG][6];
     int gx[NG];  // bit0 valid, bit1 x odd-parity sign
 #pragma unroll
@@ -456,13 +464,19 @@ __global__ void __launch_bounds__(256) k_zconv_ct(SArgs<V> a) {
         sym_product(buf + (L - kzf) * NL + kxl * 3, gr[it], sxn != sy, !sxn, !sy);
     }
     lds_sync();
-    ct_dif<L, NL, JT, -1, P.nst - 1>(buf, tws, line, jt);
-    if (line < nvalid) {
-      V* d = zdst + (long long)ky * a.so_dst + (long long)kx0 * 3 + line + (long long)jt * a.sp_dst;
-      const long long step = (long long)JT * a.sp_dst;
-      for (int p = jt; p < a.n_out; p += JT, d += step) *d = buf[pos[p] * NL + line];
+    if constexpr (PREF) {
+      ct_dif<L, NL, JT, -1, P.nst - 1>(buf, tws, line, jt);
+      if (line < nvalid) {
+        V* d = zdst + (long long)ky * a.so_dst + (long long)kx0 * 3 + line + (long long)jt * a.sp_dst;
+        const long long step = (long long)JT * a.sp_dst;
+        for (int p = jt; p < a.n_out; p += JT, d += step) *d = buf[pos[p] * NL + line];
+      }
+      lds_sync();  // the next row reuses the LDS tile
+    } else {  // the last stage fused with the store (its barrier ends the row)
+      ct_dif<L, NL, JT, -1, P.nst - 1, V, 1>(buf, tws, line, jt);
+      ct_last_store<L, NL, JT, -1>(buf, pos, zdst + (long long)ky * a.so_dst + (long long)kx0 * 3,
+                                   a.sp_dst, a.n_out, line, nvalid, jt);
     }
-    lds_sync();  // the next row reuses the LDS tile
   }
 }
 

@@ -436,15 +436,105 @@ __device__ __forceinline__ void ct_dit(V* buf, const V* tws, int line, int jt) {
     ct_dit<L, NL, JT, SGN, S + 1>(buf, tws, line, jt);
   }
 }
-// DIF over stages S down to 0 (last stage half-output pruned when even).
-template <int L, int NL, int JT, int SGN, int S, typename V>
+// DIF over stages S down to SL (last stage half-output pruned when even).
+template <int L, int NL, int JT, int SGN, int S, typename V, int SL = 0>
 __device__ __forceinline__ void ct_dif(V* buf, const V* tws, int line, int jt) {
   constexpr PlanC P = plan_c(L);
-  if constexpr (S >= 0) {
+  if constexpr (S >= SL) {
     constexpr int R = P.rad[S], M = plan_m(P, S);
     ct_stage<L, R, M, NL, JT, SGN, false, (S == 0 && R % 2 == 0) ? 2 : 0>(buf, tws, line, jt);
-    ct_dif<L, NL, JT, SGN, S - 1>(buf, tws, line, jt);
+    ct_dif<L, NL, JT, SGN, S - 1, V, SL>(buf, tws, line, jt);
   }
+}
+
+// The global load of a strided tile fused with the first (pruned) DIT stage,
+// for plans whose first radix R is even and lines of n_in <= L/2 values.
+// That stage's butterfly at LDS positions R b .. R b + R - 1 (span 1) takes
+// the signal samples n' + (L/R) q, q < R, where pos[n'] = R b (the first
+// digit of a position is its own): n' = u, u < L/R, is loaded straight from
+// HBM (rows q < R/2; the rest are zero padding), transformed in registers,
+// and written to LDS once.  Saves the load's LDS round trip and barrier.
+// ct_fetch_first issues the loads (a thread's FirstRows), ct_first_stage
+// transforms and stores them, so a caller may fetch one row while it
+// transforms another.
+template <int L, int JT, typename V> struct FirstRows {
+  static constexpr int R = plan_c(L).rad[0], NB = L / R, NIT = (NB + JT - 1) / JT;
+  V a[NIT][R / 2];  // the non-zero inputs of the thread's butterflies
+};
+template <int L, int JT, typename V>
+__device__ __forceinline__ void ct_fetch_first(FirstRows<L, JT, V>& f, const V* src, long long sp,
+                                               int n_in, int line, int nvalid, int jt) {
+  using F = FirstRows<L, JT, V>;
+  static_assert(F::R % 2 == 0, "fused first stage needs an even first radix");
+#pragma unroll
+  for (int i = 0; i < F::NIT; ++i) {
+    const int u = jt + i * JT;
+#pragma unroll
+    for (int q = 0; q < F::R / 2; ++q) {
+      const int n = u + F::NB * q;
+      f.a[i][q] = czero<V>();
+      if ((F::NB % JT == 0 || u < F::NB) && n < n_in && line < nvalid)
+        f.a[i][q] = src[(long long)n * sp + line];
+    }
+  }
+}
+template <int L, int NL, int JT, int SGN, typename V>
+__device__ __forceinline__ void ct_first_stage(V* buf, const int* pos, FirstRows<L, JT, V>& f,
+                                               int line, int jt) {
+  using F = FirstRows<L, JT, V>;
+#pragma unroll
+  for (int i = 0; i < F::NIT; ++i) {
+    const int u = jt + i * JT;
+    if (F::NB % JT == 0 || u < F::NB) {
+      V b[F::R];
+#pragma unroll
+      for (int q = 0; q < F::R / 2; ++q) b[q] = f.a[i][q];
+      DftHalfIn<F::R, SGN, V>::run(b);
+      V* p = buf + pos[u] * NL + line;
+#pragma unroll
+      for (int q = 0; q < F::R; ++q) p[q * NL] = b[q];
+    }
+  }
+  lds_sync();
+}
+template <int L, int NL, int JT, int SGN, typename V>
+__device__ __forceinline__ void ct_load_first(V* buf, const int* pos, const V* src, long long sp,
+                                              int n_in, int line, int nvalid, int jt) {
+  FirstRows<L, JT, V> f;
+  ct_fetch_first<L, JT>(f, src, sp, n_in, line, nvalid, jt);
+  ct_first_stage<L, NL, JT, SGN>(buf, pos, f, line, jt);
+}
+
+// The last (pruned) DIF stage fused with the global store of rows p < n_out
+// (<= L/2): the butterfly at positions R b + q (span 1) leaves transform
+// index n' + (L/R) q at R b + q, where pos[n'] = R b; its R/2 kept outputs go
+// from registers to HBM.  Saves the store's LDS round trip.  Ends with a
+// barrier, after which the tile may be reused.
+template <int L, int NL, int JT, int SGN, typename V>
+__device__ __forceinline__ void ct_last_store(const V* buf, const int* pos, V* dst, long long sp,
+                                              int n_out, int line, int nvalid, int jt) {
+  constexpr PlanC P = plan_c(L);
+  constexpr int R = P.rad[0], NB = L / R, NIT = (NB + JT - 1) / JT;
+  static_assert(R % 2 == 0, "fused last stage needs an even first radix");
+#pragma unroll
+  for (int i = 0; i < NIT; ++i) {
+    const int u = jt + i * JT;
+    if (NB % JT == 0 || u < NB) {
+      V a[R];
+      const V* p = buf + pos[u] * NL + line;
+#pragma unroll
+      for (int q = 0; q < R; ++q) a[q] = p[q * NL];
+      DftHalfOut<R, SGN, V>::run(a);
+      if (line < nvalid) {
+#pragma unroll
+        for (int q = 0; q < R / 2; ++q) {
+          const int n = u + NB * q;
+          if (n < n_out) dst[(long long)n * sp + line] = a[q];
+        }
+      }
+    }
+  }
+  lds_sync();
 }
 
 }  // namespace ifdda

@@ -136,19 +136,26 @@ __device__ __forceinline__ void finish_sum(KState* st, const double2* part, int 
     red[r][tid] = s;
   }
   __syncthreads();
-  if (tid < 64) {
+  // reduction r on wavefront r mod 4 (all four are full): the same sums in
+  // the same order as on one wavefront, the NR of them side by side; thread
+  // 0 stores them all (the thread that goes on to read them)
+  __shared__ double2 res[NR];
+  const int lane = tid & 63, wv = tid >> 6;
 #pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      double2 s = red[r][tid];
+  for (int r = 0; r < NR; ++r) {
+    if (r % (kThreads / 64) != wv) continue;
+    double2 s = red[r][lane];
 #pragma unroll
-      for (int k = 1; k < kThreads / 64; ++k) s = zadd(s, red[r][tid + 64 * k]);
-      for (int off = 32; off > 0; off >>= 1) {
-        s.x += __shfl_down(s.x, off, 64);
-        s.y += __shfl_down(s.y, off, 64);
-      }
-      if (tid == 0) st->dots[r] = s;
+    for (int k = 1; k < kThreads / 64; ++k) s = zadd(s, red[r][lane + 64 * k]);
+    for (int off = 32; off > 0; off >>= 1) {
+      s.x += __shfl_down(s.x, off, 64);
+      s.y += __shfl_down(s.y, off, 64);
     }
+    if (lane == 0) res[r] = s;
   }
+  __syncthreads();
+  if (tid == 0)
+    for (int r = 0; r < NR; ++r) st->dots[r] = res[r];
   __syncthreads();
 }
 
@@ -180,19 +187,26 @@ __device__ __forceinline__ void finish_sum_any(KState* st, const double2* part, 
 #pragma unroll
   for (int r = 0; r < NR; ++r) red[r][tid] = s[r];
   __syncthreads();
-  if (tid < 64) {
+  // reduction r on wavefront r mod 4 (all four are full): the same sums in
+  // the same order as on one wavefront, the NR of them side by side; thread
+  // 0 stores them all (the thread that goes on to read them)
+  __shared__ double2 res[NR];
+  const int lane = tid & 63, wv = tid >> 6;
 #pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      double2 s = red[r][tid];
+  for (int r = 0; r < NR; ++r) {
+    if (r % (kThreads / 64) != wv) continue;
+    double2 s = red[r][lane];
 #pragma unroll
-      for (int k = 1; k < kThreads / 64; ++k) s = zadd(s, red[r][tid + 64 * k]);
-      for (int off = 32; off > 0; off >>= 1) {
-        s.x += __shfl_down(s.x, off, 64);
-        s.y += __shfl_down(s.y, off, 64);
-      }
-      if (tid == 0) st->dots[r] = s;
+    for (int k = 1; k < kThreads / 64; ++k) s = zadd(s, red[r][lane + 64 * k]);
+    for (int off = 32; off > 0; off >>= 1) {
+      s.x += __shfl_down(s.x, off, 64);
+      s.y += __shfl_down(s.y, off, 64);
     }
+    if (lane == 0) res[r] = s;
   }
+  __syncthreads();
+  if (tid == 0)
+    for (int r = 0; r < NR; ++r) st->dots[r] = res[r];
   __syncthreads();
 }
 

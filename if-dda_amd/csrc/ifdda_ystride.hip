@@ -6,9 +6,10 @@
 // components (NL = 3B lines, contiguous in HBM) of one z plane, the lines
 // along y held in LDS as [point][line]; P2 loads the ny rows into digit-
 // reversed positions and runs decimation in time with its first stage
-// pruned (rows >= L/2 are zero padding), P4 loads all L rows and runs
-// decimation in frequency with its last stage pruned (rows >= ny are not
-// kept).  Here the radix plan, trip counts and LDS offsets are constants and
+// pruned (rows >= L/2 are zero padding; that stage runs on the loaded
+// registers), P4 loads all L rows and runs decimation in frequency with its
+// last stage pruned (rows >= ny are not kept; that stage's outputs go from
+// registers to HBM).  Here the radix plan, trip counts and LDS offsets are constants and
 // each butterfly's twiddles come from the stage-major table at one base
 // address (the stages of the compile-time z pass).  On the 200^3 grid the
 // runtime-plan y pass spends ~1.6x the VALU instructions per line of those
@@ -43,9 +44,9 @@ __global__ void __launch_bounds__(256) k_strided_ct(SArgs<V> a) {
   lds_sync();
   const V* src = zsrc + (long long)o * a.so_src + (long long)kx0 * 3;
   V* dst = zdst + (long long)o * a.so_dst + (long long)kx0 * 3;
-  if constexpr (SGN > 0) {
-    load_strided<true>(buf, pos, src, a.sp_src, ct_fill(L), a.n_in, NL, line, nvalid, jt, JT);
-    ct_dit<L, NL, JT, +1, 0>(buf, tws, line, jt);
+  if constexpr (SGN > 0) {  // the load fused with the first stage
+    ct_load_first<L, NL, JT, +1>(buf, pos, src, a.sp_src, a.n_in, line, nvalid, jt);
+    ct_dit<L, NL, JT, +1, 1>(buf, tws, line, jt);
     if (line < nvalid) {
       V* d = dst + line + (long long)jt * a.sp_dst;
       const long long step = (long long)JT * a.sp_dst;
@@ -53,12 +54,9 @@ __global__ void __launch_bounds__(256) k_strided_ct(SArgs<V> a) {
     }
   } else {
     load_strided<false>(buf, pos, src, a.sp_src, L, a.n_in, NL, line, nvalid, jt, JT);
-    ct_dif<L, NL, JT, -1, P.nst - 1>(buf, tws, line, jt);
-    if (line < nvalid) {
-      V* d = dst + line + (long long)jt * a.sp_dst;
-      const long long step = (long long)JT * a.sp_dst;
-      for (int p = jt; p < a.n_out; p += JT, d += step) *d = buf[pos[p] * NL + line];
-    }
+    // the last stage fused with the store
+    ct_dif<L, NL, JT, -1, P.nst - 1, V, 1>(buf, tws, line, jt);
+    ct_last_store<L, NL, JT, -1>(buf, pos, dst, a.sp_dst, a.n_out, line, nvalid, jt);
   }
 }
 
