@@ -810,24 +810,35 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
         }
     }
     if constexpr (kND > 0) {
-      // one partial per block and inner product, in a fixed order: threads
-      // in LDS, the first wavefront sums four strided values, then shuffles
-      __shared__ double2 red[256];
+      // one partial per block and inner product, in a fixed order: every
+      // thread's value in LDS, then lane l of a full wavefront sums the values
+      // of threads l, l + 64, l + 128, l + 192 and the wavefront shuffles.
+      // The tile is free once every thread has gathered, so the products
+      // are staged there as many at a time as fit (two barriers per group),
+      // and the full wavefronts take one product each.
+      const int nt = (int)blockDim.x, nfull = nt >> 6;
+      const size_t dyn = ((size_t)L * sizeof(V) + (size_t)L * sizeof(int) + 15 & ~(size_t)15) +
+                         (size_t)L * NLS * sizeof(V);
+      const int per = max(1, min(kND, (int)(dyn / (sizeof(double2) * (size_t)nt))));
+      double2* red = reinterpret_cast<double2*>(smem_d4);
       double2* part = f->part + (long long)blockIdx.z * kND * f->pstride + blockIdx.x;
+      const int lane = tid0 & 63, wv = tid0 >> 6;
+      for (int q0 = 0; q0 < kND; q0 += per) {
+        __syncthreads();
 #pragma unroll
-      for (int q = 0; q < kND; ++q) {
+        for (int q = 0; q < kND; ++q)
+          if (q >= q0 && q < q0 + per) red[(q - q0) * nt + tid0] = acc[q];
         __syncthreads();
-        red[tid0] = acc[q];
-        __syncthreads();
-        if (tid0 < 64) {
-          double2 sum = red[tid0];
+        for (int q = q0 + wv; q < min(kND, q0 + per) && wv < nfull; q += nfull) {
+          const double2* rq = red + (q - q0) * nt;
+          double2 sum = rq[lane];
           for (int k = 1; k < 4; ++k)
-            if (tid0 + 64 * k < (int)blockDim.x) sum = zadd(sum, red[tid0 + 64 * k]);
+            if (lane + 64 * k < nt) sum = zadd(sum, rq[lane + 64 * k]);
           for (int off = 32; off > 0; off >>= 1) {
             sum.x += __shfl_down(sum.x, off, 64);
             sum.y += __shfl_down(sum.y, off, 64);
           }
-          if (tid0 == 0) part[(long long)q * f->pstride] = sum;
+          if (lane == 0) part[(long long)q * f->pstride] = sum;
         }
       }
     }
