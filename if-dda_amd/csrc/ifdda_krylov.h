@@ -33,6 +33,7 @@ struct KState {
   int maxit;
   int methode;     // QMRBICGSTAB omega formula
   int ntrace, tracecap;
+  int lapend;      // GPBICG look-ahead: a residual check is due (g_f2xL)
 };
 
 // ---------------------------------------------------------------- complex FP64

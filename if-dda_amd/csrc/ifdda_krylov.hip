@@ -357,9 +357,11 @@ __global__ void __launch_bounds__(kThreads) g_f4xL(KState* st, const double2* pa
   const double2* d = st->dots;
   const double2 ct = zsub(zsub(d[5], zmul(st->eta, d[6])), zmul(st->dzeta, d[7]));
   st->ct_la = ct;
-  // gpbicg.f:308-316: with r0rn = 0 the iteration ends in a breakdown (g_f5L)
+  // gpbicg.f:308-316: with r0rn = 0 the iteration ends in a breakdown (the
+  // check of g_f2xL)
   st->beta_la = zzero(st->r0rn) ? make_double2(0.0, 0.0)
                                 : zdiv(zdiv(zmul(st->alpha, ct), st->dzeta), st->r0rn);
+  st->lapend = 1;
 }
 
 // u, z, x, r updates and the next p; partial |r|^2 (gpbicg.f:270-281,
@@ -394,15 +396,9 @@ __global__ void __launch_bounds__(kThreads) g_k5L(const KState* st, GV<V> v, VN 
   block_partials<1>(acc, part, nblk, bx_);
 }
 
-// g_f5 with <r0*, r_new> and beta from g_f4xL
-__global__ void __launch_bounds__(kThreads) g_f5L(KState* st, const double2* part, int nblk,
-                                                 double2* trace) {
-  const int rh_ = blockIdx.y;  // right-hand side
-  st += rh_;
-  part += (long long)rh_ * kMaxDots * nblk;
-  if (st->done) return;
-  finish_sum<1>(st, part, nblk);
-  if (threadIdx.x != 0) return;
+// The end of a look-ahead iteration (g_f5 with <r0*, r_new> and beta from
+// g_f4xL), on the reduced |r|^2 in st->dots[0], by one thread.
+__device__ __forceinline__ void gp_check_la(KState* st, double2* trace) {
   const double res = sqrt(st->dots[0].x) / st->norm;
   st->residu = res;
   push_trace(st, trace, st->alpha, st->beta, st->eta, st->dzeta, st->r0rn, res);
@@ -426,6 +422,32 @@ __global__ void __launch_bounds__(kThreads) g_f5L(KState* st, const double2* par
     return;
   }
   st->it++;
+}
+
+// The previous iteration's end (gp_check_la on the |r|^2 partials of g_k5L,
+// when g_f4xL left it due) and then alpha from the first gather's <r0*, Ap>
+// partials (g_f2x): one finish kernel per iteration fewer than g_f5L +
+// g_f2x.  The check of iteration k thus runs after iteration k+1's first
+// matvec, which is wasted once per solve; the iterations, matvec counts and
+// results are those of the order g_f5L, g_f2x.
+__global__ void __launch_bounds__(kThreads) g_f2xL(KState* st, const double2* part, int nblk,
+                                                  const double2* part5, int nb5, double2* trace) {
+  const int rh_ = blockIdx.y;
+  st += rh_;
+  if (st->done) return;
+  if (st->lapend) {
+    __shared__ int stop;
+    finish_sum<1>(st, part + (long long)rh_ * kMaxDots * nblk, nblk);
+    if (threadIdx.x == 0) {
+      st->lapend = 0;
+      gp_check_la(st, trace);
+      stop = st->done;
+    }
+    __syncthreads();
+    if (stop) return;
+  }
+  finish_sum_any<1>(st, part5 + (long long)rh_ * nb5, nb5);
+  if (threadIdx.x == 0) gp_alpha(st);
 }
 
 // ======================================================= QMRBICGSTAB kernels
@@ -1944,7 +1966,8 @@ static bool krylov(const MvOp& op, Solver* S, V** vv, int R, int method, const d
       f1.part = S->part5;
       f1.pstride = nb5;
       if (!mvf<V>(op, v.p, v.p, v.Ap, s, done, R, n, dstride, f1)) return false;
-      hipLaunchKernelGGL(g_f2x, F, T, 0, s, S->st, (const double2*)S->part5, nb5);
+      hipLaunchKernelGGL(g_f2xL, F, T, 0, s, S->st, (const double2*)S->part, nb,
+                         (const double2*)S->part5, nb5, trace);
       hipLaunchKernelGGL(g_k3<V>, G, T, 0, s, (const KState*)S->st, v, vn, gp2);
       KFuse<V> f2;
       f2.p5 = 3;
@@ -1955,7 +1978,6 @@ static bool krylov(const MvOp& op, Solver* S, V** vv, int R, int method, const d
       if (!mvf<V>(op, v.t, v.t, v.At, s, done, R, n, dstride, f2)) return false;
       hipLaunchKernelGGL(g_f4xL, F, T, 0, s, S->st, (const double2*)S->part5, nb5, gp2);
       hipLaunchKernelGGL(g_k5L<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
-      hipLaunchKernelGGL(g_f5L, F, T, 0, s, S->st, (const double2*)S->part, nb, trace);
       if (!hip_ok(hipGetLastError(), "gpbicg launch")) return false;
       const int st = step_poll(it);
       if (st < 0) return false;
@@ -2274,6 +2296,7 @@ int solve_core(const MvOp& op, Solver* S, const ifdda_solve_opts* o, int R, cons
   // ax_ok[r]: Ax of RHS r holds the FP64 matvec of its current x (the last
   // refinement check), which is then also the final true residual's
   std::vector<char> ax_ok(R, 0);
+  std::vector<double> e(R), bn(R);  // final |b - A x| and |b|
   bool cancelled = false;
   if (o->precision == 64) {
     std::vector<double> tols(R, o->tol);
@@ -2313,6 +2336,8 @@ int solve_core(const MvOp& op, Solver* S, const ifdda_solve_opts* o, int R, cons
       int nact = 0;
       for (int r = 0; r < R; ++r) {
         if (!active[r]) continue;
+        e[r] = rn[r];  // the final residual too, if x stays as it is
+        bn[r] = bnorm[r];
         rho[r] = bnorm[r] > 0.0 ? rn[r] / bnorm[r] : 0.0;
         outer[r] = sweep;
         if (rho[r] <= o->tol || sweep >= 30) {
@@ -2373,14 +2398,18 @@ int solve_core(const MvOp& op, Solver* S, const ifdda_solve_opts* o, int R, cons
   // Final true residual with the FP64 matvec (inverserig.f:95-105); after
   // refinement the last check already holds it for every RHS.
   const bool final_mv = std::find(ax_ok.begin(), ax_ok.end(), 0) != ax_ok.end();
-  if (final_mv && !mv<double2>(op, x, x, Ax, s, nullptr, R, n, 0)) return IFDDA_ERR_HIP;
-  hipLaunchKernelGGL(k_resid<double2>, G, T, 0, s, (const double2*)Ax, (const double2*)b, vn,
-                     S->part, nb);
-  std::vector<double> e(R), bn(R);
-  if (!resid_norms(S, R, e.data(), s)) return IFDDA_ERR_HIP;
-  hipLaunchKernelGGL(k_init<double2>, G, T, 0, s, (const double2*)b, Ax, (const double2*)Ax, vn,
-                     S->part, nb);
-  if (!resid_norms(S, R, bn.data(), s)) return IFDDA_ERR_HIP;
+  // Mixed precision with x unmoved since every right-hand side's last
+  // refinement check: that check's FP64 residual |b - A x| and |b| are the
+  // final ones (same matvec, same sums).  Else the true residual is formed.
+  if (o->precision != 32 || final_mv) {
+    if (final_mv && !mv<double2>(op, x, x, Ax, s, nullptr, R, n, 0)) return IFDDA_ERR_HIP;
+    hipLaunchKernelGGL(k_resid<double2>, G, T, 0, s, (const double2*)Ax, (const double2*)b, vn,
+                       S->part, nb);
+    if (!resid_norms(S, R, e.data(), s)) return IFDDA_ERR_HIP;
+    hipLaunchKernelGGL(k_init<double2>, G, T, 0, s, (const double2*)b, Ax, (const double2*)Ax, vn,
+                       S->part, nb);
+    if (!resid_norms(S, R, bn.data(), s)) return IFDDA_ERR_HIP;
+  }
   if (FFloc &&
       !hip_ok(hipMemcpyAsync(FFloc, x, sizeof(double2) * S->total, hipMemcpyDeviceToDevice, s),
               "copy E"))
