@@ -12,11 +12,13 @@ nquicklens = 1) and their pieces:
   image_assembly      pupil fill, lens rotation, |E|^2 sums    microsbf.f:676-845,
                       (dark=True)                               microsdf.f:655-790
   direct_geometry     pupil step of nquicklens = 0             microsbf.f:139-146
-                      (parity unpinned, as the direct=True steps of
-                      incidences: microsbf.f needs HDF5 and is not built,
-                      and no fixture holds these steps)
   lens_field_direct   pupil field as a direct sum (nquicklens  microsbf.f:629-668,
                       = 0)                                      microssansinc.f:600-640
+
+The pieces are pinned one by one to the compiled reference and its
+fixtures, and the whole pipeline (these functions around the oracle's
+inverserig) to the reference's own microsbf_, microsdf_, microsbfneg_ and
+microsdfneg_, quick and direct-sum lens (tests/golden/microscopes.npz).
 
 Only tests/ imports this module.
 """

@@ -470,6 +470,71 @@ class Ref:
         E = np.transpose(Ep[:, :n, :n], (2, 1, 0)).copy()
         return E, imax.value, dkx.value, nstop.value, info.raw.decode(errors="replace").rstrip()
 
+    MICROSCOPES = ("microsbf", "microsdf", "microsbfneg", "microsdfneg")
+
+    def microscope(self, kind, G, pos, polarisa, d, k0, nfft2d, numaper, numaperinc, P0, w0, gross,
+                   zlens, nquicklens, method="GPBICG1", tol=1e-4):
+        """One of the four whole microscopes, run as cdmlib.f:3674-3800 calls
+        them (numaper scaled by k0, nrig = 0, nproche = -1 so each incidence
+        goes through inverserig, nmat = 1 so no file is written, npolainc = 0:
+        both polarisations): microsbf_ (microsbf.f:1-12), microsdf_
+        (microsdf.f:1-12), and the nside = -1 microsbfneg_ / microsdfneg_
+        (microssansinc.f:854-863, 1-10), which have no with-incident image
+        arguments.  The box is dense: every cell a dipole (Tabdip(i) = i,
+        nbsphere = nx ny nz), empty cells with zero polarisability, pos the
+        box-ordered cell centres.  Returns the six images [scattered x, y, z,
+        with-incident x, y, z][y][x] (the square roots of the summed
+        intensities; the nside = -1 kinds repeat the scattered image), kxy,
+        xy, P0 after the call (P0 / number of illuminations), irra, nstop,
+        infostr."""
+        assert kind in self.MICROSCOPES
+        neg = kind.endswith("neg")
+        nside = -1 if neg else 1
+        nz2, ny2, nx2 = G.shape[1:]
+        nx, ny, nz = nx2 // 2, ny2 // 2, nz2 // 2
+        M = nx2 * ny2 * nz2
+        nmax = nx * ny * nz
+        assert pos.shape == (nmax, 3) and polarisa.shape == (nmax, 3, 3)
+        Gs = [np.ascontiguousarray(G[c]).reshape(-1) for c in range(6)]
+        V = self._vect(M)
+        tabfft2 = np.zeros(nfft2d, np.int32)
+        tab = np.arange(1, nmax + 1, dtype=np.int32)
+        XI, XR, FF, FF0, FFloc = (np.zeros(3 * nmax, np.complex128) for _ in range(5))
+        wrk = np.zeros(3 * nmax * 12, np.complex128)
+        pol_f = np.asfortranarray(np.asarray(polarisa, np.complex128))
+        eps_f = np.zeros((nmax, 3, 3), np.complex128, order="F")
+        xs, ys, zs = (np.ascontiguousarray(pos[:, a], np.float64) for a in range(3))
+        planes = [np.zeros(nfft2d * nfft2d, np.complex128) for _ in range(12)]
+        Ekp = np.zeros(3 * nfft2d * nfft2d, np.complex128)
+        Ekn = np.zeros(3 * nfft2d * nfft2d, np.complex128)
+        kxy = np.zeros(nfft2d, np.float64)
+        xy = np.zeros(nfft2d, np.float64)
+        meth = method.ljust(12).encode()
+        info = C.create_string_buffer(b" " * 64, 64)
+        tol1, p0, irra = C.c_double(0), C.c_double(float(P0)), C.c_double(0)
+        nloop, ncompte, nstop = C.c_int(0), C.c_int(0), C.c_int(0)
+        file_id, group_id = C.c_int(0), C.c_int(0)  # integer(hid_t), hid_t = 4 without HDF5
+        P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+        # Eimagex..z, Efourierx..z, Eimageincx..z, Efourierincx..z in argument order
+        Eim, Efo, Eiminc, Efoinc = planes[0:3], planes[3:6], planes[6:9], planes[9:12]
+        inc_args = [] if neg else [*[P(a) for a in Eiminc], *[P(a) for a in Efoinc]]
+        args = [*[P(g) for g in Gs], *[P(v) for v in V], _ri(M), _ri(M), _ri(3 * nmax), _ri(12),
+                _ri(nmax), _ri(nx), _ri(ny), _ri(nz), _ri(nx), _ri(ny), _ri(nz), _ri(nx2), _ri(ny2),
+                _ri(nx2 * ny2), _ri(nz2), _ri(nmax), _ri(nmax), _ri(3 * nmax), _ri(-1), _ri(0),
+                _ri(nfft2d), P(tabfft2), P(tab), P(XI), P(XR), P(wrk), P(FF), P(FF0), P(FFloc),
+                P(pol_f), P(eps_f), C.c_char_p(meth), _rd(tol), C.byref(tol1), C.byref(nloop),
+                C.byref(ncompte), P(xs), P(ys), P(zs), _rd(d), _rd(numaper * k0),
+                _rd(numaperinc), _ri(0), _ri(nquicklens), _rd(1.0), _rd(k0), C.byref(p0),
+                C.byref(irra), _rd(w0), _rd(gross), _rd(zlens), *[P(a) for a in Eim],
+                *[P(a) for a in Efo], *inc_args, P(Ekp), P(Ekn), P(kxy), P(xy), _ri(nside),
+                _ri8(), _ri8(), _ri8(), _ri8(), _ri(1), C.byref(file_id), C.byref(group_id),
+                C.byref(nstop), info, C.c_size_t(12), C.c_size_t(64)]
+        getattr(self.lib, kind + "_")(*args)
+        self.flush_stdout()
+        img = np.stack([a.reshape(nfft2d, nfft2d) for a in (*Eim, *(Eim if neg else Eiminc))])
+        return dict(images=img, kxy=kxy, xy=xy, P0=p0.value, irra=irra.value, nstop=nstop.value,
+                    infostr=info.raw.decode(errors="replace").rstrip())
+
     def fouriertoimage(self, dk, gross, planes):
         """fouriertoimage_ on six N x N planes ([y][x]): scattered x, y, z then
         with-incident x, y, z."""

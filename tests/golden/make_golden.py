@@ -38,6 +38,15 @@ Fixtures (inputs and the reference's outputs, nothing else):
                    dipole, alpha = 0 outside a sphere) on a 12 x 10 x 9 box for
                    GPBICG1, QMRBICGSTAB1 and TFQMR at tol 1e-4 and 1e-6: E
                    (FFloc), FF, tol1, nloop, ncompte (`make_golden.py opt`)
+  microscopes.npz  the four whole microscopes microsbf_, microsdf_,
+                   microsbfneg_, microsdfneg_ (quick lens nquicklens = 1 and
+                   direct sum nquicklens = 0; darkfield at NA 0.8 / condenser
+                   0.5 and 0.5 / 0.8) on a 120 nm sphere (6^3 cells,
+                   eps 2.25+0.1i, RR) at nfft2d 128, GPBICG1 tol 1e-10: per case
+                   the number of illuminations, the E0 they carry, and each
+                   image plane as 16 projections on seeded real weights
+                   (micro_weights), its values at 64 seeded pixels, its max
+                   and its sum (`make_golden.py micro`)
   farfield.npz     diffractefft2d_ and computegcfft2d_: a seeded box-ordered
                    FF on 6x5x4 at nfft2d 64 and 65 (odd), and the config-1
                    GPBICG1 moments (alpha E) at nfft2d 128; Epos / Eneg cut to
@@ -54,6 +63,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 sys.path.insert(0, os.path.join(ROOT, "if-dda_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 from oracle import Ref  # noqa: E402
 from ifdda import scene  # noqa: E402
@@ -236,6 +246,38 @@ def force(r):
     np.savez_compressed(os.path.join(HERE, "force.npz"), **out)
 
 
+def microscopes(r):
+    from micro_pipeline import (MICRO_CASES, MICRO_GROSS, MICRO_NFFT, MICRO_TOL, MICRO_W0,
+                                MICRO_ZLENS, micro_key, micro_pixels, micro_scene, micro_weights)
+    s, box = micro_scene()
+    alpha = r.poladiff(s.d, 2.25 + 0.1j, K0, "RR")
+    nmax = s.nx * s.ny * s.nz
+    pol = np.zeros((nmax, 3, 3), np.complex128)
+    for a in range(3):
+        pol[s.tabdip > 0, a, a] = alpha
+    G = r.green_fft(s.nx, s.ny, s.nz, s.d, K0)
+    N2 = MICRO_NFFT * MICRO_NFFT
+    W = micro_weights(N2)
+    pix = micro_pixels(N2)
+    out = dict(alpha=alpha, pixels=pix, nfft2d=MICRO_NFFT, w0=MICRO_W0, gross=MICRO_GROSS,
+               zlens=MICRO_ZLENS, tol=MICRO_TOL, k0=K0)
+    for kind, na, nainc, nq in MICRO_CASES:
+        zl = -MICRO_ZLENS if kind.endswith("neg") else MICRO_ZLENS
+        o = r.microscope(kind, G, box, pol, s.d, K0, MICRO_NFFT, na, nainc, 1.0, MICRO_W0,
+                         MICRO_GROSS, zl, nq, "GPBICG1", MICRO_TOL)
+        assert o["nstop"] == 0, o["infostr"]
+        img = o["images"].reshape(6, N2)
+        assert np.abs(img.imag).max() == 0.0
+        img = img.real
+        E0, _ = r.irradiance(o["P0"], MICRO_W0)
+        k = micro_key(kind, na, nainc, nq)
+        out.update({f"{k}_nillum": int(round(1.0 / o["P0"])), f"{k}_E0": E0,
+                    f"{k}_proj": img @ W.T, f"{k}_pix": img[:, pix], f"{k}_max": img.max(axis=1),
+                    f"{k}_sum": img.sum(axis=1)})
+        print(k, "illuminations", out[f"{k}_nillum"], "max", img.max(axis=1))
+    np.savez_compressed(os.path.join(HERE, "microscopes.npz"), **out)
+
+
 def opt_case():
     """The dense-box problem of solve_opt.npz: a sphere of radius 100 nm
     (eps 4+0.5i, RR) centred in a 12 x 10 x 9 box of 20 nm cells; every box
@@ -278,6 +320,8 @@ if __name__ == "__main__":
         solve_opt(Ref())
     elif len(sys.argv) > 1 and sys.argv[1] == "more":
         solve_more(Ref())
+    elif len(sys.argv) > 1 and sys.argv[1] == "micro":
+        microscopes(Ref())
     elif len(sys.argv) > 1 and sys.argv[1] == "force":
         force(Ref())
     elif len(sys.argv) > 1 and sys.argv[1] == "gp":

@@ -1,9 +1,10 @@
-"""Brightfield and darkfield microscopes (microsbf.f, microsdf.f, quick lens)
-on the GPU against an independent CPU pipeline: the oracle's inverserig per
-incidence, the numpy restatements of diffractefft2dlens / ondeplanekxky /
-the image assembly of microsbf.f and microsdf.f (whose pieces are pinned to
-the compiled reference in test_microscopy_oracle.py).  Images agree to the
-solver tolerance."""
+"""Brightfield and darkfield microscopes (microsbf.f, microsdf.f,
+microssansinc.f; quick and direct-sum lens) on the GPU against the compiled
+reference's own whole microscopes (golden/microscopes.npz) and against an
+independent CPU pipeline: the oracle's inverserig per incidence, the numpy
+restatements of diffractefft2dlens / ondeplanekxky / the image assembly
+(tests/micro_pipeline.py, itself checked against the same fixture in
+test_microscopy_oracle.py).  Images agree to the solver tolerance."""
 import numpy as np
 import pytest
 
@@ -40,40 +41,8 @@ def test_incidence_lattice(small):
 
 
 def _cpu_images(s, alpha, oracle, m, tol, dark=False):
-    import microscopy as mi
-    direct = bool(m.direct_lens)
-    if dark:
-        ring, dkinc, _ = mi.incidences_df(K0, m.numaperinc, s.d, m.nfft2d, s.pos[:, 0], s.pos[:, 1],
-                                          direct=direct)
-        waves = [(idl, 0, kx, ky, pp) for (idl, pp, kx, ky) in ring]
-    else:
-        inc, dkinc, _ = mi.incidences(K0, m.numaperinc, s.d, m.nfft2d, s.pos[:, 0], s.pos[:, 1],
-                                      direct=direct)
-        waves = [(idl, jdl, kx, ky, pp) for pp in (0.0, 1.0) for (idl, jdl, kx, ky) in inc]
-    G = oracle.green_fft(s.nx, s.ny, s.nz, (2 * s.nx, 2 * s.ny, 2 * s.nz), s.d, K0)
-    pol = np.zeros((s.nx * s.ny * s.nz, 3, 3), np.complex128)
-    for a in range(3):
-        pol[: s.nd, a, a] = alpha
-    E0 = complex(m.E0[0], m.E0[1])
-    zs = s.origin[2] + s.d * np.arange(s.nz)
-    cells = np.nonzero(s.tabdip)[0]
-    lens = []
-    for (idl, jdl, kx, ky, pp) in waves:
-        Einc = mi.ondeplanekxky(s.pos, K0, E0, pp, kx, ky)
-        o = oracle.inverserig(G, s.tabdip, s.nx, s.ny, s.nz, pol, Einc, "GPBICG1", tol)
-        p = (alpha[:, None] * o["FFloc"].reshape(-1, 3))
-        FFbox = np.zeros((s.nx * s.ny * s.nz, 3), np.complex128)
-        FFbox[cells] = p[s.tabdip[cells] - 1]
-        if direct:  # nquicklens = 0: the pupil field as the direct sum over dipoles
-            imax, dk = mi.direct_geometry(m.nfft2d, K0, s.d)
-            El = mi.lens_field_direct(s.pos, p, K0, dk, imax, m.numaper * K0, m.nside)
-        else:
-            El, imax, dk = mi.diffractefft2dlens(FFbox.reshape(-1), s.nx, s.ny, s.nz, m.nfft2d, K0,
-                                                 s.origin[0], s.origin[1], zs, s.d, m.numaper * K0,
-                                                 m.nside)
-        lens.append(El)
-    return mi.image_assembly(lens, waves, K0, E0, m.numaper * K0, m.zlens, m.gross, dkinc, imax, dk,
-                             m.nfft2d, dark=dark, nside=m.nside)
+    from micro_pipeline import cpu_images
+    return cpu_images(s, alpha, oracle, m, tol, dark=dark)
 
 
 def test_brightfield_images_match_cpu_pipeline(small, oracle):
@@ -223,3 +192,46 @@ def test_given_incidences_equal_the_condenser_lattice(small):
     imf, inff = sv.microsbf(mf, tol=1e-8)
     assert len(inff) == 4 and all(i["nstop"] == 0 for i in inff)
     assert np.allclose(imf[3:], imf[:3], rtol=1e-12, atol=0)
+
+
+@pytest.fixture(scope="module")
+def golden_sphere(gpu):
+    from conftest import golden
+    from micro_pipeline import micro_scene
+    fx = golden("microscopes.npz")
+    s, _ = micro_scene()
+    sv = gpu.Solver(s.nx, s.ny, s.nz)
+    sv.greencalculfft(s.d, K0)
+    sv.set_dipoles(s.tabdip)
+    sv.set_polarizability(np.full(s.nd, complex(fx["alpha"])))
+    yield s, sv, fx
+    sv.close()
+
+
+def _golden_cases():
+    from micro_pipeline import MICRO_CASES, micro_key
+    return [pytest.param(c, id=micro_key(*c)) for c in MICRO_CASES]
+
+
+@pytest.mark.parametrize("case", _golden_cases())
+def test_microscopes_match_reference_fixture(golden_sphere, case):
+    """The device microscope against the compiled reference's microsbf_,
+    microsdf_, microsbfneg_, microsdfneg_ (quick lens nquicklens = 1 and the
+    direct sum nquicklens = 0) on the same sphere and incident power: the
+    number of illuminations, and each image plane at 64 seeded pixels, its
+    projections, max and sum to 1e-8 of the plane's max (both solves at tol
+    1e-10)."""
+    from micro_pipeline import MICRO_GROSS, MICRO_NFFT, MICRO_TOL, case_geometry, check_images, micro_key
+    s, sv, fx = golden_sphere
+    kind, na, nainc, nq = case
+    key = micro_key(*case)
+    dark, nside, zlens, direct = case_geometry(*case)
+    E0 = complex(fx[f"{key}_E0"])
+    m = sv.microscope_opts(MICRO_NFFT, na, nainc, K0, s.d, s.origin, gross=MICRO_GROSS, zlens=zlens,
+                           nside=nside, npolainc=0, E0=E0, direct_lens=direct)
+    print(f"reference microscope {key}", flush=True)
+    kxy, _ = sv.microscope_incidences(m, dark=dark)
+    assert kxy.shape[0] == int(fx[f"{key}_nillum"])  # both polarisations of every incidence
+    img, infos = sv.microsbf(m, "GPBICG1", MICRO_TOL, precision=64, dark=dark)
+    assert all(i["nstop"] == 0 and i["tol1"] <= MICRO_TOL for i in infos)
+    check_images(img, fx, key, 1e-8)

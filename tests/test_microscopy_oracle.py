@@ -143,3 +143,73 @@ def test_image_assembly_of_the_bare_incident_wave(dark):
         for c in range(3):
             assert np.abs(img[c]).max() == 0.0
             assert np.allclose(img[3 + c], abs(e[c]) / gross, rtol=1e-12, atol=1e-300)
+
+
+def _fixture_case(case, oracle, tol=1e-10):
+    from types import SimpleNamespace
+
+    import micro_pipeline as mp
+    fx = golden("microscopes.npz")
+    s, _ = mp.micro_scene()
+    key = mp.micro_key(*case)
+    dark, nside, zlens, direct = mp.case_geometry(*case)
+    E0 = complex(fx[f"{key}_E0"])
+    m = SimpleNamespace(direct_lens=direct, numaperinc=case[2], numaper=case[1], nfft2d=mp.MICRO_NFFT,
+                        E0=(E0.real, E0.imag), zlens=zlens, gross=mp.MICRO_GROSS, nside=nside)
+    img = mp.cpu_images(s, np.full(s.nd, complex(fx["alpha"])), oracle, m, tol, dark=dark)
+    return img, fx, key
+
+
+def _cpu_fixture_cases():
+    from micro_pipeline import MICRO_CASES, micro_key
+    # the direct-sum lens (nquicklens = 0) in every microscope, and the quick
+    # lens in the two brightfield ones (the darkfield quick-lens cases run on
+    # the GPU side only, to keep this suite short)
+    keep = [c for c in MICRO_CASES if c[3] == 0 or c[0].startswith("microsbf")]
+    return [pytest.param(c, id=micro_key(*c)) for c in keep]
+
+
+@pytest.mark.parametrize("case", _cpu_fixture_cases())
+def test_pipeline_matches_reference_microscopes(oracle, case):
+    """The whole CPU pipeline (oracle inverserig, incidences / incidences_df,
+    direct_geometry, lens_field_direct or diffractefft2dlens, image_assembly)
+    against the compiled reference's microsbf_ / microsdf_ / microsbfneg_ /
+    microsdfneg_ (golden/microscopes.npz): images to 1e-8 of each plane's
+    max, both solved to 1e-10."""
+    from micro_pipeline import check_images
+    img, fx, key = _fixture_case(case, oracle)
+    check_images(img, fx, key, 1e-8)
+
+
+def test_fixture_catches_planted_mistakes(oracle, monkeypatch):
+    """Mistakes planted in the oracle's direct-lens steps (CPU only) make the
+    direct-lens brightfield case fail the fixture: the pupil step's
+    threshold (imaxk0 > 4 instead of > 5), the phase sign of the dipole sum,
+    and the nside = +1 lens written with n_z of the other side."""
+    import microscopy as mi
+    from micro_pipeline import check_images
+    case = ("microsbf", 0.8, 0.5, 0)
+    geo, lens = mi.direct_geometry, mi.lens_field_direct
+
+    def geo_threshold(nfft2d, k0, d):
+        k = 0
+        while True:
+            dkx = 2.0 * np.pi / (nfft2d * d) / 2.0 ** k
+            if mi.nint(k0 / dkx) + 1 > 4:
+                return mi.nint(k0 / dkx) + 1, dkx
+            k += 1
+
+    def lens_phase(pos, FF, k0, dk, imax, numaper, nside=1):
+        return np.conj(lens(pos, np.conj(FF), k0, dk, imax, numaper, nside))
+
+    def lens_side(pos, FF, k0, dk, imax, numaper, nside=1):
+        return lens(pos, FF, k0, dk, imax, numaper, -nside)
+
+    for name, fn in (("direct_geometry", geo_threshold), ("lens_field_direct", lens_phase),
+                     ("lens_field_direct", lens_side)):
+        with monkeypatch.context() as mp:
+            mp.setattr(mi, name, fn)
+            img, fx, key = _fixture_case(case, oracle)
+        with pytest.raises(AssertionError):
+            check_images(img, fx, key, 1e-8)
+    assert mi.direct_geometry is geo and mi.lens_field_direct is lens
