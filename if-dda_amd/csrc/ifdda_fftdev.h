@@ -278,15 +278,17 @@ template <typename V> __device__ __forceinline__ V* tile_base(V* tw, int L) {
   const size_t off = ((size_t)L * sizeof(V) + (size_t)L * sizeof(int) + 15) & ~(size_t)15;
   return reinterpret_cast<V*>(reinterpret_cast<char*>(tw) + off);
 }
-// Row stride (in values) of the matvec x passes' LDS tile of NL lines: one
-// value of padding for wide tiles (16 lines or more: batched launches), so
-// that the rows a wavefront's butterflies touch in one access do not all
-// start on the same bank (a 24-line FP32 tile, 192 B rows, put every row of
-// a radix-8 butterfly on one bank group: 44% of LDS cycles in conflict).  On
-// the 256-RHS batch (128^3) P1 went from 1299 to 1119 us and the fused P5
-// from 1394 / 2051 to 1198 / 1860 us.  Narrow single-RHS tiles (B <= 5)
-// keep NL: padded, they measured no faster.  Other tiles: NL.
-__host__ __device__ constexpr int x_stride(int NL) { return NL >= 16 ? NL + 1 : NL; }
+// Row stride (in values) of the matvec x passes' LDS tile of NL lines.  The
+// scatter and gather phases put a wavefront's 64 lanes on 64 different rows
+// (permuted positions, same line), so the rows' bank offsets are multiples of
+// the stride: an even stride of FP32 values (8 B, two banks) shares its
+// factor 2^k with the bank count and leaves 64 / 2^k distinct offsets, i.e.
+// 2^k-way conflicts.  Wide tiles (16 lines or more: batched launches) are
+// therefore given an odd stride, one value of padding when NL is even.  A
+// 24-line FP32 tile unpadded spent 44% of its LDS cycles in conflict; the
+// 21-line tile of the 256-RHS batch (128^3) padded to 22 still spent 39%.
+// Narrow single-RHS tiles (B <= 5) keep NL: padded, they measured no faster.
+__host__ __device__ constexpr int x_stride(int NL) { return NL >= 16 && NL % 2 == 0 ? NL + 1 : NL; }
 template <typename V> size_t tile_lds_bytes(int L, int NL) {
   return (((size_t)L * sizeof(V) + (size_t)L * sizeof(int) + 15) & ~(size_t)15) +
          (size_t)L * NL * sizeof(V);
