@@ -197,14 +197,15 @@ extern "C" int ifdda_green_build(ifdda_ctx* c, double aretecube, double k0, int 
   if (!(aretecube > 0.0) || !(k0 > 0.0)) return arg_error("ifdda_green_build: need d > 0, k0 > 0");
   if (nquad < 0 || nquad > 10) return arg_error("ifdda_green_build: nquad must be in [0,10]");
   const size_t M = (size_t)c->g.L[0] * c->g.L[1] * c->g.L[2];
-  double2* full = green_full(c, M);
+  const int nc = green_build_comps(c->g);
+  double2* full = green_full(c, nc * M);
   if (!full) return IFDDA_ERR_HIP;
   const double2* tw[3] = {c->tw64[0], c->tw64[1], c->tw64[2]};
   const int* pos[3] = {c->pos[0], c->pos[1], c->pos[2]};
   c->green_ready = false;
   c->gd_ready = false;
   bool ok = green_build_device(c->g, tw, pos, aretecube, k0, nquad, full, c->G64, c->G32,
-                               &c->gscr, c->stream);
+                               &c->gscr, c->stream, nc);
   ok = ok && hip_ok(hipStreamSynchronize(c->stream), "green build");
   if (!ok) return IFDDA_ERR_HIP;
   c->green_ready = true;

@@ -1088,6 +1088,39 @@ __global__ void k_green_fill(double2* __restrict__ out, int comp, int nx, int ny
   }
 }
 
+// All six components of the circulant fill at once, component c at out +
+// c M: one T evaluation per grid point where k_green_fill makes six, and one
+// launch where it takes six.  The values are k_green_fill's.
+__global__ void k_green_fill6(double2* __restrict__ out, int nx, int ny, int nz, int L1, int L2,
+                              int L3, double d, double k0, const double2* __restrict__ near,
+                              int nquad) {
+  const long long M = (long long)L1 * L2 * L3;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < M;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int ii = idx % L1;
+    const int jj = (idx / L1) % L2;
+    const int kk = idx / ((long long)L1 * L2);
+    bool okx, oky, okz;
+    const int i = circ_off(ii, nx, L1, okx), j = circ_off(jj, ny, L2, oky),
+              k = circ_off(kk, nz, L3, okz);
+    double2 T[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) T[c] = make_double2(0.0, 0.0);
+    if (okx && oky && okz) {
+      if (nquad != 0 && i * i + j * j + k * k <= nquad * nquad) {
+        const int w = 2 * nquad + 1;
+        const double2* nv = near + (((k + nquad) * w + (j + nquad)) * w + (i + nquad)) * 6;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) T[c] = nv[c];
+      } else {
+        green_T(-(double)i * d, -(double)j * d, -(double)k * d, k0, T);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) out[c * M + idx] = T[c];
+  }
+}
+
 // One component of d T / d r_dd on the circulant grid at r = +offset * d
 // (derivativefield2.f:22-60: the observation minus the source, point values
 // only, zero on the planes at index n).
@@ -1209,6 +1242,27 @@ __global__ void k_green_fold(double2* __restrict__ g64, float2* __restrict__ g32
     const double2 v = full[((long long)kz * L2 + ky) * L1 + kx];
     g64[t * 6 + comp] = v;
     if (g32) g32[t * 6 + comp] = make_float2((float)v.x, (float)v.y);
+  }
+}
+
+// Fold all six FFT'd components (component c at full + c M) into the octant
+// arrays: the six values of an octant point are written side by side.
+__global__ void k_green_fold6(double2* __restrict__ g64, float2* __restrict__ g32,
+                              const double2* __restrict__ full, long long M, int L1, int L2,
+                              int F1, int F2, int F3) {
+  const long long n = (long long)F1 * F2 * F3;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int kx = t % F1;
+    const int kz = (t / F1) % F3;
+    const int ky = t / ((long long)F1 * F3);
+    const long long src = ((long long)kz * L2 + ky) * L1 + kx;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const double2 v = full[c * M + src];
+      g64[t * 6 + c] = v;
+      if (g32) g32[t * 6 + c] = make_float2((float)v.x, (float)v.y);
+    }
   }
 }
 
@@ -1886,13 +1940,17 @@ bool launch_matvec_fused_f32(const GridPlan& g, const MatvecDev<float2>& m, cons
 }
 
 // In-place forward 3D FFT (sign +1) of one FP64 component on the full grid.
+// In-place forward 3D FFT of nc full grids M = L1 L2 L3 values apart (the
+// grids are the launches' right-hand sides).
 static bool fft3d_full_f64(const GridPlan& g, const double2* tw[3], const int* pos[3], double2* v,
-                           hipStream_t s) {
+                           hipStream_t s, int nc = 1) {
   const int L1 = g.L[0], L2 = g.L[1], L3 = g.L[2];
+  const long long M = (long long)L1 * L2 * L3;
   {
     XArgs<double2> a{};
     a.src = v;
     a.dst = v;
+    a.zs_src = a.zs_dst = M;
     a.tw = tw[0];
     a.pos = pos[0];
     a.ax = g.ax[0];
@@ -1902,12 +1960,14 @@ static bool fft3d_full_f64(const GridPlan& g, const double2* tw[3], const int* p
     a.ld_src = a.ld_dst = L1;
     auto k = k_xpass<double2, 1, +1, 0>;
     if (!allow_lds(k, g.gx.lds)) return false;
-    hipLaunchKernelGGL(k, dim3((a.nlines + a.B - 1) / a.B), dim3(g.gx.threads), g.gx.lds, s, a);
+    hipLaunchKernelGGL(k, dim3((a.nlines + a.B - 1) / a.B, 1, nc), dim3(g.gx.threads), g.gx.lds, s,
+                       a);
   }
   {
     SArgs<double2> a{};
     a.src = v;
     a.dst = v;
+    a.zs_src = a.zs_dst = M;
     a.tw = tw[1];
     a.pos = pos[1];
     a.ax = g.ax[1];
@@ -1918,12 +1978,13 @@ static bool fft3d_full_f64(const GridPlan& g, const double2* tw[3], const int* p
     a.sp_src = a.sp_dst = L1;
     auto k = k_strided<double2, 1, +1>;
     if (!allow_lds(k, g.gy.lds)) return false;
-    hipLaunchKernelGGL(k, dim3((L1 + a.B - 1) / a.B, L3), dim3(g.gy.threads), g.gy.lds, s, a);
+    hipLaunchKernelGGL(k, dim3((L1 + a.B - 1) / a.B, L3, nc), dim3(g.gy.threads), g.gy.lds, s, a);
   }
   {
     SArgs<double2> a{};
     a.src = v;
     a.dst = v;
+    a.zs_src = a.zs_dst = M;
     a.tw = tw[2];
     a.pos = pos[2];
     a.ax = g.ax[2];
@@ -1934,9 +1995,14 @@ static bool fft3d_full_f64(const GridPlan& g, const double2* tw[3], const int* p
     a.sp_src = a.sp_dst = (long long)L1 * L2;
     auto k = k_strided<double2, 1, +1>;
     if (!allow_lds(k, g.gz.lds)) return false;
-    hipLaunchKernelGGL(k, dim3((L1 + a.B - 1) / a.B, L2), dim3(g.gz.threads), g.gz.lds, s, a);
+    hipLaunchKernelGGL(k, dim3((L1 + a.B - 1) / a.B, L2, nc), dim3(g.gz.threads), g.gz.lds, s, a);
   }
   return hip_ok(hipGetLastError(), "fft3d launch");
+}
+
+int green_build_comps(const GridPlan& g) {
+  const size_t M = (size_t)g.L[0] * g.L[1] * g.L[2];
+  return 6 * M * sizeof(double2) <= ((size_t)512 << 20) ? 6 : 1;
 }
 
 bool green_fold_component(const GridPlan& g, const double2* full, int comp, double2* g64,
@@ -1991,7 +2057,7 @@ static bool scratch_fit(T** p, size_t* cap, size_t bytes, hipStream_t s, const c
 
 bool green_build_device(const GridPlan& g, const double2* tw64[3], const int* pos[3], double d,
                         double k0, int nquad, double2* full, double2* g64, float2* g32,
-                        GreenScratch* gs, hipStream_t s) {
+                        GreenScratch* gs, hipStream_t s, int ncomp) {
   const long long M = (long long)g.L[0] * g.L[1] * g.L[2];
   double2* near = nullptr;
   bool ok = true;
@@ -2027,8 +2093,19 @@ bool green_build_device(const GridPlan& g, const double2* tw64[3], const int* po
            hip_ok(hipStreamSynchronize(s), "green near");  // h (copied from) ends here
     }
   }
+  const int blocks = (int)std::min<long long>((M + 255) / 256, 65535);
+  if (ok && ncomp == 6) {  // five launches for the six components (full holds 6 M)
+    const long long nf = (long long)g.F[0] * g.F[1] * g.F[2];
+    hipLaunchKernelGGL(k_green_fill6, dim3(blocks), dim3(256), 0, s, full, g.nx, g.ny, g.nz,
+                       g.L[0], g.L[1], g.L[2], d, k0, near, nquad);
+    ok = hip_ok(hipGetLastError(), "green fill") && fft3d_full_f64(g, tw64, pos, full, s, 6);
+    if (ok)
+      hipLaunchKernelGGL(k_green_fold6, dim3((int)std::min<long long>((nf + 255) / 256, 65535)),
+                         dim3(256), 0, s, g64, g32, full, M, g.L[0], g.L[1], g.F[0], g.F[1],
+                         g.F[2]);
+    return ok && hip_ok(hipGetLastError(), "green fold");
+  }
   for (int c = 0; ok && c < 6; ++c) {
-    const int blocks = (int)std::min<long long>((M + 255) / 256, 65535);
     hipLaunchKernelGGL(k_green_fill, dim3(blocks), dim3(256), 0, s, full, c, g.nx, g.ny, g.nz,
                        g.L[0], g.L[1], g.L[2], d, k0, near, nquad);
     ok = hip_ok(hipGetLastError(), "green fill") && fft3d_full_f64(g, tw64, pos, full, s) &&

@@ -221,9 +221,15 @@ struct GreenScratch {
   size_t offs_cap = 0;
 };
 void green_scratch_free(GreenScratch& gs);
+// The folded Green tensor of grid g (FP64, and FP32 when g32 is set).
+// ncomp = 1: one component at a time, full holds M = L1 L2 L3 values;
+// ncomp = 6: the six together in five launches, full holds 6 M values.
 bool green_build_device(const GridPlan& g, const double2* tw64[3], const int* pos[3], double d,
                         double k0, int nquad, double2* full, double2* g64, float2* g32,
-                        GreenScratch* gs, hipStream_t s);
+                        GreenScratch* gs, hipStream_t s, int ncomp = 1);
+// The ncomp green_build_device takes for g in a pack: 6 while the six full
+// grids stay within 512 MB, else 1.
+int green_build_comps(const GridPlan& g);
 // Fold one caller-supplied FFT'd component already in `full` into g64/g32.
 bool green_fold_component(const GridPlan& g, const double2* full, int comp, double2* g64,
                           float2* g32, hipStream_t s);

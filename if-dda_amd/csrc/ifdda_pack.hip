@@ -452,7 +452,8 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
     for (size_t i = 0; i < build.size(); ++i) {
       const GridPlan& g = plans[build[i]];
       need[i % PackWs::kAux] = std::max(need[i % PackWs::kAux],
-                                        sizeof(double2) * (size_t)g.L[0] * g.L[1] * g.L[2]);
+                                        sizeof(double2) * green_build_comps(g) * (size_t)g.L[0] *
+                                            g.L[1] * g.L[2]);
     }
     if (!hip_ok(hipEventRecord(w.ev_main, w.s), "green event")) return IFDDA_ERR_HIP;
     for (int k = 0; k < PackWs::kAux; ++k) {
@@ -480,7 +481,7 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
       const int* pos[3] = {jd[j].tab3[0]->pos, jd[j].tab3[1]->pos, jd[j].tab3[2]->pos};
       if (!green_build_device(g, tw, pos, keys[j].d, keys[j].k0, keys[j].nquad,
                               reinterpret_cast<double2*>(w.afull[k]), e.G64, e.G32, &w.gs[k],
-                              w.aux[k])) {
+                              w.aux[k], green_build_comps(g))) {
         green_free(e);
         return IFDDA_ERR_HIP;
       }
