@@ -427,9 +427,10 @@ __device__ __forceinline__ void gp_check_la(KState* st, double2* trace) {
 // The previous iteration's end (gp_check_la on the |r|^2 partials of g_k5L,
 // when g_f4xL left it due) and then alpha from the first gather's <r0*, Ap>
 // partials (g_f2x): one finish kernel per iteration fewer than g_f5L +
-// g_f2x.  The check of iteration k thus runs after iteration k+1's first
-// matvec, which is wasted once per solve; the iterations, matvec counts and
-// results are those of the order g_f5L, g_f2x.
+// g_f2x, from iteration kLaSplit on (g_f5L ends the earlier ones).  The check
+// of iteration k then runs after iteration k+1's first matvec, which is
+// wasted once per solve that runs past kLaSplit; the iterations, matvec
+// counts and results are those of the order g_f5L, g_f2x.
 __global__ void __launch_bounds__(kThreads) g_f2xL(KState* st, const double2* part, int nblk,
                                                   const double2* part5, int nb5, double2* trace) {
   const int rh_ = blockIdx.y;
@@ -449,6 +450,23 @@ __global__ void __launch_bounds__(kThreads) g_f2xL(KState* st, const double2* pa
   finish_sum_any<1>(st, part5 + (long long)rh_ * nb5, nb5);
   if (threadIdx.x == 0) gp_alpha(st);
 }
+
+// The same check as a kernel of its own, right after g_k5L: the first
+// kLaSplit iterations of a look-ahead loop end this way, so a solve that
+// converges within them runs no matvec past its last iteration (the merged
+// check of g_f2xL wastes one per solve: 1 of ~15 on a 7-iteration solve, 1 of
+// ~300 on a 146-iteration one).  Same sums, same order, same results.
+__global__ void __launch_bounds__(kThreads) g_f5L(KState* st, const double2* part, int nblk,
+                                                 double2* trace) {
+  const int rh_ = blockIdx.y;
+  st += rh_;
+  if (st->done || !st->lapend) return;
+  finish_sum<1>(st, part + (long long)rh_ * kMaxDots * nblk, nblk);
+  if (threadIdx.x != 0) return;
+  st->lapend = 0;
+  gp_check_la(st, trace);
+}
+constexpr int kLaSplit = 16;
 
 // ======================================================= QMRBICGSTAB kernels
 template <typename V> struct QV {  // qmrbicgstab.f:23-31 work columns
@@ -1978,6 +1996,8 @@ static bool krylov(const MvOp& op, Solver* S, V** vv, int R, int method, const d
       if (!mvf<V>(op, v.t, v.t, v.At, s, done, R, n, dstride, f2)) return false;
       hipLaunchKernelGGL(g_f4xL, F, T, 0, s, S->st, (const double2*)S->part5, nb5, gp2);
       hipLaunchKernelGGL(g_k5L<V>, G, T, 0, s, (const KState*)S->st, v, vn, S->part, nb);
+      if (it < kLaSplit)
+        hipLaunchKernelGGL(g_f5L, F, T, 0, s, S->st, (const double2*)S->part, nb, trace);
       if (!hip_ok(hipGetLastError(), "gpbicg launch")) return false;
       const int st = step_poll(it);
       if (st < 0) return false;
