@@ -368,10 +368,19 @@ constexpr int zct_ng(int L, int B) {
   return ((L / 2 + 1) * B + (256 / (3 * B)) * 3 * B - 1) / ((256 / (3 * B)) * 3 * B);
 }
 
+// Minimum waves per SIMD of k_zconv_ct<V, L, B>: FP32 tiles of 8 lines or
+// more at L <= 128 (the batched launches of 128^3 grids) are held to 128
+// VGPRs, four waves instead of three (134 unbounded, 6 values spilled):
+// 256-RHS batch 1270 -> 1295 RHS/s.  At L = 200 the bound that would add a
+// wave spills 14-46 values.
+template <typename V, int L, int B> constexpr int zct_min_waves() {
+  return sizeof(V) == 8 && B >= 8 && L <= 128 ? 4 : 1;
+}
+
 // k_zconv for L3 = L, tile width B; launched with (256 / 3B) * 3B threads.
 // Requires 2 n_in <= L and 2 n_out <= L (true for every box: L >= 2n).
 template <typename V, int L, int B>
-__global__ void __launch_bounds__(256) k_zconv_ct(SArgs<V> a) {
+__global__ void __launch_bounds__(256, (zct_min_waves<V, L, B>())) k_zconv_ct(SArgs<V> a) {
   constexpr int NL = 3 * B, JT = 256 / NL, NTHR = NL * JT, F3 = L / 2 + 1;
   constexpr int NG = zct_ng(L, B);
   constexpr PlanC P = plan_c(L);
