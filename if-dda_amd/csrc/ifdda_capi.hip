@@ -62,6 +62,7 @@ extern "C" void ifdda_destroy(ifdda_ctx* c) {
   (void)hipFree(c->G32);
   (void)hipFree(c->Gd64);
   (void)hipFree(c->tab);
+  (void)hipFree(c->rows);
   (void)hipFree(c->al64);
   (void)hipFree(c->al32);
   (void)hipFree(c->scrA);
@@ -390,6 +391,8 @@ extern "C" int ifdda_set_dipoles(ifdda_ctx* c, const int32_t* tabdip, int ndipol
   if (!c) return arg_error("ifdda_set_dipoles: NULL context");
   const GridPlan& g = c->g;
   const size_t N = (size_t)g.nx * g.ny * g.nz;
+  (void)hipFree(c->rows);
+  c->rows = nullptr;
   if (!tabdip) {
     (void)hipFree(c->tab);
     c->tab = nullptr;
@@ -438,6 +441,13 @@ extern "C" int ifdda_set_dipoles(ifdda_ctx* c, const int32_t* tabdip, int ndipol
   } else {
     if (!c->tab) CHECK_HIP(hipMalloc(&c->tab, sizeof(int32_t) * N), "hipMalloc tabdip");
     CHECK_HIP(hipMemcpy(c->tab, tabdip, sizeof(int32_t) * N, hipMemcpyHostToDevice), "copy tabdip");
+    std::vector<unsigned long long> rm;
+    if (row_masks(tabdip, g.nx, g.ny, g.nz, rm)) {
+      CHECK_HIP(hipMalloc(&c->rows, sizeof(unsigned long long) * rm.size()), "hipMalloc line masks");
+      CHECK_HIP(hipMemcpy(c->rows, rm.data(), sizeof(unsigned long long) * rm.size(),
+                          hipMemcpyHostToDevice),
+                "copy line masks");
+    }
     if (ndipole > 0) {
       CHECK_HIP(hipMalloc(&c->dipcell, sizeof(int32_t) * ndipole), "hipMalloc dipole cells");
       CHECK_HIP(hipMemcpy(c->dipcell, cell.data(), sizeof(int32_t) * ndipole, hipMemcpyHostToDevice),

@@ -197,6 +197,9 @@ __device__ __forceinline__ void strided_tile(const SArgs<V>& a, const V* zsrc, V
   const int o = lg / gx;
   const int kx0 = (lg - o * gx) * a.B;
   const int nvalid = min(a.B, a.nbx - kx0) * NC;
+  // rows of plane o that hold dipoles (SArgs::rows; the block is uniform)
+  const unsigned long long rm = a.rows ? a.rows[o] : ~0ull;
+  if (SGN < 0 && rm == 0) return;
 
   load_tables(tw, pos, a.tw, a.pos, L, tid, nthr);
   lds_sync();
@@ -205,7 +208,7 @@ __device__ __forceinline__ void strided_tile(const SArgs<V>& a, const V* zsrc, V
   if (SGN > 0) {
     const bool half = 2 * a.n_in <= L;
     load_strided<true>(buf, pos, src, a.sp_src, rows_to_fill(a.ax, half), a.n_in, NL, line,
-                       nvalid, jt, JT);
+                       nvalid, jt, JT, rm);
     tile_fft_dit<SGN>(buf, tw, a.ax, NL, line, jt, JT, half);
     if (line < nvalid) {
       V* d = dst + line + (long long)jt * a.sp_dst;
@@ -218,7 +221,8 @@ __device__ __forceinline__ void strided_tile(const SArgs<V>& a, const V* zsrc, V
     if (line < nvalid) {
       V* d = dst + line + (long long)jt * a.sp_dst;
       const long long step = (long long)JT * a.sp_dst;
-      for (int p = jt; p < a.n_out; p += JT, d += step) *d = buf[pos[p] * NL + line];
+      for (int p = jt; p < a.n_out; p += JT, d += step)
+        if (row_on(rm, p)) *d = buf[pos[p] * NL + line];
     }
   }
 }
@@ -541,7 +545,30 @@ template <typename V> struct XArgs {
   const int* skip;
   int skip_stride;
   int nthr;  // packed launches: threads of the tile shape
+  // sparse box (null: every line): rows[k] bit j is set when box line
+  // g = j + ny k holds a dipole (ny <= 64).  P1 writes only those lines of
+  // A and P5 reads only those; a tile without any is skipped.
+  const unsigned long long* rows;
+  int ny;
 };
+
+// Bit l of the result: line g0 + l (l < nl <= 64) holds a dipole (all ones
+// without a row mask).  g0 is uniform over the block, so this is scalar work.
+__device__ __forceinline__ unsigned long long tile_lines(const unsigned long long* rows, int ny,
+                                                         int g0, int nl) {
+  if (!rows) return ~0ull;
+  unsigned long long tm = 0;
+  int k = g0 / ny, j = g0 - k * ny;
+  unsigned long long w = rows[k];
+  for (int l = 0; l < nl; ++l) {
+    tm |= ((w >> j) & 1ull) << l;
+    if (++j == ny && l + 1 < nl) {
+      j = 0;
+      w = rows[++k];
+    }
+  }
+  return tm;
+}
 
 template <typename V>
 __device__ __forceinline__ void alpha_apply(const V* __restrict__ al, int mode, long long dip, V& e0,
@@ -636,6 +663,11 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
   const int tid = PK && tid0 >= nthr ? kIdle : tid0;
   const int line = tid0 % NL, jt = tid / NL, JT = nthr / NL;
   const int nl = min(BT, a.nlines - g0);
+  // lines of the tile that hold dipoles; a tile without any writes nothing
+  // (P1) or gathers nothing (P5), and only the gathering pass with inner
+  // products still runs, for its zero block partials
+  const unsigned long long tm = tile_lines(a.rows, a.ny, g0, nl);
+  if (tm == 0 && !(kP5 && kND > 0)) return;
   const bool half_in = DIT && 2 * a.n_in <= L;
   const bool half_out = !DIT && 2 * a.n_out <= L;
   const int fill = DIT ? (CT ? ct_fill(LC) : rows_to_fill(a.ax, half_in)) : L;
@@ -661,7 +693,8 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           dip[u] = -1;
-          if (ll[u] < nl && ii[u] < a.n_in) dip[u] = dip_of(a.tab, a.nx, g0 + ll[u], ii[u]);
+          if (ll[u] < nl && ii[u] < a.n_in && row_on(tm, ll[u]))
+            dip[u] = dip_of(a.tab, a.nx, g0 + ll[u], ii[u]);
         }
         if constexpr (MODE == 3) {
           // every load of the batch before the stores over p
@@ -710,7 +743,7 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
         for (int u = 0; u < U; ++u) {
 #pragma unroll
           for (int c = 0; c < NC; ++c) x[u][c] = czero<V>();
-          if (ll[u] < nl && ii[u] < a.n_in) {
+          if (ll[u] < nl && ii[u] < a.n_in && row_on(tm, ll[u])) {
             const V* sp = zsrc + (long long)(g0 + ll[u]) * a.ld_src + (long long)ii[u] * NC;
 #pragma unroll
             for (int c = 0; c < NC; ++c) x[u][c] = sp[c];
@@ -729,7 +762,9 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
   }
   lds_sync();
 
-  if constexpr (CT) {
+  if (tm == 0) {
+    // no dipole in the tile: the gathering pass's partials only (zeros)
+  } else if constexpr (CT) {
     // the stages take the row stride as their NL (it only places rows)
     constexpr int NLC = BC * NC, JTC = 256 / NLC, NLSC = NC == 3 ? x_stride(NLC) : NLC;
     if constexpr (DIT)
@@ -759,7 +794,7 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
         ii[u] = ix.i;
         ix.next();
         dip[u] = -1;
-        if (ll[u] < nl) dip[u] = dip_of(a.tab, a.nx, g0 + ll[u], ii[u]);
+        if (ll[u] < nl && row_on(tm, ll[u])) dip[u] = dip_of(a.tab, a.nx, g0 + ll[u], ii[u]);
       }
       V b[U][3];
       // r0* (MODE 5) or y (MODE 6, 7), and r0* (MODE 7), loaded before the
@@ -854,7 +889,7 @@ __device__ __forceinline__ void xpass_tile(const XArgs<V>& a, const V* zsrc, V* 
   } else {
     DivStep ix(tid, nthr, a.n_out);
     for (int e = tid; e < BT * a.n_out; e += nthr, ix.next()) {
-      if (ix.l >= nl) continue;
+      if (ix.l >= nl || !row_on(tm, ix.l)) continue;
       const V* v = buf + (DIT ? ix.i : pos[ix.i]) * NLS + ix.l * NC;
       V* d = zdst + (long long)(g0 + ix.l) * a.ld_dst + (long long)ix.i * NC;
 #pragma unroll
@@ -1513,6 +1548,10 @@ static void pass_set(const GridPlan& g, const MatvecDev<V>& m, int pi, bool sing
   o.t[2] = single ? &g.one[pi][2] : &g.p3[pi];
   o.t[3] = single ? &g.one[pi][3] : &g.p4[pi];
   o.t[4] = single ? &g.one[pi][4] : &g.p5[pi];
+  // the line masks of a sparse map, for x tiles of at most 64 lines (a mask
+  // in all four passes or in none)
+  const unsigned long long* rows =
+      m.tab && m.rows && ny <= 64 && o.t[0]->B <= 64 && o.t[4]->B <= 64 ? m.rows : nullptr;
   {  // P1: scatter(alpha x) + FFT-x
     XArgs<V>& a = o.p1;
     a = XArgs<V>{};
@@ -1531,6 +1570,8 @@ static void pass_set(const GridPlan& g, const MatvecDev<V>& m, int pi, bool sing
     a.nx = nx;
     a.alpha = m.alpha;
     a.alpha_mode = m.alpha_mode;
+    a.rows = rows;
+    a.ny = ny;
     o.gx[0] = (a.nlines + a.B - 1) / a.B;
     o.gy[0] = 1;
   }
@@ -1551,6 +1592,7 @@ static void pass_set(const GridPlan& g, const MatvecDev<V>& m, int pi, bool sing
     a.sp_src = l1c;
     a.so_dst = (long long)L2 * l1c;
     a.sp_dst = l1c;
+    a.rows = rows;
     o.gx[1] = a.gx = (L1 + a.B - 1) / a.B;
     o.gy[1] = nz;
   }
@@ -1595,6 +1637,7 @@ static void pass_set(const GridPlan& g, const MatvecDev<V>& m, int pi, bool sing
     a.sp_src = l1c;
     a.so_dst = (long long)ny * l1c;
     a.sp_dst = l1c;
+    a.rows = rows;
     o.gx[3] = a.gx = (L1 + a.B - 1) / a.B;
     o.gy[3] = nz;
   }
@@ -1614,10 +1657,31 @@ static void pass_set(const GridPlan& g, const MatvecDev<V>& m, int pi, bool sing
     a.ld_src = l1c;
     a.tab = m.tab;
     a.nx = nx;
+    a.rows = rows;
+    a.ny = ny;
     a.inv_m = (S)(1.0 / ((double)L1 * (double)L2 * (double)L3));
     o.gx[4] = (a.nlines + a.B - 1) / a.B;
     o.gy[4] = 1;
   }
+}
+
+bool row_masks(const int32_t* tab, int nx, int ny, int nz, std::vector<unsigned long long>& out) {
+  if (!tab || ny > 64 || nx <= 0 || ny <= 0 || nz <= 0) return false;
+  std::vector<unsigned long long> m((size_t)nz, 0ull);
+  bool gap = false;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j) {
+      const int32_t* t = tab + ((size_t)k * ny + j) * nx;
+      bool any = false;
+      for (int i = 0; i < nx && !any; ++i) any = t[i] > 0;
+      if (any)
+        m[k] |= 1ull << j;
+      else
+        gap = true;
+    }
+  if (!gap) return false;
+  out.swap(m);
+  return true;
 }
 
 template <typename V>

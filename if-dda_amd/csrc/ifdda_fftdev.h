@@ -222,7 +222,18 @@ template <typename V> struct SArgs {
   // k_zconv_der only: the tensor is a derivative along axis dpar (1 x, 2 y,
   // 3 z), odd under that axis's mirror on top of the Green tensor's parity
   int dpar;
+  // y passes of a sparse box (null: every row): rows[k] bit j is set when box
+  // line (j, k) holds a dipole.  P2 reads only those rows of A (the x pass
+  // writes no others) and P4 writes only those (the gathering x pass reads no
+  // others); needs ny <= 64.
+  const unsigned long long* rows;
 };
+
+// Row p of a tile under the row mask rm (all ones: every row; a mask only
+// ever comes with p < 64).
+__device__ __forceinline__ bool row_on(unsigned long long rm, int p) {
+  return p >= 64 || ((rm >> p) & 1ull);
+}
 
 __device__ __forceinline__ int fold_idx(int k, int L, int& sgn) {
   if (2 * k > L) {
@@ -309,7 +320,7 @@ __device__ __forceinline__ void load_tables(V* tw, int* pos, const V* gtw, const
 template <bool PERM, typename V>
 __device__ __forceinline__ void load_strided(V* buf, const int* pos, const V* src, long long sp,
                                              int fill, int n_in, int NL, int line, int nvalid,
-                                             int jt, int JT) {
+                                             int jt, int JT, unsigned long long rm = ~0ull) {
   const long long ustep = (long long)JT * sp;
   for (int p0 = jt; p0 < fill; p0 += kBatch * JT) {
     const V* s0 = src + (long long)p0 * sp + line;
@@ -318,7 +329,7 @@ __device__ __forceinline__ void load_strided(V* buf, const int* pos, const V* sr
     for (int u = 0; u < kBatch; ++u) {
       const int p = p0 + u * JT;
       v[u] = czero<V>();
-      if (p < n_in && line < nvalid) v[u] = s0[u * ustep];
+      if (p < n_in && line < nvalid && row_on(rm, p)) v[u] = s0[u * ustep];
     }
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
@@ -465,7 +476,8 @@ template <int L, int JT, typename V> struct FirstRows {
 };
 template <int L, int JT, typename V>
 __device__ __forceinline__ void ct_fetch_first(FirstRows<L, JT, V>& f, const V* src, long long sp,
-                                               int n_in, int line, int nvalid, int jt) {
+                                               int n_in, int line, int nvalid, int jt,
+                                               unsigned long long rm = ~0ull) {
   using F = FirstRows<L, JT, V>;
   static_assert(F::R % 2 == 0, "fused first stage needs an even first radix");
 #pragma unroll
@@ -475,7 +487,7 @@ __device__ __forceinline__ void ct_fetch_first(FirstRows<L, JT, V>& f, const V* 
     for (int q = 0; q < F::R / 2; ++q) {
       const int n = u + F::NB * q;
       f.a[i][q] = czero<V>();
-      if ((F::NB % JT == 0 || u < F::NB) && n < n_in && line < nvalid)
+      if ((F::NB % JT == 0 || u < F::NB) && n < n_in && line < nvalid && row_on(rm, n))
         f.a[i][q] = src[(long long)n * sp + line];
     }
   }
@@ -501,9 +513,10 @@ __device__ __forceinline__ void ct_first_stage(V* buf, const int* pos, FirstRows
 }
 template <int L, int NL, int JT, int SGN, typename V>
 __device__ __forceinline__ void ct_load_first(V* buf, const int* pos, const V* src, long long sp,
-                                              int n_in, int line, int nvalid, int jt) {
+                                              int n_in, int line, int nvalid, int jt,
+                                              unsigned long long rm = ~0ull) {
   FirstRows<L, JT, V> f;
-  ct_fetch_first<L, JT>(f, src, sp, n_in, line, nvalid, jt);
+  ct_fetch_first<L, JT>(f, src, sp, n_in, line, nvalid, jt, rm);
   ct_first_stage<L, NL, JT, SGN>(buf, pos, f, line, jt);
 }
 
@@ -514,7 +527,8 @@ __device__ __forceinline__ void ct_load_first(V* buf, const int* pos, const V* s
 // barrier, after which the tile may be reused.
 template <int L, int NL, int JT, int SGN, typename V>
 __device__ __forceinline__ void ct_last_store(const V* buf, const int* pos, V* dst, long long sp,
-                                              int n_out, int line, int nvalid, int jt) {
+                                              int n_out, int line, int nvalid, int jt,
+                                              unsigned long long rm = ~0ull) {
   constexpr PlanC P = plan_c(L);
   constexpr int R = P.rad[0], NB = L / R, NIT = (NB + JT - 1) / JT;
   static_assert(R % 2 == 0, "fused last stage needs an even first radix");
@@ -531,7 +545,7 @@ __device__ __forceinline__ void ct_last_store(const V* buf, const int* pos, V* d
 #pragma unroll
         for (int q = 0; q < R / 2; ++q) {
           const int n = u + NB * q;
-          if (n < n_out) dst[(long long)n * sp + line] = a[q];
+          if (n < n_out && row_on(rm, n)) dst[(long long)n * sp + line] = a[q];
         }
       }
     }

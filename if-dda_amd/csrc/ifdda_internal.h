@@ -41,6 +41,9 @@ template <typename V> struct MatvecDev {
   const V* tws[3];    // stage-major twiddles per axis (stage_twiddles)
   const int* pos[3];  // in-place FFT positions per axis (axis_positions)
   const int32_t* tab; // box -> 1-based dipole, or null (dense box)
+  // rows[k] bit j: box line (j, k) holds a dipole; null: every line (dense
+  // boxes, ny > 64, or a map without empty lines).  Built by row_masks.
+  const unsigned long long* rows = nullptr;
   const V* alpha;     // per dipole, see alpha_mode
   int alpha_mode;
   PassTimer* timer = nullptr;
@@ -99,6 +102,10 @@ bool make_tiles(GridPlan& g, size_t lds_limit);
 // Plan of a box: padded lengths, radix plans and tile shapes; false (and the
 // reason) when a length has no plan or tile.
 bool make_grid_plan(int nx, int ny, int nz, size_t lds_limit, GridPlan* out, std::string* why);
+// Line masks of a dipole map (MatvecDev::rows): out[k] bit j set when a cell
+// (i, j, k) maps to a dipole, k < nz.  False (out untouched) when they would
+// skip nothing: ny > 64, or every line holds a dipole.
+bool row_masks(const int32_t* tab, int nx, int ny, int nz, std::vector<unsigned long long>& out);
 // Tables of one axis (L values each): twiddles exp(+2 pi i e / L) in FP64 and
 // FP32, stage-major twiddles (stage_twiddles), in-place FFT positions.
 void axis_tables(const AxisPlan& ax, double2* t64, float2* t32, double2* s64, float2* s32,
@@ -254,6 +261,7 @@ struct ifdda_ctx {
   float2* G32 = nullptr;
   bool green_ready = false;
   int32_t* tab = nullptr;  // device dipole map (null: dense box)
+  unsigned long long* rows = nullptr;  // device line masks of tab (MatvecDev::rows)
   long long nd = 0;        // dipoles
   bool have_dipoles = false;
   double2* al64 = nullptr;  // polarisability (scalar or 3x3 row-major per dipole)
@@ -303,6 +311,7 @@ struct ifdda_ctx {
     m.G = G64;
     for (int i = 0; i < 3; ++i) m.tw[i] = tw64[i], m.tws[i] = tws64[i], m.pos[i] = pos[i];
     m.tab = tab;
+    m.rows = rows;
     m.alpha = al64;
     m.alpha_mode = alpha_mode;
     m.timer = timer.cap ? const_cast<ifdda::PassTimer*>(&timer) : nullptr;
@@ -317,6 +326,7 @@ struct ifdda_ctx {
     m.G = G32;
     for (int i = 0; i < 3; ++i) m.tw[i] = tw32[i], m.tws[i] = tws32[i], m.pos[i] = pos[i];
     m.tab = tab;
+    m.rows = rows;
     m.alpha = al32;
     m.alpha_mode = alpha_mode;
     m.timer = timer.cap ? const_cast<ifdda::PassTimer*>(&timer) : nullptr;

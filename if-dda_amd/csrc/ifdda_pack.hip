@@ -205,6 +205,7 @@ struct JobDev {
   const GreenEntry* green = nullptr;
   const PackTable* tab3[3] = {};
   int32_t* tab = nullptr;  // null: dense box
+  unsigned long long* rows = nullptr;  // line masks of tab (null: none)
   double2* al64 = nullptr;
   float2* al32 = nullptr;
   long long nd = 0;
@@ -514,6 +515,7 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
   }
   // ---- arena: per job map and polarisability, per member FFT scratch, E/FF/FFloc
   std::vector<size_t> o_tab(J), o_al64(J), o_al32(J), o_A(P), o_B(P), o_cell(J), o_wk(J), o_we(J);
+  std::vector<size_t> o_rows(J, 0);
   size_t used = 0;
   auto take = [&](size_t bytes) {
     const size_t at = used;
@@ -523,6 +525,8 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
   for (int j = 0; j < J; ++j) {
     const ifdda_job& s = *in[j].job;
     o_tab[j] = dense[j] ? 0 : take(sizeof(int32_t) * (size_t)s.nx * s.ny * s.nz);
+    // line masks of a sparse map (MatvecDev::rows), boxes of up to 64 rows
+    if (!dense[j] && s.ny <= 64) o_rows[j] = take(sizeof(unsigned long long) * (size_t)s.nz);
     o_al64[j] = take(sizeof(double2) * (size_t)s.nd);
     o_al32[j] = take(sizeof(float2) * (size_t)s.nd);
     if (!s.E_inc) {
@@ -558,6 +562,11 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
     const ifdda_job& s = *in[j].job;
     const size_t N = (size_t)s.nx * s.ny * s.nz;
     if (!dense[j]) std::memcpy(H + o_tab[j], s.tabdip, sizeof(int32_t) * N);
+    if (o_rows[j]) {
+      std::vector<unsigned long long> rm;
+      if (!row_masks(s.tabdip, s.nx, s.ny, s.nz, rm)) rm.assign((size_t)s.nz, ~0ull);
+      std::memcpy(H + o_rows[j], rm.data(), sizeof(unsigned long long) * (size_t)s.nz);
+    }
     double2* h64 = reinterpret_cast<double2*>(H + o_al64[j]);
     float2* h32 = reinterpret_cast<float2*>(H + o_al32[j]);
     for (int i = 0; i < s.nd; ++i) {
@@ -589,6 +598,7 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
     return IFDDA_ERR_HIP;
   for (int j = 0; j < J; ++j) {
     jd[j].tab = dense[j] ? nullptr : reinterpret_cast<int32_t*>(A + o_tab[j]);
+    jd[j].rows = o_rows[j] ? reinterpret_cast<unsigned long long*>(A + o_rows[j]) : nullptr;
     jd[j].al64 = reinterpret_cast<double2*>(A + o_al64[j]);
     jd[j].al32 = reinterpret_cast<float2*>(A + o_al32[j]);
   }
@@ -633,6 +643,7 @@ int pack_solve(PackWs& w, const std::vector<PackIn>& in, const ifdda_solve_opts*
         b.tw[ax] = d.tab3[ax]->tw32, b.tws[ax] = d.tab3[ax]->tws32, b.pos[ax] = d.tab3[ax]->pos;
       }
       a.tab = b.tab = d.tab;
+      a.rows = b.rows = d.rows;
       a.alpha = d.al64;
       b.alpha = d.al32;
       a.alpha_mode = b.alpha_mode = kAlphaScalar;

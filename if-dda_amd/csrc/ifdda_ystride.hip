@@ -9,7 +9,8 @@
 // pruned (rows >= L/2 are zero padding; that stage runs on the loaded
 // registers), P4 loads all L rows and runs decimation in frequency with its
 // last stage pruned (rows >= ny are not kept; that stage's outputs go from
-// registers to HBM).  Here the radix plan, trip counts and LDS offsets are constants and
+// registers to HBM); in a sparse box both touch only the rows of A whose box
+// line holds a dipole (SArgs::rows).  Here the radix plan, trip counts and LDS offsets are constants and
 // each butterfly's twiddles come from the stage-major table at one base
 // address (the stages of the compile-time z pass).  On the 200^3 grid the
 // runtime-plan y pass spends ~1.6x the VALU instructions per line of those
@@ -40,12 +41,15 @@ __global__ void __launch_bounds__(256) k_strided_ct(SArgs<V> a) {
   const int o = lg / gridDim.x;
   const int kx0 = (lg - o * (int)gridDim.x) * B;
   const int nvalid = min(B, a.nbx - kx0) * 3;
+  // rows of plane o that hold dipoles (a plane without any has nothing to keep)
+  const unsigned long long rm = a.rows ? a.rows[o] : ~0ull;
+  if (SGN < 0 && rm == 0) return;
   load_tables(tws, pos, a.tws, a.pos, L, tid, NTHR);
   lds_sync();
   const V* src = zsrc + (long long)o * a.so_src + (long long)kx0 * 3;
   V* dst = zdst + (long long)o * a.so_dst + (long long)kx0 * 3;
   if constexpr (SGN > 0) {  // the load fused with the first stage
-    ct_load_first<L, NL, JT, +1>(buf, pos, src, a.sp_src, a.n_in, line, nvalid, jt);
+    ct_load_first<L, NL, JT, +1>(buf, pos, src, a.sp_src, a.n_in, line, nvalid, jt, rm);
     ct_dit<L, NL, JT, +1, 1>(buf, tws, line, jt);
     if (line < nvalid) {
       V* d = dst + line + (long long)jt * a.sp_dst;
@@ -56,7 +60,7 @@ __global__ void __launch_bounds__(256) k_strided_ct(SArgs<V> a) {
     load_strided<false>(buf, pos, src, a.sp_src, L, a.n_in, NL, line, nvalid, jt, JT);
     // the last stage fused with the store
     ct_dif<L, NL, JT, -1, P.nst - 1, V, 1>(buf, tws, line, jt);
-    ct_last_store<L, NL, JT, -1>(buf, pos, dst, a.sp_dst, a.n_out, line, nvalid, jt);
+    ct_last_store<L, NL, JT, -1>(buf, pos, dst, a.sp_dst, a.n_out, line, nvalid, jt, rm);
   }
 }
 
