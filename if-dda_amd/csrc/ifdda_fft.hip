@@ -373,12 +373,14 @@ constexpr int zct_ng(int L, int B) {
 }
 
 // Minimum waves per SIMD of k_zconv_ct<V, L, B>: FP32 tiles of 8 lines or
-// more at L <= 128 (the batched launches of 128^3 grids) are held to 128
-// VGPRs, four waves instead of three (134 unbounded, 6 values spilled):
-// 256-RHS batch 1270 -> 1295 RHS/s.  At L = 200 the bound that would add a
-// wave spills 14-46 values.
+// more at L <= 128 are held to 128 VGPRs, four waves instead of three (134
+// unbounded, 8 values spilled), and the FP32 width-5 tile at L = 128 (the
+// batched launches of 128^3 grids) to 96, five waves instead of four (98
+// unbounded, no spill): the 256-RHS batch went from 1270 to 1290 RHS/s with
+// the first and from 1371 to 1405 with width 5 and the second.  At L = 200
+// the bound that would add a wave spills 14-46 values.
 template <typename V, int L, int B> constexpr int zct_min_waves() {
-  return sizeof(V) == 8 && B >= 8 && L <= 128 ? 4 : 1;
+  return sizeof(V) != 8 ? 1 : B >= 8 && L <= 128 ? 4 : (L == 128 && B == 5) ? 5 : 1;
 }
 
 // k_zconv for L3 = L, tile width B; launched with (256 / 3B) * 3B threads.
@@ -1498,7 +1500,10 @@ bool make_tiles(GridPlan& g, size_t lds_limit) {
     const size_t bs = p == 0 ? b64 : b32, bc = p == 0 ? b64c : b32c;
     g.p1[p] = tile_for(g.ax[0], 3, esz, g.ny * g.nz, bs, false, true);
     g.p2[p] = tile_for(g.ax[1], 3, esz, L1, bs, false);
-    g.p3[p] = tile_for(g.ax[2], 3, esz, L1, bc, true);
+    // FP32 batched z pass at L3 = 128: width 5 at five waves per SIMD
+    // (zct_min_waves) rather than the budget's 8 at four (VALU-bound either
+    // way; the fifth wave hides more of its latency)
+    g.p3[p] = tile_for(g.ax[2], 3, esz, (p == 1 && L3 == 128) ? std::min(L1, 5) : L1, bc, true);
     g.p4[p] = tile_for(g.ax[1], 3, esz, L1, bs, false);
     g.p5[p] = tile_for(g.ax[0], 3, esz, g.ny * g.nz, bs, false, true);
     // single right-hand side: capped widths
